@@ -1,0 +1,316 @@
+"""Synthetic inputs for the grid thermal model: floorplans, layer files,
+seeded powers, and a runner that drives any library exporting HotSpot's C
+API through :mod:`hotspot_amd.iface`.
+
+All random draws use numpy's counter-based Philox generator with an explicit
+seed, so a workload is reproduced exactly from its description.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import iface
+
+
+def rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.Philox(seed))
+
+
+# --------------------------------------------------------------------------
+# floorplans: lists of (name, width, height, leftx, bottomy[, c_v, rho])
+# --------------------------------------------------------------------------
+def tiled_floorplan(nbx: int, nby: int, width: float, height: float, prefix="b"):
+    """nbx x nby equal blocks tiling a width x height die."""
+    w, h = width / nbx, height / nby
+    return [(f"{prefix}{r}_{c}", w, h, c * w, r * h)
+            for r in range(nby) for c in range(nbx)]
+
+
+def slicing_floorplan(g: np.random.Generator, n_blocks: int, width: float,
+                      height: float, aspect=(0.5, 2.0), prefix="u", max_tries=200):
+    """Seeded recursive slicing of a width x height die into n_blocks blocks.
+    Each cut is perpendicular to the longer side at a random fraction near
+    the block-count ratio; draws are repeated until every block's aspect
+    ratio lies within ``aspect``."""
+    for _ in range(max_tries):
+        out = []
+
+        def split(x, y, w, h, n):
+            if n == 1:
+                out.append((x, y, w, h))
+                return
+            n1 = n // 2
+            frac = n1 / n * g.uniform(0.85, 1.15)
+            frac = min(max(frac, 0.2), 0.8)
+            if w >= h:
+                w1 = w * frac
+                split(x, y, w1, h, n1)
+                split(x + w1, y, w - w1, h, n - n1)
+            else:
+                h1 = h * frac
+                split(x, y, w, h1, n1)
+                split(x, y + h1, w, h - h1, n - n1)
+
+        split(0.0, 0.0, width, height, n_blocks)
+        if all(aspect[0] <= w / h <= aspect[1] for (_, _, w, h) in out):
+            return [(f"{prefix}{k}", w, h, x, y) for k, (x, y, w, h) in enumerate(out)]
+    raise RuntimeError("could not draw a slicing floorplan within the aspect bounds")
+
+
+def write_flp(path: str, units) -> None:
+    with open(path, "w") as f:
+        f.write("# name\twidth\theight\tleft-x\tbottom-y\n")
+        for u in units:
+            f.write("\t".join([u[0]] + ["%.17g" % v for v in u[1:]]) + "\n")
+
+
+@dataclass
+class LcfLayer:
+    """One record of a layer configuration file."""
+    lateral: bool
+    power: bool
+    sp: float | str          # specific heat, or a material name
+    rho: float | None        # resistivity (ignored when sp is a material)
+    thickness: float
+    flp: list | str          # unit list, or a path (.flp / .csv)
+
+
+def write_lcf(path: str, layers: list[LcfLayer], dirname: str) -> None:
+    with open(path, "w") as f:
+        for n, lay in enumerate(layers):
+            if isinstance(lay.flp, str):
+                fp = lay.flp
+            else:
+                fp = os.path.join(dirname, f"layer{n}.flp")
+                write_flp(fp, lay.flp)
+            f.write(f"# layer {n}\n{n}\n{'Y' if lay.lateral else 'N'}\n"
+                    f"{'Y' if lay.power else 'N'}\n")
+            if isinstance(lay.sp, str):
+                f.write(f"{lay.sp}\n")
+            else:
+                f.write("%.17g\n%.17g\n" % (lay.sp, lay.rho))
+            f.write("%.17g\n%s\n\n" % (lay.thickness, fp))
+
+
+# --------------------------------------------------------------------------
+# cases
+# --------------------------------------------------------------------------
+@dataclass
+class Case:
+    """A grid-model problem described by the reference's own inputs."""
+    name: str
+    config: dict = field(default_factory=dict)     # thermal_config_t overrides
+    flp: list | None = None                        # default-stack floorplan
+    lcf: list[LcfLayer] | None = None              # layer configuration
+    detailed_3D: bool = False
+    use_microchannels: bool = False
+    microchannel: dict = field(default_factory=dict)
+
+    def materialize(self, dirname: str) -> dict:
+        """Write the floorplan / layer files; returns the paths and the full
+        config overrides to hand to both the reference API and the oracle."""
+        os.makedirs(dirname, exist_ok=True)
+        cfg = dict(self.config)
+        cfg.setdefault("model_type", "grid")
+        out = dict(config=cfg, flp=None, lcf=None)
+        if self.lcf is not None:
+            lcf = os.path.join(dirname, "layers.lcf")
+            write_lcf(lcf, self.lcf, dirname)
+            cfg["grid_layer_file"] = lcf
+            out["lcf"] = lcf
+        else:
+            fp = os.path.join(dirname, "chip.flp")
+            write_flp(fp, self.flp)
+            out["flp"] = fp
+        return out
+
+
+def default_stack_case(n: int, nbx=4, nby=4, die=16e-3, **config) -> Case:
+    """Reference defaults (4-layer Si/TIM/spreader/sink stack), an n x n grid
+    and nbx x nby equal blocks on a die x die chip."""
+    cfg = dict(grid_rows=n, grid_cols=n)
+    cfg.update(config)
+    return Case(f"default-stack {n}x{n} {nbx*nby} blocks", cfg,
+                flp=tiled_floorplan(nbx, nby, die, die))
+
+
+def stacked_case(n: int, seed: int, n_dies=4, blocks_per_die=64, die=16e-3,
+                 t_die=0.1e-3, t_bond=10e-6, k_bond=1.5, **config) -> tuple[Case, list]:
+    """3D stack: n_dies silicon dies (k=100, c_v=1.75e6) interleaved with
+    bond layers (k_bond, c_v=4e6) over the default spreader and sink.  Each
+    die carries a seeded slicing floorplan; block power = density x area
+    with density ~ lognormal(ln 0.5 W/mm^2, 0.6).  Returns the case and the
+    per-die block powers."""
+    g = rng(seed)
+    layers, powers = [], []
+    for d in range(n_dies):
+        units = slicing_floorplan(g, blocks_per_die, die, die, prefix=f"d{d}u")
+        dens = g.lognormal(np.log(0.5), 0.6, size=len(units))      # W/mm^2
+        powers.append(dens * np.array([u[1] * u[2] for u in units]) * 1e6)
+        layers.append(LcfLayer(True, True, 1.75e6, 0.01, t_die, units))
+        if d < n_dies - 1:
+            bond = tiled_floorplan(1, 1, die, die, prefix=f"bond{d}_")
+            layers.append(LcfLayer(True, False, 4e6, 1.0 / k_bond, t_bond, bond))
+    cfg = dict(grid_rows=n, grid_cols=n)
+    cfg.update(config)
+    return Case(f"3D stack {n}x{n} {n_dies} dies", cfg, lcf=layers), powers
+
+
+# --------------------------------------------------------------------------
+# running a case through a HotSpot-API library
+# --------------------------------------------------------------------------
+class Run:
+    """An allocated and populated model inside one library."""
+
+    def __init__(self, lib: iface.HotSpotLib, case: Case, dirname: str,
+                 with_C=True):
+        self.lib, self.case = lib, case
+        m = case.materialize(dirname)
+        self.paths = m
+        L = lib.lib
+        cfg = lib.default_config(**m["config"])
+        self._cfg = cfg
+        self.flp = L.read_flp(m["flp"].encode(), 0, 0) if m["flp"] else None
+        self._mc = None
+        if case.use_microchannels:
+            mc = L.default_microchannel_config()
+            for k, v in case.microchannel.items():
+                setattr(mc, k, v)
+            self._mc = mc
+        self._materials = iface.MaterialsList()
+        L.default_materials(C.byref(self._materials))
+        self.model = L.alloc_RC_model(C.byref(cfg), self.flp,
+                                      C.byref(self._mc) if self._mc is not None else None,
+                                      C.byref(self._materials),
+                                      int(case.detailed_3D), int(case.use_microchannels))
+        L.populate_R_model(self.model, self.flp)
+        if with_C:
+            L.populate_C_model(self.model, self.flp)
+        self.grid = self.model.contents.grid.contents
+        self.n = lib.n_vector(self.model)
+        self._vecs = []
+
+    def vec(self, values=None):
+        p, a = self.lib.vector(self.model)
+        self._vecs.append(p)
+        if values is not None:
+            a[:] = values
+        return p, a
+
+    def layer_blocks(self):
+        """(base, n_units, has_power) for each layer of the block vector."""
+        out, base = [], 0
+        g = self.grid
+        for n in range(g.n_layers):
+            lay = g.layers[n]
+            k = lay.flp.contents.n_units
+            out.append((base, k, bool(lay.has_power)))
+            base += k
+        return out
+
+    def power_vector(self, per_power_layer: list) -> np.ndarray:
+        """Full-length power vector with the given block powers placed on the
+        power-dissipating layers (hotspot.c:578-587) and zeros elsewhere."""
+        p = np.zeros(self.n)
+        k = 0
+        for base, nu, hp in self.layer_blocks():
+            if hp:
+                p[base:base + nu] = per_power_layer[k]
+                k += 1
+        return p
+
+    def steady(self, power: np.ndarray) -> np.ndarray:
+        pp, pa = self.vec(power)
+        tp, ta = self.vec()
+        self.lib.lib.steady_state_temp(self.model, pp, tp)
+        return ta.copy()
+
+    def free(self):
+        L = self.lib.lib
+        for p in self._vecs:
+            L.free_dvector(p)
+        self._vecs = []
+        if self.model:
+            L.delete_RC_model(self.model)
+            self.model = None
+        if self.flp:
+            L.free_flp(self.flp, 0, 0)
+            self.flp = None
+
+
+def channel_network(nr: int, nc: int, width_cells=1, pitch_cells=3, first=1,
+                    direction="we") -> np.ndarray:
+    """Straight parallel channels across the layer: cell types per
+    microchannel.h:19-23 (0 solid, 1 fluid, 2 inlet, 3 outlet).  Channels
+    run west->east ('we', inlet on column 0) or north->south ('ns')."""
+    t = np.zeros((nr, nc), dtype=np.int64)
+    span = nr if direction == "we" else nc
+    k = first
+    while k + width_cells <= span - 1:
+        for w in range(width_cells):
+            if direction == "we":
+                t[k + w, :] = 1
+                t[k + w, 0] = 2
+                t[k + w, nc - 1] = 3
+            else:
+                t[:, k + w] = 1
+                t[0, k + w] = 2
+                t[nr - 1, k + w] = 3
+        k += pitch_cells
+    return t
+
+
+def write_network_csv(path: str, types: np.ndarray) -> None:
+    with open(path, "w") as f:
+        for row in types:
+            f.write(", ".join(str(int(v)) for v in row) + "\n")
+
+
+def microchannel_case(n: int, seed: int, dirname: str, die=4e-3, width_cells=1,
+                      pitch_cells=3, t_channel=100e-6, pumping_pressure=5000.0,
+                      **config) -> tuple[Case, list]:
+    """Silicon device layer (power) / microchannel layer / silicon cap on the
+    default package, detailed 3D on, with straight west->east channels."""
+    g = rng(seed)
+    os.makedirs(dirname, exist_ok=True)
+    csv = os.path.join(dirname, "channels.csv")
+    write_network_csv(csv, channel_network(n, n, width_cells, pitch_cells))
+    dev = slicing_floorplan(g, 8, die, die, prefix="dev")
+    cap = tiled_floorplan(1, 1, die, die, prefix="cap")
+    layers = [LcfLayer(True, True, 1.75e6, 0.01, 50e-6, dev),
+              LcfLayer(True, False, 1.6303e6, 1.0 / 130.0, t_channel, csv),
+              LcfLayer(True, False, 1.75e6, 0.01, 100e-6, cap)]
+    cfg = dict(grid_rows=n, grid_cols=n)
+    cfg.update(config)
+    powers = [g.uniform(0.1, 1.0, size=len(dev))]
+    return Case(f"microchannel {n}x{n}", cfg, lcf=layers, detailed_3D=True,
+                use_microchannels=True,
+                microchannel=dict(pumping_pressure=pumping_pressure)), powers
+
+
+def jittered_powers(base: np.ndarray, n_steps: int, seed: int, jitter=0.2):
+    """n_steps copies of a power vector, each scaled per block by
+    U(1-jitter, 1+jitter) with a seeded generator."""
+    g = rng(seed)
+    return [base * g.uniform(1 - jitter, 1 + jitter, size=base.shape)
+            for _ in range(n_steps)]
+
+
+def run_transient(run: "Run", init_block_temp: np.ndarray, powers: list,
+                  dt: float) -> np.ndarray:
+    """The CLI's transient loop (hotspot.c:598-610): the first compute_temp
+    call hands over the block temperatures, later calls pass NULL and the
+    library writes into the remembered vector.  Returns [n_steps, n]."""
+    L = run.lib.lib
+    tp, ta = run.vec(init_block_temp)
+    pp, pa = run.vec()
+    out = []
+    for k, p in enumerate(powers):
+        pa[:] = p
+        L.compute_temp(run.model, pp, tp if k == 0 else None, dt)
+        out.append(ta.copy())
+    return np.array(out)
