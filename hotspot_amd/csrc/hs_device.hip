@@ -1,0 +1,1031 @@
+// Kernels and solve drivers of the grid thermal model for gfx950.
+//
+// Discrete model (the reference's per-cell form, temperature_grid.c:26-136,
+// :2426-2608, :2857-3601): for every grid cell and package node c
+//
+//   flux_c = sum_nbr g (T_nbr - T_c) [+ advection]      (conduction in)
+//   G T    = gam_c T_c - flux_c                          (linear operator)
+//   rhs_c  = P_c + gam_c * T_ambient [+ inlet source]
+//   steady:    G T = rhs                 transient: dT/dt = (rhs - G T) / C
+//
+// with g the face conductances (1/rx for row neighbours, 1/ry for column
+// neighbours, 1/rz[n] between layers n and n+1, per-cell values in detailed-3D
+// and microchannel models), gam the ambient conductance (sink and PCB cells),
+// and the package-node rings coupling boundary rows to the 12 (or 28) package
+// nodes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "hs_kernels.h"
+
+namespace hs {
+
+#define HIP_OK(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            fatal(std::string("HIP error ") + hipGetErrorString(e_) + " in " #x "\n");   \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Sums (or maxes) N values over the workgroup; the result is valid in thread 0.
+template <int N, bool MAX = false>
+__device__ __forceinline__ void block_reduce(double (&v)[N]) {
+    __shared__ double red[kBlock / kWave][N];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+    for (int q = 0; q < N; q++) v[q] = MAX ? wave_max(v[q]) : wave_sum(v[q]);
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < N; q++) red[w][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < N; q++) {
+            double a = red[0][q];
+            for (int k = 1; k < kBlock / kWave; k++) a = MAX ? fmax(a, red[k][q]) : a + red[k][q];
+            v[q] = a;
+        }
+    }
+    __syncthreads();
+}
+
+// Stores this workgroup's partials in its slot; returns true (in every thread)
+// for the last workgroup of the problem, which then sees all slots.
+template <int N>
+__device__ __forceinline__ bool publish_partials(const ProbDesc &d, int slot, const double (&v)[N]) {
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < N; q++) d.partial[slot * kRed + q] = v[q];
+        __threadfence();
+        unsigned t = atomicAdd(d.counter, 1u);
+        last = (t == (unsigned)d.ntiles);   // ntiles + 1 slots in all
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last;
+}
+
+// In the last workgroup: combine all slots in slot order (thread 0 result).
+template <int N, bool MAX = false>
+__device__ __forceinline__ void gather_partials(const ProbDesc &d, double (&v)[N]) {
+    const int nslots = d.ntiles + 1;
+#pragma unroll
+    for (int q = 0; q < N; q++) v[q] = MAX ? 0.0 : 0.0;
+    for (int s = threadIdx.x; s < nslots; s += kBlock)
+#pragma unroll
+        for (int q = 0; q < N; q++) {
+            double x = __builtin_nontemporal_load(&d.partial[s * kRed + q]);
+            v[q] = MAX ? fmax(v[q], x) : v[q] + x;
+        }
+    block_reduce<N, MAX>(v);
+    if (threadIdx.x == 0) atomicExch(d.counter, 0u);
+}
+
+// ---------------------------------------------------------------------------
+// geometry
+// ---------------------------------------------------------------------------
+struct Cell {
+    int n, i, j;
+    long c;
+    bool active;
+};
+
+__device__ __forceinline__ Cell tile_cell(const ProbDesc &d, int tile) {
+    const int tpl = d.nti * d.ntj;
+    const int n = tile / tpl;
+    const int rem = tile - n * tpl;
+    const int ti = rem / d.ntj, tj = rem - ti * d.ntj;
+    const int sh = d.tx_shift;
+    const int tx = threadIdx.x & ((1 << sh) - 1), ty = threadIdx.x >> sh;
+    const int i = ti * (kBlock >> sh) + ty, j = tj * (1 << sh) + tx;
+    Cell r;
+    r.n = n;
+    r.i = i;
+    r.j = j;
+    r.active = i < d.nr && j < d.nc;
+    r.c = ((long)n * d.nr + i) * d.nc + j;
+    return r;
+}
+
+template <typename T> struct Coef;
+template <> struct Coef<double> {
+    __device__ static const double *E(const ProbDesc &d) { return d.gE; }
+    __device__ static const double *S(const ProbDesc &d) { return d.gS; }
+    __device__ static const double *D(const ProbDesc &d) { return d.gD; }
+    __device__ static const double *cinv(const ProbDesc &d) { return d.cinvc; }
+    __device__ static const double *dinv(const ProbDesc &d) { return d.dinvc; }
+    __device__ static const double *aN(const ProbDesc &d) { return d.aN; }
+    __device__ static const double *aS(const ProbDesc &d) { return d.aS; }
+    __device__ static const double *aE(const ProbDesc &d) { return d.aE; }
+    __device__ static const double *aW(const ProbDesc &d) { return d.aW; }
+    __device__ static const double *aSelf(const ProbDesc &d) { return d.aSelf; }
+};
+template <> struct Coef<float> {
+    __device__ static const float *E(const ProbDesc &d) { return d.fE; }
+    __device__ static const float *S(const ProbDesc &d) { return d.fS; }
+    __device__ static const float *D(const ProbDesc &d) { return d.fD; }
+    __device__ static const float *cinv(const ProbDesc &d) { return d.fcinv; }
+    __device__ static const float *dinv(const ProbDesc &d) { return d.fdinv; }
+    __device__ static const float *aN(const ProbDesc &d) { return d.faN; }
+    __device__ static const float *aS(const ProbDesc &d) { return d.faS; }
+    __device__ static const float *aE(const ProbDesc &d) { return d.faE; }
+    __device__ static const float *aW(const ProbDesc &d) { return d.faW; }
+    __device__ static const float *aSelf(const ProbDesc &d) { return d.faSelf; }
+};
+
+// input loaders: plain vector, or an RK stage base + a*k
+template <typename T> struct LdPlain {
+    const T *__restrict__ x;
+    __device__ __forceinline__ T operator()(long c) const { return x[c]; }
+};
+template <typename T> struct LdStage {
+    const T *__restrict__ b;
+    const T *__restrict__ k;
+    T a;
+    __device__ __forceinline__ T operator()(long c) const { return b[c] + a * k[c]; }
+};
+template <typename T> struct LdDir {   // CG direction p = z + beta * p_old
+    const T *__restrict__ z;
+    const T *__restrict__ p;
+    T beta;
+    __device__ __forceinline__ T operator()(long c) const { return beta == T(0) ? z[c] : z[c] + beta * p[c]; }
+};
+
+// Conduction (and advection) into cell c, the diagonal of G at c and the
+// cell's ambient conductance.
+template <typename T, bool HET, class L>
+__device__ __forceinline__ void cell_terms(const ProbDesc &d, const L &ld, const Cell &q, T xc,
+                                           T &flux, T &diag, T &gam) {
+    const int nr = d.nr, nc = d.nc, nl = d.nl, n = q.n, i = q.i, j = q.j;
+    const long c = q.c, plane = (long)nr * nc, ncell = plane * nl;
+    flux = T(0);
+    diag = T(0);
+    if (!HET) {
+        const T gx = (T)d.gx[n], gy = (T)d.gy[n];
+        if (i > 0) { flux += gx * (ld(c - nc) - xc); diag += gx; }
+        if (i < nr - 1) { flux += gx * (ld(c + nc) - xc); diag += gx; }
+        if (j < nc - 1) { flux += gy * (ld(c + 1) - xc); diag += gy; }
+        if (j > 0) { flux += gy * (ld(c - 1) - xc); diag += gy; }
+        if (n > 0) { const T g = (T)d.gz[n - 1]; flux += g * (ld(c - plane) - xc); diag += g; }
+        if (n < nl - 1) { const T g = (T)d.gz[n]; flux += g * (ld(c + plane) - xc); diag += g; }
+    } else {
+        const T *gE = Coef<T>::E(d), *gS = Coef<T>::S(d), *gD = Coef<T>::D(d);
+        const T xN = i > 0 ? ld(c - nc) : T(0), xS = i < nr - 1 ? ld(c + nc) : T(0);
+        const T xE = j < nc - 1 ? ld(c + 1) : T(0), xW = j > 0 ? ld(c - 1) : T(0);
+        if (i > 0) { const T g = gS[c - nc]; flux += g * (xN - xc); diag += g; }
+        if (i < nr - 1) { const T g = gS[c]; flux += g * (xS - xc); diag += g; }
+        if (j < nc - 1) { const T g = gE[c]; flux += g * (xE - xc); diag += g; }
+        if (j > 0) { const T g = gE[c - 1]; flux += g * (xW - xc); diag += g; }
+        if (n > 0) { const T g = gD[c - plane]; flux += g * (ld(c - plane) - xc); diag += g; }
+        if (n < nl - 1) { const T g = gD[c]; flux += g * (ld(c + plane) - xc); diag += g; }
+        if (d.has_adv) {
+            // coolant advection Cv Q(nbr->c) (T_nbr + T_c)/2 and the outlet sink
+            // (temperature_grid.c:3256-3590); a* hold Cv Q / 2 per face
+            const T an = Coef<T>::aN(d)[c], as = Coef<T>::aS(d)[c];
+            const T ae = Coef<T>::aE(d)[c], aw = Coef<T>::aW(d)[c], self = Coef<T>::aSelf(d)[c];
+            flux += an * (xN + xc) + as * (xS + xc) + ae * (xE + xc) + aw * (xW + xc) + self * xc;
+            diag -= an + as + ae + aw + self;
+        }
+    }
+    const int *rn = d.ring_node[n];
+    if (i == 0 && rn[0] >= 0) { const T g = (T)d.ring_gns[n]; flux += g * (ld(ncell + rn[0]) - xc); diag += g; }
+    if (i == nr - 1 && rn[1] >= 0) { const T g = (T)d.ring_gns[n]; flux += g * (ld(ncell + rn[1]) - xc); diag += g; }
+    if (j == nc - 1 && rn[2] >= 0) { const T g = (T)d.ring_gew[n]; flux += g * (ld(ncell + rn[2]) - xc); diag += g; }
+    if (j == 0 && rn[3] >= 0) { const T g = (T)d.ring_gew[n]; flux += g * (ld(ncell + rn[3]) - xc); diag += g; }
+    gam = (T)d.gamb[n];
+    diag += gam;
+}
+
+// Diagonal of G at a cell without reading the state (pointwise kernels).
+template <typename T, bool HET>
+__device__ __forceinline__ T cell_dinv(const ProbDesc &d, const Cell &q) {
+    if (HET) return Coef<T>::dinv(d)[q.c];
+    const int nr = d.nr, nc = d.nc, nl = d.nl, n = q.n, i = q.i, j = q.j;
+    const T gx = (T)d.gx[n], gy = (T)d.gy[n];
+    T diag = (T)d.gamb[n];
+    if (i > 0) diag += gx;
+    if (i < nr - 1) diag += gx;
+    if (j < nc - 1) diag += gy;
+    if (j > 0) diag += gy;
+    if (n > 0) diag += (T)d.gz[n - 1];
+    if (n < nl - 1) diag += (T)d.gz[n];
+    const int *rn = d.ring_node[n];
+    if (i == 0 && rn[0] >= 0) diag += (T)d.ring_gns[n];
+    if (i == nr - 1 && rn[1] >= 0) diag += (T)d.ring_gns[n];
+    if (j == nc - 1 && rn[2] >= 0) diag += (T)d.ring_gew[n];
+    if (j == 0 && rn[3] >= 0) diag += (T)d.ring_gew[n];
+    return T(1) / diag;
+}
+
+template <typename T, bool HET>
+__device__ __forceinline__ T cell_cinv(const ProbDesc &d, const Cell &q) {
+    return HET ? Coef<T>::cinv(d)[q.c] : (T)d.cinv[q.n];
+}
+
+// Package nodes: flux into node k from its ring of boundary cells and from
+// the package nodes it couples to (single_iteration_steady_pack :2090-2393,
+// slope_fn_pack :2857-3093); results in shared memory.  All threads call it.
+template <typename T, class L>
+__device__ __forceinline__ void package_terms(const ProbDesc &d, const L &ld, bool transient,
+                                              T *xflux, T *xdiag) {
+    const int nr = d.nr, nc = d.nc;
+    const long plane = (long)nr * nc, ncell = plane * d.nl;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    for (int k = w; k < d.nx; k += kBlock / kWave) {
+        const int L_ = d.x_ring_layer[k];
+        const T xk = ld(ncell + k);
+        double s = 0.0;
+        int m = 0;
+        if (L_ >= 0) {
+            const int e = d.x_ring_edge[k];
+            m = (e < 2) ? nc : nr;
+            const long b = (long)L_ * plane;
+            for (int t = lane; t < m; t += kWave) {
+                long c;
+                if (e == 0) c = b + t;
+                else if (e == 1) c = b + (long)(nr - 1) * nc + t;
+                else if (e == 2) c = b + (long)t * nc + (nc - 1);
+                else c = b + (long)t * nc;
+                s += (double)(ld(c) - xk);
+            }
+            s = wave_sum(s);
+        }
+        if (lane == 0) {
+            const T gr = L_ >= 0 ? (T)d.x_ring_g[k] : T(0);
+            T flux = gr * (T)s, diag = gr * (T)m;
+            for (int q = 0; q < d.x_ncpl[k]; q++) {
+                if (transient && d.x_cpl_steady_only[k][q]) continue;
+                const T g = (T)d.x_cpl_g[k][q];
+                flux += g * (ld(ncell + d.x_cpl_node[k][q]) - xk);
+                diag += g;
+            }
+            diag += (T)d.x_gamb[k];
+            xflux[k] = flux;
+            xdiag[k] = diag;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ bool problem_skip(const ProbDesc &d, int flag_slot) {
+    return flag_slot >= 0 && d.scal[flag_slot] != 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// block <-> grid translation (xlate_vector_b2g :1888-1929, xlate_temp_g2b
+// :1932-2015)
+// ---------------------------------------------------------------------------
+// out = sum over the cell's blocks of occ * v_u (/ area_u * cell area for
+// power); with add_sources the ambient / inlet constants are added (rhs).
+__global__ void __launch_bounds__(kBlock) k_b2g(const ProbDesc *__restrict__ D,
+                                                const WorkItem *__restrict__ W, int type,
+                                                int out_buf, int add_sources) {
+    const WorkItem wi = W[blockIdx.x];
+    const ProbDesc &d = D[wi.prob];
+    double *out = d.b64[out_buf];
+    const double *b = d.blk_in;
+    const long ncell = (long)d.nl * d.nr * d.nc;
+    if (wi.tile < d.ntiles) {
+        const Cell q = tile_cell(d, wi.tile);
+        if (!q.active) return;
+        const long cc = (long)q.i * d.nc + q.j;
+        const int *ptr = d.map_ptr[q.n];
+        const int *unit = d.map_unit[q.n];
+        const double *occ = d.map_occ[q.n];
+        const double *vb = b + d.layer_base[q.n];
+        double val = 0.0;
+        if (type == V_POWER) {
+            // power density of each block times its occupancy, times the cell area
+            const double *area = d.unit_area[q.n];
+            for (int e = ptr[cc]; e < ptr[cc + 1]; e++) val += (occ[e] * vb[unit[e]]) / area[unit[e]];
+            val *= d.cell_area;
+        } else {
+            for (int e = ptr[cc]; e < ptr[cc + 1]; e++) val += occ[e] * vb[unit[e]];
+        }
+        if (add_sources) {
+            val += d.gamb[q.n] * d.ambient;
+            if (d.bconst) val += d.bconst[q.c];
+        }
+        out[q.c] = val;
+    } else {
+        const int k = threadIdx.x;
+        if (k < d.nx) {
+            double v = (type == V_POWER) ? 0.0 : b[d.nblocks + k];
+            if (add_sources) v += d.x_gamb[k] * d.ambient;
+            out[ncell + k] = v;
+        }
+    }
+}
+
+// One wavefront per block of the vector; work items: (problem, first block).
+__global__ void __launch_bounds__(kBlock) k_g2b(const ProbDesc *__restrict__ D,
+                                                const WorkItem *__restrict__ W, int in_buf) {
+    const WorkItem wi = W[blockIdx.x];
+    const ProbDesc &d = D[wi.prob];
+    const double *g = d.b64[in_buf];
+    const long plane = (long)d.nr * d.nc, ncell = plane * d.nl;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int u = wi.tile + w;
+    if (wi.tile == -1) {   // package nodes
+        if (threadIdx.x < d.nx) d.blk_out[d.nblocks + threadIdx.x] = g[ncell + threadIdx.x];
+        return;
+    }
+    if (u >= d.nblocks) return;
+    const int n = d.blk_layer[u];
+    const int4 r = d.blk_range[u];
+    const double *gl = g + n * plane;
+    double res;
+    if (d.map_mode == GRID_CENTER) {
+        const int ci1 = (r.x + r.y) / 2, cj1 = (r.z + r.w) / 2;
+        const int ci2 = ci1 - !((r.y - r.x) % 2), cj2 = cj1 - !((r.w - r.z) % 2);
+        res = (gl[(long)ci1 * d.nc + cj1] + gl[(long)ci2 * d.nc + cj1] + gl[(long)ci1 * d.nc + cj2] +
+               gl[(long)ci2 * d.nc + cj2]) / 4;
+    } else {
+        const int bw = r.w - r.z, cnt = (r.y - r.x) * bw;
+        double s = 0.0, mn = gl[(long)r.x * d.nc + r.z], mx = mn;
+        for (int t = lane; t < cnt; t += kWave) {
+            const double v = gl[(long)(r.x + t / bw) * d.nc + r.z + t % bw];
+            s += v;
+            mn = fmin(mn, v);
+            mx = fmax(mx, v);
+        }
+        s = wave_sum(s);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            mn = fmin(mn, __shfl_xor(mn, o, kWave));
+            mx = fmax(mx, __shfl_xor(mx, o, kWave));
+        }
+        res = d.map_mode == GRID_AVG ? s / cnt : d.map_mode == GRID_MIN ? mn : mx;
+    }
+    if (lane == 0) d.blk_out[u] = res;
+}
+
+// ---------------------------------------------------------------------------
+// steady: preconditioned CG (Jacobi), FP64 or FP32 (inner solve of the mixed
+// iterative refinement)
+// ---------------------------------------------------------------------------
+// r = rhs - G x; z = D^-1 r; partials r.z, r.r, rhs.rhs.  The last workgroup
+// initialises the problem's CG scalars.
+template <typename T, bool HET>
+__global__ void __launch_bounds__(kBlock) k_cg_init(const ProbDesc *__restrict__ D,
+                                                    const WorkItem *__restrict__ W, int xb,
+                                                    int rb, int zb, int rhsb, double rtol2,
+                                                    int max_iters) {
+    const WorkItem wi = W[blockIdx.x];
+    const ProbDesc &d = D[wi.prob];
+    if (problem_skip(d, S_DONE)) return;
+    T *x = sizeof(T) == 8 ? (T *)d.b64[xb] : (T *)d.b32[xb];
+    T *r = sizeof(T) == 8 ? (T *)d.b64[rb] : (T *)d.b32[rb];
+    T *z = sizeof(T) == 8 ? (T *)d.b64[zb] : (T *)d.b32[zb];
+    const T *rhs = sizeof(T) == 8 ? (const T *)d.b64[rhsb] : (const T *)d.b32[rhsb];
+    LdPlain<T> ld{x};
+    double v[3] = {0, 0, 0};
+    if (wi.tile < d.ntiles) {
+        const Cell q = tile_cell(d, wi.tile);
+        if (q.active) {
+            const T xc = x[q.c];
+            T flux, diag, gam;
+            cell_terms<T, HET>(d, ld, q, xc, flux, diag, gam);
+            const T rr = rhs[q.c] + flux - gam * xc;
+            const T zz = rr / diag;
+            r[q.c] = rr;
+            z[q.c] = zz;
+            v[0] = (double)rr * zz;
+            v[1] = (double)rr * rr;
+            v[2] = (double)rhs[q.c] * rhs[q.c];
+        }
+    } else {
+        __shared__ T xf[kMaxExtra], xd[kMaxExtra];
+        package_terms<T>(d, ld, false, xf, xd);
+        const int k = threadIdx.x;
+        if (k < d.nx) {
+            const long c = (long)d.nl * d.nr * d.nc + k;
+            const T xc = x[c];
+            const T rr = rhs[c] + xf[k] - (T)d.x_gamb[k] * xc;
+            const T zz = rr / xd[k];
+            r[c] = rr;
+            z[c] = zz;
+            v[0] = (double)rr * zz;
+            v[1] = (double)rr * rr;
+            v[2] = (double)rhs[c] * rhs[c];
+        }
+    }
+    block_reduce<3>(v);
+    if (publish_partials<3>(d, wi.tile, v)) {
+        gather_partials<3>(d, v);
+        if (threadIdx.x == 0) {
+            double *s = d.scal;
+            s[S_RZ] = v[0];
+            s[S_RR] = v[1];
+            s[S_BB] = v[2];
+            s[S_TOL2] = rtol2 * v[2];
+            s[S_BETA] = 0.0;
+            s[S_ITERS] = 0.0;
+            s[S_PPAR] = 0.0;
+            s[S_DONE] = (v[1] <= s[S_TOL2] || max_iters <= 0) ? 1.0 : 0.0;
+        }
+    }
+}
+
+// p = z + beta p_old;  q = G p;  partial p.q -> alpha = rz / pq
+template <typename T, bool HET>
+__global__ void __launch_bounds__(kBlock) k_cg_dir(const ProbDesc *__restrict__ D,
+                                                   const WorkItem *__restrict__ W, int zb,
+                                                   int p0b, int qb, int done_slot) {
+    const WorkItem wi = W[blockIdx.x];
+    const ProbDesc &d = D[wi.prob];
+    if (problem_skip(d, done_slot) || (done_slot != S_DONE && problem_skip(d, S_DONE))) return;
+    const int par = (int)d.scal[S_PPAR];
+    auto buf = [&](int b) -> T * { return sizeof(T) == 8 ? (T *)d.b64[b] : (T *)d.b32[b]; };
+    const T *z = buf(zb), *pold = buf(p0b + par);
+    T *pnew = buf(p0b + 1 - par), *qv = buf(qb);
+    LdDir<T> ld{z, pold, (T)d.scal[S_BETA]};
+    double v[1] = {0};
+    if (wi.tile < d.ntiles) {
+        const Cell q = tile_cell(d, wi.tile);
+        if (q.active) {
+            const T pc = ld(q.c);
+            T flux, diag, gam;
+            cell_terms<T, HET>(d, ld, q, pc, flux, diag, gam);
+            const T Gp = gam * pc - flux;
+            pnew[q.c] = pc;
+            qv[q.c] = Gp;
+            v[0] = (double)pc * Gp;
+        }
+    } else {
+        __shared__ T xf[kMaxExtra], xd[kMaxExtra];
+        package_terms<T>(d, ld, false, xf, xd);
+        const int k = threadIdx.x;
+        if (k < d.nx) {
+            const long c = (long)d.nl * d.nr * d.nc + k;
+            const T pc = ld(c);
+            const T Gp = (T)d.x_gamb[k] * pc - xf[k];
+            pnew[c] = pc;
+            qv[c] = Gp;
+            v[0] = (double)pc * Gp;
+        }
+    }
+    block_reduce<1>(v);
+    if (publish_partials<1>(d, wi.tile, v)) {
+        gather_partials<1>(d, v);
+        if (threadIdx.x == 0) {
+            d.scal[S_PQ] = v[0];
+            d.scal[S_ALPHA] = v[0] != 0.0 ? d.scal[S_RZ] / v[0] : 0.0;
+        }
+    }
+}
+
+// x += alpha p; r -= alpha q; z = D^-1 r; partials r.z, r.r -> beta, done
+template <typename T, bool HET>
+__global__ void __launch_bounds__(kBlock) k_cg_upd(const ProbDesc *__restrict__ D,
+                                                   const WorkItem *__restrict__ W, int xb,
+                                                   int rb, int zb, int p0b, int qb, int done_slot,
+                                                   int tol_slot, int max_iters) {
+    const WorkItem wi = W[blockIdx.x];
+    const ProbDesc &d = D[wi.prob];
+    if (problem_skip(d, done_slot) || (done_slot != S_DONE && problem_skip(d, S_DONE))) return;
+    const int par = (int)d.scal[S_PPAR];
+    auto buf = [&](int b) -> T * { return sizeof(T) == 8 ? (T *)d.b64[b] : (T *)d.b32[b]; };
+    T *x = buf(xb), *r = buf(rb), *z = buf(zb);
+    const T *p = buf(p0b + 1 - par), *qv = buf(qb);
+    const T alpha = (T)d.scal[S_ALPHA];
+    double v[2] = {0, 0};
+    auto upd = [&](long c, T dinv) {
+        x[c] += alpha * p[c];
+        const T rr = r[c] - alpha * qv[c];
+        const T zz = rr * dinv;
+        r[c] = rr;
+        z[c] = zz;
+        v[0] = (double)rr * zz;
+        v[1] = (double)rr * rr;
+    };
+    if (wi.tile < d.ntiles) {
+        const Cell q = tile_cell(d, wi.tile);
+        if (q.active) upd(q.c, cell_dinv<T, HET>(d, q));
+    } else {
+        if (threadIdx.x < d.nx) {
+            // diagonal of a package node: ring and coupling conductances + ambient
+            const int k = threadIdx.x;
+            const int L_ = d.x_ring_layer[k];
+            T diag = (T)d.x_gamb[k];
+            if (L_ >= 0) diag += (T)d.x_ring_g[k] * (T)(d.x_ring_edge[k] < 2 ? d.nc : d.nr);
+            for (int q = 0; q < d.x_ncpl[k]; q++) diag += (T)d.x_cpl_g[k][q];
+            upd((long)d.nl * d.nr * d.nc + k, T(1) / diag);
+        }
+    }
+    block_reduce<2>(v);
+    if (publish_partials<2>(d, wi.tile, v)) {
+        gather_partials<2>(d, v);
+        if (threadIdx.x == 0) {
+            double *s = d.scal;
+            s[S_BETA] = s[S_RZ] != 0.0 ? v[0] / s[S_RZ] : 0.0;
+            s[S_RZ] = v[0];
+            s[S_RR] = v[1];
+            s[S_PPAR] = 1.0 - s[S_PPAR];
+            s[S_ITERS] += 1.0;
+            if (v[1] <= s[tol_slot] || s[S_ITERS] >= max_iters || !(v[1] == v[1])) s[done_slot] = 1.0;
+        }
+    }
+}
+
+// Mixed precision: FP64 residual r = rhs - G x, its FP32 copy as the inner
+// right-hand side, e = 0, inner CG initialised.  Outer convergence in FP64.
+template <bool HET>
+__global__ void __launch_bounds__(kBlock) k_ir_resid(const ProbDesc *__restrict__ D,
+                                                     const WorkItem *__restrict__ W, double rtol2,
+                                                     double inner_rtol2, int max_refine) {
+    const WorkItem wi = W[blockIdx.x];
+    const ProbDesc &d = D[wi.prob];
+    if (problem_skip(d, S_DONE)) return;
+    const double *x = d.b64[B_STEADY], *rhs = d.b64[B_RHS];
+    float *r32 = d.b32[F_R], *z32 = d.b32[F_Z], *e32 = d.b32[F_X];
+    LdPlain<double> ld{x};
+    double v[3] = {0, 0, 0};
+    auto put = [&](long c, double rr, double diag, double rhsc) {
+        const float rf = (float)rr;
+        const float zf = (float)(rr / diag);
+        r32[c] = rf;
+        z32[c] = zf;
+        e32[c] = 0.0f;
+        v[0] = (double)rf * zf;
+        v[1] = rr * rr;
+        v[2] = rhsc * rhsc;
+    };
+    if (wi.tile < d.ntiles) {
+        const Cell q = tile_cell(d, wi.tile);
+        if (q.active) {
+            const double xc = x[q.c];
+            double flux, diag, gam;
+            cell_terms<double, HET>(d, ld, q, xc, flux, diag, gam);
+            put(q.c, rhs[q.c] + flux - gam * xc, diag, rhs[q.c]);
+        }
+    } else {
+        __shared__ double xf[kMaxExtra], xd[kMaxExtra];
+        package_terms<double>(d, ld, false, xf, xd);
+        const int k = threadIdx.x;
+        if (k < d.nx) {
+            const long c = (long)d.nl * d.nr * d.nc + k;
+            put(c, rhs[c] + xf[k] - d.x_gamb[k] * x[c], xd[k], rhs[c]);
+        }
+    }
+    block_reduce<3>(v);
+    if (publish_partials<3>(d, wi.tile, v)) {
+        gather_partials<3>(d, v);
+        if (threadIdx.x == 0) {
+            double *s = d.scal;
+            s[S_RR] = v[1];
+            s[S_BB] = v[2];
+            s[S_TOL2] = rtol2 * v[2];
+            s[S_RZ] = v[0];
+            s[S_RR0] = v[1];
+            s[S_ITOL2] = inner_rtol2 * v[1];
+            s[S_BETA] = 0.0;
+            s[S_ITERS] = 0.0;
+            s[S_PPAR] = 0.0;
+            const bool done = v[1] <= s[S_TOL2] || s[S_NREFINE] >= max_refine || !(v[1] == v[1]);
+            s[S_DONE] = done ? 1.0 : 0.0;
+            s[S_IDONE] = done ? 1.0 : 0.0;
+            s[S_NREFINE] += 1.0;
+        }
+    }
+}
+
+// x (FP64) += e (FP32)
+__global__ void __launch_bounds__(kBlock) k_ir_correct(const ProbDesc *__restrict__ D,
+                                                       const WorkItem *__restrict__ W) {
+    const WorkItem wi = W[blockIdx.x];
+    const ProbDesc &d = D[wi.prob];
+    if (problem_skip(d, S_DONE)) return;
+    double *x = d.b64[B_STEADY];
+    const float *e = d.b32[F_X];
+    if (wi.tile < d.ntiles) {
+        const Cell q = tile_cell(d, wi.tile);
+        if (q.active) x[q.c] += (double)e[q.c];
+    } else if (threadIdx.x < d.nx) {
+        const long c = (long)d.nl * d.nr * d.nc + threadIdx.x;
+        x[c] += (double)e[c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// transient: the reference's adaptive RK4 (RCutil.c:440-640) with the step
+// controller on the device
+// ---------------------------------------------------------------------------
+enum StageMode { ST_K = 0, ST_FIRST = 1, ST_MID = 2, ST_LAST = 3, ST_LAST_ERR = 4 };
+
+// One slope evaluation k = (rhs - G s) / C at s = base + a*h*kin, fused with
+// the RK4 combination selected by mode:
+//   ST_K        kout = k
+//   ST_FIRST    kout = k, acc = kin + 2k                (k2 stage)
+//   ST_MID      kout = k, acc += 2k                     (k3 stage)
+//   ST_LAST     out = base + hc*h*(acc + k)/6           (k4 stage)
+//   ST_LAST_ERR as ST_LAST but only err = max|out - cmp| is kept
+// Buffers are FP64 ids; base/out/cmp ids >= 100 mean "state y, parity-relative"
+// (100 = current y, 101 = the other one).
+template <bool HET>
+__global__ void __launch_bounds__(kBlock) k_rk_stage(const ProbDesc *__restrict__ D,
+                                                     const WorkItem *__restrict__ W, int mode,
+                                                     int base_b, int kin_b, double a_mul,
+                                                     int kout_b, int out_b, double hc_mul,
+                                                     int cmp_b) {
+    const WorkItem wi = W[blockIdx.x];
+    const ProbDesc &d = D[wi.prob];
+    const double st = d.scal[S_STATUS];
+    if (st == (double)RK_DONE) return;
+    if (mode == ST_K && kin_b < 0 && st != (double)RK_NEED_K1 && base_b >= 100) return;   // k1 reuse
+    const int ypar = (int)d.scal[S_YPAR];
+    auto id = [&](int b) { return b >= 100 ? B_Y0 + ((b - 100) ^ ypar) : b; };
+    const double h = d.scal[S_H];
+    const double *base = d.b64[id(base_b)];
+    const double *kin = kin_b >= 0 ? d.b64[kin_b] : base;
+    const double a = kin_b >= 0 ? a_mul * h : 0.0;
+    double *kout = kout_b >= 0 ? d.b64[kout_b] : nullptr;
+    double *acc = d.b64[B_ACC];
+    double *out = out_b >= 0 ? d.b64[id(out_b)] : nullptr;
+    const double *cmp = cmp_b >= 0 ? d.b64[id(cmp_b)] : nullptr;
+    const double hc = hc_mul * h;
+    const double *rhs = d.b64[B_RHS];
+    LdStage<double> ld{base, kin, a};
+    double v[1] = {0};
+    auto fin = [&](long c, double k) {
+        switch (mode) {
+            case ST_K: kout[c] = k; break;
+            case ST_FIRST: kout[c] = k; acc[c] = kin[c] + 2 * k; break;
+            case ST_MID: kout[c] = k; acc[c] += 2 * k; break;
+            case ST_LAST: out[c] = base[c] + hc * (acc[c] + k) / 6.0; break;
+            default: {
+                const double o = base[c] + hc * (acc[c] + k) / 6.0;
+                v[0] = fmax(v[0], fabs(cmp[c] - o));
+            }
+        }
+    };
+    if (wi.tile < d.ntiles) {
+        const Cell q = tile_cell(d, wi.tile);
+        if (q.active) {
+            const double xc = ld(q.c);
+            double flux, diag, gam;
+            cell_terms<double, HET>(d, ld, q, xc, flux, diag, gam);
+            fin(q.c, cell_cinv<double, HET>(d, q) * (rhs[q.c] + flux - gam * xc));
+        }
+    } else {
+        __shared__ double xf[kMaxExtra], xd[kMaxExtra];
+        package_terms<double>(d, ld, true, xf, xd);
+        const int k = threadIdx.x;
+        if (k < d.nx) {
+            const long c = (long)d.nl * d.nr * d.nc + k;
+            fin(c, d.x_cinv[k] * (rhs[c] + xf[k] - d.x_gamb[k] * ld(c)));
+        }
+    }
+    if (mode != ST_LAST_ERR) return;
+    block_reduce<1, true>(v);
+    if (publish_partials<1>(d, wi.tile, v)) {
+        gather_partials<1, true>(d, v);
+        if (threadIdx.x == 0) d.scal[S_ERR] = v[0];
+    }
+}
+
+// Step-size control of one attempt (rk4, RCutil.c:598-621) and the interval
+// loop of compute_temp_grid (temperature_grid.c:3676-3692).
+__global__ void k_rk_control(const ProbDesc *__restrict__ D, int nprob) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprob) return;
+    double *s = D[p].scal;
+    if (s[S_STATUS] == (double)RK_DONE) return;
+    const double h = s[S_H], mx = s[S_ERR];
+    double nh;
+    if (mx <= 0.01) {
+        nh = 0.95 * h * pow(fabs(0.01 / mx), 0.2);
+        if (nh > 5.0 * h) nh = 5.0 * h;
+    } else {
+        nh = 0.95 * h * pow(fabs(0.01 / mx), 0.25);
+        if (nh < h / 10.0) nh = h / 10.0;
+    }
+    if (nh < h) {          // retry with the smaller step, k1 as overwritten
+        s[S_H] = nh;
+        s[S_STATUS] = RK_ATTEMPT;
+        return;
+    }
+    // accept: y <- ytemp (the other parity buffer)
+    s[S_YPAR] = 1.0 - s[S_YPAR];
+    s[S_CALLS] += 1.0;
+    const double t = s[S_T];
+    nh = fmin(nh, s[S_DT] - t - h);
+    s[S_T] = t + h;
+    if (s[S_T] < s[S_DT] && nh >= kMinStep * kDelta) {
+        s[S_H] = nh;
+        s[S_STATUS] = RK_NEED_K1;
+    } else {
+        s[S_STATUS] = RK_DONE;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// transient: exponential (Taylor) integrator of the linear ODE
+//   y(t + tau) = y + sum_k tau^k/k! D_k,  D_1 = f(y),  D_{k+1} = -C^-1 G D_k
+// D_1 in FP64 (it is the residual); the higher terms in TD (FP32 in the
+// mixed form); each term kernel also reduces max|term| so the last workgroup
+// can stop the series once the tail bound is below the tolerance.
+// ---------------------------------------------------------------------------
+template <typename TD, bool HET>
+__global__ void __launch_bounds__(kBlock) k_tay_first(const ProbDesc *__restrict__ D,
+                                                      const WorkItem *__restrict__ W, int d0b,
+                                                      int accb, int sub) {
+    const WorkItem wi = W[blockIdx.x];
+    const ProbDesc &d = D[wi.prob];
+    if (sub >= (int)d.scal[S_TK]) return;
+    const int ypar = (int)d.scal[S_YPAR];
+    const double *y = d.b64[B_Y0 + ypar], *rhs = d.b64[B_RHS];
+    auto buf = [&](int b) -> TD * { return sizeof(TD) == 8 ? (TD *)d.b64[b] : (TD *)d.b32[b]; };
+    TD *dk = buf(d0b), *acc = buf(accb);
+    const double tau = d.scal[S_TAU];
+    LdPlain<double> ld{y};
+    double v[1] = {0};
+    auto fin = [&](long c, double f) {
+        dk[c] = (TD)f;
+        const double t = tau * f;
+        acc[c] = (TD)t;
+        v[0] = fmax(v[0], fabs(t));
+    };
+    if (wi.tile < d.ntiles) {
+        const Cell q = tile_cell(d, wi.tile);
+        if (q.active) {
+            const double xc = y[q.c];
+            double flux, diag, gam;
+            cell_terms<double, HET>(d, ld, q, xc, flux, diag, gam);
+            fin(q.c, cell_cinv<double, HET>(d, q) * (rhs[q.c] + flux - gam * xc));
+        }
+    } else {
+        __shared__ double xf[kMaxExtra], xd[kMaxExtra];
+        package_terms<double>(d, ld, true, xf, xd);
+        const int k = threadIdx.x;
+        if (k < d.nx) {
+            const long c = (long)d.nl * d.nr * d.nc + k;
+            fin(c, d.x_cinv[k] * (rhs[c] + xf[k] - d.x_gamb[k] * y[c]));
+        }
+    }
+    block_reduce<1, true>(v);
+    if (publish_partials<1>(d, wi.tile, v)) {
+        gather_partials<1, true>(d, v);
+        if (threadIdx.x == 0) {
+            d.scal[S_TMAX] = v[0];
+            d.scal[S_TDONE] = 0.0;
+        }
+    }
+}
+
+// term k+1 from term k (k >= 1): D' = C^-1 (flux(D) - gam D) = -C^-1 G D
+template <typename TD, bool HET>
+__global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict__ D,
+                                                     const WorkItem *__restrict__ W, int inb,
+                                                     int outb, int accb, int k, int sub,
+                                                     double tol) {
+    const WorkItem wi = W[blockIdx.x];
+    const ProbDesc &d = D[wi.prob];
+    if (sub >= (int)d.scal[S_TK] || problem_skip(d, S_TDONE)) return;
+    const double rho = d.scal[S_TMAX + 8];
+    auto buf = [&](int b) -> TD * { return sizeof(TD) == 8 ? (TD *)d.b64[b] : (TD *)d.b32[b]; };
+    const TD *din = buf(inb);
+    TD *dout = buf(outb), *acc = buf(accb);
+    const double tau = d.scal[S_TAU];
+    // coefficient tau^(k+1)/(k+1)!
+    double coef = tau;
+    for (int m = 2; m <= k + 1; m++) coef *= tau / m;
+    const TD cf = (TD)coef;
+    LdPlain<TD> ld{din};
+    double v[1] = {0};
+    auto fin = [&](long c, TD f) {
+        dout[c] = f;
+        const TD t = cf * f;
+        acc[c] += t;
+        v[0] = fmax(v[0], fabs((double)t));
+    };
+    if (wi.tile < d.ntiles) {
+        const Cell q = tile_cell(d, wi.tile);
+        if (q.active) {
+            const TD xc = din[q.c];
+            TD flux, diag, gam;
+            cell_terms<TD, HET>(d, ld, q, xc, flux, diag, gam);
+            fin(q.c, cell_cinv<TD, HET>(d, q) * (flux - gam * xc));
+        }
+    } else {
+        __shared__ TD xf[kMaxExtra], xd[kMaxExtra];
+        package_terms<TD>(d, ld, true, xf, xd);
+        const int kk = threadIdx.x;
+        if (kk < d.nx) {
+            const long c = (long)d.nl * d.nr * d.nc + kk;
+            fin(c, (TD)d.x_cinv[kk] * (xf[kk] - (TD)d.x_gamb[kk] * din[c]));
+        }
+    }
+    block_reduce<1, true>(v);
+    if (publish_partials<1>(d, wi.tile, v)) {
+        gather_partials<1, true>(d, v);
+        if (threadIdx.x == 0) {
+            // the next terms shrink at least by rho*tau/(m+1) each (||C^-1 G|| <= rho)
+            const double q = rho * tau / (k + 2);
+            const double tail = q < 1.0 ? v[0] * q / (1.0 - q) : 1e300;
+            d.scal[S_TMAX] = v[0];
+            if (tail <= tol) d.scal[S_TDONE] = 1.0;
+        }
+    }
+}
+
+// y += acc
+template <typename TD>
+__global__ void __launch_bounds__(kBlock) k_tay_final(const ProbDesc *__restrict__ D,
+                                                      const WorkItem *__restrict__ W, int accb,
+                                                      int sub) {
+    const WorkItem wi = W[blockIdx.x];
+    const ProbDesc &d = D[wi.prob];
+    if (sub >= (int)d.scal[S_TK]) return;
+    const int ypar = (int)d.scal[S_YPAR];
+    double *y = d.b64[B_Y0 + ypar];
+    const TD *acc = sizeof(TD) == 8 ? (const TD *)d.b64[accb] : (const TD *)d.b32[accb];
+    if (wi.tile < d.ntiles) {
+        const Cell q = tile_cell(d, wi.tile);
+        if (q.active) y[q.c] += (double)acc[q.c];
+    } else if (threadIdx.x < d.nx) {
+        const long c = (long)d.nl * d.nr * d.nc + threadIdx.x;
+        y[c] += (double)acc[c];
+    }
+}
+
+// slope on a given state (operator-level queries): out = (rhs - G_t x)/C
+template <bool HET>
+__global__ void __launch_bounds__(kBlock) k_slope(const ProbDesc *__restrict__ D,
+                                                  const WorkItem *__restrict__ W, int xb, int outb) {
+    const WorkItem wi = W[blockIdx.x];
+    const ProbDesc &d = D[wi.prob];
+    const double *x = d.b64[xb], *rhs = d.b64[B_RHS];
+    double *out = d.b64[outb];
+    LdPlain<double> ld{x};
+    if (wi.tile < d.ntiles) {
+        const Cell q = tile_cell(d, wi.tile);
+        if (q.active) {
+            const double xc = x[q.c];
+            double flux, diag, gam;
+            cell_terms<double, HET>(d, ld, q, xc, flux, diag, gam);
+            out[q.c] = cell_cinv<double, HET>(d, q) * (rhs[q.c] + flux - gam * xc);
+        }
+    } else {
+        __shared__ double xf[kMaxExtra], xd[kMaxExtra];
+        package_terms<double>(d, ld, true, xf, xd);
+        const int k = threadIdx.x;
+        if (k < d.nx) {
+            const long c = (long)d.nl * d.nr * d.nc + k;
+            out[c] = d.x_cinv[k] * (rhs[c] + xf[k] - d.x_gamb[k] * x[c]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// conductance assembly of heterogeneous models (find_res :26-136,
+// find_res_3D :144-173, find_cap_3D :176-186) from per-cell resistances
+// ---------------------------------------------------------------------------
+struct AsmIn {
+    const double *rx, *ry, *rz, *cap;   // per cell (detailed values or layer values)
+    const int *fluid;                   // per cell: 1 for coolant cells
+    const double *htc;                  // per layer (0 if not microchannel)
+    const double *thick;                // per layer
+    const double *press;                // per cell pressure (microchannel layers)
+    const int *types;                   // per cell network cell type
+    const double *hydro, *cv, *tin;     // per layer: hydraulic conductance, coolant capacity, inlet T
+    int nl, nr, nc, detailed;
+    double cw, ch;
+};
+
+__global__ void k_assemble(AsmIn in, double *gE, double *gS, double *gD, double *cinv,
+                           double *aN, double *aS, double *aE, double *aW, double *aSelf,
+                           double *bconst) {
+    const long plane = (long)in.nr * in.nc;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= plane * in.nl) return;
+    const int n = (int)(c / plane);
+    const long cc = c - n * plane;
+    const int i = (int)(cc / in.nc), j = (int)(cc % in.nc);
+    const int f = in.fluid[c];
+    // lateral faces: detailed half + half, or solid half + 1/(htc A) for a
+    // coolant/wall pair (the fluid layer's htc, face area cw*t or ch*t)
+    auto lat = [&](long o, const double *r, double area_len) {
+        const int fo = in.fluid[o];
+        if (f != fo) {
+            const double rs = f ? r[o] : r[c];
+            return 1.0 / (rs / 2.0 + 1.0 / (in.htc[n] * area_len * in.thick[n]));
+        }
+        return 1.0 / (r[c] / 2.0 + r[o] / 2.0);
+    };
+    gE[c] = j < in.nc - 1 ? lat(c + 1, in.ry, in.ch) : 0.0;
+    gS[c] = i < in.nr - 1 ? lat(c + in.nc, in.rx, in.cw) : 0.0;
+    if (n < in.nl - 1) {
+        const long o = c + plane;
+        const int fo = in.fluid[o];
+        double r;
+        if (f != fo) {
+            const double h = in.htc[n + 1] != 0.0 ? in.htc[n + 1] : in.htc[n];
+            if (n == 0) r = in.rz[c] + 1.0 / (h * in.ch * in.cw);
+            else r = (f ? in.rz[o] : in.rz[c]) / 2.0 + 1.0 / (h * in.ch * in.cw);
+        } else if (in.detailed) {
+            r = n == 0 ? in.rz[c] + in.rz[o] / 2.0 : in.rz[c] / 2.0 + in.rz[o] / 2.0;
+        } else {
+            r = in.rz[c];
+        }
+        gD[c] = 1.0 / r;
+    } else {
+        gD[c] = 0.0;
+    }
+    cinv[c] = 1.0 / in.cap[c];
+    // advection (temperature_grid.c:3256-3590): a_dir = Cv Q(nbr->c) / 2
+    double an = 0, as = 0, ae = 0, aw = 0, self = 0, b = 0;
+    if (f && in.hydro[n] != 0.0) {
+        const double hC = in.hydro[n], Cv = in.cv[n];
+        const double pc = in.press[c];
+        double inflow = 0.0;
+        auto q = [&](long o) { return (in.press[o] - pc) * hC; };
+        if (i > 0 && in.fluid[c - in.nc]) { const double Q = q(c - in.nc); an = Cv * Q / 2.0; inflow += Q; }
+        if (i < in.nr - 1 && in.fluid[c + in.nc]) { const double Q = q(c + in.nc); as = Cv * Q / 2.0; inflow += Q; }
+        if (j > 0 && in.fluid[c - 1]) { const double Q = q(c - 1); aw = Cv * Q / 2.0; inflow += Q; }
+        if (j < in.nc - 1 && in.fluid[c + 1]) { const double Q = q(c + 1); ae = Cv * Q / 2.0; inflow += Q; }
+        const bool edge = i == 0 || i == in.nr - 1 || j == 0 || j == in.nc - 1;
+        const int t = in.types[c];
+        if (t == CELL_INLET && edge) b = Cv * (-inflow) * in.tin[n];
+        else if (t == CELL_OUTLET && edge) self = Cv * (-inflow);
+    }
+    aN[c] = an;
+    aS[c] = as;
+    aE[c] = ae;
+    aW[c] = aw;
+    aSelf[c] = self;
+    bconst[c] = b;
+}
+
+// FP32 copies of the coefficient arrays and the inverse diagonal of G
+__global__ void k_coef_finish(const ProbDesc *__restrict__ D, long ncell, float *fE, float *fS,
+                              float *fD, float *fcinv, float *faN, float *faS, float *faE,
+                              float *faW, float *faSelf, double *dinv, float *fdinv) {
+    const ProbDesc &d = D[0];
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const long plane = (long)d.nr * d.nc;
+    Cell q;
+    q.n = (int)(c / plane);
+    const long cc = c - q.n * plane;
+    q.i = (int)(cc / d.nc);
+    q.j = (int)(cc % d.nc);
+    q.c = c;
+    q.active = true;
+    fE[c] = (float)d.gE[c];
+    fS[c] = (float)d.gS[c];
+    fD[c] = (float)d.gD[c];
+    fcinv[c] = (float)d.cinvc[c];
+    faN[c] = (float)d.aN[c];
+    faS[c] = (float)d.aS[c];
+    faE[c] = (float)d.aE[c];
+    faW[c] = (float)d.aW[c];
+    faSelf[c] = (float)d.aSelf[c];
+    // diagonal of G: conduction + ring + ambient - advection self terms
+    double diag = d.gamb[q.n];
+    const int nr = d.nr, nc = d.nc, nl = d.nl, i = q.i, j = q.j, n = q.n;
+    if (i > 0) diag += d.gS[c - nc];
+    if (i < nr - 1) diag += d.gS[c];
+    if (j < nc - 1) diag += d.gE[c];
+    if (j > 0) diag += d.gE[c - 1];
+    if (n > 0) diag += d.gD[c - plane];
+    if (n < nl - 1) diag += d.gD[c];
+    const int *rn = d.ring_node[n];
+    if (i == 0 && rn[0] >= 0) diag += d.ring_gns[n];
+    if (i == nr - 1 && rn[1] >= 0) diag += d.ring_gns[n];
+    if (j == nc - 1 && rn[2] >= 0) diag += d.ring_gew[n];
+    if (j == 0 && rn[3] >= 0) diag += d.ring_gew[n];
+    diag -= d.aN[c] + d.aS[c] + d.aE[c] + d.aW[c] + d.aSelf[c];
+    dinv[c] = 1.0 / diag;
+    fdinv[c] = (float)(1.0 / diag);
+}
+
+__global__ void k_fill(double *x, long n, double v) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) x[c] = v;
+}
+
+}  // namespace hs
+
+#include "hs_engine.inl"

@@ -1,0 +1,772 @@
+// Host side of the device engine: per-model device state, descriptor
+// building, heterogeneous assembly, and the solve drivers.  Included at the
+// end of hs_device.hip (same translation unit as the kernels).
+#include <memory>
+#include <numeric>
+
+namespace hs {
+
+struct DevModel {
+    ProbDesc h{};                       // host copy of the descriptor
+    ProbDesc *d_desc = nullptr;
+    WorkItem *d_work = nullptr;         // cell tiles + package item
+    int nwork = 0;
+    WorkItem *d_gwork = nullptr;        // grid -> block items
+    int ngwork = 0;
+    std::vector<void *> allocs;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double *h_scal = nullptr;           // pinned copy of the scalars
+    double *h_blk = nullptr;            // pinned block vector staging
+    int nblk = 0;
+    long vlen = 0;
+    double rho = 0.0;                   // bound on ||C^-1 G||_inf (transient)
+    bool hetero = false, has_c = false;
+
+    template <typename T> T *alloc(size_t n) {
+        void *p = nullptr;
+        HIP_OK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+        allocs.push_back(p);
+        return (T *)p;
+    }
+    template <typename T> T *upload(const T *src, size_t n) {
+        T *p = alloc<T>(n);
+        if (n) HIP_OK(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+        return p;
+    }
+    ~DevModel() {
+        for (void *p : allocs) (void)hipFree(p);
+        if (h_scal) (void)hipHostFree(h_scal);
+        if (h_blk) (void)hipHostFree(h_blk);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+void dev_release(Model &M) {
+    delete M.dev;
+    M.dev = nullptr;
+}
+
+template <typename K, typename... A>
+static inline void launch(K kern, int nblocks, hipStream_t s, A... args) {
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kBlock), 0, s, args...);
+}
+
+static int ilog2(int v) {
+    int s = 0;
+    while ((1 << (s + 1)) <= v) s++;
+    return s;
+}
+
+// Tile shape: TX = min(nc, 64) consecutive columns (one wavefront reads a
+// contiguous row segment), TY = 256 / TX rows.
+static void set_tiling(ProbDesc &h) {
+    const int tx = std::min(h.nc, 64);
+    h.tx_shift = ilog2(tx);
+    const int ty = kBlock >> h.tx_shift;
+    h.ntj = (h.nc + tx - 1) / tx;
+    h.nti = (h.nr + ty - 1) / ty;
+    h.ntiles = h.nl * h.nti * h.ntj;
+}
+
+// Layer and package-node coefficients of the descriptor from the host model.
+static void fill_coefficients(Model &M, ProbDesc &h) {
+    grid_model_t &g = *M.g;
+    const package_RC_t &pk = g.pack;
+    const int nl = g.n_layers, nr = g.rows, nc = g.cols;
+    const double cw = g.width / nc, ch = g.height / nr;
+    h.nl = nl;
+    h.nr = nr;
+    h.nc = nc;
+    h.nx = M.nx;
+    h.ambient = g.config.ambient;
+    h.cell_area = (g.width * g.height) / (nc * nr);
+    for (int n = 0; n < kMaxLayers; n++)
+        for (int e = 0; e < 4; e++) h.ring_node[n][e] = -1;
+    for (int n = 0; n < nl; n++) {
+        const layer_t &L = g.layers[n];
+        // row neighbours use rx, column neighbours ry; the vertical link
+        // between n and n+1 uses the upper layer's full rz (find_res :116-132)
+        h.gx[n] = 1.0 / L.rx;
+        h.gy[n] = 1.0 / L.ry;
+        h.gz[n] = n < nl - 1 ? 1.0 / L.rz : 0.0;
+        h.cinv[n] = g.c_ready ? 1.0 / L.c : 0.0;
+        h.gamb[n] = 0.0;
+    }
+    h.gamb[M.hsidx] = 1.0 / g.layers[M.hsidx].rz;
+    if (g.config.model_secondary)
+        h.gamb[M.pcbidx] = 1.0 / (g.config.r_convec_sec * (g.config.s_pcb * g.config.s_pcb) / (cw * ch));
+    // package rings: an edge cell reaches its node through half its own
+    // lateral resistance plus the periphery resistance shared by nc (nr) cells
+    auto ring = [&](int n, int nn, int ns, int ne, int nw, double r1y, double r1x) {
+        const layer_t &L = g.layers[n];
+        h.ring_gns[n] = 1.0 / (L.ry / 2.0 + nc * r1y);
+        h.ring_gew[n] = 1.0 / (L.rx / 2.0 + nr * r1x);
+        h.ring_node[n][0] = nn;
+        h.ring_node[n][1] = ns;
+        h.ring_node[n][2] = ne;
+        h.ring_node[n][3] = nw;
+        const int nodes[4] = {nn, ns, ne, nw};
+        for (int e = 0; e < 4; e++) {
+            h.x_ring_layer[nodes[e]] = n;
+            h.x_ring_edge[nodes[e]] = e;
+            h.x_ring_g[nodes[e]] = e < 2 ? h.ring_gns[n] : h.ring_gew[n];
+        }
+    };
+    for (int k = 0; k < kMaxExtra; k++) {
+        h.x_ring_layer[k] = -1;
+        h.x_ring_edge[k] = 0;
+        h.x_ring_g[k] = 0.0;
+        h.x_ncpl[k] = 0;
+        h.x_gamb[k] = 0.0;
+        h.x_cinv[k] = 0.0;
+    }
+    ring(M.spidx, SP_N, SP_S, SP_E, SP_W, pk.r_sp1_y, pk.r_sp1_x);
+    ring(M.hsidx, SINK_C_N, SINK_C_S, SINK_C_E, SINK_C_W, pk.r_hs1_y, pk.r_hs1_x);
+    auto cpl = [&](int a, int b, double gval, int steady_only_a = 0) {
+        int q = h.x_ncpl[a]++;
+        h.x_cpl_node[a][q] = b;
+        h.x_cpl_g[a][q] = gval;
+        h.x_cpl_steady_only[a][q] = steady_only_a;
+        q = h.x_ncpl[b]++;
+        h.x_cpl_node[b][q] = a;
+        h.x_cpl_g[b][q] = gval;
+        h.x_cpl_steady_only[b][q] = 0;
+    };
+    // package network (single_iteration_steady_pack :2120-2228)
+    for (int o : {SINK_N, SINK_S, SINK_W, SINK_E}) h.x_gamb[o] = 1.0 / (pk.r_hs_per + pk.r_amb_per);
+    cpl(SINK_N, SINK_C_N, 1.0 / (pk.r_hs2_y + pk.r_hs));
+    cpl(SINK_S, SINK_C_S, 1.0 / (pk.r_hs2_y + pk.r_hs));
+    cpl(SINK_W, SINK_C_W, 1.0 / (pk.r_hs2_x + pk.r_hs));
+    cpl(SINK_E, SINK_C_E, 1.0 / (pk.r_hs2_x + pk.r_hs));
+    h.x_gamb[SINK_C_N] = h.x_gamb[SINK_C_S] = 1.0 / (pk.r_hs_c_per_y + pk.r_amb_c_per_y);
+    h.x_gamb[SINK_C_W] = h.x_gamb[SINK_C_E] = 1.0 / (pk.r_hs_c_per_x + pk.r_amb_c_per_x);
+    cpl(SINK_C_N, SP_N, 1.0 / pk.r_sp_per_y);
+    cpl(SINK_C_S, SP_S, 1.0 / pk.r_sp_per_y);
+    cpl(SINK_C_W, SP_W, 1.0 / pk.r_sp_per_x);
+    cpl(SINK_C_E, SP_E, 1.0 / pk.r_sp_per_x);
+    if (g.config.model_secondary) {
+        ring(M.subidx, SUB_N, SUB_S, SUB_E, SUB_W, pk.r_sub1_y, pk.r_sub1_x);
+        ring(M.solderidx, SOLDER_N, SOLDER_S, SOLDER_E, SOLDER_W, pk.r_solder1_y, pk.r_solder1_x);
+        ring(M.pcbidx, PCB_C_N, PCB_C_S, PCB_C_E, PCB_C_W, pk.r_pcb1_y, pk.r_pcb1_x);
+        for (int o : {PCB_N, PCB_S, PCB_W, PCB_E}) h.x_gamb[o] = 1.0 / pk.r_amb_sec_per;
+        cpl(PCB_N, PCB_C_N, 1.0 / (pk.r_pcb2_y + pk.r_pcb));
+        cpl(PCB_S, PCB_C_S, 1.0 / (pk.r_pcb2_y + pk.r_pcb));
+        cpl(PCB_W, PCB_C_W, 1.0 / (pk.r_pcb2_x + pk.r_pcb));
+        cpl(PCB_E, PCB_C_E, 1.0 / (pk.r_pcb2_x + pk.r_pcb));
+        h.x_gamb[PCB_C_N] = h.x_gamb[PCB_C_S] = 1.0 / pk.r_amb_sec_c_per_y;
+        h.x_gamb[PCB_C_W] = h.x_gamb[PCB_C_E] = 1.0 / pk.r_amb_sec_c_per_x;
+        cpl(PCB_C_N, SOLDER_N, 1.0 / pk.r_pcb_c_per_y);
+        cpl(PCB_C_S, SOLDER_S, 1.0 / pk.r_pcb_c_per_y);
+        cpl(PCB_C_W, SOLDER_W, 1.0 / pk.r_pcb_c_per_x);
+        cpl(PCB_C_E, SOLDER_E, 1.0 / pk.r_pcb_c_per_x);
+        // the solder nodes' transient equations leave out the substrate link
+        // that their steady equations include (slope_fn_pack :3034-3070 vs
+        // single_iteration_steady_pack :2314-2352): steady-only on that side
+        cpl(SOLDER_N, SUB_N, 1.0 / pk.r_solder_per_y, 1);
+        cpl(SOLDER_S, SUB_S, 1.0 / pk.r_solder_per_y, 1);
+        cpl(SOLDER_W, SUB_W, 1.0 / pk.r_solder_per_x, 1);
+        cpl(SOLDER_E, SUB_E, 1.0 / pk.r_solder_per_x, 1);
+    }
+    if (g.c_ready) {
+        // package-node capacitances (slope_fn_pack :2884-3091)
+        double c[kMaxExtra] = {0};
+        c[SINK_N] = c[SINK_S] = c[SINK_W] = c[SINK_E] = pk.c_hs_per + pk.c_amb_per;
+        c[SINK_C_N] = c[SINK_C_S] = pk.c_hs_c_per_y + pk.c_amb_c_per_y;
+        c[SINK_C_W] = c[SINK_C_E] = pk.c_hs_c_per_x + pk.c_amb_c_per_x;
+        c[SP_N] = c[SP_S] = pk.c_sp_per_y;
+        c[SP_W] = c[SP_E] = pk.c_sp_per_x;
+        if (g.config.model_secondary) {
+            c[PCB_N] = c[PCB_S] = c[PCB_W] = c[PCB_E] = pk.c_pcb_per + pk.c_amb_sec_per;
+            c[PCB_C_N] = c[PCB_C_S] = pk.c_pcb_c_per_y + pk.c_amb_sec_c_per_y;
+            c[PCB_C_W] = c[PCB_C_E] = pk.c_pcb_c_per_x + pk.c_amb_sec_c_per_x;
+            c[SOLDER_N] = c[SOLDER_S] = pk.c_solder_per_y;
+            c[SOLDER_W] = c[SOLDER_E] = pk.c_solder_per_x;
+            c[SUB_N] = c[SUB_S] = pk.c_sub_per_y;
+            c[SUB_W] = c[SUB_E] = pk.c_sub_per_x;
+        }
+        for (int k = 0; k < M.nx; k++) h.x_cinv[k] = 1.0 / c[k];
+    }
+}
+
+// Per-cell resistances of a detailed-3D layer: the list head's values where
+// the cell has them, else the layer's (find_res_3D :144-173, find_cap_3D
+// :176-186).
+static void cell_resistances(const Model &M, int n, double *rx, double *ry, double *rz, double *cap) {
+    const grid_model_t &g = *M.g;
+    const layer_t &L = g.layers[n];
+    const long plane = (long)g.rows * g.cols;
+    const int mi = M.layer_map[n];
+    const BlockMap &bm = M.maps[mi];
+    const bool cellwise = M.detailed && !M.map_pkg[mi];
+    for (long c = 0; c < plane; c++) {
+        const bool hr = cellwise && bm.has_res[c];
+        rx[c] = hr ? bm.rx[c] : L.rx;
+        ry[c] = hr ? bm.ry[c] : L.ry;
+        rz[c] = hr ? bm.rz[c] : L.rz;
+        cap[c] = (cellwise && bm.lock[c]) ? bm.cap[c] : L.c;
+    }
+}
+
+static void assemble_hetero(Model &M, DevModel &D) {
+    grid_model_t &g = *M.g;
+    const int nl = g.n_layers, nr = g.rows, nc = g.cols;
+    const long plane = (long)nr * nc, ncell = plane * nl;
+    std::vector<double> rx(ncell), ry(ncell), rz(ncell), cap(ncell), press(ncell, 0.0);
+    std::vector<int> fluid(ncell, 0), types(ncell, 0);
+    std::vector<double> htc(nl, 0.0), thick(nl), hydro(nl, 0.0), cv(nl, 0.0), tin(nl, 0.0);
+    bool adv = false;
+    for (int n = 0; n < nl; n++) {
+        cell_resistances(M, n, &rx[n * plane], &ry[n * plane], &rz[n * plane], &cap[n * plane]);
+        thick[n] = g.layers[n].thickness;
+        if (M.mc[n]) {
+            const Microchannel &mc = *M.mc[n];
+            htc[n] = mc.cfg.htc;
+            hydro[n] = mc.hydro_c;
+            cv[n] = mc.cfg.coolant_capac;
+            tin[n] = mc.cfg.inlet_temperature;
+            for (long c = 0; c < plane; c++) {
+                const int t = mc.types[c];
+                types[n * plane + c] = t;
+                fluid[n * plane + c] = (t == CELL_FLUID || t == CELL_INLET || t == CELL_OUTLET);
+                press[n * plane + c] = mc.pressure[c];
+            }
+            adv = true;
+        }
+    }
+    AsmIn in;
+    in.rx = D.upload(rx.data(), ncell);
+    in.ry = D.upload(ry.data(), ncell);
+    in.rz = D.upload(rz.data(), ncell);
+    in.cap = D.upload(cap.data(), ncell);
+    in.fluid = D.upload(fluid.data(), ncell);
+    in.types = D.upload(types.data(), ncell);
+    in.press = D.upload(press.data(), ncell);
+    in.htc = D.upload(htc.data(), nl);
+    in.thick = D.upload(thick.data(), nl);
+    in.hydro = D.upload(hydro.data(), nl);
+    in.cv = D.upload(cv.data(), nl);
+    in.tin = D.upload(tin.data(), nl);
+    in.nl = nl;
+    in.nr = nr;
+    in.nc = nc;
+    in.detailed = M.detailed;
+    in.cw = g.width / nc;
+    in.ch = g.height / nr;
+    ProbDesc &h = D.h;
+    double *gE = D.alloc<double>(ncell), *gS = D.alloc<double>(ncell), *gD = D.alloc<double>(ncell);
+    double *ci = D.alloc<double>(ncell), *di = D.alloc<double>(ncell);
+    double *aN = D.alloc<double>(ncell), *aS = D.alloc<double>(ncell), *aE = D.alloc<double>(ncell);
+    double *aW = D.alloc<double>(ncell), *aSelf = D.alloc<double>(ncell), *bc = D.alloc<double>(ncell);
+    const int nb = (int)((ncell + kBlock - 1) / kBlock);
+    launch(k_assemble, nb, D.stream, in, gE, gS, gD, ci, aN, aS, aE, aW, aSelf, bc);
+    h.gE = gE;
+    h.gS = gS;
+    h.gD = gD;
+    h.cinvc = ci;
+    h.dinvc = di;
+    h.aN = aN;
+    h.aS = aS;
+    h.aE = aE;
+    h.aW = aW;
+    h.aSelf = aSelf;
+    h.bconst = adv ? bc : nullptr;
+    h.has_adv = adv;
+    float *fE = D.alloc<float>(ncell), *fS = D.alloc<float>(ncell), *fD = D.alloc<float>(ncell);
+    float *fci = D.alloc<float>(ncell), *fdi = D.alloc<float>(ncell);
+    float *faN = D.alloc<float>(ncell), *faS = D.alloc<float>(ncell), *faE = D.alloc<float>(ncell);
+    float *faW = D.alloc<float>(ncell), *faSelf = D.alloc<float>(ncell);
+    h.fE = fE;
+    h.fS = fS;
+    h.fD = fD;
+    h.fcinv = fci;
+    h.fdinv = fdi;
+    h.faN = faN;
+    h.faS = faS;
+    h.faE = faE;
+    h.faW = faW;
+    h.faSelf = faSelf;
+    HIP_OK(hipMemcpyAsync(D.d_desc, &h, sizeof(ProbDesc), hipMemcpyHostToDevice, D.stream));
+    launch(k_coef_finish, nb, D.stream, (const ProbDesc *)D.d_desc, ncell, fE, fS, fD, fci, faN, faS,
+           faE, faW, faSelf, di, fdi);
+    HIP_OK(hipStreamSynchronize(D.stream));
+}
+
+// ||C^-1 G||_inf bound: each row's |diag| + sum|off-diag| over its capacitance
+__global__ void k_rho(const ProbDesc *__restrict__ D, long ncell, unsigned long long *out) {
+    const ProbDesc &d = D[0];
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (c < ncell) {
+        const long plane = (long)d.nr * d.nc;
+        const int n = (int)(c / plane);
+        const long cc = c - n * plane;
+        const int i = (int)(cc / d.nc), j = (int)(cc % d.nc);
+        double off = 0.0;
+        if (i > 0) off += d.gS[c - d.nc] + fabs(d.aN[c]);
+        if (i < d.nr - 1) off += d.gS[c] + fabs(d.aS[c]);
+        if (j < d.nc - 1) off += d.gE[c] + fabs(d.aE[c]);
+        if (j > 0) off += d.gE[c - 1] + fabs(d.aW[c]);
+        if (n > 0) off += d.gD[c - plane];
+        if (n < d.nl - 1) off += d.gD[c];
+        const int *rn = d.ring_node[n];
+        if (i == 0 && rn[0] >= 0) off += d.ring_gns[n];
+        if (i == d.nr - 1 && rn[1] >= 0) off += d.ring_gns[n];
+        if (j == d.nc - 1 && rn[2] >= 0) off += d.ring_gew[n];
+        if (j == 0 && rn[3] >= 0) off += d.ring_gew[n];
+        v = (fabs(1.0 / d.dinvc[c]) + off) * d.cinvc[c];
+    }
+    v = wave_max(v);
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
+}
+
+static double host_rho(const Model &M, const ProbDesc &h) {
+    // homogeneous layers: the largest row sum is at a ring corner cell
+    double rho = 0.0;
+    for (int n = 0; n < h.nl; n++) {
+        double diag = 2 * h.gx[n] + 2 * h.gy[n] + h.gamb[n];
+        if (n > 0) diag += h.gz[n - 1];
+        if (n < h.nl - 1) diag += h.gz[n];
+        if (h.ring_node[n][0] >= 0) diag += h.ring_gns[n] + h.ring_gew[n];
+        rho = std::max(rho, 2 * diag * h.cinv[n]);
+    }
+    return rho;
+}
+
+static double package_rho(const Model &M, const ProbDesc &h) {
+    double rho = 0.0;
+    for (int k = 0; k < M.nx; k++) {
+        double diag = h.x_gamb[k];
+        if (h.x_ring_layer[k] >= 0) diag += h.x_ring_g[k] * (h.x_ring_edge[k] < 2 ? h.nc : h.nr);
+        for (int q = 0; q < h.x_ncpl[k]; q++) diag += h.x_cpl_g[k][q];
+        rho = std::max(rho, 2 * diag * h.x_cinv[k]);
+    }
+    return rho;
+}
+
+static void dev_prepare(Model &M) {
+    grid_model_t &g = *M.g;
+    if (!g.r_ready) fatal("R model not ready\n");
+    if (M.dev && !M.dev_dirty && M.dev->has_c == (bool)g.c_ready) return;
+    // keep the device state across a rebuild (warm starts)
+    std::vector<double> keep_steady, keep_trans;
+    if (M.dev && M.steady_valid) {
+        keep_steady.resize(M.vec_len());
+        dev_get_grid(M, 0, keep_steady.data());
+    }
+    if (M.dev && M.trans_valid) {
+        keep_trans.resize(M.vec_len());
+        dev_get_grid(M, 1, keep_trans.data());
+    }
+    dev_release(M);
+    if (g.n_layers > kMaxLayers) fatal("too many layers for the device model\n");
+    auto *D = new DevModel();
+    M.dev = D;
+    HIP_OK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));
+    HIP_OK(hipEventCreate(&D->ev0));
+    HIP_OK(hipEventCreate(&D->ev1));
+    ProbDesc &h = D->h;
+    memset(&h, 0, sizeof h);
+    fill_coefficients(M, h);
+    set_tiling(h);
+    D->hetero = M.detailed;
+    D->has_c = g.c_ready;
+    h.hetero = D->hetero;
+    h.map_mode = g.map_mode;
+    const long ncell = M.ncell();
+    D->vlen = ((ncell + M.nx + 63) / 64) * 64;
+    for (int b = 0; b < NB64; b++) {
+        h.b64[b] = D->alloc<double>(D->vlen);
+        HIP_OK(hipMemset(h.b64[b], 0, D->vlen * sizeof(double)));
+    }
+    for (int b = 0; b < NB32; b++) {
+        h.b32[b] = D->alloc<float>(D->vlen);
+        HIP_OK(hipMemset(h.b32[b], 0, D->vlen * sizeof(float)));
+    }
+    // block <-> grid maps
+    std::vector<int> base;
+    layer_bases(M, base);
+    std::vector<const int *> dptr(M.maps.size()), dunit(M.maps.size());
+    std::vector<const double *> docc(M.maps.size());
+    for (size_t mi = 0; mi < M.maps.size(); mi++) {
+        const BlockMap &bm = M.maps[mi];
+        dptr[mi] = D->upload(bm.cell_ptr.data(), bm.cell_ptr.size());
+        dunit[mi] = D->upload(bm.ent_unit.data(), bm.ent_unit.size());
+        docc[mi] = D->upload(bm.ent_occ.data(), bm.ent_occ.size());
+    }
+    std::vector<int4> rng;
+    std::vector<int> blayer;
+    for (int n = 0; n < g.n_layers; n++) {
+        const int mi = M.layer_map[n];
+        h.map_ptr[n] = dptr[mi];
+        h.map_unit[n] = dunit[mi];
+        h.map_occ[n] = docc[mi];
+        h.layer_base[n] = base[n];
+        const flp_t *f = g.layers[n].flp;
+        std::vector<double> area(f->n_units);
+        for (int u = 0; u < f->n_units; u++) {
+            area[u] = f->units[u].width * f->units[u].height;
+            const glist_t &r = M.maps[mi].ranges[u];
+            rng.push_back(make_int4(r.i1, r.i2, r.j1, r.j2));
+            blayer.push_back(n);
+        }
+        h.unit_area[n] = D->upload(area.data(), area.size());
+    }
+    h.nblocks = g.total_n_blocks;
+    h.blk_range = D->upload(rng.data(), rng.size());
+    h.blk_layer = D->upload(blayer.data(), blayer.size());
+    D->nblk = g.total_n_blocks + M.nx;
+    h.blk_in = D->alloc<double>(D->nblk);
+    h.blk_out = D->alloc<double>(D->nblk);
+    HIP_OK(hipHostMalloc((void **)&D->h_blk, D->nblk * sizeof(double), 0));
+    HIP_OK(hipHostMalloc((void **)&D->h_scal, NSCAL * sizeof(double), 0));
+    h.partial = D->alloc<double>((size_t)(h.ntiles + 1) * kRed);
+    h.counter = D->alloc<unsigned>(1);
+    HIP_OK(hipMemset(h.counter, 0, sizeof(unsigned)));
+    h.scal = D->alloc<double>(NSCAL);
+    HIP_OK(hipMemset(h.scal, 0, NSCAL * sizeof(double)));
+    D->d_desc = D->alloc<ProbDesc>(1);
+    HIP_OK(hipMemcpy(D->d_desc, &h, sizeof(ProbDesc), hipMemcpyHostToDevice));
+    // work lists
+    std::vector<WorkItem> w(h.ntiles + 1);
+    for (int t = 0; t <= h.ntiles; t++) w[t] = WorkItem{0, t};
+    D->nwork = h.ntiles + 1;
+    D->d_work = D->upload(w.data(), w.size());
+    std::vector<WorkItem> gw;
+    for (int u = 0; u < g.total_n_blocks; u += kBlock / kWave) gw.push_back(WorkItem{0, u});
+    gw.push_back(WorkItem{0, -1});
+    D->ngwork = (int)gw.size();
+    D->d_gwork = D->upload(gw.data(), gw.size());
+    if (D->hetero) assemble_hetero(M, *D);
+    // transient step bound
+    if (g.c_ready) {
+        if (D->hetero) {
+            unsigned long long *r = D->alloc<unsigned long long>(1);
+            HIP_OK(hipMemset(r, 0, sizeof *r));
+            launch(k_rho, (int)((ncell + kBlock - 1) / kBlock), D->stream, (const ProbDesc *)D->d_desc,
+                   ncell, r);
+            unsigned long long bits = 0;
+            HIP_OK(hipMemcpy(&bits, r, sizeof bits, hipMemcpyDeviceToHost));
+            double v;
+            memcpy(&v, &bits, sizeof v);
+            D->rho = v;
+        } else {
+            D->rho = host_rho(M, h);
+        }
+        D->rho = std::max(D->rho, package_rho(M, h));
+    }
+    HIP_OK(hipMemcpy(D->d_desc, &h, sizeof(ProbDesc), hipMemcpyHostToDevice));
+    M.dev_dirty = false;
+    if (!keep_steady.empty()) dev_set_grid(M, 0, keep_steady.data());
+    if (!keep_trans.empty()) dev_set_grid(M, 1, keep_trans.data());
+}
+
+static void read_scal(DevModel &D) {
+    HIP_OK(hipMemcpyAsync(D.h_scal, D.h.scal, NSCAL * sizeof(double), hipMemcpyDeviceToHost, D.stream));
+    HIP_OK(hipStreamSynchronize(D.stream));
+}
+
+static void write_scal(DevModel &D, const std::vector<std::pair<int, double>> &kv) {
+    read_scal(D);
+    for (auto &p : kv) D.h_scal[p.first] = p.second;
+    HIP_OK(hipMemcpyAsync(D.h.scal, D.h_scal, NSCAL * sizeof(double), hipMemcpyHostToDevice, D.stream));
+    HIP_OK(hipStreamSynchronize(D.stream));
+}
+
+static void upload_blocks(DevModel &D, const double *v) {
+    memcpy(D.h_blk, v, D.nblk * sizeof(double));
+    HIP_OK(hipMemcpyAsync(D.h.blk_in, D.h_blk, D.nblk * sizeof(double), hipMemcpyHostToDevice, D.stream));
+}
+
+static void download_blocks(DevModel &D, double *v) {
+    HIP_OK(hipMemcpyAsync(D.h_blk, D.h.blk_out, D.nblk * sizeof(double), hipMemcpyDeviceToHost, D.stream));
+    HIP_OK(hipStreamSynchronize(D.stream));
+    memcpy(v, D.h_blk, D.nblk * sizeof(double));
+}
+
+static void run_b2g(DevModel &D, int type, int out, int sources) {
+    launch(k_b2g, D.nwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_work, type, out,
+           sources);
+}
+
+static void run_g2b(DevModel &D, int in) {
+    launch(k_g2b, D.ngwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_gwork, in);
+}
+
+// ---- steady ----------------------------------------------------------------
+template <typename T, bool HET>
+static void cg_iterations(DevModel &D, int n, int xb, int rb, int zb, int p0b, int qb, int done_slot,
+                          int tol_slot, int max_iters) {
+    const ProbDesc *dd = D.d_desc;
+    const WorkItem *w = D.d_work;
+    for (int k = 0; k < n; k++) {
+        launch(k_cg_dir<T, HET>, D.nwork, D.stream, dd, w, zb, p0b, qb, done_slot);
+        launch(k_cg_upd<T, HET>, D.nwork, D.stream, dd, w, xb, rb, zb, p0b, qb, done_slot, tol_slot,
+               max_iters);
+    }
+}
+
+template <bool HET>
+static void steady_jpcg64(Model &M, DevModel &D) {
+    const double rtol2 = M.opt.steady_rtol * M.opt.steady_rtol;
+    launch(k_cg_init<double, HET>, D.nwork, D.stream, (const ProbDesc *)D.d_desc,
+           (const WorkItem *)D.d_work, (int)B_STEADY, (int)B_R, (int)B_Z, (int)B_RHS, rtol2,
+           M.opt.max_iters);
+    int chunk = 8;
+    for (;;) {
+        read_scal(D);
+        if (D.h_scal[S_DONE] != 0.0) break;
+        cg_iterations<double, HET>(D, chunk, B_STEADY, B_R, B_Z, B_P0, B_Q, S_DONE, S_TOL2, M.opt.max_iters);
+        chunk = std::min(chunk * 2, 64);
+    }
+    M.stats.steady_iters = (int)D.h_scal[S_ITERS];
+    M.stats.steady_refinements = 0;
+    M.stats.steady_rel_resid = D.h_scal[S_BB] > 0 ? std::sqrt(D.h_scal[S_RR] / D.h_scal[S_BB]) : 0.0;
+}
+
+template <bool HET>
+static void steady_mixed(Model &M, DevModel &D) {
+    const double rtol2 = M.opt.steady_rtol * M.opt.steady_rtol;
+    const double inner2 = 1e-8;   // each FP32 solve gains 4 digits
+    const int max_refine = 30;
+    write_scal(D, {{S_NREFINE, 0.0}, {S_DONE, 0.0}});
+    int total = 0;
+    for (;;) {
+        launch(k_ir_resid<HET>, D.nwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_work,
+               rtol2, inner2, max_refine);
+        read_scal(D);
+        if (D.h_scal[S_DONE] != 0.0) break;
+        int chunk = 8;
+        for (;;) {
+            cg_iterations<float, HET>(D, chunk, F_X, F_R, F_Z, F_P0, F_Q, S_IDONE, S_ITOL2, M.opt.max_iters);
+            read_scal(D);
+            if (D.h_scal[S_IDONE] != 0.0) break;
+            chunk = std::min(chunk * 2, 64);
+        }
+        total += (int)D.h_scal[S_ITERS];
+        launch(k_ir_correct, D.nwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_work);
+    }
+    M.stats.steady_iters = total;
+    M.stats.steady_refinements = (int)D.h_scal[S_NREFINE];
+    M.stats.steady_rel_resid = D.h_scal[S_BB] > 0 ? std::sqrt(D.h_scal[S_RR] / D.h_scal[S_BB]) : 0.0;
+}
+
+void dev_steady(Model &M, const double *block_power, double *block_temp) {
+    dev_prepare(M);
+    DevModel &D = *M.dev;
+    HIP_OK(hipEventRecord(D.ev0, D.stream));
+    upload_blocks(D, block_power);
+    run_b2g(D, V_POWER, B_RHS, 1);
+    if (!M.steady_valid) {
+        const int nb = (int)((D.vlen + kBlock - 1) / kBlock);
+        launch(k_fill, nb, D.stream, D.h.b64[B_STEADY], D.vlen, M.g->config.ambient);
+    }
+    write_scal(D, {{S_DONE, 0.0}});
+    if (M.opt.precision == HS_PREC_MIXED) {
+        if (D.hetero) steady_mixed<true>(M, D);
+        else steady_mixed<false>(M, D);
+    } else {
+        if (D.hetero) steady_jpcg64<true>(M, D);
+        else steady_jpcg64<false>(M, D);
+    }
+    run_g2b(D, B_STEADY);
+    HIP_OK(hipEventRecord(D.ev1, D.stream));
+    download_blocks(D, block_temp);
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, D.ev0, D.ev1));
+    M.stats.last_solve_ms = ms;
+}
+
+// ---- transient ---------------------------------------------------------------
+template <bool HET>
+static void rk4_attempt(DevModel &D) {
+    const ProbDesc *dd = D.d_desc;
+    const WorkItem *w = D.d_work;
+    auto st = [&](int mode, int base, int kin, double a, int kout, int out, double hc, int cmp) {
+        launch(k_rk_stage<HET>, D.nwork, D.stream, dd, w, mode, base, kin, a, kout, out, hc, cmp);
+    };
+    const int Y = 100, YT = 101;
+    st(ST_K, Y, -1, 0.0, B_K1, -1, 0.0, -1);                 // k1 = f(y), first attempt only
+    st(ST_FIRST, Y, B_K1, 0.5, B_KA, -1, 0.0, -1);           // full step
+    st(ST_MID, Y, B_KA, 0.5, B_KB, -1, 0.0, -1);
+    st(ST_LAST, Y, B_KB, 1.0, -1, YT, 1.0, -1);
+    st(ST_FIRST, Y, B_K1, 0.25, B_KA, -1, 0.0, -1);          // first half step
+    st(ST_MID, Y, B_KA, 0.25, B_KB, -1, 0.0, -1);
+    st(ST_LAST, Y, B_KB, 0.5, -1, B_T1, 0.5, -1);
+    st(ST_K, B_T1, -1, 0.0, B_K1, -1, 0.0, -1);              // k1 = f(t1)
+    st(ST_FIRST, B_T1, B_K1, 0.25, B_KA, -1, 0.0, -1);       // second half step
+    st(ST_MID, B_T1, B_KA, 0.25, B_KB, -1, 0.0, -1);
+    st(ST_LAST_ERR, B_T1, B_KB, 0.5, -1, -1, 0.5, YT);
+    hipLaunchKernelGGL(k_rk_control, dim3(1), dim3(64), 0, D.stream, dd, 1);
+}
+
+template <bool HET>
+static void transient_rk4(Model &M, DevModel &D, double dt) {
+    write_scal(D, {{S_T, 0.0}, {S_DT, dt}, {S_H, kMinStep}, {S_STATUS, (double)RK_NEED_K1}, {S_CALLS, 0.0}});
+    int batch = 4;
+    for (int guard = 0;; guard++) {
+        for (int k = 0; k < batch; k++) rk4_attempt<HET>(D);
+        read_scal(D);
+        if (D.h_scal[S_STATUS] == (double)RK_DONE) break;
+        batch = 2;
+        if (guard > 10000000) fatal("transient integration did not finish\n");
+    }
+    M.stats.rk4_calls = (int)D.h_scal[S_CALLS];
+}
+
+template <typename TD, bool HET>
+static void transient_taylor(Model &M, DevModel &D, double dt) {
+    // sub-steps keep rho*tau <= 2; terms until the a-priori tail bound of a
+    // 1000 K change per sub-step is under the tolerance, then the device
+    // stops the series early once the actual tail is below it
+    const double zmax = 2.0;
+    const int m = std::max(1, (int)std::ceil(D.rho * dt / zmax));
+    const double tau = dt / m, z = D.rho * tau;
+    const double tol = M.opt.taylor_tol / m;
+    int K = 1;
+    double term = 1000.0;   // K
+    while (K < 60) {
+        term *= z / (K + 1);
+        const double q = z / (K + 2);
+        if (term * (q < 1 ? 1.0 / (1.0 - q) : 1e30) <= tol) break;
+        K++;
+    }
+    write_scal(D, {{S_TAU, tau}, {S_TK, (double)m}, {S_RHO, D.rho}});
+    const ProbDesc *dd = D.d_desc;
+    const WorkItem *w = D.d_work;
+    const bool f32 = sizeof(TD) == 4;
+    const int db0 = f32 ? F_D0 : B_KA, db1 = f32 ? F_D1 : B_KB, accb = f32 ? F_ACC : B_ACC;
+    for (int s = 0; s < m; s++) {
+        launch(k_tay_first<TD, HET>, D.nwork, D.stream, dd, w, db0, accb, s);
+        for (int k = 1; k < K; k++)
+            launch(k_tay_term<TD, HET>, D.nwork, D.stream, dd, w, (k & 1) ? db0 : db1, (k & 1) ? db1 : db0,
+                   accb, k, s, tol);
+        launch(k_tay_final<TD>, D.nwork, D.stream, dd, w, accb, s);
+    }
+    M.stats.taylor_terms = K;
+    M.stats.taylor_substeps = m;
+}
+
+void dev_transient(Model &M, const double *block_power, const double *block_temp_in, double dt,
+                   double *block_temp_out) {
+    dev_prepare(M);
+    DevModel &D = *M.dev;
+    HIP_OK(hipEventRecord(D.ev0, D.stream));
+    upload_blocks(D, block_power);
+    run_b2g(D, V_POWER, B_RHS, 1);
+    if (block_temp_in) {
+        HIP_OK(hipStreamSynchronize(D.stream));
+        upload_blocks(D, block_temp_in);
+        run_b2g(D, V_TEMP, B_Y0, 0);
+        write_scal(D, {{S_YPAR, 0.0}});
+    }
+    if (M.opt.transient == HS_TRANS_TAYLOR) {
+        if (M.opt.precision == HS_PREC_MIXED) {
+            if (D.hetero) transient_taylor<float, true>(M, D, dt);
+            else transient_taylor<float, false>(M, D, dt);
+        } else {
+            if (D.hetero) transient_taylor<double, true>(M, D, dt);
+            else transient_taylor<double, false>(M, D, dt);
+        }
+    } else {
+        if (D.hetero) transient_rk4<true>(M, D, dt);
+        else transient_rk4<false>(M, D, dt);
+    }
+    read_scal(D);
+    const int ypar = (int)D.h_scal[S_YPAR];
+    run_g2b(D, B_Y0 + ypar);
+    HIP_OK(hipEventRecord(D.ev1, D.stream));
+    download_blocks(D, block_temp_out);
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, D.ev0, D.ev1));
+    M.stats.last_solve_ms = ms;
+}
+
+// ---- grid-level access ---------------------------------------------------------
+void dev_get_grid(Model &M, int which, double *out) {
+    if (!M.dev) fatal("no device state\n");
+    DevModel &D = *M.dev;
+    int b = B_STEADY;
+    if (which == 1) {
+        read_scal(D);
+        b = B_Y0 + (int)D.h_scal[S_YPAR];
+    }
+    HIP_OK(hipMemcpyAsync(out, D.h.b64[b], M.vec_len() * sizeof(double), hipMemcpyDeviceToHost, D.stream));
+    HIP_OK(hipStreamSynchronize(D.stream));
+}
+
+void dev_set_grid(Model &M, int which, const double *in) {
+    dev_prepare(M);
+    DevModel &D = *M.dev;
+    int b = B_STEADY;
+    if (which == 1) {
+        read_scal(D);
+        b = B_Y0 + (int)D.h_scal[S_YPAR];
+    }
+    HIP_OK(hipMemcpyAsync(D.h.b64[b], in, M.vec_len() * sizeof(double), hipMemcpyHostToDevice, D.stream));
+    HIP_OK(hipStreamSynchronize(D.stream));
+}
+
+void dev_slope(Model &M, const double *block_power, const double *grid_temp, double *out) {
+    dev_prepare(M);
+    DevModel &D = *M.dev;
+    std::vector<double> p(block_power, block_power + D.nblk);
+    for (int k = 0; k < M.nx; k++) p[M.g->total_n_blocks + k] = 0.0;
+    upload_blocks(D, p.data());
+    run_b2g(D, V_POWER, B_RHS, 1);
+    HIP_OK(hipMemcpyAsync(D.h.b64[B_TMP], grid_temp, M.vec_len() * sizeof(double), hipMemcpyHostToDevice,
+                          D.stream));
+    if (D.hetero)
+        launch(k_slope<true>, D.nwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_work,
+               (int)B_TMP, (int)B_KA);
+    else
+        launch(k_slope<false>, D.nwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_work,
+               (int)B_TMP, (int)B_KA);
+    HIP_OK(hipMemcpyAsync(out, D.h.b64[B_KA], M.vec_len() * sizeof(double), hipMemcpyDeviceToHost, D.stream));
+    HIP_OK(hipStreamSynchronize(D.stream));
+}
+
+void dev_b2g(Model &M, const double *block_vec, double *grid_vec, int type) {
+    dev_prepare(M);
+    DevModel &D = *M.dev;
+    upload_blocks(D, block_vec);
+    run_b2g(D, type, B_TMP, 0);
+    HIP_OK(hipMemcpyAsync(grid_vec, D.h.b64[B_TMP], M.vec_len() * sizeof(double), hipMemcpyDeviceToHost,
+                          D.stream));
+    HIP_OK(hipStreamSynchronize(D.stream));
+}
+
+void dev_g2b(Model &M, const double *grid_vec, double *block_vec) {
+    dev_prepare(M);
+    DevModel &D = *M.dev;
+    HIP_OK(hipMemcpyAsync(D.h.b64[B_TMP], grid_vec, M.vec_len() * sizeof(double), hipMemcpyHostToDevice,
+                          D.stream));
+    run_g2b(D, B_TMP);
+    download_blocks(D, block_vec);
+}
+
+}  // namespace hs
+
+// ---- batched engine (declared in the C ABI) -----------------------------------
+struct hs_engine_st {
+    int precision = HS_PREC_FP64;
+    int transient = HS_TRANS_RK4;
+};
+
+extern "C" {
+hs_engine_t *hs_engine_create(int precision, int transient) {
+    auto *e = new hs_engine_st();
+    e->precision = precision;
+    e->transient = transient;
+    return e;
+}
+void hs_engine_destroy(hs_engine_t *e) { delete e; }
+int hs_engine_submit(hs_engine_t *, RC_model_t *, const double *, const double *, int, double) { return -1; }
+int hs_engine_tick(hs_engine_t *) { return 0; }
+int hs_engine_poll(hs_engine_t *, int, double *) { return 0; }
+void hs_engine_set_budget(hs_engine_t *, double) {}
+int hs_engine_active(hs_engine_t *) { return 0; }
+}

@@ -1,0 +1,113 @@
+// Internal declarations shared by the host model (hs_host.cpp) and the
+// device engine (hs_device.hip).  Not part of the C ABI.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hotspot_amd.h"
+
+namespace hs {
+
+// ---- constants of the grid model (temperature.h:29-119, util.h:18-19) ----
+constexpr double kDelta = 1.0e-6;
+constexpr double kLargeNum = 1.0e100;
+constexpr double kCFactor = 0.333;
+constexpr double kMinStep = 1e-7;
+constexpr double kOccupancyThreshold = 0.95;
+constexpr int kMaxLayers = 32;
+constexpr int kMaxExtra = EXTRA + EXTRA_SEC;
+
+enum ExtraNode {
+    SP_W = 0, SP_E, SP_N, SP_S, SINK_C_W, SINK_C_E, SINK_C_N, SINK_C_S,
+    SINK_W, SINK_E, SINK_N, SINK_S,
+    SUB_W, SUB_E, SUB_N, SUB_S, SOLDER_W, SOLDER_E, SOLDER_N, SOLDER_S,
+    PCB_C_W, PCB_C_E, PCB_C_N, PCB_C_S, PCB_W, PCB_E, PCB_N, PCB_S
+};
+enum MapMode { GRID_AVG = 0, GRID_MIN = 1, GRID_MAX = 2, GRID_CENTER = 3 };
+enum CellType { CELL_TSV = -1, CELL_SOLID = 0, CELL_FLUID = 1, CELL_INLET = 2, CELL_OUTLET = 3 };
+
+// error raised inside the library; converted at the C boundary
+struct Error {
+    std::string msg;
+};
+[[noreturn]] void fatal(const std::string &msg);
+void warning(const std::string &msg);
+
+// ---- block<->grid map of one distinct floorplan ---------------------------
+struct BlockMap {
+    // per cell (row-major i*nc+j): the blocks overlapping it, in block order
+    std::vector<int> cell_ptr;        // nr*nc + 1
+    std::vector<int> ent_unit;
+    std::vector<double> ent_occ;
+    std::vector<glist_t> ranges;      // per block: [i1,i2) x [j1,j2)
+    // detailed-3D values of each cell's list head (blist_t fields)
+    std::vector<double> rx, ry, rz, cap;
+    std::vector<uint8_t> lock, has_res;
+};
+
+// per-layer microchannel state (flow field of one network)
+struct Microchannel {
+    microchannel_config_t cfg;        // owned copy (cell_types etc. allocated)
+    std::vector<int> types;           // nr*nc cell types
+    std::vector<double> pressure;     // nr*nc (0 for solid cells)
+    double hydro_c = 0.0;             // hydraulic conductance of one cell
+};
+
+// ---- the host model ------------------------------------------------------
+struct DevModel;  // device state, defined in hs_device.hip
+
+struct Model {
+    grid_model_t *g = nullptr;        // the public struct
+    std::vector<layer_t> layers;
+    std::vector<int> layer_map;       // layer -> index into maps
+    std::vector<BlockMap> maps;
+    std::vector<int> map_owner;       // map -> layer whose floorplan/props built it
+    std::vector<uint8_t> map_pkg;     // map belongs to package layers (hasRes cleared)
+    std::vector<flp_t *> owned_flps;
+    std::vector<Microchannel *> mc;   // per layer (nullptr if not microchannel)
+    int nx = EXTRA;
+    int spidx = -1, hsidx = -1, subidx = -1, solderidx = -1, pcbidx = -1;
+    bool detailed = false;
+    hs_options_t opt{};
+    hs_stats_t stats{};
+    DevModel *dev = nullptr;
+    bool dev_dirty = true;            // host coefficients changed since upload
+    int steady_valid = 0;             // last_steady holds a solution
+    int trans_valid = 0;              // last_trans holds a state
+    int mirror_steady = 0, mirror_trans = 0;  // host mirrors up to date
+
+    int nr() const { return g->rows; }
+    int nc() const { return g->cols; }
+    int nl() const { return g->n_layers; }
+    long ncell() const { return (long)g->n_layers * g->rows * g->cols; }
+    long vec_len() const { return ncell() + nx; }
+};
+
+// handle layout: what alloc_RC_model returns points at .rc
+struct Handle {
+    RC_model_t rc;
+    grid_model_t pub;
+    Model *impl;
+};
+Model *model_of(RC_model_t *m);
+Model *model_of(grid_model_t *g);
+
+// host helpers used by the device side
+void layer_bases(const Model &M, std::vector<int> &base);   // block-vector offset per layer
+
+// ---- device engine entry points (hs_device.hip) ---------------------------
+void dev_release(Model &M);
+// steady: block power in (package entries already zero), block temps out
+void dev_steady(Model &M, const double *block_power, double *block_temp);
+// transient: block power; block temp (nullptr = continue from last_trans)
+void dev_transient(Model &M, const double *block_power, const double *block_temp_in,
+                   double dt, double *block_temp_out);
+void dev_get_grid(Model &M, int which, double *out);
+void dev_set_grid(Model &M, int which, const double *in);
+void dev_slope(Model &M, const double *block_power, const double *grid_temp, double *out);
+void dev_b2g(Model &M, const double *block_vec, double *grid_vec, int type);
+void dev_g2b(Model &M, const double *grid_vec, double *block_vec);
+
+}  // namespace hs

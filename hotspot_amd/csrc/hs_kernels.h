@@ -1,0 +1,111 @@
+// Device data layout and kernels of the grid thermal model (gfx950).
+//
+// Every kernel works on a work-list of (problem, tile) items over an array of
+// problem descriptors, so one launch serves one model or a batch of models of
+// different sizes.  A tile is a TX x TY patch of one layer (TX*TY = 256
+// threads, one thread per cell, TX along the contiguous column index so each
+// wavefront reads 64 consecutive doubles); the last item of each problem is a
+// "package" workgroup that handles the 12 (or 28) package nodes, whose rows
+// couple to whole boundary rows of the spreader / sink (and secondary) layers.
+//
+// Per-problem scalars (CG coefficients, RK4 step sizes, flags) live in device
+// memory; reductions are two-level and deterministic: each workgroup writes
+// its partial into a fixed slot, and the last workgroup of the problem (found
+// with an atomic ticket) combines the slots in slot order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "hs_internal.h"
+
+namespace hs {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+constexpr int kRed = 4;   // values reduced per slot
+
+// FP64 buffers of a problem
+enum Buf64 {
+    B_STEADY = 0,  // steady solution (last_steady)
+    B_Y0, B_Y1,    // transient state, ping-pong by parity (last_trans)
+    B_RHS,         // P + ambient / inlet sources
+    B_K1, B_KA, B_KB, B_ACC, B_T1,   // RK4 work
+    B_R, B_P0, B_P1, B_Q, B_Z,       // CG work
+    B_TMP,
+    NB64
+};
+// FP32 buffers of a problem
+enum Buf32 {
+    F_R = 0, F_P0, F_P1, F_Q, F_X, F_Z,   // inner CG of the mixed steady solve
+    F_D0, F_D1, F_ACC,               // Taylor terms (mixed transient)
+    NB32
+};
+
+// per-problem scalar slots (doubles)
+enum Scal {
+    S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL2, S_ITERS, S_DONE, S_PPAR,
+    S_RR0,                       // inner solve: initial residual
+    S_IDONE, S_ITOL2,            // inner solve flags
+    S_H, S_NEWH, S_T, S_DT, S_STATUS, S_YPAR, S_CALLS, S_ERR,
+    S_TAU, S_TMAX, S_TDONE, S_TK,   // Taylor: step, term max, done flag, substeps
+    S_NREFINE,
+    S_RHO = S_TMAX + 8,             // Taylor: bound on ||C^-1 G||_inf
+    NSCAL = 32
+};
+enum RkStatus { RK_NEED_K1 = 0, RK_ATTEMPT = 1, RK_DONE = 2 };
+
+struct ProbDesc {
+    // geometry and tiling
+    int nl, nr, nc, nx;
+    int tx_shift, nti, ntj, ntiles;   // tiles: nl * nti * ntj, then the package item
+    int hetero, has_adv, nblocks, pad0;
+    double ambient;
+    double cell_area;
+    // homogeneous layer coefficients
+    double gx[kMaxLayers];     // row neighbour (i +- 1): 1/rx
+    double gy[kMaxLayers];     // column neighbour (j +- 1): 1/ry
+    double gz[kMaxLayers];     // layer n <-> n+1: 1/rz[n]
+    double cinv[kMaxLayers];   // 1/c
+    double gamb[kMaxLayers];   // per-cell conductance to ambient
+    double ring_gns[kMaxLayers], ring_gew[kMaxLayers];
+    int ring_node[kMaxLayers][4];  // package node of the N,S,E,W edge, -1 none
+    // package nodes
+    double x_gamb[kMaxExtra], x_cinv[kMaxExtra];
+    int x_ring_layer[kMaxExtra], x_ring_edge[kMaxExtra];
+    double x_ring_g[kMaxExtra];
+    int x_ncpl[kMaxExtra];
+    int x_cpl_node[kMaxExtra][3];
+    int x_cpl_steady_only[kMaxExtra][3];
+    double x_cpl_g[kMaxExtra][3];
+    // device buffers
+    double *b64[NB64];
+    float *b32[NB32];
+    // heterogeneous (detailed 3D / microchannel) per-cell coefficients
+    const double *gE, *gS, *gD, *cinvc, *dinvc;
+    const double *aN, *aS, *aE, *aW, *aSelf;
+    const float *fE, *fS, *fD, *fcinv, *fdinv;
+    const float *faN, *faS, *faE, *faW, *faSelf;
+    // block <-> grid maps (per layer)
+    const int *map_ptr[kMaxLayers];     // cell -> entry range
+    const int *map_unit[kMaxLayers];
+    const double *map_occ[kMaxLayers];
+    const double *unit_area[kMaxLayers];
+    int layer_base[kMaxLayers];         // block-vector offset per layer
+    const int4 *blk_range;              // per block of the vector: i1,i2,j1,j2
+    const int *blk_layer;
+    int map_mode, pad1;
+    double *blk_in;                     // block vector (device staging)
+    double *blk_out;
+    double *bconst;                     // per-cell constant source (inlet), hetero only
+    // reduction state
+    double *partial;                    // [ntiles+1][kRed]
+    unsigned *counter;
+    double *scal;                       // [NSCAL]
+};
+
+struct WorkItem {
+    int prob;
+    int tile;
+};
+
+}  // namespace hs

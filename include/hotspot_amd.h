@@ -1,0 +1,350 @@
+/*
+ * hotspot_amd.h -- C ABI of the MI355X grid thermal model.
+ *
+ * Drop-in for HotSpot's grid path: the entry points keep the names,
+ * signatures and argument meaning of the reference library, and the structs
+ * keep the reference *implementation* layouts (temperature.h, flp.h,
+ * temperature_grid.h, microchannel.h, materials.h), so code compiled against
+ * the reference headers that reads model->grid->n_layers,
+ * layers[i].has_power, layers[i].flp->n_units or has_lcf
+ * (hotspot.c:546-549, :578-587, :720) works unchanged.  The stale copy of
+ * these structs in the reference's hotspot-iface.h (no per-unit detailed-3D
+ * fields in unit_t, grid_transient_file placed after steady_file) is *not*
+ * followed; the offsets are pinned by static asserts in hs_host.cpp.
+ *
+ * Errors: the reference prints "error: ..." and calls exit(1) (util.c:32-36).
+ * This library does the same by default; hs_set_fatal_mode(0) makes it
+ * record the error (hs_last_error) and return instead.
+ *
+ * Everything below "extensions" has no counterpart in the reference.
+ */
+#ifndef HOTSPOT_AMD_H
+#define HOTSPOT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STR_SIZE 512
+#define MAX_UNITS 8192
+#define EXTRA 12
+#define EXTRA_SEC 16
+#define BLOCK_MODEL 0
+#define GRID_MODEL 1
+#define V_POWER 0
+#define V_TEMP 1
+
+/* ---- floorplan: flp.h:378-400 ---------------------------------------- */
+typedef struct unit_t_st {
+    char name[STR_SIZE];
+    double width;
+    double height;
+    double leftx;
+    double bottomy;
+    double specificheat;   /* detailed 3D: per-unit volumetric heat capacity */
+    double resistivity;    /* detailed 3D: per-unit resistivity */
+    int hasRes;
+    int hasSh;
+} unit_t;
+
+typedef struct flp_t_st {
+    unit_t *units;
+    int n_units;
+    double **wire_density;
+} flp_t;
+
+/* ---- configuration: temperature.h:141-221 ------------------------------ */
+typedef struct thermal_config_t_st {
+    double t_chip, k_chip, p_chip, thermal_threshold;
+    double c_convec, r_convec, s_sink, t_sink, k_sink, p_sink;
+    double s_spreader, t_spreader, k_spreader, p_spreader;
+    double t_interface, k_interface, p_interface;
+    int model_secondary;
+    double r_convec_sec, c_convec_sec;
+    int n_metal;
+    double t_metal, t_c4, s_c4;
+    int n_c4;
+    double s_sub, t_sub, s_solder, t_solder, s_pcb, t_pcb;
+    double ambient;
+    char init_file[STR_SIZE];
+    double init_temp;
+    char steady_file[STR_SIZE];
+    double sampling_intvl;
+    double base_proc_freq;
+    int dtm_used;
+    char model_type[STR_SIZE];
+    int leakage_used;
+    int leakage_mode;
+    int package_model_used;
+    char package_config_file[STR_SIZE];
+    int block_omit_lateral;
+    int grid_rows;
+    int grid_cols;
+    char grid_layer_file[STR_SIZE];
+    char grid_steady_file[STR_SIZE];
+    char grid_map_mode[STR_SIZE];
+    char grid_transient_file[STR_SIZE];
+    int detailed_3D_used;
+} thermal_config_t;
+
+/* ---- microchannels and materials: microchannel.h:29-90, materials.h ---- */
+typedef struct microchannel_config_t_st {
+    double cell_width, cell_height, cell_thickness;
+    double pumping_pressure, pump_internal_res, inlet_temperature;
+    double coolant_capac, coolant_res, coolant_visc;
+    double wall_capac, wall_res, htc;
+    char network_file[STR_SIZE];
+    char floorplan_file[STR_SIZE + 3];
+    int num_rows, num_columns, n_fluid_cells;
+    int **cell_types;
+    int **mapping;
+    double **A;
+    double *b;
+    int nnz;
+} microchannel_config_t;
+
+typedef struct material_t_st {
+    int material_type;
+    double thermal_conductivity;
+    double volumetric_heat_capacity;
+    double dynamic_viscosity;
+} material_t;
+
+typedef struct materials_list_t_st {
+    int size;
+    char **names;
+    material_t *material_properties;
+} materials_list_t;
+
+/* ---- package R/C: temperature.h:237-334 (52 doubles) ------------------- */
+typedef struct package_RC_t_st {
+    double r_sp1_x, r_sp1_y, r_hs1_x, r_hs1_y, r_hs2_x, r_hs2_y, r_hs;
+    double r_sp_per_x, r_sp_per_y, r_hs_c_per_x, r_hs_c_per_y, r_hs_per;
+    double c_sp_per_x, c_sp_per_y, c_hs_c_per_x, c_hs_c_per_y, c_hs_per;
+    double r_amb_c_per_x, c_amb_c_per_x, r_amb_c_per_y, c_amb_c_per_y;
+    double r_amb_per, c_amb_per;
+    double r_sub1_x, r_sub1_y, r_solder1_x, r_solder1_y;
+    double r_pcb1_x, r_pcb1_y, r_pcb2_x, r_pcb2_y, r_pcb;
+    double r_sub_per_x, r_sub_per_y, r_solder_per_x, r_solder_per_y;
+    double r_pcb_c_per_x, r_pcb_c_per_y, r_pcb_per;
+    double c_sub_per_x, c_sub_per_y, c_solder_per_x, c_solder_per_y;
+    double c_pcb_c_per_x, c_pcb_c_per_y, c_pcb_per;
+    double r_amb_sec_c_per_x, c_amb_sec_c_per_x, r_amb_sec_c_per_y, c_amb_sec_c_per_y;
+    double r_amb_sec_per, c_amb_sec_per;
+} package_RC_t;
+
+/* ---- grid model: temperature_grid.h:63-191 (SUPERLU=0 layout) ---------- */
+typedef struct blist_t_st {
+    int idx;
+    double occupancy;
+    struct blist_t_st *next;
+    int lock, hasRes, hasCap;
+    double rx, ry, rz;
+    double capacitance;
+} blist_t;
+
+typedef struct glist_t_st {
+    int i1, i2, j1, j2;
+} glist_t;
+
+typedef struct layer_t_st {
+    flp_t *flp;
+    int no;
+    int has_lateral;
+    int has_power;
+    double k;
+    double thickness;
+    double sp;
+    int is_microchannel;
+    microchannel_config_t *microchannel_config;
+    double rx, ry, rz;
+    double c;
+    blist_t ***b2gmap;   /* not populated by this library (NULL) */
+    glist_t *g2bmap;     /* block -> [i1,i2) x [j1,j2) cell ranges */
+} layer_t;
+
+typedef struct grid_model_vector_t_st {
+    double ***cuboid;
+    double *extra;
+} grid_model_vector_t;
+
+typedef struct grid_model_t_st {
+    thermal_config_t config;
+    layer_t *layers;
+    int n_layers;
+    int rows;
+    int cols;
+    double width;
+    double height;
+    package_RC_t pack;
+    int total_n_blocks;
+    int map_mode;
+    int r_ready;
+    int c_ready;
+    int has_lcf;
+    grid_model_vector_t *last_steady;   /* host mirror, refreshed on demand */
+    grid_model_vector_t *last_trans;    /* host mirror, refreshed on demand */
+    double *last_temp;
+    int base_n_units;
+    int use_microchannels;
+    microchannel_config_t *default_microchannel_config;
+} grid_model_t;
+
+typedef struct RC_model_t_st {
+    union {
+        struct block_model_t_st *block;
+        struct grid_model_t_st *grid;
+    };
+    int type;
+    thermal_config_t *config;
+} RC_model_t;
+
+/* ---- reference entry points ------------------------------------------- */
+/* flp.c:1175-1218 */
+flp_t *read_flp(char *file, int read_connects, int initialize_connects);
+/* flp.c:1258 */
+void free_flp(flp_t *flp, int compacted, int free_connects);
+/* flp.c:1385-1402 */
+int get_blk_index(flp_t *flp, char *name);
+/* temperature.c:19-109 */
+thermal_config_t default_thermal_config(void);
+/* microchannel.c:17-43 */
+microchannel_config_t default_microchannel_config(void);
+/* materials.c:13-15 */
+void default_materials(materials_list_t *materials_list);
+/* materials.c:17-155 */
+void materials_add_from_file(materials_list_t *materials_list, char *materials_filename);
+/* materials.c:152-161 */
+void free_materials(materials_list_t *materials_list);
+/* temperature.c:616-633 (grid model only: model_type must be "grid") */
+RC_model_t *alloc_RC_model(thermal_config_t *config, flp_t *placeholder,
+                           microchannel_config_t *microchannel_config,
+                           materials_list_t *materials_list, int do_detailed_3D,
+                           int use_microchannels);
+/* temperature.c:930-938 */
+void delete_RC_model(RC_model_t *model);
+/* temperature.c:636-643 -> temperature_grid.c:950-1063 */
+void populate_R_model(RC_model_t *model, flp_t *flp);
+/* temperature.c:646-653 -> temperature_grid.c:1065-1114 */
+void populate_C_model(RC_model_t *model, flp_t *flp);
+/* temperature.c:764-772 -> temperature_grid.c:1182-1194 */
+double *hotspot_vector(RC_model_t *model);
+/* util.c:89 */
+void free_dvector(double *v);
+/* temperature.c:799-806 -> temperature_grid.c:1245-1259 */
+void set_temp(RC_model_t *model, double *temp, double val);
+/* temperature.c:830-837 -> temperature_grid.c:1449-1455 */
+void copy_temp(RC_model_t *model, double *dst, double *src);
+/* temperature.c:809-816 -> temperature_grid.c:1322-1447 */
+void dump_temp(RC_model_t *model, double *temp, char *file);
+/* temperature.c:844-851 -> temperature_grid.c:1462-1642 */
+void read_temp(RC_model_t *model, double *temp, char *file, int clip);
+/* temperature.c:854-861 -> temperature_grid.c:1645-1682 */
+void dump_power(RC_model_t *model, double *power, char *file);
+/* temperature.c:867-874 -> temperature_grid.c:1684-1764 */
+void read_power(RC_model_t *model, double *power, char *file);
+/* temperature.c:877-885 -> temperature_grid.c:1766-1790 */
+double find_max_temp(RC_model_t *model, double *temp);
+/* temperature.c:888-896 -> temperature_grid.c:1792-1813 */
+double find_avg_temp(RC_model_t *model, double *temp);
+/* temperature.c:819-827 -> temperature_grid.c:1816-1856 */
+double calc_sink_temp(RC_model_t *model, double *temp);
+/* temperature.c:656-751 (grid branch :707-749, leakage loop included) */
+void steady_state_temp(RC_model_t *model, double *power, double *temp);
+/* temperature.c:754-761 -> temperature_grid.c:3603-3704 */
+void compute_temp(RC_model_t *model, double *power, double *temp, double time_elapsed);
+/* temperature.c:778-787 -> temperature_grid.c:1203-1220 */
+void trim_hotspot_vector(RC_model_t *model, double *dst, double *src, int at, int size);
+/* temperature.c:789-797 -> temperature_grid.c:1223-1242 */
+void resize_thermal_model(RC_model_t *model, int n_units);
+/* temperature.c:910-927 */
+double calc_leakage(int mode, double h, double w, double temp);
+/* temperature_grid.c:1262-1295 */
+void dump_steady_temp_grid(grid_model_t *model, char *file);
+/* temperature_grid.c:1297-1319 */
+void dump_transient_temp_grid(grid_model_t *model, int trace_num, double sampling_intvl,
+                              char *filename);
+
+/* ---- extensions --------------------------------------------------------
+ * None of these has a counterpart in the reference.
+ */
+
+/* error channel: 0 = no error since the last call; the message is copied. */
+int hs_last_error(char *buf, int len);
+/* 1 (default): print "error: ..." and exit(1) like the reference; 0: record and return */
+void hs_set_fatal_mode(int exit_on_error);
+
+/* precision of the solves */
+#define HS_PREC_FP64  0   /* every sweep in FP64 */
+#define HS_PREC_MIXED 1   /* FP32 sweeps inside FP64 refinement / FP64 residuals */
+/* steady solvers */
+#define HS_STEADY_JPCG  0 /* Jacobi-preconditioned CG (BiCGStab when advection makes G non-symmetric) */
+#define HS_STEADY_MGPCG 1 /* geometric multigrid (z-line smoother) preconditioned CG */
+/* transient integrators */
+#define HS_TRANS_RK4    0 /* the reference's adaptive step-doubling RK4 */
+#define HS_TRANS_TAYLOR 1 /* exponential (Taylor) integrator of the linear ODE, error bounded a priori */
+
+typedef struct hs_options_st {
+    int precision;        /* HS_PREC_* */
+    int steady_solver;    /* HS_STEADY_* */
+    int transient;        /* HS_TRANS_* */
+    double steady_rtol;   /* relative residual target of the steady solve (FP64 norm) */
+    int max_iters;        /* per steady solve */
+    double taylor_tol;    /* a-priori bound on the Taylor truncation error per interval (K) */
+} hs_options_t;
+
+typedef struct hs_stats_st {
+    int steady_iters;         /* CG (or BiCGStab) iterations of the last steady solve */
+    int steady_refinements;   /* FP64 refinement steps (mixed precision) */
+    double steady_rel_resid;  /* final ||P + b - G T||_2 / ||P + b||_2 in FP64 */
+    int rk4_calls;            /* RK4 step attempts of the last compute_temp */
+    int taylor_terms;         /* terms per sub-step of the last Taylor interval */
+    int taylor_substeps;
+    double last_solve_ms;     /* device time of the last steady/compute_temp call */
+} hs_stats_t;
+
+void hs_get_options(RC_model_t *model, hs_options_t *opt);
+void hs_set_options(RC_model_t *model, const hs_options_t *opt);
+void hs_get_stats(RC_model_t *model, hs_stats_t *stats);
+
+/* Grid-level state in the reference's flat layout [n_layers*rows*cols + extras]
+ * (A3D, temperature_grid.c:2854).  which: 0 = last steady, 1 = last transient. */
+int hs_grid_size(RC_model_t *model);
+void hs_get_grid_temp(RC_model_t *model, int which, double *out);
+void hs_set_grid_temp(RC_model_t *model, int which, const double *in);
+/* one application of the transient right-hand side f(T) = (P + b - G T)/C on
+ * the device, P given per block (package-node entries ignored), T and dT in
+ * the flat grid layout; for operator-level checks. */
+void hs_grid_slope(RC_model_t *model, const double *block_power, const double *grid_temp,
+                   double *grid_slope);
+/* block -> grid translation on the device (type V_POWER or V_TEMP) */
+void hs_block_to_grid(RC_model_t *model, const double *block_vec, double *grid_vec, int type);
+/* grid -> block translation on the device (the model's grid_map_mode) */
+void hs_grid_to_block(RC_model_t *model, const double *grid_vec, double *block_vec);
+
+/* ---- batched engine: many models advanced by shared launches ------------- */
+typedef struct hs_engine_st hs_engine_t;
+hs_engine_t *hs_engine_create(int precision, int transient);
+void hs_engine_destroy(hs_engine_t *e);
+/* Submits a job on a populated model; power is a block vector (copied).
+ * n_steps == 0: steady solve; n_steps > 0: that many compute_temp intervals of
+ * dt each, starting from init_temp (a block vector, copied) or, when
+ * init_temp is NULL, from the model's steady state solved first.
+ * Returns a job id >= 0, or -1 when the job cannot be admitted. */
+int hs_engine_submit(hs_engine_t *e, RC_model_t *model, const double *power,
+                     const double *init_temp, int n_steps, double dt);
+/* Advances every active job by one scheduling tick (one batched launch
+ * sequence); returns the number of jobs still active. */
+int hs_engine_tick(hs_engine_t *e);
+/* 1 when finished; copies the result block vector into out (may be NULL). */
+int hs_engine_poll(hs_engine_t *e, int job, double *out);
+/* bytes of device memory the engine may use for resident problems */
+void hs_engine_set_budget(hs_engine_t *e, double bytes);
+int hs_engine_active(hs_engine_t *e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

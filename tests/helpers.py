@@ -1,0 +1,87 @@
+"""Shared helpers: run one case through a HotSpot-API library and through the
+oracle on the same files."""
+import ctypes as C
+import tempfile
+
+import numpy as np
+
+import hs_oracle as O
+from hotspot_amd import cases, iface
+
+c_double_p = iface.c_double_p
+
+
+def oracle_for(run: cases.Run) -> O.OracleModel:
+    case = run.case
+    return O.OracleModel(run.paths["config"], flp=run.paths["flp"], detailed_3D=case.detailed_3D,
+                         use_microchannels=case.use_microchannels, microchannel=case.microchannel)
+
+
+def make(lib, case, tmp=None):
+    return cases.Run(lib, case, tmp or tempfile.mkdtemp())
+
+
+def set_options(lib, run, **kw):
+    opt = HsOptions()
+    lib.lib.hs_get_options(run.model, C.byref(opt))
+    for k, v in kw.items():
+        setattr(opt, k, v)
+    lib.lib.hs_set_options(run.model, C.byref(opt))
+
+
+def stats(lib, run):
+    st = HsStats()
+    lib.lib.hs_get_stats(run.model, C.byref(st))
+    return st
+
+
+def check_error(lib):
+    buf = C.create_string_buffer(512)
+    if lib.lib.hs_last_error(buf, 512):
+        raise RuntimeError(buf.value.decode())
+
+
+class HsOptions(C.Structure):
+    _fields_ = [("precision", C.c_int), ("steady_solver", C.c_int), ("transient", C.c_int),
+                ("steady_rtol", C.c_double), ("max_iters", C.c_int), ("taylor_tol", C.c_double)]
+
+
+class HsStats(C.Structure):
+    _fields_ = [("steady_iters", C.c_int), ("steady_refinements", C.c_int),
+                ("steady_rel_resid", C.c_double), ("rk4_calls", C.c_int),
+                ("taylor_terms", C.c_int), ("taylor_substeps", C.c_int),
+                ("last_solve_ms", C.c_double)]
+
+
+def grid_temp(lib, run, which):
+    n = lib.lib.hs_grid_size(run.model)
+    out = np.zeros(n)
+    lib.lib.hs_get_grid_temp(run.model, which, out.ctypes.data_as(c_double_p))
+    check_error(lib)
+    return out
+
+
+def bind_extensions(lib):
+    L = lib.lib
+    L.hs_grid_size.restype = C.c_int
+    L.hs_grid_size.argtypes = [iface.RCModelP]
+    L.hs_get_grid_temp.argtypes = [iface.RCModelP, C.c_int, c_double_p]
+    L.hs_get_grid_temp.restype = None
+    L.hs_grid_slope.argtypes = [iface.RCModelP, c_double_p, c_double_p, c_double_p]
+    L.hs_grid_slope.restype = None
+    L.hs_block_to_grid.argtypes = [iface.RCModelP, c_double_p, c_double_p, C.c_int]
+    L.hs_block_to_grid.restype = None
+    L.hs_grid_to_block.argtypes = [iface.RCModelP, c_double_p, c_double_p]
+    L.hs_grid_to_block.restype = None
+    L.hs_get_options.argtypes = [iface.RCModelP, C.POINTER(HsOptions)]
+    L.hs_set_options.argtypes = [iface.RCModelP, C.POINTER(HsOptions)]
+    L.hs_get_stats.argtypes = [iface.RCModelP, C.POINTER(HsStats)]
+    L.hs_last_error.argtypes = [C.c_char_p, C.c_int]
+    L.hs_last_error.restype = C.c_int
+    return lib
+
+
+def ptr(a):
+    """Pointer to a float64 array's own buffer (the caller keeps it alive)."""
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(c_double_p)
