@@ -69,37 +69,40 @@ __device__ __forceinline__ void block_reduce(double (&v)[N]) {
     __syncthreads();
 }
 
-// Stores this workgroup's partials in its slot; returns true (in every thread)
-// for the last workgroup of the problem, which then sees all slots.
+// Each workgroup stores its partials (valid in thread 0) in its own slot with
+// plain stores; a finish kernel launched next combines the slots of each
+// problem in slot order.  No device-scope fences in the hot kernels (on a
+// multi-XCD part such a fence writes back the XCD's L2) and no contended
+// atomics: the kernel boundary makes the slots visible.
 template <int N>
-__device__ __forceinline__ bool publish_partials(const ProbDesc &d, int slot, const double (&v)[N]) {
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
+__device__ __forceinline__ void write_partials(const ProbDesc &d, int slot, const double (&v)[N]) {
+    if (threadIdx.x == 0)
 #pragma unroll
         for (int q = 0; q < N; q++) d.partial[slot * kRed + q] = v[q];
-        __threadfence();
-        unsigned t = atomicAdd(d.counter, 1u);
-        last = (t == (unsigned)d.ntiles);   // ntiles + 1 slots in all
-    }
-    __syncthreads();
-    if (last) __threadfence();
-    return last;
 }
 
-// In the last workgroup: combine all slots in slot order (thread 0 result).
+// Whole workgroup: combine a problem's slots in slot order (thread 0 result).
 template <int N, bool MAX = false>
-__device__ __forceinline__ void gather_partials(const ProbDesc &d, double (&v)[N]) {
+__device__ __forceinline__ void reduce_slots(const ProbDesc &d, double (&v)[N]) {
     const int nslots = d.ntiles + 1;
 #pragma unroll
-    for (int q = 0; q < N; q++) v[q] = MAX ? 0.0 : 0.0;
+    for (int q = 0; q < N; q++) v[q] = 0.0;
     for (int s = threadIdx.x; s < nslots; s += kBlock)
 #pragma unroll
         for (int q = 0; q < N; q++) {
-            double x = __builtin_nontemporal_load(&d.partial[s * kRed + q]);
+            const double x = d.partial[s * kRed + q];
             v[q] = MAX ? fmax(v[q], x) : v[q] + x;
         }
     block_reduce<N, MAX>(v);
-    if (threadIdx.x == 0) atomicExch(d.counter, 0u);
+}
+
+// Workgroup -> work item, XCD-aware: the hardware hands workgroup b to XCD
+// b % 8, so give each XCD a contiguous run of work items (neighbouring tiles,
+// whose halos then stay in that XCD's L2 from one kernel to the next).
+__device__ __forceinline__ int work_index() {
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
 // ---------------------------------------------------------------------------
@@ -298,13 +301,15 @@ __device__ __forceinline__ bool problem_skip(const ProbDesc &d, int flag_slot) {
 // ---------------------------------------------------------------------------
 // out = sum over the cell's blocks of occ * v_u (/ area_u * cell area for
 // power); with add_sources the ambient / inlet constants are added (rhs).
+// from_seq: read the block vector of the current sequence step instead of
+// the staging vector.
 __global__ void __launch_bounds__(kBlock) k_b2g(const ProbDesc *__restrict__ D,
                                                 const WorkItem *__restrict__ W, int type,
-                                                int out_buf, int add_sources) {
-    const WorkItem wi = W[blockIdx.x];
+                                                int out_buf, int add_sources, int from_seq) {
+    const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     double *out = d.b64[out_buf];
-    const double *b = d.blk_in;
+    const double *b = from_seq ? d.seq_in + (long)d.scal[S_STEP] * (d.nblocks + d.nx) : d.blk_in;
     const long ncell = (long)d.nl * d.nr * d.nc;
     if (wi.tile < d.ntiles) {
         const Cell q = tile_cell(d, wi.tile);
@@ -339,16 +344,20 @@ __global__ void __launch_bounds__(kBlock) k_b2g(const ProbDesc *__restrict__ D,
 }
 
 // One wavefront per block of the vector; work items: (problem, first block).
+// in_buf < 0: the current transient state; to_seq: write the current step of
+// the output sequence instead of the staging vector.
 __global__ void __launch_bounds__(kBlock) k_g2b(const ProbDesc *__restrict__ D,
-                                                const WorkItem *__restrict__ W, int in_buf) {
-    const WorkItem wi = W[blockIdx.x];
+                                                const WorkItem *__restrict__ W, int in_buf,
+                                                int to_seq) {
+    const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
-    const double *g = d.b64[in_buf];
+    const double *g = d.b64[in_buf >= 0 ? in_buf : B_Y0 + (int)d.scal[S_YPAR]];
+    double *bo = to_seq ? d.seq_out + (long)d.scal[S_STEP] * (d.nblocks + d.nx) : d.blk_out;
     const long plane = (long)d.nr * d.nc, ncell = plane * d.nl;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     const int u = wi.tile + w;
     if (wi.tile == -1) {   // package nodes
-        if (threadIdx.x < d.nx) d.blk_out[d.nblocks + threadIdx.x] = g[ncell + threadIdx.x];
+        if (threadIdx.x < d.nx) bo[d.nblocks + threadIdx.x] = g[ncell + threadIdx.x];
         return;
     }
     if (u >= d.nblocks) return;
@@ -378,7 +387,7 @@ __global__ void __launch_bounds__(kBlock) k_g2b(const ProbDesc *__restrict__ D,
         }
         res = d.map_mode == GRID_AVG ? s / cnt : d.map_mode == GRID_MIN ? mn : mx;
     }
-    if (lane == 0) d.blk_out[u] = res;
+    if (lane == 0) bo[u] = res;
 }
 
 // ---------------------------------------------------------------------------
@@ -392,7 +401,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const ProbDesc *__restrict__
                                                     const WorkItem *__restrict__ W, int xb,
                                                     int rb, int zb, int rhsb, double rtol2,
                                                     int max_iters) {
-    const WorkItem wi = W[blockIdx.x];
+    const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     if (problem_skip(d, S_DONE)) return;
     T *x = sizeof(T) == 8 ? (T *)d.b64[xb] : (T *)d.b32[xb];
@@ -432,20 +441,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const ProbDesc *__restrict__
         }
     }
     block_reduce<3>(v);
-    if (publish_partials<3>(d, wi.tile, v)) {
-        gather_partials<3>(d, v);
-        if (threadIdx.x == 0) {
-            double *s = d.scal;
-            s[S_RZ] = v[0];
-            s[S_RR] = v[1];
-            s[S_BB] = v[2];
-            s[S_TOL2] = rtol2 * v[2];
-            s[S_BETA] = 0.0;
-            s[S_ITERS] = 0.0;
-            s[S_PPAR] = 0.0;
-            s[S_DONE] = (v[1] <= s[S_TOL2] || max_iters <= 0) ? 1.0 : 0.0;
-        }
-    }
+    write_partials<3>(d, wi.tile, v);
 }
 
 // p = z + beta p_old;  q = G p;  partial p.q -> alpha = rz / pq
@@ -453,7 +449,7 @@ template <typename T, bool HET>
 __global__ void __launch_bounds__(kBlock) k_cg_dir(const ProbDesc *__restrict__ D,
                                                    const WorkItem *__restrict__ W, int zb,
                                                    int p0b, int qb, int done_slot) {
-    const WorkItem wi = W[blockIdx.x];
+    const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     if (problem_skip(d, done_slot) || (done_slot != S_DONE && problem_skip(d, S_DONE))) return;
     const int par = (int)d.scal[S_PPAR];
@@ -487,13 +483,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_dir(const ProbDesc *__restrict__ 
         }
     }
     block_reduce<1>(v);
-    if (publish_partials<1>(d, wi.tile, v)) {
-        gather_partials<1>(d, v);
-        if (threadIdx.x == 0) {
-            d.scal[S_PQ] = v[0];
-            d.scal[S_ALPHA] = v[0] != 0.0 ? d.scal[S_RZ] / v[0] : 0.0;
-        }
-    }
+    write_partials<1>(d, wi.tile, v);
 }
 
 // x += alpha p; r -= alpha q; z = D^-1 r; partials r.z, r.r -> beta, done
@@ -502,7 +492,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_upd(const ProbDesc *__restrict__ 
                                                    const WorkItem *__restrict__ W, int xb,
                                                    int rb, int zb, int p0b, int qb, int done_slot,
                                                    int tol_slot, int max_iters) {
-    const WorkItem wi = W[blockIdx.x];
+    const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     if (problem_skip(d, done_slot) || (done_slot != S_DONE && problem_skip(d, S_DONE))) return;
     const int par = (int)d.scal[S_PPAR];
@@ -535,18 +525,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_upd(const ProbDesc *__restrict__ 
         }
     }
     block_reduce<2>(v);
-    if (publish_partials<2>(d, wi.tile, v)) {
-        gather_partials<2>(d, v);
-        if (threadIdx.x == 0) {
-            double *s = d.scal;
-            s[S_BETA] = s[S_RZ] != 0.0 ? v[0] / s[S_RZ] : 0.0;
-            s[S_RZ] = v[0];
-            s[S_RR] = v[1];
-            s[S_PPAR] = 1.0 - s[S_PPAR];
-            s[S_ITERS] += 1.0;
-            if (v[1] <= s[tol_slot] || s[S_ITERS] >= max_iters || !(v[1] == v[1])) s[done_slot] = 1.0;
-        }
-    }
+    write_partials<2>(d, wi.tile, v);
 }
 
 // Mixed precision: FP64 residual r = rhs - G x, its FP32 copy as the inner
@@ -555,7 +534,7 @@ template <bool HET>
 __global__ void __launch_bounds__(kBlock) k_ir_resid(const ProbDesc *__restrict__ D,
                                                      const WorkItem *__restrict__ W, double rtol2,
                                                      double inner_rtol2, int max_refine) {
-    const WorkItem wi = W[blockIdx.x];
+    const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     if (problem_skip(d, S_DONE)) return;
     const double *x = d.b64[B_STEADY], *rhs = d.b64[B_RHS];
@@ -590,31 +569,13 @@ __global__ void __launch_bounds__(kBlock) k_ir_resid(const ProbDesc *__restrict_
         }
     }
     block_reduce<3>(v);
-    if (publish_partials<3>(d, wi.tile, v)) {
-        gather_partials<3>(d, v);
-        if (threadIdx.x == 0) {
-            double *s = d.scal;
-            s[S_RR] = v[1];
-            s[S_BB] = v[2];
-            s[S_TOL2] = rtol2 * v[2];
-            s[S_RZ] = v[0];
-            s[S_RR0] = v[1];
-            s[S_ITOL2] = inner_rtol2 * v[1];
-            s[S_BETA] = 0.0;
-            s[S_ITERS] = 0.0;
-            s[S_PPAR] = 0.0;
-            const bool done = v[1] <= s[S_TOL2] || s[S_NREFINE] >= max_refine || !(v[1] == v[1]);
-            s[S_DONE] = done ? 1.0 : 0.0;
-            s[S_IDONE] = done ? 1.0 : 0.0;
-            s[S_NREFINE] += 1.0;
-        }
-    }
+    write_partials<3>(d, wi.tile, v);
 }
 
 // x (FP64) += e (FP32)
 __global__ void __launch_bounds__(kBlock) k_ir_correct(const ProbDesc *__restrict__ D,
                                                        const WorkItem *__restrict__ W) {
-    const WorkItem wi = W[blockIdx.x];
+    const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     if (problem_skip(d, S_DONE)) return;
     double *x = d.b64[B_STEADY];
@@ -649,7 +610,7 @@ __global__ void __launch_bounds__(kBlock) k_rk_stage(const ProbDesc *__restrict_
                                                      int base_b, int kin_b, double a_mul,
                                                      int kout_b, int out_b, double hc_mul,
                                                      int cmp_b) {
-    const WorkItem wi = W[blockIdx.x];
+    const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     const double st = d.scal[S_STATUS];
     if (st == (double)RK_DONE) return;
@@ -699,20 +660,21 @@ __global__ void __launch_bounds__(kBlock) k_rk_stage(const ProbDesc *__restrict_
     }
     if (mode != ST_LAST_ERR) return;
     block_reduce<1, true>(v);
-    if (publish_partials<1>(d, wi.tile, v)) {
-        gather_partials<1, true>(d, v);
-        if (threadIdx.x == 0) d.scal[S_ERR] = v[0];
-    }
+    write_partials<1>(d, wi.tile, v);
 }
 
 // Step-size control of one attempt (rk4, RCutil.c:598-621) and the interval
-// loop of compute_temp_grid (temperature_grid.c:3676-3692).
-__global__ void k_rk_control(const ProbDesc *__restrict__ D, int nprob) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nprob) return;
-    double *s = D[p].scal;
+// loop of compute_temp_grid (temperature_grid.c:3676-3692); one workgroup
+// per problem, which first combines the step-doubling error of the attempt.
+__global__ void __launch_bounds__(kBlock) k_rk_control(const ProbDesc *__restrict__ D) {
+    const ProbDesc &d = D[blockIdx.x];
+    double *s = d.scal;
     if (s[S_STATUS] == (double)RK_DONE) return;
-    const double h = s[S_H], mx = s[S_ERR];
+    double v[1];
+    reduce_slots<1, true>(d, v);
+    if (threadIdx.x != 0) return;
+    const double h = s[S_H], mx = v[0];
+    s[S_ERR] = mx;
     double nh;
     if (mx <= 0.01) {
         nh = 0.95 * h * pow(fabs(0.01 / mx), 0.2);
@@ -740,32 +702,152 @@ __global__ void k_rk_control(const ProbDesc *__restrict__ D, int nprob) {
     }
 }
 
+// Finish kernels of the CG / refinement / Taylor reductions: one workgroup
+// per problem, same skip condition as the producer kernel.
+enum FinOp { FIN_CG_INIT = 0, FIN_ALPHA, FIN_BETA, FIN_IR, FIN_TAY };
+struct FinArgs {
+    double rtol2, inner_rtol2, tol;
+    int max_iters, max_refine, done_slot, tol_slot, sub;
+};
+
+__global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ D, int op, FinArgs a) {
+    const ProbDesc &d = D[blockIdx.x];
+    double *s = d.scal;
+    double v[3];
+    switch (op) {
+        case FIN_CG_INIT: {
+            if (s[S_DONE] != 0.0) return;
+            reduce_slots<3>(d, v);
+            if (threadIdx.x) return;
+            s[S_RZ] = v[0];
+            s[S_RR] = v[1];
+            s[S_BB] = v[2];
+            s[S_TOL2] = a.rtol2 * v[2];
+            s[S_BETA] = 0.0;
+            s[S_ITERS] = 0.0;
+            s[S_PPAR] = 0.0;
+            s[S_DONE] = (v[1] <= s[S_TOL2] || a.max_iters <= 0) ? 1.0 : 0.0;
+            return;
+        }
+        case FIN_ALPHA: {
+            if (s[a.done_slot] != 0.0 || s[S_DONE] != 0.0) return;
+            double w[1];
+            reduce_slots<1>(d, w);
+            if (threadIdx.x) return;
+            s[S_PQ] = w[0];
+            s[S_ALPHA] = w[0] != 0.0 ? s[S_RZ] / w[0] : 0.0;
+            return;
+        }
+        case FIN_BETA: {
+            if (s[a.done_slot] != 0.0 || s[S_DONE] != 0.0) return;
+            double w[2];
+            reduce_slots<2>(d, w);
+            if (threadIdx.x) return;
+            s[S_BETA] = s[S_RZ] != 0.0 ? w[0] / s[S_RZ] : 0.0;
+            s[S_RZ] = w[0];
+            s[S_RR] = w[1];
+            s[S_PPAR] = 1.0 - s[S_PPAR];
+            s[S_ITERS] += 1.0;
+            if (w[1] <= s[a.tol_slot] || s[S_ITERS] >= a.max_iters || !(w[1] == w[1])) s[a.done_slot] = 1.0;
+            return;
+        }
+        case FIN_IR: {
+            if (s[S_DONE] != 0.0) return;
+            reduce_slots<3>(d, v);
+            if (threadIdx.x) return;
+            s[S_RR] = v[1];
+            s[S_BB] = v[2];
+            s[S_TOL2] = a.rtol2 * v[2];
+            s[S_RZ] = v[0];
+            s[S_RR0] = v[1];
+            s[S_ITOL2] = a.inner_rtol2 * v[1];
+            s[S_BETA] = 0.0;
+            s[S_ITERS] = 0.0;
+            s[S_PPAR] = 0.0;
+            const bool done = v[1] <= s[S_TOL2] || s[S_NREFINE] >= a.max_refine || !(v[1] == v[1]);
+            s[S_DONE] = done ? 1.0 : 0.0;
+            s[S_IDONE] = done ? 1.0 : 0.0;
+            s[S_NREFINE] += 1.0;
+            return;
+        }
+        case FIN_TAY: {
+            // number of terms: the smallest K whose geometric tail bound from
+            // max|E_1| (||E_{k+1}|| <= ||E_k|| rho tau/(k+1)) is under tol
+            if (a.sub >= (int)s[S_TK]) return;
+            double w[1];
+            reduce_slots<1, true>(d, w);
+            if (threadIdx.x) return;
+            const double z = s[S_RHO] * s[S_TAU];
+            double term = w[0];
+            int K = 1;
+            while (K < 60) {
+                term *= z / (K + 1);
+                const double q = z / (K + 2);
+                if (q < 1.0 && term / (1.0 - q) <= a.tol) break;
+                K++;
+            }
+            s[S_TMAX] = w[0];
+            s[S_KNEED] = K;
+            return;
+        }
+    }
+}
+
+// interval set-up / check / step counter of device-resident sequences
+__global__ void k_rk_begin(const ProbDesc *__restrict__ D, int nprob, double dt) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprob) return;
+    double *s = D[p].scal;
+    s[S_T] = 0.0;
+    s[S_DT] = dt;
+    s[S_H] = kMinStep;
+    s[S_STATUS] = RK_NEED_K1;
+    s[S_CALLS] = 0.0;
+}
+
+__global__ void k_rk_end(const ProbDesc *__restrict__ D, int nprob) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprob) return;
+    double *s = D[p].scal;
+    if (s[S_STATUS] != (double)RK_DONE) s[S_OVERFLOW] = 1.0;
+    s[S_TCALLS] += s[S_CALLS];
+}
+
+__global__ void k_step_advance(const ProbDesc *__restrict__ D, int nprob) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprob) return;
+    D[p].scal[S_STEP] += 1.0;
+}
+
 // ---------------------------------------------------------------------------
-// transient: exponential (Taylor) integrator of the linear ODE
-//   y(t + tau) = y + sum_k tau^k/k! D_k,  D_1 = f(y),  D_{k+1} = -C^-1 G D_k
-// D_1 in FP64 (it is the residual); the higher terms in TD (FP32 in the
-// mixed form); each term kernel also reduces max|term| so the last workgroup
-// can stop the series once the tail bound is below the tolerance.
+// transient: exponential (Taylor) integrator of the linear ODE over a
+// sub-step tau (P constant within an interval):
+//   y(t + tau) = y + sum_{k>=1} E_k,  E_1 = tau f(y),
+//   E_{k+1} = tau/(k+1) (-C^-1 G) E_k
+// E_1 comes from the FP64 residual; the increments E_k are kept in TD (FP32
+// in the mixed form: they are temperature changes, not temperatures).  Each
+// term kernel reduces max|E_{k+1}| so the last workgroup can stop the series
+// once the geometric tail bound (||C^-1 G||_inf <= rho) is under tolerance.
 // ---------------------------------------------------------------------------
 template <typename TD, bool HET>
 __global__ void __launch_bounds__(kBlock) k_tay_first(const ProbDesc *__restrict__ D,
                                                       const WorkItem *__restrict__ W, int d0b,
                                                       int accb, int sub) {
-    const WorkItem wi = W[blockIdx.x];
+    const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     if (sub >= (int)d.scal[S_TK]) return;
     const int ypar = (int)d.scal[S_YPAR];
     const double *y = d.b64[B_Y0 + ypar], *rhs = d.b64[B_RHS];
     auto buf = [&](int b) -> TD * { return sizeof(TD) == 8 ? (TD *)d.b64[b] : (TD *)d.b32[b]; };
-    TD *dk = buf(d0b), *acc = buf(accb);
+    TD *ek = buf(d0b), *acc = buf(accb);
     const double tau = d.scal[S_TAU];
     LdPlain<double> ld{y};
     double v[1] = {0};
     auto fin = [&](long c, double f) {
-        dk[c] = (TD)f;
-        const double t = tau * f;
-        acc[c] = (TD)t;
-        v[0] = fmax(v[0], fabs(t));
+        const TD e = (TD)(tau * f);
+        ek[c] = e;
+        acc[c] = e;
+        v[0] = fmax(v[0], fabs((double)e));
     };
     if (wi.tile < d.ntiles) {
         const Cell q = tile_cell(d, wi.tile);
@@ -785,45 +867,32 @@ __global__ void __launch_bounds__(kBlock) k_tay_first(const ProbDesc *__restrict
         }
     }
     block_reduce<1, true>(v);
-    if (publish_partials<1>(d, wi.tile, v)) {
-        gather_partials<1, true>(d, v);
-        if (threadIdx.x == 0) {
-            d.scal[S_TMAX] = v[0];
-            d.scal[S_TDONE] = 0.0;
-        }
-    }
+    write_partials<1>(d, wi.tile, v);
 }
 
-// term k+1 from term k (k >= 1): D' = C^-1 (flux(D) - gam D) = -C^-1 G D
+// E_{k+1} from E_k (k >= 1); skipped once the series has its K terms
 template <typename TD, bool HET>
 __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict__ D,
                                                      const WorkItem *__restrict__ W, int inb,
-                                                     int outb, int accb, int k, int sub,
-                                                     double tol) {
-    const WorkItem wi = W[blockIdx.x];
+                                                     int outb, int accb, int k, int sub) {
+    const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
-    if (sub >= (int)d.scal[S_TK] || problem_skip(d, S_TDONE)) return;
-    const double rho = d.scal[S_TMAX + 8];
+    if (sub >= (int)d.scal[S_TK] || k >= (int)d.scal[S_KNEED]) return;
     auto buf = [&](int b) -> TD * { return sizeof(TD) == 8 ? (TD *)d.b64[b] : (TD *)d.b32[b]; };
-    const TD *din = buf(inb);
-    TD *dout = buf(outb), *acc = buf(accb);
+    const TD *ein = buf(inb);
+    TD *eout = buf(outb), *acc = buf(accb);
     const double tau = d.scal[S_TAU];
-    // coefficient tau^(k+1)/(k+1)!
-    double coef = tau;
-    for (int m = 2; m <= k + 1; m++) coef *= tau / m;
-    const TD cf = (TD)coef;
-    LdPlain<TD> ld{din};
-    double v[1] = {0};
+    const TD s = (TD)(tau / (k + 1));
+    LdPlain<TD> ld{ein};
     auto fin = [&](long c, TD f) {
-        dout[c] = f;
-        const TD t = cf * f;
-        acc[c] += t;
-        v[0] = fmax(v[0], fabs((double)t));
+        const TD e = s * f;
+        eout[c] = e;
+        acc[c] += e;
     };
     if (wi.tile < d.ntiles) {
         const Cell q = tile_cell(d, wi.tile);
         if (q.active) {
-            const TD xc = din[q.c];
+            const TD xc = ein[q.c];
             TD flux, diag, gam;
             cell_terms<TD, HET>(d, ld, q, xc, flux, diag, gam);
             fin(q.c, cell_cinv<TD, HET>(d, q) * (flux - gam * xc));
@@ -834,18 +903,7 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict_
         const int kk = threadIdx.x;
         if (kk < d.nx) {
             const long c = (long)d.nl * d.nr * d.nc + kk;
-            fin(c, (TD)d.x_cinv[kk] * (xf[kk] - (TD)d.x_gamb[kk] * din[c]));
-        }
-    }
-    block_reduce<1, true>(v);
-    if (publish_partials<1>(d, wi.tile, v)) {
-        gather_partials<1, true>(d, v);
-        if (threadIdx.x == 0) {
-            // the next terms shrink at least by rho*tau/(m+1) each (||C^-1 G|| <= rho)
-            const double q = rho * tau / (k + 2);
-            const double tail = q < 1.0 ? v[0] * q / (1.0 - q) : 1e300;
-            d.scal[S_TMAX] = v[0];
-            if (tail <= tol) d.scal[S_TDONE] = 1.0;
+            fin(c, (TD)d.x_cinv[kk] * (xf[kk] - (TD)d.x_gamb[kk] * ein[c]));
         }
     }
 }
@@ -855,7 +913,7 @@ template <typename TD>
 __global__ void __launch_bounds__(kBlock) k_tay_final(const ProbDesc *__restrict__ D,
                                                       const WorkItem *__restrict__ W, int accb,
                                                       int sub) {
-    const WorkItem wi = W[blockIdx.x];
+    const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     if (sub >= (int)d.scal[S_TK]) return;
     const int ypar = (int)d.scal[S_YPAR];
@@ -874,7 +932,7 @@ __global__ void __launch_bounds__(kBlock) k_tay_final(const ProbDesc *__restrict
 template <bool HET>
 __global__ void __launch_bounds__(kBlock) k_slope(const ProbDesc *__restrict__ D,
                                                   const WorkItem *__restrict__ W, int xb, int outb) {
-    const WorkItem wi = W[blockIdx.x];
+    const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     const double *x = d.b64[xb], *rhs = d.b64[B_RHS];
     double *out = d.b64[outb];

@@ -49,11 +49,35 @@ void dev_release(Model &M) {
     M.dev = nullptr;
 }
 
+// Optional per-launch timing of one kernel family (events around each
+// launch, capped): the live measurement behind the roofline figures.
+struct KernelTimer {
+    std::string name;
+    int cap = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;   // pending (plain launches)
+    double graph_ms = 0.0;                               // sampled from graph replays
+    int graph_n = 0;
+    int taken() const { return (int)ev.size() + graph_n; }
+};
+static KernelTimer g_timer;
+
 template <typename K, typename... A>
-static inline void launch(K kern, int nblocks, hipStream_t s, A... args) {
+static inline void launch_named(const char *name, K kern, int nblocks, hipStream_t s, A... args) {
     if (nblocks <= 0) return;
+    const bool timed = g_timer.cap > 0 && g_timer.taken() < g_timer.cap && g_timer.name == name;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (timed) {
+        HIP_OK(hipEventCreate(&a));
+        HIP_OK(hipEventCreate(&b));
+        HIP_OK(hipEventRecord(a, s));
+    }
     hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kBlock), 0, s, args...);
+    if (timed) {
+        HIP_OK(hipEventRecord(b, s));
+        g_timer.ev.emplace_back(a, b);
+    }
 }
+#define launch(kern, ...) launch_named(#kern, kern, __VA_ARGS__)
 
 static int ilog2(int v) {
     int s = 0;
@@ -487,34 +511,48 @@ static void download_blocks(DevModel &D, double *v) {
     memcpy(v, D.h_blk, D.nblk * sizeof(double));
 }
 
-static void run_b2g(DevModel &D, int type, int out, int sources) {
+static void run_b2g(DevModel &D, int type, int out, int sources, int from_seq = 0) {
     launch(k_b2g, D.nwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_work, type, out,
-           sources);
+           sources, from_seq);
 }
 
-static void run_g2b(DevModel &D, int in) {
-    launch(k_g2b, D.ngwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_gwork, in);
+static void run_g2b(DevModel &D, int in, int to_seq = 0) {
+    launch(k_g2b, D.ngwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_gwork, in, to_seq);
 }
 
 // ---- steady ----------------------------------------------------------------
+static void finish(DevModel &D, int nprob, int op, const FinArgs &a) {
+    launch(k_finish, nprob, D.stream, (const ProbDesc *)D.d_desc, op, a);
+}
+
 template <typename T, bool HET>
 static void cg_iterations(DevModel &D, int n, int xb, int rb, int zb, int p0b, int qb, int done_slot,
                           int tol_slot, int max_iters) {
     const ProbDesc *dd = D.d_desc;
     const WorkItem *w = D.d_work;
+    FinArgs a{};
+    a.done_slot = done_slot;
+    a.tol_slot = tol_slot;
+    a.max_iters = max_iters;
     for (int k = 0; k < n; k++) {
-        launch(k_cg_dir<T, HET>, D.nwork, D.stream, dd, w, zb, p0b, qb, done_slot);
-        launch(k_cg_upd<T, HET>, D.nwork, D.stream, dd, w, xb, rb, zb, p0b, qb, done_slot, tol_slot,
+        launch_named("k_cg_dir", k_cg_dir<T, HET>, D.nwork, D.stream, dd, w, zb, p0b, qb, done_slot);
+        finish(D, 1, FIN_ALPHA, a);
+        launch_named("k_cg_upd", k_cg_upd<T, HET>, D.nwork, D.stream, dd, w, xb, rb, zb, p0b, qb, done_slot, tol_slot,
                max_iters);
+        finish(D, 1, FIN_BETA, a);
     }
 }
 
 template <bool HET>
 static void steady_jpcg64(Model &M, DevModel &D) {
     const double rtol2 = M.opt.steady_rtol * M.opt.steady_rtol;
-    launch(k_cg_init<double, HET>, D.nwork, D.stream, (const ProbDesc *)D.d_desc,
+    launch_named("k_cg_init", k_cg_init<double, HET>, D.nwork, D.stream, (const ProbDesc *)D.d_desc,
            (const WorkItem *)D.d_work, (int)B_STEADY, (int)B_R, (int)B_Z, (int)B_RHS, rtol2,
            M.opt.max_iters);
+    FinArgs a{};
+    a.rtol2 = rtol2;
+    a.max_iters = M.opt.max_iters;
+    finish(D, 1, FIN_CG_INIT, a);
     int chunk = 8;
     for (;;) {
         read_scal(D);
@@ -535,8 +573,13 @@ static void steady_mixed(Model &M, DevModel &D) {
     write_scal(D, {{S_NREFINE, 0.0}, {S_DONE, 0.0}});
     int total = 0;
     for (;;) {
-        launch(k_ir_resid<HET>, D.nwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_work,
+        launch_named("k_ir_resid", k_ir_resid<HET>, D.nwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_work,
                rtol2, inner2, max_refine);
+        FinArgs a{};
+        a.rtol2 = rtol2;
+        a.inner_rtol2 = inner2;
+        a.max_refine = max_refine;
+        finish(D, 1, FIN_IR, a);
         read_scal(D);
         if (D.h_scal[S_DONE] != 0.0) break;
         int chunk = 8;
@@ -560,7 +603,7 @@ void dev_steady(Model &M, const double *block_power, double *block_temp) {
     HIP_OK(hipEventRecord(D.ev0, D.stream));
     upload_blocks(D, block_power);
     run_b2g(D, V_POWER, B_RHS, 1);
-    if (!M.steady_valid) {
+    if (!M.steady_valid || !M.opt.steady_warm_start) {
         const int nb = (int)((D.vlen + kBlock - 1) / kBlock);
         launch(k_fill, nb, D.stream, D.h.b64[B_STEADY], D.vlen, M.g->config.ambient);
     }
@@ -586,7 +629,7 @@ static void rk4_attempt(DevModel &D) {
     const ProbDesc *dd = D.d_desc;
     const WorkItem *w = D.d_work;
     auto st = [&](int mode, int base, int kin, double a, int kout, int out, double hc, int cmp) {
-        launch(k_rk_stage<HET>, D.nwork, D.stream, dd, w, mode, base, kin, a, kout, out, hc, cmp);
+        launch_named("k_rk_stage", k_rk_stage<HET>, D.nwork, D.stream, dd, w, mode, base, kin, a, kout, out, hc, cmp);
     };
     const int Y = 100, YT = 101;
     st(ST_K, Y, -1, 0.0, B_K1, -1, 0.0, -1);                 // k1 = f(y), first attempt only
@@ -600,7 +643,7 @@ static void rk4_attempt(DevModel &D) {
     st(ST_FIRST, B_T1, B_K1, 0.25, B_KA, -1, 0.0, -1);       // second half step
     st(ST_MID, B_T1, B_KA, 0.25, B_KB, -1, 0.0, -1);
     st(ST_LAST_ERR, B_T1, B_KB, 0.5, -1, -1, 0.5, YT);
-    hipLaunchKernelGGL(k_rk_control, dim3(1), dim3(64), 0, D.stream, dd, 1);
+    launch(k_rk_control, 1, D.stream, dd);
 }
 
 template <bool HET>
@@ -617,37 +660,57 @@ static void transient_rk4(Model &M, DevModel &D, double dt) {
     M.stats.rk4_calls = (int)D.h_scal[S_CALLS];
 }
 
-template <typename TD, bool HET>
-static void transient_taylor(Model &M, DevModel &D, double dt) {
-    // sub-steps keep rho*tau <= 2; terms until the a-priori tail bound of a
-    // 1000 K change per sub-step is under the tolerance, then the device
-    // stops the series early once the actual tail is below it
-    const double zmax = 2.0;
-    const int m = std::max(1, (int)std::ceil(D.rho * dt / zmax));
-    const double tau = dt / m, z = D.rho * tau;
-    const double tol = M.opt.taylor_tol / m;
-    int K = 1;
-    double term = 1000.0;   // K
-    while (K < 60) {
-        term *= z / (K + 1);
-        const double q = z / (K + 2);
-        if (term * (q < 1 ? 1.0 / (1.0 - q) : 1e30) <= tol) break;
-        K++;
+// Taylor set-up: sub-steps keep rho*tau <= 4; the term count is the a-priori
+// bound for a 1000 K change per sub-step, and the device stops each series
+// early once its actual tail bound is below the tolerance.
+struct TaylorPlan {
+    int m, K;
+    double tau, tol;
+};
+static TaylorPlan taylor_plan(const Model &M, const DevModel &D, double dt) {
+    TaylorPlan p;
+    const double zmax = 4.0;
+    p.m = std::max(1, (int)std::ceil(D.rho * dt / zmax));
+    p.tau = dt / p.m;
+    const double z = D.rho * p.tau;
+    p.tol = M.opt.taylor_tol / p.m;
+    p.K = 1;
+    double term = 1000.0;
+    while (p.K < 60) {
+        term *= z / (p.K + 1);
+        const double q = z / (p.K + 2);
+        if (term * (q < 1 ? 1.0 / (1.0 - q) : 1e30) <= p.tol) break;
+        p.K++;
     }
-    write_scal(D, {{S_TAU, tau}, {S_TK, (double)m}, {S_RHO, D.rho}});
+    return p;
+}
+
+template <typename TD, bool HET>
+static void taylor_interval(DevModel &D, const TaylorPlan &p) {
     const ProbDesc *dd = D.d_desc;
     const WorkItem *w = D.d_work;
     const bool f32 = sizeof(TD) == 4;
     const int db0 = f32 ? F_D0 : B_KA, db1 = f32 ? F_D1 : B_KB, accb = f32 ? F_ACC : B_ACC;
-    for (int s = 0; s < m; s++) {
-        launch(k_tay_first<TD, HET>, D.nwork, D.stream, dd, w, db0, accb, s);
-        for (int k = 1; k < K; k++)
-            launch(k_tay_term<TD, HET>, D.nwork, D.stream, dd, w, (k & 1) ? db0 : db1, (k & 1) ? db1 : db0,
-                   accb, k, s, tol);
-        launch(k_tay_final<TD>, D.nwork, D.stream, dd, w, accb, s);
+    for (int s = 0; s < p.m; s++) {
+        launch_named("k_tay_first", k_tay_first<TD, HET>, D.nwork, D.stream, dd, w, db0, accb, s);
+        FinArgs a{};
+        a.tol = p.tol;
+        a.sub = s;
+        finish(D, 1, FIN_TAY, a);
+        for (int k = 1; k < p.K; k++)
+            launch_named("k_tay_term", k_tay_term<TD, HET>, D.nwork, D.stream, dd, w, (k & 1) ? db0 : db1,
+                         (k & 1) ? db1 : db0, accb, k, s);
+        launch_named("k_tay_final", k_tay_final<TD>, D.nwork, D.stream, dd, w, accb, s);
     }
-    M.stats.taylor_terms = K;
-    M.stats.taylor_substeps = m;
+}
+
+template <typename TD, bool HET>
+static void transient_taylor(Model &M, DevModel &D, double dt) {
+    const TaylorPlan p = taylor_plan(M, D, dt);
+    write_scal(D, {{S_TAU, p.tau}, {S_TK, (double)p.m}, {S_RHO, D.rho}});
+    taylor_interval<TD, HET>(D, p);
+    M.stats.taylor_terms = p.K;
+    M.stats.taylor_substeps = p.m;
 }
 
 void dev_transient(Model &M, const double *block_power, const double *block_temp_in, double dt,
@@ -685,6 +748,147 @@ void dev_transient(Model &M, const double *block_power, const double *block_temp
     M.stats.last_solve_ms = ms;
 }
 
+// ---- device-resident sequences of intervals -----------------------------------
+static void ensure_seq(DevModel &D, int steps, std::vector<double *> &tmp) {
+    const size_t n = (size_t)steps * D.nblk;
+    double *in = nullptr, *out = nullptr;
+    HIP_OK(hipMalloc((void **)&in, n * sizeof(double)));
+    HIP_OK(hipMalloc((void **)&out, n * sizeof(double)));
+    tmp.push_back(in);
+    tmp.push_back(out);
+    D.h.seq_in = in;
+    D.h.seq_out = out;
+    HIP_OK(hipMemcpy(D.d_desc, &D.h, sizeof(ProbDesc), hipMemcpyHostToDevice));
+}
+
+// Captures one interval's launches once as a hipGraph and replays it for
+// every interval (the kernels take the interval index from device memory, so
+// the same graph serves all of them).  Kernel-timer events captured with it
+// become event nodes, read back every `stride` replays.
+template <class F>
+static void replay(DevModel &D, int steps, F body) {
+    const size_t ev0 = g_timer.ev.size();
+    HIP_OK(hipStreamBeginCapture(D.stream, hipStreamCaptureModeThreadLocal));
+    body();
+    hipGraph_t graph = nullptr;
+    HIP_OK(hipStreamEndCapture(D.stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> cap(g_timer.ev.begin() + ev0, g_timer.ev.end());
+    g_timer.ev.resize(ev0);
+    const int stride = 25;
+    for (int k = 0; k < steps; k++) {
+        HIP_OK(hipGraphLaunch(exec, D.stream));
+        if (!cap.empty() && k % stride == stride - 1 && g_timer.graph_n < g_timer.cap) {
+            HIP_OK(hipStreamSynchronize(D.stream));
+            for (auto &e : cap) {
+                float ms = 0;
+                if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
+                    g_timer.graph_ms += ms;
+                    g_timer.graph_n++;
+                }
+            }
+        }
+    }
+    HIP_OK(hipStreamSynchronize(D.stream));
+    for (auto &e : cap) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+}
+
+template <bool HET>
+static void rk4_sequence(Model &M, DevModel &D, int steps, double dt, int slots) {
+    const ProbDesc *dd = D.d_desc;
+    replay(D, steps, [&] {
+        run_b2g(D, V_POWER, B_RHS, 1, 1);
+        hipLaunchKernelGGL(k_rk_begin, dim3(1), dim3(64), 0, D.stream, dd, 1, dt);
+        for (int a = 0; a < slots; a++) rk4_attempt<HET>(D);
+        hipLaunchKernelGGL(k_rk_end, dim3(1), dim3(64), 0, D.stream, dd, 1);
+        run_g2b(D, -1, 1);
+        hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, D.stream, dd, 1);
+    });
+}
+
+template <typename TD, bool HET>
+static void taylor_sequence(DevModel &D, const TaylorPlan &p, int steps) {
+    const ProbDesc *dd = D.d_desc;
+    replay(D, steps, [&] {
+        run_b2g(D, V_POWER, B_RHS, 1, 1);
+        taylor_interval<TD, HET>(D, p);
+        run_g2b(D, -1, 1);
+        hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, D.stream, dd, 1);
+    });
+}
+
+void dev_transient_run(Model &M, const double *power_seq, int steps, double dt, const double *init_temp,
+                       double *temp_seq) {
+    dev_prepare(M);
+    DevModel &D = *M.dev;
+    if (!init_temp && !M.trans_valid) fatal("hs_transient_run: no transient state to continue from\n");
+    std::vector<double *> tmp;
+    ensure_seq(D, steps, tmp);
+    HIP_OK(hipEventRecord(D.ev0, D.stream));
+    HIP_OK(hipMemcpyAsync(D.h.seq_in, power_seq, (size_t)steps * D.nblk * sizeof(double),
+                          hipMemcpyHostToDevice, D.stream));
+    if (init_temp) {
+        upload_blocks(D, init_temp);
+        run_b2g(D, V_TEMP, B_Y0, 0);
+        HIP_OK(hipStreamSynchronize(D.stream));
+        write_scal(D, {{S_YPAR, 0.0}});
+    }
+    // keep the start state: an RK4 interval that needs more attempts than
+    // launched is redone with host-side control
+    const int ypar0 = (read_scal(D), (int)D.h_scal[S_YPAR]);
+    HIP_OK(hipMemcpyAsync(D.h.b64[B_TMP], D.h.b64[B_Y0 + ypar0], D.vlen * sizeof(double),
+                          hipMemcpyDeviceToDevice, D.stream));
+    write_scal(D, {{S_STEP, 0.0}, {S_OVERFLOW, 0.0}, {S_TCALLS, 0.0}});
+    if (M.opt.transient == HS_TRANS_TAYLOR) {
+        const TaylorPlan p = taylor_plan(M, D, dt);
+        write_scal(D, {{S_TAU, p.tau}, {S_TK, (double)p.m}, {S_RHO, D.rho}});
+        const bool f32 = M.opt.precision == HS_PREC_MIXED;
+        if (f32 && D.hetero) taylor_sequence<float, true>(D, p, steps);
+        else if (f32) taylor_sequence<float, false>(D, p, steps);
+        else if (D.hetero) taylor_sequence<double, true>(D, p, steps);
+        else taylor_sequence<double, false>(D, p, steps);
+        M.stats.taylor_terms = p.K;
+        M.stats.taylor_substeps = p.m;
+    } else {
+        const int slots = 5;
+        if (D.hetero) rk4_sequence<true>(M, D, steps, dt, slots);
+        else rk4_sequence<false>(M, D, steps, dt, slots);
+        read_scal(D);
+        if (D.h_scal[S_OVERFLOW] != 0.0) {
+            // restore the start state and integrate interval by interval
+            HIP_OK(hipMemcpyAsync(D.h.b64[B_Y0 + ypar0], D.h.b64[B_TMP], D.vlen * sizeof(double),
+                                  hipMemcpyDeviceToDevice, D.stream));
+            write_scal(D, {{S_YPAR, (double)ypar0}, {S_STEP, 0.0}, {S_TCALLS, 0.0}});
+            for (int k = 0; k < steps; k++) {
+                run_b2g(D, V_POWER, B_RHS, 1, 1);
+                if (D.hetero) transient_rk4<true>(M, D, dt);
+                else transient_rk4<false>(M, D, dt);
+                run_g2b(D, -1, 1);
+                hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, D.stream, (const ProbDesc *)D.d_desc, 1);
+            }
+        }
+        read_scal(D);
+        M.stats.rk4_calls = (int)D.h_scal[S_TCALLS];
+    }
+    HIP_OK(hipEventRecord(D.ev1, D.stream));
+    if (temp_seq)
+        HIP_OK(hipMemcpyAsync(temp_seq, D.h.seq_out, (size_t)steps * D.nblk * sizeof(double),
+                              hipMemcpyDeviceToHost, D.stream));
+    HIP_OK(hipStreamSynchronize(D.stream));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, D.ev0, D.ev1));
+    M.stats.last_solve_ms = ms;
+    for (double *p : tmp) (void)hipFree(p);
+    D.h.seq_in = D.h.seq_out = nullptr;
+    HIP_OK(hipMemcpy(D.d_desc, &D.h, sizeof(ProbDesc), hipMemcpyHostToDevice));
+}
+
 // ---- grid-level access ---------------------------------------------------------
 void dev_get_grid(Model &M, int which, double *out) {
     if (!M.dev) fatal("no device state\n");
@@ -720,10 +924,10 @@ void dev_slope(Model &M, const double *block_power, const double *grid_temp, dou
     HIP_OK(hipMemcpyAsync(D.h.b64[B_TMP], grid_temp, M.vec_len() * sizeof(double), hipMemcpyHostToDevice,
                           D.stream));
     if (D.hetero)
-        launch(k_slope<true>, D.nwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_work,
+        launch_named("k_slope", k_slope<true>, D.nwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_work,
                (int)B_TMP, (int)B_KA);
     else
-        launch(k_slope<false>, D.nwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_work,
+        launch_named("k_slope", k_slope<false>, D.nwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_work,
                (int)B_TMP, (int)B_KA);
     HIP_OK(hipMemcpyAsync(out, D.h.b64[B_KA], M.vec_len() * sizeof(double), hipMemcpyDeviceToHost, D.stream));
     HIP_OK(hipStreamSynchronize(D.stream));
@@ -757,6 +961,36 @@ struct hs_engine_st {
 };
 
 extern "C" {
+void hs_kernel_timer(const char *name, int max_samples) {
+    using namespace hs;
+    for (auto &e : g_timer.ev) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    g_timer.ev.clear();
+    g_timer.graph_ms = 0.0;
+    g_timer.graph_n = 0;
+    g_timer.name = name ? name : "";
+    g_timer.cap = name ? max_samples : 0;
+}
+
+int hs_kernel_timer_read(double *avg_ms, double *total_ms) {
+    using namespace hs;
+    double tot = g_timer.graph_ms;
+    int n = g_timer.graph_n;
+    for (auto &e : g_timer.ev) {
+        if (hipEventSynchronize(e.second) != hipSuccess) continue;
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
+            tot += ms;
+            n++;
+        }
+    }
+    if (avg_ms) *avg_ms = n ? tot / n : 0.0;
+    if (total_ms) *total_ms = tot;
+    return n;
+}
+
 hs_engine_t *hs_engine_create(int precision, int transient) {
     auto *e = new hs_engine_st();
     e->precision = precision;

@@ -1117,6 +1117,7 @@ RC_model_t *alloc_RC_model(thermal_config_t *config, flp_t *placeholder,
     M->opt.steady_rtol = 1e-12;
     M->opt.max_iters = 200000;
     M->opt.taylor_tol = 1e-7;
+    M->opt.steady_warm_start = 0;
     h->rc.grid = &h->pub;
     h->rc.type = GRID_MODEL;
     h->rc.config = &h->pub.config;
@@ -1507,8 +1508,16 @@ void steady_state_temp(RC_model_t *model, double *power, double *temp) {
         steady_grid(M, power, temp);
         return;
     }
-    // temperature-leakage fixed point (temperature.c:708-746)
+    // temperature-leakage fixed point (temperature.c:708-746); each re-solve
+    // starts from the previous one
     const int tn = total_nodes(M);
+    const int warm = M.opt.steady_warm_start;
+    M.opt.steady_warm_start = 1;
+    struct Restore {
+        Model &m;
+        int w;
+        ~Restore() { m.opt.steady_warm_start = w; }
+    } restore{M, warm};
     std::vector<double> d_temp(tn, 0.0), temp_old(tn, 0.0), power_new(tn, 0.0);
     bool conv = false;
     for (int it = 0; !conv && it <= 100; it++) {
@@ -1609,6 +1618,18 @@ void hs_set_grid_temp(RC_model_t *model, int which, const double *in) {
     dev_set_grid(M, which, in);
     if (which == 0) { M.steady_valid = 1; M.mirror_steady = 0; }
     else { M.trans_valid = 1; M.mirror_trans = 0; }
+    HS_END()
+}
+
+void hs_transient_run(RC_model_t *model, const double *power_seq, int n_steps, double dt,
+                      const double *init_temp, double *temp_seq) {
+    HS_BEGIN
+    Model &M = *model_of(model);
+    if (!M.g->r_ready || !M.g->c_ready) fatal("grid model not ready\n");
+    if (n_steps <= 0) fatal("hs_transient_run: n_steps must be positive\n");
+    dev_transient_run(M, power_seq, n_steps, dt, init_temp, temp_seq);
+    M.trans_valid = 1;
+    M.mirror_trans = 0;
     HS_END()
 }
 

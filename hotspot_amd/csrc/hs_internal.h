@@ -104,6 +104,8 @@ void dev_steady(Model &M, const double *block_power, double *block_temp);
 // transient: block power; block temp (nullptr = continue from last_trans)
 void dev_transient(Model &M, const double *block_power, const double *block_temp_in,
                    double dt, double *block_temp_out);
+void dev_transient_run(Model &M, const double *power_seq, int steps, double dt,
+                       const double *init_temp, double *temp_seq);
 void dev_get_grid(Model &M, int which, double *out);
 void dev_set_grid(Model &M, int which, const double *in);
 void dev_slope(Model &M, const double *block_power, const double *grid_temp, double *out);

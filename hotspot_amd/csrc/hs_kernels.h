@@ -49,7 +49,11 @@ enum Scal {
     S_H, S_NEWH, S_T, S_DT, S_STATUS, S_YPAR, S_CALLS, S_ERR,
     S_TAU, S_TMAX, S_TDONE, S_TK,   // Taylor: step, term max, done flag, substeps
     S_NREFINE,
-    S_RHO = S_TMAX + 8,             // Taylor: bound on ||C^-1 G||_inf
+    S_STEP,                      // interval index of a device-resident sequence
+    S_OVERFLOW,                  // an interval needed more RK4 attempts than launched
+    S_RHO,                       // Taylor: bound on ||C^-1 G||_inf
+    S_TCALLS,                    // RK4 calls summed over a sequence
+    S_KNEED,                     // Taylor: terms of the current sub-step
     NSCAL = 32
 };
 enum RkStatus { RK_NEED_K1 = 0, RK_ATTEMPT = 1, RK_DONE = 2 };
@@ -96,6 +100,8 @@ struct ProbDesc {
     int map_mode, pad1;
     double *blk_in;                     // block vector (device staging)
     double *blk_out;
+    double *seq_in;                     // [steps][nblocks + nx] power sequence
+    double *seq_out;                    // [steps][nblocks + nx] temperatures
     double *bconst;                     // per-cell constant source (inlet), hetero only
     // reduction state
     double *partial;                    // [ntiles+1][kRed]

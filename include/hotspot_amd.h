@@ -292,7 +292,8 @@ typedef struct hs_options_st {
     int transient;        /* HS_TRANS_* */
     double steady_rtol;   /* relative residual target of the steady solve (FP64 norm) */
     int max_iters;        /* per steady solve */
-    double taylor_tol;    /* a-priori bound on the Taylor truncation error per interval (K) */
+    double taylor_tol;    /* bound on the Taylor truncation error per interval (K) */
+    int steady_warm_start; /* 1: start a steady solve from the previous solution (0: from ambient) */
 } hs_options_t;
 
 typedef struct hs_stats_st {
@@ -323,6 +324,22 @@ void hs_grid_slope(RC_model_t *model, const double *block_power, const double *g
 void hs_block_to_grid(RC_model_t *model, const double *block_vec, double *grid_vec, int type);
 /* grid -> block translation on the device (the model's grid_map_mode) */
 void hs_grid_to_block(RC_model_t *model, const double *grid_vec, double *block_vec);
+
+/* Runs n_steps transient intervals of dt seconds on the device without a host
+ * round trip per interval: equivalent to n_steps calls
+ * compute_temp(model, power_seq + k*n, NULL, dt) with n = total blocks + package
+ * nodes (the package-node entries of each power vector are ignored).
+ * init_temp (a block vector) starts the state like compute_temp's first call;
+ * NULL continues from the current transient state.  The block temperatures of
+ * every interval are written to temp_seq (n_steps*n doubles) when non-NULL. */
+void hs_transient_run(RC_model_t *model, const double *power_seq, int n_steps, double dt,
+                      const double *init_temp, double *temp_seq);
+
+/* Times every launch of one kernel family (e.g. "k_tay_term", "k_rk_stage",
+ * "k_cg_dir") with device events, up to max_samples launches; NULL stops. */
+void hs_kernel_timer(const char *name, int max_samples);
+/* Number of timed launches; their mean and total duration in ms. */
+int hs_kernel_timer_read(double *avg_ms, double *total_ms);
 
 /* ---- batched engine: many models advanced by shared launches ------------- */
 typedef struct hs_engine_st hs_engine_t;

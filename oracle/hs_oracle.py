@@ -898,6 +898,17 @@ class OracleModel:
                 break
         return ytemp, h, new_h
 
+    def advance_exact(self, Tg, Pg, dt):
+        """Exact solution of the interval's linear ODE C dT/dt = Pg + b - Gt T:
+        T(dt) = T_inf + exp(-C^-1 Gt dt) (T - T_inf), T_inf = Gt^-1 (Pg + b)
+        (scipy's expm_multiply).  The reference's adaptive RK4 accepts up to
+        0.01 K of step-doubling error per step, so this is the yardstick for
+        integrators that are more accurate than it."""
+        rhs = Pg + self.b
+        tinf = spla.spsolve(self.Gt.tocsc(), rhs)
+        A = sp.diags(1.0 / self.Cvec) @ self.Gt
+        return tinf + spla.expm_multiply(-dt * A, Tg - tinf)
+
     def advance(self, Tg, Pg, dt):
         """compute_temp_grid's integration loop (:3676-3692).  Returns the new
         grid state and the number of RK4 calls."""

@@ -43,7 +43,8 @@ def check_error(lib):
 
 class HsOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("steady_solver", C.c_int), ("transient", C.c_int),
-                ("steady_rtol", C.c_double), ("max_iters", C.c_int), ("taylor_tol", C.c_double)]
+                ("steady_rtol", C.c_double), ("max_iters", C.c_int), ("taylor_tol", C.c_double),
+                ("steady_warm_start", C.c_int)]
 
 
 class HsStats(C.Structure):

@@ -131,9 +131,14 @@ def test_transient_rk4_matches_oracle(lib, name, case, pw):
     run.free()
 
 
-@pytest.mark.parametrize("prec,tol", [(0, FP64_TOL), (1, MIXED_TOL)], ids=["fp64", "mixed"])
+# The Taylor integrator sums the exponential series to a 1e-7 K tail, so it is
+# checked against the exact solution of each interval's linear ODE (the
+# oracle's expm); the reference's adaptive RK4 itself sits up to ~2e-4 K from
+# that solution when the first interval starts from block temperatures
+# mapped onto the grid (piecewise constant per block, fast modes excited).
+@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (1, 1e-4)], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("name,case,pw", [CASES[0], CASES[3]], ids=["default-64", "3d-stack-32"])
-def test_transient_taylor_matches_oracle(lib, name, case, pw, prec, tol):
+def test_transient_taylor_matches_exact(lib, name, case, pw, prec, tol):
     run = make(lib, case)
     set_options(lib, run, transient=1, precision=prec)
     om = oracle_for(run)
@@ -145,7 +150,7 @@ def test_transient_taylor_matches_oracle(lib, name, case, pw, prec, tol):
     Tg = om.b2g(T0, power=False)
     worst = 0.0
     for k, p in enumerate(seq):
-        Tg, _ = om.advance(Tg, om.power_grid(p), 3.333e-6)
+        Tg = om.advance_exact(Tg, om.power_grid(p), 3.333e-6)
         worst = max(worst, np.abs(tr[k] - om.g2b(Tg)).max())
     assert worst <= tol
     run.free()
