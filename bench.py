@@ -183,7 +183,6 @@ def main():
         tw, _, _, _, _ = job.once()
         log(f"warmup {w}: {tw:.3f} s")
     device_sync()
-    L.hs_kernel_timer(b"k_tay_term", 4000)
     times, steady_t, trans_t = [], [], []
     for k in range(args.steps):
         t, ts, tt, _, _ = job.once()
@@ -192,9 +191,16 @@ def main():
         trans_t.append(tt)
         log(f"step {k}: {t:.3f} s (steady {ts*1e3:.1f} ms, transient {tt:.3f} s)")
     device_sync()
+    # roofline pass: the same job with every launch of the dominant kernel
+    # bracketed by device events on the library's stream (plain launches
+    # instead of graph replay while the timer is on)
+    L.hs_kernel_timer(b"k_tay_term", 20000)
+    t_roof, _, _, _, _ = job.once()
+    device_sync()
     avg_ms, tot_ms = C.c_double(), C.c_double()
     nt = L.hs_kernel_timer_read(C.byref(avg_ms), C.byref(tot_ms))
     L.hs_kernel_timer(None, 0)
+    log(f"roofline pass {t_roof:.3f} s: {nt} launches of k_tay_term, mean {avg_ms.value*1e3:.2f} us")
     st = job.stats()
     cells = job.cells
     updates = cells * (1 + args.intervals)

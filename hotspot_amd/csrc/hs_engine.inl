@@ -765,8 +765,18 @@ static void ensure_seq(DevModel &D, int steps, std::vector<double *> &tmp) {
 // every interval (the kernels take the interval index from device memory, so
 // the same graph serves all of them).  Kernel-timer events captured with it
 // become event nodes, read back every `stride` replays.
+static bool graphs_enabled() {
+    static const bool off = getenv("HS_NO_GRAPH") != nullptr;
+    return !off && g_timer.cap == 0;
+}
+
 template <class F>
 static void replay(DevModel &D, int steps, F body) {
+    if (!graphs_enabled()) {   // plain launches: per-launch timing / profilers
+        for (int k = 0; k < steps; k++) body();
+        HIP_OK(hipStreamSynchronize(D.stream));
+        return;
+    }
     const size_t ev0 = g_timer.ev.size();
     HIP_OK(hipStreamBeginCapture(D.stream, hipStreamCaptureModeThreadLocal));
     body();
