@@ -296,21 +296,29 @@ __device__ __forceinline__ bool problem_skip(const ProbDesc &d, int flag_slot) {
     return flag_slot >= 0 && d.scal[flag_slot] != 0.0;
 }
 
+// sequence mode: a problem whose intervals are all done takes no part
+__device__ __forceinline__ bool seq_finished(const ProbDesc &d, int seq) {
+    return seq && d.scal[S_STEP] >= d.scal[S_NSTEPS];
+}
+
 // ---------------------------------------------------------------------------
 // block <-> grid translation (xlate_vector_b2g :1888-1929, xlate_temp_g2b
 // :1932-2015)
 // ---------------------------------------------------------------------------
 // out = sum over the cell's blocks of occ * v_u (/ area_u * cell area for
 // power); with add_sources the ambient / inlet constants are added (rhs).
-// from_seq: read the block vector of the current sequence step instead of
-// the staging vector.
+// from_seq: 1 reads the block vector of the current sequence step, 2 the
+// block output vector (a steady result handed to a transient), 0 the staging
+// vector.
 __global__ void __launch_bounds__(kBlock) k_b2g(const ProbDesc *__restrict__ D,
                                                 const WorkItem *__restrict__ W, int type,
                                                 int out_buf, int add_sources, int from_seq) {
     const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
+    if (seq_finished(d, from_seq == 1)) return;
     double *out = d.b64[out_buf];
-    const double *b = from_seq ? d.seq_in + (long)d.scal[S_STEP] * (d.nblocks + d.nx) : d.blk_in;
+    const double *b = from_seq == 1 ? d.seq_in + (long)d.scal[S_STEP] * (d.nblocks + d.nx)
+                      : from_seq == 2 ? d.blk_out : d.blk_in;
     const long ncell = (long)d.nl * d.nr * d.nc;
     if (wi.tile < d.ntiles) {
         const Cell q = tile_cell(d, wi.tile);
@@ -352,6 +360,7 @@ __global__ void __launch_bounds__(kBlock) k_g2b(const ProbDesc *__restrict__ D,
                                                 int to_seq) {
     const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
+    if (seq_finished(d, to_seq)) return;
     const double *g = d.b64[in_buf >= 0 ? in_buf : B_Y0 + (int)d.scal[S_YPAR]];
     double *bo = to_seq ? d.seq_out + (long)d.scal[S_STEP] * (d.nblocks + d.nx) : d.blk_out;
     const long plane = (long)d.nr * d.nc, ncell = plane * d.nl;
@@ -537,7 +546,7 @@ __global__ void __launch_bounds__(kBlock) k_ir_resid(const ProbDesc *__restrict_
                                                      double inner_rtol2, int max_refine) {
     const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
-    if (problem_skip(d, S_DONE)) return;
+    if (problem_skip(d, S_DONE) || d.scal[S_IDONE] == 0.0) return;   // converged / inner solve running
     const double *x = d.b64[B_STEADY], *rhs = d.b64[B_RHS];
     float *r32 = d.b32[F_R], *z32 = d.b32[F_Z], *e32 = d.b32[F_X];
     LdPlain<double> ld{x};
@@ -573,12 +582,13 @@ __global__ void __launch_bounds__(kBlock) k_ir_resid(const ProbDesc *__restrict_
     write_partials<3>(d, wi.tile, v);
 }
 
-// x (FP64) += e (FP32)
+// x (FP64) += e (FP32), for problems whose inner solve finished with a
+// correction pending
 __global__ void __launch_bounds__(kBlock) k_ir_correct(const ProbDesc *__restrict__ D,
                                                        const WorkItem *__restrict__ W) {
     const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
-    if (problem_skip(d, S_DONE)) return;
+    if (problem_skip(d, S_DONE) || d.scal[S_IDONE] == 0.0 || d.scal[S_CORR] == 0.0) return;
     double *x = d.b64[B_STEADY];
     const float *e = d.b32[F_X];
     if (wi.tile < d.ntiles) {
@@ -707,8 +717,8 @@ __global__ void __launch_bounds__(kBlock) k_rk_control(const ProbDesc *__restric
 // per problem, same skip condition as the producer kernel.
 enum FinOp { FIN_CG_INIT = 0, FIN_ALPHA, FIN_BETA, FIN_IR, FIN_TAY };
 struct FinArgs {
-    double rtol2, inner_rtol2, tol;
-    int max_iters, max_refine, done_slot, tol_slot, sub;
+    double rtol2, inner_rtol2;
+    int max_iters, max_refine, done_slot, tol_slot, sub, seq;
 };
 
 __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ D, int op, FinArgs a) {
@@ -753,7 +763,7 @@ __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ 
             return;
         }
         case FIN_IR: {
-            if (s[S_DONE] != 0.0) return;
+            if (s[S_DONE] != 0.0 || s[S_IDONE] == 0.0) return;
             reduce_slots<3>(d, v);
             if (threadIdx.x) return;
             s[S_RR] = v[1];
@@ -763,28 +773,31 @@ __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ 
             s[S_RR0] = v[1];
             s[S_ITOL2] = a.inner_rtol2 * v[1];
             s[S_BETA] = 0.0;
+            s[S_TITERS] += s[S_ITERS];
             s[S_ITERS] = 0.0;
             s[S_PPAR] = 0.0;
             const bool done = v[1] <= s[S_TOL2] || s[S_NREFINE] >= a.max_refine || !(v[1] == v[1]);
             s[S_DONE] = done ? 1.0 : 0.0;
             s[S_IDONE] = done ? 1.0 : 0.0;
+            s[S_CORR] = done ? 0.0 : 1.0;
             s[S_NREFINE] += 1.0;
             return;
         }
         case FIN_TAY: {
             // number of terms: the smallest K whose geometric tail bound from
             // max|E_1| (||E_{k+1}|| <= ||E_k|| rho tau/(k+1)) is under tol
-            if (a.sub >= (int)s[S_TK]) return;
+            if (a.sub >= (int)s[S_TK] || seq_finished(d, a.seq)) return;
             double w[1];
             reduce_slots<1, true>(d, w);
             if (threadIdx.x) return;
             const double z = s[S_RHO] * s[S_TAU];
+            const int kmax = (int)s[S_KPLAN];
             double term = w[0];
             int K = 1;
-            while (K < 60) {
+            while (K < kmax) {
                 term *= z / (K + 1);
                 const double q = z / (K + 2);
-                if (q < 1.0 && term / (1.0 - q) <= a.tol) break;
+                if (q < 1.0 && term / (1.0 - q) <= s[S_TTOL]) break;
                 K++;
             }
             s[S_TMAX] = w[0];
@@ -794,13 +807,16 @@ __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ 
     }
 }
 
-// interval set-up / check / step counter of device-resident sequences
-__global__ void k_rk_begin(const ProbDesc *__restrict__ D, int nprob, double dt) {
+// Intervals of a device-resident sequence.  An interval that needs more RK4
+// attempts than one launch sequence holds is continued by the next one: a
+// problem starts a new interval only when its previous one is done, and its
+// step counter moves only then.
+__global__ void k_rk_begin(const ProbDesc *__restrict__ D, int nprob) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nprob) return;
     double *s = D[p].scal;
+    if (seq_finished(D[p], 1) || s[S_STATUS] != (double)RK_DONE) return;
     s[S_T] = 0.0;
-    s[S_DT] = dt;
     s[S_H] = kMinStep;
     s[S_STATUS] = RK_NEED_K1;
     s[S_CALLS] = 0.0;
@@ -810,14 +826,34 @@ __global__ void k_rk_end(const ProbDesc *__restrict__ D, int nprob) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nprob) return;
     double *s = D[p].scal;
-    if (s[S_STATUS] != (double)RK_DONE) s[S_OVERFLOW] = 1.0;
+    if (seq_finished(D[p], 1) || s[S_STATUS] != (double)RK_DONE) return;
     s[S_TCALLS] += s[S_CALLS];
+    s[S_CALLS] = 0.0;
 }
 
 __global__ void k_step_advance(const ProbDesc *__restrict__ D, int nprob) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nprob) return;
+    if (seq_finished(D[p], 1) || D[p].scal[S_STATUS] != (double)RK_DONE) return;
     D[p].scal[S_STEP] += 1.0;
+}
+
+// copies up to 8 scalar slots of every problem into out[p * 8 + i]
+struct Slots {
+    int n;
+    int s[8];
+};
+__global__ void k_gather(const ProbDesc *__restrict__ D, int nprob, Slots sl, double *out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprob) return;
+    for (int i = 0; i < sl.n; i++) out[p * 8 + i] = D[p].scal[sl.s[i]];
+}
+
+// writes scalar slots of every problem: vals[p * 8 + i] into slot sl.s[i]
+__global__ void k_scatter(const ProbDesc *__restrict__ D, int nprob, Slots sl, const double *vals) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprob) return;
+    for (int i = 0; i < sl.n; i++) D[p].scal[sl.s[i]] = vals[p * 8 + i];
 }
 
 // ---------------------------------------------------------------------------
@@ -833,10 +869,10 @@ __global__ void k_step_advance(const ProbDesc *__restrict__ D, int nprob) {
 template <typename TD, bool HET>
 __global__ void __launch_bounds__(kBlock) k_tay_first(const ProbDesc *__restrict__ D,
                                                       const WorkItem *__restrict__ W, int d0b,
-                                                      int accb, int sub) {
+                                                      int accb, int sub, int seq) {
     const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
-    if (sub >= (int)d.scal[S_TK]) return;
+    if (sub >= (int)d.scal[S_TK] || seq_finished(d, seq)) return;
     const int ypar = (int)d.scal[S_YPAR];
     const double *y = d.b64[B_Y0 + ypar], *rhs = d.b64[B_RHS];
     auto buf = [&](int b) -> TD * { return sizeof(TD) == 8 ? (TD *)d.b64[b] : (TD *)d.b32[b]; };
@@ -875,10 +911,10 @@ __global__ void __launch_bounds__(kBlock) k_tay_first(const ProbDesc *__restrict
 template <typename TD, bool HET>
 __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict__ D,
                                                      const WorkItem *__restrict__ W, int inb,
-                                                     int outb, int accb, int k, int sub) {
+                                                     int outb, int accb, int k, int sub, int seq) {
     const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
-    if (sub >= (int)d.scal[S_TK] || k >= (int)d.scal[S_KNEED]) return;
+    if (sub >= (int)d.scal[S_TK] || k >= (int)d.scal[S_KNEED] || seq_finished(d, seq)) return;
     auto buf = [&](int b) -> TD * { return sizeof(TD) == 8 ? (TD *)d.b64[b] : (TD *)d.b32[b]; };
     const TD *ein = buf(inb);
     TD *eout = buf(outb), *acc = buf(accb);
@@ -913,10 +949,10 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict_
 template <typename TD>
 __global__ void __launch_bounds__(kBlock) k_tay_final(const ProbDesc *__restrict__ D,
                                                       const WorkItem *__restrict__ W, int accb,
-                                                      int sub) {
+                                                      int sub, int seq) {
     const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
-    if (sub >= (int)d.scal[S_TK]) return;
+    if (sub >= (int)d.scal[S_TK] || seq_finished(d, seq)) return;
     const int ypar = (int)d.scal[S_YPAR];
     double *y = d.b64[B_Y0 + ypar];
     const TD *acc = sizeof(TD) == 8 ? (const TD *)d.b64[accb] : (const TD *)d.b32[accb];

@@ -6,7 +6,22 @@
 
 namespace hs {
 
+// A set of problems advanced by shared launches: the descriptor array, the
+// cell-tile work-list (tiles of every problem, then its package item) and the
+// grid->block work-list.  One model is a group of one.
+struct Group {
+    ProbDesc *d_desc = nullptr;
+    int nprob = 0;
+    WorkItem *d_work = nullptr;
+    int nwork = 0;
+    WorkItem *d_gwork = nullptr;
+    int ngwork = 0;
+    hipStream_t stream = nullptr;
+    bool hetero = false;
+};
+
 struct DevModel {
+    Group grp;                          // this model alone
     ProbDesc h{};                       // host copy of the descriptor
     ProbDesc *d_desc = nullptr;
     WorkItem *d_work = nullptr;         // cell tiles + package item
@@ -14,7 +29,9 @@ struct DevModel {
     WorkItem *d_gwork = nullptr;        // grid -> block items
     int ngwork = 0;
     std::vector<void *> allocs;
+    size_t bytes = 0;                   // device memory held
     hipStream_t stream = nullptr;
+    bool own_stream = true;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double *h_scal = nullptr;           // pinned copy of the scalars
     double *h_blk = nullptr;            // pinned block vector staging
@@ -25,8 +42,10 @@ struct DevModel {
 
     template <typename T> T *alloc(size_t n) {
         void *p = nullptr;
-        HIP_OK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+        const size_t sz = std::max<size_t>(n, 1) * sizeof(T);
+        HIP_OK(hipMalloc(&p, sz));
         allocs.push_back(p);
+        bytes += sz;
         return (T *)p;
     }
     template <typename T> T *upload(const T *src, size_t n) {
@@ -35,12 +54,13 @@ struct DevModel {
         return p;
     }
     ~DevModel() {
+        if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : allocs) (void)hipFree(p);
         if (h_scal) (void)hipHostFree(h_scal);
         if (h_blk) (void)hipHostFree(h_blk);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -371,25 +391,19 @@ static double package_rho(const Model &M, const ProbDesc &h) {
     return rho;
 }
 
-static void dev_prepare(Model &M) {
+// Device state of a model: coefficients, maps, work-lists and the solver
+// buffers.  With `shared` the launches go to that stream (the engine's).
+static DevModel *dev_build(Model &M, hipStream_t shared) {
     grid_model_t &g = *M.g;
-    if (!g.r_ready) fatal("R model not ready\n");
-    if (M.dev && !M.dev_dirty && M.dev->has_c == (bool)g.c_ready) return;
-    // keep the device state across a rebuild (warm starts)
-    std::vector<double> keep_steady, keep_trans;
-    if (M.dev && M.steady_valid) {
-        keep_steady.resize(M.vec_len());
-        dev_get_grid(M, 0, keep_steady.data());
-    }
-    if (M.dev && M.trans_valid) {
-        keep_trans.resize(M.vec_len());
-        dev_get_grid(M, 1, keep_trans.data());
-    }
-    dev_release(M);
     if (g.n_layers > kMaxLayers) fatal("too many layers for the device model\n");
-    auto *D = new DevModel();
-    M.dev = D;
-    HIP_OK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));
+    std::unique_ptr<DevModel> own(new DevModel());
+    DevModel *D = own.get();
+    if (shared) {
+        D->stream = shared;
+        D->own_stream = false;
+    } else {
+        HIP_OK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));
+    }
     HIP_OK(hipEventCreate(&D->ev0));
     HIP_OK(hipEventCreate(&D->ev1));
     ProbDesc &h = D->h;
@@ -464,6 +478,14 @@ static void dev_prepare(Model &M) {
     gw.push_back(WorkItem{0, -1});
     D->ngwork = (int)gw.size();
     D->d_gwork = D->upload(gw.data(), gw.size());
+    D->grp.d_desc = D->d_desc;
+    D->grp.nprob = 1;
+    D->grp.d_work = D->d_work;
+    D->grp.nwork = D->nwork;
+    D->grp.d_gwork = D->d_gwork;
+    D->grp.ngwork = D->ngwork;
+    D->grp.stream = D->stream;
+    D->grp.hetero = M.detailed;
     if (D->hetero) assemble_hetero(M, *D);
     // transient step bound
     if (g.c_ready) {
@@ -483,6 +505,25 @@ static void dev_prepare(Model &M) {
         D->rho = std::max(D->rho, package_rho(M, h));
     }
     HIP_OK(hipMemcpy(D->d_desc, &h, sizeof(ProbDesc), hipMemcpyHostToDevice));
+    return own.release();
+}
+
+static void dev_prepare(Model &M) {
+    grid_model_t &g = *M.g;
+    if (!g.r_ready) fatal("R model not ready\n");
+    if (M.dev && !M.dev_dirty && M.dev->has_c == (bool)g.c_ready) return;
+    // keep the device state across a rebuild (warm starts)
+    std::vector<double> keep_steady, keep_trans;
+    if (M.dev && M.steady_valid) {
+        keep_steady.resize(M.vec_len());
+        dev_get_grid(M, 0, keep_steady.data());
+    }
+    if (M.dev && M.trans_valid) {
+        keep_trans.resize(M.vec_len());
+        dev_get_grid(M, 1, keep_trans.data());
+    }
+    dev_release(M);
+    M.dev = dev_build(M, nullptr);
     M.dev_dirty = false;
     if (!keep_steady.empty()) dev_set_grid(M, 0, keep_steady.data());
     if (!keep_trans.empty()) dev_set_grid(M, 1, keep_trans.data());
@@ -511,53 +552,83 @@ static void download_blocks(DevModel &D, double *v) {
     memcpy(v, D.h_blk, D.nblk * sizeof(double));
 }
 
-static void run_b2g(DevModel &D, int type, int out, int sources, int from_seq = 0) {
-    launch(k_b2g, D.nwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_work, type, out,
+static inline int small_grid(int n) { return (n + 63) / 64; }
+
+static void run_b2g(const Group &G, int type, int out, int sources, int from_seq = 0) {
+    launch(k_b2g, G.nwork, G.stream, (const ProbDesc *)G.d_desc, (const WorkItem *)G.d_work, type, out,
            sources, from_seq);
 }
 
-static void run_g2b(DevModel &D, int in, int to_seq = 0) {
-    launch(k_g2b, D.ngwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_gwork, in, to_seq);
+static void run_g2b(const Group &G, int in, int to_seq = 0) {
+    launch(k_g2b, G.ngwork, G.stream, (const ProbDesc *)G.d_desc, (const WorkItem *)G.d_gwork, in, to_seq);
 }
 
 // ---- steady ----------------------------------------------------------------
-static void finish(DevModel &D, int nprob, int op, const FinArgs &a) {
-    launch(k_finish, nprob, D.stream, (const ProbDesc *)D.d_desc, op, a);
+static void finish(const Group &G, int op, const FinArgs &a) {
+    launch(k_finish, G.nprob, G.stream, (const ProbDesc *)G.d_desc, op, a);
 }
 
 template <typename T, bool HET>
-static void cg_iterations(DevModel &D, int n, int xb, int rb, int zb, int p0b, int qb, int done_slot,
+static void cg_iterations(const Group &G, int n, int xb, int rb, int zb, int p0b, int qb, int done_slot,
                           int tol_slot, int max_iters) {
-    const ProbDesc *dd = D.d_desc;
-    const WorkItem *w = D.d_work;
+    const ProbDesc *dd = G.d_desc;
+    const WorkItem *w = G.d_work;
     FinArgs a{};
     a.done_slot = done_slot;
     a.tol_slot = tol_slot;
     a.max_iters = max_iters;
     for (int k = 0; k < n; k++) {
-        launch_named("k_cg_dir", k_cg_dir<T, HET>, D.nwork, D.stream, dd, w, zb, p0b, qb, done_slot);
-        finish(D, 1, FIN_ALPHA, a);
-        launch_named("k_cg_upd", k_cg_upd<T, HET>, D.nwork, D.stream, dd, w, xb, rb, zb, p0b, qb, done_slot, tol_slot,
-               max_iters);
-        finish(D, 1, FIN_BETA, a);
+        launch_named("k_cg_dir", k_cg_dir<T, HET>, G.nwork, G.stream, dd, w, zb, p0b, qb, done_slot);
+        finish(G, FIN_ALPHA, a);
+        launch_named("k_cg_upd", k_cg_upd<T, HET>, G.nwork, G.stream, dd, w, xb, rb, zb, p0b, qb, done_slot,
+                     tol_slot, max_iters);
+        finish(G, FIN_BETA, a);
     }
 }
+
+// FP64 Jacobi-PCG: r = rhs - G x, z, and the CG scalars of every problem
+template <bool HET>
+static void jpcg64_start(const Group &G, double rtol2, int max_iters) {
+    launch_named("k_cg_init", k_cg_init<double, HET>, G.nwork, G.stream, (const ProbDesc *)G.d_desc,
+                 (const WorkItem *)G.d_work, (int)B_STEADY, (int)B_R, (int)B_Z, (int)B_RHS, rtol2, max_iters);
+    FinArgs a{};
+    a.rtol2 = rtol2;
+    a.max_iters = max_iters;
+    finish(G, FIN_CG_INIT, a);
+}
+
+// Mixed-precision refinement, one round: problems whose inner FP32 solve has
+// finished take its correction and start the next one from a fresh FP64
+// residual (or stop); then `chunk` inner iterations for every problem.
+constexpr double kInnerRtol2 = 1e-8;   // each FP32 solve gains 4 digits
+constexpr int kMaxRefine = 30;
+
+template <bool HET>
+static void mixed_round(const Group &G, double rtol2, int max_iters, int chunk) {
+    launch(k_ir_correct, G.nwork, G.stream, (const ProbDesc *)G.d_desc, (const WorkItem *)G.d_work);
+    launch_named("k_ir_resid", k_ir_resid<HET>, G.nwork, G.stream, (const ProbDesc *)G.d_desc,
+                 (const WorkItem *)G.d_work, rtol2, kInnerRtol2, kMaxRefine);
+    FinArgs a{};
+    a.rtol2 = rtol2;
+    a.inner_rtol2 = kInnerRtol2;
+    a.max_refine = kMaxRefine;
+    finish(G, FIN_IR, a);
+    cg_iterations<float, HET>(G, chunk, F_X, F_R, F_Z, F_P0, F_Q, S_IDONE, S_ITOL2, max_iters);
+}
+
+// scalar slots that start a mixed solve
+static const std::vector<std::pair<int, double>> kMixedStart = {
+    {S_NREFINE, 0.0}, {S_DONE, 0.0}, {S_IDONE, 1.0}, {S_CORR, 0.0}, {S_TITERS, 0.0}, {S_ITERS, 0.0}};
 
 template <bool HET>
 static void steady_jpcg64(Model &M, DevModel &D) {
     const double rtol2 = M.opt.steady_rtol * M.opt.steady_rtol;
-    launch_named("k_cg_init", k_cg_init<double, HET>, D.nwork, D.stream, (const ProbDesc *)D.d_desc,
-           (const WorkItem *)D.d_work, (int)B_STEADY, (int)B_R, (int)B_Z, (int)B_RHS, rtol2,
-           M.opt.max_iters);
-    FinArgs a{};
-    a.rtol2 = rtol2;
-    a.max_iters = M.opt.max_iters;
-    finish(D, 1, FIN_CG_INIT, a);
+    jpcg64_start<HET>(D.grp, rtol2, M.opt.max_iters);
     int chunk = 8;
     for (;;) {
         read_scal(D);
         if (D.h_scal[S_DONE] != 0.0) break;
-        cg_iterations<double, HET>(D, chunk, B_STEADY, B_R, B_Z, B_P0, B_Q, S_DONE, S_TOL2, M.opt.max_iters);
+        cg_iterations<double, HET>(D.grp, chunk, B_STEADY, B_R, B_Z, B_P0, B_Q, S_DONE, S_TOL2, M.opt.max_iters);
         chunk = std::min(chunk * 2, 64);
     }
     M.stats.steady_iters = (int)D.h_scal[S_ITERS];
@@ -568,31 +639,16 @@ static void steady_jpcg64(Model &M, DevModel &D) {
 template <bool HET>
 static void steady_mixed(Model &M, DevModel &D) {
     const double rtol2 = M.opt.steady_rtol * M.opt.steady_rtol;
-    const double inner2 = 1e-8;   // each FP32 solve gains 4 digits
-    const int max_refine = 30;
-    write_scal(D, {{S_NREFINE, 0.0}, {S_DONE, 0.0}});
-    int total = 0;
+    write_scal(D, kMixedStart);
+    int chunk = 8;
     for (;;) {
-        launch_named("k_ir_resid", k_ir_resid<HET>, D.nwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_work,
-               rtol2, inner2, max_refine);
-        FinArgs a{};
-        a.rtol2 = rtol2;
-        a.inner_rtol2 = inner2;
-        a.max_refine = max_refine;
-        finish(D, 1, FIN_IR, a);
+        mixed_round<HET>(D.grp, rtol2, M.opt.max_iters, chunk);
         read_scal(D);
         if (D.h_scal[S_DONE] != 0.0) break;
-        int chunk = 8;
-        for (;;) {
-            cg_iterations<float, HET>(D, chunk, F_X, F_R, F_Z, F_P0, F_Q, S_IDONE, S_ITOL2, M.opt.max_iters);
-            read_scal(D);
-            if (D.h_scal[S_IDONE] != 0.0) break;
-            chunk = std::min(chunk * 2, 64);
-        }
-        total += (int)D.h_scal[S_ITERS];
-        launch(k_ir_correct, D.nwork, D.stream, (const ProbDesc *)D.d_desc, (const WorkItem *)D.d_work);
+        // a new inner solve starts with short chunks
+        chunk = D.h_scal[S_IDONE] != 0.0 ? 8 : std::min(chunk * 2, 64);
     }
-    M.stats.steady_iters = total;
+    M.stats.steady_iters = (int)D.h_scal[S_TITERS];
     M.stats.steady_refinements = (int)D.h_scal[S_NREFINE];
     M.stats.steady_rel_resid = D.h_scal[S_BB] > 0 ? std::sqrt(D.h_scal[S_RR] / D.h_scal[S_BB]) : 0.0;
 }
@@ -600,9 +656,11 @@ static void steady_mixed(Model &M, DevModel &D) {
 void dev_steady(Model &M, const double *block_power, double *block_temp) {
     dev_prepare(M);
     DevModel &D = *M.dev;
+    if (D.h.has_adv) fatal("steady solve of a microchannel model: the advection terms make G non-symmetric, "
+                           "which the CG solvers do not handle\n");
     HIP_OK(hipEventRecord(D.ev0, D.stream));
     upload_blocks(D, block_power);
-    run_b2g(D, V_POWER, B_RHS, 1);
+    run_b2g(D.grp, V_POWER, B_RHS, 1);
     if (!M.steady_valid || !M.opt.steady_warm_start) {
         const int nb = (int)((D.vlen + kBlock - 1) / kBlock);
         launch(k_fill, nb, D.stream, D.h.b64[B_STEADY], D.vlen, M.g->config.ambient);
@@ -615,7 +673,7 @@ void dev_steady(Model &M, const double *block_power, double *block_temp) {
         if (D.hetero) steady_jpcg64<true>(M, D);
         else steady_jpcg64<false>(M, D);
     }
-    run_g2b(D, B_STEADY);
+    run_g2b(D.grp, B_STEADY);
     HIP_OK(hipEventRecord(D.ev1, D.stream));
     download_blocks(D, block_temp);
     float ms = 0;
@@ -625,11 +683,12 @@ void dev_steady(Model &M, const double *block_power, double *block_temp) {
 
 // ---- transient ---------------------------------------------------------------
 template <bool HET>
-static void rk4_attempt(DevModel &D) {
-    const ProbDesc *dd = D.d_desc;
-    const WorkItem *w = D.d_work;
+static void rk4_attempt(const Group &G) {
+    const ProbDesc *dd = G.d_desc;
+    const WorkItem *w = G.d_work;
     auto st = [&](int mode, int base, int kin, double a, int kout, int out, double hc, int cmp) {
-        launch_named("k_rk_stage", k_rk_stage<HET>, D.nwork, D.stream, dd, w, mode, base, kin, a, kout, out, hc, cmp);
+        launch_named("k_rk_stage", k_rk_stage<HET>, G.nwork, G.stream, dd, w, mode, base, kin, a, kout, out, hc,
+                     cmp);
     };
     const int Y = 100, YT = 101;
     st(ST_K, Y, -1, 0.0, B_K1, -1, 0.0, -1);                 // k1 = f(y), first attempt only
@@ -643,7 +702,7 @@ static void rk4_attempt(DevModel &D) {
     st(ST_FIRST, B_T1, B_K1, 0.25, B_KA, -1, 0.0, -1);       // second half step
     st(ST_MID, B_T1, B_KA, 0.25, B_KB, -1, 0.0, -1);
     st(ST_LAST_ERR, B_T1, B_KB, 0.5, -1, -1, 0.5, YT);
-    launch(k_rk_control, 1, D.stream, dd);
+    launch(k_rk_control, G.nprob, G.stream, dd);
 }
 
 template <bool HET>
@@ -651,7 +710,7 @@ static void transient_rk4(Model &M, DevModel &D, double dt) {
     write_scal(D, {{S_T, 0.0}, {S_DT, dt}, {S_H, kMinStep}, {S_STATUS, (double)RK_NEED_K1}, {S_CALLS, 0.0}});
     int batch = 4;
     for (int guard = 0;; guard++) {
-        for (int k = 0; k < batch; k++) rk4_attempt<HET>(D);
+        for (int k = 0; k < batch; k++) rk4_attempt<HET>(D.grp);
         read_scal(D);
         if (D.h_scal[S_STATUS] == (double)RK_DONE) break;
         batch = 2;
@@ -664,16 +723,16 @@ static void transient_rk4(Model &M, DevModel &D, double dt) {
 // bound for a 1000 K change per sub-step, and the device stops each series
 // early once its actual tail bound is below the tolerance.
 struct TaylorPlan {
-    int m, K;
-    double tau, tol;
+    int m = 1, K = 1;
+    double tau = 0.0, tol = 0.0;
 };
-static TaylorPlan taylor_plan(const Model &M, const DevModel &D, double dt) {
+static TaylorPlan taylor_plan(double rho, double dt, double taylor_tol) {
     TaylorPlan p;
     const double zmax = 4.0;
-    p.m = std::max(1, (int)std::ceil(D.rho * dt / zmax));
+    p.m = std::max(1, (int)std::ceil(rho * dt / zmax));
     p.tau = dt / p.m;
-    const double z = D.rho * p.tau;
-    p.tol = M.opt.taylor_tol / p.m;
+    const double z = rho * p.tau;
+    p.tol = taylor_tol / p.m;
     p.K = 1;
     double term = 1000.0;
     while (p.K < 60) {
@@ -685,30 +744,37 @@ static TaylorPlan taylor_plan(const Model &M, const DevModel &D, double dt) {
     return p;
 }
 
+// scalar slots of a problem's Taylor plan
+static std::vector<std::pair<int, double>> taylor_slots(const TaylorPlan &p, double rho) {
+    return {{S_TAU, p.tau}, {S_TK, (double)p.m}, {S_RHO, rho}, {S_TTOL, p.tol}, {S_KPLAN, (double)p.K}};
+}
+
+// One interval of every problem: m sub-steps of at most K terms (the largest
+// plan of the group; each problem stops at its own).
 template <typename TD, bool HET>
-static void taylor_interval(DevModel &D, const TaylorPlan &p) {
-    const ProbDesc *dd = D.d_desc;
-    const WorkItem *w = D.d_work;
+static void taylor_interval(const Group &G, int m, int K, int seq) {
+    const ProbDesc *dd = G.d_desc;
+    const WorkItem *w = G.d_work;
     const bool f32 = sizeof(TD) == 4;
     const int db0 = f32 ? F_D0 : B_KA, db1 = f32 ? F_D1 : B_KB, accb = f32 ? F_ACC : B_ACC;
-    for (int s = 0; s < p.m; s++) {
-        launch_named("k_tay_first", k_tay_first<TD, HET>, D.nwork, D.stream, dd, w, db0, accb, s);
+    for (int s = 0; s < m; s++) {
+        launch_named("k_tay_first", k_tay_first<TD, HET>, G.nwork, G.stream, dd, w, db0, accb, s, seq);
         FinArgs a{};
-        a.tol = p.tol;
         a.sub = s;
-        finish(D, 1, FIN_TAY, a);
-        for (int k = 1; k < p.K; k++)
-            launch_named("k_tay_term", k_tay_term<TD, HET>, D.nwork, D.stream, dd, w, (k & 1) ? db0 : db1,
-                         (k & 1) ? db1 : db0, accb, k, s);
-        launch_named("k_tay_final", k_tay_final<TD>, D.nwork, D.stream, dd, w, accb, s);
+        a.seq = seq;
+        finish(G, FIN_TAY, a);
+        for (int k = 1; k < K; k++)
+            launch_named("k_tay_term", k_tay_term<TD, HET>, G.nwork, G.stream, dd, w, (k & 1) ? db0 : db1,
+                         (k & 1) ? db1 : db0, accb, k, s, seq);
+        launch_named("k_tay_final", k_tay_final<TD>, G.nwork, G.stream, dd, w, accb, s, seq);
     }
 }
 
 template <typename TD, bool HET>
 static void transient_taylor(Model &M, DevModel &D, double dt) {
-    const TaylorPlan p = taylor_plan(M, D, dt);
-    write_scal(D, {{S_TAU, p.tau}, {S_TK, (double)p.m}, {S_RHO, D.rho}});
-    taylor_interval<TD, HET>(D, p);
+    const TaylorPlan p = taylor_plan(D.rho, dt, M.opt.taylor_tol);
+    write_scal(D, taylor_slots(p, D.rho));
+    taylor_interval<TD, HET>(D.grp, p.m, p.K, 0);
     M.stats.taylor_terms = p.K;
     M.stats.taylor_substeps = p.m;
 }
@@ -719,11 +785,11 @@ void dev_transient(Model &M, const double *block_power, const double *block_temp
     DevModel &D = *M.dev;
     HIP_OK(hipEventRecord(D.ev0, D.stream));
     upload_blocks(D, block_power);
-    run_b2g(D, V_POWER, B_RHS, 1);
+    run_b2g(D.grp, V_POWER, B_RHS, 1);
     if (block_temp_in) {
         HIP_OK(hipStreamSynchronize(D.stream));
         upload_blocks(D, block_temp_in);
-        run_b2g(D, V_TEMP, B_Y0, 0);
+        run_b2g(D.grp, V_TEMP, B_Y0, 0);
         write_scal(D, {{S_YPAR, 0.0}});
     }
     if (M.opt.transient == HS_TRANS_TAYLOR) {
@@ -740,7 +806,7 @@ void dev_transient(Model &M, const double *block_power, const double *block_temp
     }
     read_scal(D);
     const int ypar = (int)D.h_scal[S_YPAR];
-    run_g2b(D, B_Y0 + ypar);
+    run_g2b(D.grp, B_Y0 + ypar);
     HIP_OK(hipEventRecord(D.ev1, D.stream));
     download_blocks(D, block_temp_out);
     float ms = 0;
@@ -749,88 +815,87 @@ void dev_transient(Model &M, const double *block_power, const double *block_temp
 }
 
 // ---- device-resident sequences of intervals -----------------------------------
-static void ensure_seq(DevModel &D, int steps, std::vector<double *> &tmp) {
-    const size_t n = (size_t)steps * D.nblk;
-    double *in = nullptr, *out = nullptr;
-    HIP_OK(hipMalloc((void **)&in, n * sizeof(double)));
-    HIP_OK(hipMalloc((void **)&out, n * sizeof(double)));
-    tmp.push_back(in);
-    tmp.push_back(out);
-    D.h.seq_in = in;
-    D.h.seq_out = out;
-    HIP_OK(hipMemcpy(D.d_desc, &D.h, sizeof(ProbDesc), hipMemcpyHostToDevice));
-}
-
 // Captures one interval's launches once as a hipGraph and replays it for
 // every interval (the kernels take the interval index from device memory, so
-// the same graph serves all of them).  Kernel-timer events captured with it
-// become event nodes, read back every `stride` replays.
+// the same graph serves all of them).
 static bool graphs_enabled() {
     static const bool off = getenv("HS_NO_GRAPH") != nullptr;
     return !off && g_timer.cap == 0;
 }
 
-template <class F>
-static void replay(DevModel &D, int steps, F body) {
-    if (!graphs_enabled()) {   // plain launches: per-launch timing / profilers
-        for (int k = 0; k < steps; k++) body();
-        HIP_OK(hipStreamSynchronize(D.stream));
-        return;
-    }
-    const size_t ev0 = g_timer.ev.size();
-    HIP_OK(hipStreamBeginCapture(D.stream, hipStreamCaptureModeThreadLocal));
-    body();
+struct Replayer {
     hipGraph_t graph = nullptr;
-    HIP_OK(hipStreamEndCapture(D.stream, &graph));
     hipGraphExec_t exec = nullptr;
-    HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> cap(g_timer.ev.begin() + ev0, g_timer.ev.end());
-    g_timer.ev.resize(ev0);
-    const int stride = 25;
-    for (int k = 0; k < steps; k++) {
-        HIP_OK(hipGraphLaunch(exec, D.stream));
-        if (!cap.empty() && k % stride == stride - 1 && g_timer.graph_n < g_timer.cap) {
-            HIP_OK(hipStreamSynchronize(D.stream));
-            for (auto &e : cap) {
-                float ms = 0;
-                if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
-                    g_timer.graph_ms += ms;
-                    g_timer.graph_n++;
-                }
-            }
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~Replayer() { reset(); }
+    // runs body `steps` times on the stream: plain launches while a kernel
+    // timer or profiler wants them, else a graph captured on first use
+    template <class F> void run(hipStream_t s, int steps, F body) {
+        if (!graphs_enabled()) {
+            for (int k = 0; k < steps; k++) body();
+            return;
         }
+        if (!exec) {
+            HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            body();
+            HIP_OK(hipStreamEndCapture(s, &graph));
+            HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        }
+        for (int k = 0; k < steps; k++) HIP_OK(hipGraphLaunch(exec, s));
     }
-    HIP_OK(hipStreamSynchronize(D.stream));
-    for (auto &e : cap) {
-        (void)hipEventDestroy(e.first);
-        (void)hipEventDestroy(e.second);
-    }
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
-}
+};
 
+// RK4 interval of every problem of the group with `slots` attempts; an
+// interval that needs more continues in the next launch of the body.
 template <bool HET>
-static void rk4_sequence(Model &M, DevModel &D, int steps, double dt, int slots) {
-    const ProbDesc *dd = D.d_desc;
-    replay(D, steps, [&] {
-        run_b2g(D, V_POWER, B_RHS, 1, 1);
-        hipLaunchKernelGGL(k_rk_begin, dim3(1), dim3(64), 0, D.stream, dd, 1, dt);
-        for (int a = 0; a < slots; a++) rk4_attempt<HET>(D);
-        hipLaunchKernelGGL(k_rk_end, dim3(1), dim3(64), 0, D.stream, dd, 1);
-        run_g2b(D, -1, 1);
-        hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, D.stream, dd, 1);
-    });
+static void rk4_seq_body(const Group &G, int slots) {
+    run_b2g(G, V_POWER, B_RHS, 1, 1);
+    hipLaunchKernelGGL(k_rk_begin, dim3(small_grid(G.nprob)), dim3(64), 0, G.stream, (const ProbDesc *)G.d_desc,
+                       G.nprob);
+    for (int a = 0; a < slots; a++) rk4_attempt<HET>(G);
+    hipLaunchKernelGGL(k_rk_end, dim3(small_grid(G.nprob)), dim3(64), 0, G.stream, (const ProbDesc *)G.d_desc,
+                       G.nprob);
+    run_g2b(G, -1, 1);
+    hipLaunchKernelGGL(k_step_advance, dim3(small_grid(G.nprob)), dim3(64), 0, G.stream,
+                       (const ProbDesc *)G.d_desc, G.nprob);
 }
 
 template <typename TD, bool HET>
-static void taylor_sequence(DevModel &D, const TaylorPlan &p, int steps) {
-    const ProbDesc *dd = D.d_desc;
-    replay(D, steps, [&] {
-        run_b2g(D, V_POWER, B_RHS, 1, 1);
-        taylor_interval<TD, HET>(D, p);
-        run_g2b(D, -1, 1);
-        hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, D.stream, dd, 1);
-    });
+static void taylor_seq_body(const Group &G, int m, int K) {
+    run_b2g(G, V_POWER, B_RHS, 1, 1);
+    taylor_interval<TD, HET>(G, m, K, 1);
+    run_g2b(G, -1, 1);
+    hipLaunchKernelGGL(k_step_advance, dim3(small_grid(G.nprob)), dim3(64), 0, G.stream,
+                       (const ProbDesc *)G.d_desc, G.nprob);
+}
+
+// The interval body of a group for the given form (precision, integrator).
+struct SeqForm {
+    int precision, transient, m, K;
+    static constexpr int kSlots = 5;   // RK4 attempts per interval launch
+};
+static void seq_body(const Group &G, const SeqForm &f) {
+    if (f.transient == HS_TRANS_TAYLOR) {
+        const bool f32 = f.precision == HS_PREC_MIXED;
+        if (f32 && G.hetero) taylor_seq_body<float, true>(G, f.m, f.K);
+        else if (f32) taylor_seq_body<float, false>(G, f.m, f.K);
+        else if (G.hetero) taylor_seq_body<double, true>(G, f.m, f.K);
+        else taylor_seq_body<double, false>(G, f.m, f.K);
+    } else {
+        if (G.hetero) rk4_seq_body<true>(G, SeqForm::kSlots);
+        else rk4_seq_body<false>(G, SeqForm::kSlots);
+    }
+}
+
+// scalar slots that start a sequence of n intervals of length dt
+static std::vector<std::pair<int, double>> seq_slots(int n, double dt) {
+    return {{S_STEP, 0.0}, {S_NSTEPS, (double)n}, {S_TCALLS, 0.0}, {S_CALLS, 0.0},
+            {S_STATUS, (double)RK_DONE}, {S_DT, dt}};
 }
 
 void dev_transient_run(Model &M, const double *power_seq, int steps, double dt, const double *init_temp,
@@ -838,63 +903,58 @@ void dev_transient_run(Model &M, const double *power_seq, int steps, double dt, 
     dev_prepare(M);
     DevModel &D = *M.dev;
     if (!init_temp && !M.trans_valid) fatal("hs_transient_run: no transient state to continue from\n");
-    std::vector<double *> tmp;
-    ensure_seq(D, steps, tmp);
+    const size_t n = (size_t)steps * D.nblk;
+    double *seq_in = nullptr, *seq_out = nullptr;
+    HIP_OK(hipMalloc((void **)&seq_in, n * sizeof(double)));
+    struct Free {
+        std::vector<double *> p;
+        ~Free() {
+            for (double *x : p) (void)hipFree(x);
+        }
+    } tmp;
+    tmp.p.push_back(seq_in);
+    HIP_OK(hipMalloc((void **)&seq_out, n * sizeof(double)));
+    tmp.p.push_back(seq_out);
+    D.h.seq_in = seq_in;
+    D.h.seq_out = seq_out;
+    HIP_OK(hipMemcpy(D.d_desc, &D.h, sizeof(ProbDesc), hipMemcpyHostToDevice));
     HIP_OK(hipEventRecord(D.ev0, D.stream));
-    HIP_OK(hipMemcpyAsync(D.h.seq_in, power_seq, (size_t)steps * D.nblk * sizeof(double),
-                          hipMemcpyHostToDevice, D.stream));
+    HIP_OK(hipMemcpyAsync(seq_in, power_seq, n * sizeof(double), hipMemcpyHostToDevice, D.stream));
     if (init_temp) {
         upload_blocks(D, init_temp);
-        run_b2g(D, V_TEMP, B_Y0, 0);
+        run_b2g(D.grp, V_TEMP, B_Y0, 0);
         HIP_OK(hipStreamSynchronize(D.stream));
         write_scal(D, {{S_YPAR, 0.0}});
     }
-    // keep the start state: an RK4 interval that needs more attempts than
-    // launched is redone with host-side control
-    const int ypar0 = (read_scal(D), (int)D.h_scal[S_YPAR]);
-    HIP_OK(hipMemcpyAsync(D.h.b64[B_TMP], D.h.b64[B_Y0 + ypar0], D.vlen * sizeof(double),
-                          hipMemcpyDeviceToDevice, D.stream));
-    write_scal(D, {{S_STEP, 0.0}, {S_OVERFLOW, 0.0}, {S_TCALLS, 0.0}});
+    write_scal(D, seq_slots(steps, dt));
+    SeqForm f{M.opt.precision, M.opt.transient, 1, 1};
     if (M.opt.transient == HS_TRANS_TAYLOR) {
-        const TaylorPlan p = taylor_plan(M, D, dt);
-        write_scal(D, {{S_TAU, p.tau}, {S_TK, (double)p.m}, {S_RHO, D.rho}});
-        const bool f32 = M.opt.precision == HS_PREC_MIXED;
-        if (f32 && D.hetero) taylor_sequence<float, true>(D, p, steps);
-        else if (f32) taylor_sequence<float, false>(D, p, steps);
-        else if (D.hetero) taylor_sequence<double, true>(D, p, steps);
-        else taylor_sequence<double, false>(D, p, steps);
+        const TaylorPlan p = taylor_plan(D.rho, dt, M.opt.taylor_tol);
+        write_scal(D, taylor_slots(p, D.rho));
+        f.m = p.m;
+        f.K = p.K;
         M.stats.taylor_terms = p.K;
         M.stats.taylor_substeps = p.m;
-    } else {
-        const int slots = 5;
-        if (D.hetero) rk4_sequence<true>(M, D, steps, dt, slots);
-        else rk4_sequence<false>(M, D, steps, dt, slots);
-        read_scal(D);
-        if (D.h_scal[S_OVERFLOW] != 0.0) {
-            // restore the start state and integrate interval by interval
-            HIP_OK(hipMemcpyAsync(D.h.b64[B_Y0 + ypar0], D.h.b64[B_TMP], D.vlen * sizeof(double),
-                                  hipMemcpyDeviceToDevice, D.stream));
-            write_scal(D, {{S_YPAR, (double)ypar0}, {S_STEP, 0.0}, {S_TCALLS, 0.0}});
-            for (int k = 0; k < steps; k++) {
-                run_b2g(D, V_POWER, B_RHS, 1, 1);
-                if (D.hetero) transient_rk4<true>(M, D, dt);
-                else transient_rk4<false>(M, D, dt);
-                run_g2b(D, -1, 1);
-                hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, D.stream, (const ProbDesc *)D.d_desc, 1);
-            }
-        }
-        read_scal(D);
-        M.stats.rk4_calls = (int)D.h_scal[S_TCALLS];
     }
+    {
+        Replayer rp;
+        int todo = steps;
+        for (int guard = 0; todo > 0; guard++) {
+            rp.run(D.stream, todo, [&] { seq_body(D.grp, f); });
+            read_scal(D);
+            todo = steps - (int)D.h_scal[S_STEP];
+            if (guard > 1000000) fatal("transient sequence did not finish\n");
+        }
+        HIP_OK(hipStreamSynchronize(D.stream));
+    }
+    if (M.opt.transient == HS_TRANS_RK4) M.stats.rk4_calls = (int)D.h_scal[S_TCALLS];
     HIP_OK(hipEventRecord(D.ev1, D.stream));
     if (temp_seq)
-        HIP_OK(hipMemcpyAsync(temp_seq, D.h.seq_out, (size_t)steps * D.nblk * sizeof(double),
-                              hipMemcpyDeviceToHost, D.stream));
+        HIP_OK(hipMemcpyAsync(temp_seq, seq_out, n * sizeof(double), hipMemcpyDeviceToHost, D.stream));
     HIP_OK(hipStreamSynchronize(D.stream));
     float ms = 0;
     HIP_OK(hipEventElapsedTime(&ms, D.ev0, D.ev1));
     M.stats.last_solve_ms = ms;
-    for (double *p : tmp) (void)hipFree(p);
     D.h.seq_in = D.h.seq_out = nullptr;
     HIP_OK(hipMemcpy(D.d_desc, &D.h, sizeof(ProbDesc), hipMemcpyHostToDevice));
 }
@@ -930,7 +990,7 @@ void dev_slope(Model &M, const double *block_power, const double *grid_temp, dou
     std::vector<double> p(block_power, block_power + D.nblk);
     for (int k = 0; k < M.nx; k++) p[M.g->total_n_blocks + k] = 0.0;
     upload_blocks(D, p.data());
-    run_b2g(D, V_POWER, B_RHS, 1);
+    run_b2g(D.grp, V_POWER, B_RHS, 1);
     HIP_OK(hipMemcpyAsync(D.h.b64[B_TMP], grid_temp, M.vec_len() * sizeof(double), hipMemcpyHostToDevice,
                           D.stream));
     if (D.hetero)
@@ -947,7 +1007,7 @@ void dev_b2g(Model &M, const double *block_vec, double *grid_vec, int type) {
     dev_prepare(M);
     DevModel &D = *M.dev;
     upload_blocks(D, block_vec);
-    run_b2g(D, type, B_TMP, 0);
+    run_b2g(D.grp, type, B_TMP, 0);
     HIP_OK(hipMemcpyAsync(grid_vec, D.h.b64[B_TMP], M.vec_len() * sizeof(double), hipMemcpyDeviceToHost,
                           D.stream));
     HIP_OK(hipStreamSynchronize(D.stream));
@@ -958,17 +1018,13 @@ void dev_g2b(Model &M, const double *grid_vec, double *block_vec) {
     DevModel &D = *M.dev;
     HIP_OK(hipMemcpyAsync(D.h.b64[B_TMP], grid_vec, M.vec_len() * sizeof(double), hipMemcpyHostToDevice,
                           D.stream));
-    run_g2b(D, B_TMP);
+    run_g2b(D.grp, B_TMP);
     download_blocks(D, block_vec);
 }
 
 }  // namespace hs
 
-// ---- batched engine (declared in the C ABI) -----------------------------------
-struct hs_engine_st {
-    int precision = HS_PREC_FP64;
-    int transient = HS_TRANS_RK4;
-};
+#include "hs_batch.inl"
 
 extern "C" {
 void hs_kernel_timer(const char *name, int max_samples) {
@@ -1001,16 +1057,4 @@ int hs_kernel_timer_read(double *avg_ms, double *total_ms) {
     return n;
 }
 
-hs_engine_t *hs_engine_create(int precision, int transient) {
-    auto *e = new hs_engine_st();
-    e->precision = precision;
-    e->transient = transient;
-    return e;
-}
-void hs_engine_destroy(hs_engine_t *e) { delete e; }
-int hs_engine_submit(hs_engine_t *, RC_model_t *, const double *, const double *, int, double) { return -1; }
-int hs_engine_tick(hs_engine_t *) { return 0; }
-int hs_engine_poll(hs_engine_t *, int, double *) { return 0; }
-void hs_engine_set_budget(hs_engine_t *, double) {}
-int hs_engine_active(hs_engine_t *) { return 0; }
 }

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <strings.h>
 
@@ -1580,8 +1581,7 @@ void hs_set_options(RC_model_t *model, const hs_options_t *opt) {
     HS_BEGIN
     Model &M = *model_of(model);
     if (opt->precision != HS_PREC_FP64 && opt->precision != HS_PREC_MIXED) fatal("unknown precision\n");
-    if (opt->steady_solver != HS_STEADY_JPCG && opt->steady_solver != HS_STEADY_MGPCG)
-        fatal("unknown steady solver\n");
+    if (opt->steady_solver != HS_STEADY_JPCG) fatal("unknown steady solver\n");
     if (opt->transient != HS_TRANS_RK4 && opt->transient != HS_TRANS_TAYLOR)
         fatal("unknown transient integrator\n");
     if (!(opt->steady_rtol > 0) || opt->max_iters <= 0 || !(opt->taylor_tol > 0)) fatal("invalid solver tolerances\n");
@@ -1656,6 +1656,103 @@ void hs_grid_to_block(RC_model_t *model, const double *grid_vec, double *block_v
     if (!M.g->r_ready) fatal("R model not ready\n");
     dev_g2b(M, grid_vec, block_vec);
     HS_END()
+}
+
+
+// ---- batched engine ---------------------------------------------------------
+struct hs_engine_st {
+    hs::Engine *impl;
+};
+
+static hs::Engine &engine_of(hs_engine_t *e) {
+    if (!e || !e->impl) fatal("null engine\n");
+    return *e->impl;
+}
+
+hs_engine_t *hs_engine_create(int precision, int transient) {
+    HS_BEGIN
+    if (precision != HS_PREC_FP64 && precision != HS_PREC_MIXED) fatal("unknown precision\n");
+    if (transient != HS_TRANS_RK4 && transient != HS_TRANS_TAYLOR) fatal("unknown transient integrator\n");
+    std::unique_ptr<hs_engine_st> e(new hs_engine_st{nullptr});
+    e->impl = eng_create(precision, transient);
+    return e.release();
+    HS_END(nullptr)
+}
+
+void hs_engine_destroy(hs_engine_t *e) {
+    HS_BEGIN
+    if (!e) return;
+    eng_destroy(e->impl);
+    delete e;
+    HS_END()
+}
+
+void hs_engine_set_options(hs_engine_t *e, const hs_options_t *opt) {
+    HS_BEGIN
+    if (!opt) fatal("null options\n");
+    if (opt->precision != HS_PREC_FP64 && opt->precision != HS_PREC_MIXED) fatal("unknown precision\n");
+    if (opt->steady_solver != HS_STEADY_JPCG) fatal("unknown steady solver\n");
+    if (opt->transient != HS_TRANS_RK4 && opt->transient != HS_TRANS_TAYLOR) fatal("unknown transient integrator\n");
+    if (!(opt->steady_rtol > 0) || opt->max_iters <= 0 || !(opt->taylor_tol > 0)) fatal("invalid solver tolerances\n");
+    eng_set_options(engine_of(e), *opt);
+    HS_END()
+}
+
+void hs_engine_set_tick(hs_engine_t *e, int intervals, int steady_iters) {
+    HS_BEGIN
+    eng_set_tick(engine_of(e), intervals, steady_iters);
+    HS_END()
+}
+
+void hs_engine_set_budget(hs_engine_t *e, double bytes) {
+    HS_BEGIN
+    eng_set_budget(engine_of(e), bytes);
+    HS_END()
+}
+
+int hs_engine_submit(hs_engine_t *e, RC_model_t *model, const double *power, int power_rows,
+                     const double *init_temp, int n_steps, double dt) {
+    HS_BEGIN
+    return eng_submit(engine_of(e), *model_of(model), power, power_rows, init_temp, n_steps, dt);
+    HS_END(-1)
+}
+
+int hs_engine_tick(hs_engine_t *e) {
+    HS_BEGIN
+    return eng_tick(engine_of(e));
+    HS_END(-1)
+}
+
+int hs_engine_poll(hs_engine_t *e, int job, double *out) {
+    HS_BEGIN
+    return eng_poll(engine_of(e), job, out);
+    HS_END(-1)
+}
+
+int hs_engine_result_seq(hs_engine_t *e, int job, double *out) {
+    HS_BEGIN
+    return eng_result_seq(engine_of(e), job, out);
+    HS_END(-1)
+}
+
+int hs_engine_job_info(hs_engine_t *e, int job, hs_job_info_t *info) {
+    HS_BEGIN
+    if (!info) fatal("null job info\n");
+    eng_job_info(engine_of(e), job, info);
+    return 0;
+    HS_END(-1)
+}
+
+void hs_engine_release(hs_engine_t *e, int job) {
+    HS_BEGIN
+    eng_release(engine_of(e), job);
+    HS_END()
+}
+
+int hs_engine_active(hs_engine_t *e) {
+    HS_BEGIN
+    return eng_active(engine_of(e));
+    HS_END(-1)
 }
 
 }  // extern "C"

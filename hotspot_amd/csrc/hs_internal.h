@@ -112,4 +112,19 @@ void dev_slope(Model &M, const double *block_power, const double *grid_temp, dou
 void dev_b2g(Model &M, const double *block_vec, double *grid_vec, int type);
 void dev_g2b(Model &M, const double *grid_vec, double *block_vec);
 
+// batched engine (hs_batch.inl)
+struct Engine;
+Engine *eng_create(int precision, int transient);
+void eng_destroy(Engine *E);
+void eng_set_options(Engine &E, const hs_options_t &o);
+void eng_set_tick(Engine &E, int intervals, int iters);
+void eng_set_budget(Engine &E, double bytes);
+int eng_submit(Engine &E, Model &M, const double *power, int rows, const double *init, int nsteps, double dt);
+int eng_tick(Engine &E);
+int eng_poll(Engine &E, int id, double *out);
+int eng_result_seq(Engine &E, int id, double *out);
+void eng_job_info(Engine &E, int id, hs_job_info_t *info);
+void eng_release(Engine &E, int id);
+int eng_active(Engine &E);
+
 }  // namespace hs

@@ -50,11 +50,15 @@ enum Scal {
     S_TAU, S_TMAX, S_TDONE, S_TK,   // Taylor: step, term max, done flag, substeps
     S_NREFINE,
     S_STEP,                      // interval index of a device-resident sequence
-    S_OVERFLOW,                  // an interval needed more RK4 attempts than launched
+    S_TITERS,                    // mixed steady: inner iterations summed over refinements
     S_RHO,                       // Taylor: bound on ||C^-1 G||_inf
     S_TCALLS,                    // RK4 calls summed over a sequence
     S_KNEED,                     // Taylor: terms of the current sub-step
-    NSCAL = 32
+    S_NSTEPS,                    // intervals of a device-resident sequence
+    S_CORR,                      // mixed steady: a correction e is pending
+    S_TTOL,                      // Taylor: truncation tolerance per sub-step
+    S_KPLAN,                     // Taylor: most terms a sub-step may take
+    NSCAL = 48
 };
 enum RkStatus { RK_NEED_K1 = 0, RK_ATTEMPT = 1, RK_DONE = 2 };
 

@@ -280,8 +280,9 @@ void hs_set_fatal_mode(int exit_on_error);
 #define HS_PREC_FP64  0   /* every sweep in FP64 */
 #define HS_PREC_MIXED 1   /* FP32 sweeps inside FP64 refinement / FP64 residuals */
 /* steady solvers */
-#define HS_STEADY_JPCG  0 /* Jacobi-preconditioned CG (BiCGStab when advection makes G non-symmetric) */
-#define HS_STEADY_MGPCG 1 /* geometric multigrid (z-line smoother) preconditioned CG */
+#define HS_STEADY_JPCG  0 /* Jacobi-preconditioned CG (G symmetric positive definite; a
+                           * microchannel model's advection makes G non-symmetric: its
+                           * steady solve is refused with an error) */
 /* transient integrators */
 #define HS_TRANS_RK4    0 /* the reference's adaptive step-doubling RK4 */
 #define HS_TRANS_TAYLOR 1 /* exponential (Taylor) integrator of the linear ODE, error bounded a priori */
@@ -341,24 +342,56 @@ void hs_kernel_timer(const char *name, int max_samples);
 /* Number of timed launches; their mean and total duration in ms. */
 int hs_kernel_timer_read(double *avg_ms, double *total_ms);
 
-/* ---- batched engine: many models advanced by shared launches ------------- */
+/* ---- batched engine: many models advanced by shared launches -------------
+ * Jobs are admitted first come, first served while their device footprint
+ * fits the budget (default: 80 % of the free device memory at creation).
+ * Each tick runs a chunk of steady iterations for every job in its steady
+ * phase and a chunk of transient intervals for every job in its transient
+ * phase (one launch sequence per group, on two streams), then moves jobs
+ * between phases.  A job's result is the same as that of the same solve run
+ * alone.  The engine's options (hs_engine_set_options) apply to every job;
+ * a model must stay allocated and unchanged until its jobs are admitted. */
 typedef struct hs_engine_st hs_engine_t;
 hs_engine_t *hs_engine_create(int precision, int transient);
 void hs_engine_destroy(hs_engine_t *e);
-/* Submits a job on a populated model; power is a block vector (copied).
- * n_steps == 0: steady solve; n_steps > 0: that many compute_temp intervals of
- * dt each, starting from init_temp (a block vector, copied) or, when
- * init_temp is NULL, from the model's steady state solved first.
- * Returns a job id >= 0, or -1 when the job cannot be admitted. */
-int hs_engine_submit(hs_engine_t *e, RC_model_t *model, const double *power,
-                     const double *init_temp, int n_steps, double dt);
-/* Advances every active job by one scheduling tick (one batched launch
- * sequence); returns the number of jobs still active. */
-int hs_engine_tick(hs_engine_t *e);
-/* 1 when finished; copies the result block vector into out (may be NULL). */
-int hs_engine_poll(hs_engine_t *e, int job, double *out);
+/* only while no job is queued or running */
+void hs_engine_set_options(hs_engine_t *e, const hs_options_t *opt);
+/* transient intervals and steady iterations per tick (defaults 32, 128) */
+void hs_engine_set_tick(hs_engine_t *e, int intervals, int steady_iters);
 /* bytes of device memory the engine may use for resident problems */
 void hs_engine_set_budget(hs_engine_t *e, double bytes);
+/* Submits a job on a populated model.  power holds power_rows block vectors
+ * (total blocks + package nodes each; package-node entries are ignored):
+ * 1 (the same power throughout), n_steps (one per interval, the steady solve
+ * uses the first) or n_steps + 1 (the steady power, then one per interval).
+ * n_steps == 0: a steady solve (steady_state_temp); n_steps > 0: n_steps
+ * intervals of dt (compute_temp) from init_temp (a block vector) or, when
+ * init_temp is NULL, from the block temperatures of a steady solve run
+ * first.  Inputs are copied.  Returns the job id, or -1 on error (e.g. a
+ * footprint above the budget). */
+int hs_engine_submit(hs_engine_t *e, RC_model_t *model, const double *power, int power_rows,
+                     const double *init_temp, int n_steps, double dt);
+/* Admits what fits and advances every resident job by one tick; returns the
+ * number of jobs queued or running. */
+int hs_engine_tick(hs_engine_t *e);
+/* 1 when finished (the final block vector is copied into out, may be NULL),
+ * 0 while queued or running, -1 for an unknown job. */
+int hs_engine_poll(hs_engine_t *e, int job, double *out);
+/* the block temperatures of every interval (n_steps block vectors) of a
+ * finished transient job; 1 when copied, 0 while it runs, -1 on error */
+int hs_engine_result_seq(hs_engine_t *e, int job, double *out);
+typedef struct hs_job_info_st {
+    int state;                /* 0 queued, 1 steady phase, 2 transient phase, 3 done */
+    int steady_iters;         /* CG iterations (inner ones for mixed precision) */
+    int steady_refinements;
+    int rk4_calls;            /* RK4 step attempts over the whole sequence */
+    double wait_s;            /* submission -> admission */
+    double latency_s;         /* submission -> result (so far, while running) */
+    double device_bytes;      /* device memory held while resident */
+} hs_job_info_t;
+int hs_engine_job_info(hs_engine_t *e, int job, hs_job_info_t *info);
+/* frees a finished job's results */
+void hs_engine_release(hs_engine_t *e, int job);
 int hs_engine_active(hs_engine_t *e);
 
 #ifdef __cplusplus

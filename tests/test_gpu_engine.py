@@ -1,0 +1,240 @@
+"""The device-resident interval sequence (hs_transient_run) and the batched
+engine (hs_engine_*), through the C ABI.
+
+* a sequence equals the same intervals run one compute_temp call at a time;
+* a job's result in the engine is bit-for-bit the one it gets alone, whatever
+  it is batched with (problems of other sizes, layer counts, heterogeneous
+  coefficients, other phases), and equals the single-model API's result;
+* the engine's results meet the oracle's bar (FP64 steady within 1e-4 K of
+  the converged operator, RK4 intervals within 1e-8 K of the oracle's RK4);
+* admission respects the device-memory budget, first come first served.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from hotspot_amd import cases, engine, iface
+from helpers import bind_extensions, check_error, make, oracle_for, ptr, set_options
+
+pytestmark = pytest.mark.gpu
+
+DT = 3.333e-6
+
+
+@pytest.fixture(scope="module")
+def lib(prod, gpu):
+    L = bind_extensions(prod)
+    L.lib.hs_transient_run.argtypes = [iface.RCModelP, iface.c_double_p, C.c_int, C.c_double,
+                                       iface.c_double_p, iface.c_double_p]
+    L.lib.hs_transient_run.restype = None
+    return L
+
+
+def transient_run(lib, run, seq, init):
+    seq = np.ascontiguousarray(seq)
+    out = np.zeros_like(seq)
+    lib.lib.hs_transient_run(run.model, ptr(seq), seq.shape[0], DT, ptr(init) if init is not None else None,
+                             ptr(out))
+    check_error(lib)
+    return out
+
+
+def _detailed16():
+    c, pw = cases.stacked_case(16, seed=4, n_dies=2, blocks_per_die=8)
+    c.lcf[0].flp = [u + (1.5e6 + 1e5 * i, 0.01 * (1 + 0.3 * (i % 3))) for i, u in enumerate(c.lcf[0].flp)]
+    c.detailed_3D = True
+    return c, pw
+
+
+def job_specs():
+    """(name, case, block powers per power layer, intervals, start from
+    given temperatures?)"""
+    c32, p32 = cases.stacked_case(32, seed=2, blocks_per_die=16)
+    cdet, pdet = _detailed16()
+    specs = [("default-64", cases.default_stack_case(64), [cases.rng(1).uniform(0.5, 5, 16)], 12, False),
+             ("stack-32", c32, p32, 10, False),
+             ("detailed-16", cdet, pdet, 8, False),
+             ("steady-16", cases.default_stack_case(16), [cases.rng(3).uniform(0.5, 5, 16)], 0, False),
+             ("secondary-16-init", cases.default_stack_case(16, model_secondary=1),
+              [cases.rng(1).uniform(0.5, 5, 16)], 9, True)]
+    for k in range(6):   # small problems of other shapes
+        n = (16, 32)[k % 2]
+        c, p = cases.stacked_case(n, seed=10 + k, n_dies=1 + k % 3, blocks_per_die=(4, 8, 16)[k % 3])
+        specs.append((f"small-{k}", c, p, 3 + 2 * k if k != 2 else 0, k == 4))
+    return specs
+
+
+class Job:
+    def __init__(self, lib, spec):
+        self.name, case, pw, self.steps, self.from_init = spec
+        self.run = make(lib, case)
+        self.P = self.run.power_vector(pw)
+        self.seq = np.array(cases.jittered_powers(self.P, self.steps, seed=7)) if self.steps else None
+        self.init = np.full(self.run.n, 330.0) + cases.rng(5).uniform(0, 5, self.run.n) if self.from_init else None
+
+    def power_rows(self):
+        """the steady power first, then one row per interval"""
+        if not self.steps:
+            return self.P[None, :]
+        if self.from_init:
+            return self.seq
+        return np.vstack([self.P[None, :], self.seq])
+
+    def submit(self, eng):
+        return eng.submit(self.run.model, self.power_rows(), init=self.init, n_steps=self.steps, dt=DT)
+
+
+def run_engine(lib, jobs, form, budget=None, tick=(32, 128)):
+    """Runs the jobs to completion; returns their (final block vector,
+    sequence, info), the most jobs resident at once, the most device bytes
+    resident at once and the tick at which each job was admitted."""
+    eng = engine.Engine(lib, *form)
+    if budget is not None:
+        eng.set_budget(budget)
+    eng.set_tick(*tick)
+    ids = [j.submit(eng) for j in jobs]
+    resident_max, bytes_max = 0, 0.0
+    admit_tick = [None] * len(ids)
+    for t in range(10 ** 6):
+        active = eng.tick()
+        infos = [eng.info(i) for i in ids]
+        res = [x for x in infos if x.state in (engine.STEADY, engine.TRANSIENT)]
+        resident_max = max(resident_max, len(res))
+        bytes_max = max(bytes_max, sum(x.device_bytes for x in res))
+        for k, x in enumerate(infos):
+            if admit_tick[k] is None and x.state != engine.QUEUED:
+                admit_tick[k] = t
+        if active == 0:
+            break
+    out = []
+    for j, i in zip(jobs, ids):
+        r = eng.poll(i)
+        assert r is not None, j.name
+        seq = eng.result_seq(i) if j.steps else None
+        out.append((r, seq, eng.info(i)))
+        eng.release(i)
+    eng.close()
+    return out, resident_max, bytes_max, admit_tick
+
+
+FORMS = [(engine.FP64, engine.RK4), (engine.MIXED, engine.TAYLOR)]
+FORM_IDS = ["fp64-rk4", "mixed-taylor"]
+
+
+@pytest.fixture(scope="module")
+def jobs(lib):
+    js = [Job(lib, s) for s in job_specs()]
+    yield js
+    for j in js:
+        j.run.free()
+
+
+@pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
+def test_sequence_equals_interval_by_interval(lib, jobs, form):
+    for j in jobs[:3]:
+        set_options(lib, j.run, precision=form[0], transient=form[1])
+        T0 = j.run.steady(j.P.copy())
+        check_error(lib)
+        one = cases.run_transient(j.run, T0, list(j.seq), DT)
+        check_error(lib)
+        seq = transient_run(lib, j.run, j.seq, T0)
+        assert np.array_equal(one, seq), j.name
+        set_options(lib, j.run, precision=0, transient=0)
+
+
+@pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
+def test_engine_alone_equals_in_mix(lib, jobs, form):
+    mix, resident, _, _ = run_engine(lib, jobs, form)
+    assert resident >= 2
+    for j, (r, seq, info) in zip(jobs, mix):
+        alone = run_engine(lib, [j], form)[0]
+        assert np.array_equal(alone[0][0], r), j.name
+        if j.steps:
+            assert np.array_equal(alone[0][1], seq), j.name
+        assert info.latency_s >= info.wait_s >= 0
+
+
+@pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
+def test_engine_equals_single_model_api(lib, jobs, form):
+    """steady_state_temp, then hs_transient_run from its block temperatures
+    (or from the given ones), on the model itself."""
+    sel = [j for j in jobs if j.name in ("default-64", "detailed-16", "steady-16", "secondary-16-init")]
+    res = run_engine(lib, sel, form)[0]
+    for j, (r, seq, info) in zip(sel, res):
+        set_options(lib, j.run, precision=form[0], transient=form[1])
+        if j.from_init:
+            T0 = j.init
+        else:
+            T0 = j.run.steady(j.P.copy())
+            check_error(lib)
+        if not j.steps:
+            assert np.array_equal(T0, r), j.name
+        else:
+            s = transient_run(lib, j.run, j.seq, T0)
+            assert np.array_equal(s, seq), j.name
+            assert np.array_equal(s[-1], r), j.name
+        set_options(lib, j.run, precision=0, transient=0)
+
+
+def test_engine_fp64_matches_oracle(lib, jobs):
+    sel = [j for j in jobs if j.name in ("default-64", "stack-32", "steady-16", "secondary-16-init")]
+    res = run_engine(lib, sel, FORMS[0])[0]
+    for j, (r, seq, info) in zip(sel, res):
+        om = oracle_for(j.run)
+        if not j.steps:
+            Tb, _ = om.steady(j.P)
+            assert np.abs(r - Tb).max() <= 1e-4, j.name
+            assert info.steady_iters > 0
+            continue
+        if j.from_init:
+            Tg, tol = om.b2g(j.init, power=False), 1e-8
+        else:
+            # the device starts from its own FP64 steady state (1e-12 relative
+            # residual), the oracle from its direct solve
+            Tb, _ = om.steady(j.P)
+            Tg, tol = om.b2g(Tb, power=False), 1e-6
+        worst = 0.0
+        for k in range(j.steps):
+            Tg, _ = om.advance(Tg, om.power_grid(j.seq[k]), DT)
+            worst = max(worst, np.abs(seq[k] - om.g2b(Tg)).max())
+        assert worst <= tol, j.name
+        assert info.rk4_calls >= j.steps
+
+
+def test_engine_budget_admits_in_order(lib, jobs):
+    sel = jobs[:4]
+    eng = engine.Engine(lib, *FORMS[1])
+    ids = [j.submit(eng) for j in sel]
+    need = [eng.info(i).device_bytes for i in ids]
+    eng.close()
+    budget = 1.2 * max(need)
+    res, resident, bytes_max, admit_tick = run_engine(lib, sel, FORMS[1], budget=budget)
+    free = run_engine(lib, sel, FORMS[1])[0]
+    assert bytes_max <= budget
+    assert resident < len(sel) or sum(need) <= budget
+    assert admit_tick == sorted(admit_tick)
+    for (a, sa, _), (b, sb, _) in zip(res, free):
+        assert np.array_equal(a, b)
+        if sa is not None:
+            assert np.array_equal(sa, sb)
+
+
+def test_engine_refuses_what_it_cannot_do(lib, jobs):
+    j = jobs[0]
+    eng = engine.Engine(lib, *FORMS[0])
+    eng.set_budget(1e5)
+    with pytest.raises(RuntimeError, match="budget"):
+        j.submit(eng)
+    eng.set_budget(1e10)
+    with pytest.raises(RuntimeError, match="power rows"):
+        eng.submit(j.run.model, np.vstack([j.P, j.P]), n_steps=5, dt=DT)
+    mc = make(lib, cases.microchannel_case(16, 5, __import__("tempfile").mkdtemp())[0])
+    with pytest.raises(RuntimeError, match="microchannel"):
+        eng.submit(mc.model, np.zeros(mc.n), n_steps=0)
+    # a transient from given temperatures is fine for a microchannel model
+    i = eng.submit(mc.model, np.zeros(mc.n), init=np.full(mc.n, 320.0), n_steps=2, dt=DT)
+    eng.run()
+    assert eng.poll(i) is not None
+    eng.close()
+    mc.free()
