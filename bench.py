@@ -1,19 +1,23 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X grid thermal solver on its headline workload.
 
-Workload (one "step" = one whole job): a 3D-stacked chip on a 256 x 256 grid,
-4 silicon dies (0.1 mm, k = 100 W/mK) interleaved with 3 bond layers (10 um,
-k = 1.5 W/mK) over the default spreader and sink (9 layers, 589 824 cells),
-64 seeded slicing blocks per die with power density lognormal(ln 0.5 W/mm^2,
-0.6); a steady solve, then 10 000 transient intervals of 3.333 us with
-+-20 % seeded per-interval power jitter, starting from the steady state.
+Workload: jobs on a 3D-stacked chip on a 256 x 256 grid, 4 silicon dies
+(0.1 mm, k = 100 W/mK) interleaved with 3 bond layers (10 um, k = 1.5 W/mK)
+over the default spreader and sink (9 layers, 589 824 cells), 64 seeded
+slicing blocks per die with power density lognormal(ln 0.5 W/mm^2, 0.6);
+each job is a steady solve, then 10 000 transient intervals of 3.333 us with
++-20 % seeded per-interval power jitter, starting from the steady state's
+block temperatures.  One "step" serves a batch of such jobs (different seeds)
+through the batched engine, in the mixed-precision form (FP32 stencil sweeps
+inside FP64 refinement / FP64 residuals; Taylor transient with FP32 terms and
+FP64 state); inputs are admitted into device memory before the timed region.
 
-metric: delivered cell-updates/s = cells x (1 steady + intervals) / job time,
-for the mixed-precision form (FP32 stencil sweeps inside FP64 refinement /
-FP64 residuals, Taylor transient) with inputs resident on the device.  The
-plain FP64 GPU form (FP64 Jacobi-CG steady, the reference's adaptive RK4
-transient) runs the same job once; the JSON line carries its rate, the
-speedup and the largest block-temperature difference between the two forms.
+metric: delivered cell-updates/s = jobs x cells x (1 steady + intervals) /
+step time, with the p50/p99 job latency.  The same first job also runs alone
+through the single-model API (its result must equal the served one bit for
+bit) and in the plain FP64 GPU form (FP64 Jacobi-CG steady, the reference's
+adaptive RK4 transient); the JSON line carries their rates, the speedup and
+the largest block-temperature difference between the mixed and FP64 forms.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -32,7 +36,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from hotspot_amd import cases, iface  # noqa: E402
+from hotspot_amd import cases, engine, iface  # noqa: E402
 
 PROD_LIB = os.path.join(ROOT, "hotspot_amd", "libhotspot_amd.so")
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libhotspot_ref.so")
@@ -92,6 +96,7 @@ class Job:
     """The model, its block powers and the seeded power sequence."""
 
     def __init__(self, lib, n, intervals, seed=2):
+        self.rows = None
         self.lib = lib
         self.case, pw = cases.stacked_case(n, seed=seed)
         self.run = cases.Run(lib, self.case, tempfile.mkdtemp(prefix="hsbench"))
@@ -155,6 +160,32 @@ def cpu_baseline(n, intervals_sample):
                        f"solve (1 of {1 + 10000} cell-updates per cell) is not sampled")
 
 
+def served(lib, jobs, intervals, tick, keep_first=False, timer=None):
+    """One batch of jobs through the engine (mixed precision, Taylor): the
+    jobs are submitted and admitted (inputs copied into device memory) before
+    the timed region, which runs ticks until every result is back on the
+    host.  Returns (timed seconds, admission seconds, job infos, first job's
+    block-temperature sequence or None)."""
+    eng = engine.Engine(lib, engine.MIXED, engine.TAYLOR)
+    eng.set_tick(tick, 128)
+    ids = [eng.submit(j.run.model, j.rows, n_steps=intervals, dt=DT) for j in jobs]
+    ta = time.perf_counter()
+    if eng.admit() != 0:
+        raise RuntimeError("the batch does not fit the engine's device-memory budget")
+    device_sync()
+    t0 = time.perf_counter()
+    if timer:
+        lib.lib.hs_kernel_timer(timer, 200000)
+    eng.run()
+    t1 = time.perf_counter()
+    infos = [eng.info(i) for i in ids]
+    first = eng.result_seq(ids[0]) if keep_first else None
+    for i in ids:
+        eng.release(i)
+    eng.close()
+    return t1 - t0, t0 - ta, infos, first
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,9 +193,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--intervals", type=int, default=10000)
+    ap.add_argument("--batch", type=int, default=8, help="jobs served together per step")
+    ap.add_argument("--tick", type=int, default=100, help="transient intervals per engine tick")
     ap.add_argument("--cpu-intervals", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -176,55 +210,71 @@ def main():
     bind(lib)
     L = lib.lib
 
-    log(f"building {args.grid}x{args.grid} 3D stack, {args.intervals} intervals")
-    job = Job(lib, args.grid, args.intervals)
-    job.options(1, 1)    # mixed precision, Taylor transient
+    log(f"building {args.batch} jobs: {args.grid}x{args.grid} 3D stack, {args.intervals} intervals")
+    jobs = [Job(lib, args.grid, args.intervals, seed=2 + b) for b in range(args.batch)]
+    for j in jobs:
+        j.rows = np.ascontiguousarray(np.vstack([j.P[None, :], j.seq]))
+    cells = jobs[0].cells
+    updates_job = cells * (1 + args.intervals)
+
+    # ---- served form: the batch through the engine ----------------------------
     for w in range(args.warmup):
-        tw, _, _, _, _ = job.once()
-        log(f"warmup {w}: {tw:.3f} s")
-    device_sync()
-    times, steady_t, trans_t = [], [], []
+        tw, _, _, _ = served(lib, jobs, args.intervals, args.tick)
+        log(f"warmup {w}: batch of {args.batch} in {tw:.3f} s")
+    times, admit_t, lat = [], [], []
     for k in range(args.steps):
-        t, ts, tt, _, _ = job.once()
+        t, ta, infos, _ = served(lib, jobs, args.intervals, args.tick)
         times.append(t)
-        steady_t.append(ts)
-        trans_t.append(tt)
-        log(f"step {k}: {t:.3f} s (steady {ts*1e3:.1f} ms, transient {tt:.3f} s)")
-    device_sync()
-    # roofline pass: the same job with every launch of the dominant kernel
-    # bracketed by device events on the library's stream (plain launches
+        admit_t.append(ta)
+        lat += [i.latency_s for i in infos]
+        log(f"step {k}: batch of {args.batch} in {t:.3f} s (admission {ta*1e3:.0f} ms), "
+            f"steady iters {[i.steady_iters for i in infos][:4]}...")
+    t_step = float(np.mean(times))
+    value = args.batch * updates_job / t_step
+
+    # roofline pass: the same batch with every launch of the dominant kernel
+    # bracketed by device events on the engine's stream (plain launches
     # instead of graph replay while the timer is on)
-    L.hs_kernel_timer(b"k_tay_term", 20000)
-    t_roof, _, _, _, _ = job.once()
-    device_sync()
-    avg_ms, tot_ms = C.c_double(), C.c_double()
-    nt = L.hs_kernel_timer_read(C.byref(avg_ms), C.byref(tot_ms))
-    L.hs_kernel_timer(None, 0)
-    log(f"roofline pass {t_roof:.3f} s: {nt} launches of k_tay_term, mean {avg_ms.value*1e3:.2f} us")
-    st = job.stats()
-    cells = job.cells
-    updates = cells * (1 + args.intervals)
-    t_job = float(np.mean(times))
-    value = updates / t_job
-    nvec = cells + (iface.EXTRA)
-    bytes_per_launch = nvec * 16   # FP32: read term, read+write accumulator, write next term
-    achieved = bytes_per_launch / (avg_ms.value * 1e-3) / 1e9 if nt else None
+    achieved = avg_ms = nt = None
+    nvec = cells + iface.EXTRA
+    bytes_per_launch = args.batch * nvec * 16   # FP32: read term, read+write accumulator, write next term
+    if not args.no_roofline:
+        t_roof, _, _, _ = served(lib, jobs, args.intervals, args.tick, timer=b"k_tay_term")
+        device_sync()
+        a, tot = C.c_double(), C.c_double()
+        nt = L.hs_kernel_timer_read(C.byref(a), C.byref(tot))
+        L.hs_kernel_timer(None, 0)
+        avg_ms = a.value
+        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if nt else None
+        log(f"roofline pass {t_roof:.3f} s: {nt} launches of k_tay_term, mean {avg_ms*1e3:.2f} us, "
+            f"{achieved:.0f} GB/s")
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_k_tay_term_3dstack256.json")
+    tf = os.path.join(ROOT, "profiles", f"traffic_k_tay_term_3dstack{args.grid}_batch{args.batch}.json")
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
-    # mixed-precision result of one job, kept for the comparison with FP64
-    _, _, _, Ts_mixed, seq_mixed = job.once(keep_seq=True)
+    # ---- the same first job alone through the single-model API ------------------
+    j0 = jobs[0]
+    j0.options(1, 1)
+    j0.once()
+    t1s, ts1, tt1, Ts_mixed, seq_mixed = j0.once(keep_seq=True)
+    st = j0.stats()
+    log(f"single mixed job {t1s:.3f} s (steady {ts1*1e3:.1f} ms, transient {tt1:.3f} s)")
+    single = dict(value=updates_job / t1s, job_s=t1s, steady_ms=ts1 * 1e3, transient_s=tt1,
+                  steady_inner_iters=st.steady_iters, steady_refinements=st.steady_refinements,
+                  taylor_terms_max=st.taylor_terms, taylor_substeps=st.taylor_substeps)
+    # the served result of the same job equals it bit for bit
+    _, _, _, served_seq = served(lib, jobs[:1], args.intervals, args.tick, keep_first=True)
+    single["served_equals_single"] = bool(np.array_equal(served_seq, seq_mixed))
 
     fp64 = None
     if not args.no_fp64:
-        job.options(0, 0)    # FP64 Jacobi-CG + the reference's adaptive RK4
-        t64, ts64, tt64, Ts64, seq64 = job.once(keep_seq=True)
-        st64 = job.stats()
+        j0.options(0, 0)    # FP64 Jacobi-CG + the reference's adaptive RK4
+        t64, ts64, tt64, Ts64, seq64 = j0.once(keep_seq=True)
+        st64 = j0.stats()
         err = float(max(np.abs(seq64 - seq_mixed).max(), np.abs(Ts64 - Ts_mixed).max()))
-        fp64 = dict(value=updates / t64, job_s=t64, steady_ms=ts64 * 1e3, transient_s=tt64,
+        fp64 = dict(value=updates_job / t64, job_s=t64, steady_ms=ts64 * 1e3, transient_s=tt64,
                     rk4_calls_per_interval=st64.rk4_calls / args.intervals,
                     max_abs_diff_K_vs_mixed=err)
         log(f"fp64 job {t64:.3f} s (steady {ts64*1e3:.1f} ms), max |dT| mixed vs fp64 {err:.2e} K")
@@ -241,30 +291,30 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": t_job * 1e3,
+        "ms_per_step": t_step * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32-sweeps/f64-residual",
         "data": "synthetic (seeded floorplans and powers)",
-        "config": {"workload": f"3D stack {args.grid}x{args.grid}x9 (4 dies + 3 bonds + spreader + sink), "
-                               f"256 blocks, steady + {args.intervals} x 3.333us intervals",
-                   "cells": cells, "intervals": args.intervals, "precision": "mixed",
+        "config": {"workload": f"{args.batch} jobs served together, each a 3D stack {args.grid}x{args.grid}x9 "
+                               f"(4 dies + 3 bonds + spreader + sink, 256 blocks): steady + "
+                               f"{args.intervals} x 3.333us intervals",
+                   "cells_per_job": cells, "intervals": args.intervals, "batch": args.batch,
+                   "tick_intervals": args.tick, "precision": "mixed",
                    "steady": "JPCG FP32 inside FP64 refinement", "transient": "Taylor (exponential) FP32/FP64"},
-        "latency_ms": {"job_p50": float(np.percentile(times, 50)) * 1e3,
-                       "job_p99": float(np.percentile(times, 99)) * 1e3,
-                       "steady_p50": float(np.percentile(steady_t, 50)) * 1e3,
-                       "steady_p99": float(np.percentile(steady_t, 99)) * 1e3,
-                       "interval_mean": float(np.mean(trans_t)) / args.intervals * 1e3},
-        "roofline": {"bound": "hbm", "kernel": "k_tay_term<float>",
+        "latency_ms": {"job_p50": float(np.percentile(lat, 50)) * 1e3,
+                       "job_p99": float(np.percentile(lat, 99)) * 1e3,
+                       "note": "submission -> result on the host, admission included"},
+        "admission_ms": float(np.mean(admit_t)) * 1e3,
+        "value_with_admission": args.batch * updates_job / (t_step + float(np.mean(admit_t))),
+        "roofline": {"bound": "hbm", "kernel": "k_tay_term<float,false>",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic, "launches_timed": nt,
-                     "avg_launch_us": avg_ms.value * 1e3,
-                     "bytes_per_launch": bytes_per_launch,
-                     "note": "working set per vector 2.4 MB (FP32): cache-resident"},
-        "solver": {"steady_iters": st.steady_iters, "taylor_terms_max": st.taylor_terms,
-                   "taylor_substeps": st.taylor_substeps},
+                     "avg_launch_us": avg_ms * 1e3 if avg_ms else None,
+                     "bytes_per_launch": bytes_per_launch},
+        "single_job": single,
         "fp64_gpu": fp64,
         "speedup_vs_fp64_gpu": (value / fp64["value"]) if fp64 else None,
         "cpu_baseline": cpu,

@@ -365,14 +365,19 @@ int eng_submit(Engine &E, Model &M, const double *power, int rows, const double 
     return id;
 }
 
-int eng_tick(Engine &E) {
-    // admission: first come, first served while the footprint fits
+// admission: first come, first served while the footprint fits
+int eng_admit(Engine &E) {
     while (!E.queue.empty()) {
         EngJob &j = *E.queue.front();
         if (E.used + j.footprint > E.budget && E.used > 0) break;
         E.queue.pop_front();
         admit(E, j);
     }
+    return (int)E.queue.size();
+}
+
+int eng_tick(Engine &E) {
+    eng_admit(E);
     // members launched this tick per group: only their gathered flags are
     // fresh (a group may gain members below, or be empty this tick)
     int ran_steady[2] = {0, 0}, ran_trans[2] = {0, 0};

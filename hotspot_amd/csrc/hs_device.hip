@@ -115,22 +115,56 @@ struct Cell {
     bool active;
 };
 
-__device__ __forceinline__ Cell tile_cell(const ProbDesc &d, int tile) {
+// A tile is TX columns x (TY * rpt) rows of one layer; thread (tx, ty) owns
+// the rpt consecutive rows starting at ty * rpt of its column.  Cells beyond
+// the grid or beyond rpt are inactive; their index is clamped to a valid cell
+// so that loads stay in bounds.
+__device__ __forceinline__ Cell tile_cell(const ProbDesc &d, int tile, int r) {
     const int tpl = d.nti * d.ntj;
     const int n = tile / tpl;
     const int rem = tile - n * tpl;
     const int ti = rem / d.ntj, tj = rem - ti * d.ntj;
     const int sh = d.tx_shift;
     const int tx = threadIdx.x & ((1 << sh) - 1), ty = threadIdx.x >> sh;
-    const int i = ti * (kBlock >> sh) + ty, j = tj * (1 << sh) + tx;
-    Cell r;
-    r.n = n;
-    r.i = i;
-    r.j = j;
-    r.active = i < d.nr && j < d.nc;
-    r.c = ((long)n * d.nr + i) * d.nc + j;
-    return r;
+    const int i = ti * ((kBlock >> sh) * d.rpt) + ty * d.rpt + r, j = (tj << sh) + tx;
+    Cell q;
+    q.active = r < d.rpt && i < d.nr && j < d.nc;
+    q.n = n;
+    q.i = min(i, d.nr - 1);
+    q.j = min(j, d.nc - 1);
+    q.c = ((long)n * d.nr + q.i) * d.nc + q.j;
+    return q;
 }
+
+__device__ __forceinline__ int tile_layer(const ProbDesc &d, int tile) { return tile / (d.nti * d.ntj); }
+
+// Runs a kernel body over the cells a thread owns in a tile, RM cells at a
+// time: load(q) performs every global read of a cell and returns the values,
+// fin(q, v) does the arithmetic and the stores.  All reads of a group are
+// issued before its first store, so each thread keeps RM cells' reads in
+// flight instead of waiting on one cell at a time.
+template <int RM, class Load, class Fin>
+__device__ __forceinline__ void tile_loop(const ProbDesc &d, int tile, Load load, Fin fin) {
+    using V = decltype(load(Cell{}));
+    for (int r0 = 0; r0 < d.rpt; r0 += RM) {
+        Cell q[RM];
+        V v[RM];
+#pragma unroll
+        for (int k = 0; k < RM; k++) {
+            q[k] = tile_cell(d, tile, r0 + k);
+            if (r0 + k < d.rpt) v[k] = load(q[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < RM; k++)
+            if (q[k].active) fin(q[k], v[k]);
+    }
+}
+
+// cells in flight per thread: FP32 homogeneous 8, FP64 or heterogeneous 4,
+// FP64 heterogeneous 2 (register budget)
+template <typename T, bool HET> struct InFlight {
+    static constexpr int value = (sizeof(T) == 4 ? 8 : 4) / (HET ? 2 : 1);
+};
 
 template <typename T> struct Coef;
 template <> struct Coef<double> {
@@ -176,47 +210,103 @@ template <typename T> struct LdDir {   // CG direction p = z + beta * p_old
     __device__ __forceinline__ T operator()(long c) const { return beta == T(0) ? z[c] : z[c] + beta * p[c]; }
 };
 
-// Conduction (and advection) into cell c, the diagonal of G at c and the
-// cell's ambient conductance.
+// The values around a cell that the operator reads: the state at the cell
+// and its six neighbours (the cell itself where a neighbour does not exist)
+// and, for heterogeneous models, the face conductances and advection
+// coefficients.
+template <typename T, bool HET> struct Pts {
+    T c, n, s, e, w, u, dn;
+    T gn, gs, ge, gw, gu, gd;
+    T an, as, ae, aw, self;
+};
+
 template <typename T, bool HET, class L>
-__device__ __forceinline__ void cell_terms(const ProbDesc &d, const L &ld, const Cell &q, T xc,
-                                           T &flux, T &diag, T &gam) {
+__device__ __forceinline__ void load_pts(const ProbDesc &d, const L &ld, const Cell &q, Pts<T, HET> &p) {
     const int nr = d.nr, nc = d.nc, nl = d.nl, n = q.n, i = q.i, j = q.j;
-    const long c = q.c, plane = (long)nr * nc, ncell = plane * nl;
-    flux = T(0);
-    diag = T(0);
-    if (!HET) {
-        const T gx = (T)d.gx[n], gy = (T)d.gy[n];
-        if (i > 0) { flux += gx * (ld(c - nc) - xc); diag += gx; }
-        if (i < nr - 1) { flux += gx * (ld(c + nc) - xc); diag += gx; }
-        if (j < nc - 1) { flux += gy * (ld(c + 1) - xc); diag += gy; }
-        if (j > 0) { flux += gy * (ld(c - 1) - xc); diag += gy; }
-        if (n > 0) { const T g = (T)d.gz[n - 1]; flux += g * (ld(c - plane) - xc); diag += g; }
-        if (n < nl - 1) { const T g = (T)d.gz[n]; flux += g * (ld(c + plane) - xc); diag += g; }
-    } else {
+    const long c = q.c, plane = (long)nr * nc;
+    const long cn = i > 0 ? c - nc : c, cs = i < nr - 1 ? c + nc : c;
+    const long ce = j < nc - 1 ? c + 1 : c, cw = j > 0 ? c - 1 : c;
+    const long cu = n > 0 ? c - plane : c, cd = n < nl - 1 ? c + plane : c;
+    p.c = ld(c);
+    p.n = ld(cn);
+    p.s = ld(cs);
+    p.e = ld(ce);
+    p.w = ld(cw);
+    p.u = ld(cu);
+    p.dn = ld(cd);
+    if (HET) {
         const T *gE = Coef<T>::E(d), *gS = Coef<T>::S(d), *gD = Coef<T>::D(d);
-        const T xN = i > 0 ? ld(c - nc) : T(0), xS = i < nr - 1 ? ld(c + nc) : T(0);
-        const T xE = j < nc - 1 ? ld(c + 1) : T(0), xW = j > 0 ? ld(c - 1) : T(0);
-        if (i > 0) { const T g = gS[c - nc]; flux += g * (xN - xc); diag += g; }
-        if (i < nr - 1) { const T g = gS[c]; flux += g * (xS - xc); diag += g; }
-        if (j < nc - 1) { const T g = gE[c]; flux += g * (xE - xc); diag += g; }
-        if (j > 0) { const T g = gE[c - 1]; flux += g * (xW - xc); diag += g; }
-        if (n > 0) { const T g = gD[c - plane]; flux += g * (ld(c - plane) - xc); diag += g; }
-        if (n < nl - 1) { const T g = gD[c]; flux += g * (ld(c + plane) - xc); diag += g; }
+        p.gn = gS[cn];
+        p.gs = gS[c];
+        p.ge = gE[c];
+        p.gw = gE[cw];
+        p.gu = gD[cu];
+        p.gd = gD[c];
         if (d.has_adv) {
-            // coolant advection Cv Q(nbr->c) (T_nbr + T_c)/2 and the outlet sink
-            // (temperature_grid.c:3256-3590); a* hold Cv Q / 2 per face
-            const T an = Coef<T>::aN(d)[c], as = Coef<T>::aS(d)[c];
-            const T ae = Coef<T>::aE(d)[c], aw = Coef<T>::aW(d)[c], self = Coef<T>::aSelf(d)[c];
-            flux += an * (xN + xc) + as * (xS + xc) + ae * (xE + xc) + aw * (xW + xc) + self * xc;
-            diag -= an + as + ae + aw + self;
+            p.an = Coef<T>::aN(d)[c];
+            p.as = Coef<T>::aS(d)[c];
+            p.ae = Coef<T>::aE(d)[c];
+            p.aw = Coef<T>::aW(d)[c];
+            p.self = Coef<T>::aSelf(d)[c];
         }
     }
+}
+
+// The state of the package nodes a layer's edges couple to (N, S, E, W),
+// read once per tile (every cell of a tile is in one layer).
+template <typename T, class L>
+__device__ __forceinline__ void load_ring(const ProbDesc &d, const L &ld, int n, T (&ring)[4]) {
+    const long ncell = (long)d.nl * d.nr * d.nc;
+#pragma unroll
+    for (int e = 0; e < 4; e++) ring[e] = ld(ncell + max(d.ring_node[n][e], 0));
+}
+
+// Conduction (and advection) into the cell, the diagonal of G at the cell
+// and its ambient conductance, from the loaded values.  A missing neighbour
+// has zero conductance and reads the cell itself, so it adds exactly zero.
+template <typename T, bool HET>
+__device__ __forceinline__ void stencil(const ProbDesc &d, const Cell &q, const Pts<T, HET> &p,
+                                        const T (&ring)[4], T &flux, T &diag, T &gam) {
+    const int nr = d.nr, nc = d.nc, nl = d.nl, n = q.n, i = q.i, j = q.j;
+    const bool hn = i > 0, hs = i < nr - 1, he = j < nc - 1, hw = j > 0, hu = n > 0, hd = n < nl - 1;
+    const T xc = p.c;
+    T gn, gs, ge, gw, gu, gd;
+    if (!HET) {
+        const T gx = (T)d.gx[n], gy = (T)d.gy[n];
+        gn = hn ? gx : T(0);
+        gs = hs ? gx : T(0);
+        ge = he ? gy : T(0);
+        gw = hw ? gy : T(0);
+        gu = hu ? (T)d.gz[max(n - 1, 0)] : T(0);
+        gd = hd ? (T)d.gz[n] : T(0);
+    } else {
+        gn = hn ? p.gn : T(0);
+        gs = hs ? p.gs : T(0);
+        ge = he ? p.ge : T(0);
+        gw = hw ? p.gw : T(0);
+        gu = hu ? p.gu : T(0);
+        gd = hd ? p.gd : T(0);
+    }
+    flux = gn * (p.n - xc);
+    flux += gs * (p.s - xc);
+    flux += ge * (p.e - xc);
+    flux += gw * (p.w - xc);
+    flux += gu * (p.u - xc);
+    flux += gd * (p.dn - xc);
+    diag = gn + gs + ge + gw + gu + gd;
+    if (HET && d.has_adv) {
+        // coolant advection Cv Q(nbr->c) (T_nbr + T_c)/2 and the outlet sink
+        // (temperature_grid.c:3256-3590); a* hold Cv Q / 2 per face
+        const T xn = hn ? p.n : T(0), xs = hs ? p.s : T(0), xe = he ? p.e : T(0), xw = hw ? p.w : T(0);
+        flux += p.an * (xn + xc) + p.as * (xs + xc) + p.ae * (xe + xc) + p.aw * (xw + xc) + p.self * xc;
+        diag -= p.an + p.as + p.ae + p.aw + p.self;
+    }
     const int *rn = d.ring_node[n];
-    if (i == 0 && rn[0] >= 0) { const T g = (T)d.ring_gns[n]; flux += g * (ld(ncell + rn[0]) - xc); diag += g; }
-    if (i == nr - 1 && rn[1] >= 0) { const T g = (T)d.ring_gns[n]; flux += g * (ld(ncell + rn[1]) - xc); diag += g; }
-    if (j == nc - 1 && rn[2] >= 0) { const T g = (T)d.ring_gew[n]; flux += g * (ld(ncell + rn[2]) - xc); diag += g; }
-    if (j == 0 && rn[3] >= 0) { const T g = (T)d.ring_gew[n]; flux += g * (ld(ncell + rn[3]) - xc); diag += g; }
+    const T rgn = (T)d.ring_gns[n], rge = (T)d.ring_gew[n];
+    if (i == 0 && rn[0] >= 0) { flux += rgn * (ring[0] - xc); diag += rgn; }
+    if (i == nr - 1 && rn[1] >= 0) { flux += rgn * (ring[1] - xc); diag += rgn; }
+    if (j == nc - 1 && rn[2] >= 0) { flux += rge * (ring[2] - xc); diag += rge; }
+    if (j == 0 && rn[3] >= 0) { flux += rge * (ring[3] - xc); diag += rge; }
     gam = (T)d.gamb[n];
     diag += gam;
 }
@@ -240,11 +330,6 @@ __device__ __forceinline__ T cell_dinv(const ProbDesc &d, const Cell &q) {
     if (j == nc - 1 && rn[2] >= 0) diag += (T)d.ring_gew[n];
     if (j == 0 && rn[3] >= 0) diag += (T)d.ring_gew[n];
     return T(1) / diag;
-}
-
-template <typename T, bool HET>
-__device__ __forceinline__ T cell_cinv(const ProbDesc &d, const Cell &q) {
-    return HET ? Coef<T>::cinv(d)[q.c] : (T)d.cinv[q.n];
 }
 
 // Package nodes: flux into node k from its ring of boundary cells and from
@@ -321,27 +406,29 @@ __global__ void __launch_bounds__(kBlock) k_b2g(const ProbDesc *__restrict__ D,
                       : from_seq == 2 ? d.blk_out : d.blk_in;
     const long ncell = (long)d.nl * d.nr * d.nc;
     if (wi.tile < d.ntiles) {
-        const Cell q = tile_cell(d, wi.tile);
-        if (!q.active) return;
-        const long cc = (long)q.i * d.nc + q.j;
-        const int *ptr = d.map_ptr[q.n];
-        const int *unit = d.map_unit[q.n];
-        const double *occ = d.map_occ[q.n];
-        const double *vb = b + d.layer_base[q.n];
-        double val = 0.0;
-        if (type == V_POWER) {
-            // power density of each block times its occupancy, times the cell area
-            const double *area = d.unit_area[q.n];
-            for (int e = ptr[cc]; e < ptr[cc + 1]; e++) val += (occ[e] * vb[unit[e]]) / area[unit[e]];
-            val *= d.cell_area;
-        } else {
-            for (int e = ptr[cc]; e < ptr[cc + 1]; e++) val += occ[e] * vb[unit[e]];
-        }
-        if (add_sources) {
-            val += d.gamb[q.n] * d.ambient;
-            if (d.bconst) val += d.bconst[q.c];
-        }
-        out[q.c] = val;
+        const int n = tile_layer(d, wi.tile);
+        const int *ptr = d.map_ptr[n];
+        const int *unit = d.map_unit[n];
+        const double *occ = d.map_occ[n];
+        const double *vb = b + d.layer_base[n];
+        const double *area = d.unit_area[n];
+        const double src = d.gamb[n] * d.ambient;
+        tile_loop<1>(d, wi.tile, [&](const Cell &) { return 0; }, [&](const Cell &q, int) {
+            const long cc = (long)q.i * d.nc + q.j;
+            double val = 0.0;
+            if (type == V_POWER) {
+                // power density of each block times its occupancy, times the cell area
+                for (int e = ptr[cc]; e < ptr[cc + 1]; e++) val += (occ[e] * vb[unit[e]]) / area[unit[e]];
+                val *= d.cell_area;
+            } else {
+                for (int e = ptr[cc]; e < ptr[cc + 1]; e++) val += occ[e] * vb[unit[e]];
+            }
+            if (add_sources) {
+                val += src;
+                if (d.bconst) val += d.bconst[q.c];
+            }
+            out[q.c] = val;
+        });
     } else {
         const int k = threadIdx.x;
         if (k < d.nx) {
@@ -421,19 +508,31 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const ProbDesc *__restrict__
     LdPlain<T> ld{x};
     double v[3] = {0, 0, 0};
     if (wi.tile < d.ntiles) {
-        const Cell q = tile_cell(d, wi.tile);
-        if (q.active) {
-            const T xc = x[q.c];
-            T flux, diag, gam;
-            cell_terms<T, HET>(d, ld, q, xc, flux, diag, gam);
-            const T rr = rhs[q.c] + flux - gam * xc;
-            const T zz = rr / diag;
-            r[q.c] = rr;
-            z[q.c] = zz;
-            v[0] = (double)rr * zz;
-            v[1] = (double)rr * rr;
-            v[2] = (double)rhs[q.c] * rhs[q.c];
-        }
+        T ring[4];
+        load_ring<T>(d, ld, tile_layer(d, wi.tile), ring);
+        struct V {
+            Pts<T, HET> p;
+            T b;
+        };
+        tile_loop<InFlight<T, HET>::value>(
+            d, wi.tile,
+            [&](const Cell &q) {
+                V a{};
+                load_pts<T, HET>(d, ld, q, a.p);
+                a.b = rhs[q.c];
+                return a;
+            },
+            [&](const Cell &q, const V &a) {
+                T flux, diag, gam;
+                stencil<T, HET>(d, q, a.p, ring, flux, diag, gam);
+                const T rr = a.b + flux - gam * a.p.c;
+                const T zz = rr / diag;
+                r[q.c] = rr;
+                z[q.c] = zz;
+                v[0] += (double)rr * zz;
+                v[1] += (double)rr * rr;
+                v[2] += (double)a.b * a.b;
+            });
     } else {
         __shared__ T xf[kMaxExtra], xd[kMaxExtra];
         package_terms<T>(d, ld, false, xf, xd);
@@ -469,16 +568,23 @@ __global__ void __launch_bounds__(kBlock) k_cg_dir(const ProbDesc *__restrict__ 
     LdDir<T> ld{z, pold, (T)d.scal[S_BETA]};
     double v[1] = {0};
     if (wi.tile < d.ntiles) {
-        const Cell q = tile_cell(d, wi.tile);
-        if (q.active) {
-            const T pc = ld(q.c);
-            T flux, diag, gam;
-            cell_terms<T, HET>(d, ld, q, pc, flux, diag, gam);
-            const T Gp = gam * pc - flux;
-            pnew[q.c] = pc;
-            qv[q.c] = Gp;
-            v[0] = (double)pc * Gp;
-        }
+        T ring[4];
+        load_ring<T>(d, ld, tile_layer(d, wi.tile), ring);
+        tile_loop<InFlight<T, HET>::value>(
+            d, wi.tile,
+            [&](const Cell &q) {
+                Pts<T, HET> p{};
+                load_pts<T, HET>(d, ld, q, p);
+                return p;
+            },
+            [&](const Cell &q, const Pts<T, HET> &p) {
+                T flux, diag, gam;
+                stencil<T, HET>(d, q, p, ring, flux, diag, gam);
+                const T Gp = gam * p.c - flux;
+                pnew[q.c] = p.c;
+                qv[q.c] = Gp;
+                v[0] += (double)p.c * Gp;
+            });
     } else {
         __shared__ T xf[kMaxExtra], xd[kMaxExtra];
         package_terms<T>(d, ld, false, xf, xd);
@@ -511,18 +617,31 @@ __global__ void __launch_bounds__(kBlock) k_cg_upd(const ProbDesc *__restrict__ 
     const T *p = buf(p0b + 1 - par), *qv = buf(qb);
     const T alpha = (T)d.scal[S_ALPHA];
     double v[2] = {0, 0};
-    auto upd = [&](long c, T dinv) {
-        x[c] += alpha * p[c];
-        const T rr = r[c] - alpha * qv[c];
+    auto upd = [&](long c, T xc, T pc, T rc, T qc, T dinv) {
+        x[c] = xc + alpha * pc;
+        const T rr = rc - alpha * qc;
         const T zz = rr * dinv;
         r[c] = rr;
         z[c] = zz;
-        v[0] = (double)rr * zz;
-        v[1] = (double)rr * rr;
+        v[0] += (double)rr * zz;
+        v[1] += (double)rr * rr;
     };
     if (wi.tile < d.ntiles) {
-        const Cell q = tile_cell(d, wi.tile);
-        if (q.active) upd(q.c, cell_dinv<T, HET>(d, q));
+        struct V {
+            T x, p, r, q, di;
+        };
+        tile_loop<8>(
+            d, wi.tile,
+            [&](const Cell &q) {
+                V a{};
+                a.x = x[q.c];
+                a.p = p[q.c];
+                a.r = r[q.c];
+                a.q = qv[q.c];
+                if (HET) a.di = Coef<T>::dinv(d)[q.c];
+                return a;
+            },
+            [&](const Cell &q, const V &a) { upd(q.c, a.x, a.p, a.r, a.q, HET ? a.di : cell_dinv<T, false>(d, q)); });
     } else {
         if (threadIdx.x < d.nx) {
             // diagonal of a package node: ring and coupling conductances + ambient
@@ -531,7 +650,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_upd(const ProbDesc *__restrict__ 
             T diag = (T)d.x_gamb[k];
             if (L_ >= 0) diag += (T)d.x_ring_g[k] * (T)(d.x_ring_edge[k] < 2 ? d.nc : d.nr);
             for (int q = 0; q < d.x_ncpl[k]; q++) diag += (T)d.x_cpl_g[k][q];
-            upd((long)d.nl * d.nr * d.nc + k, T(1) / diag);
+            const long c = (long)d.nl * d.nr * d.nc + k;
+            upd(c, x[c], p[c], r[c], qv[c], T(1) / diag);
         }
     }
     block_reduce<2>(v);
@@ -557,18 +677,30 @@ __global__ void __launch_bounds__(kBlock) k_ir_resid(const ProbDesc *__restrict_
         r32[c] = rf;
         z32[c] = zf;
         e32[c] = 0.0f;
-        v[0] = (double)rf * zf;
-        v[1] = rr * rr;
-        v[2] = rhsc * rhsc;
+        v[0] += (double)rf * zf;
+        v[1] += rr * rr;
+        v[2] += rhsc * rhsc;
     };
     if (wi.tile < d.ntiles) {
-        const Cell q = tile_cell(d, wi.tile);
-        if (q.active) {
-            const double xc = x[q.c];
-            double flux, diag, gam;
-            cell_terms<double, HET>(d, ld, q, xc, flux, diag, gam);
-            put(q.c, rhs[q.c] + flux - gam * xc, diag, rhs[q.c]);
-        }
+        double ring[4];
+        load_ring<double>(d, ld, tile_layer(d, wi.tile), ring);
+        struct V {
+            Pts<double, HET> p;
+            double b;
+        };
+        tile_loop<InFlight<double, HET>::value>(
+            d, wi.tile,
+            [&](const Cell &q) {
+                V a{};
+                load_pts<double, HET>(d, ld, q, a.p);
+                a.b = rhs[q.c];
+                return a;
+            },
+            [&](const Cell &q, const V &a) {
+                double flux, diag, gam;
+                stencil<double, HET>(d, q, a.p, ring, flux, diag, gam);
+                put(q.c, a.b + flux - gam * a.p.c, diag, a.b);
+            });
     } else {
         __shared__ double xf[kMaxExtra], xd[kMaxExtra];
         package_terms<double>(d, ld, false, xf, xd);
@@ -592,8 +724,19 @@ __global__ void __launch_bounds__(kBlock) k_ir_correct(const ProbDesc *__restric
     double *x = d.b64[B_STEADY];
     const float *e = d.b32[F_X];
     if (wi.tile < d.ntiles) {
-        const Cell q = tile_cell(d, wi.tile);
-        if (q.active) x[q.c] += (double)e[q.c];
+        struct V {
+            double x;
+            float e;
+        };
+        tile_loop<8>(
+            d, wi.tile,
+            [&](const Cell &q) {
+                V a;
+                a.x = x[q.c];
+                a.e = e[q.c];
+                return a;
+            },
+            [&](const Cell &q, const V &a) { x[q.c] = a.x + (double)a.e; });
     } else if (threadIdx.x < d.nx) {
         const long c = (long)d.nl * d.nr * d.nc + threadIdx.x;
         x[c] += (double)e[c];
@@ -640,33 +783,55 @@ __global__ void __launch_bounds__(kBlock) k_rk_stage(const ProbDesc *__restrict_
     const double *rhs = d.b64[B_RHS];
     LdStage<double> ld{base, kin, a};
     double v[1] = {0};
-    auto fin = [&](long c, double k) {
+    // k and the raw values the mode combines it with (kin, acc, base, cmp at c)
+    auto fin = [&](long c, double k, double kinc, double accc, double basec, double cmpc) {
         switch (mode) {
             case ST_K: kout[c] = k; break;
-            case ST_FIRST: kout[c] = k; acc[c] = kin[c] + 2 * k; break;
-            case ST_MID: kout[c] = k; acc[c] += 2 * k; break;
-            case ST_LAST: out[c] = base[c] + hc * (acc[c] + k) / 6.0; break;
+            case ST_FIRST: kout[c] = k; acc[c] = kinc + 2 * k; break;
+            case ST_MID: kout[c] = k; acc[c] = accc + 2 * k; break;
+            case ST_LAST: out[c] = basec + hc * (accc + k) / 6.0; break;
             default: {
-                const double o = base[c] + hc * (acc[c] + k) / 6.0;
-                v[0] = fmax(v[0], fabs(cmp[c] - o));
+                const double o = basec + hc * (accc + k) / 6.0;
+                v[0] = fmax(v[0], fabs(cmpc - o));
             }
         }
     };
+    const bool need_kin = mode == ST_FIRST, need_acc = mode >= ST_MID;
+    const bool need_base = mode >= ST_LAST, need_cmp = mode == ST_LAST_ERR;
     if (wi.tile < d.ntiles) {
-        const Cell q = tile_cell(d, wi.tile);
-        if (q.active) {
-            const double xc = ld(q.c);
-            double flux, diag, gam;
-            cell_terms<double, HET>(d, ld, q, xc, flux, diag, gam);
-            fin(q.c, cell_cinv<double, HET>(d, q) * (rhs[q.c] + flux - gam * xc));
-        }
+        double ring[4];
+        load_ring<double>(d, ld, tile_layer(d, wi.tile), ring);
+        struct V {
+            Pts<double, HET> p;
+            double b, kin, acc, base, cmp, ci;
+        };
+        tile_loop<InFlight<double, HET>::value>(
+            d, wi.tile,
+            [&](const Cell &q) {
+                V t{};
+                load_pts<double, HET>(d, ld, q, t.p);
+                t.b = rhs[q.c];
+                if (need_kin) t.kin = kin[q.c];
+                if (need_acc) t.acc = acc[q.c];
+                if (need_base) t.base = base[q.c];
+                if (need_cmp) t.cmp = cmp[q.c];
+                if (HET) t.ci = Coef<double>::cinv(d)[q.c];
+                return t;
+            },
+            [&](const Cell &q, const V &t) {
+                double flux, diag, gam;
+                stencil<double, HET>(d, q, t.p, ring, flux, diag, gam);
+                const double ci = HET ? t.ci : d.cinv[q.n];
+                fin(q.c, ci * (t.b + flux - gam * t.p.c), t.kin, t.acc, t.base, t.cmp);
+            });
     } else {
         __shared__ double xf[kMaxExtra], xd[kMaxExtra];
         package_terms<double>(d, ld, true, xf, xd);
         const int k = threadIdx.x;
         if (k < d.nx) {
             const long c = (long)d.nl * d.nr * d.nc + k;
-            fin(c, d.x_cinv[k] * (rhs[c] + xf[k] - d.x_gamb[k] * ld(c)));
+            fin(c, d.x_cinv[k] * (rhs[c] + xf[k] - d.x_gamb[k] * ld(c)), need_kin ? kin[c] : 0.0,
+                need_acc ? acc[c] : 0.0, need_base ? base[c] : 0.0, need_cmp ? cmp[c] : 0.0);
         }
     }
     if (mode != ST_LAST_ERR) return;
@@ -887,13 +1052,27 @@ __global__ void __launch_bounds__(kBlock) k_tay_first(const ProbDesc *__restrict
         v[0] = fmax(v[0], fabs((double)e));
     };
     if (wi.tile < d.ntiles) {
-        const Cell q = tile_cell(d, wi.tile);
-        if (q.active) {
-            const double xc = y[q.c];
-            double flux, diag, gam;
-            cell_terms<double, HET>(d, ld, q, xc, flux, diag, gam);
-            fin(q.c, cell_cinv<double, HET>(d, q) * (rhs[q.c] + flux - gam * xc));
-        }
+        double ring[4];
+        load_ring<double>(d, ld, tile_layer(d, wi.tile), ring);
+        struct V {
+            Pts<double, HET> p;
+            double b, ci;
+        };
+        tile_loop<InFlight<double, HET>::value>(
+            d, wi.tile,
+            [&](const Cell &q) {
+                V t{};
+                load_pts<double, HET>(d, ld, q, t.p);
+                t.b = rhs[q.c];
+                if (HET) t.ci = Coef<double>::cinv(d)[q.c];
+                return t;
+            },
+            [&](const Cell &q, const V &t) {
+                double flux, diag, gam;
+                stencil<double, HET>(d, q, t.p, ring, flux, diag, gam);
+                const double ci = HET ? t.ci : d.cinv[q.n];
+                fin(q.c, ci * (t.b + flux - gam * t.p.c));
+            });
     } else {
         __shared__ double xf[kMaxExtra], xd[kMaxExtra];
         package_terms<double>(d, ld, true, xf, xd);
@@ -921,26 +1100,39 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict_
     const double tau = d.scal[S_TAU];
     const TD s = (TD)(tau / (k + 1));
     LdPlain<TD> ld{ein};
-    auto fin = [&](long c, TD f) {
-        const TD e = s * f;
-        eout[c] = e;
-        acc[c] += e;
-    };
     if (wi.tile < d.ntiles) {
-        const Cell q = tile_cell(d, wi.tile);
-        if (q.active) {
-            const TD xc = ein[q.c];
-            TD flux, diag, gam;
-            cell_terms<TD, HET>(d, ld, q, xc, flux, diag, gam);
-            fin(q.c, cell_cinv<TD, HET>(d, q) * (flux - gam * xc));
-        }
+        TD ring[4];
+        load_ring<TD>(d, ld, tile_layer(d, wi.tile), ring);
+        struct V {
+            Pts<TD, HET> p;
+            TD acc, ci;
+        };
+        tile_loop<InFlight<TD, HET>::value>(
+            d, wi.tile,
+            [&](const Cell &q) {
+                V t{};
+                load_pts<TD, HET>(d, ld, q, t.p);
+                t.acc = acc[q.c];
+                if (HET) t.ci = Coef<TD>::cinv(d)[q.c];
+                return t;
+            },
+            [&](const Cell &q, const V &t) {
+                TD flux, diag, gam;
+                stencil<TD, HET>(d, q, t.p, ring, flux, diag, gam);
+                const TD ci = HET ? t.ci : (TD)d.cinv[q.n];
+                const TD e = s * (ci * (flux - gam * t.p.c));
+                eout[q.c] = e;
+                acc[q.c] = t.acc + e;
+            });
     } else {
         __shared__ TD xf[kMaxExtra], xd[kMaxExtra];
         package_terms<TD>(d, ld, true, xf, xd);
         const int kk = threadIdx.x;
         if (kk < d.nx) {
             const long c = (long)d.nl * d.nr * d.nc + kk;
-            fin(c, (TD)d.x_cinv[kk] * (xf[kk] - (TD)d.x_gamb[kk] * ein[c]));
+            const TD e = s * ((TD)d.x_cinv[kk] * (xf[kk] - (TD)d.x_gamb[kk] * ein[c]));
+            eout[c] = e;
+            acc[c] += e;
         }
     }
 }
@@ -957,8 +1149,19 @@ __global__ void __launch_bounds__(kBlock) k_tay_final(const ProbDesc *__restrict
     double *y = d.b64[B_Y0 + ypar];
     const TD *acc = sizeof(TD) == 8 ? (const TD *)d.b64[accb] : (const TD *)d.b32[accb];
     if (wi.tile < d.ntiles) {
-        const Cell q = tile_cell(d, wi.tile);
-        if (q.active) y[q.c] += (double)acc[q.c];
+        struct V {
+            double y;
+            TD a;
+        };
+        tile_loop<8>(
+            d, wi.tile,
+            [&](const Cell &q) {
+                V t;
+                t.y = y[q.c];
+                t.a = acc[q.c];
+                return t;
+            },
+            [&](const Cell &q, const V &t) { y[q.c] = t.y + (double)t.a; });
     } else if (threadIdx.x < d.nx) {
         const long c = (long)d.nl * d.nr * d.nc + threadIdx.x;
         y[c] += (double)acc[c];
@@ -975,13 +1178,27 @@ __global__ void __launch_bounds__(kBlock) k_slope(const ProbDesc *__restrict__ D
     double *out = d.b64[outb];
     LdPlain<double> ld{x};
     if (wi.tile < d.ntiles) {
-        const Cell q = tile_cell(d, wi.tile);
-        if (q.active) {
-            const double xc = x[q.c];
-            double flux, diag, gam;
-            cell_terms<double, HET>(d, ld, q, xc, flux, diag, gam);
-            out[q.c] = cell_cinv<double, HET>(d, q) * (rhs[q.c] + flux - gam * xc);
-        }
+        double ring[4];
+        load_ring<double>(d, ld, tile_layer(d, wi.tile), ring);
+        struct V {
+            Pts<double, HET> p;
+            double b, ci;
+        };
+        tile_loop<InFlight<double, HET>::value>(
+            d, wi.tile,
+            [&](const Cell &q) {
+                V t{};
+                load_pts<double, HET>(d, ld, q, t.p);
+                t.b = rhs[q.c];
+                if (HET) t.ci = Coef<double>::cinv(d)[q.c];
+                return t;
+            },
+            [&](const Cell &q, const V &t) {
+                double flux, diag, gam;
+                stencil<double, HET>(d, q, t.p, ring, flux, diag, gam);
+                const double ci = HET ? t.ci : d.cinv[q.n];
+                out[q.c] = ci * (t.b + flux - gam * t.p.c);
+            });
     } else {
         __shared__ double xf[kMaxExtra], xd[kMaxExtra];
         package_terms<double>(d, ld, true, xf, xd);

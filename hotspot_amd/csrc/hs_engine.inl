@@ -106,13 +106,27 @@ static int ilog2(int v) {
 }
 
 // Tile shape: TX = min(nc, 64) consecutive columns (one wavefront reads a
-// contiguous row segment), TY = 256 / TX rows.
+// contiguous row segment), TY = 256 / TX thread rows, each thread owning rpt
+// grid rows.  rpt grows (up to 8) while a layer keeps at least 8 tiles
+// along its rows: fewer, longer-lived workgroups amortise each workgroup's
+// fixed cost (work item, descriptor and flag reads) over more cells.
+static int rows_per_thread(int nr, int ty) {
+    static const int forced = [] {
+        const char *e = getenv("HS_ROWS");
+        return e ? atoi(e) : 0;
+    }();
+    if (forced == 1 || forced == 2 || forced == 4 || forced == 8) return forced;
+    int r = 1;
+    while (r < 8 && nr >= ty * r * 2 * 8) r *= 2;
+    return r;
+}
+
 static void set_tiling(ProbDesc &h) {
-    const int tx = std::min(h.nc, 64);
-    h.tx_shift = ilog2(tx);
-    const int ty = kBlock >> h.tx_shift;
+    h.tx_shift = ilog2(std::min(h.nc, 64));
+    const int tx = 1 << h.tx_shift, ty = kBlock >> h.tx_shift;
+    h.rpt = rows_per_thread(h.nr, ty);
     h.ntj = (h.nc + tx - 1) / tx;
-    h.nti = (h.nr + ty - 1) / ty;
+    h.nti = (h.nr + ty * h.rpt - 1) / (ty * h.rpt);
     h.ntiles = h.nl * h.nti * h.ntj;
 }
 

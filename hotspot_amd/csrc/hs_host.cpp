@@ -1717,6 +1717,12 @@ int hs_engine_submit(hs_engine_t *e, RC_model_t *model, const double *power, int
     HS_END(-1)
 }
 
+int hs_engine_admit(hs_engine_t *e) {
+    HS_BEGIN
+    return eng_admit(engine_of(e));
+    HS_END(-1)
+}
+
 int hs_engine_tick(hs_engine_t *e) {
     HS_BEGIN
     return eng_tick(engine_of(e));

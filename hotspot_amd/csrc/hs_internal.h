@@ -120,6 +120,7 @@ void eng_set_options(Engine &E, const hs_options_t &o);
 void eng_set_tick(Engine &E, int intervals, int iters);
 void eng_set_budget(Engine &E, double bytes);
 int eng_submit(Engine &E, Model &M, const double *power, int rows, const double *init, int nsteps, double dt);
+int eng_admit(Engine &E);
 int eng_tick(Engine &E);
 int eng_poll(Engine &E, int id, double *out);
 int eng_result_seq(Engine &E, int id, double *out);

@@ -2,11 +2,12 @@
 //
 // Every kernel works on a work-list of (problem, tile) items over an array of
 // problem descriptors, so one launch serves one model or a batch of models of
-// different sizes.  A tile is a TX x TY patch of one layer (TX*TY = 256
-// threads, one thread per cell, TX along the contiguous column index so each
-// wavefront reads 64 consecutive doubles); the last item of each problem is a
-// "package" workgroup that handles the 12 (or 28) package nodes, whose rows
-// couple to whole boundary rows of the spreader / sink (and secondary) layers.
+// different sizes.  A tile is a TX x (TY * rpt) patch of one layer (TX*TY =
+// 256 threads, TX along the contiguous column index so each wavefront reads
+// 64 consecutive values; each thread owns rpt consecutive rows of its
+// column); the last item of each problem is a "package" workgroup that
+// handles the 12 (or 28) package nodes, whose rows couple to whole boundary
+// rows of the spreader / sink (and secondary) layers.
 //
 // Per-problem scalars (CG coefficients, RK4 step sizes, flags) live in device
 // memory; reductions are two-level and deterministic: each workgroup writes
@@ -66,7 +67,8 @@ struct ProbDesc {
     // geometry and tiling
     int nl, nr, nc, nx;
     int tx_shift, nti, ntj, ntiles;   // tiles: nl * nti * ntj, then the package item
-    int hetero, has_adv, nblocks, pad0;
+    int hetero, has_adv, nblocks;
+    int rpt;                          // grid rows per thread in a tile
     double ambient;
     double cell_area;
     // homogeneous layer coefficients

@@ -42,6 +42,8 @@ def _bind(L):
     L.hs_engine_set_budget.restype = None
     L.hs_engine_submit.argtypes = [C.c_void_p, iface.RCModelP, dp, C.c_int, dp, C.c_int, C.c_double]
     L.hs_engine_submit.restype = C.c_int
+    L.hs_engine_admit.argtypes = [C.c_void_p]
+    L.hs_engine_admit.restype = C.c_int
     L.hs_engine_tick.argtypes = [C.c_void_p]
     L.hs_engine_tick.restype = C.c_int
     L.hs_engine_poll.argtypes = [C.c_void_p, C.c_int, dp]
@@ -109,6 +111,12 @@ class Engine:
             raise RuntimeError("hs_engine_submit refused the job")
         self.meta[job] = (n, n_steps)
         return job
+
+    def admit(self):
+        """Admits what fits; returns the number of jobs still queued."""
+        q = self.L.hs_engine_admit(self.h)
+        self._check()
+        return q
 
     def tick(self):
         a = self.L.hs_engine_tick(self.h)

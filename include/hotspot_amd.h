@@ -371,6 +371,9 @@ void hs_engine_set_budget(hs_engine_t *e, double bytes);
  * footprint above the budget). */
 int hs_engine_submit(hs_engine_t *e, RC_model_t *model, const double *power, int power_rows,
                      const double *init_temp, int n_steps, double dt);
+/* Admits queued jobs while they fit (their inputs are then resident on the
+ * device); returns the number still queued.  hs_engine_tick admits too. */
+int hs_engine_admit(hs_engine_t *e);
 /* Admits what fits and advances every resident job by one tick; returns the
  * number of jobs queued or running. */
 int hs_engine_tick(hs_engine_t *e);
