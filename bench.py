@@ -175,7 +175,7 @@ def served(lib, jobs, intervals, tick, keep_first=False, timer=None):
     device_sync()
     t0 = time.perf_counter()
     if timer:
-        lib.lib.hs_kernel_timer(timer, 200000)
+        lib.lib.hs_kernel_timer(timer, 10 ** 7)
     eng.run()
     t1 = time.perf_counter()
     infos = [eng.info(i) for i in ids]
@@ -235,19 +235,26 @@ def main():
     # roofline pass: the same batch with every launch of the dominant kernel
     # bracketed by device events on the engine's stream (plain launches
     # instead of graph replay while the timer is on)
-    achieved = avg_ms = nt = None
+    # Algorithmic bytes of one term sweep of one job: FP32 read of E_k, read and
+    # write of the accumulator, write of E_k+1 = 16 B per node.  A launch
+    # skips the jobs whose series has already converged, so the achieved rate
+    # is the bytes of the sweeps the jobs actually took (counted on the
+    # device) over the summed duration of every launch, skipping ones included.
+    achieved = avg_ms = nt = sweeps = None
     nvec = cells + iface.EXTRA
-    bytes_per_launch = args.batch * nvec * 16   # FP32: read term, read+write accumulator, write next term
+    bytes_per_sweep = nvec * 16
+    bytes_per_launch = args.batch * bytes_per_sweep
     if not args.no_roofline:
-        t_roof, _, _, _ = served(lib, jobs, args.intervals, args.tick, timer=b"k_tay_term")
+        t_roof, _, rinfo, _ = served(lib, jobs, args.intervals, args.tick, timer=b"k_tay_term")
         device_sync()
         a, tot = C.c_double(), C.c_double()
         nt = L.hs_kernel_timer_read(C.byref(a), C.byref(tot))
         L.hs_kernel_timer(None, 0)
         avg_ms = a.value
-        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if nt else None
+        sweeps = float(sum(i.taylor_sweeps for i in rinfo))
+        achieved = sweeps * bytes_per_sweep / (tot.value * 1e-3) / 1e9 if nt else None
         log(f"roofline pass {t_roof:.3f} s: {nt} launches of k_tay_term, mean {avg_ms*1e3:.2f} us, "
-            f"{achieved:.0f} GB/s")
+            f"{sweeps / args.batch / nt:.3f} of the job sweeps active per launch, {achieved:.0f} GB/s")
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_k_tay_term_3dstack{args.grid}_batch{args.batch}.json")
     if os.path.exists(tf):
@@ -313,7 +320,10 @@ def main():
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic, "launches_timed": nt,
                      "avg_launch_us": avg_ms * 1e3 if avg_ms else None,
-                     "bytes_per_launch": bytes_per_launch},
+                     "bytes_per_launch": bytes_per_launch,
+                     "active_sweep_fraction": (sweeps / args.batch / nt) if nt else None,
+                     "traffic_note": "HBM bytes per launch from FETCH_SIZE and WRITE_SIZE (rocprofv3 "
+                                     "--pmc, scaled against a plain copy)"},
         "single_job": single,
         "fp64_gpu": fp64,
         "speedup_vs_fp64_gpu": (value / fp64["value"]) if fp64 else None,

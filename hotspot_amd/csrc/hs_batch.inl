@@ -39,6 +39,7 @@ struct EngJob {
     std::vector<double> seq;            // nsteps x nblk
     double t_submit = 0, t_admit = 0, t_done = 0;
     int steady_iters = 0, refinements = 0, rk4_calls = 0;
+    long taylor_sweeps = 0;
 };
 
 // A group of resident jobs in one phase with one kernel form (homogeneous or
@@ -126,9 +127,11 @@ static void group_sync(Engine &E, BatchGroup &B, hipStream_t s, bool het, bool t
     std::vector<ProbDesc> desc(n);
     std::vector<WorkItem> w, gw;
     int m = 1, K = 1;
+    bool v4 = true;
     for (int i = 0; i < n; i++) {
         const DevModel &D = *B.members[i]->D;
         desc[i] = D.h;
+        v4 = v4 && vec4_ok(D.h);
         for (int t = 0; t <= D.h.ntiles; t++) w.push_back(WorkItem{i, t});
         for (int u = 0; u < D.h.nblocks; u += kBlock / kWave) gw.push_back(WorkItem{i, u});
         gw.push_back(WorkItem{i, -1});
@@ -157,6 +160,7 @@ static void group_sync(Engine &E, BatchGroup &B, hipStream_t s, bool het, bool t
     B.g.ngwork = (int)gw.size();
     B.g.stream = s;
     B.g.hetero = het;
+    B.g.vec4 = v4;
     B.form = SeqForm{E.opt.precision, E.opt.transient, m, K};
     B.dirty = false;
 }
@@ -264,6 +268,7 @@ static void steady_done(Engine &E, EngJob &j, const double *flags) {
 static void trans_done(Engine &E, EngJob &j, const double *flags) {
     DevModel &D = *j.D;
     j.rk4_calls = E.opt.transient == HS_TRANS_RK4 ? (int)flags[1] : 0;
+    j.taylor_sweeps = E.opt.transient == HS_TRANS_TAYLOR ? (long)flags[2] : 0;
     const size_t n = (size_t)j.nsteps * D.nblk;
     j.seq.resize(n);
     HIP_OK(hipMemcpyAsync(j.seq.data(), D.h.seq_out, n * sizeof(double), hipMemcpyDeviceToHost, D.stream));
@@ -396,7 +401,7 @@ int eng_tick(Engine &E) {
         group_sync(E, B, E.s_trans, h, true);
         ran_trans[h] = B.g.nprob;
         B.rp.run(E.s_trans, E.tick_intervals, [&] { seq_body(B.g, B.form); });
-        gather(B, {S_STEP, S_TCALLS});
+        gather(B, {S_STEP, S_TCALLS, S_TTERMS});
     }
     HIP_OK(hipStreamSynchronize(E.s_steady));
     HIP_OK(hipStreamSynchronize(E.s_trans));
@@ -452,6 +457,7 @@ void eng_job_info(Engine &E, int id, hs_job_info_t *info) {
     info->steady_iters = j.steady_iters;
     info->steady_refinements = j.refinements;
     info->rk4_calls = j.rk4_calls;
+    info->taylor_sweeps = (double)j.taylor_sweeps;
     info->wait_s = j.phase == J_QUEUED ? now_s() - j.t_submit : j.t_admit - j.t_submit;
     info->latency_s = j.phase == J_DONE ? j.t_done - j.t_submit : now_s() - j.t_submit;
     info->device_bytes = (double)(j.held ? j.held : j.footprint);

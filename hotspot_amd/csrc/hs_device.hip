@@ -111,9 +111,19 @@ __device__ __forceinline__ int work_index() {
 // ---------------------------------------------------------------------------
 struct Cell {
     int n, i, j;
-    long c;
+    unsigned c;   // node index within the problem (< 2^29 for every supported grid)
     bool active;
 };
+
+// Element access through a 32-bit byte offset from a wave-uniform base: the
+// address is an SGPR pair plus one VGPR (global_load ... saddr), half the
+// registers of a 64-bit address per load.
+template <typename T> __device__ __forceinline__ T ld_at(const T *base, unsigned i) {
+    return *(const T *)((const char *)base + i * (unsigned)sizeof(T));
+}
+template <typename T> __device__ __forceinline__ void st_at(T *base, unsigned i, T v) {
+    *(T *)((char *)base + i * (unsigned)sizeof(T)) = v;
+}
 
 // A tile is TX columns x (TY * rpt) rows of one layer; thread (tx, ty) owns
 // the rpt consecutive rows starting at ty * rpt of its column.  Cells beyond
@@ -132,7 +142,7 @@ __device__ __forceinline__ Cell tile_cell(const ProbDesc &d, int tile, int r) {
     q.n = n;
     q.i = min(i, d.nr - 1);
     q.j = min(j, d.nc - 1);
-    q.c = ((long)n * d.nr + q.i) * d.nc + q.j;
+    q.c = ((unsigned)n * d.nr + q.i) * d.nc + q.j;
     return q;
 }
 
@@ -195,19 +205,21 @@ template <> struct Coef<float> {
 // input loaders: plain vector, or an RK stage base + a*k
 template <typename T> struct LdPlain {
     const T *__restrict__ x;
-    __device__ __forceinline__ T operator()(long c) const { return x[c]; }
+    __device__ __forceinline__ T operator()(unsigned c) const { return ld_at(x, c); }
 };
 template <typename T> struct LdStage {
     const T *__restrict__ b;
     const T *__restrict__ k;
     T a;
-    __device__ __forceinline__ T operator()(long c) const { return b[c] + a * k[c]; }
+    __device__ __forceinline__ T operator()(unsigned c) const { return ld_at(b, c) + a * ld_at(k, c); }
 };
 template <typename T> struct LdDir {   // CG direction p = z + beta * p_old
     const T *__restrict__ z;
     const T *__restrict__ p;
     T beta;
-    __device__ __forceinline__ T operator()(long c) const { return beta == T(0) ? z[c] : z[c] + beta * p[c]; }
+    __device__ __forceinline__ T operator()(unsigned c) const {
+        return beta == T(0) ? ld_at(z, c) : ld_at(z, c) + beta * ld_at(p, c);
+    }
 };
 
 // The values around a cell that the operator reads: the state at the cell
@@ -223,10 +235,10 @@ template <typename T, bool HET> struct Pts {
 template <typename T, bool HET, class L>
 __device__ __forceinline__ void load_pts(const ProbDesc &d, const L &ld, const Cell &q, Pts<T, HET> &p) {
     const int nr = d.nr, nc = d.nc, nl = d.nl, n = q.n, i = q.i, j = q.j;
-    const long c = q.c, plane = (long)nr * nc;
-    const long cn = i > 0 ? c - nc : c, cs = i < nr - 1 ? c + nc : c;
-    const long ce = j < nc - 1 ? c + 1 : c, cw = j > 0 ? c - 1 : c;
-    const long cu = n > 0 ? c - plane : c, cd = n < nl - 1 ? c + plane : c;
+    const unsigned c = q.c, plane = (unsigned)nr * nc;
+    const unsigned cn = i > 0 ? c - nc : c, cs = i < nr - 1 ? c + nc : c;
+    const unsigned ce = j < nc - 1 ? c + 1 : c, cw = j > 0 ? c - 1 : c;
+    const unsigned cu = n > 0 ? c - plane : c, cd = n < nl - 1 ? c + plane : c;
     p.c = ld(c);
     p.n = ld(cn);
     p.s = ld(cs);
@@ -236,18 +248,18 @@ __device__ __forceinline__ void load_pts(const ProbDesc &d, const L &ld, const C
     p.dn = ld(cd);
     if (HET) {
         const T *gE = Coef<T>::E(d), *gS = Coef<T>::S(d), *gD = Coef<T>::D(d);
-        p.gn = gS[cn];
-        p.gs = gS[c];
-        p.ge = gE[c];
-        p.gw = gE[cw];
-        p.gu = gD[cu];
-        p.gd = gD[c];
+        p.gn = ld_at(gS, cn);
+        p.gs = ld_at(gS, c);
+        p.ge = ld_at(gE, c);
+        p.gw = ld_at(gE, cw);
+        p.gu = ld_at(gD, cu);
+        p.gd = ld_at(gD, c);
         if (d.has_adv) {
-            p.an = Coef<T>::aN(d)[c];
-            p.as = Coef<T>::aS(d)[c];
-            p.ae = Coef<T>::aE(d)[c];
-            p.aw = Coef<T>::aW(d)[c];
-            p.self = Coef<T>::aSelf(d)[c];
+            p.an = ld_at(Coef<T>::aN(d), c);
+            p.as = ld_at(Coef<T>::aS(d), c);
+            p.ae = ld_at(Coef<T>::aE(d), c);
+            p.aw = ld_at(Coef<T>::aW(d), c);
+            p.self = ld_at(Coef<T>::aSelf(d), c);
         }
     }
 }
@@ -256,7 +268,7 @@ __device__ __forceinline__ void load_pts(const ProbDesc &d, const L &ld, const C
 // read once per tile (every cell of a tile is in one layer).
 template <typename T, class L>
 __device__ __forceinline__ void load_ring(const ProbDesc &d, const L &ld, int n, T (&ring)[4]) {
-    const long ncell = (long)d.nl * d.nr * d.nc;
+    const unsigned ncell = (unsigned)d.nl * d.nr * d.nc;
 #pragma unroll
     for (int e = 0; e < 4; e++) ring[e] = ld(ncell + max(d.ring_node[n][e], 0));
 }
@@ -425,9 +437,9 @@ __global__ void __launch_bounds__(kBlock) k_b2g(const ProbDesc *__restrict__ D,
             }
             if (add_sources) {
                 val += src;
-                if (d.bconst) val += d.bconst[q.c];
+                if (d.bconst) val += ld_at(d.bconst, q.c);
             }
-            out[q.c] = val;
+            st_at(out, q.c, val);
         });
     } else {
         const int k = threadIdx.x;
@@ -519,7 +531,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const ProbDesc *__restrict__
             [&](const Cell &q) {
                 V a{};
                 load_pts<T, HET>(d, ld, q, a.p);
-                a.b = rhs[q.c];
+                a.b = ld_at(rhs, q.c);
                 return a;
             },
             [&](const Cell &q, const V &a) {
@@ -527,8 +539,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const ProbDesc *__restrict__
                 stencil<T, HET>(d, q, a.p, ring, flux, diag, gam);
                 const T rr = a.b + flux - gam * a.p.c;
                 const T zz = rr / diag;
-                r[q.c] = rr;
-                z[q.c] = zz;
+                st_at(r, q.c, rr);
+                st_at(z, q.c, zz);
                 v[0] += (double)rr * zz;
                 v[1] += (double)rr * rr;
                 v[2] += (double)a.b * a.b;
@@ -581,8 +593,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_dir(const ProbDesc *__restrict__ 
                 T flux, diag, gam;
                 stencil<T, HET>(d, q, p, ring, flux, diag, gam);
                 const T Gp = gam * p.c - flux;
-                pnew[q.c] = p.c;
-                qv[q.c] = Gp;
+                st_at(pnew, q.c, p.c);
+                st_at(qv, q.c, Gp);
                 v[0] += (double)p.c * Gp;
             });
     } else {
@@ -617,12 +629,12 @@ __global__ void __launch_bounds__(kBlock) k_cg_upd(const ProbDesc *__restrict__ 
     const T *p = buf(p0b + 1 - par), *qv = buf(qb);
     const T alpha = (T)d.scal[S_ALPHA];
     double v[2] = {0, 0};
-    auto upd = [&](long c, T xc, T pc, T rc, T qc, T dinv) {
-        x[c] = xc + alpha * pc;
+    auto upd = [&](unsigned c, T xc, T pc, T rc, T qc, T dinv) {
+        st_at(x, c, xc + alpha * pc);
         const T rr = rc - alpha * qc;
         const T zz = rr * dinv;
-        r[c] = rr;
-        z[c] = zz;
+        st_at(r, c, rr);
+        st_at(z, c, zz);
         v[0] += (double)rr * zz;
         v[1] += (double)rr * rr;
     };
@@ -634,11 +646,11 @@ __global__ void __launch_bounds__(kBlock) k_cg_upd(const ProbDesc *__restrict__ 
             d, wi.tile,
             [&](const Cell &q) {
                 V a{};
-                a.x = x[q.c];
-                a.p = p[q.c];
-                a.r = r[q.c];
-                a.q = qv[q.c];
-                if (HET) a.di = Coef<T>::dinv(d)[q.c];
+                a.x = ld_at(x, q.c);
+                a.p = ld_at(p, q.c);
+                a.r = ld_at(r, q.c);
+                a.q = ld_at(qv, q.c);
+                if (HET) a.di = ld_at(Coef<T>::dinv(d), q.c);
                 return a;
             },
             [&](const Cell &q, const V &a) { upd(q.c, a.x, a.p, a.r, a.q, HET ? a.di : cell_dinv<T, false>(d, q)); });
@@ -671,12 +683,12 @@ __global__ void __launch_bounds__(kBlock) k_ir_resid(const ProbDesc *__restrict_
     float *r32 = d.b32[F_R], *z32 = d.b32[F_Z], *e32 = d.b32[F_X];
     LdPlain<double> ld{x};
     double v[3] = {0, 0, 0};
-    auto put = [&](long c, double rr, double diag, double rhsc) {
+    auto put = [&](unsigned c, double rr, double diag, double rhsc) {
         const float rf = (float)rr;
         const float zf = (float)(rr / diag);
-        r32[c] = rf;
-        z32[c] = zf;
-        e32[c] = 0.0f;
+        st_at(r32, c, rf);
+        st_at(z32, c, zf);
+        st_at(e32, c, 0.0f);
         v[0] += (double)rf * zf;
         v[1] += rr * rr;
         v[2] += rhsc * rhsc;
@@ -693,7 +705,7 @@ __global__ void __launch_bounds__(kBlock) k_ir_resid(const ProbDesc *__restrict_
             [&](const Cell &q) {
                 V a{};
                 load_pts<double, HET>(d, ld, q, a.p);
-                a.b = rhs[q.c];
+                a.b = ld_at(rhs, q.c);
                 return a;
             },
             [&](const Cell &q, const V &a) {
@@ -732,11 +744,11 @@ __global__ void __launch_bounds__(kBlock) k_ir_correct(const ProbDesc *__restric
             d, wi.tile,
             [&](const Cell &q) {
                 V a;
-                a.x = x[q.c];
-                a.e = e[q.c];
+                a.x = ld_at(x, q.c);
+                a.e = ld_at(e, q.c);
                 return a;
             },
-            [&](const Cell &q, const V &a) { x[q.c] = a.x + (double)a.e; });
+            [&](const Cell &q, const V &a) { st_at(x, q.c, a.x + (double)a.e); });
     } else if (threadIdx.x < d.nx) {
         const long c = (long)d.nl * d.nr * d.nc + threadIdx.x;
         x[c] += (double)e[c];
@@ -784,12 +796,12 @@ __global__ void __launch_bounds__(kBlock) k_rk_stage(const ProbDesc *__restrict_
     LdStage<double> ld{base, kin, a};
     double v[1] = {0};
     // k and the raw values the mode combines it with (kin, acc, base, cmp at c)
-    auto fin = [&](long c, double k, double kinc, double accc, double basec, double cmpc) {
+    auto fin = [&](unsigned c, double k, double kinc, double accc, double basec, double cmpc) {
         switch (mode) {
-            case ST_K: kout[c] = k; break;
-            case ST_FIRST: kout[c] = k; acc[c] = kinc + 2 * k; break;
-            case ST_MID: kout[c] = k; acc[c] = accc + 2 * k; break;
-            case ST_LAST: out[c] = basec + hc * (accc + k) / 6.0; break;
+            case ST_K: st_at(kout, c, k); break;
+            case ST_FIRST: st_at(kout, c, k); st_at(acc, c, kinc + 2 * k); break;
+            case ST_MID: st_at(kout, c, k); st_at(acc, c, accc + 2 * k); break;
+            case ST_LAST: st_at(out, c, basec + hc * (accc + k) / 6.0); break;
             default: {
                 const double o = basec + hc * (accc + k) / 6.0;
                 v[0] = fmax(v[0], fabs(cmpc - o));
@@ -810,12 +822,12 @@ __global__ void __launch_bounds__(kBlock) k_rk_stage(const ProbDesc *__restrict_
             [&](const Cell &q) {
                 V t{};
                 load_pts<double, HET>(d, ld, q, t.p);
-                t.b = rhs[q.c];
-                if (need_kin) t.kin = kin[q.c];
-                if (need_acc) t.acc = acc[q.c];
-                if (need_base) t.base = base[q.c];
-                if (need_cmp) t.cmp = cmp[q.c];
-                if (HET) t.ci = Coef<double>::cinv(d)[q.c];
+                t.b = ld_at(rhs, q.c);
+                if (need_kin) t.kin = ld_at(kin, q.c);
+                if (need_acc) t.acc = ld_at(acc, q.c);
+                if (need_base) t.base = ld_at(base, q.c);
+                if (need_cmp) t.cmp = ld_at(cmp, q.c);
+                if (HET) t.ci = ld_at(Coef<double>::cinv(d), q.c);
                 return t;
             },
             [&](const Cell &q, const V &t) {
@@ -967,6 +979,7 @@ __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ 
             }
             s[S_TMAX] = w[0];
             s[S_KNEED] = K;
+            s[S_TTERMS] += K - 1;
             return;
         }
     }
@@ -1045,10 +1058,10 @@ __global__ void __launch_bounds__(kBlock) k_tay_first(const ProbDesc *__restrict
     const double tau = d.scal[S_TAU];
     LdPlain<double> ld{y};
     double v[1] = {0};
-    auto fin = [&](long c, double f) {
+    auto fin = [&](unsigned c, double f) {
         const TD e = (TD)(tau * f);
-        ek[c] = e;
-        acc[c] = e;
+        st_at(ek, c, e);
+        st_at(acc, c, e);
         v[0] = fmax(v[0], fabs((double)e));
     };
     if (wi.tile < d.ntiles) {
@@ -1063,8 +1076,8 @@ __global__ void __launch_bounds__(kBlock) k_tay_first(const ProbDesc *__restrict
             [&](const Cell &q) {
                 V t{};
                 load_pts<double, HET>(d, ld, q, t.p);
-                t.b = rhs[q.c];
-                if (HET) t.ci = Coef<double>::cinv(d)[q.c];
+                t.b = ld_at(rhs, q.c);
+                if (HET) t.ci = ld_at(Coef<double>::cinv(d), q.c);
                 return t;
             },
             [&](const Cell &q, const V &t) {
@@ -1084,6 +1097,90 @@ __global__ void __launch_bounds__(kBlock) k_tay_first(const ProbDesc *__restrict
     }
     block_reduce<1, true>(v);
     write_partials<1>(d, wi.tile, v);
+}
+
+// FP32 term sweep of a homogeneous problem with float4 accesses: within the
+// same tile, each thread owns 4 consecutive columns x rpt/4 rows (rpt >= 4,
+// tile width >= 16), so a cell costs 2 load instructions instead of 8.  The
+// arithmetic per cell is that of stencil() in the same order.
+__device__ __forceinline__ float4 ld4(const float *base, unsigned i) {
+    return *(const float4 *)((const char *)base + i * 4u);
+}
+__device__ __forceinline__ void st4(float *base, unsigned i, float4 v) {
+    *(float4 *)((char *)base + i * 4u) = v;
+}
+__device__ __forceinline__ float comp(const float4 &v, int k) {
+    return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
+}
+
+__device__ __forceinline__ void term_vec4(const ProbDesc &d, int tile, const float *ein, float *eout,
+                                          float *acc, float s, const float (&ring)[4]) {
+    const int tpl = d.nti * d.ntj;
+    const int n = tile / tpl, rem = tile - n * tpl;
+    const int ti = rem / d.ntj, tj = rem - ti * d.ntj;
+    const int sh = d.tx_shift;
+    const int tx4 = threadIdx.x & ((1 << (sh - 2)) - 1), ty4 = threadIdx.x >> (sh - 2);
+    const int G = d.rpt >> 2;
+    const int rows = (kBlock >> sh) * d.rpt;
+    const int nr = d.nr, nc = d.nc, nl = d.nl;
+    const unsigned plane = (unsigned)nr * nc;
+    const int j0 = (tj << sh) + 4 * tx4;
+    const float gx = (float)d.gx[n], gy = (float)d.gy[n];
+    const float gu = n > 0 ? (float)d.gz[max(n - 1, 0)] : 0.f, gd = n < nl - 1 ? (float)d.gz[n] : 0.f;
+    const float ci = (float)d.cinv[n], gam = (float)d.gamb[n];
+    const int *rn = d.ring_node[n];
+    const float rgn = (float)d.ring_gns[n], rge = (float)d.ring_gew[n];
+    constexpr int GM = 2;
+    float4 C[GM], N[GM], S[GM], U[GM], Dn[GM], A[GM];
+    float Wv[GM], Ev[GM];
+    int I[GM];
+    unsigned cc[GM];
+#pragma unroll
+    for (int g = 0; g < GM; g++) {
+        I[g] = ti * rows + ty4 * G + g;
+        if (g < G) {
+            const int i = min(I[g], nr - 1);
+            const unsigned c = ((unsigned)n * nr + i) * nc + j0;
+            cc[g] = c;
+            C[g] = ld4(ein, c);
+            N[g] = ld4(ein, i > 0 ? c - nc : c);
+            S[g] = ld4(ein, i < nr - 1 ? c + nc : c);
+            U[g] = ld4(ein, n > 0 ? c - plane : c);
+            Dn[g] = ld4(ein, n < nl - 1 ? c + plane : c);
+            Wv[g] = ld_at(ein, j0 > 0 ? c - 1 : c);
+            Ev[g] = ld_at(ein, j0 + 4 < nc ? c + 4 : c);
+            A[g] = ld4(acc, c);
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < GM; g++) {
+        if (g >= G || I[g] >= nr) continue;
+        const int i = I[g];
+        const float gn = i > 0 ? gx : 0.f, gs = i < nr - 1 ? gx : 0.f;
+        float e[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int j = j0 + k;
+            const float xc = comp(C[g], k);
+            const float xw = k == 0 ? Wv[g] : comp(C[g], k - 1);
+            const float xe = k == 3 ? Ev[g] : comp(C[g], k + 1);
+            const float ge = j < nc - 1 ? gy : 0.f, gw = j > 0 ? gy : 0.f;
+            float flux = gn * (comp(N[g], k) - xc);
+            flux += gs * (comp(S[g], k) - xc);
+            flux += ge * (xe - xc);
+            flux += gw * (xw - xc);
+            flux += gu * (comp(U[g], k) - xc);
+            flux += gd * (comp(Dn[g], k) - xc);
+            if (i == 0 && rn[0] >= 0) flux += rgn * (ring[0] - xc);
+            if (i == nr - 1 && rn[1] >= 0) flux += rgn * (ring[1] - xc);
+            if (j == nc - 1 && rn[2] >= 0) flux += rge * (ring[2] - xc);
+            if (j == 0 && rn[3] >= 0) flux += rge * (ring[3] - xc);
+            e[k] = s * (ci * (flux - gam * xc));
+        }
+        const float4 ev = make_float4(e[0], e[1], e[2], e[3]);
+        st4(eout, cc[g], ev);
+        st4(acc, cc[g], make_float4(A[g].x + e[0], A[g].y + e[1], A[g].z + e[2], A[g].w + e[3]));
+    }
 }
 
 // E_{k+1} from E_k (k >= 1); skipped once the series has its K terms
@@ -1112,8 +1209,8 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict_
             [&](const Cell &q) {
                 V t{};
                 load_pts<TD, HET>(d, ld, q, t.p);
-                t.acc = acc[q.c];
-                if (HET) t.ci = Coef<TD>::cinv(d)[q.c];
+                t.acc = ld_at(acc, q.c);
+                if (HET) t.ci = ld_at(Coef<TD>::cinv(d), q.c);
                 return t;
             },
             [&](const Cell &q, const V &t) {
@@ -1121,8 +1218,8 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict_
                 stencil<TD, HET>(d, q, t.p, ring, flux, diag, gam);
                 const TD ci = HET ? t.ci : (TD)d.cinv[q.n];
                 const TD e = s * (ci * (flux - gam * t.p.c));
-                eout[q.c] = e;
-                acc[q.c] = t.acc + e;
+                st_at(eout, q.c, e);
+                st_at(acc, q.c, t.acc + e);
             });
     } else {
         __shared__ TD xf[kMaxExtra], xd[kMaxExtra];
@@ -1131,6 +1228,35 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict_
         if (kk < d.nx) {
             const long c = (long)d.nl * d.nr * d.nc + kk;
             const TD e = s * ((TD)d.x_cinv[kk] * (xf[kk] - (TD)d.x_gamb[kk] * ein[c]));
+            eout[c] = e;
+            acc[c] += e;
+        }
+    }
+}
+
+// k_tay_term<float, false> for groups whose problems all take float4 tiles
+// (vec4_ok): the package item is shared with the scalar kernel.
+__global__ void __launch_bounds__(kBlock) k_tay_term4(const ProbDesc *__restrict__ D,
+                                                      const WorkItem *__restrict__ W, int inb,
+                                                      int outb, int accb, int k, int sub, int seq) {
+    const WorkItem wi = W[work_index()];
+    const ProbDesc &d = D[wi.prob];
+    if (sub >= (int)d.scal[S_TK] || k >= (int)d.scal[S_KNEED] || seq_finished(d, seq)) return;
+    const float *ein = d.b32[inb];
+    float *eout = d.b32[outb], *acc = d.b32[accb];
+    const float s = (float)(d.scal[S_TAU] / (k + 1));
+    LdPlain<float> ld{ein};
+    if (wi.tile < d.ntiles) {
+        float ring[4];
+        load_ring<float>(d, ld, tile_layer(d, wi.tile), ring);
+        term_vec4(d, wi.tile, ein, eout, acc, s, ring);
+    } else {
+        __shared__ float xf[kMaxExtra], xd[kMaxExtra];
+        package_terms<float>(d, ld, true, xf, xd);
+        const int kk = threadIdx.x;
+        if (kk < d.nx) {
+            const unsigned c = (unsigned)d.nl * d.nr * d.nc + kk;
+            const float e = s * ((float)d.x_cinv[kk] * (xf[kk] - (float)d.x_gamb[kk] * ein[c]));
             eout[c] = e;
             acc[c] += e;
         }
@@ -1157,11 +1283,11 @@ __global__ void __launch_bounds__(kBlock) k_tay_final(const ProbDesc *__restrict
             d, wi.tile,
             [&](const Cell &q) {
                 V t;
-                t.y = y[q.c];
-                t.a = acc[q.c];
+                t.y = ld_at(y, q.c);
+                t.a = ld_at(acc, q.c);
                 return t;
             },
-            [&](const Cell &q, const V &t) { y[q.c] = t.y + (double)t.a; });
+            [&](const Cell &q, const V &t) { st_at(y, q.c, t.y + (double)t.a); });
     } else if (threadIdx.x < d.nx) {
         const long c = (long)d.nl * d.nr * d.nc + threadIdx.x;
         y[c] += (double)acc[c];
@@ -1189,15 +1315,15 @@ __global__ void __launch_bounds__(kBlock) k_slope(const ProbDesc *__restrict__ D
             [&](const Cell &q) {
                 V t{};
                 load_pts<double, HET>(d, ld, q, t.p);
-                t.b = rhs[q.c];
-                if (HET) t.ci = Coef<double>::cinv(d)[q.c];
+                t.b = ld_at(rhs, q.c);
+                if (HET) t.ci = ld_at(Coef<double>::cinv(d), q.c);
                 return t;
             },
             [&](const Cell &q, const V &t) {
                 double flux, diag, gam;
                 stencil<double, HET>(d, q, t.p, ring, flux, diag, gam);
                 const double ci = HET ? t.ci : d.cinv[q.n];
-                out[q.c] = ci * (t.b + flux - gam * t.p.c);
+                st_at(out, q.c, ci * (t.b + flux - gam * t.p.c));
             });
     } else {
         __shared__ double xf[kMaxExtra], xd[kMaxExtra];

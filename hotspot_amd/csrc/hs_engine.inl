@@ -18,7 +18,12 @@ struct Group {
     int ngwork = 0;
     hipStream_t stream = nullptr;
     bool hetero = false;
+    bool vec4 = false;   // every problem takes float4 term sweeps (vec4_ok)
 };
+
+// A homogeneous problem whose tiles are at least 16 columns wide with at
+// least 4 rows per thread runs its FP32 term sweeps with float4 accesses.
+static bool vec4_ok(const ProbDesc &h) { return !h.hetero && h.rpt >= 4 && h.tx_shift >= 4; }
 
 struct DevModel {
     Group grp;                          // this model alone
@@ -500,6 +505,7 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
     D->grp.ngwork = D->ngwork;
     D->grp.stream = D->stream;
     D->grp.hetero = M.detailed;
+    D->grp.vec4 = vec4_ok(h);
     if (D->hetero) assemble_hetero(M, *D);
     // transient step bound
     if (g.c_ready) {
@@ -777,9 +783,14 @@ static void taylor_interval(const Group &G, int m, int K, int seq) {
         a.sub = s;
         a.seq = seq;
         finish(G, FIN_TAY, a);
-        for (int k = 1; k < K; k++)
-            launch_named("k_tay_term", k_tay_term<TD, HET>, G.nwork, G.stream, dd, w, (k & 1) ? db0 : db1,
-                         (k & 1) ? db1 : db0, accb, k, s, seq);
+        for (int k = 1; k < K; k++) {
+            if (f32 && !HET && G.vec4)
+                launch_named("k_tay_term", k_tay_term4, G.nwork, G.stream, dd, w, (k & 1) ? db0 : db1,
+                             (k & 1) ? db1 : db0, accb, k, s, seq);
+            else
+                launch_named("k_tay_term", k_tay_term<TD, HET>, G.nwork, G.stream, dd, w, (k & 1) ? db0 : db1,
+                             (k & 1) ? db1 : db0, accb, k, s, seq);
+        }
         launch_named("k_tay_final", k_tay_final<TD>, G.nwork, G.stream, dd, w, accb, s, seq);
     }
 }
@@ -909,7 +920,7 @@ static void seq_body(const Group &G, const SeqForm &f) {
 // scalar slots that start a sequence of n intervals of length dt
 static std::vector<std::pair<int, double>> seq_slots(int n, double dt) {
     return {{S_STEP, 0.0}, {S_NSTEPS, (double)n}, {S_TCALLS, 0.0}, {S_CALLS, 0.0},
-            {S_STATUS, (double)RK_DONE}, {S_DT, dt}};
+            {S_STATUS, (double)RK_DONE}, {S_DT, dt}, {S_TTERMS, 0.0}};
 }
 
 void dev_transient_run(Model &M, const double *power_seq, int steps, double dt, const double *init_temp,

@@ -59,6 +59,7 @@ enum Scal {
     S_CORR,                      // mixed steady: a correction e is pending
     S_TTOL,                      // Taylor: truncation tolerance per sub-step
     S_KPLAN,                     // Taylor: most terms a sub-step may take
+    S_TTERMS,                    // Taylor: term sweeps (E_k -> E_k+1) summed over a sequence
     NSCAL = 48
 };
 enum RkStatus { RK_NEED_K1 = 0, RK_ATTEMPT = 1, RK_DONE = 2 };

@@ -24,7 +24,7 @@ class Options(C.Structure):
 class JobInfo(C.Structure):
     """hs_job_info_t"""
     _fields_ = [("state", C.c_int), ("steady_iters", C.c_int), ("steady_refinements", C.c_int),
-                ("rk4_calls", C.c_int), ("wait_s", C.c_double), ("latency_s", C.c_double),
+                ("rk4_calls", C.c_int), ("taylor_sweeps", C.c_double), ("wait_s", C.c_double), ("latency_s", C.c_double),
                 ("device_bytes", C.c_double)]
 
 

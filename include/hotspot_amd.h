@@ -388,6 +388,7 @@ typedef struct hs_job_info_st {
     int steady_iters;         /* CG iterations (inner ones for mixed precision) */
     int steady_refinements;
     int rk4_calls;            /* RK4 step attempts over the whole sequence */
+    double taylor_sweeps;     /* Taylor term sweeps E_k -> E_k+1 over the whole sequence */
     double wait_s;            /* submission -> admission */
     double latency_s;         /* submission -> result (so far, while running) */
     double device_bytes;      /* device memory held while resident */
