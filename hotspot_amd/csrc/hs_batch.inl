@@ -32,7 +32,7 @@ struct EngJob {
     int nsteps = 0;
     double dt = 0.0;
     int nblk = 0;
-    bool hetero = false;
+    bool hetero = false, adv = false;   // heterogeneous coefficients; advection (non-symmetric G)
     size_t footprint = 0;               // estimate used for admission
     size_t held = 0;                    // device bytes while resident
     std::vector<double> result;         // final block vector
@@ -80,13 +80,13 @@ struct Engine {
     hipStream_t s_steady = nullptr, s_trans = nullptr;
     std::deque<EngJob *> queue;
     std::map<int, std::unique_ptr<EngJob>> jobs;
-    BatchGroup steady[2], trans[2];     // [hetero]
+    // steady: homogeneous CG, heterogeneous CG, non-symmetric (GMRES);
+    // transient: homogeneous, heterogeneous
+    BatchGroup steady[3], trans[2];
     ~Engine() {
         for (auto &kv : jobs) kv.second->D.reset();
-        for (int h = 0; h < 2; h++) {
-            steady[h].release();
-            trans[h].release();
-        }
+        for (auto &g : steady) g.release();
+        for (auto &g : trans) g.release();
         if (s_steady) (void)hipStreamDestroy(s_steady);
         if (s_trans) (void)hipStreamDestroy(s_trans);
     }
@@ -110,6 +110,10 @@ static size_t dev_footprint(const Model &M, int nsteps) {
     b += NSCAL * 8 + sizeof(ProbDesc) + 64;
     if (M.detailed) b += ncell * 176 + (size_t)g.n_layers * 40;   // assembly inputs + per-cell coefficients
     b += 2 * (size_t)nsteps * nblk * 8;
+    bool adv = false;
+    for (auto *m : M.mc) adv |= m != nullptr;
+    if (adv)   // GMRES state, partials and one basis (FP32 or FP64)
+        b += GM_SIZE * 8 + (size_t)(t.ntiles + 1) * (kGmM + 1) * 8 + (size_t)(kGmM + 1) * vlen * 8;
     return b + b / 50 + (64 << 10);
 }
 
@@ -180,6 +184,8 @@ static void release_device(Engine &E, EngJob &j) {
     j.D.reset();
 }
 
+static int steady_kind(const EngJob &j) { return j.adv ? 2 : j.hetero ? 1 : 0; }
+
 static void start_transient(Engine &E, EngJob &j) {
     DevModel &D = *j.D;
     auto kv = seq_slots(j.nsteps, j.dt);
@@ -224,7 +230,10 @@ static void admit(Engine &E, EngJob &j) {
         run_b2g(D.grp, V_POWER, B_RHS, 1);
         const int nbk = (int)((D.vlen + kBlock - 1) / kBlock);
         launch(k_fill, nbk, D.stream, D.h.b64[B_STEADY], D.vlen, M.g->config.ambient);
-        if (E.opt.precision == HS_PREC_MIXED) {
+        if (j.adv) {
+            ensure_gmres(D, E.opt.precision == HS_PREC_MIXED);
+            write_scal(D, kMixedStart);
+        } else if (E.opt.precision == HS_PREC_MIXED) {
             write_scal(D, kMixedStart);
         } else {
             write_scal(D, {{S_DONE, 0.0}});
@@ -233,7 +242,7 @@ static void admit(Engine &E, EngJob &j) {
             else jpcg64_start<false>(D.grp, rtol2, E.opt.max_iters);
         }
         j.phase = J_STEADY;
-        BatchGroup &B = E.steady[j.hetero];
+        BatchGroup &B = E.steady[steady_kind(j)];
         B.members.push_back(&j);
         B.dirty = true;
     } else {
@@ -248,9 +257,9 @@ static void admit(Engine &E, EngJob &j) {
 // the start of the job's transient sequence from them.
 static void steady_done(Engine &E, EngJob &j, const double *flags) {
     DevModel &D = *j.D;
-    const bool mixed = E.opt.precision == HS_PREC_MIXED;
+    const bool refined = E.opt.precision == HS_PREC_MIXED || j.adv;
     j.steady_iters = (int)flags[1];
-    j.refinements = mixed ? (int)flags[2] : 0;
+    j.refinements = refined ? (int)flags[2] : 0;
     run_g2b(D.grp, B_STEADY);
     if (j.nsteps == 0) {
         j.result.resize(D.nblk);
@@ -280,9 +289,15 @@ static void trans_done(Engine &E, EngJob &j, const double *flags) {
 }
 
 template <bool HET>
-static void steady_chunk(Engine &E, const Group &G) {
+static void steady_chunk(Engine &E, const Group &G, bool adv) {
     const double rtol2 = E.opt.steady_rtol * E.opt.steady_rtol;
-    if (E.opt.precision == HS_PREC_MIXED) {
+    if (adv) {   // GMRES cycles inside FP64 refinement
+        const int cycles = std::max(1, E.tick_iters / kGmM);
+        for (int c = 0; c < cycles; c++) {
+            if (E.opt.precision == HS_PREC_MIXED) gm_round<float, true>(G, rtol2, 1e-8, kGmM);
+            else gm_round<double, true>(G, rtol2, 1e-12, kGmM);
+        }
+    } else if (E.opt.precision == HS_PREC_MIXED) {
         for (int done = 0; done < E.tick_iters; done += 64) mixed_round<HET>(G, rtol2, E.opt.max_iters, 64);
     } else {
         cg_iterations<double, HET>(G, E.tick_iters, B_STEADY, B_R, B_Z, B_P0, B_Q, S_DONE, S_TOL2,
@@ -312,7 +327,8 @@ void eng_destroy(Engine *E) { delete E; }
 
 int eng_active(Engine &E) {
     int n = (int)E.queue.size();
-    for (int h = 0; h < 2; h++) n += (int)(E.steady[h].members.size() + E.trans[h].members.size());
+    for (auto &g : E.steady) n += (int)g.members.size();
+    for (auto &g : E.trans) n += (int)g.members.size();
     return n;
 }
 
@@ -343,8 +359,6 @@ int eng_submit(Engine &E, Model &M, const double *power, int rows, const double 
     if (nsteps == 0 && init) fatal("hs_engine_submit: a steady job takes no start temperatures\n");
     bool adv = false;
     for (auto *m : M.mc) adv |= m != nullptr;
-    if (adv && !init)
-        fatal("hs_engine_submit: steady solve of a microchannel model (non-symmetric G) is not supported\n");
     if (g.n_layers > kMaxLayers) fatal("too many layers for the device model\n");
     std::unique_ptr<EngJob> j(new EngJob());
     j->id = E.next_id++;
@@ -359,6 +373,7 @@ int eng_submit(Engine &E, Model &M, const double *power, int rows, const double 
     j->nsteps = nsteps;
     j->dt = dt;
     j->hetero = M.detailed;
+    j->adv = adv;
     j->footprint = dev_footprint(M, nsteps);
     if (j->footprint > E.budget)
         fatal("hs_engine_submit: the job needs " + std::to_string(j->footprint) +
@@ -385,15 +400,16 @@ int eng_tick(Engine &E) {
     eng_admit(E);
     // members launched this tick per group: only their gathered flags are
     // fresh (a group may gain members below, or be empty this tick)
-    int ran_steady[2] = {0, 0}, ran_trans[2] = {0, 0};
-    for (int h = 0; h < 2; h++) {
+    int ran_steady[3] = {0, 0, 0}, ran_trans[2] = {0, 0};
+    for (int h = 0; h < 3; h++) {
         BatchGroup &B = E.steady[h];
         if (B.members.empty()) continue;
-        group_sync(E, B, E.s_steady, h, false);
+        group_sync(E, B, E.s_steady, h > 0, false);
         ran_steady[h] = B.g.nprob;
-        if (h) steady_chunk<true>(E, B.g);
-        else steady_chunk<false>(E, B.g);
-        gather(B, {S_DONE, E.opt.precision == HS_PREC_MIXED ? S_TITERS : S_ITERS, S_NREFINE});
+        if (h) steady_chunk<true>(E, B.g, h == 2);
+        else steady_chunk<false>(E, B.g, false);
+        const bool refined = E.opt.precision == HS_PREC_MIXED || h == 2;
+        gather(B, {S_DONE, refined ? S_TITERS : S_ITERS, S_NREFINE});
     }
     for (int h = 0; h < 2; h++) {
         BatchGroup &B = E.trans[h];
@@ -405,7 +421,7 @@ int eng_tick(Engine &E) {
     }
     HIP_OK(hipStreamSynchronize(E.s_steady));
     HIP_OK(hipStreamSynchronize(E.s_trans));
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < 3; h++) {
         BatchGroup &B = E.steady[h];
         std::vector<std::pair<EngJob *, std::vector<double>>> fin;
         for (int i = 0; i < ran_steady[h]; i++)

@@ -726,6 +726,19 @@ __global__ void __launch_bounds__(kBlock) k_ir_resid(const ProbDesc *__restrict_
     write_partials<3>(d, wi.tile, v);
 }
 
+// diagonal of G at a package node: ring and coupling conductances + ambient
+template <typename T> __device__ __forceinline__ T pkg_diag(const ProbDesc &d, int k) {
+    const int L_ = d.x_ring_layer[k];
+    T diag = (T)d.x_gamb[k];
+    if (L_ >= 0) diag += (T)d.x_ring_g[k] * (T)(d.x_ring_edge[k] < 2 ? d.nc : d.nr);
+    for (int q = 0; q < d.x_ncpl[k]; q++) diag += (T)d.x_cpl_g[k][q];
+    return diag;
+}
+
+template <typename T> __device__ __forceinline__ T *vec_of(const ProbDesc &d, int b) {
+    return sizeof(T) == 8 ? (T *)d.b64[b] : (T *)d.b32[b];
+}
+
 // x (FP64) += e (FP32), for problems whose inner solve finished with a
 // correction pending
 __global__ void __launch_bounds__(kBlock) k_ir_correct(const ProbDesc *__restrict__ D,
@@ -892,11 +905,308 @@ __global__ void __launch_bounds__(kBlock) k_rk_control(const ProbDesc *__restric
 
 // Finish kernels of the CG / refinement / Taylor reductions: one workgroup
 // per problem, same skip condition as the producer kernel.
-enum FinOp { FIN_CG_INIT = 0, FIN_ALPHA, FIN_BETA, FIN_IR, FIN_TAY };
+enum FinOp { FIN_CG_INIT = 0, FIN_ALPHA, FIN_BETA, FIN_IR, FIN_TAY, FIN_GM_IR, FIN_GM_H, FIN_GM_NORM };
 struct FinArgs {
     double rtol2, inner_rtol2;
     int max_iters, max_refine, done_slot, tol_slot, sub, seq;
 };
+
+// ---------------------------------------------------------------------------
+// steady, non-symmetric G (coolant advection): restarted GMRES(kGmM), right-
+// preconditioned with the Jacobi diagonal, as the inner solve of FP64
+// iterative refinement (one GMRES cycle from e = 0 per refinement, on the
+// FP64 residual).  The basis is kept unnormalised: v_i = scale_i * vhat_i.
+// Per iteration j: w = G z_j / scale_j (z_j = diag^-1 v_j); h_ij = (w, v_i) /
+// scale_i; v_j+1 = w - sum_i h_ij v_i / scale_i, z_j+1 = diag^-1 v_j+1,
+// h_j+1,j = ||v_j+1||; Givens rotations and the least-squares solve run in
+// the problem's finish workgroup.  Per-problem state (H, rotations, g, y,
+// scales, j) lives in ProbDesc::gm.
+// ---------------------------------------------------------------------------
+enum GmState {
+    GM_J = 0, GM_K, GM_TOL,
+    GM_SCALE = 4,                        // [kGmM + 1]
+    GM_G = GM_SCALE + kGmM + 1,          // [kGmM + 1]
+    GM_CS = GM_G + kGmM + 1,             // [kGmM]
+    GM_SN = GM_CS + kGmM,                // [kGmM]
+    GM_Y = GM_SN + kGmM,                 // [kGmM]
+    GM_H = GM_Y + kGmM,                  // column-major [(kGmM + 1) x kGmM]
+    GM_SIZE = GM_H + (kGmM + 1) * kGmM
+};
+
+template <typename T> __device__ __forceinline__ T *gm_vec(const ProbDesc &d, int i) {
+    const size_t off = (size_t)i * d.vlen;
+    return sizeof(T) == 8 ? (T *)(d.gmv64 + off) : (T *)(d.gmv32 + off);
+}
+
+__device__ __forceinline__ bool gm_skip(const ProbDesc &d) {
+    return problem_skip(d, S_DONE) || d.scal[S_IDONE] != 0.0;
+}
+
+template <typename T, bool HET> __device__ __forceinline__ T node_dinv(const ProbDesc &d, const Cell &q) {
+    return HET ? ld_at(Coef<T>::dinv(d), q.c) : cell_dinv<T, false>(d, q);
+}
+
+// FP64 residual r = rhs - G x -> v_0 = r, z = diag^-1 r (T); partials
+// (T(r), T(r)), (r, r), (rhs, rhs).
+template <typename T, bool HET>
+__global__ void __launch_bounds__(kBlock) k_gm_resid(const ProbDesc *__restrict__ D,
+                                                     const WorkItem *__restrict__ W, int zb) {
+    const WorkItem wi = W[work_index()];
+    const ProbDesc &d = D[wi.prob];
+    if (problem_skip(d, S_DONE) || d.scal[S_IDONE] == 0.0) return;   // converged / cycle running
+    const double *x = d.b64[B_STEADY], *rhs = d.b64[B_RHS];
+    T *v0 = gm_vec<T>(d, 0), *z = vec_of<T>(d, zb);
+    LdPlain<double> ld{x};
+    double w[3] = {0, 0, 0};
+    auto put = [&](unsigned c, double rr, T dinv, double bc) {
+        const T rt = (T)rr;
+        st_at(v0, c, rt);
+        st_at(z, c, rt * dinv);
+        w[0] += (double)rt * rt;
+        w[1] += rr * rr;
+        w[2] += bc * bc;
+    };
+    if (wi.tile < d.ntiles) {
+        double ring[4];
+        load_ring<double>(d, ld, tile_layer(d, wi.tile), ring);
+        struct V {
+            Pts<double, HET> p;
+            double b;
+            T di;
+        };
+        tile_loop<InFlight<double, HET>::value>(
+            d, wi.tile,
+            [&](const Cell &q) {
+                V a{};
+                load_pts<double, HET>(d, ld, q, a.p);
+                a.b = ld_at(rhs, q.c);
+                a.di = node_dinv<T, HET>(d, q);
+                return a;
+            },
+            [&](const Cell &q, const V &a) {
+                double flux, diag, gam;
+                stencil<double, HET>(d, q, a.p, ring, flux, diag, gam);
+                put(q.c, a.b + flux - gam * a.p.c, a.di, a.b);
+            });
+    } else {
+        __shared__ double xf[kMaxExtra], xd[kMaxExtra];
+        package_terms<double>(d, ld, false, xf, xd);
+        const int k = threadIdx.x;
+        if (k < d.nx) {
+            const unsigned c = (unsigned)d.nl * d.nr * d.nc + k;
+            put(c, rhs[c] + xf[k] - d.x_gamb[k] * x[c], T(1) / pkg_diag<T>(d, k), rhs[c]);
+        }
+    }
+    block_reduce<3>(w);
+    write_partials<3>(d, wi.tile, w);
+}
+
+// w = G z / scale_j
+template <typename T, bool HET>
+__global__ void __launch_bounds__(kBlock) k_gm_apply(const ProbDesc *__restrict__ D,
+                                                     const WorkItem *__restrict__ W, int zb, int wb) {
+    const WorkItem wi = W[work_index()];
+    const ProbDesc &d = D[wi.prob];
+    if (gm_skip(d)) return;
+    const T *z = vec_of<T>(d, zb);
+    T *out = vec_of<T>(d, wb);
+    const T inv = (T)(1.0 / d.gm[GM_SCALE + (int)d.gm[GM_J]]);
+    LdPlain<T> ld{z};
+    if (wi.tile < d.ntiles) {
+        T ring[4];
+        load_ring<T>(d, ld, tile_layer(d, wi.tile), ring);
+        tile_loop<InFlight<T, HET>::value>(
+            d, wi.tile,
+            [&](const Cell &q) {
+                Pts<T, HET> p{};
+                load_pts<T, HET>(d, ld, q, p);
+                return p;
+            },
+            [&](const Cell &q, const Pts<T, HET> &p) {
+                T flux, diag, gam;
+                stencil<T, HET>(d, q, p, ring, flux, diag, gam);
+                st_at(out, q.c, (gam * p.c - flux) * inv);
+            });
+    } else {
+        __shared__ T xf[kMaxExtra], xd[kMaxExtra];
+        package_terms<T>(d, ld, false, xf, xd);
+        const int k = threadIdx.x;
+        if (k < d.nx) {
+            const unsigned c = (unsigned)d.nl * d.nr * d.nc + k;
+            out[c] = ((T)d.x_gamb[k] * z[c] - xf[k]) * inv;
+        }
+    }
+}
+
+// partials (w, v_i), i = 0..j, into the problem's GMRES slots
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_gm_dots(const ProbDesc *__restrict__ D,
+                                                    const WorkItem *__restrict__ W, int wb) {
+    const WorkItem wi = W[work_index()];
+    const ProbDesc &d = D[wi.prob];
+    if (gm_skip(d)) return;
+    const int j = (int)d.gm[GM_J];
+    const T *w = vec_of<T>(d, wb);
+    T wr[8];
+    unsigned cr[8];
+    bool ok[8];
+    const bool pkg = wi.tile >= d.ntiles;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        if (pkg) {
+            ok[r] = r == 0 && threadIdx.x < d.nx;
+            cr[r] = (unsigned)d.nl * d.nr * d.nc + (ok[r] ? threadIdx.x : 0);
+        } else {
+            const Cell q = tile_cell(d, wi.tile, r);
+            ok[r] = q.active;
+            cr[r] = q.c;
+        }
+        wr[r] = ok[r] ? ld_at(w, cr[r]) : T(0);
+    }
+    __shared__ double red[kBlock / kWave][kGmM + 1];
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    for (int i = 0; i <= j; i++) {
+        const T *vi = gm_vec<T>(d, i);
+        double a = 0.0;
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+            if (ok[r]) a += (double)wr[r] * ld_at(vi, cr[r]);
+        a = wave_sum(a);
+        if (lane == 0) red[wv][i] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x <= j) {
+        double s = 0.0;
+        for (int q = 0; q < kBlock / kWave; q++) s += red[q][threadIdx.x];
+        d.gpart[(size_t)wi.tile * (kGmM + 1) + threadIdx.x] = s;
+    }
+}
+
+// v_j+1 = w - sum_i h_ij v_i / scale_i, z = diag^-1 v_j+1; partial ||v_j+1||^2
+template <typename T, bool HET>
+__global__ void __launch_bounds__(kBlock) k_gm_orth(const ProbDesc *__restrict__ D,
+                                                    const WorkItem *__restrict__ W, int wb, int zb) {
+    const WorkItem wi = W[work_index()];
+    const ProbDesc &d = D[wi.prob];
+    if (gm_skip(d)) return;
+    const int j = (int)d.gm[GM_J];
+    const T *w = vec_of<T>(d, wb);
+    T *vn = gm_vec<T>(d, j + 1), *z = vec_of<T>(d, zb);
+    double acc[1] = {0.0};
+    auto put = [&](unsigned c, T dinv) {
+        T x = ld_at(w, c);
+        for (int i = 0; i <= j; i++) {
+            const T hc = (T)(d.gm[GM_H + j * (kGmM + 1) + i] / d.gm[GM_SCALE + i]);
+            x -= hc * ld_at(gm_vec<T>(d, i), c);
+        }
+        st_at(vn, c, x);
+        st_at(z, c, x * dinv);
+        acc[0] += (double)x * x;
+    };
+    if (wi.tile < d.ntiles) {
+        tile_loop<1>(
+            d, wi.tile, [&](const Cell &q) { return node_dinv<T, HET>(d, q); },
+            [&](const Cell &q, T dinv) { put(q.c, dinv); });
+    } else if (threadIdx.x < d.nx) {
+        const int k = threadIdx.x;
+        put((unsigned)d.nl * d.nr * d.nc + k, T(1) / pkg_diag<T>(d, k));
+    }
+    block_reduce<1>(acc);
+    if (threadIdx.x == 0) d.gpart[(size_t)wi.tile * (kGmM + 1)] = acc[0];
+}
+
+// x (FP64) += diag^-1 sum_i y_i v_i / scale_i, for problems whose cycle
+// finished with a correction pending
+template <typename T, bool HET>
+__global__ void __launch_bounds__(kBlock) k_gm_correct(const ProbDesc *__restrict__ D,
+                                                       const WorkItem *__restrict__ W) {
+    const WorkItem wi = W[work_index()];
+    const ProbDesc &d = D[wi.prob];
+    if (problem_skip(d, S_DONE) || d.scal[S_IDONE] == 0.0 || d.scal[S_CORR] == 0.0) return;
+    const int k = (int)d.gm[GM_K];
+    double *x = d.b64[B_STEADY];
+    auto put = [&](unsigned c, T dinv) {
+        T e = T(0);
+        for (int i = 0; i < k; i++)
+            e += (T)(d.gm[GM_Y + i] / d.gm[GM_SCALE + i]) * ld_at(gm_vec<T>(d, i), c);
+        st_at(x, c, ld_at(x, c) + (double)(e * dinv));
+    };
+    if (wi.tile < d.ntiles) {
+        tile_loop<1>(
+            d, wi.tile, [&](const Cell &q) { return node_dinv<T, HET>(d, q); },
+            [&](const Cell &q, T dinv) { put(q.c, dinv); });
+    } else if (threadIdx.x < d.nx) {
+        const int kk = threadIdx.x;
+        put((unsigned)d.nl * d.nr * d.nc + kk, T(1) / pkg_diag<T>(d, kk));
+    }
+}
+
+// per-problem GMRES bookkeeping (one thread): start of a cycle, the
+// Hessenberg column of an iteration, its rotation and the stopping test
+__device__ void gm_finish_resid(const ProbDesc &d, const double (&v)[3], const FinArgs &a) {
+    double *s = d.scal, *g = d.gm;
+    s[S_RR] = v[1];
+    s[S_BB] = v[2];
+    s[S_TOL2] = a.rtol2 * v[2];
+    s[S_TITERS] += s[S_ITERS];
+    s[S_ITERS] = 0.0;
+    const double nv = sqrt(v[0]);
+    const bool done = v[1] <= s[S_TOL2] || s[S_NREFINE] >= a.max_refine || !(v[1] == v[1]) || nv == 0.0;
+    s[S_DONE] = done ? 1.0 : 0.0;
+    s[S_IDONE] = done ? 1.0 : 0.0;
+    s[S_CORR] = done ? 0.0 : 1.0;
+    s[S_NREFINE] += 1.0;
+    g[GM_J] = 0.0;
+    g[GM_K] = 0.0;
+    g[GM_TOL] = sqrt(a.inner_rtol2) * nv;
+    g[GM_SCALE] = nv > 0.0 ? nv : 1.0;
+    for (int i = 0; i <= kGmM; i++) g[GM_G + i] = 0.0;
+    g[GM_G] = nv;
+}
+
+__device__ void gm_finish_norm(const ProbDesc &d, double hn) {
+    double *s = d.scal, *g = d.gm;
+    const int j = (int)g[GM_J];
+    double *h = g + GM_H + j * (kGmM + 1);
+    h[j + 1] = hn;
+    g[GM_SCALE + j + 1] = hn > 0.0 ? hn : 1.0;
+    for (int i = 0; i < j; i++) {
+        const double c = g[GM_CS + i], sn = g[GM_SN + i];
+        const double a0 = h[i], a1 = h[i + 1];
+        h[i] = c * a0 + sn * a1;
+        h[i + 1] = -sn * a0 + c * a1;
+    }
+    const double rr = hypot(h[j], h[j + 1]);
+    const double c = rr > 0.0 ? h[j] / rr : 1.0, sn = rr > 0.0 ? h[j + 1] / rr : 0.0;
+    g[GM_CS + j] = c;
+    g[GM_SN + j] = sn;
+    h[j] = rr;
+    h[j + 1] = 0.0;
+    g[GM_G + j + 1] = -sn * g[GM_G + j];
+    g[GM_G + j] = c * g[GM_G + j];
+    s[S_ITERS] += 1.0;
+    const int k = j + 1;
+    g[GM_J] = k;
+    if (fabs(g[GM_G + k]) <= g[GM_TOL] || k == kGmM || hn == 0.0 || !(hn == hn)) {
+        // least squares: R y = g (upper triangular, k x k)
+        for (int i = k - 1; i >= 0; i--) {
+            double t = g[GM_G + i];
+            for (int l = i + 1; l < k; l++) t -= g[GM_H + l * (kGmM + 1) + i] * g[GM_Y + l];
+            const double dii = g[GM_H + i * (kGmM + 1) + i];
+            g[GM_Y + i] = dii != 0.0 ? t / dii : 0.0;
+        }
+        g[GM_K] = k;
+        s[S_IDONE] = 1.0;
+    }
+}
+
+// Whole workgroup: sum of one GMRES partial over a problem's slots, in slot order
+__device__ __forceinline__ void reduce_gslots(const ProbDesc &d, int i, double (&v)[1]) {
+    const int nslots = d.ntiles + 1;
+    v[0] = 0.0;
+    for (int t = threadIdx.x; t < nslots; t += kBlock) v[0] += d.gpart[(size_t)t * (kGmM + 1) + i];
+    block_reduce<1>(v);
+}
 
 __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ D, int op, FinArgs a) {
     const ProbDesc &d = D[blockIdx.x];
@@ -958,6 +1268,31 @@ __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ 
             s[S_IDONE] = done ? 1.0 : 0.0;
             s[S_CORR] = done ? 0.0 : 1.0;
             s[S_NREFINE] += 1.0;
+            return;
+        }
+        case FIN_GM_IR: {
+            if (s[S_DONE] != 0.0 || s[S_IDONE] == 0.0) return;
+            reduce_slots<3>(d, v);
+            if (threadIdx.x) return;
+            gm_finish_resid(d, v, a);
+            return;
+        }
+        case FIN_GM_H: {
+            if (gm_skip(d)) return;
+            const int j = (int)d.gm[GM_J];
+            for (int i = 0; i <= j; i++) {
+                double w[1];
+                reduce_gslots(d, i, w);
+                if (threadIdx.x == 0) d.gm[GM_H + j * (kGmM + 1) + i] = w[0] / d.gm[GM_SCALE + i];
+            }
+            return;
+        }
+        case FIN_GM_NORM: {
+            if (gm_skip(d)) return;
+            double w[1];
+            reduce_gslots(d, 0, w);
+            if (threadIdx.x) return;
+            gm_finish_norm(d, sqrt(w[0]));
             return;
         }
         case FIN_TAY: {

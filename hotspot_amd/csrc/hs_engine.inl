@@ -435,6 +435,7 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
     h.map_mode = g.map_mode;
     const long ncell = M.ncell();
     D->vlen = ((ncell + M.nx + 63) / 64) * 64;
+    h.vlen = (unsigned)D->vlen;
     for (int b = 0; b < NB64; b++) {
         h.b64[b] = D->alloc<double>(D->vlen);
         HIP_OK(hipMemset(h.b64[b], 0, D->vlen * sizeof(double)));
@@ -617,6 +618,51 @@ static void jpcg64_start(const Group &G, double rtol2, int max_iters) {
     finish(G, FIN_CG_INIT, a);
 }
 
+// GMRES state and basis of a problem, allocated on its first non-symmetric
+// steady solve in the given precision.
+static void ensure_gmres(DevModel &D, bool f32) {
+    bool changed = false;
+    if (!D.h.gm) {
+        D.h.gm = D.alloc<double>(GM_SIZE);
+        D.h.gpart = D.alloc<double>((size_t)(D.h.ntiles + 1) * (kGmM + 1));
+        changed = true;
+    }
+    if (f32 && !D.h.gmv32) {
+        D.h.gmv32 = D.alloc<float>((size_t)(kGmM + 1) * D.vlen);
+        changed = true;
+    }
+    if (!f32 && !D.h.gmv64) {
+        D.h.gmv64 = D.alloc<double>((size_t)(kGmM + 1) * D.vlen);
+        changed = true;
+    }
+    if (changed) HIP_OK(hipMemcpy(D.d_desc, &D.h, sizeof(ProbDesc), hipMemcpyHostToDevice));
+}
+
+// One refinement round of the non-symmetric solve: problems whose GMRES
+// cycle has finished take its correction and start the next cycle from a
+// fresh FP64 residual (or stop); then `iters` Arnoldi iterations.
+template <typename T, bool HET>
+static void gm_round(const Group &G, double rtol2, double inner2, int iters) {
+    const ProbDesc *dd = G.d_desc;
+    const WorkItem *w = G.d_work;
+    const bool f32 = sizeof(T) == 4;
+    const int zb = f32 ? F_Z : B_Z, wb = f32 ? F_Q : B_Q;
+    launch_named("k_gm_correct", k_gm_correct<T, HET>, G.nwork, G.stream, dd, w);
+    launch_named("k_gm_resid", k_gm_resid<T, HET>, G.nwork, G.stream, dd, w, zb);
+    FinArgs a{};
+    a.rtol2 = rtol2;
+    a.inner_rtol2 = inner2;
+    a.max_refine = 2000;
+    finish(G, FIN_GM_IR, a);
+    for (int k = 0; k < iters; k++) {
+        launch_named("k_gm_apply", k_gm_apply<T, HET>, G.nwork, G.stream, dd, w, zb, wb);
+        launch_named("k_gm_dots", k_gm_dots<T>, G.nwork, G.stream, dd, w, wb);
+        finish(G, FIN_GM_H, a);
+        launch_named("k_gm_orth", k_gm_orth<T, HET>, G.nwork, G.stream, dd, w, wb, zb);
+        finish(G, FIN_GM_NORM, a);
+    }
+}
+
 // Mixed-precision refinement, one round: problems whose inner FP32 solve has
 // finished take its correction and start the next one from a fresh FP64
 // residual (or stop); then `chunk` inner iterations for every problem.
@@ -673,11 +719,28 @@ static void steady_mixed(Model &M, DevModel &D) {
     M.stats.steady_rel_resid = D.h_scal[S_BB] > 0 ? std::sqrt(D.h_scal[S_RR] / D.h_scal[S_BB]) : 0.0;
 }
 
+// Non-symmetric G (advection): FP64 refinement around GMRES(kGmM) cycles in
+// FP64 (each to a 1e-6 relative residual) or FP32 (1e-4).
+template <typename T, bool HET>
+static void steady_gmres(Model &M, DevModel &D) {
+    ensure_gmres(D, sizeof(T) == 4);
+    const double rtol2 = M.opt.steady_rtol * M.opt.steady_rtol;
+    const double inner2 = sizeof(T) == 4 ? 1e-8 : 1e-12;
+    write_scal(D, kMixedStart);
+    for (int guard = 0;; guard++) {
+        gm_round<T, HET>(D.grp, rtol2, inner2, kGmM);
+        read_scal(D);
+        if (D.h_scal[S_DONE] != 0.0) break;
+        if (guard > 100000) fatal("non-symmetric steady solve did not finish\n");
+    }
+    M.stats.steady_iters = (int)D.h_scal[S_TITERS];
+    M.stats.steady_refinements = (int)D.h_scal[S_NREFINE];
+    M.stats.steady_rel_resid = D.h_scal[S_BB] > 0 ? std::sqrt(D.h_scal[S_RR] / D.h_scal[S_BB]) : 0.0;
+}
+
 void dev_steady(Model &M, const double *block_power, double *block_temp) {
     dev_prepare(M);
     DevModel &D = *M.dev;
-    if (D.h.has_adv) fatal("steady solve of a microchannel model: the advection terms make G non-symmetric, "
-                           "which the CG solvers do not handle\n");
     HIP_OK(hipEventRecord(D.ev0, D.stream));
     upload_blocks(D, block_power);
     run_b2g(D.grp, V_POWER, B_RHS, 1);
@@ -686,7 +749,10 @@ void dev_steady(Model &M, const double *block_power, double *block_temp) {
         launch(k_fill, nb, D.stream, D.h.b64[B_STEADY], D.vlen, M.g->config.ambient);
     }
     write_scal(D, {{S_DONE, 0.0}});
-    if (M.opt.precision == HS_PREC_MIXED) {
+    if (D.h.has_adv) {
+        if (M.opt.precision == HS_PREC_MIXED) steady_gmres<float, true>(M, D);
+        else steady_gmres<double, true>(M, D);
+    } else if (M.opt.precision == HS_PREC_MIXED) {
         if (D.hetero) steady_mixed<true>(M, D);
         else steady_mixed<false>(M, D);
     } else {

@@ -24,6 +24,7 @@ namespace hs {
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kRed = 4;   // values reduced per slot
+constexpr int kGmM = 32;  // GMRES restart length (non-symmetric steady solves)
 
 // FP64 buffers of a problem
 enum Buf64 {
@@ -62,6 +63,7 @@ enum Scal {
     S_TTERMS,                    // Taylor: term sweeps (E_k -> E_k+1) summed over a sequence
     NSCAL = 48
 };
+static_assert(S_TTERMS < NSCAL, "scalar slots");
 enum RkStatus { RK_NEED_K1 = 0, RK_ATTEMPT = 1, RK_DONE = 2 };
 
 struct ProbDesc {
@@ -110,6 +112,12 @@ struct ProbDesc {
     double *seq_in;                     // [steps][nblocks + nx] power sequence
     double *seq_out;                    // [steps][nblocks + nx] temperatures
     double *bconst;                     // per-cell constant source (inlet), hetero only
+    // GMRES (non-symmetric steady solves; allocated on first use)
+    double *gm;                         // per-problem state (GmState)
+    double *gpart;                      // [ntiles+1][kGmM+1] partials
+    float *gmv32;                       // [kGmM+1][vlen] basis, FP32 inner solves
+    double *gmv64;                      // [kGmM+1][vlen] basis, FP64 inner solves
+    unsigned vlen, pad2;
     // reduction state
     double *partial;                    // [ntiles+1][kRed]
     unsigned *counter;

@@ -280,9 +280,10 @@ void hs_set_fatal_mode(int exit_on_error);
 #define HS_PREC_FP64  0   /* every sweep in FP64 */
 #define HS_PREC_MIXED 1   /* FP32 sweeps inside FP64 refinement / FP64 residuals */
 /* steady solvers */
-#define HS_STEADY_JPCG  0 /* Jacobi-preconditioned CG (G symmetric positive definite; a
-                           * microchannel model's advection makes G non-symmetric: its
-                           * steady solve is refused with an error) */
+#define HS_STEADY_JPCG  0 /* Jacobi-preconditioned CG where G is symmetric; where coolant
+                           * advection (microchannel layers) makes G non-symmetric, restarted
+                           * GMRES(32) cycles, Jacobi right-preconditioned, inside FP64
+                           * iterative refinement */
 /* transient integrators */
 #define HS_TRANS_RK4    0 /* the reference's adaptive step-doubling RK4 */
 #define HS_TRANS_TAYLOR 1 /* exponential (Taylor) integrator of the linear ODE, error bounded a priori */

@@ -229,12 +229,29 @@ def test_engine_refuses_what_it_cannot_do(lib, jobs):
     eng.set_budget(1e10)
     with pytest.raises(RuntimeError, match="power rows"):
         eng.submit(j.run.model, np.vstack([j.P, j.P]), n_steps=5, dt=DT)
-    mc = make(lib, cases.microchannel_case(16, 5, __import__("tempfile").mkdtemp())[0])
-    with pytest.raises(RuntimeError, match="microchannel"):
-        eng.submit(mc.model, np.zeros(mc.n), n_steps=0)
-    # a transient from given temperatures is fine for a microchannel model
-    i = eng.submit(mc.model, np.zeros(mc.n), init=np.full(mc.n, 320.0), n_steps=2, dt=DT)
-    eng.run()
-    assert eng.poll(i) is not None
     eng.close()
+
+
+@pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
+def test_engine_microchannel_steady_equals_single_model_api(lib, form):
+    """Non-symmetric (microchannel) steady jobs run GMRES cycles in the
+    engine's third steady group; the result equals the model's own solve."""
+    import tempfile
+    case, pw = cases.microchannel_case(16, 5, tempfile.mkdtemp())
+    mc = make(lib, case)
+    P = mc.power_vector(pw)
+    eng = engine.Engine(lib, *form)
+    i = eng.submit(mc.model, P, n_steps=0)
+    k = eng.submit(mc.model, P, n_steps=4, dt=DT)
+    eng.run()
+    r, info = eng.poll(i), eng.info(i)
+    seq = eng.result_seq(k)
+    eng.close()
+    assert info.steady_iters > 0 and info.steady_refinements > 0
+    set_options(lib, mc, precision=form[0], transient=form[1])
+    T = mc.steady(P.copy())
+    check_error(lib)
+    assert np.array_equal(T, r)
+    s = transient_run(lib, mc, np.array([P] * 4), T)
+    assert np.array_equal(s, seq)
     mc.free()

@@ -110,6 +110,27 @@ def test_slope_and_maps_match_oracle(lib, name, case, pw):
     run.free()
 
 
+@pytest.mark.parametrize("prec,tol", [(0, FP64_TOL), (1, MIXED_TOL)], ids=["fp64", "mixed"])
+def test_microchannel_steady_gmres(lib, prec, tol):
+    """Advection makes G non-symmetric: the steady solve runs restarted
+    GMRES cycles (FP64 or FP32) inside FP64 refinement and is checked against
+    the oracle's direct sparse solve of the same operator."""
+    case, pw = _mc_case()
+    run = make(lib, case)
+    set_options(lib, run, precision=prec)
+    om = oracle_for(run)
+    assert not om.symmetric
+    P = run.power_vector(pw)
+    T = run.steady(P)
+    check_error(lib)
+    To, Tg = om.steady(P)
+    assert np.abs(T - To).max() <= tol
+    assert np.abs(grid_temp(lib, run, 0) - Tg).max() <= tol
+    st = stats(lib, run)
+    assert st.steady_rel_resid <= 1e-10 and st.steady_iters > 0
+    run.free()
+
+
 @pytest.mark.parametrize("name,case,pw", [CASES[0], CASES[1], CASES[3], ("microchannel-16",) + _mc_case()],
                          ids=["default-64", "secondary-16", "3d-stack-32", "microchannel-16"])
 def test_transient_rk4_matches_oracle(lib, name, case, pw):

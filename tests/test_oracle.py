@@ -225,3 +225,17 @@ def test_rk4_fixed_point_and_order():
     yh = om.rk4_core(y0, om.slope(y0, Pg), Pg, 1e-5)
     e2 = np.abs(om.rk4_core(yh, om.slope(yh, Pg), Pg, 1e-5) - ex).max()
     assert 8 < e1 / e2 < 40
+
+
+def test_microchannel_direct_solve_satisfies_the_operator():
+    """The reference cannot solve microchannel steady states without SuperLU;
+    the oracle's direct solve of its (golden-pinned) non-symmetric operator is
+    pinned by its residual and by energy balance with the coolant."""
+    om, P, d = _load("microchannel16")
+    assert not om.symmetric
+    Pg = om.power_grid(P)
+    Tg = om.steady_grid(Pg)
+    r = om.residual(Tg, Pg)
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(Pg + om.b)
+    asym = abs(om.G - om.G.T).max()
+    assert asym > 1e-6 * abs(om.G).max()
