@@ -186,8 +186,150 @@ def served(lib, jobs, intervals, tick, keep_first=False, timer=None):
     return t1 - t0, t0 - ta, infos, first
 
 
+def mode_many(lib, args):
+    """Batched workload: uneven problems (grids drawn from 32^2..256^2, 2-4
+    chip layers over spreader and sink, 8-128 slicing blocks, powers
+    U(0.1, 10) W), each a steady solve then `--many-steps` intervals, all
+    submitted together to the engine (mixed precision, Taylor); inputs
+    admitted before the timed region."""
+    g = cases.rng(41)
+    jobs = []
+    log(f"building {args.many} problems")
+    for k in range(args.many):
+        n = int(g.choice([32, 64, 128, 256]))
+        cl = int(g.integers(2, 5))
+        nb = int(g.choice([8, 16, 32, 64, 128]))
+        case, pw = cases.chip_stack_case(n, seed=1000 + k, chip_layers=cl, n_blocks=nb, power=(0.1, 10.0))
+        run = cases.Run(lib, case, tempfile.mkdtemp(prefix="hsmany"))
+        P = run.power_vector(pw)
+        seq = np.vstack([P[None, :], P[None, :] * cases.rng(2000 + k).uniform(0.8, 1.2, (args.many_steps, run.n))])
+        gr = run.grid
+        jobs.append((run, np.ascontiguousarray(seq), gr.n_layers * gr.rows * gr.cols))
+    cells = sum(c for _, _, c in jobs)
+    updates = sum(c * (1 + args.many_steps) for _, _, c in jobs)
+    res = {}
+    forms = [("mixed", engine.MIXED, engine.TAYLOR)] + ([] if args.no_fp64 else [("fp64", engine.FP64, engine.RK4)])
+    for name, prec, trans in forms:
+        eng = engine.Engine(lib, prec, trans)
+        eng.set_tick(args.tick, 128)
+        ids = [eng.submit(r.model, seq, n_steps=args.many_steps, dt=DT) for r, seq, _ in jobs]
+        if eng.admit() != 0:
+            raise RuntimeError("the batch does not fit the engine's device-memory budget")
+        device_sync()
+        t0 = time.perf_counter()
+        eng.run()
+        t = time.perf_counter() - t0
+        lat = [eng.info(i).latency_s for i in ids]
+        finals = [eng.poll(i) for i in ids]
+        for i in ids:
+            eng.release(i)
+        eng.close()
+        res[name] = dict(value=updates / t, seconds=t, latency_ms_p50=float(np.percentile(lat, 50)) * 1e3,
+                         latency_ms_p99=float(np.percentile(lat, 99)) * 1e3, finals=finals)
+        log(f"{name}: {len(jobs)} problems in {t:.2f} s, {updates / t:.3e} cell-updates/s")
+    err = None
+    if "fp64" in res:
+        err = float(max(np.abs(a - b).max() for a, b in zip(res["mixed"]["finals"], res["fp64"]["finals"])))
+    for v in res.values():
+        v.pop("finals")
+    for r, _, _ in jobs:
+        r.free()
+    return {"metric": "cell-updates/s (batched uneven problems, mixed precision)", "mode": "many",
+            "value": res["mixed"]["value"], "unit": "cell-updates/s", "n_gpus": 1,
+            "config": {"workload": f"{args.many} problems, grids 32^2-256^2, 4-6 layers, 8-128 blocks, "
+                                   f"steady + {args.many_steps} intervals of 3.333us each",
+                       "cells_total": cells, "problems": args.many},
+            "forms": res, "max_abs_diff_K_mixed_vs_fp64": err,
+            "speedup_vs_fp64": (res["mixed"]["value"] / res["fp64"]["value"]) if "fp64" in res else None}
+
+
+def mode_serve(lib, args):
+    """Served stream: seeded Poisson arrivals (rate --serve-rate) over
+    --serve-seconds on a pool of resident models (grids 64^2..512^2, 4-8
+    layers, 10 % microchannel stacks; 256-block floorplans with 8 hotspots
+    at 10x the mean density 0.5 W/mm^2); 60 % steady solves, 40 % transient
+    sequences of U{100..10000} intervals scaled by --serve-steps-scale.
+    Requests are submitted when they arrive (wall clock) and the engine
+    ticks continuously; latency is submission -> result."""
+    g = cases.rng(77)
+    pool = []
+    log(f"building a pool of {args.serve_pool} resident models")
+    for k in range(args.serve_pool):
+        n = int(g.choice([64, 128, 256, 512]))
+        if k % 10 == 9:
+            d = tempfile.mkdtemp(prefix="hsmc")
+            case, pw = cases.microchannel_case(n, 300 + k, d, die=16e-3, width_cells=2, pitch_cells=4)
+            run = cases.Run(lib, case, d)
+        else:
+            cl = int(g.integers(2, 7))
+            case, pw = cases.chip_stack_case(n, seed=500 + k, chip_layers=cl, n_blocks=256,
+                                             power=("density", 0.5), hotspots=8)
+            run = cases.Run(lib, case, tempfile.mkdtemp(prefix="hsserve"))
+        gr = run.grid
+        pool.append((run, run.power_vector(pw), gr.n_layers * gr.rows * gr.cols))
+    # the request stream
+    t, reqs = 0.0, []
+    while True:
+        t += g.exponential(1.0 / args.serve_rate)
+        if t > args.serve_seconds:
+            break
+        m = int(g.integers(len(pool)))
+        steps = 0 if g.random() < 0.6 else max(1, int(round(int(g.integers(100, 10001)) * args.serve_steps_scale)))
+        reqs.append((t, m, steps, pool[m][1] * g.uniform(0.8, 1.2, size=pool[m][1].shape)))
+    updates = sum(pool[m][2] * (steps if steps else 1) for _, m, steps, _ in reqs)
+    log(f"{len(reqs)} requests, {sum(1 for r in reqs if r[2])} transient, {updates:.3e} cell-updates")
+    out = {}
+    forms = [("mixed", engine.MIXED, engine.TAYLOR)] + ([] if args.no_fp64 else [("fp64", engine.FP64, engine.RK4)])
+    for name, prec, trans in forms:
+        eng = engine.Engine(lib, prec, trans)
+        eng.set_tick(args.tick, 128)
+        ids, k = [], 0
+        t0 = time.perf_counter()
+        while True:
+            now = time.perf_counter() - t0
+            while k < len(reqs) and reqs[k][0] <= now:
+                _, m, steps, P = reqs[k]
+                ids.append(eng.submit(pool[m][0].model, P, n_steps=steps, dt=DT))
+                k += 1
+            active = eng.tick()
+            if k == len(reqs) and active == 0:
+                break
+            if active == 0 and k < len(reqs):
+                time.sleep(max(0.0, reqs[k][0] - (time.perf_counter() - t0)))
+        wall = time.perf_counter() - t0
+        lat = np.array([eng.info(i).latency_s for i in ids])
+        steady_lat = np.array([eng.info(i).latency_s for i, r in zip(ids, reqs) if not r[2]])
+        out[name] = dict(value=updates / wall, wall_s=wall, p50_ms=float(np.percentile(lat, 50)) * 1e3,
+                         p99_ms=float(np.percentile(lat, 99)) * 1e3,
+                         steady_p50_ms=float(np.percentile(steady_lat, 50)) * 1e3 if len(steady_lat) else None,
+                         steady_p99_ms=float(np.percentile(steady_lat, 99)) * 1e3 if len(steady_lat) else None)
+        for i in ids:
+            eng.release(i)
+        eng.close()
+        log(f"{name}: wall {wall:.2f} s, p50 {out[name]['p50_ms']:.0f} ms, p99 {out[name]['p99_ms']:.0f} ms")
+    for r, _, _ in pool:
+        r.free()
+    return {"metric": "p50/p99 solve latency + cell-updates/s (served Poisson stream)", "mode": "serve",
+            "value": out["mixed"]["value"], "unit": "cell-updates/s", "n_gpus": 1,
+            "config": {"workload": f"Poisson {args.serve_rate}/s for {args.serve_seconds} s, pool of "
+                                   f"{args.serve_pool} models 64^2-512^2 x 4-8 layers (10 % microchannel), "
+                                   f"60 % steady / 40 % transient of U{{100..10000}} x {args.serve_steps_scale} intervals",
+                       "requests": len(reqs), "cell_updates": updates},
+            "forms": out,
+            "speedup_vs_fp64": (out["mixed"]["value"] / out["fp64"]["value"]) if "fp64" in out else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--mode", choices=["batch", "many", "serve"], default="batch",
+                    help="batch: the headline served batch (default); many: uneven batched problems; "
+                         "serve: Poisson request stream")
+    ap.add_argument("--many", type=int, default=512)
+    ap.add_argument("--many-steps", type=int, default=500)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--serve-pool", type=int, default=20)
+    ap.add_argument("--serve-steps-scale", type=float, default=0.1)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -209,6 +351,10 @@ def main():
     lib.lib.hs_set_fatal_mode(0)
     bind(lib)
     L = lib.lib
+    if args.mode != "batch":
+        res = mode_many(lib, args) if args.mode == "many" else mode_serve(lib, args)
+        print(json.dumps(res), flush=True)
+        return
 
     log(f"building {args.batch} jobs: {args.grid}x{args.grid} 3D stack, {args.intervals} intervals")
     jobs = [Job(lib, args.grid, args.intervals, seed=2 + b) for b in range(args.batch)]

@@ -160,6 +160,38 @@ def stacked_case(n: int, seed: int, n_dies=4, blocks_per_die=64, die=16e-3,
     return Case(f"3D stack {n}x{n} {n_dies} dies", cfg, lcf=layers), powers
 
 
+def chip_stack_case(n: int, seed: int, chip_layers: int, n_blocks: int, die=16e-3,
+                    power=(0.1, 10.0), hotspots=0, t_die=0.1e-3, t_bond=10e-6,
+                    k_bond=1.5, aspect=(0.3, 3.4), **config) -> tuple[Case, list]:
+    """chip_layers layers over the default spreader and sink, alternating
+    silicon dies (k=100, c_v=1.75e6, power) and bond layers (k_bond) from the
+    bottom up.  Each die has a seeded slicing floorplan of n_blocks blocks
+    (aspect window widened for block counts that are odd powers of two).
+    power=(lo, hi): block powers U(lo, hi) W; power=("density", mean):
+    density lognormal(ln mean W/mm^2, 0.6) x area, with `hotspots` blocks
+    per die at 10x the mean density."""
+    g = rng(seed)
+    layers, powers = [], []
+    for k in range(chip_layers):
+        if k % 2 == 0:
+            units = slicing_floorplan(g, n_blocks, die, die, aspect=aspect, prefix=f"l{k}u")
+            area = np.array([u[1] * u[2] for u in units]) * 1e6     # mm^2
+            if power[0] == "density":
+                dens = g.lognormal(np.log(power[1]), 0.6, size=len(units))
+                if hotspots:
+                    dens[g.choice(len(units), size=min(hotspots, len(units)), replace=False)] = 10 * power[1]
+                powers.append(dens * area)
+            else:
+                powers.append(g.uniform(power[0], power[1], size=len(units)))
+            layers.append(LcfLayer(True, True, 1.75e6, 0.01, t_die, units))
+        else:
+            bond = tiled_floorplan(1, 1, die, die, prefix=f"bond{k}_")
+            layers.append(LcfLayer(True, False, 4e6, 1.0 / k_bond, t_bond, bond))
+    cfg = dict(grid_rows=n, grid_cols=n)
+    cfg.update(config)
+    return Case(f"stack {n}x{n}, {chip_layers} chip layers, {n_blocks} blocks", cfg, lcf=layers), powers
+
+
 # --------------------------------------------------------------------------
 # running a case through a HotSpot-API library
 # --------------------------------------------------------------------------
