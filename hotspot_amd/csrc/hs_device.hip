@@ -84,8 +84,8 @@ __device__ __forceinline__ void write_partials(const ProbDesc &d, int slot, cons
 
 // Whole workgroup: combine a problem's slots in slot order (thread 0 result).
 template <int N, bool MAX = false>
-__device__ __forceinline__ void reduce_slots(const ProbDesc &d, double (&v)[N]) {
-    const int nslots = d.ntiles + 1;
+__device__ __forceinline__ void reduce_slots(const ProbDesc &d, double (&v)[N], int nslots = -1) {
+    if (nslots < 0) nslots = d.ntiles + 1;
 #pragma unroll
     for (int q = 0; q < N; q++) v[q] = 0.0;
     for (int s = threadIdx.x; s < nslots; s += kBlock)
@@ -619,23 +619,25 @@ template <typename T, bool HET>
 __global__ void __launch_bounds__(kBlock) k_cg_upd(const ProbDesc *__restrict__ D,
                                                    const WorkItem *__restrict__ W, int xb,
                                                    int rb, int zb, int p0b, int qb, int done_slot,
-                                                   int tol_slot, int max_iters) {
+                                                   int tol_slot, int max_iters, int mg) {
     const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     if (problem_skip(d, done_slot) || (done_slot != S_DONE && problem_skip(d, S_DONE))) return;
     const int par = (int)d.scal[S_PPAR];
     auto buf = [&](int b) -> T * { return sizeof(T) == 8 ? (T *)d.b64[b] : (T *)d.b32[b]; };
-    T *x = buf(xb), *r = buf(rb), *z = buf(zb);
+    T *x = buf(xb), *r = buf(rb), *z = buf(zb);   // mg: z comes from a V-cycle afterwards
     const T *p = buf(p0b + 1 - par), *qv = buf(qb);
     const T alpha = (T)d.scal[S_ALPHA];
     double v[2] = {0, 0};
     auto upd = [&](unsigned c, T xc, T pc, T rc, T qc, T dinv) {
         st_at(x, c, xc + alpha * pc);
         const T rr = rc - alpha * qc;
-        const T zz = rr * dinv;
         st_at(r, c, rr);
-        st_at(z, c, zz);
-        v[0] += (double)rr * zz;
+        if (!mg) {
+            const T zz = rr * dinv;
+            st_at(z, c, zz);
+            v[0] += (double)rr * zz;
+        }
         v[1] += (double)rr * rr;
     };
     if (wi.tile < d.ntiles) {
@@ -905,11 +907,272 @@ __global__ void __launch_bounds__(kBlock) k_rk_control(const ProbDesc *__restric
 
 // Finish kernels of the CG / refinement / Taylor reductions: one workgroup
 // per problem, same skip condition as the producer kernel.
-enum FinOp { FIN_CG_INIT = 0, FIN_ALPHA, FIN_BETA, FIN_IR, FIN_TAY, FIN_GM_IR, FIN_GM_H, FIN_GM_NORM };
+enum FinOp { FIN_CG_INIT = 0, FIN_ALPHA, FIN_BETA, FIN_IR, FIN_TAY, FIN_GM_IR, FIN_GM_H, FIN_GM_NORM,
+             FIN_MG_RZ0, FIN_MG_RR, FIN_MG_BETA };
 struct FinArgs {
     double rtol2, inner_rtol2;
-    int max_iters, max_refine, done_slot, tol_slot, sub, seq;
+    int max_iters, max_refine, done_slot, tol_slot, sub, seq, mg;
 };
+
+// ---------------------------------------------------------------------------
+// geometric multigrid (FP32 preconditioner of the inner solves).  Levels
+// halve rows and columns and keep the layers and the package nodes.  Coarse
+// operators aggregate the fine ones: vertical, ambient, ring and advection
+// couplings sum over the fine faces of a coarse face; lateral conduction sums
+// and halves (the coarse cell centres are twice as far apart), which is the
+// rediscretised operator for homogeneous layers.  Internal advection of an
+// aggregate moves into its self term.  Smoother: z-line Jacobi (a tridiagonal
+// solve over the layers of each column: thin bond layers make the vertical
+// coupling far stronger than the lateral one).  Transfer: restriction sums
+// the four fine residuals, prolongation is piecewise constant (its transpose).
+// ---------------------------------------------------------------------------
+
+// fine-level face and advection coefficients (per layer for homogeneous
+// levels, per cell otherwise); 0 where the face does not exist
+struct FineCoef {
+    const ProbDesc &d;
+    bool het;
+    __device__ unsigned idx(int n, int i, int j) const { return ((unsigned)n * d.nr + i) * d.nc + j; }
+    __device__ float gS(int n, int i, int j) const {
+        if (i >= d.nr - 1) return 0.f;
+        return het ? d.fS[idx(n, i, j)] : (float)d.gx[n];
+    }
+    __device__ float gE(int n, int i, int j) const {
+        if (j >= d.nc - 1) return 0.f;
+        return het ? d.fE[idx(n, i, j)] : (float)d.gy[n];
+    }
+    __device__ float gD(int n, int i, int j) const {
+        if (n >= d.nl - 1) return 0.f;
+        return het ? d.fD[idx(n, i, j)] : (float)d.gz[n];
+    }
+    __device__ bool adv() const { return het && d.has_adv; }
+    __device__ float aN(int n, int i, int j) const { return adv() ? d.faN[idx(n, i, j)] : 0.f; }
+    __device__ float aS(int n, int i, int j) const { return adv() ? d.faS[idx(n, i, j)] : 0.f; }
+    __device__ float aE(int n, int i, int j) const { return adv() ? d.faE[idx(n, i, j)] : 0.f; }
+    __device__ float aW(int n, int i, int j) const { return adv() ? d.faW[idx(n, i, j)] : 0.f; }
+    __device__ float self(int n, int i, int j) const { return adv() ? d.faSelf[idx(n, i, j)] : 0.f; }
+};
+
+// coarse faces from the fine level
+__device__ __forceinline__ float cg_S(const FineCoef &f, int n, int I, int J, int NR) {
+    return I < NR - 1 ? 0.5f * (f.gS(n, 2 * I + 1, 2 * J) + f.gS(n, 2 * I + 1, 2 * J + 1)) : 0.f;
+}
+__device__ __forceinline__ float cg_E(const FineCoef &f, int n, int I, int J, int NC) {
+    return J < NC - 1 ? 0.5f * (f.gE(n, 2 * I, 2 * J + 1) + f.gE(n, 2 * I + 1, 2 * J + 1)) : 0.f;
+}
+__device__ __forceinline__ float cg_D(const FineCoef &f, int n, int I, int J) {
+    return f.gD(n, 2 * I, 2 * J) + f.gD(n, 2 * I, 2 * J + 1) + f.gD(n, 2 * I + 1, 2 * J) +
+           f.gD(n, 2 * I + 1, 2 * J + 1);
+}
+
+// coarse-level coefficients and Jacobi diagonal, one thread per coarse cell
+__global__ void k_mg_coarsen(const ProbDesc *__restrict__ F, const ProbDesc *__restrict__ Cp, int fine_het) {
+    const ProbDesc &d = *F;
+    const ProbDesc &c = *Cp;
+    const unsigned ncell = (unsigned)c.nl * c.nr * c.nc;
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncell) return;
+    const unsigned plane = (unsigned)c.nr * c.nc;
+    const int n = t / plane, rem = t - n * plane, I = rem / c.nc, J = rem - I * c.nc;
+    const FineCoef f{d, fine_het != 0};
+    const float gs = cg_S(f, n, I, J, c.nr), ge = cg_E(f, n, I, J, c.nc), gd = n < c.nl - 1 ? cg_D(f, n, I, J) : 0.f;
+    const float gn = I > 0 ? cg_S(f, n, I - 1, J, c.nr) : 0.f, gw = J > 0 ? cg_E(f, n, I, J - 1, c.nc) : 0.f;
+    const float gu = n > 0 ? cg_D(f, n - 1, I, J) : 0.f;
+    float an = 0, as = 0, ae = 0, aw = 0, self = 0;
+    if (f.adv()) {
+        const int i0 = 2 * I, j0 = 2 * J;
+        an = f.aN(n, i0, j0) + f.aN(n, i0, j0 + 1);
+        as = f.aS(n, i0 + 1, j0) + f.aS(n, i0 + 1, j0 + 1);
+        ae = f.aE(n, i0, j0 + 1) + f.aE(n, i0 + 1, j0 + 1);
+        aw = f.aW(n, i0, j0) + f.aW(n, i0 + 1, j0);
+        self = f.self(n, i0, j0) + f.self(n, i0, j0 + 1) + f.self(n, i0 + 1, j0) + f.self(n, i0 + 1, j0 + 1) +
+               2.f * (f.aS(n, i0, j0) + f.aE(n, i0, j0) + f.aS(n, i0, j0 + 1) + f.aW(n, i0, j0 + 1) +
+                      f.aN(n, i0 + 1, j0) + f.aE(n, i0 + 1, j0) + f.aN(n, i0 + 1, j0 + 1) + f.aW(n, i0 + 1, j0 + 1));
+        ((float *)c.faN)[t] = an;
+        ((float *)c.faS)[t] = as;
+        ((float *)c.faE)[t] = ae;
+        ((float *)c.faW)[t] = aw;
+        ((float *)c.faSelf)[t] = self;
+    }
+    ((float *)c.fS)[t] = gs;
+    ((float *)c.fE)[t] = ge;
+    ((float *)c.fD)[t] = gd;
+    float diag = gn + gs + ge + gw + gu + gd - (an + as + ae + aw + self) + (float)c.gamb[n];
+    const int *rn = c.ring_node[n];
+    if (I == 0 && rn[0] >= 0) diag += (float)c.ring_gns[n];
+    if (I == c.nr - 1 && rn[1] >= 0) diag += (float)c.ring_gns[n];
+    if (J == c.nc - 1 && rn[2] >= 0) diag += (float)c.ring_gew[n];
+    if (J == 0 && rn[3] >= 0) diag += (float)c.ring_gew[n];
+    ((float *)c.fdinv)[t] = 1.f / diag;
+}
+
+// when a multigrid kernel runs for a problem (flags of the fine problem,
+// whose scalars every level shares)
+enum MgWhen { MG_RUNNING = 0, MG_INIT = 1, MG_CORRECT = 2 };
+__device__ __forceinline__ bool mg_skip(const ProbDesc &d, int when) {
+    const double *s = d.scal;
+    if (s[S_DONE] != 0.0) return true;
+    if (when == MG_INIT) return s[S_MGINIT] == 0.0;
+    if (when == MG_CORRECT) return s[S_IDONE] == 0.0 || s[S_CORR] == 0.0;
+    return s[S_IDONE] != 0.0;
+}
+
+// column of a 2D tile: TX = min(nc, 64) columns x 256/TX rows, all layers
+struct Col {
+    int i, j;
+    bool active;
+};
+__device__ __forceinline__ Col col_of(const ProbDesc &d, int tile) {
+    const int sh = d.tx_shift;
+    const int ti = tile / d.ntj, tj = tile - ti * d.ntj;
+    const int i = ti * (kBlock >> sh) + (threadIdx.x >> sh), j = (tj << sh) + (threadIdx.x & ((1 << sh) - 1));
+    return Col{min(i, d.nr - 1), min(j, d.nc - 1), i < d.nr && j < d.nc};
+}
+
+// state loader of a smoothing sweep: x (0 when zero), plus the coarse
+// correction of the parent cell when prolonging
+struct LdMg {
+    const float *x;          // nullptr: zero
+    const float *xc;         // coarse solution (prolongation) or nullptr
+    unsigned ncell, cncell;  // node counts of this and the coarse level
+    int nr, nc, cnr, cnc;
+    __device__ __forceinline__ float operator()(unsigned c) const {
+        float v = x ? ld_at(x, c) : 0.f;
+        if (xc) {
+            unsigned p;
+            if (c >= ncell) {
+                p = cncell + (c - ncell);
+            } else {
+                const unsigned plane = (unsigned)nr * nc;
+                const unsigned n = c / plane, rem = c - n * plane, i = rem / nc, j = rem - i * nc;
+                p = (n * cnr + (i >> 1)) * cnc + (j >> 1);
+            }
+            v += ld_at(xc, p);
+        }
+        return v;
+    }
+};
+
+// One z-line Jacobi sweep x_out = x + omega T^-1 (b - A x) (T: the
+// tridiagonal vertical part of A), x as given by the loader.  Package nodes:
+// point Jacobi.  With dot >= 0, the partial (b, x_out) goes to the problem's
+// reduction slot (the last sweep of a V-cycle at the finest level).
+template <bool HET>
+__global__ void __launch_bounds__(kBlock) k_mg_smooth(const ProbDesc *__restrict__ D,
+                                                      const ProbDesc *__restrict__ DC,
+                                                      const WorkItem *__restrict__ W, int xin, int xcb,
+                                                      int xout, int bb, float omega, int when, int dot) {
+    const WorkItem wi = W[work_index()];
+    const ProbDesc &d = D[wi.prob];
+    if (mg_skip(d, when)) return;
+    const unsigned ncell = (unsigned)d.nl * d.nr * d.nc;
+    LdMg ld{xin >= 0 ? d.b32[xin] : nullptr, nullptr, ncell, 0, d.nr, d.nc, 0, 0};
+    if (xcb >= 0) {
+        const ProbDesc &c = DC[wi.prob];
+        ld.xc = c.b32[xcb];
+        ld.cncell = (unsigned)c.nl * c.nr * c.nc;
+        ld.cnr = c.nr;
+        ld.cnc = c.nc;
+    }
+    const float *b = d.b32[bb];
+    float *xo = d.b32[xout];
+    double acc[1] = {0.0};
+    const int ntiles2 = d.nti2 * d.ntj;
+    if (wi.tile < ntiles2) {
+        extern __shared__ float lds[];   // [2][nl][kBlock]: Thomas coefficients
+        float *cp = lds, *dp = lds + d.nl * kBlock;
+        const Col col = col_of(d, wi.tile);
+        const unsigned plane = (unsigned)d.nr * d.nc;
+        const unsigned c0 = (unsigned)col.i * d.nc + col.j;
+        float ring[4];
+        float lower = 0.f;
+        for (int n = 0; n < d.nl; n++) {
+            load_ring<float>(d, ld, n, ring);
+            Cell q{n, col.i, col.j, n * plane + c0, col.active};
+            Pts<float, HET> p{};
+            load_pts<float, HET>(d, ld, q, p);
+            float flux, diag, gam;
+            stencil<float, HET>(d, q, p, ring, flux, diag, gam);
+            const float res = ld_at(b, q.c) - (gam * p.c - flux);
+            // vertical couplings of the tridiagonal part
+            const float gd = n < d.nl - 1 ? (HET ? ld_at(Coef<float>::D(d), q.c) : (float)d.gz[n]) : 0.f;
+            const float up = -gd;   // coefficient of x(n+1) in row n
+            const float m = n == 0 ? diag : diag - lower * cp[(n - 1) * kBlock + threadIdx.x];
+            cp[n * kBlock + threadIdx.x] = up / m;
+            dp[n * kBlock + threadIdx.x] = (res - (n == 0 ? 0.f : lower * dp[(n - 1) * kBlock + threadIdx.x])) / m;
+            lower = -gd;            // coefficient of x(n) in row n+1
+        }
+        float dx = 0.f;
+        for (int n = d.nl - 1; n >= 0; n--) {
+            dx = dp[n * kBlock + threadIdx.x] - (n == d.nl - 1 ? 0.f : cp[n * kBlock + threadIdx.x] * dx);
+            const unsigned c = n * plane + c0;
+            const float xn = ld(c) + omega * dx;
+            if (col.active) {
+                st_at(xo, c, xn);
+                if (dot) acc[0] += (double)ld_at(b, c) * xn;
+            }
+        }
+    } else {
+        __shared__ float xf[kMaxExtra], xd[kMaxExtra];
+        package_terms<float>(d, ld, false, xf, xd);
+        const int k = threadIdx.x;
+        if (k < d.nx) {
+            const unsigned c = ncell + k;
+            const float xc = ld(c);
+            const float res = b[c] - ((float)d.x_gamb[k] * xc - xf[k]);
+            const float xn = xc + omega * res / xd[k];
+            xo[c] = xn;
+            if (dot) acc[0] += (double)b[c] * xn;
+        }
+    }
+    if (dot < 0) return;
+    block_reduce<1>(acc);
+    write_partials<1>(d, wi.tile, acc);   // column tiles, then the package item at nti2 * ntj
+}
+
+// coarse right-hand side: sum of the four fine residuals b - A x
+template <bool HET>
+__global__ void __launch_bounds__(kBlock) k_mg_restrict(const ProbDesc *__restrict__ DF,
+                                                        const ProbDesc *__restrict__ DC,
+                                                        const WorkItem *__restrict__ W, int xf_b, int bf_b,
+                                                        int bc_b, int when) {
+    const WorkItem wi = W[work_index()];
+    const ProbDesc &f = DF[wi.prob];
+    const ProbDesc &c = DC[wi.prob];
+    if (mg_skip(f, when)) return;
+    LdPlain<float> ld{f.b32[xf_b]};
+    const float *bf = f.b32[bf_b];
+    float *bc = c.b32[bc_b];
+    const unsigned fncell = (unsigned)f.nl * f.nr * f.nc, cncell = (unsigned)c.nl * c.nr * c.nc;
+    const int ntiles2 = c.nti2 * c.ntj;
+    if (wi.tile < ntiles2) {
+        const Col col = col_of(c, wi.tile);
+        const unsigned fplane = (unsigned)f.nr * f.nc, cplane = (unsigned)c.nr * c.nc;
+        float ring[4];
+        for (int n = 0; n < c.nl; n++) {
+            load_ring<float>(f, ld, n, ring);
+            float s = 0.f;
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    const int i = 2 * col.i + a, j = 2 * col.j + e;
+                    Cell q{n, i, j, n * fplane + (unsigned)i * f.nc + j, true};
+                    Pts<float, HET> p{};
+                    load_pts<float, HET>(f, ld, q, p);
+                    float flux, diag, gam;
+                    stencil<float, HET>(f, q, p, ring, flux, diag, gam);
+                    s += ld_at(bf, q.c) - (gam * p.c - flux);
+                }
+            if (col.active) st_at(bc, n * cplane + (unsigned)col.i * c.nc + col.j, s);
+        }
+    } else {
+        __shared__ float xf[kMaxExtra], xd[kMaxExtra];
+        package_terms<float>(f, ld, false, xf, xd);
+        const int k = threadIdx.x;
+        if (k < f.nx) bc[cncell + k] = bf[fncell + k] - ((float)f.x_gamb[k] * ld(fncell + k) - xf[k]);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // steady, non-symmetric G (coolant advection): restarted GMRES(kGmM), right-
@@ -1268,6 +1531,38 @@ __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ 
             s[S_IDONE] = done ? 1.0 : 0.0;
             s[S_CORR] = done ? 0.0 : 1.0;
             s[S_NREFINE] += 1.0;
+            s[S_MGINIT] = (a.mg && !done) ? 1.0 : 0.0;
+            return;
+        }
+        case FIN_MG_RZ0: {   // first V-cycle of an inner solve done: z = M r
+            if (s[S_DONE] != 0.0 || s[S_MGINIT] == 0.0) return;
+            double w[1];
+            reduce_slots<1>(d, w, d.nti2 * d.ntj + 1);
+            if (threadIdx.x) return;
+            s[S_RZ] = w[0];
+            s[S_BETA] = 0.0;
+            s[S_PPAR] = 0.0;
+            s[S_MGINIT] = 0.0;
+            return;
+        }
+        case FIN_MG_RR: {    // after the update of x and r: stop test
+            if (s[a.done_slot] != 0.0 || s[S_DONE] != 0.0) return;
+            double w[2];
+            reduce_slots<2>(d, w);
+            if (threadIdx.x) return;
+            s[S_RR] = w[1];
+            s[S_ITERS] += 1.0;
+            if (w[1] <= s[a.tol_slot] || s[S_ITERS] >= a.max_iters || !(w[1] == w[1])) s[a.done_slot] = 1.0;
+            return;
+        }
+        case FIN_MG_BETA: {  // after z = M r: beta and the next direction's parity
+            if (s[a.done_slot] != 0.0 || s[S_DONE] != 0.0) return;
+            double w[1];
+            reduce_slots<1>(d, w, d.nti2 * d.ntj + 1);
+            if (threadIdx.x) return;
+            s[S_BETA] = s[S_RZ] != 0.0 ? w[0] / s[S_RZ] : 0.0;
+            s[S_RZ] = w[0];
+            s[S_PPAR] = 1.0 - s[S_PPAR];
             return;
         }
         case FIN_GM_IR: {

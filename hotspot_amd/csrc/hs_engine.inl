@@ -25,8 +25,19 @@ struct Group {
 // least 4 rows per thread runs its FP32 term sweeps with float4 accesses.
 static bool vec4_ok(const ProbDesc &h) { return !h.hetero && h.rpt >= 4 && h.tx_shift >= 4; }
 
+// One multigrid level of one problem: its descriptor (the finest level is
+// the problem's own) and the column work-list (2D tiles, all layers per
+// thread, then the package item).
+struct MgLevel {
+    ProbDesc h{};
+    ProbDesc *d_desc = nullptr;
+    WorkItem *d_cwork = nullptr;
+    int ncwork = 0;
+};
+
 struct DevModel {
     Group grp;                          // this model alone
+    std::vector<MgLevel> mg;            // multigrid levels (built on first use); [0] is this model
     ProbDesc h{};                       // host copy of the descriptor
     ProbDesc *d_desc = nullptr;
     WorkItem *d_work = nullptr;         // cell tiles + package item
@@ -133,7 +144,11 @@ static void set_tiling(ProbDesc &h) {
     h.ntj = (h.nc + tx - 1) / tx;
     h.nti = (h.nr + ty * h.rpt - 1) / (ty * h.rpt);
     h.ntiles = h.nl * h.nti * h.ntj;
+    h.nti2 = (h.nr + ty - 1) / ty;
 }
+
+// reduction slots a problem needs: layer tiles or column tiles, + package item
+static int slot_count(const ProbDesc &h) { return std::max(h.ntiles, h.nti2 * h.ntj) + 1; }
 
 // Layer and package-node coefficients of the descriptor from the host model.
 static void fill_coefficients(Model &M, ProbDesc &h) {
@@ -481,7 +496,7 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
     h.blk_out = D->alloc<double>(D->nblk);
     HIP_OK(hipHostMalloc((void **)&D->h_blk, D->nblk * sizeof(double), 0));
     HIP_OK(hipHostMalloc((void **)&D->h_scal, NSCAL * sizeof(double), 0));
-    h.partial = D->alloc<double>((size_t)(h.ntiles + 1) * kRed);
+    h.partial = D->alloc<double>((size_t)slot_count(h) * kRed);
     h.counter = D->alloc<unsigned>(1);
     HIP_OK(hipMemset(h.counter, 0, sizeof(unsigned)));
     h.scal = D->alloc<double>(NSCAL);
@@ -602,7 +617,7 @@ static void cg_iterations(const Group &G, int n, int xb, int rb, int zb, int p0b
         launch_named("k_cg_dir", k_cg_dir<T, HET>, G.nwork, G.stream, dd, w, zb, p0b, qb, done_slot);
         finish(G, FIN_ALPHA, a);
         launch_named("k_cg_upd", k_cg_upd<T, HET>, G.nwork, G.stream, dd, w, xb, rb, zb, p0b, qb, done_slot,
-                     tol_slot, max_iters);
+                     tol_slot, max_iters, 0);
         finish(G, FIN_BETA, a);
     }
 }
@@ -663,11 +678,188 @@ static void gm_round(const Group &G, double rtol2, double inner2, int iters) {
     }
 }
 
+constexpr double kInnerRtol2 = 1e-8;   // each FP32 solve gains 4 digits
+constexpr int kMaxRefine = 30;
+
+// ---- multigrid -------------------------------------------------------------
+constexpr float kMgOmega = 0.8f;   // line-Jacobi damping
+constexpr int kMgCoarseSweeps = 8;
+
+static std::vector<WorkItem> column_items(const ProbDesc &h, int prob) {
+    std::vector<WorkItem> w;
+    for (int t = 0; t < h.nti2 * h.ntj; t++) w.push_back(WorkItem{prob, t});
+    w.push_back(WorkItem{prob, h.nti2 * h.ntj});
+    return w;
+}
+
+// Coarse levels down to about 4 x 4 cells per layer; coefficients built on
+// the device from the level above.
+static void build_mg(DevModel &D) {
+    if (!D.mg.empty()) return;
+    MgLevel l0;
+    l0.h = D.h;
+    l0.d_desc = D.d_desc;
+    auto w0 = column_items(D.h, 0);
+    l0.d_cwork = D.upload(w0.data(), w0.size());
+    l0.ncwork = (int)w0.size();
+    D.mg.push_back(l0);
+    bool prev_het = D.hetero;
+    while (D.mg.back().h.nr >= 8 && D.mg.back().h.nc >= 8) {
+        const ProbDesc f = D.mg.back().h;
+        MgLevel L;
+        ProbDesc &h = L.h;
+        h = f;
+        h.nr = f.nr / 2;
+        h.nc = f.nc / 2;
+        set_tiling(h);
+        h.hetero = 1;
+        for (int n = 0; n < h.nl; n++) {
+            h.gamb[n] = f.gamb[n] * 4.0;
+            h.ring_gns[n] = f.ring_gns[n] * 2.0;
+            h.ring_gew[n] = f.ring_gew[n] * 2.0;
+        }
+        for (int k = 0; k < kMaxExtra; k++) h.x_ring_g[k] = f.x_ring_g[k] * 2.0;
+        const size_t ncell = (size_t)h.nl * h.nr * h.nc, vlen = ((ncell + h.nx + 63) / 64) * 64;
+        h.vlen = (unsigned)vlen;
+        for (int b = 0; b < NB64; b++) h.b64[b] = nullptr;
+        for (int b = 0; b < NB32; b++) h.b32[b] = nullptr;
+        for (int b : {(int)F_R, (int)F_D0, (int)F_D1}) {
+            h.b32[b] = D.alloc<float>(vlen);
+            HIP_OK(hipMemset(h.b32[b], 0, vlen * sizeof(float)));
+        }
+        h.gE = h.gS = h.gD = h.cinvc = h.dinvc = nullptr;
+        h.aN = h.aS = h.aE = h.aW = h.aSelf = nullptr;
+        h.bconst = nullptr;
+        h.fcinv = nullptr;
+        h.fE = D.alloc<float>(ncell);
+        h.fS = D.alloc<float>(ncell);
+        h.fD = D.alloc<float>(ncell);
+        h.fdinv = D.alloc<float>(ncell);
+        if (h.has_adv) {
+            h.faN = D.alloc<float>(ncell);
+            h.faS = D.alloc<float>(ncell);
+            h.faE = D.alloc<float>(ncell);
+            h.faW = D.alloc<float>(ncell);
+            h.faSelf = D.alloc<float>(ncell);
+        } else {
+            h.faN = h.faS = h.faE = h.faW = h.faSelf = nullptr;
+        }
+        h.partial = D.alloc<double>((size_t)slot_count(h) * kRed);
+        h.gm = h.gpart = nullptr;
+        h.gmv32 = nullptr;
+        h.gmv64 = nullptr;
+        h.seq_in = h.seq_out = nullptr;
+        L.d_desc = D.upload(&h, 1);
+        auto w = column_items(h, 0);
+        L.d_cwork = D.upload(w.data(), w.size());
+        L.ncwork = (int)w.size();
+        hipLaunchKernelGGL(k_mg_coarsen, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, D.stream,
+                           (const ProbDesc *)D.mg.back().d_desc, (const ProbDesc *)L.d_desc, (int)prev_het);
+        prev_het = true;
+        D.mg.push_back(L);
+    }
+    HIP_OK(hipStreamSynchronize(D.stream));
+}
+
+// The levels of a group: descriptor arrays and column work-lists per level.
+struct MgGroup {
+    const ProbDesc *d;
+    const WorkItem *w;
+    int nw, nl_max;
+    bool het;
+};
+
+static void mg_smooth(const MgGroup &L, const MgGroup *C, hipStream_t s, int xin, int xcb, int xout, int bb,
+                      int when, int dot) {
+    const size_t lds = (size_t)2 * L.nl_max * kBlock * sizeof(float);
+    const ProbDesc *dc = C ? C->d : L.d;
+    if (L.het)
+        hipLaunchKernelGGL(k_mg_smooth<true>, dim3(L.nw), dim3(kBlock), lds, s, L.d, dc, L.w, xin, xcb, xout, bb,
+                           kMgOmega, when, dot);
+    else
+        hipLaunchKernelGGL(k_mg_smooth<false>, dim3(L.nw), dim3(kBlock), lds, s, L.d, dc, L.w, xin, xcb, xout, bb,
+                           kMgOmega, when, dot);
+}
+
+// z = M b on the finest level (b0, z0 are FP32 buffer ids there): one V-cycle,
+// one line-Jacobi sweep before and after each coarse correction; with dot,
+// the finest sweep leaves the partial (b, z) in the reduction slots.
+static void vcycle(const std::vector<MgGroup> &lv, hipStream_t s, int when, int b0, int z0, bool dot) {
+    const int L = (int)lv.size() - 1;
+    std::vector<int> cur(L + 1);
+    auto other = [](int b) { return b == F_D0 ? (int)F_D1 : (int)F_D0; };
+    for (int l = 0; l < L; l++) {
+        const int b = l == 0 ? b0 : (int)F_R;
+        mg_smooth(lv[l], nullptr, s, -1, -1, F_D0, b, when, -1);
+        cur[l] = F_D0;
+        const MgGroup &f = lv[l], &c = lv[l + 1];
+        if (f.het)
+            hipLaunchKernelGGL(k_mg_restrict<true>, dim3(c.nw), dim3(kBlock), 0, s, f.d, c.d, c.w, cur[l], b,
+                               (int)F_R, when);
+        else
+            hipLaunchKernelGGL(k_mg_restrict<false>, dim3(c.nw), dim3(kBlock), 0, s, f.d, c.d, c.w, cur[l], b,
+                               (int)F_R, when);
+    }
+    const int bL = L == 0 ? b0 : (int)F_R;
+    mg_smooth(lv[L], nullptr, s, -1, -1, F_D0, bL, when, -1);
+    cur[L] = F_D0;
+    for (int k = 1; k < kMgCoarseSweeps; k++) {
+        mg_smooth(lv[L], nullptr, s, cur[L], -1, other(cur[L]), bL, when, -1);
+        cur[L] = other(cur[L]);
+    }
+    if (L == 0) {   // a single level: the sweeps are the preconditioner
+        mg_smooth(lv[0], nullptr, s, cur[0], -1, z0, b0, when, dot ? 1 : -1);
+        return;
+    }
+    for (int l = L - 1; l >= 0; l--) {
+        const int b = l == 0 ? b0 : (int)F_R;
+        const int out = l == 0 ? z0 : other(cur[l]);
+        mg_smooth(lv[l], &lv[l + 1], s, cur[l], cur[l + 1], out, b, when, (l == 0 && dot) ? 1 : -1);
+        cur[l] = out;
+    }
+}
+
+static std::vector<MgGroup> mg_levels_of(const DevModel &D) {
+    std::vector<MgGroup> lv;
+    for (size_t l = 0; l < D.mg.size(); l++)
+        lv.push_back(MgGroup{D.mg[l].d_desc, D.mg[l].d_cwork, D.mg[l].ncwork, D.h.nl, l > 0 || D.hetero});
+    return lv;
+}
+
+// Multigrid-preconditioned CG (FP32) inside FP64 refinement, one round.
+template <bool HET>
+static void mixed_round_mg(const Group &G, const std::vector<MgGroup> &lv, double rtol2, int max_iters,
+                           int chunk) {
+    const ProbDesc *dd = G.d_desc;
+    const WorkItem *w = G.d_work;
+    launch(k_ir_correct, G.nwork, G.stream, dd, w);
+    launch_named("k_ir_resid", k_ir_resid<HET>, G.nwork, G.stream, dd, w, rtol2, kInnerRtol2, kMaxRefine);
+    FinArgs a{};
+    a.rtol2 = rtol2;
+    a.inner_rtol2 = kInnerRtol2;
+    a.max_refine = kMaxRefine;
+    a.mg = 1;
+    finish(G, FIN_IR, a);
+    vcycle(lv, G.stream, MG_INIT, F_R, F_Z, true);
+    finish(G, FIN_MG_RZ0, a);
+    a.done_slot = S_IDONE;
+    a.tol_slot = S_ITOL2;
+    a.max_iters = max_iters;
+    for (int k = 0; k < chunk; k++) {
+        launch_named("k_cg_dir", k_cg_dir<float, HET>, G.nwork, G.stream, dd, w, (int)F_Z, (int)F_P0, (int)F_Q,
+                     (int)S_IDONE);
+        finish(G, FIN_ALPHA, a);
+        launch_named("k_cg_upd", k_cg_upd<float, HET>, G.nwork, G.stream, dd, w, (int)F_X, (int)F_R, (int)F_Z,
+                     (int)F_P0, (int)F_Q, (int)S_IDONE, (int)S_ITOL2, max_iters, 1);
+        finish(G, FIN_MG_RR, a);
+        vcycle(lv, G.stream, MG_RUNNING, F_R, F_Z, true);
+        finish(G, FIN_MG_BETA, a);
+    }
+}
+
 // Mixed-precision refinement, one round: problems whose inner FP32 solve has
 // finished take its correction and start the next one from a fresh FP64
 // residual (or stop); then `chunk` inner iterations for every problem.
-constexpr double kInnerRtol2 = 1e-8;   // each FP32 solve gains 4 digits
-constexpr int kMaxRefine = 30;
 
 template <bool HET>
 static void mixed_round(const Group &G, double rtol2, int max_iters, int chunk) {
@@ -706,9 +898,16 @@ template <bool HET>
 static void steady_mixed(Model &M, DevModel &D) {
     const double rtol2 = M.opt.steady_rtol * M.opt.steady_rtol;
     write_scal(D, kMixedStart);
+    const bool mg = M.opt.steady_solver == HS_STEADY_MGPCG;
+    std::vector<MgGroup> lv;
+    if (mg) {
+        build_mg(D);
+        lv = mg_levels_of(D);
+    }
     int chunk = 8;
     for (;;) {
-        mixed_round<HET>(D.grp, rtol2, M.opt.max_iters, chunk);
+        if (mg) mixed_round_mg<HET>(D.grp, lv, rtol2, M.opt.max_iters, chunk);
+        else mixed_round<HET>(D.grp, rtol2, M.opt.max_iters, chunk);
         read_scal(D);
         if (D.h_scal[S_DONE] != 0.0) break;
         // a new inner solve starts with short chunks
@@ -752,7 +951,7 @@ void dev_steady(Model &M, const double *block_power, double *block_temp) {
     if (D.h.has_adv) {
         if (M.opt.precision == HS_PREC_MIXED) steady_gmres<float, true>(M, D);
         else steady_gmres<double, true>(M, D);
-    } else if (M.opt.precision == HS_PREC_MIXED) {
+    } else if (M.opt.precision == HS_PREC_MIXED || M.opt.steady_solver == HS_STEADY_MGPCG) {
         if (D.hetero) steady_mixed<true>(M, D);
         else steady_mixed<false>(M, D);
     } else {

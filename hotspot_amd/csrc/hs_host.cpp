@@ -1581,7 +1581,8 @@ void hs_set_options(RC_model_t *model, const hs_options_t *opt) {
     HS_BEGIN
     Model &M = *model_of(model);
     if (opt->precision != HS_PREC_FP64 && opt->precision != HS_PREC_MIXED) fatal("unknown precision\n");
-    if (opt->steady_solver != HS_STEADY_JPCG) fatal("unknown steady solver\n");
+    if (opt->steady_solver != HS_STEADY_JPCG && opt->steady_solver != HS_STEADY_MGPCG)
+        fatal("unknown steady solver\n");
     if (opt->transient != HS_TRANS_RK4 && opt->transient != HS_TRANS_TAYLOR)
         fatal("unknown transient integrator\n");
     if (!(opt->steady_rtol > 0) || opt->max_iters <= 0 || !(opt->taylor_tol > 0)) fatal("invalid solver tolerances\n");
@@ -1691,7 +1692,8 @@ void hs_engine_set_options(hs_engine_t *e, const hs_options_t *opt) {
     HS_BEGIN
     if (!opt) fatal("null options\n");
     if (opt->precision != HS_PREC_FP64 && opt->precision != HS_PREC_MIXED) fatal("unknown precision\n");
-    if (opt->steady_solver != HS_STEADY_JPCG) fatal("unknown steady solver\n");
+    if (opt->steady_solver != HS_STEADY_JPCG && opt->steady_solver != HS_STEADY_MGPCG)
+        fatal("unknown steady solver\n");
     if (opt->transient != HS_TRANS_RK4 && opt->transient != HS_TRANS_TAYLOR) fatal("unknown transient integrator\n");
     if (!(opt->steady_rtol > 0) || opt->max_iters <= 0 || !(opt->taylor_tol > 0)) fatal("invalid solver tolerances\n");
     eng_set_options(engine_of(e), *opt);

@@ -61,9 +61,10 @@ enum Scal {
     S_TTOL,                      // Taylor: truncation tolerance per sub-step
     S_KPLAN,                     // Taylor: most terms a sub-step may take
     S_TTERMS,                    // Taylor: term sweeps (E_k -> E_k+1) summed over a sequence
+    S_MGINIT,                    // multigrid-preconditioned inner solve: first V-cycle pending
     NSCAL = 48
 };
-static_assert(S_TTERMS < NSCAL, "scalar slots");
+static_assert(S_MGINIT < NSCAL, "scalar slots");
 enum RkStatus { RK_NEED_K1 = 0, RK_ATTEMPT = 1, RK_DONE = 2 };
 
 struct ProbDesc {
@@ -72,6 +73,7 @@ struct ProbDesc {
     int tx_shift, nti, ntj, ntiles;   // tiles: nl * nti * ntj, then the package item
     int hetero, has_adv, nblocks;
     int rpt;                          // grid rows per thread in a tile
+    int nti2;                         // column tiles along the rows (multigrid: all layers per thread)
     double ambient;
     double cell_area;
     // homogeneous layer coefficients

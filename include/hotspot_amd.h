@@ -284,6 +284,9 @@ void hs_set_fatal_mode(int exit_on_error);
                            * advection (microchannel layers) makes G non-symmetric, restarted
                            * GMRES(32) cycles, Jacobi right-preconditioned, inside FP64
                            * iterative refinement */
+#define HS_STEADY_MGPCG 1 /* FP32 CG preconditioned by a geometric-multigrid V-cycle (z-line
+                           * Jacobi smoothing, coarsening rows and columns), inside FP64
+                           * iterative refinement, for either precision option */
 /* transient integrators */
 #define HS_TRANS_RK4    0 /* the reference's adaptive step-doubling RK4 */
 #define HS_TRANS_TAYLOR 1 /* exponential (Taylor) integrator of the linear ODE, error bounded a priori */

@@ -110,6 +110,30 @@ def test_slope_and_maps_match_oracle(lib, name, case, pw):
     run.free()
 
 
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("name,case,pw", CASES, ids=[c[0] for c in CASES])
+def test_steady_multigrid_matches_oracle(lib, name, case, pw, prec):
+    """Multigrid-preconditioned FP32 CG inside FP64 refinement (either
+    precision option) reaches the same 1e-12 FP64 residual: within the FP64
+    bar of the oracle, and with far fewer inner iterations than Jacobi."""
+    run = make(lib, case)
+    set_options(lib, run, precision=prec, steady_solver=1)
+    om = oracle_for(run)
+    P = run.power_vector(pw)
+    T = run.steady(P.copy())
+    check_error(lib)
+    To, Tg = om.steady(P)
+    assert np.abs(T - To).max() <= FP64_TOL
+    assert np.abs(grid_temp(lib, run, 0) - Tg).max() <= FP64_TOL
+    st = stats(lib, run)
+    assert st.steady_rel_resid <= 1e-10
+    mg_iters = st.steady_iters
+    set_options(lib, run, precision=1, steady_solver=0)
+    run.steady(P.copy())
+    assert mg_iters < stats(lib, run).steady_iters
+    run.free()
+
+
 @pytest.mark.parametrize("prec,tol", [(0, FP64_TOL), (1, MIXED_TOL)], ids=["fp64", "mixed"])
 def test_microchannel_steady_gmres(lib, prec, tol):
     """Advection makes G non-symmetric: the steady solve runs restarted
