@@ -52,9 +52,15 @@ struct BatchGroup {
     double *d_flags = nullptr, *h_flags = nullptr;
     Replayer rp;
     SeqForm form{};
+    // multigrid steady groups: per level, the descriptors and column work-lists
+    std::vector<MgGroup> lv;
+    std::vector<void *> lv_bufs;
 
     void release() {
         rp.reset();
+        for (void *p : lv_bufs) (void)hipFree(p);
+        lv_bufs.clear();
+        lv.clear();
         if (g.d_desc) (void)hipFree(g.d_desc);
         if (g.d_work) (void)hipFree(g.d_work);
         if (g.d_gwork) (void)hipFree(g.d_gwork);
@@ -80,12 +86,14 @@ struct Engine {
     hipStream_t s_steady = nullptr, s_trans = nullptr;
     std::deque<EngJob *> queue;
     std::map<int, std::unique_ptr<EngJob>> jobs;
-    // steady: homogeneous CG, heterogeneous CG, non-symmetric (GMRES);
+    // steady groups by key: kind (0 homogeneous CG, 1 heterogeneous CG,
+    // 2 non-symmetric GMRES) + 4 x multigrid levels (0: Jacobi);
     // transient: homogeneous, heterogeneous
-    BatchGroup steady[3], trans[2];
+    std::map<int, BatchGroup> steady;
+    BatchGroup trans[2];
     ~Engine() {
         for (auto &kv : jobs) kv.second->D.reset();
-        for (auto &g : steady) g.release();
+        for (auto &kv : steady) kv.second.release();
         for (auto &g : trans) g.release();
         if (s_steady) (void)hipStreamDestroy(s_steady);
         if (s_trans) (void)hipStreamDestroy(s_trans);
@@ -93,7 +101,7 @@ struct Engine {
 };
 
 // Device bytes dev_build allocates for a model, plus the sequence buffers.
-static size_t dev_footprint(const Model &M, int nsteps) {
+static size_t dev_footprint(const Model &M, int nsteps, bool mg) {
     const grid_model_t &g = *M.g;
     ProbDesc t{};
     t.nl = g.n_layers;
@@ -112,6 +120,8 @@ static size_t dev_footprint(const Model &M, int nsteps) {
     b += 2 * (size_t)nsteps * nblk * 8;
     bool adv = false;
     for (auto *m : M.mc) adv |= m != nullptr;
+    if (mg)    // coarse levels: 3 FP32 vectors, 4 coefficient arrays; a third of the cells in all
+        b += (ncell / 3 + 64 * 8) * 7 * 4 + (size_t)g.n_layers * 4096;
     if (adv)   // GMRES state, partials and one basis (FP32 or FP64)
         b += GM_SIZE * 8 + (size_t)(t.ntiles + 1) * (kGmM + 1) * 8 + (size_t)(kGmM + 1) * vlen * 8;
     return b + b / 50 + (64 << 10);
@@ -166,6 +176,32 @@ static void group_sync(Engine &E, BatchGroup &B, hipStream_t s, bool het, bool t
     B.g.hetero = het;
     B.g.vec4 = v4;
     B.form = SeqForm{E.opt.precision, E.opt.transient, m, K};
+    // multigrid levels of a steady group (every member has the same count)
+    for (void *p : B.lv_bufs) HIP_OK(hipFree(p));
+    B.lv_bufs.clear();
+    B.lv.clear();
+    const int L = (!trans && n) ? (int)B.members[0]->D->mg.size() : 0;
+    for (int l = 0; l < L; l++) {
+        std::vector<ProbDesc> ld(n);
+        std::vector<WorkItem> lw;
+        int nlmax = 1;
+        for (int i = 0; i < n; i++) {
+            const DevModel &D = *B.members[i]->D;
+            ld[i] = l == 0 ? D.h : D.mg[l].h;
+            nlmax = std::max(nlmax, ld[i].nl);
+            auto ci = column_items(ld[i], i);
+            lw.insert(lw.end(), ci.begin(), ci.end());
+        }
+        ProbDesc *dd = nullptr;
+        WorkItem *dw = nullptr;
+        HIP_OK(hipMalloc((void **)&dd, n * sizeof(ProbDesc)));
+        HIP_OK(hipMalloc((void **)&dw, lw.size() * sizeof(WorkItem)));
+        B.lv_bufs.push_back(dd);
+        B.lv_bufs.push_back(dw);
+        HIP_OK(hipMemcpy(dd, ld.data(), n * sizeof(ProbDesc), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(dw, lw.data(), lw.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+        B.lv.push_back(MgGroup{dd, dw, (int)lw.size(), nlmax, l > 0 || het});
+    }
     B.dirty = false;
 }
 
@@ -184,7 +220,9 @@ static void release_device(Engine &E, EngJob &j) {
     j.D.reset();
 }
 
-static int steady_kind(const EngJob &j) { return j.adv ? 2 : j.hetero ? 1 : 0; }
+static int steady_key(const EngJob &j) {
+    return (j.adv ? 2 : j.hetero ? 1 : 0) + 4 * (int)j.D->mg.size();
+}
 
 static void start_transient(Engine &E, EngJob &j) {
     DevModel &D = *j.D;
@@ -231,7 +269,12 @@ static void admit(Engine &E, EngJob &j) {
         const int nbk = (int)((D.vlen + kBlock - 1) / kBlock);
         launch(k_fill, nbk, D.stream, D.h.b64[B_STEADY], D.vlen, M.g->config.ambient);
         if (j.adv) {
-            ensure_gmres(D, E.opt.precision == HS_PREC_MIXED);
+            const bool mg = E.opt.steady_solver == HS_STEADY_MGPCG;
+            ensure_gmres(D, E.opt.precision == HS_PREC_MIXED || mg);
+            if (mg) build_mg(D);
+            write_scal(D, kMixedStart);
+        } else if (E.opt.steady_solver == HS_STEADY_MGPCG) {
+            build_mg(D);
             write_scal(D, kMixedStart);
         } else if (E.opt.precision == HS_PREC_MIXED) {
             write_scal(D, kMixedStart);
@@ -242,7 +285,7 @@ static void admit(Engine &E, EngJob &j) {
             else jpcg64_start<false>(D.grp, rtol2, E.opt.max_iters);
         }
         j.phase = J_STEADY;
-        BatchGroup &B = E.steady[steady_kind(j)];
+        BatchGroup &B = E.steady[steady_key(j)];
         B.members.push_back(&j);
         B.dirty = true;
     } else {
@@ -251,13 +294,16 @@ static void admit(Engine &E, EngJob &j) {
         HIP_OK(hipStreamSynchronize(D.stream));
         start_transient(E, j);
     }
+    // device memory held, multigrid levels and GMRES state included
+    E.used += D.bytes - j.held;
+    j.held = D.bytes;
 }
 
 // A finished steady solve: block temperatures, then either the result or
 // the start of the job's transient sequence from them.
 static void steady_done(Engine &E, EngJob &j, const double *flags) {
     DevModel &D = *j.D;
-    const bool refined = E.opt.precision == HS_PREC_MIXED || j.adv;
+    const bool refined = E.opt.precision == HS_PREC_MIXED || j.adv || E.opt.steady_solver == HS_STEADY_MGPCG;
     j.steady_iters = (int)flags[1];
     j.refinements = refined ? (int)flags[2] : 0;
     run_g2b(D.grp, B_STEADY);
@@ -289,9 +335,13 @@ static void trans_done(Engine &E, EngJob &j, const double *flags) {
 }
 
 template <bool HET>
-static void steady_chunk(Engine &E, const Group &G, bool adv) {
+static void steady_chunk(Engine &E, const Group &G, bool adv, const std::vector<MgGroup> &lv) {
     const double rtol2 = E.opt.steady_rtol * E.opt.steady_rtol;
-    if (adv) {   // GMRES cycles inside FP64 refinement
+    if (!lv.empty() && adv) {   // multigrid-preconditioned GMRES cycles
+        for (int done = 0; done < E.tick_iters; done += 64) gm_round_mg<true>(G, lv, rtol2, 1e-8, kGmM);
+    } else if (!lv.empty()) {   // multigrid-preconditioned CG rounds
+        for (int done = 0; done < E.tick_iters; done += 64) mixed_round_mg<HET>(G, lv, rtol2, E.opt.max_iters, 16);
+    } else if (adv) {   // GMRES cycles inside FP64 refinement
         const int cycles = std::max(1, E.tick_iters / kGmM);
         for (int c = 0; c < cycles; c++) {
             if (E.opt.precision == HS_PREC_MIXED) gm_round<float, true>(G, rtol2, 1e-8, kGmM);
@@ -309,7 +359,8 @@ static void steady_chunk(Engine &E, const Group &G, bool adv) {
 Engine *eng_create(int precision, int transient) {
     std::unique_ptr<Engine> E(new Engine());
     E->opt.precision = precision;
-    E->opt.steady_solver = HS_STEADY_JPCG;
+    // the mixed (served) form solves steady states with multigrid-preconditioned CG
+    E->opt.steady_solver = precision == HS_PREC_MIXED ? HS_STEADY_MGPCG : HS_STEADY_JPCG;
     E->opt.transient = transient;
     E->opt.steady_rtol = 1e-12;
     E->opt.max_iters = 200000;
@@ -327,7 +378,7 @@ void eng_destroy(Engine *E) { delete E; }
 
 int eng_active(Engine &E) {
     int n = (int)E.queue.size();
-    for (auto &g : E.steady) n += (int)g.members.size();
+    for (auto &kv : E.steady) n += (int)kv.second.members.size();
     for (auto &g : E.trans) n += (int)g.members.size();
     return n;
 }
@@ -374,7 +425,7 @@ int eng_submit(Engine &E, Model &M, const double *power, int rows, const double 
     j->dt = dt;
     j->hetero = M.detailed;
     j->adv = adv;
-    j->footprint = dev_footprint(M, nsteps);
+    j->footprint = dev_footprint(M, nsteps, !init && E.opt.steady_solver == HS_STEADY_MGPCG);
     if (j->footprint > E.budget)
         fatal("hs_engine_submit: the job needs " + std::to_string(j->footprint) +
               " bytes of device memory, above the engine budget of " + std::to_string(E.budget) + "\n");
@@ -400,15 +451,17 @@ int eng_tick(Engine &E) {
     eng_admit(E);
     // members launched this tick per group: only their gathered flags are
     // fresh (a group may gain members below, or be empty this tick)
-    int ran_steady[3] = {0, 0, 0}, ran_trans[2] = {0, 0};
-    for (int h = 0; h < 3; h++) {
-        BatchGroup &B = E.steady[h];
+    std::map<int, int> ran_steady;
+    int ran_trans[2] = {0, 0};
+    for (auto &kv : E.steady) {
+        BatchGroup &B = kv.second;
+        const int kind = kv.first % 4;
         if (B.members.empty()) continue;
-        group_sync(E, B, E.s_steady, h > 0, false);
-        ran_steady[h] = B.g.nprob;
-        if (h) steady_chunk<true>(E, B.g, h == 2);
-        else steady_chunk<false>(E, B.g, false);
-        const bool refined = E.opt.precision == HS_PREC_MIXED || h == 2;
+        group_sync(E, B, E.s_steady, kind > 0, false);
+        ran_steady[kv.first] = B.g.nprob;
+        if (kind) steady_chunk<true>(E, B.g, kind == 2, B.lv);
+        else steady_chunk<false>(E, B.g, false, B.lv);
+        const bool refined = E.opt.precision == HS_PREC_MIXED || kind == 2 || !B.lv.empty();
         gather(B, {S_DONE, refined ? S_TITERS : S_ITERS, S_NREFINE});
     }
     for (int h = 0; h < 2; h++) {
@@ -421,10 +474,10 @@ int eng_tick(Engine &E) {
     }
     HIP_OK(hipStreamSynchronize(E.s_steady));
     HIP_OK(hipStreamSynchronize(E.s_trans));
-    for (int h = 0; h < 3; h++) {
-        BatchGroup &B = E.steady[h];
+    for (auto &kv : E.steady) {
+        BatchGroup &B = kv.second;
         std::vector<std::pair<EngJob *, std::vector<double>>> fin;
-        for (int i = 0; i < ran_steady[h]; i++)
+        for (int i = 0; i < ran_steady[kv.first]; i++)
             if (B.h_flags[i * 8] != 0.0)
                 fin.push_back({B.members[i], std::vector<double>(B.h_flags + i * 8, B.h_flags + i * 8 + 8)});
         for (auto &f : fin) {

@@ -981,13 +981,19 @@ __global__ void k_mg_coarsen(const ProbDesc *__restrict__ F, const ProbDesc *__r
     float an = 0, as = 0, ae = 0, aw = 0, self = 0;
     if (f.adv()) {
         const int i0 = 2 * I, j0 = 2 * J;
-        an = f.aN(n, i0, j0) + f.aN(n, i0, j0 + 1);
-        as = f.aS(n, i0 + 1, j0) + f.aS(n, i0 + 1, j0 + 1);
-        ae = f.aE(n, i0, j0 + 1) + f.aE(n, i0 + 1, j0 + 1);
-        aw = f.aW(n, i0, j0) + f.aW(n, i0 + 1, j0);
+        // Galerkin sums across the coarse faces, halved like the conduction;
+        // the self term keeps the Galerkin row sum (internal advection of
+        // the aggregate twice, plus the halved-away external part)
+        const float sn = f.aN(n, i0, j0) + f.aN(n, i0, j0 + 1), ss = f.aS(n, i0 + 1, j0) + f.aS(n, i0 + 1, j0 + 1);
+        const float se = f.aE(n, i0, j0 + 1) + f.aE(n, i0 + 1, j0 + 1), sw = f.aW(n, i0, j0) + f.aW(n, i0 + 1, j0);
+        an = 0.5f * sn;
+        as = 0.5f * ss;
+        ae = 0.5f * se;
+        aw = 0.5f * sw;
         self = f.self(n, i0, j0) + f.self(n, i0, j0 + 1) + f.self(n, i0 + 1, j0) + f.self(n, i0 + 1, j0 + 1) +
                2.f * (f.aS(n, i0, j0) + f.aE(n, i0, j0) + f.aS(n, i0, j0 + 1) + f.aW(n, i0, j0 + 1) +
-                      f.aN(n, i0 + 1, j0) + f.aE(n, i0 + 1, j0) + f.aN(n, i0 + 1, j0 + 1) + f.aW(n, i0 + 1, j0 + 1));
+                      f.aN(n, i0 + 1, j0) + f.aE(n, i0 + 1, j0) + f.aN(n, i0 + 1, j0 + 1) + f.aW(n, i0 + 1, j0 + 1)) +
+               (sn + ss + se + sw);
         ((float *)c.faN)[t] = an;
         ((float *)c.faS)[t] = as;
         ((float *)c.faE)[t] = ae;
@@ -1130,6 +1136,84 @@ __global__ void __launch_bounds__(kBlock) k_mg_smooth(const ProbDesc *__restrict
     write_partials<1>(d, wi.tile, acc);   // column tiles, then the package item at nti2 * ntj
 }
 
+// One x-line Jacobi sweep x_out = x + omega T^-1 (b - A x), T the tridiagonal
+// part of A along the columns of a row segment of TX = min(nc, 64) cells
+// (couplings across segment ends dropped), for every layer of the thread's
+// column.  The segments are solved by parallel cyclic reduction across the
+// lanes of the wavefront (TX divides 64, so a wavefront holds whole
+// segments).  Package nodes: point Jacobi.  For advection-dominated layers,
+// where the coolant carries heat along the rows.
+template <bool HET>
+__global__ void __launch_bounds__(kBlock) k_mg_xline(const ProbDesc *__restrict__ D,
+                                                     const WorkItem *__restrict__ W, int xin, int xout, int bb,
+                                                     float omega, int when) {
+    const WorkItem wi = W[work_index()];
+    const ProbDesc &d = D[wi.prob];
+    if (mg_skip(d, when)) return;
+    const unsigned ncell = (unsigned)d.nl * d.nr * d.nc;
+    LdMg ld{xin >= 0 ? d.b32[xin] : nullptr, nullptr, ncell, 0, d.nr, d.nc, 0, 0};
+    const float *b = d.b32[bb];
+    float *xo = d.b32[xout];
+    const int ntiles2 = d.nti2 * d.ntj;
+    if (wi.tile < ntiles2) {
+        const Col col = col_of(d, wi.tile);
+        const int seg = 1 << d.tx_shift, ls = threadIdx.x & (seg - 1);
+        const unsigned plane = (unsigned)d.nr * d.nc;
+        float ring[4];
+        for (int n = 0; n < d.nl; n++) {
+            load_ring<float>(d, ld, n, ring);
+            Cell q{n, col.i, col.j, n * plane + (unsigned)col.i * d.nc + col.j, col.active};
+            Pts<float, HET> p{};
+            load_pts<float, HET>(d, ld, q, p);
+            float flux, diag, gam;
+            stencil<float, HET>(d, q, p, ring, flux, diag, gam);
+            float rhs = ld_at(b, q.c) - (gam * p.c - flux);
+            // row of T: coefficients of x(j-1), x(j), x(j+1) in (A x)_c
+            float lo, up;
+            if (HET) {
+                const float gw = col.j > 0 ? p.gw : 0.f, ge = col.j < d.nc - 1 ? p.ge : 0.f;
+                lo = -gw - ((d.has_adv && col.j > 0) ? p.aw : 0.f);
+                up = -ge - ((d.has_adv && col.j < d.nc - 1) ? p.ae : 0.f);
+            } else {
+                lo = col.j > 0 ? -(float)d.gy[n] : 0.f;
+                up = col.j < d.nc - 1 ? -(float)d.gy[n] : 0.f;
+            }
+            if (ls == 0) lo = 0.f;
+            if (ls == seg - 1) up = 0.f;
+            float bd = diag;
+            if (!col.active) {   // an idle lane solves x = 0
+                lo = up = rhs = 0.f;
+                bd = 1.f;
+            }
+            // parallel cyclic reduction within the segment
+            for (int s = 1; s < seg; s <<= 1) {
+                const float lo_m = __shfl_up(lo, s, kWave), bd_m = __shfl_up(bd, s, kWave);
+                const float up_m = __shfl_up(up, s, kWave), r_m = __shfl_up(rhs, s, kWave);
+                const float lo_p = __shfl_down(lo, s, kWave), bd_p = __shfl_down(bd, s, kWave);
+                const float up_p = __shfl_down(up, s, kWave), r_p = __shfl_down(rhs, s, kWave);
+                const bool hm = ls - s >= 0, hp = ls + s < seg;
+                const float al = hm ? -lo / bd_m : 0.f, ga = hp ? -up / bd_p : 0.f;
+                const float nlo = hm ? al * lo_m : 0.f, nup = hp ? ga * up_p : 0.f;
+                bd = bd + (hm ? al * up_m : 0.f) + (hp ? ga * lo_p : 0.f);
+                rhs = rhs + (hm ? al * r_m : 0.f) + (hp ? ga * r_p : 0.f);
+                lo = nlo;
+                up = nup;
+            }
+            if (col.active) st_at(xo, q.c, p.c + omega * (rhs / bd));
+        }
+    } else {
+        __shared__ float xf[kMaxExtra], xd[kMaxExtra];
+        package_terms<float>(d, ld, false, xf, xd);
+        const int k = threadIdx.x;
+        if (k < d.nx) {
+            const unsigned c = ncell + k;
+            const float xc = ld(c);
+            const float res = b[c] - ((float)d.x_gamb[k] * xc - xf[k]);
+            xo[c] = xc + omega * res / xd[k];
+        }
+    }
+}
+
 // coarse right-hand side: sum of the four fine residuals b - A x
 template <bool HET>
 __global__ void __launch_bounds__(kBlock) k_mg_restrict(const ProbDesc *__restrict__ DF,
@@ -1213,7 +1297,7 @@ template <typename T, bool HET> __device__ __forceinline__ T node_dinv(const Pro
 // (T(r), T(r)), (r, r), (rhs, rhs).
 template <typename T, bool HET>
 __global__ void __launch_bounds__(kBlock) k_gm_resid(const ProbDesc *__restrict__ D,
-                                                     const WorkItem *__restrict__ W, int zb) {
+                                                     const WorkItem *__restrict__ W, int zb, int mg) {
     const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     if (problem_skip(d, S_DONE) || d.scal[S_IDONE] == 0.0) return;   // converged / cycle running
@@ -1224,7 +1308,7 @@ __global__ void __launch_bounds__(kBlock) k_gm_resid(const ProbDesc *__restrict_
     auto put = [&](unsigned c, double rr, T dinv, double bc) {
         const T rt = (T)rr;
         st_at(v0, c, rt);
-        st_at(z, c, rt * dinv);
+        st_at(z, c, mg ? rt : rt * dinv);   // mg: z is the multigrid input (v_0)
         w[0] += (double)rt * rt;
         w[1] += rr * rr;
         w[2] += bc * bc;
@@ -1348,7 +1432,7 @@ __global__ void __launch_bounds__(kBlock) k_gm_dots(const ProbDesc *__restrict__
 // v_j+1 = w - sum_i h_ij v_i / scale_i, z = diag^-1 v_j+1; partial ||v_j+1||^2
 template <typename T, bool HET>
 __global__ void __launch_bounds__(kBlock) k_gm_orth(const ProbDesc *__restrict__ D,
-                                                    const WorkItem *__restrict__ W, int wb, int zb) {
+                                                    const WorkItem *__restrict__ W, int wb, int zb, int mg) {
     const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     if (gm_skip(d)) return;
@@ -1363,7 +1447,7 @@ __global__ void __launch_bounds__(kBlock) k_gm_orth(const ProbDesc *__restrict__
             x -= hc * ld_at(gm_vec<T>(d, i), c);
         }
         st_at(vn, c, x);
-        st_at(z, c, x * dinv);
+        st_at(z, c, mg ? x : x * dinv);   // mg: z is the multigrid input (v_j+1)
         acc[0] += (double)x * x;
     };
     if (wi.tile < d.ntiles) {
@@ -1382,17 +1466,19 @@ __global__ void __launch_bounds__(kBlock) k_gm_orth(const ProbDesc *__restrict__
 // finished with a correction pending
 template <typename T, bool HET>
 __global__ void __launch_bounds__(kBlock) k_gm_correct(const ProbDesc *__restrict__ D,
-                                                       const WorkItem *__restrict__ W) {
+                                                       const WorkItem *__restrict__ W, int mgout) {
     const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     if (problem_skip(d, S_DONE) || d.scal[S_IDONE] == 0.0 || d.scal[S_CORR] == 0.0) return;
     const int k = (int)d.gm[GM_K];
     double *x = d.b64[B_STEADY];
+    T *u = mgout >= 0 ? vec_of<T>(d, mgout) : nullptr;
     auto put = [&](unsigned c, T dinv) {
         T e = T(0);
         for (int i = 0; i < k; i++)
             e += (T)(d.gm[GM_Y + i] / d.gm[GM_SCALE + i]) * ld_at(gm_vec<T>(d, i), c);
-        st_at(x, c, ld_at(x, c) + (double)(e * dinv));
+        if (u) st_at(u, c, e);   // multigrid: the V-cycle of V y follows
+        else st_at(x, c, ld_at(x, c) + (double)(e * dinv));
     };
     if (wi.tile < d.ntiles) {
         tile_loop<1>(
@@ -1419,6 +1505,7 @@ __device__ void gm_finish_resid(const ProbDesc &d, const double (&v)[3], const F
     s[S_IDONE] = done ? 1.0 : 0.0;
     s[S_CORR] = done ? 0.0 : 1.0;
     s[S_NREFINE] += 1.0;
+    s[S_MGINIT] = (a.mg && !done) ? 1.0 : 0.0;
     g[GM_J] = 0.0;
     g[GM_K] = 0.0;
     g[GM_TOL] = sqrt(a.inner_rtol2) * nv;

@@ -662,8 +662,8 @@ static void gm_round(const Group &G, double rtol2, double inner2, int iters) {
     const WorkItem *w = G.d_work;
     const bool f32 = sizeof(T) == 4;
     const int zb = f32 ? F_Z : B_Z, wb = f32 ? F_Q : B_Q;
-    launch_named("k_gm_correct", k_gm_correct<T, HET>, G.nwork, G.stream, dd, w);
-    launch_named("k_gm_resid", k_gm_resid<T, HET>, G.nwork, G.stream, dd, w, zb);
+    launch_named("k_gm_correct", k_gm_correct<T, HET>, G.nwork, G.stream, dd, w, -1);
+    launch_named("k_gm_resid", k_gm_resid<T, HET>, G.nwork, G.stream, dd, w, zb, 0);
     FinArgs a{};
     a.rtol2 = rtol2;
     a.inner_rtol2 = inner2;
@@ -673,10 +673,11 @@ static void gm_round(const Group &G, double rtol2, double inner2, int iters) {
         launch_named("k_gm_apply", k_gm_apply<T, HET>, G.nwork, G.stream, dd, w, zb, wb);
         launch_named("k_gm_dots", k_gm_dots<T>, G.nwork, G.stream, dd, w, wb);
         finish(G, FIN_GM_H, a);
-        launch_named("k_gm_orth", k_gm_orth<T, HET>, G.nwork, G.stream, dd, w, wb, zb);
+        launch_named("k_gm_orth", k_gm_orth<T, HET>, G.nwork, G.stream, dd, w, wb, zb, 0);
         finish(G, FIN_GM_NORM, a);
     }
 }
+
 
 constexpr double kInnerRtol2 = 1e-8;   // each FP32 solve gains 4 digits
 constexpr int kMaxRefine = 30;
@@ -784,14 +785,35 @@ static void mg_smooth(const MgGroup &L, const MgGroup *C, hipStream_t s, int xin
 // z = M b on the finest level (b0, z0 are FP32 buffer ids there): one V-cycle,
 // one line-Jacobi sweep before and after each coarse correction; with dot,
 // the finest sweep leaves the partial (b, z) in the reduction slots.
-static void vcycle(const std::vector<MgGroup> &lv, hipStream_t s, int when, int b0, int z0, bool dot) {
+static void mg_xline(const MgGroup &L, hipStream_t s, int xin, int xout, int bb, int when) {
+    if (L.het)
+        hipLaunchKernelGGL(k_mg_xline<true>, dim3(L.nw), dim3(kBlock), 0, s, L.d, L.w, xin, xout, bb, kMgOmega, when);
+    else
+        hipLaunchKernelGGL(k_mg_xline<false>, dim3(L.nw), dim3(kBlock), 0, s, L.d, L.w, xin, xout, bb, kMgOmega,
+                           when);
+}
+
+// xz: advection-dominated problems smooth with an x-line sweep before the
+// z-line sweep (and after it on the way up).
+static void vcycle(const std::vector<MgGroup> &lv, hipStream_t s, int when, int b0, int z0, bool dot,
+                   bool xz = false) {
     const int L = (int)lv.size() - 1;
     std::vector<int> cur(L + 1);
     auto other = [](int b) { return b == F_D0 ? (int)F_D1 : (int)F_D0; };
+    // pre-smoothing of level l from zero (x then z with xz)
+    auto pre = [&](int l, int b) {
+        if (xz) {
+            mg_xline(lv[l], s, -1, F_D0, b, when);
+            mg_smooth(lv[l], nullptr, s, F_D0, -1, F_D1, b, when, -1);
+            cur[l] = F_D1;
+        } else {
+            mg_smooth(lv[l], nullptr, s, -1, -1, F_D0, b, when, -1);
+            cur[l] = F_D0;
+        }
+    };
     for (int l = 0; l < L; l++) {
         const int b = l == 0 ? b0 : (int)F_R;
-        mg_smooth(lv[l], nullptr, s, -1, -1, F_D0, b, when, -1);
-        cur[l] = F_D0;
+        pre(l, b);
         const MgGroup &f = lv[l], &c = lv[l + 1];
         if (f.het)
             hipLaunchKernelGGL(k_mg_restrict<true>, dim3(c.nw), dim3(kBlock), 0, s, f.d, c.d, c.w, cur[l], b,
@@ -801,9 +823,12 @@ static void vcycle(const std::vector<MgGroup> &lv, hipStream_t s, int when, int 
                                (int)F_R, when);
     }
     const int bL = L == 0 ? b0 : (int)F_R;
-    mg_smooth(lv[L], nullptr, s, -1, -1, F_D0, bL, when, -1);
-    cur[L] = F_D0;
+    pre(L, bL);
     for (int k = 1; k < kMgCoarseSweeps; k++) {
+        if (xz) {
+            mg_xline(lv[L], s, cur[L], other(cur[L]), bL, when);
+            cur[L] = other(cur[L]);
+        }
         mg_smooth(lv[L], nullptr, s, cur[L], -1, other(cur[L]), bL, when, -1);
         cur[L] = other(cur[L]);
     }
@@ -813,6 +838,14 @@ static void vcycle(const std::vector<MgGroup> &lv, hipStream_t s, int when, int 
     }
     for (int l = L - 1; l >= 0; l--) {
         const int b = l == 0 ? b0 : (int)F_R;
+        if (xz) {   // prolong + z-line, then x-line
+            mg_smooth(lv[l], &lv[l + 1], s, cur[l], cur[l + 1], other(cur[l]), b, when, -1);
+            cur[l] = other(cur[l]);
+            const int out = l == 0 ? z0 : other(cur[l]);
+            mg_xline(lv[l], s, cur[l], out, b, when);
+            cur[l] = out;
+            continue;
+        }
         const int out = l == 0 ? z0 : other(cur[l]);
         mg_smooth(lv[l], &lv[l + 1], s, cur[l], cur[l + 1], out, b, when, (l == 0 && dot) ? 1 : -1);
         cur[l] = out;
@@ -854,6 +887,35 @@ static void mixed_round_mg(const Group &G, const std::vector<MgGroup> &lv, doubl
         finish(G, FIN_MG_RR, a);
         vcycle(lv, G.stream, MG_RUNNING, F_R, F_Z, true);
         finish(G, FIN_MG_BETA, a);
+    }
+}
+
+// The same with the multigrid V-cycle (x- and z-line smoothing) as the right
+// preconditioner, FP32: v_j goes to F_Z, M v_j to F_X; the cycle's
+// correction is M (V y), added to x by k_ir_correct.
+template <bool HET>
+static void gm_round_mg(const Group &G, const std::vector<MgGroup> &lv, double rtol2, double inner2, int iters) {
+    const ProbDesc *dd = G.d_desc;
+    const WorkItem *w = G.d_work;
+    launch_named("k_gm_correct", k_gm_correct<float, HET>, G.nwork, G.stream, dd, w, (int)F_Z);
+    vcycle(lv, G.stream, MG_CORRECT, F_Z, F_X, false, true);
+    launch(k_ir_correct, G.nwork, G.stream, dd, w);
+    launch_named("k_gm_resid", k_gm_resid<float, HET>, G.nwork, G.stream, dd, w, (int)F_Z, 1);
+    FinArgs a{};
+    a.rtol2 = rtol2;
+    a.inner_rtol2 = inner2;
+    a.max_refine = 2000;
+    a.mg = 1;
+    finish(G, FIN_GM_IR, a);
+    vcycle(lv, G.stream, MG_INIT, F_Z, F_X, true, true);
+    finish(G, FIN_MG_RZ0, a);
+    for (int k = 0; k < iters; k++) {
+        launch_named("k_gm_apply", k_gm_apply<float, HET>, G.nwork, G.stream, dd, w, (int)F_X, (int)F_Q);
+        launch_named("k_gm_dots", k_gm_dots<float>, G.nwork, G.stream, dd, w, (int)F_Q);
+        finish(G, FIN_GM_H, a);
+        launch_named("k_gm_orth", k_gm_orth<float, HET>, G.nwork, G.stream, dd, w, (int)F_Q, (int)F_Z, 1);
+        finish(G, FIN_GM_NORM, a);
+        vcycle(lv, G.stream, MG_RUNNING, F_Z, F_X, false, true);
     }
 }
 
@@ -937,6 +999,25 @@ static void steady_gmres(Model &M, DevModel &D) {
     M.stats.steady_rel_resid = D.h_scal[S_BB] > 0 ? std::sqrt(D.h_scal[S_RR] / D.h_scal[S_BB]) : 0.0;
 }
 
+// Non-symmetric G with the multigrid preconditioner: FP32 GMRES cycles
+// (each to a 1e-4 relative residual) inside FP64 refinement.
+static void steady_gmres_mg(Model &M, DevModel &D) {
+    ensure_gmres(D, true);
+    build_mg(D);
+    const std::vector<MgGroup> lv = mg_levels_of(D);
+    const double rtol2 = M.opt.steady_rtol * M.opt.steady_rtol;
+    write_scal(D, kMixedStart);
+    for (int guard = 0;; guard++) {
+        gm_round_mg<true>(D.grp, lv, rtol2, 1e-8, kGmM);
+        read_scal(D);
+        if (D.h_scal[S_DONE] != 0.0) break;
+        if (guard > 100000) fatal("non-symmetric steady solve did not finish\n");
+    }
+    M.stats.steady_iters = (int)D.h_scal[S_TITERS];
+    M.stats.steady_refinements = (int)D.h_scal[S_NREFINE];
+    M.stats.steady_rel_resid = D.h_scal[S_BB] > 0 ? std::sqrt(D.h_scal[S_RR] / D.h_scal[S_BB]) : 0.0;
+}
+
 void dev_steady(Model &M, const double *block_power, double *block_temp) {
     dev_prepare(M);
     DevModel &D = *M.dev;
@@ -949,7 +1030,8 @@ void dev_steady(Model &M, const double *block_power, double *block_temp) {
     }
     write_scal(D, {{S_DONE, 0.0}});
     if (D.h.has_adv) {
-        if (M.opt.precision == HS_PREC_MIXED) steady_gmres<float, true>(M, D);
+        if (M.opt.steady_solver == HS_STEADY_MGPCG) steady_gmres_mg(M, D);
+        else if (M.opt.precision == HS_PREC_MIXED) steady_gmres<float, true>(M, D);
         else steady_gmres<double, true>(M, D);
     } else if (M.opt.precision == HS_PREC_MIXED || M.opt.steady_solver == HS_STEADY_MGPCG) {
         if (D.hetero) steady_mixed<true>(M, D);

@@ -162,7 +162,8 @@ def test_engine_equals_single_model_api(lib, jobs, form):
     sel = [j for j in jobs if j.name in ("default-64", "detailed-16", "steady-16", "secondary-16-init")]
     res = run_engine(lib, sel, form)[0]
     for j, (r, seq, info) in zip(sel, res):
-        set_options(lib, j.run, precision=form[0], transient=form[1])
+        # the mixed engine solves steady states with multigrid-preconditioned CG
+        set_options(lib, j.run, precision=form[0], transient=form[1], steady_solver=form[0])
         if j.from_init:
             T0 = j.init
         else:
@@ -174,7 +175,7 @@ def test_engine_equals_single_model_api(lib, jobs, form):
             s = transient_run(lib, j.run, j.seq, T0)
             assert np.array_equal(s, seq), j.name
             assert np.array_equal(s[-1], r), j.name
-        set_options(lib, j.run, precision=0, transient=0)
+        set_options(lib, j.run, precision=0, transient=0, steady_solver=0)
 
 
 def test_engine_fp64_matches_oracle(lib, jobs):
@@ -248,7 +249,8 @@ def test_engine_microchannel_steady_equals_single_model_api(lib, form):
     seq = eng.result_seq(k)
     eng.close()
     assert info.steady_iters > 0 and info.steady_refinements > 0
-    set_options(lib, mc, precision=form[0], transient=form[1])
+    # the mixed engine preconditions with multigrid, the FP64 one with Jacobi
+    set_options(lib, mc, precision=form[0], transient=form[1], steady_solver=form[0])
     T = mc.steady(P.copy())
     check_error(lib)
     assert np.array_equal(T, r)
