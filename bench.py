@@ -322,11 +322,56 @@ def mode_serve(lib, args):
             "speedup_vs_fp64": (out["mixed"]["value"] / out["fp64"]["value"]) if "fp64" in out else None}
 
 
+def mode_large(lib, args):
+    """Large steady solve: a --large-grid^2 x 8-layer stack with two
+    microchannel layers (2-cell channels on a 4-cell pitch, 25.6 mm die,
+    water at 300 K), 256 blocks with 8 hotspots; FP32 multigrid-preconditioned
+    GMRES inside FP64 refinement to a 1e-12 relative FP64 residual."""
+    d = tempfile.mkdtemp(prefix="hslarge")
+    log(f"building the {args.large_grid}^2 two-channel stack")
+    t0 = time.perf_counter()
+    case, pw = cases.two_channel_stack_case(args.large_grid, 7, d)
+    run = cases.Run(lib, case, d)
+    log(f"model built in {time.perf_counter() - t0:.1f} s")
+    P = run.power_vector(pw)
+    o = Opts()
+    L = lib.lib
+    L.hs_get_options(run.model, C.byref(o))
+    o.precision, o.steady_solver = 1, 1
+    L.hs_set_options(run.model, C.byref(o))
+    check(lib)
+    run.steady(P.copy())    # warm-up: device set-up and multigrid hierarchy
+    check(lib)
+    times, st = [], None
+    for k in range(args.steps):
+        t = time.perf_counter()
+        T = run.steady(P.copy())
+        check(lib)
+        times.append(time.perf_counter() - t)
+        s_ = Stats()
+        L.hs_get_stats(run.model, C.byref(s_))
+        st = s_
+        log(f"steady {k}: {times[-1]*1e3:.0f} ms, {st.steady_iters} GMRES iterations, "
+            f"{st.steady_refinements} refinements, residual {st.steady_rel_resid:.1e}")
+    g = run.grid
+    cells = g.n_layers * g.rows * g.cols
+    res = {"metric": "steady solve time (large two-microchannel stack)", "mode": "large",
+           "value": cells / float(np.mean(times)), "unit": "cell-updates/s", "n_gpus": 1,
+           "ms_per_solve": float(np.mean(times)) * 1e3, "device_ms": st.last_solve_ms,
+           "gmres_iterations": st.steady_iters, "refinements": st.steady_refinements,
+           "rel_residual_fp64": st.steady_rel_resid, "t_max_K": float(T.max()),
+           "config": {"workload": f"{args.large_grid}x{args.large_grid}x8, two microchannel layers, 256 blocks, "
+                                  f"8 hotspots", "cells": cells}}
+    run.free()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", choices=["batch", "many", "serve"], default="batch",
+    ap.add_argument("--mode", choices=["batch", "many", "serve", "large"], default="batch",
                     help="batch: the headline served batch (default); many: uneven batched problems; "
-                         "serve: Poisson request stream")
+                         "serve: Poisson request stream; large: 1024^2 x 8 two-microchannel steady solve")
+    ap.add_argument("--large-grid", type=int, default=1024)
     ap.add_argument("--many", type=int, default=512)
     ap.add_argument("--many-steps", type=int, default=500)
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -355,7 +400,7 @@ def main():
     bind(lib)
     L = lib.lib
     if args.mode != "batch":
-        res = mode_many(lib, args) if args.mode == "many" else mode_serve(lib, args)
+        res = {"many": mode_many, "serve": mode_serve, "large": mode_large}[args.mode](lib, args)
         print(json.dumps(res), flush=True)
         return
 

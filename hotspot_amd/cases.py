@@ -324,6 +324,52 @@ def microchannel_case(n: int, seed: int, dirname: str, die=4e-3, width_cells=1,
                 microchannel=dict(pumping_pressure=pumping_pressure)), powers
 
 
+def channel_pressure(n: int, die: float, velocity: float, t_channel=100e-6, visc=0.000889) -> float:
+    """Pumping pressure that drives `velocity` (m/s) through straight
+    channels of n cells of die/n x die/n x t_channel: each row of channel
+    cells is a series of n - 1 cell-to-cell rectangular-duct conductances
+    (the reference's hydroC, microchannel.c:361-376)."""
+    cw = die / n
+    h, w, L = t_channel, cw, cw
+    if h == w:
+        hc = 0.42229 * h ** 4 / (12 * visc * L)
+    elif h > w:
+        hc = (1 - 0.63 * (w / h)) * w ** 3 * h / (12 * visc * L)
+    else:
+        hc = (1 - 0.63 * (h / w)) * h ** 3 * w / (12 * visc * L)
+    return velocity * (n - 1) * cw * t_channel / hc
+
+
+def two_channel_stack_case(n: int, seed: int, dirname: str, die=25.6e-3, width_cells=2, pitch_cells=4,
+                           velocity=1.0, n_blocks=256, hotspots=8, density=0.5,
+                           **config) -> tuple[Case, list]:
+    """Device Si 50 um (power) / Si 100 um / microchannel 100 um / Si 100 um /
+    microchannel 100 um / Si cap 100 um over the default spreader and sink,
+    detailed 3D; straight west->east water channels width_cells wide on a
+    pitch_cells pitch (2 and 4 cells: 50 um channels at a 100 um pitch on a
+    25.6 mm die at 1024 x 1024), inlet 300 K, the pumping pressure for the
+    given mean velocity; n_blocks slicing blocks with density lognormal(ln
+    density W/mm^2, 0.6) and `hotspots` blocks at 10x the mean."""
+    pumping_pressure = channel_pressure(n, die, velocity)
+    g = rng(seed)
+    os.makedirs(dirname, exist_ok=True)
+    csv = os.path.join(dirname, "channels.csv")
+    write_network_csv(csv, channel_network(n, n, width_cells, pitch_cells))
+    dev = slicing_floorplan(g, n_blocks, die, die, aspect=(0.3, 3.4), prefix="dev")
+    area = np.array([u[1] * u[2] for u in dev]) * 1e6
+    dens = g.lognormal(np.log(density), 0.6, size=len(dev))
+    if hotspots:
+        dens[g.choice(len(dev), size=hotspots, replace=False)] = 10 * density
+    si = lambda name, t: LcfLayer(True, False, 1.75e6, 0.01, t, tiled_floorplan(1, 1, die, die, prefix=name))
+    layers = [LcfLayer(True, True, 1.75e6, 0.01, 50e-6, dev), si("si1_", 100e-6),
+              LcfLayer(True, False, 1.6303e6, 1.0 / 130.0, 100e-6, csv), si("si2_", 100e-6),
+              LcfLayer(True, False, 1.6303e6, 1.0 / 130.0, 100e-6, csv), si("cap_", 100e-6)]
+    cfg = dict(grid_rows=n, grid_cols=n, s_spreader=max(30e-3, die + 2e-3), s_sink=max(60e-3, die + 30e-3))
+    cfg.update(config)
+    return Case(f"two-channel stack {n}x{n}", cfg, lcf=layers, detailed_3D=True, use_microchannels=True,
+                microchannel=dict(pumping_pressure=pumping_pressure, inlet_temperature=300.0)), [dens * area]
+
+
 def jittered_powers(base: np.ndarray, n_steps: int, seed: int, jitter=0.2):
     """n_steps copies of a power vector, each scaled per block by
     U(1-jitter, 1+jitter) with a seeded generator."""

@@ -253,3 +253,24 @@ def test_microchannel_steady_multigrid_at_scale(lib):
     assert np.abs(T - To).max() <= FP64_TOL
     assert st.steady_rel_resid <= 1e-10 and st.steady_iters < 1000
     run.free()
+
+
+def test_two_channel_stack_multigrid(lib):
+    """Device Si / Si / channels / Si / channels / Si cap over the package
+    (the large steady configuration's stack) at 64 x 64: FP32 multigrid-
+    preconditioned GMRES inside FP64 refinement against the oracle's direct
+    solve."""
+    d = tempfile.mkdtemp()
+    case, pw = cases.two_channel_stack_case(64, 7, d)
+    run = cases.Run(lib, case, d)
+    set_options(lib, run, precision=1, steady_solver=1)
+    P = run.power_vector(pw)
+    T = run.steady(P.copy())
+    check_error(lib)
+    st = stats(lib, run)
+    om = oracle_for(run)
+    To, Tg = om.steady(P)
+    assert np.abs(T - To).max() <= FP64_TOL
+    assert np.abs(grid_temp(lib, run, 0) - Tg).max() <= FP64_TOL
+    assert st.steady_rel_resid <= 1e-10
+    run.free()
