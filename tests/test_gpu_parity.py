@@ -1,7 +1,7 @@
 """Parity of the HIP path (through the C ABI) with the oracle and, where it
 was built, the reference library, on seeded inputs.
 
-Tolerances (BASELINE north star): FP64 GPU path within 1e-4 K of the
+Tolerances: FP64 GPU path within 1e-4 K of the
 converged reference operator; mixed-precision path within 1e-2 K.
 """
 import tempfile
