@@ -421,6 +421,7 @@ def main():
         times.append(t)
         admit_t.append(ta)
         lat += [i.latency_s for i in infos]
+        step_sweeps = float(sum(i.taylor_sweeps for i in infos))
         log(f"step {k}: batch of {args.batch} in {t:.3f} s (admission {ta*1e3:.0f} ms), "
             f"steady iters {[i.steady_iters for i in infos][:4]}...")
     t_step = float(np.mean(times))
@@ -429,14 +430,16 @@ def main():
     # roofline pass: the same batch with every launch of the dominant kernel
     # bracketed by device events on the engine's stream (plain launches
     # instead of graph replay while the timer is on)
-    # Algorithmic bytes of one term sweep of one job: FP32 read of E_k, read and
-    # write of the accumulator, write of E_k+1 = 16 B per node.  A launch
-    # skips the jobs whose series has already converged, so the achieved rate
-    # is the bytes of the sweeps the jobs actually took (counted on the
-    # device) over the summed duration of every launch, skipping ones included.
+    # Algorithmic bytes of one Horner step h <- E_1 + (tau/j) M h of one job:
+    # FP32 read of h, read of E_1, write of the new h = 12 B per node; the
+    # first step of a sub-step reads h = E_1, one array, so it moves 8 B.  A
+    # launch skips the jobs whose series needs fewer steps, so the achieved
+    # rate is the bytes of the steps the jobs actually took (counted on the
+    # device; one first step per job, interval and sub-step) over the summed
+    # duration of every launch, skipping ones included.
     achieved = avg_ms = nt = sweeps = None
     nvec = cells + iface.EXTRA
-    bytes_per_sweep = nvec * 16
+    bytes_per_sweep = nvec * 12
     bytes_per_launch = args.batch * bytes_per_sweep
     if not args.no_roofline:
         t_roof, _, rinfo, _ = served(lib, jobs, args.intervals, args.tick, timer=b"k_tay_term")
@@ -446,9 +449,7 @@ def main():
         L.hs_kernel_timer(None, 0)
         avg_ms = a.value
         sweeps = float(sum(i.taylor_sweeps for i in rinfo))
-        achieved = sweeps * bytes_per_sweep / (tot.value * 1e-3) / 1e9 if nt else None
-        log(f"roofline pass {t_roof:.3f} s: {nt} launches of k_tay_term, mean {avg_ms*1e3:.2f} us, "
-            f"{sweeps / args.batch / nt:.3f} of the job sweeps active per launch, {achieved:.0f} GB/s")
+        tot_ms = tot.value
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_k_tay_term_3dstack{args.grid}_batch{args.batch}.json")
     if os.path.exists(tf):
@@ -468,6 +469,11 @@ def main():
     # the served result of the same job equals it bit for bit
     _, _, _, served_seq = served(lib, jobs[:1], args.intervals, args.tick, keep_first=True)
     single["served_equals_single"] = bool(np.array_equal(served_seq, seq_mixed))
+    if nt:
+        first = min(sweeps, float(args.batch * args.intervals * st.taylor_substeps))
+        achieved = (sweeps * bytes_per_sweep - first * nvec * 4) / (tot_ms * 1e-3) / 1e9
+        log(f"roofline pass: {nt} launches of k_tay_term, mean {avg_ms*1e3:.2f} us, "
+            f"{sweeps / args.batch / nt:.3f} of the job steps active per launch, {achieved:.0f} GB/s")
 
     fp64 = None
     if not args.no_fp64:
@@ -510,13 +516,15 @@ def main():
                        "note": "submission -> result on the host, admission included"},
         "admission_ms": float(np.mean(admit_t)) * 1e3,
         "value_with_admission": args.batch * updates_job / (t_step + float(np.mean(admit_t))),
-        "roofline": {"bound": "hbm", "kernel": "k_tay_term<float,false>",
+        "roofline": {"bound": "hbm", "kernel": "k_tay_term4 (FP32 Horner step of the Taylor series, float4 tiles)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic, "launches_timed": nt,
                      "avg_launch_us": avg_ms * 1e3 if avg_ms else None,
                      "bytes_per_launch": bytes_per_launch,
                      "active_sweep_fraction": (sweeps / args.batch / nt) if nt else None,
+                     "steps_per_batch": step_sweeps,
+                     "first_steps_per_batch": float(args.batch * args.intervals * single["taylor_substeps"]),
                      "traffic_note": "HBM bytes per launch from FETCH_SIZE and WRITE_SIZE (rocprofv3 "
                                      "--pmc, scaled against a plain copy)"},
         "single_job": single,

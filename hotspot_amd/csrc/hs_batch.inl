@@ -48,7 +48,7 @@ struct BatchGroup {
     Group g;
     std::vector<EngJob *> members;
     bool dirty = true;
-    size_t cap_desc = 0, cap_work = 0, cap_gwork = 0, cap_flags = 0;
+    size_t cap_desc = 0, cap_work = 0, cap_gwork = 0, cap_flags = 0, cap_trec = 0;
     double *d_flags = nullptr, *h_flags = nullptr;
     Replayer rp;
     SeqForm form{};
@@ -66,7 +66,10 @@ struct BatchGroup {
         if (g.d_gwork) (void)hipFree(g.d_gwork);
         if (d_flags) (void)hipFree(d_flags);
         if (h_flags) (void)hipHostFree(h_flags);
+        if (g.d_kmax) (void)hipFree(g.d_kmax);
+        if (g.d_trec) (void)hipFree(g.d_trec);
         g = Group{};
+        cap_trec = 0;
         d_flags = h_flags = nullptr;
         cap_desc = cap_work = cap_gwork = cap_flags = 0;
     }
@@ -159,6 +162,8 @@ static void group_sync(Engine &E, BatchGroup &B, hipStream_t s, bool het, bool t
     ensure_dev(B.g.d_desc, B.cap_desc, (size_t)n);
     ensure_dev(B.g.d_work, B.cap_work, w.size());
     ensure_dev(B.g.d_gwork, B.cap_gwork, gw.size());
+    if (!B.g.d_kmax) HIP_OK(hipMalloc((void **)&B.g.d_kmax, sizeof(int)));
+    ensure_dev(B.g.d_trec, B.cap_trec, (size_t)2 * n);
     if ((size_t)n * 8 > B.cap_flags) {
         if (B.d_flags) HIP_OK(hipFree(B.d_flags));
         if (B.h_flags) HIP_OK(hipHostFree(B.h_flags));
@@ -175,6 +180,9 @@ static void group_sync(Engine &E, BatchGroup &B, hipStream_t s, bool het, bool t
     B.g.stream = s;
     B.g.hetero = het;
     B.g.vec4 = v4;
+    B.g.nper = (n > 0 && (int)w.size() == n * (desc[0].ntiles + 1)) ? desc[0].ntiles + 1 : 0;
+    for (int i = 0; i < n && B.g.nper; i++)
+        if (desc[i].ntiles != desc[0].ntiles) B.g.nper = 0;
     B.form = SeqForm{E.opt.precision, E.opt.transient, m, K};
     // multigrid levels of a steady group (every member has the same count)
     for (void *p : B.lv_bufs) HIP_OK(hipFree(p));

@@ -912,6 +912,8 @@ enum FinOp { FIN_CG_INIT = 0, FIN_ALPHA, FIN_BETA, FIN_IR, FIN_TAY, FIN_GM_IR, F
 struct FinArgs {
     double rtol2, inner_rtol2;
     int max_iters, max_refine, done_slot, tol_slot, sub, seq, mg;
+    int *kmax;     // FIN_TAY: largest term count of the group
+    double *trec;  // FIN_TAY: per problem {tau, terms (0: no sub-step)}
 };
 
 // ---------------------------------------------------------------------------
@@ -1680,15 +1682,21 @@ __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ 
         case FIN_TAY: {
             // number of terms: the smallest K whose geometric tail bound from
             // max|E_1| (||E_{k+1}|| <= ||E_k|| rho tau/(k+1)) is under tol
-            if (a.sub >= (int)s[S_TK] || seq_finished(d, a.seq)) return;
+            if (a.sub >= (int)s[S_TK] || seq_finished(d, a.seq)) {
+                if (threadIdx.x == 0) {
+                    a.trec[2 * blockIdx.x] = 0.0;
+                    a.trec[2 * blockIdx.x + 1] = 0.0;
+                }
+                return;
+            }
             double w[1];
             reduce_slots<1, true>(d, w);
             if (threadIdx.x) return;
             const double z = s[S_RHO] * s[S_TAU];
-            const int kmax = (int)s[S_KPLAN];
+            const int kplan = (int)s[S_KPLAN];
             double term = w[0];
             int K = 1;
-            while (K < kmax) {
+            while (K < kplan) {
                 term *= z / (K + 1);
                 const double q = z / (K + 2);
                 if (q < 1.0 && term / (1.0 - q) <= s[S_TTOL]) break;
@@ -1697,6 +1705,9 @@ __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ 
             s[S_TMAX] = w[0];
             s[S_KNEED] = K;
             s[S_TTERMS] += K - 1;
+            a.trec[2 * blockIdx.x] = s[S_TAU];
+            a.trec[2 * blockIdx.x + 1] = K;
+            atomicMax(a.kmax, K);
             return;
         }
     }
@@ -1754,31 +1765,38 @@ __global__ void k_scatter(const ProbDesc *__restrict__ D, int nprob, Slots sl, c
 // ---------------------------------------------------------------------------
 // transient: exponential (Taylor) integrator of the linear ODE over a
 // sub-step tau (P constant within an interval):
-//   y(t + tau) = y + sum_{k>=1} E_k,  E_1 = tau f(y),
-//   E_{k+1} = tau/(k+1) (-C^-1 G) E_k
-// E_1 comes from the FP64 residual; the increments E_k are kept in TD (FP32
-// in the mixed form: they are temperature changes, not temperatures).  Each
-// term kernel reduces max|E_{k+1}| so the last workgroup can stop the series
-// once the geometric tail bound (||C^-1 G||_inf <= rho) is under tolerance.
+//   y(t + tau) = y + sum_{k=1..K} E_k,  E_1 = tau f(y),
+//   E_{k+1} = tau/(k+1) M E_k,  M = -C^-1 G (the operator without sources).
+// The sum is evaluated in nested (Horner) form,
+//   h = E_1 + (tau M/2)(E_1 + (tau M/3)(E_1 + ... + (tau M/K) E_1)),
+// one sweep per step h <- E_1 + (tau/j) M h for j = K, ..., 2: a sweep reads
+// h (with its halo) and E_1 and writes h, 12 bytes per FP32 node instead of
+// the 16 of a forward sweep that also carries the running sum.  E_1 comes
+// from the FP64 residual; h and E_1 are kept in TD (FP32 in the mixed form:
+// they are temperature changes, not temperatures).  K comes from max|E_1|
+// and the geometric tail bound (||C^-1 G||_inf <= rho), so the number of
+// steps is known before the first one.  A group launches the steps of its
+// largest plan Kg as j = Kg, ..., 2; a problem with K terms takes the steps
+// j <= K and skips the others, and a launch with j above every problem's K
+// ends at its first load.
 // ---------------------------------------------------------------------------
 template <typename TD, bool HET>
 __global__ void __launch_bounds__(kBlock) k_tay_first(const ProbDesc *__restrict__ D,
-                                                      const WorkItem *__restrict__ W, int d0b,
-                                                      int accb, int sub, int seq) {
+                                                      const WorkItem *__restrict__ W, int e1b, int sub,
+                                                      int seq, int *__restrict__ kmax) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *kmax = 0;   // raised by the finish of each problem
     const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     if (sub >= (int)d.scal[S_TK] || seq_finished(d, seq)) return;
     const int ypar = (int)d.scal[S_YPAR];
     const double *y = d.b64[B_Y0 + ypar], *rhs = d.b64[B_RHS];
-    auto buf = [&](int b) -> TD * { return sizeof(TD) == 8 ? (TD *)d.b64[b] : (TD *)d.b32[b]; };
-    TD *ek = buf(d0b), *acc = buf(accb);
+    TD *e1 = sizeof(TD) == 8 ? (TD *)d.b64[e1b] : (TD *)d.b32[e1b];
     const double tau = d.scal[S_TAU];
     LdPlain<double> ld{y};
     double v[1] = {0};
     auto fin = [&](unsigned c, double f) {
         const TD e = (TD)(tau * f);
-        st_at(ek, c, e);
-        st_at(acc, c, e);
+        st_at(e1, c, e);
         v[0] = fmax(v[0], fabs((double)e));
     };
     if (wi.tile < d.ntiles) {
@@ -1816,7 +1834,47 @@ __global__ void __launch_bounds__(kBlock) k_tay_first(const ProbDesc *__restrict
     write_partials<1>(d, wi.tile, v);
 }
 
-// FP32 term sweep of a homogeneous problem with float4 accesses: within the
+// Buffers of Horner step j of a problem with K = S_KNEED terms: the first
+// step (j = K) reads h = E_1; the steps then alternate d0 -> d1 -> d0.
+struct HornerBufs {
+    int in, out;
+};
+__device__ __forceinline__ HornerBufs horner_bufs(int K, int j, int e1b, int d0b, int d1b) {
+    const int q = K - j;
+    return HornerBufs{q == 0 ? e1b : ((q & 1) ? d0b : d1b), (q & 1) ? d1b : d0b};
+}
+// the buffer that holds the sum after the last step
+__device__ __forceinline__ int horner_result(int K, int e1b, int d0b, int d1b) {
+    return K <= 1 ? e1b : ((K & 1) ? d1b : d0b);
+}
+
+// Problem and tile of this workgroup in a Taylor launch, and the problem's
+// {tau, terms} record of the current sub-step (terms 0: nothing to do).  A
+// group whose problems all have the same tile count (nper > 0) maps the
+// workgroup arithmetically, so the record and the descriptor are the first
+// loads; otherwise through the work-list.
+struct TayItem {
+    int prob, tile, K;
+    double tau;
+};
+__device__ __forceinline__ TayItem tay_item(const WorkItem *__restrict__ W, int nper,
+                                            const double *__restrict__ trec) {
+    TayItem t;
+    const int b = work_index();
+    if (nper > 0) {
+        t.prob = b / nper;
+        t.tile = b - t.prob * nper;
+    } else {
+        const WorkItem wi = W[b];
+        t.prob = wi.prob;
+        t.tile = wi.tile;
+    }
+    t.tau = trec[2 * t.prob];
+    t.K = (int)trec[2 * t.prob + 1];
+    return t;
+}
+
+// FP32 Horner step of a homogeneous problem with float4 accesses: within the
 // same tile, each thread owns 4 consecutive columns x rpt/4 rows (rpt >= 4,
 // tile width >= 16), so a cell costs 2 load instructions instead of 8.  The
 // arithmetic per cell is that of stencil() in the same order.
@@ -1830,8 +1888,8 @@ __device__ __forceinline__ float comp(const float4 &v, int k) {
     return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
 }
 
-__device__ __forceinline__ void term_vec4(const ProbDesc &d, int tile, const float *ein, float *eout,
-                                          float *acc, float s, const float (&ring)[4]) {
+__device__ __forceinline__ void term_vec4(const ProbDesc &d, int tile, const float *hin, float *hout,
+                                          const float *e1, float s, const float (&ring)[4]) {
     const int tpl = d.nti * d.ntj;
     const int n = tile / tpl, rem = tile - n * tpl;
     const int ti = rem / d.ntj, tj = rem - ti * d.ntj;
@@ -1859,14 +1917,14 @@ __device__ __forceinline__ void term_vec4(const ProbDesc &d, int tile, const flo
             const int i = min(I[g], nr - 1);
             const unsigned c = ((unsigned)n * nr + i) * nc + j0;
             cc[g] = c;
-            C[g] = ld4(ein, c);
-            N[g] = ld4(ein, i > 0 ? c - nc : c);
-            S[g] = ld4(ein, i < nr - 1 ? c + nc : c);
-            U[g] = ld4(ein, n > 0 ? c - plane : c);
-            Dn[g] = ld4(ein, n < nl - 1 ? c + plane : c);
-            Wv[g] = ld_at(ein, j0 > 0 ? c - 1 : c);
-            Ev[g] = ld_at(ein, j0 + 4 < nc ? c + 4 : c);
-            A[g] = ld4(acc, c);
+            C[g] = ld4(hin, c);
+            N[g] = ld4(hin, i > 0 ? c - nc : c);
+            S[g] = ld4(hin, i < nr - 1 ? c + nc : c);
+            U[g] = ld4(hin, n > 0 ? c - plane : c);
+            Dn[g] = ld4(hin, n < nl - 1 ? c + plane : c);
+            Wv[g] = ld_at(hin, j0 > 0 ? c - 1 : c);
+            Ev[g] = ld_at(hin, j0 + 4 < nc ? c + 4 : c);
+            A[g] = ld4(e1, c);
         }
     }
 #pragma unroll
@@ -1892,41 +1950,42 @@ __device__ __forceinline__ void term_vec4(const ProbDesc &d, int tile, const flo
             if (i == nr - 1 && rn[1] >= 0) flux += rgn * (ring[1] - xc);
             if (j == nc - 1 && rn[2] >= 0) flux += rge * (ring[2] - xc);
             if (j == 0 && rn[3] >= 0) flux += rge * (ring[3] - xc);
-            e[k] = s * (ci * (flux - gam * xc));
+            e[k] = comp(A[g], k) + s * (ci * (flux - gam * xc));
         }
-        const float4 ev = make_float4(e[0], e[1], e[2], e[3]);
-        st4(eout, cc[g], ev);
-        st4(acc, cc[g], make_float4(A[g].x + e[0], A[g].y + e[1], A[g].z + e[2], A[g].w + e[3]));
+        st4(hout, cc[g], make_float4(e[0], e[1], e[2], e[3]));
     }
 }
 
-// E_{k+1} from E_k (k >= 1); skipped once the series has its K terms
+// Horner step j: h <- E_1 + (tau/j) M h
 template <typename TD, bool HET>
 __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict__ D,
-                                                     const WorkItem *__restrict__ W, int inb,
-                                                     int outb, int accb, int k, int sub, int seq) {
-    const WorkItem wi = W[work_index()];
-    const ProbDesc &d = D[wi.prob];
-    if (sub >= (int)d.scal[S_TK] || k >= (int)d.scal[S_KNEED] || seq_finished(d, seq)) return;
+                                                     const WorkItem *__restrict__ W, int nper, int e1b,
+                                                     int d0b, int d1b, int j, const int *__restrict__ kmax,
+                                                     const double *__restrict__ trec) {
+    if (j > *kmax) return;
+    const TayItem it = tay_item(W, nper, trec);
+    if (j > it.K) return;
+    const ProbDesc &d = D[it.prob];
+    const HornerBufs hb = horner_bufs(it.K, j, e1b, d0b, d1b);
     auto buf = [&](int b) -> TD * { return sizeof(TD) == 8 ? (TD *)d.b64[b] : (TD *)d.b32[b]; };
-    const TD *ein = buf(inb);
-    TD *eout = buf(outb), *acc = buf(accb);
-    const double tau = d.scal[S_TAU];
-    const TD s = (TD)(tau / (k + 1));
-    LdPlain<TD> ld{ein};
+    const TD *hin = buf(hb.in), *e1 = buf(e1b);
+    TD *hout = buf(hb.out);
+    const TD s = (TD)(it.tau / j);
+    LdPlain<TD> ld{hin};
+    const WorkItem wi{it.prob, it.tile};
     if (wi.tile < d.ntiles) {
         TD ring[4];
         load_ring<TD>(d, ld, tile_layer(d, wi.tile), ring);
         struct V {
             Pts<TD, HET> p;
-            TD acc, ci;
+            TD e1, ci;
         };
         tile_loop<InFlight<TD, HET>::value>(
             d, wi.tile,
             [&](const Cell &q) {
                 V t{};
                 load_pts<TD, HET>(d, ld, q, t.p);
-                t.acc = ld_at(acc, q.c);
+                t.e1 = ld_at(e1, q.c);
                 if (HET) t.ci = ld_at(Coef<TD>::cinv(d), q.c);
                 return t;
             },
@@ -1934,9 +1993,7 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict_
                 TD flux, diag, gam;
                 stencil<TD, HET>(d, q, t.p, ring, flux, diag, gam);
                 const TD ci = HET ? t.ci : (TD)d.cinv[q.n];
-                const TD e = s * (ci * (flux - gam * t.p.c));
-                st_at(eout, q.c, e);
-                st_at(acc, q.c, t.acc + e);
+                st_at(hout, q.c, t.e1 + s * (ci * (flux - gam * t.p.c)));
             });
     } else {
         __shared__ TD xf[kMaxExtra], xd[kMaxExtra];
@@ -1944,9 +2001,7 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict_
         const int kk = threadIdx.x;
         if (kk < d.nx) {
             const long c = (long)d.nl * d.nr * d.nc + kk;
-            const TD e = s * ((TD)d.x_cinv[kk] * (xf[kk] - (TD)d.x_gamb[kk] * ein[c]));
-            eout[c] = e;
-            acc[c] += e;
+            hout[c] = e1[c] + s * ((TD)d.x_cinv[kk] * (xf[kk] - (TD)d.x_gamb[kk] * hin[c]));
         }
     }
 }
@@ -1954,43 +2009,46 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict_
 // k_tay_term<float, false> for groups whose problems all take float4 tiles
 // (vec4_ok): the package item is shared with the scalar kernel.
 __global__ void __launch_bounds__(kBlock) k_tay_term4(const ProbDesc *__restrict__ D,
-                                                      const WorkItem *__restrict__ W, int inb,
-                                                      int outb, int accb, int k, int sub, int seq) {
-    const WorkItem wi = W[work_index()];
-    const ProbDesc &d = D[wi.prob];
-    if (sub >= (int)d.scal[S_TK] || k >= (int)d.scal[S_KNEED] || seq_finished(d, seq)) return;
-    const float *ein = d.b32[inb];
-    float *eout = d.b32[outb], *acc = d.b32[accb];
-    const float s = (float)(d.scal[S_TAU] / (k + 1));
-    LdPlain<float> ld{ein};
-    if (wi.tile < d.ntiles) {
+                                                      const WorkItem *__restrict__ W, int nper, int e1b,
+                                                      int d0b, int d1b, int j, const int *__restrict__ kmax,
+                                                      const double *__restrict__ trec) {
+    if (j > *kmax) return;
+    const TayItem it = tay_item(W, nper, trec);
+    if (j > it.K) return;
+    const ProbDesc &d = D[it.prob];
+    const HornerBufs hb = horner_bufs(it.K, j, e1b, d0b, d1b);
+    const float *hin = d.b32[hb.in], *e1 = d.b32[e1b];
+    float *hout = d.b32[hb.out];
+    const float s = (float)(it.tau / j);
+    LdPlain<float> ld{hin};
+    if (it.tile < d.ntiles) {
         float ring[4];
-        load_ring<float>(d, ld, tile_layer(d, wi.tile), ring);
-        term_vec4(d, wi.tile, ein, eout, acc, s, ring);
+        load_ring<float>(d, ld, tile_layer(d, it.tile), ring);
+        term_vec4(d, it.tile, hin, hout, e1, s, ring);
     } else {
         __shared__ float xf[kMaxExtra], xd[kMaxExtra];
         package_terms<float>(d, ld, true, xf, xd);
         const int kk = threadIdx.x;
         if (kk < d.nx) {
             const unsigned c = (unsigned)d.nl * d.nr * d.nc + kk;
-            const float e = s * ((float)d.x_cinv[kk] * (xf[kk] - (float)d.x_gamb[kk] * ein[c]));
-            eout[c] = e;
-            acc[c] += e;
+            hout[c] = e1[c] + s * ((float)d.x_cinv[kk] * (xf[kk] - (float)d.x_gamb[kk] * hin[c]));
         }
     }
 }
 
-// y += acc
+// y += h (the sum of the sub-step's terms)
 template <typename TD>
 __global__ void __launch_bounds__(kBlock) k_tay_final(const ProbDesc *__restrict__ D,
-                                                      const WorkItem *__restrict__ W, int accb,
-                                                      int sub, int seq) {
-    const WorkItem wi = W[work_index()];
-    const ProbDesc &d = D[wi.prob];
-    if (sub >= (int)d.scal[S_TK] || seq_finished(d, seq)) return;
+                                                      const WorkItem *__restrict__ W, int nper, int e1b,
+                                                      int d0b, int d1b, const double *__restrict__ trec) {
+    const TayItem it = tay_item(W, nper, trec);
+    if (it.K == 0) return;
+    const ProbDesc &d = D[it.prob];
+    const WorkItem wi{it.prob, it.tile};
     const int ypar = (int)d.scal[S_YPAR];
     double *y = d.b64[B_Y0 + ypar];
-    const TD *acc = sizeof(TD) == 8 ? (const TD *)d.b64[accb] : (const TD *)d.b32[accb];
+    const int hb = horner_result(it.K, e1b, d0b, d1b);
+    const TD *acc = sizeof(TD) == 8 ? (const TD *)d.b64[hb] : (const TD *)d.b32[hb];
     if (wi.tile < d.ntiles) {
         struct V {
             double y;

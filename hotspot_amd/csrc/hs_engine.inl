@@ -19,6 +19,9 @@ struct Group {
     hipStream_t stream = nullptr;
     bool hetero = false;
     bool vec4 = false;   // every problem takes float4 term sweeps (vec4_ok)
+    int *d_kmax = nullptr;   // Taylor: largest term count of the current sub-step
+    double *d_trec = nullptr;   // Taylor: per problem {tau, terms} of the current sub-step
+    int nper = 0;   // work items per problem when every problem has the same count, else 0
 };
 
 // A homogeneous problem whose tiles are at least 16 columns wide with at
@@ -522,6 +525,9 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
     D->grp.stream = D->stream;
     D->grp.hetero = M.detailed;
     D->grp.vec4 = vec4_ok(h);
+    D->grp.d_kmax = D->alloc<int>(1);
+    D->grp.d_trec = D->alloc<double>(2);
+    D->grp.nper = D->nwork;
     if (D->hetero) assemble_hetero(M, *D);
     // transient step bound
     if (g.c_ready) {
@@ -1117,7 +1123,9 @@ static std::vector<std::pair<int, double>> taylor_slots(const TaylorPlan &p, dou
 }
 
 // One interval of every problem: m sub-steps of at most K terms (the largest
-// plan of the group; each problem stops at its own).
+// plan of the group).  E_1 goes to the accumulator buffer; the Horner steps
+// j = K..2 alternate between the two term buffers, and each problem takes the
+// last of them that its own term count needs.
 template <typename TD, bool HET>
 static void taylor_interval(const Group &G, int m, int K, int seq) {
     const ProbDesc *dd = G.d_desc;
@@ -1125,20 +1133,24 @@ static void taylor_interval(const Group &G, int m, int K, int seq) {
     const bool f32 = sizeof(TD) == 4;
     const int db0 = f32 ? F_D0 : B_KA, db1 = f32 ? F_D1 : B_KB, accb = f32 ? F_ACC : B_ACC;
     for (int s = 0; s < m; s++) {
-        launch_named("k_tay_first", k_tay_first<TD, HET>, G.nwork, G.stream, dd, w, db0, accb, s, seq);
+        launch_named("k_tay_first", k_tay_first<TD, HET>, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax);
         FinArgs a{};
         a.sub = s;
         a.seq = seq;
+        a.kmax = G.d_kmax;
+        a.trec = G.d_trec;
         finish(G, FIN_TAY, a);
-        for (int k = 1; k < K; k++) {
+        const int *km = G.d_kmax;
+        const double *tr = G.d_trec;
+        for (int j = K; j >= 2; j--) {
             if (f32 && !HET && G.vec4)
-                launch_named("k_tay_term", k_tay_term4, G.nwork, G.stream, dd, w, (k & 1) ? db0 : db1,
-                             (k & 1) ? db1 : db0, accb, k, s, seq);
+                launch_named("k_tay_term", k_tay_term4, G.nwork, G.stream, dd, w, G.nper, accb, db0, db1, j, km,
+                             tr);
             else
-                launch_named("k_tay_term", k_tay_term<TD, HET>, G.nwork, G.stream, dd, w, (k & 1) ? db0 : db1,
-                             (k & 1) ? db1 : db0, accb, k, s, seq);
+                launch_named("k_tay_term", k_tay_term<TD, HET>, G.nwork, G.stream, dd, w, G.nper, accb, db0, db1,
+                             j, km, tr);
         }
-        launch_named("k_tay_final", k_tay_final<TD>, G.nwork, G.stream, dd, w, accb, s, seq);
+        launch_named("k_tay_final", k_tay_final<TD>, G.nwork, G.stream, dd, w, G.nper, accb, db0, db1, tr);
     }
 }
 
