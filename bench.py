@@ -383,7 +383,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--intervals", type=int, default=10000)
-    ap.add_argument("--batch", type=int, default=8, help="jobs served together per step")
+    ap.add_argument("--batch", type=int, default=16, help="jobs served together per step")
     ap.add_argument("--tick", type=int, default=100, help="transient intervals per engine tick")
     ap.add_argument("--cpu-intervals", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhotspot_amd.so")
 
-SOURCES = ["hs_host.cpp", "hs_device.hip"]
+SOURCES = ["hs_host.cpp", "hs_config.cpp", "hs_device.hip"]
 DEPS = ["hs_internal.h", "hs_kernels.h", "hs_engine.inl", "hs_batch.inl",
         os.path.join("..", "..", "include", "hotspot_amd.h")]
 
@@ -57,7 +57,25 @@ def build(verbose: bool = True, jobs: int = 2) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_cli(verbose)
     return LIB
+
+
+CLI = os.path.join(HERE, "hotspot")
+
+
+def build_cli(verbose: bool = True) -> str:
+    """Links the command-line simulator (cli/hotspot.cpp) against the library,
+    found next to it at run time."""
+    tmp = CLI + ".tmp"
+    cmd = [hipcc(), "-O2", "-std=c++17", "-Wall", "-x", "c++", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(HERE, "cli", "hotspot.cpp"), "-L" + HERE, "-lhotspot_amd",
+           "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -40,7 +40,7 @@ static thread_local int g_has_error = 0;
 void fatal(const std::string &msg) { throw Error{msg}; }
 void warning(const std::string &msg) { fprintf(stderr, "warning: %s", msg.c_str()); }
 
-static void report(const std::string &msg) {
+void report(const std::string &msg) {
     if (g_exit_on_error) {
         fprintf(stderr, "error: %s", msg.c_str());
         exit(1);
@@ -860,22 +860,6 @@ static const char *layer_prefix(const Model &M, int n, char *buf) {
 }  // namespace hs
 
 using namespace hs;
-
-#define HS_BEGIN try {
-#define HS_END(ret)                                   \
-    }                                                 \
-    catch (const hs::Error &e) {                      \
-        hs::report(e.msg);                            \
-        return ret;                                   \
-    }                                                 \
-    catch (const std::bad_alloc &) {                  \
-        hs::report("memory allocation error\n");      \
-        return ret;                                   \
-    }                                                 \
-    catch (const std::exception &e) {                 \
-        hs::report(std::string(e.what()) + "\n");     \
-        return ret;                                   \
-    }
 
 extern "C" {
 

@@ -34,6 +34,26 @@ struct Error {
 };
 [[noreturn]] void fatal(const std::string &msg);
 void warning(const std::string &msg);
+// at the C boundary: print and exit (the reference's fatal) or record for
+// hs_last_error, by hs_set_fatal_mode
+void report(const std::string &msg);
+
+// body of a C entry point: errors raised inside become report()
+#define HS_BEGIN try {
+#define HS_END(ret)                                   \
+    }                                                 \
+    catch (const hs::Error &e) {                      \
+        hs::report(e.msg);                            \
+        return ret;                                   \
+    }                                                 \
+    catch (const std::bad_alloc &) {                  \
+        hs::report("memory allocation error\n");      \
+        return ret;                                   \
+    }                                                 \
+    catch (const std::exception &e) {                 \
+        hs::report(std::string(e.what()) + "\n");     \
+        return ret;                                   \
+    }
 
 // ---- block<->grid map of one distinct floorplan ---------------------------
 struct BlockMap {

@@ -191,6 +191,20 @@ RCModelP = C.POINTER(RCModel)
 FlpP = C.POINTER(Flp)
 
 
+class StrPair(C.Structure):
+    """str_pair (util.h:92-96)"""
+    _fields_ = [("name", C.c_char * STR_SIZE), ("value", C.c_char * STR_SIZE)]
+
+
+def str_table(pairs, size: int = 512):
+    """A str_pair array holding (name, value) pairs."""
+    t = (StrPair * size)()
+    for i, (k, v) in enumerate(pairs):
+        t[i].name = k.encode()
+        t[i].value = v.encode()
+    return t
+
+
 class HotSpotLib:
     """Loads a library exporting HotSpot's C API and binds its signatures.
 
@@ -253,6 +267,30 @@ class HotSpotLib:
         L.get_blk_index.argtypes = [FlpP, C.c_char_p]
         L.dump_steady_temp_grid.restype = None
         L.dump_steady_temp_grid.argtypes = [C.POINTER(GridModel), C.c_char_p]
+        # configuration tables (util.c:322-439, temperature.c:115-402, microchannel.c:49-156)
+        SP = C.POINTER(StrPair)
+        L.read_str_pairs.argtypes = [SP, C.c_int, C.c_char_p]
+        L.read_str_pairs.restype = C.c_int
+        L.parse_cmdline.argtypes = [SP, C.c_int, C.c_int, C.POINTER(C.c_char_p)]
+        L.parse_cmdline.restype = C.c_int
+        L.str_pairs_remove_duplicates.argtypes = [SP, C.c_int]
+        L.str_pairs_remove_duplicates.restype = C.c_int
+        L.get_str_index.argtypes = [SP, C.c_int, C.c_char_p]
+        L.get_str_index.restype = C.c_int
+        L.dump_str_pairs.argtypes = [SP, C.c_int, C.c_char_p, C.c_char_p]
+        L.dump_str_pairs.restype = None
+        L.thermal_config_add_from_strs.argtypes = [C.POINTER(ThermalConfig), C.POINTER(MaterialsList), SP,
+                                                   C.c_int]
+        L.thermal_config_add_from_strs.restype = None
+        L.thermal_config_to_strs.argtypes = [C.POINTER(ThermalConfig), SP, C.c_int]
+        L.thermal_config_to_strs.restype = C.c_int
+        L.microchannel_config_add_from_strs.argtypes = [C.POINTER(MicrochannelConfig), C.POINTER(MaterialsList),
+                                                        SP, C.c_int]
+        L.microchannel_config_add_from_strs.restype = None
+        L.microchannel_config_to_strs.argtypes = [C.POINTER(MicrochannelConfig), SP, C.c_int]
+        L.microchannel_config_to_strs.restype = C.c_int
+        L.materials_add_from_file.argtypes = [C.POINTER(MaterialsList), C.c_char_p]
+        L.materials_add_from_file.restype = None
 
     # ---- thin wrappers -------------------------------------------------
     def default_config(self, **overrides) -> ThermalConfig:

@@ -267,6 +267,34 @@ void dump_steady_temp_grid(grid_model_t *model, char *file);
 void dump_transient_temp_grid(grid_model_t *model, int trace_num, double sampling_intvl,
                               char *filename);
 
+/* ---- configuration tables: util.h:92-117, util.c:322-439 --------------- */
+typedef struct str_pair_st {
+    char name[STR_SIZE];
+    char value[STR_SIZE];
+} str_pair;
+
+/* util.c:322-353: "-name value" lines of a file ('#' comments); returns the count */
+int read_str_pairs(str_pair *table, int max_entries, char *file);
+/* util.c:359-376: "-name value" pairs of a command line */
+int parse_cmdline(str_pair *table, int max_entries, int argc, char **argv);
+/* util.c:379-391: writes "<prefix>name\tvalue" lines */
+void dump_str_pairs(str_pair *table, int size, char *file, char *prefix);
+/* util.c:394-404: case-insensitive lookup, -1 if absent */
+int get_str_index(str_pair *table, int size, char *str);
+/* util.c:426-439: keeps the first of entries with the same name; returns the new size */
+int str_pairs_remove_duplicates(str_pair *table, int size);
+/* temperature.c:115-339: overrides config fields named in the table, then validates */
+void thermal_config_add_from_strs(thermal_config_t *config, materials_list_t *materials_list,
+                                  str_pair *table, int size);
+/* temperature.c:345-402: the 51 fields as name/value pairs; returns 51 */
+int thermal_config_to_strs(thermal_config_t *config, str_pair *table, int max_entries);
+/* microchannel.c:49-108 */
+void microchannel_config_add_from_strs(microchannel_config_t *config, materials_list_t *materials_list,
+                                       str_pair *table, int size);
+/* microchannel.c:114-156: writes 17 name/value pairs and returns 16, as the
+ * reference does (the last pair, n_fluid_cells, is not counted) */
+int microchannel_config_to_strs(microchannel_config_t *config, str_pair *table, int max_entries);
+
 /* ---- extensions --------------------------------------------------------
  * None of these has a counterpart in the reference.
  */

@@ -1,0 +1,93 @@
+"""Writes the command-line golden files under tests/golden/cli/: the inputs of
+the reference's examples 2-5 (floorplans, layer files, power traces, configs,
+materials: data files) and the outputs of the reference's own command-line
+simulator on them, run with the command lines of the examples' run.sh.
+
+Run in a container that holds the reference sources, after
+``make -C oracle ref cli`` (which compiles the reference from its sources into
+oracle/_ref/).  The GPU tests compare the command-line simulator of this
+library with these outputs.
+"""
+from __future__ import annotations
+
+import gzip
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference/examples"
+EXE = os.path.join(ROOT, "oracle", "_ref", "hotspot_ref")
+OUT = os.path.join(HERE, "cli")
+
+# case -> (example dir, input files, runs); a run is (argv, outputs to keep).
+# "@init" in argv names the steady file of the case's first run as input.
+CASES = {
+    "ev6": ("example2", ["ev6.flp", "gcc.ptrace", "example.config", "example.materials"], [
+        ("-c example.config -f ev6.flp -p gcc.ptrace -materials_file example.materials -model_type grid "
+         "-steady_file gcc.steady -grid_steady_file gcc.grid.steady", ["gcc.steady", "gcc.grid.steady"]),
+        ("-c example.config -init_file @init -f ev6.flp -p gcc.ptrace -materials_file example.materials "
+         "-model_type grid -o gcc.ttrace", ["gcc.ttrace"]),
+    ]),
+    "lcf_detailed": ("example3", ["example.config", "example.lcf", "example.materials", "example.ptrace",
+                                  "floorplan1.flp", "floorplan2.flp"], [
+        ("-c example.config -p example.ptrace -grid_layer_file example.lcf -materials_file example.materials "
+         "-model_type grid -detailed_3D on -steady_file example.steady -grid_steady_file example.grid.steady",
+         ["example.steady", "example.grid.steady"]),
+        ("-c example.config -p example.ptrace -grid_layer_file example.lcf -materials_file example.materials "
+         "-model_type grid -detailed_3D on -init_file @init -o example.ttrace "
+         "-grid_transient_file example.grid.ttrace", ["example.ttrace", "example.grid.ttrace"]),
+    ]),
+    "ev6_3d": ("example4", ["example.config", "ev6_3D.lcf", "ev6_3D.ptrace", "ev6_3D_TIM.flp",
+                            "ev6_3D_TIM_TSV.flp", "ev6_3D_cache_1.flp", "ev6_3D_cache_2.flp",
+                            "ev6_3D_core_layer.flp"], [
+        ("-c example.config -p ev6_3D.ptrace -grid_layer_file ev6_3D.lcf -model_type grid -detailed_3D on "
+         "-grid_steady_file example.grid.steady -steady_file example.steady",
+         ["example.steady", "example.grid.steady"]),
+        ("-c example.config -p ev6_3D.ptrace -grid_layer_file ev6_3D.lcf -init_file @init -model_type grid "
+         "-detailed_3D on -o example.transient", ["example.transient"]),
+    ]),
+}
+
+
+def run_case(name, spec):
+    exdir, inputs, runs = spec
+    dst = os.path.join(OUT, name)
+    os.makedirs(os.path.join(dst, "expected"), exist_ok=True)
+    for f in inputs:
+        shutil.copy(os.path.join(REF, exdir, f), os.path.join(dst, f))
+    with tempfile.TemporaryDirectory() as tmp:
+        for f in inputs:
+            shutil.copy(os.path.join(REF, exdir, f), os.path.join(tmp, f))
+        init = None
+        for argv, keep in runs:
+            args = argv.split()
+            if "@init" in args:
+                args[args.index("@init")] = init
+            print(name, " ".join(args), flush=True)
+            subprocess.run([EXE] + args, cwd=tmp, check=True, stdout=subprocess.DEVNULL)
+            for k in keep:
+                src = os.path.join(tmp, k)
+                if ".grid." in k:   # per-cell dumps: kept compressed
+                    with open(src, "rb") as fi, gzip.open(os.path.join(dst, "expected", k + ".gz"), "wb") as fo:
+                        fo.write(fi.read())
+                else:
+                    shutil.copy(src, os.path.join(dst, "expected", k))
+            if init is None:
+                init = keep[0]
+                shutil.copy(os.path.join(tmp, init), os.path.join(tmp, init + ".init"))
+                init = init + ".init"
+
+
+def main():
+    if not os.path.exists(EXE):
+        sys.exit("build the reference first: make -C oracle ref cli")
+    for name, spec in CASES.items():
+        run_case(name, spec)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,105 @@
+"""The command-line simulator (hotspot_amd/hotspot) on the reference's
+examples 2-4, against the reference's own command-line outputs on the same
+inputs (tests/golden/cli, written by tests/golden/make_cli_goldens.py).
+
+Both print temperatures with "%.2f".  Steady: the reference stops its
+Gauss-Seidel sweeps at a 1e-6 K change per sweep, a few 1e-4 to 1e-3 K from
+the converged state that this library reaches, so a printed value may differ
+by one unit of the last digit (0.01 K) and by no more.  Transient: both
+start from the reference's printed steady state (its -init_file) and
+integrate with the same adaptive RK4, within 1e-8 K of each other, so the
+printed traces agree except where a value sits on a rounding boundary:
+at most 0.01 K, and at least 99 % of the entries exactly.
+"""
+import gzip
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI_DIR = os.path.join(ROOT, "tests", "golden", "cli")
+EXE = os.path.join(ROOT, "hotspot_amd", "hotspot")
+PRINT_UNIT = 0.01
+
+pytestmark = pytest.mark.gpu
+
+# case -> (steady argv, its steady file, transient argv with @init, outputs to compare)
+CASES = {
+    "ev6": ("-c example.config -f ev6.flp -p gcc.ptrace -materials_file example.materials -model_type grid "
+            "-steady_file gcc.steady -grid_steady_file gcc.grid.steady", ["gcc.steady", "gcc.grid.steady"],
+            "-c example.config -init_file @init -f ev6.flp -p gcc.ptrace -materials_file example.materials "
+            "-model_type grid -o gcc.ttrace", ["gcc.ttrace"]),
+    "lcf_detailed": ("-c example.config -p example.ptrace -grid_layer_file example.lcf -materials_file "
+                     "example.materials -model_type grid -detailed_3D on -steady_file example.steady "
+                     "-grid_steady_file example.grid.steady", ["example.steady", "example.grid.steady"],
+                     "-c example.config -p example.ptrace -grid_layer_file example.lcf -materials_file "
+                     "example.materials -model_type grid -detailed_3D on -init_file @init -o example.ttrace "
+                     "-grid_transient_file example.grid.ttrace", ["example.ttrace", "example.grid.ttrace"]),
+    "ev6_3d": ("-c example.config -p ev6_3D.ptrace -grid_layer_file ev6_3D.lcf -model_type grid -detailed_3D on "
+               "-grid_steady_file example.grid.steady -steady_file example.steady",
+               ["example.steady", "example.grid.steady"],
+               "-c example.config -p ev6_3D.ptrace -grid_layer_file ev6_3D.lcf -init_file @init -model_type grid "
+               "-detailed_3D on -o example.transient", ["example.transient"]),
+}
+
+
+def read_text(path):
+    if os.path.exists(path):
+        return open(path).read()
+    with gzip.open(path + ".gz", "rt") as f:
+        return f.read()
+
+
+def numbers(text):
+    """Every numeric field of a HotSpot output file, in order (names,
+    'Layer n:' and 't = ...' headers give their numbers too: they must
+    match exactly)."""
+    out = []
+    for tok in text.split():
+        try:
+            out.append(float(tok))
+        except ValueError:
+            out.append(tok)
+    return out
+
+
+def compare(got_text, want_text, exact_frac):
+    got, want = numbers(got_text), numbers(want_text)
+    assert len(got) == len(want)
+    strs = [(g, w) for g, w in zip(got, want) if isinstance(w, str)]
+    assert all(g == w for g, w in strs)
+    g = np.array([x for x in got if not isinstance(x, str)])
+    w = np.array([x for x in want if not isinstance(x, str)])
+    d = np.abs(g - w)
+    assert d.max() <= PRINT_UNIT + 1e-9, d.max()
+    assert (d == 0).mean() >= exact_frac, (d == 0).mean()
+    return d.max(), (d == 0).mean()
+
+
+def run_cli(argv, cwd):
+    print("cli:", argv, flush=True)
+    r = subprocess.run([EXE] + argv.split(), cwd=cwd, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_cli_matches_reference_outputs(gpu, case, tmp_path):
+    src = os.path.join(CLI_DIR, case)
+    for f in os.listdir(src):
+        if os.path.isfile(os.path.join(src, f)):
+            shutil.copy(os.path.join(src, f), tmp_path / f)
+    steady_argv, steady_out, trans_argv, trans_out = CASES[case]
+    run_cli(steady_argv, tmp_path)
+    for f in steady_out:
+        m, ex = compare(open(tmp_path / f).read(), read_text(os.path.join(src, "expected", f)), 0.5)
+        print(f, "max |diff|", m, "exact", ex)
+    # the transient starts from the reference's printed steady state
+    init = "ref.init"
+    shutil.copy(os.path.join(src, "expected", steady_out[0]), tmp_path / init)
+    run_cli(trans_argv.replace("@init", init), tmp_path)
+    for f in trans_out:
+        m, ex = compare(open(tmp_path / f).read(), read_text(os.path.join(src, "expected", f)), 0.99)
+        print(f, "max |diff|", m, "exact", ex)
