@@ -1888,6 +1888,8 @@ __device__ __forceinline__ float comp(const float4 &v, int k) {
     return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
 }
 
+// Grid sizes are powers of two, so a row holds whole tiles (tile width
+// min(nc, 64)) and every float4 group lies inside its row.
 __device__ __forceinline__ void term_vec4(const ProbDesc &d, int tile, const float *hin, float *hout,
                                           const float *e1, float s, const float (&ring)[4]) {
     const int tpl = d.nti * d.ntj;

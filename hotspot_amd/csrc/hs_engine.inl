@@ -25,8 +25,11 @@ struct Group {
 };
 
 // A homogeneous problem whose tiles are at least 16 columns wide with at
-// least 4 rows per thread runs its FP32 term sweeps with float4 accesses.
-static bool vec4_ok(const ProbDesc &h) { return !h.hetero && h.rpt >= 4 && h.tx_shift >= 4; }
+// least 4 rows per thread runs its FP32 term sweeps with float4 accesses
+// (rows hold whole tiles: grid sizes are powers of two).
+static bool vec4_ok(const ProbDesc &h) {
+    return !h.hetero && h.rpt >= 4 && h.tx_shift >= 4 && h.nc % (1 << h.tx_shift) == 0;
+}
 
 // One multigrid level of one problem: its descriptor (the finest level is
 // the problem's own) and the column work-list (2D tiles, all layers per
@@ -711,7 +714,10 @@ static void build_mg(DevModel &D) {
     l0.ncwork = (int)w0.size();
     D.mg.push_back(l0);
     bool prev_het = D.hetero;
-    while (D.mg.back().h.nr >= 8 && D.mg.back().h.nc >= 8) {
+    // coarsen while both sizes are even (always, for power-of-two grids):
+    // every fine cell has its parent (i/2, j/2)
+    while (D.mg.back().h.nr >= 8 && D.mg.back().h.nc >= 8 && D.mg.back().h.nr % 2 == 0 &&
+           D.mg.back().h.nc % 2 == 0) {
         const ProbDesc f = D.mg.back().h;
         MgLevel L;
         ProbDesc &h = L.h;

@@ -274,3 +274,48 @@ def test_two_channel_stack_multigrid(lib):
     assert np.abs(grid_temp(lib, run, 0) - Tg).max() <= FP64_TOL
     assert st.steady_rel_resid <= 1e-10
     run.free()
+
+
+# Rectangular grids (sizes are powers of two, as the reference requires):
+# 512 x 16 and 256 x 32 take float4 Taylor sweeps with 16- and 32-column
+# tiles; 64 x 256 has four column tiles per row and scalar sweeps.
+RECT = [("rect-512x16", cases.default_stack_case(64, grid_rows=512, grid_cols=16),
+         [cases.rng(3).uniform(0.5, 5, 16)]),
+        ("rect-256x32", cases.default_stack_case(64, grid_rows=256, grid_cols=32),
+         [cases.rng(4).uniform(0.5, 5, 16)]),
+        ("rect-64x256", cases.default_stack_case(64, grid_rows=64, grid_cols=256),
+         [cases.rng(6).uniform(0.5, 5, 16)])]
+
+
+@pytest.mark.parametrize("name,case,pw", RECT, ids=[c[0] for c in RECT])
+def test_rectangular_grid_steady_multigrid(lib, name, case, pw):
+    run = make(lib, case)
+    set_options(lib, run, precision=1, steady_solver=1)
+    om = oracle_for(run)
+    P = run.power_vector(pw)
+    T = run.steady(P.copy())
+    check_error(lib)
+    To, Tg = om.steady(P)
+    assert np.abs(T - To).max() <= FP64_TOL
+    assert np.abs(grid_temp(lib, run, 0) - Tg).max() <= FP64_TOL
+    assert stats(lib, run).steady_rel_resid <= 1e-10
+    run.free()
+
+
+@pytest.mark.parametrize("name,case,pw", RECT, ids=[c[0] for c in RECT])
+def test_rectangular_grid_taylor_matches_exact(lib, name, case, pw):
+    run = make(lib, case)
+    set_options(lib, run, transient=1, precision=1)
+    om = oracle_for(run)
+    P = run.power_vector(pw)
+    T0, _ = om.steady(P)
+    seq = cases.jittered_powers(P, 5, seed=9)
+    tr = cases.run_transient(run, T0, seq, 3.333e-6)
+    check_error(lib)
+    Tg = om.b2g(T0, power=False)
+    worst = 0.0
+    for k, p in enumerate(seq):
+        Tg = om.advance_exact(Tg, om.power_grid(p), 3.333e-6)
+        worst = max(worst, np.abs(tr[k] - om.g2b(Tg)).max())
+    assert worst <= 1e-4
+    run.free()
