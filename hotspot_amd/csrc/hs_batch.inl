@@ -48,7 +48,7 @@ struct BatchGroup {
     Group g;
     std::vector<EngJob *> members;
     bool dirty = true;
-    size_t cap_desc = 0, cap_work = 0, cap_gwork = 0, cap_flags = 0, cap_trec = 0;
+    size_t cap_desc = 0, cap_work = 0, cap_gwork = 0, cap_flags = 0, cap_trec = 0, cap_twork = 0;
     double *d_flags = nullptr, *h_flags = nullptr;
     Replayer rp;
     SeqForm form{};
@@ -68,8 +68,9 @@ struct BatchGroup {
         if (h_flags) (void)hipHostFree(h_flags);
         if (g.d_kmax) (void)hipFree(g.d_kmax);
         if (g.d_trec) (void)hipFree(g.d_trec);
+        if (g.d_twork) (void)hipFree(g.d_twork);
         g = Group{};
-        cap_trec = 0;
+        cap_trec = cap_twork = 0;
         d_flags = h_flags = nullptr;
         cap_desc = cap_work = cap_gwork = cap_flags = 0;
     }
@@ -184,6 +185,14 @@ static void group_sync(Engine &E, BatchGroup &B, hipStream_t s, bool het, bool t
     for (int i = 0; i < n && B.g.nper; i++)
         if (desc[i].ntiles != desc[0].ntiles) B.g.nper = 0;
     B.form = SeqForm{E.opt.precision, E.opt.transient, m, K};
+    // the float4 term sweep's work-list; its tiling timed for FP32 Taylor groups
+    if (trans && v4 && E.opt.transient == HS_TRANS_TAYLOR && E.opt.precision == HS_PREC_MIXED) {
+        tune_term_tiling(B.g, desc, B.cap_twork);
+    } else {
+        std::vector<WorkItem> tw;
+        const int per = term_items(desc, tw);
+        upload_term_items(B.g, tw, per, B.cap_twork);
+    }
     // multigrid levels of a steady group (every member has the same count)
     for (void *p : B.lv_bufs) HIP_OK(hipFree(p));
     B.lv_bufs.clear();

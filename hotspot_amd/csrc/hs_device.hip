@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "hs_kernels.h"
@@ -1890,15 +1891,18 @@ __device__ __forceinline__ float comp(const float4 &v, int k) {
 
 // Grid sizes are powers of two, so a row holds whole tiles (tile width
 // min(nc, 64)) and every float4 group lies inside its row.
+__device__ __forceinline__ int term_ntj(const ProbDesc &d) { return d.nc >> d.t_shift; }
+__device__ __forceinline__ int term_layer(const ProbDesc &d, int tile) { return tile / (d.t_nti * term_ntj(d)); }
+
 __device__ __forceinline__ void term_vec4(const ProbDesc &d, int tile, const float *hin, float *hout,
                                           const float *e1, float s, const float (&ring)[4]) {
-    const int tpl = d.nti * d.ntj;
+    const int ntj = term_ntj(d), tpl = d.t_nti * ntj;
     const int n = tile / tpl, rem = tile - n * tpl;
-    const int ti = rem / d.ntj, tj = rem - ti * d.ntj;
-    const int sh = d.tx_shift;
+    const int ti = rem / ntj, tj = rem - ti * ntj;
+    const int sh = d.t_shift;
     const int tx4 = threadIdx.x & ((1 << (sh - 2)) - 1), ty4 = threadIdx.x >> (sh - 2);
-    const int G = d.rpt >> 2;
-    const int rows = (kBlock >> sh) * d.rpt;
+    const int G = d.t_rpt >> 2;
+    const int rows = (kBlock >> sh) * d.t_rpt;
     const int nr = d.nr, nc = d.nc, nl = d.nl;
     const unsigned plane = (unsigned)nr * nc;
     const int j0 = (tj << sh) + 4 * tx4;
@@ -2009,7 +2013,8 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict_
 }
 
 // k_tay_term<float, false> for groups whose problems all take float4 tiles
-// (vec4_ok): the package item is shared with the scalar kernel.
+// (vec4_ok), over the term tiling (t_*) and its work-list: the package item
+// is the scalar kernel's.
 __global__ void __launch_bounds__(kBlock) k_tay_term4(const ProbDesc *__restrict__ D,
                                                       const WorkItem *__restrict__ W, int nper, int e1b,
                                                       int d0b, int d1b, int j, const int *__restrict__ kmax,
@@ -2023,9 +2028,9 @@ __global__ void __launch_bounds__(kBlock) k_tay_term4(const ProbDesc *__restrict
     float *hout = d.b32[hb.out];
     const float s = (float)(it.tau / j);
     LdPlain<float> ld{hin};
-    if (it.tile < d.ntiles) {
+    if (it.tile < d.t_ntiles) {
         float ring[4];
-        load_ring<float>(d, ld, tile_layer(d, it.tile), ring);
+        load_ring<float>(d, ld, term_layer(d, it.tile), ring);
         term_vec4(d, it.tile, hin, hout, e1, s, ring);
     } else {
         __shared__ float xf[kMaxExtra], xd[kMaxExtra];

@@ -22,13 +22,16 @@ struct Group {
     int *d_kmax = nullptr;   // Taylor: largest term count of the current sub-step
     double *d_trec = nullptr;   // Taylor: per problem {tau, terms} of the current sub-step
     int nper = 0;   // work items per problem when every problem has the same count, else 0
+    // work-list of the float4 Taylor sweep over the term tiling (t_*)
+    WorkItem *d_twork = nullptr;
+    int ntwork = 0, tnper = 0;
 };
 
 // A homogeneous problem whose tiles are at least 16 columns wide with at
 // least 4 rows per thread runs its FP32 term sweeps with float4 accesses
 // (rows hold whole tiles: grid sizes are powers of two).
 static bool vec4_ok(const ProbDesc &h) {
-    return !h.hetero && h.rpt >= 4 && h.tx_shift >= 4 && h.nc % (1 << h.tx_shift) == 0;
+    return !h.hetero && h.t_rpt >= 4 && h.t_shift >= 4 && h.nc % (1 << h.t_shift) == 0;
 }
 
 // One multigrid level of one problem: its descriptor (the finest level is
@@ -61,6 +64,8 @@ struct DevModel {
     long vlen = 0;
     double rho = 0.0;                   // bound on ||C^-1 G||_inf (transient)
     bool hetero = false, has_c = false;
+    size_t cap_twork = 0;               // grp.d_twork capacity (its own allocation)
+    bool term_tuned = false;            // term tiling chosen by timing
 
     template <typename T> T *alloc(size_t n) {
         void *p = nullptr;
@@ -78,6 +83,7 @@ struct DevModel {
     ~DevModel() {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : allocs) (void)hipFree(p);
+        if (grp.d_twork) (void)hipFree(grp.d_twork);
         if (h_scal) (void)hipHostFree(h_scal);
         if (h_blk) (void)hipHostFree(h_blk);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -143,6 +149,16 @@ static int rows_per_thread(int nr, int ty) {
     return r;
 }
 
+// tiling of the float4 Taylor sweep: tile width 1 << shift (at most nc),
+// rpt rows per thread
+static void set_term_tiling(ProbDesc &h, int shift, int rpt) {
+    const int ty = kBlock >> shift;
+    h.t_shift = shift;
+    h.t_rpt = rpt;
+    h.t_nti = (h.nr + ty * rpt - 1) / (ty * rpt);
+    h.t_ntiles = h.nl * h.t_nti * (h.nc >> shift);
+}
+
 static void set_tiling(ProbDesc &h) {
     h.tx_shift = ilog2(std::min(h.nc, 64));
     const int tx = 1 << h.tx_shift, ty = kBlock >> h.tx_shift;
@@ -151,7 +167,113 @@ static void set_tiling(ProbDesc &h) {
     h.nti = (h.nr + ty * h.rpt - 1) / (ty * h.rpt);
     h.ntiles = h.nl * h.nti * h.ntj;
     h.nti2 = (h.nr + ty - 1) / ty;
+    set_term_tiling(h, h.tx_shift, h.rpt);
 }
+
+// Work-list of the float4 term sweep: (problem, tile) over each problem's
+// term tiling, its package item last.  Returns the items per problem when
+// every problem has the same count, else 0.
+static int term_items(const std::vector<ProbDesc> &desc, std::vector<WorkItem> &w) {
+    w.clear();
+    int per = desc.empty() ? 0 : desc[0].t_ntiles + 1;
+    for (int i = 0; i < (int)desc.size(); i++) {
+        for (int t = 0; t <= desc[i].t_ntiles; t++) w.push_back(WorkItem{i, t});
+        if (desc[i].t_ntiles + 1 != per) per = 0;
+    }
+    return per;
+}
+
+// Tile shape of the float4 term sweep, picked by timing.  Candidates: tile
+// widths of 16 to 128 columns (at most nc) with 4 or 8 rows per thread.
+// Each is timed over a few sweeps of the group's own problems on its stream
+// (one Horner step reading the E_1 buffer and writing a term buffer: both
+// are free between intervals), and the fastest kept.  The arithmetic of a
+// cell is the same under every tiling, so the choice changes only the speed.
+// Choices are remembered per group composition (problem shapes).
+struct TermShape {
+    int shift = 0, rpt = 0;
+};
+static std::map<std::vector<int>, TermShape> &term_shape_cache() {
+    static std::map<std::vector<int>, TermShape> m;
+    return m;
+}
+
+static void upload_term_items(Group &G, const std::vector<WorkItem> &w, int per, size_t &cap) {
+    if (w.size() > cap) {
+        if (G.d_twork) HIP_OK(hipFree(G.d_twork));
+        cap = std::max(w.size(), 2 * cap);
+        HIP_OK(hipMalloc((void **)&G.d_twork, cap * sizeof(WorkItem)));
+    }
+    HIP_OK(hipMemcpy(G.d_twork, w.data(), w.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    G.ntwork = (int)w.size();
+    G.tnper = per;
+}
+
+static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap) {
+    std::vector<int> key;
+    for (const ProbDesc &h : desc) key.insert(key.end(), {h.nl, h.nr, h.nc});
+    auto apply = [&](TermShape t) {
+        for (ProbDesc &h : desc) set_term_tiling(h, std::min(t.shift, ilog2(h.nc)), t.rpt);
+        HIP_OK(hipMemcpy(G.d_desc, desc.data(), desc.size() * sizeof(ProbDesc), hipMemcpyHostToDevice));
+        std::vector<WorkItem> w;
+        const int per = term_items(desc, w);
+        upload_term_items(G, w, per, cap);
+    };
+    static const bool off = getenv("HS_NO_TUNE") != nullptr;
+    auto &cache = term_shape_cache();
+    auto hit = cache.find(key);
+    if (hit != cache.end() || off) {
+        if (hit != cache.end()) apply(hit->second);
+        else apply(TermShape{desc[0].t_shift, desc[0].t_rpt});
+        return;
+    }
+    int min_nc = 1 << 30;
+    for (const ProbDesc &h : desc) min_nc = std::min(min_nc, h.nc);
+    // every problem active for one step j = 2 (its first: E_1 buffer -> term buffer)
+    const int n = (int)desc.size();
+    std::vector<double> rec(2 * n);
+    for (int i = 0; i < n; i++) {
+        rec[2 * i] = 1e-9;
+        rec[2 * i + 1] = 2.0;
+    }
+    const int two = 2;
+    HIP_OK(hipMemcpy(G.d_trec, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(G.d_kmax, &two, sizeof(int), hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    TermShape best;
+    float best_ms = 1e30f;
+    for (int shift = 4; shift <= std::min(7, ilog2(min_nc)); shift++)
+        for (int rpt : {4, 8}) {
+            const TermShape t{shift, rpt};
+            apply(t);
+            constexpr int kWarm = 2, kTimed = 6;
+            for (int r = 0; r < kWarm + kTimed; r++) {
+                if (r == kWarm) HIP_OK(hipEventRecord(e0, G.stream));
+                hipLaunchKernelGGL(k_tay_term4, dim3(G.ntwork), dim3(kBlock), 0, G.stream,
+                                   (const ProbDesc *)G.d_desc, (const WorkItem *)G.d_twork, G.tnper, (int)F_ACC,
+                                   (int)F_D0, (int)F_D1, 2, (const int *)G.d_kmax, (const double *)G.d_trec);
+            }
+            HIP_OK(hipEventRecord(e1, G.stream));
+            HIP_OK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+            static const bool log = getenv("HS_TUNE_LOG") != nullptr;
+            if (log)
+                fprintf(stderr, "term tiling %d problems: %d columns x %d rows per thread: %.2f us per sweep\n",
+                        n, 1 << shift, rpt, 1e3f * ms / kTimed);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        }
+    HIP_OK(hipEventDestroy(e0));
+    HIP_OK(hipEventDestroy(e1));
+    cache[key] = best;
+    apply(best);
+}
+
 
 // reduction slots a problem needs: layer tiles or column tiles, + package item
 static int slot_count(const ProbDesc &h) { return std::max(h.ntiles, h.nti2 * h.ntj) + 1; }
@@ -531,6 +653,11 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
     D->grp.d_kmax = D->alloc<int>(1);
     D->grp.d_trec = D->alloc<double>(2);
     D->grp.nper = D->nwork;
+    {
+        std::vector<WorkItem> tw;
+        const int per = term_items({h}, tw);
+        upload_term_items(D->grp, tw, per, D->cap_twork);
+    }
     if (D->hetero) assemble_hetero(M, *D);
     // transient step bound
     if (g.c_ready) {
@@ -1150,8 +1277,8 @@ static void taylor_interval(const Group &G, int m, int K, int seq) {
         const double *tr = G.d_trec;
         for (int j = K; j >= 2; j--) {
             if (f32 && !HET && G.vec4)
-                launch_named("k_tay_term", k_tay_term4, G.nwork, G.stream, dd, w, G.nper, accb, db0, db1, j, km,
-                             tr);
+                launch_named("k_tay_term", k_tay_term4, G.ntwork, G.stream, dd, (const WorkItem *)G.d_twork,
+                             G.tnper, accb, db0, db1, j, km, tr);
             else
                 launch_named("k_tay_term", k_tay_term<TD, HET>, G.nwork, G.stream, dd, w, G.nper, accb, db0, db1,
                              j, km, tr);
@@ -1169,10 +1296,21 @@ static void transient_taylor(Model &M, DevModel &D, double dt) {
     M.stats.taylor_substeps = p.m;
 }
 
+// the float4 term tiling of a single model, timed on its first FP32 Taylor use
+static void ensure_term_tuned(const Model &M, DevModel &D) {
+    if (D.term_tuned || !D.grp.vec4 || M.opt.transient != HS_TRANS_TAYLOR || M.opt.precision != HS_PREC_MIXED)
+        return;
+    std::vector<ProbDesc> d{D.h};
+    tune_term_tiling(D.grp, d, D.cap_twork);
+    D.h = d[0];
+    D.term_tuned = true;
+}
+
 void dev_transient(Model &M, const double *block_power, const double *block_temp_in, double dt,
                    double *block_temp_out) {
     dev_prepare(M);
     DevModel &D = *M.dev;
+    ensure_term_tuned(M, D);
     HIP_OK(hipEventRecord(D.ev0, D.stream));
     upload_blocks(D, block_power);
     run_b2g(D.grp, V_POWER, B_RHS, 1);
@@ -1292,6 +1430,7 @@ void dev_transient_run(Model &M, const double *power_seq, int steps, double dt, 
                        double *temp_seq) {
     dev_prepare(M);
     DevModel &D = *M.dev;
+    ensure_term_tuned(M, D);
     if (!init_temp && !M.trans_valid) fatal("hs_transient_run: no transient state to continue from\n");
     const size_t n = (size_t)steps * D.nblk;
     double *seq_in = nullptr, *seq_out = nullptr;

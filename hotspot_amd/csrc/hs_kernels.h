@@ -74,6 +74,10 @@ struct ProbDesc {
     int hetero, has_adv, nblocks;
     int rpt;                          // grid rows per thread in a tile
     int nti2;                         // column tiles along the rows (multigrid: all layers per thread)
+    // tiling of the float4 Taylor sweep (chosen by timing; the arithmetic of
+    // a cell does not depend on it): tile width 1 << t_shift, t_rpt rows per
+    // thread, t_nti tiles along the rows, t_ntiles tiles, then the package item
+    int t_shift, t_rpt, t_nti, t_ntiles;
     double ambient;
     double cell_area;
     // homogeneous layer coefficients
