@@ -321,6 +321,7 @@ __device__ __forceinline__ void stencil(const ProbDesc &d, const Cell &q, const 
     if (j == nc - 1 && rn[2] >= 0) { flux += rge * (ring[2] - xc); diag += rge; }
     if (j == 0 && rn[3] >= 0) { flux += rge * (ring[3] - xc); diag += rge; }
     gam = (T)d.gamb[n];
+    if (HET && d.bshift) gam += (T)d.bshift[q.c];
     diag += gam;
 }
 
@@ -388,6 +389,15 @@ __device__ __forceinline__ void package_terms(const ProbDesc &d, const L &ld, bo
         }
     }
     __syncthreads();
+}
+
+// diagonal of G at package node k: ambient, ring and coupling conductances
+template <typename T> __device__ __forceinline__ T pkg_diag(const ProbDesc &d, int k) {
+    const int L_ = d.x_ring_layer[k];
+    T diag = (T)d.x_gamb[k];
+    if (L_ >= 0) diag += (T)d.x_ring_g[k] * (T)(d.x_ring_edge[k] < 2 ? d.nc : d.nr);
+    for (int q = 0; q < d.x_ncpl[k]; q++) diag += (T)d.x_cpl_g[k][q];
+    return diag;
 }
 
 __device__ __forceinline__ bool problem_skip(const ProbDesc &d, int flag_slot) {
@@ -661,12 +671,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_upd(const ProbDesc *__restrict__ 
         if (threadIdx.x < d.nx) {
             // diagonal of a package node: ring and coupling conductances + ambient
             const int k = threadIdx.x;
-            const int L_ = d.x_ring_layer[k];
-            T diag = (T)d.x_gamb[k];
-            if (L_ >= 0) diag += (T)d.x_ring_g[k] * (T)(d.x_ring_edge[k] < 2 ? d.nc : d.nr);
-            for (int q = 0; q < d.x_ncpl[k]; q++) diag += (T)d.x_cpl_g[k][q];
             const long c = (long)d.nl * d.nr * d.nc + k;
-            upd(c, x[c], p[c], r[c], qv[c], T(1) / diag);
+            upd(c, x[c], p[c], r[c], qv[c], T(1) / pkg_diag<T>(d, k));
         }
     }
     block_reduce<2>(v);
@@ -730,13 +736,6 @@ __global__ void __launch_bounds__(kBlock) k_ir_resid(const ProbDesc *__restrict_
 }
 
 // diagonal of G at a package node: ring and coupling conductances + ambient
-template <typename T> __device__ __forceinline__ T pkg_diag(const ProbDesc &d, int k) {
-    const int L_ = d.x_ring_layer[k];
-    T diag = (T)d.x_gamb[k];
-    if (L_ >= 0) diag += (T)d.x_ring_g[k] * (T)(d.x_ring_edge[k] < 2 ? d.nc : d.nr);
-    for (int q = 0; q < d.x_ncpl[k]; q++) diag += (T)d.x_cpl_g[k][q];
-    return diag;
-}
 
 template <typename T> __device__ __forceinline__ T *vec_of(const ProbDesc &d, int b) {
     return sizeof(T) == 8 ? (T *)d.b64[b] : (T *)d.b32[b];
@@ -2133,6 +2132,51 @@ struct AsmIn {
     int nl, nr, nc, detailed;
     double cw, ch;
 };
+
+// ---------------------------------------------------------------------------
+// backward Euler (the SuperLU path's integrator, RCutil.c:268-428 as called
+// by compute_temp_grid :3632-3662): one implicit step over the interval,
+//   (C/h + G) y' = C y / h + P + sources,
+// solved by CG with a descriptor whose ambient conductances carry the C/h
+// shift.  G is the assembled matrix of that path, whose secondary package
+// links are the steady ones (build_transient_grid_matrix, both directions of
+// the solder-substrate link), and the secondary-path package nodes have no
+// capacitance in it (build_diagonal_matrix :6750-6768).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double be_cap(const ProbDesc &d, const Cell &q, bool het) {
+    return het ? 1.0 / d.cinvc[q.c] : 1.0 / d.cinv[q.n];
+}
+
+// out = rhs + sigma C y (descriptor without the shift)
+template <bool HET>
+__global__ void __launch_bounds__(kBlock) k_be_rhs(const ProbDesc *__restrict__ D,
+                                                   const WorkItem *__restrict__ W, int yb, int rhsb, int outb,
+                                                   double sigma) {
+    const WorkItem wi = W[work_index()];
+    const ProbDesc &d = D[wi.prob];
+    const double *y = d.b64[yb], *rhs = d.b64[rhsb];
+    double *out = d.b64[outb];
+    if (wi.tile < d.ntiles) {
+        tile_loop<8>(
+            d, wi.tile, [&](const Cell &q) { return ld_at(rhs, q.c) + sigma * be_cap(d, q, HET) * ld_at(y, q.c); },
+            [&](const Cell &q, double v) { st_at(out, q.c, v); });
+    } else if ((int)threadIdx.x < d.nx) {
+        const int k = threadIdx.x;
+        const unsigned c = (unsigned)d.nl * d.nr * d.nc + k;
+        out[c] = rhs[c] + (k < EXTRA ? sigma / d.x_cinv[k] : 0.0) * y[c];
+    }
+}
+
+// per-cell shift sigma C and the shifted Jacobi diagonal of a heterogeneous model
+__global__ void k_be_coef(const ProbDesc *__restrict__ D, long ncell, double sigma, double *shift,
+                          double *dinv) {
+    const ProbDesc &d = *D;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const double s = sigma / d.cinvc[c];
+    shift[c] = s;
+    dinv[c] = 1.0 / (1.0 / d.dinvc[c] + s);
+}
 
 __global__ void k_assemble(AsmIn in, double *gE, double *gS, double *gD, double *cinv,
                            double *aN, double *aS, double *aE, double *aW, double *aSelf,

@@ -66,6 +66,11 @@ struct DevModel {
     bool hetero = false, has_c = false;
     size_t cap_twork = 0;               // grp.d_twork capacity (its own allocation)
     bool term_tuned = false;            // term tiling chosen by timing
+    // backward Euler: descriptor of the shifted operator G + C/h (its group
+    // shares the work-lists), per-cell shift and Jacobi diagonal (hetero)
+    ProbDesc *d_be_desc = nullptr;
+    Group be_grp;
+    double *be_shift = nullptr, *be_dinv = nullptr;
 
     template <typename T> T *alloc(size_t n) {
         void *p = nullptr;
@@ -760,9 +765,9 @@ static void cg_iterations(const Group &G, int n, int xb, int rb, int zb, int p0b
 
 // FP64 Jacobi-PCG: r = rhs - G x, z, and the CG scalars of every problem
 template <bool HET>
-static void jpcg64_start(const Group &G, double rtol2, int max_iters) {
+static void jpcg64_start(const Group &G, double rtol2, int max_iters, int xb = B_STEADY, int rhsb = B_RHS) {
     launch_named("k_cg_init", k_cg_init<double, HET>, G.nwork, G.stream, (const ProbDesc *)G.d_desc,
-                 (const WorkItem *)G.d_work, (int)B_STEADY, (int)B_R, (int)B_Z, (int)B_RHS, rtol2, max_iters);
+                 (const WorkItem *)G.d_work, xb, (int)B_R, (int)B_Z, rhsb, rtol2, max_iters);
     FinArgs a{};
     a.rtol2 = rtol2;
     a.max_iters = max_iters;
@@ -1296,6 +1301,62 @@ static void transient_taylor(Model &M, DevModel &D, double dt) {
     M.stats.taylor_substeps = p.m;
 }
 
+// Backward Euler (RCutil.c:268-428 as compute_temp_grid calls it,
+// temperature_grid.c:3632-3662): one implicit step of the whole interval h,
+//   (C/h + G) y' = C y / h + P + sources,
+// G the assembled matrix of the SuperLU path (build_transient_grid_matrix),
+// whose secondary package links are the steady ones (symmetric); the
+// secondary package nodes carry no capacitance in it (build_diagonal_matrix
+// :6750-6768).  FP64 Jacobi-preconditioned CG from y, to the steady
+// tolerance; the shift C/h goes into the ambient conductances of a second
+// descriptor (per layer, or per cell for heterogeneous models).
+static void build_be(DevModel &D, double h) {
+    const double sigma = 1.0 / h;
+    ProbDesc b = D.h;
+    if (!D.hetero)
+        for (int n = 0; n < b.nl; n++) b.gamb[n] += sigma / b.cinv[n];
+    for (int k = 0; k < std::min(b.nx, (int)EXTRA); k++) b.x_gamb[k] += sigma / b.x_cinv[k];
+    if (D.hetero) {
+        const long ncell = (long)b.nl * b.nr * b.nc;
+        if (!D.be_shift) {
+            D.be_shift = D.alloc<double>(ncell);
+            D.be_dinv = D.alloc<double>(ncell);
+        }
+        hipLaunchKernelGGL(k_be_coef, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, D.stream,
+                           (const ProbDesc *)D.d_desc, ncell, sigma, D.be_shift, D.be_dinv);
+        b.bshift = D.be_shift;
+        b.dinvc = D.be_dinv;
+    }
+    if (!D.d_be_desc) D.d_be_desc = D.alloc<ProbDesc>(1);
+    HIP_OK(hipMemcpyAsync(D.d_be_desc, &b, sizeof(ProbDesc), hipMemcpyHostToDevice, D.stream));
+    HIP_OK(hipStreamSynchronize(D.stream));
+    D.be_grp = D.grp;
+    D.be_grp.d_desc = D.d_be_desc;
+}
+
+template <bool HET>
+static void transient_be(Model &M, DevModel &D, double dt) {
+    if (D.h.has_adv) fatal("backward Euler: models with microchannel layers are not supported\n");
+    build_be(D, dt);
+    read_scal(D);
+    const int yb = B_Y0 + (int)D.h_scal[S_YPAR];
+    launch_named("k_be_rhs", k_be_rhs<HET>, D.grp.nwork, D.stream, (const ProbDesc *)D.d_desc,
+                 (const WorkItem *)D.d_work, yb, (int)B_RHS, (int)B_T1, 1.0 / dt);
+    // C y / h dominates the right-hand side's norm, so the relative residual
+    // goes 100x below the steady tolerance to keep every node as close
+    const double rtol = 1e-2 * M.opt.steady_rtol, rtol2 = rtol * rtol;
+    write_scal(D, {{S_DONE, 0.0}});   // the CG start skips a problem still marked done
+    jpcg64_start<HET>(D.be_grp, rtol2, M.opt.max_iters, yb, B_T1);
+    int chunk = 8;
+    for (;;) {
+        read_scal(D);
+        if (D.h_scal[S_DONE] != 0.0) break;
+        cg_iterations<double, HET>(D.be_grp, chunk, yb, B_R, B_Z, B_P0, B_Q, S_DONE, S_TOL2, M.opt.max_iters);
+        chunk = std::min(chunk * 2, 64);
+    }
+    M.stats.be_iters = (int)D.h_scal[S_ITERS];
+}
+
 // the float4 term tiling of a single model, timed on its first FP32 Taylor use
 static void ensure_term_tuned(const Model &M, DevModel &D) {
     if (D.term_tuned || !D.grp.vec4 || M.opt.transient != HS_TRANS_TAYLOR || M.opt.precision != HS_PREC_MIXED)
@@ -1311,6 +1372,7 @@ void dev_transient(Model &M, const double *block_power, const double *block_temp
     dev_prepare(M);
     DevModel &D = *M.dev;
     ensure_term_tuned(M, D);
+    M.stats.be_iters = 0;
     HIP_OK(hipEventRecord(D.ev0, D.stream));
     upload_blocks(D, block_power);
     run_b2g(D.grp, V_POWER, B_RHS, 1);
@@ -1328,6 +1390,9 @@ void dev_transient(Model &M, const double *block_power, const double *block_temp
             if (D.hetero) transient_taylor<double, true>(M, D, dt);
             else transient_taylor<double, false>(M, D, dt);
         }
+    } else if (M.opt.transient == HS_TRANS_BE) {
+        if (D.hetero) transient_be<true>(M, D, dt);
+        else transient_be<false>(M, D, dt);
     } else {
         if (D.hetero) transient_rk4<true>(M, D, dt);
         else transient_rk4<false>(M, D, dt);
@@ -1432,6 +1497,13 @@ void dev_transient_run(Model &M, const double *power_seq, int steps, double dt, 
     DevModel &D = *M.dev;
     ensure_term_tuned(M, D);
     if (!init_temp && !M.trans_valid) fatal("hs_transient_run: no transient state to continue from\n");
+    if (M.opt.transient == HS_TRANS_BE) {
+        // one implicit solve per interval, each converged under host control
+        for (int k = 0; k < steps; k++)
+            dev_transient(M, power_seq + (size_t)k * D.nblk, k == 0 ? init_temp : nullptr, dt,
+                          temp_seq ? temp_seq + (size_t)k * D.nblk : nullptr);
+        return;
+    }
     const size_t n = (size_t)steps * D.nblk;
     double *seq_in = nullptr, *seq_out = nullptr;
     HIP_OK(hipMalloc((void **)&seq_in, n * sizeof(double)));

@@ -104,6 +104,9 @@ struct ProbDesc {
     const double *aN, *aS, *aE, *aW, *aSelf;
     const float *fE, *fS, *fD, *fcinv, *fdinv;
     const float *faN, *faS, *faE, *faW, *faSelf;
+    // backward-Euler descriptors of heterogeneous models: sigma C per cell,
+    // added to the ambient conductance of the stencil (nullptr otherwise)
+    const double *bshift;
     // block <-> grid maps (per layer)
     const int *map_ptr[kMaxLayers];     // cell -> entry range
     const int *map_unit[kMaxLayers];

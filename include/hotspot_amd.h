@@ -318,6 +318,10 @@ void hs_set_fatal_mode(int exit_on_error);
 /* transient integrators */
 #define HS_TRANS_RK4    0 /* the reference's adaptive step-doubling RK4 */
 #define HS_TRANS_TAYLOR 1 /* exponential (Taylor) integrator of the linear ODE, error bounded a priori */
+#define HS_TRANS_BE     2 /* the reference's SuperLU-build integrator: one backward-Euler step per
+                           * interval, (C/h + G) T' = C T / h + P (RCutil.c:268-428), solved by FP64
+                           * Jacobi CG; secondary package nodes without capacitance; not for
+                           * microchannel models */
 
 typedef struct hs_options_st {
     int precision;        /* HS_PREC_* */
@@ -337,6 +341,7 @@ typedef struct hs_stats_st {
     int taylor_terms;         /* terms per sub-step of the last Taylor interval */
     int taylor_substeps;
     double last_solve_ms;     /* device time of the last steady/compute_temp call */
+    int be_iters;             /* CG iterations of the last backward-Euler interval */
 } hs_stats_t;
 
 void hs_get_options(RC_model_t *model, hs_options_t *opt);

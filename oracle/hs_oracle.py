@@ -26,7 +26,9 @@ How each part is pinned (see tests/test_oracle.py):
   * the steady solve: the reference's own Gauss-Seidel sweep driven to a
     per-sweep change below 1e-13 K, and invariants (symmetry, energy balance,
     maximum principle, linearity);
-  * the transient: the reference's own ``compute_temp`` trajectories.
+  * the transient: the reference's own ``compute_temp`` trajectories;
+  * backward Euler (``advance_be``): parity unpinned against the reference,
+    whose SuperLU build is not available; pinned by its limits.
 """
 from __future__ import annotations
 
@@ -908,6 +910,22 @@ class OracleModel:
         tinf = spla.spsolve(self.Gt.tocsc(), rhs)
         A = sp.diags(1.0 / self.Cvec) @ self.Gt
         return tinf + spla.expm_multiply(-dt * A, Tg - tinf)
+
+    def advance_be(self, Tg, Pg, h):
+        """One backward-Euler step of the whole interval, as compute_temp_grid
+        does in the reference's SuperLU build (:3632-3662 calling
+        backward_euler, RCutil.c:268-428): (C/h + G) T' = C T / h + Pg + b.
+        G is the matrix that build_transient_grid_matrix assembles, whose
+        secondary package links are the steady ones (the solder-substrate
+        link in both directions, :6354 and :6440), and the secondary package
+        nodes' capacitance is 0 (build_diagonal_matrix :6750-6768).  Parity
+        unpinned against the reference (SuperLU absent); pinned by its limits
+        in tests/test_oracle.py: h -> infinity gives the steady state, and the
+        error against the exact solution falls linearly with h."""
+        cbe = self.Cvec.copy()
+        cbe[self.ncell + EXTRA:] = 0.0
+        A = (sp.diags(cbe / h) + self.G).tocsc()
+        return spla.spsolve(A, cbe / h * Tg + Pg + self.b)
 
     def advance(self, Tg, Pg, dt):
         """compute_temp_grid's integration loop (:3676-3692).  Returns the new

@@ -51,7 +51,7 @@ class HsStats(C.Structure):
     _fields_ = [("steady_iters", C.c_int), ("steady_refinements", C.c_int),
                 ("steady_rel_resid", C.c_double), ("rk4_calls", C.c_int),
                 ("taylor_terms", C.c_int), ("taylor_substeps", C.c_int),
-                ("last_solve_ms", C.c_double)]
+                ("last_solve_ms", C.c_double), ("be_iters", C.c_int)]
 
 
 def grid_temp(lib, run, which):

@@ -257,3 +257,24 @@ def test_engine_microchannel_steady_equals_single_model_api(lib, form):
     s = transient_run(lib, mc, np.array([P] * 4), T)
     assert np.array_equal(s, seq)
     mc.free()
+
+
+def test_backward_euler_sequence_equals_calls(lib):
+    """hs_transient_run with the backward-Euler integrator gives, bit for
+    bit, what one compute_temp call per interval gives; the engine refuses
+    the integrator with an error."""
+    case = cases.default_stack_case(32)
+    pw = [cases.rng(2).uniform(0.5, 5, 16)]
+    a, b = make(lib, case), make(lib, case)
+    for r in (a, b):
+        set_options(lib, r, transient=2)
+    P = a.power_vector(pw)
+    seq = np.array(cases.jittered_powers(P, 5, seed=3))
+    init = np.full(a.n, 330.0)
+    calls = cases.run_transient(a, init, list(seq), DT)
+    got = transient_run(lib, b, seq, init)
+    assert np.array_equal(got, calls)
+    with pytest.raises(RuntimeError, match="HS_TRANS_RK4 or HS_TRANS_TAYLOR"):
+        engine.Engine(lib, engine.MIXED, 2)
+    a.free()
+    b.free()

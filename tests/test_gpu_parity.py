@@ -319,3 +319,30 @@ def test_rectangular_grid_taylor_matches_exact(lib, name, case, pw):
         worst = max(worst, np.abs(tr[k] - om.g2b(Tg)).max())
     assert worst <= 1e-4
     run.free()
+
+
+# Backward Euler (the reference's SuperLU-build integrator): one implicit
+# step per interval, against the oracle's restatement of the same step.  The
+# CG stops at a 1e-14 relative residual of C T/h + P + b.  At 3.3 us the
+# system mixes rows scaled by C/h with the capacitance-free rows of the
+# secondary package nodes, and the CG and the direct solve then differ by up
+# to ~2e-6 K there; the bar is 1e-5 K, ten times under the FP64 bar of
+# 1e-4 K.  Short (3.333 us) and long (10 ms) intervals.
+@pytest.mark.parametrize("dt", [3.333e-6, 1e-2], ids=["3.3us", "10ms"])
+@pytest.mark.parametrize("name,case,pw", [CASES[0], CASES[1], CASES[3], CASES[4]],
+                         ids=["default-64", "secondary-16", "3d-stack-32", "detailed-3d-16"])
+def test_transient_backward_euler_matches_oracle(lib, name, case, pw, dt):
+    run = make(lib, case)
+    set_options(lib, run, transient=2)
+    om = oracle_for(run)
+    P = run.power_vector(pw)
+    T0, _ = om.steady(P)
+    seq = cases.jittered_powers(P, 4, seed=5)
+    tr = cases.run_transient(run, T0, seq, dt)
+    check_error(lib)
+    Tg = om.b2g(T0, power=False)
+    for k, p in enumerate(seq):
+        Tg = om.advance_be(Tg, om.power_grid(p), dt)
+        assert np.abs(tr[k] - om.g2b(Tg)).max() <= 1e-5
+    assert stats(lib, run).be_iters > 0
+    run.free()

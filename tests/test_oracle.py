@@ -239,3 +239,31 @@ def test_microchannel_direct_solve_satisfies_the_operator():
     assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(Pg + om.b)
     asym = abs(om.G - om.G.T).max()
     assert asym > 1e-6 * abs(om.G).max()
+
+
+def test_backward_euler_limits():
+    """advance_be: its fixed point is the steady state (h -> infinity), and its error against the exact solution falls linearly
+    with the step (first order: halving h halves the error); a step without
+    the 1/h on C T would break both."""
+    om = _default()
+    amb = om.cfg["ambient"]
+    P = np.zeros(om.n_vector()); P[:16] = 3.0
+    Pg = om.power_grid(P)
+    y0 = np.full(om.N, amb + 3.0)
+    tinf = spsolve_t(om, Pg)
+    assert np.abs(om.advance_be(y0, Pg, 1e9) - tinf).max() <= 1e-6
+    assert np.abs(om.advance_be(tinf, Pg, 1e-5) - tinf).max() <= 1e-9
+    dt = 2e-5
+    ex = om.advance_exact(y0, Pg, dt)
+    errs = []
+    for n in (4, 8, 16):
+        y = y0.copy()
+        for _ in range(n):
+            y = om.advance_be(y, Pg, dt / n)
+        errs.append(np.abs(y - ex).max())
+    assert 1.7 < errs[0] / errs[1] < 2.3 and 1.7 < errs[1] / errs[2] < 2.3
+
+
+def spsolve_t(om, Pg):
+    import scipy.sparse.linalg as spla
+    return spla.spsolve(om.G.tocsc(), Pg + om.b)
