@@ -430,28 +430,34 @@ def main():
     # roofline pass: the same batch with every launch of the dominant kernel
     # bracketed by device events on the engine's stream (plain launches
     # instead of graph replay while the timer is on)
-    # Algorithmic bytes of one Horner step h <- E_1 + (tau/j) M h of one job:
-    # FP32 read of h, read of E_1, write of the new h = 12 B per node; the
-    # first step of a sub-step reads h = E_1, one array, so it moves 8 B.  A
-    # launch skips the jobs whose series needs fewer steps, so the achieved
-    # rate is the bytes of the steps the jobs actually took (counted on the
-    # device; one first step per job, interval and sub-step) over the summed
-    # duration of every launch, skipping ones included.
+    # The dominant kernel is the paired Horner sweep k_tay_pair (two steps
+    # h <- E_1 + (tau/j) M h per pass).  Its algorithmic bytes for one job:
+    # FP32 read of h, read of E_1, write of the new h = 12 B per node; 8 B when
+    # h is E_1 itself (the job's first sweep of a sub-step).  A launch skips
+    # the jobs whose series needs fewer steps, so the achieved rate is the
+    # bytes of the sweeps the jobs actually took (counted on the device per
+    # job: taylor_node_bytes x nodes) over the summed duration of every
+    # launch, skipping ones included.
     achieved = avg_ms = nt = sweeps = None
     nvec = cells + iface.EXTRA
     bytes_per_sweep = nvec * 12
     bytes_per_launch = args.batch * bytes_per_sweep
+    kernel = "k_tay_pair"
     if not args.no_roofline:
-        t_roof, _, rinfo, _ = served(lib, jobs, args.intervals, args.tick, timer=b"k_tay_term")
-        device_sync()
-        a, tot = C.c_double(), C.c_double()
-        nt = L.hs_kernel_timer_read(C.byref(a), C.byref(tot))
-        L.hs_kernel_timer(None, 0)
+        for kernel in ("k_tay_pair", "k_tay_term"):
+            t_roof, _, rinfo, _ = served(lib, jobs, args.intervals, args.tick, timer=kernel.encode())
+            device_sync()
+            a, tot = C.c_double(), C.c_double()
+            nt = L.hs_kernel_timer_read(C.byref(a), C.byref(tot))
+            L.hs_kernel_timer(None, 0)
+            if nt:
+                break
         avg_ms = a.value
         sweeps = float(sum(i.taylor_sweeps for i in rinfo))
+        node_bytes = float(sum(i.taylor_node_bytes for i in rinfo))
         tot_ms = tot.value
     traffic = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_k_tay_term_3dstack{args.grid}_batch{args.batch}.json")
+    tf = os.path.join(ROOT, "profiles", f"traffic_{kernel}_3dstack{args.grid}_batch{args.batch}.json")
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -470,10 +476,9 @@ def main():
     _, _, _, served_seq = served(lib, jobs[:1], args.intervals, args.tick, keep_first=True)
     single["served_equals_single"] = bool(np.array_equal(served_seq, seq_mixed))
     if nt:
-        first = min(sweeps, float(args.batch * args.intervals * st.taylor_substeps))
-        achieved = (sweeps * bytes_per_sweep - first * nvec * 4) / (tot_ms * 1e-3) / 1e9
-        log(f"roofline pass: {nt} launches of k_tay_term, mean {avg_ms*1e3:.2f} us, "
-            f"{sweeps / args.batch / nt:.3f} of the job steps active per launch, {achieved:.0f} GB/s")
+        achieved = node_bytes * nvec / (tot_ms * 1e-3) / 1e9
+        log(f"roofline pass: {nt} launches of {kernel}, mean {avg_ms*1e3:.2f} us, "
+            f"{node_bytes * nvec / nt / 1e6:.1f} MB per launch on average, {achieved:.0f} GB/s")
 
     fp64 = None
     if not args.no_fp64:
@@ -516,15 +521,17 @@ def main():
                        "note": "submission -> result on the host, admission included"},
         "admission_ms": float(np.mean(admit_t)) * 1e3,
         "value_with_admission": args.batch * updates_job / (t_step + float(np.mean(admit_t))),
-        "roofline": {"bound": "hbm", "kernel": "k_tay_term4 (FP32 Horner step of the Taylor series, float4 tiles)",
+        "roofline": {"bound": "hbm", "kernel": {"k_tay_pair": "k_tay_pair (FP32, two Horner steps of the Taylor "
+                                                              "series per sweep)",
+                                                "k_tay_term": "k_tay_term4 (FP32 Horner step of the Taylor series, "
+                                                              "float4 tiles)"}[kernel],
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic, "launches_timed": nt,
                      "avg_launch_us": avg_ms * 1e3 if avg_ms else None,
                      "bytes_per_launch": bytes_per_launch,
-                     "active_sweep_fraction": (sweeps / args.batch / nt) if nt else None,
+                     "algorithmic_bytes_avg_per_launch": (node_bytes * nvec / nt) if nt else None,
                      "steps_per_batch": step_sweeps,
-                     "first_steps_per_batch": float(args.batch * args.intervals * single["taylor_substeps"]),
                      "traffic_note": "HBM bytes per launch from FETCH_SIZE and WRITE_SIZE (rocprofv3 "
                                      "--pmc, scaled against a plain copy)"},
         "single_job": single,

@@ -40,6 +40,7 @@ struct EngJob {
     double t_submit = 0, t_admit = 0, t_done = 0;
     int steady_iters = 0, refinements = 0, rk4_calls = 0;
     long taylor_sweeps = 0;
+    double taylor_node_bytes = 0;
 };
 
 // A group of resident jobs in one phase with one kernel form (homogeneous or
@@ -48,7 +49,7 @@ struct BatchGroup {
     Group g;
     std::vector<EngJob *> members;
     bool dirty = true;
-    size_t cap_desc = 0, cap_work = 0, cap_gwork = 0, cap_flags = 0, cap_trec = 0, cap_twork = 0;
+    size_t cap_desc = 0, cap_work = 0, cap_gwork = 0, cap_flags = 0, cap_trec = 0, cap_twork = 0, cap_pwork = 0;
     double *d_flags = nullptr, *h_flags = nullptr;
     Replayer rp;
     SeqForm form{};
@@ -69,8 +70,9 @@ struct BatchGroup {
         if (g.d_kmax) (void)hipFree(g.d_kmax);
         if (g.d_trec) (void)hipFree(g.d_trec);
         if (g.d_twork) (void)hipFree(g.d_twork);
+        if (g.d_pwork) (void)hipFree(g.d_pwork);
         g = Group{};
-        cap_trec = cap_twork = 0;
+        cap_trec = cap_twork = cap_pwork = 0;
         d_flags = h_flags = nullptr;
         cap_desc = cap_work = cap_gwork = cap_flags = 0;
     }
@@ -185,6 +187,16 @@ static void group_sync(Engine &E, BatchGroup &B, hipStream_t s, bool het, bool t
     for (int i = 0; i < n && B.g.nper; i++)
         if (desc[i].ntiles != desc[0].ntiles) B.g.nper = 0;
     B.form = SeqForm{E.opt.precision, E.opt.transient, m, K};
+    // the paired sweeps' work-list (patches through all layers)
+    {
+        std::vector<WorkItem> pw;
+        B.g.pairs = trans && !het && E.opt.transient == HS_TRANS_TAYLOR && E.opt.precision == HS_PREC_MIXED &&
+                    pairs_enabled();
+        B.g.pnper = pair_items(desc, pw);
+        ensure_dev(B.g.d_pwork, B.cap_pwork, pw.size());
+        HIP_OK(hipMemcpy(B.g.d_pwork, pw.data(), pw.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+        B.g.npwork = (int)pw.size();
+    }
     // the float4 term sweep's work-list; its tiling timed for FP32 Taylor groups
     if (trans && v4 && E.opt.transient == HS_TRANS_TAYLOR && E.opt.precision == HS_PREC_MIXED) {
         tune_term_tiling(B.g, desc, B.cap_twork);
@@ -341,6 +353,7 @@ static void trans_done(Engine &E, EngJob &j, const double *flags) {
     DevModel &D = *j.D;
     j.rk4_calls = E.opt.transient == HS_TRANS_RK4 ? (int)flags[1] : 0;
     j.taylor_sweeps = E.opt.transient == HS_TRANS_TAYLOR ? (long)flags[2] : 0;
+    j.taylor_node_bytes = E.opt.transient == HS_TRANS_TAYLOR ? flags[3] : 0.0;
     const size_t n = (size_t)j.nsteps * D.nblk;
     j.seq.resize(n);
     HIP_OK(hipMemcpyAsync(j.seq.data(), D.h.seq_out, n * sizeof(double), hipMemcpyDeviceToHost, D.stream));
@@ -487,7 +500,7 @@ int eng_tick(Engine &E) {
         group_sync(E, B, E.s_trans, h, true);
         ran_trans[h] = B.g.nprob;
         B.rp.run(E.s_trans, E.tick_intervals, [&] { seq_body(B.g, B.form); });
-        gather(B, {S_STEP, S_TCALLS, S_TTERMS});
+        gather(B, {S_STEP, S_TCALLS, S_TTERMS, S_TBYTES});
     }
     HIP_OK(hipStreamSynchronize(E.s_steady));
     HIP_OK(hipStreamSynchronize(E.s_trans));
@@ -544,6 +557,7 @@ void eng_job_info(Engine &E, int id, hs_job_info_t *info) {
     info->steady_refinements = j.refinements;
     info->rk4_calls = j.rk4_calls;
     info->taylor_sweeps = (double)j.taylor_sweeps;
+    info->taylor_node_bytes = j.taylor_node_bytes;
     info->wait_s = j.phase == J_QUEUED ? now_s() - j.t_submit : j.t_admit - j.t_submit;
     info->latency_s = j.phase == J_DONE ? j.t_done - j.t_submit : now_s() - j.t_submit;
     info->device_bytes = (double)(j.held ? j.held : j.footprint);

@@ -914,6 +914,8 @@ struct FinArgs {
     int max_iters, max_refine, done_slot, tol_slot, sub, seq, mg;
     int *kmax;     // FIN_TAY: largest term count of the group
     double *trec;  // FIN_TAY: per problem {tau, terms (0: no sub-step)}
+    int kg;        // FIN_TAY: the group's plan (launches of the sub-step)
+    int pairs;     // FIN_TAY: steps run two per sweep (k_tay_pair)
 };
 
 // ---------------------------------------------------------------------------
@@ -1705,6 +1707,16 @@ __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ 
             s[S_TMAX] = w[0];
             s[S_KNEED] = K;
             s[S_TTERMS] += K - 1;
+            // algorithmic bytes per node of the dominant sweeps: 12 (read h
+            // and E_1, write h), 8 when h is E_1 itself (a problem's first)
+            if (a.pairs) {
+                const bool single = (a.kg % 2 == 0) && K == a.kg;
+                const int jf = (K & 1) ? K : (single ? K - 1 : K + 1);
+                const int np = (K >= 2 && jf >= 3) ? (jf - 3) / 2 + 1 : 0;
+                s[S_TBYTES] += np > 0 ? (single ? 12.0 * np : 8.0 + 12.0 * (np - 1)) : 0.0;
+            } else {
+                s[S_TBYTES] += K >= 2 ? 8.0 + 12.0 * (K - 2) : 0.0;
+            }
             a.trec[2 * blockIdx.x] = s[S_TAU];
             a.trec[2 * blockIdx.x + 1] = K;
             atomicMax(a.kmax, K);
@@ -2042,18 +2054,284 @@ __global__ void __launch_bounds__(kBlock) k_tay_term4(const ProbDesc *__restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+// Two Horner steps per sweep (FP32, homogeneous layers): h1 = E_1 + s1 M h0,
+// h2 = E_1 + s2 M h1 in one pass over HBM (read h0 and E_1, write h2: 12
+// bytes per node for two steps instead of 24).  A workgroup owns a patch of
+// kPX x kPY cells through every layer and streams down the layers: plane t
+// of h0 (patch + 2) goes to LDS, then h1 on plane t-1 (patch + 1) from three
+// h0 planes, then h2 on plane t-2 (the patch) from three h1 planes.  Ring
+// terms need the package nodes' h1, which depends on whole boundary rows:
+// h2 at the boundary cells of ring layers is written without them, and
+// k_tay_pair_fix (one workgroup per problem) adds them and computes the
+// package nodes.  Launches pair the steps (3, 2), (5, 4), ...; a group whose
+// largest plan Kg is even runs its step Kg alone first.  A problem whose
+// term count K is even and below Kg takes only the second step of its first
+// pair.
+// ---------------------------------------------------------------------------
+constexpr int kPX = 64, kPY = 8;
+
+struct PairPlan {
+    int in, out, mode;   // mode 2: steps j and j-1; 1: step j-1 only
+    float s1, s2;
+};
+
+// The pair of top step j for a problem of K terms in a group of plan kg:
+// false when the problem takes no step of it.  Buffers alternate between the
+// problem's active launches, the first reading E_1.
+__device__ __forceinline__ bool pair_plan(int K, int kg, int j, double tau, int e1b, int d0b, int d1b,
+                                          PairPlan &p) {
+    if (K < 2 || j - 1 > K) return false;
+    const bool single = (kg % 2 == 0) && K == kg;   // step K ran alone first
+    const int jf = (K & 1) ? K : (single ? K - 1 : K + 1);
+    if (j > jf) return false;
+    const int idx = (single ? 1 : 0) + (jf - j) / 2;
+    p.mode = j <= K ? 2 : 1;
+    p.in = idx == 0 ? e1b : (((idx - 1) & 1) ? d1b : d0b);
+    p.out = (idx & 1) ? d1b : d0b;
+    p.s1 = (float)(tau / j);
+    p.s2 = (float)(tau / (j - 1));
+    return true;
+}
+
+// the buffer holding the sum after the last launch of a paired sequence
+__device__ __forceinline__ int pair_result(int K, int kg, int e1b, int d0b, int d1b) {
+    if (K < 2) return e1b;
+    const bool single = (kg % 2 == 0) && K == kg;
+    const int jf = (K & 1) ? K : (single ? K - 1 : K + 1);
+    const int n_act = (single ? 1 : 0) + (jf >= 3 ? (jf - 3) / 2 + 1 : 0);
+    return ((n_act - 1) & 1) ? d1b : d0b;
+}
+
+// per-layer coefficients of the FP32 homogeneous stencil
+struct LayerCoef {
+    float gx, gy, gu, gd, ci, gam, rgn, rge;
+    int rn[4];
+};
+__device__ __forceinline__ LayerCoef layer_coef(const ProbDesc &d, int n) {
+    LayerCoef L;
+    L.gx = (float)d.gx[n];
+    L.gy = (float)d.gy[n];
+    L.gu = n > 0 ? (float)d.gz[n - 1] : 0.f;
+    L.gd = n < d.nl - 1 ? (float)d.gz[n] : 0.f;
+    L.ci = (float)d.cinv[n];
+    L.gam = (float)d.gamb[n];
+    L.rgn = (float)d.ring_gns[n];
+    L.rge = (float)d.ring_gew[n];
+    for (int e = 0; e < 4; e++) L.rn[e] = d.ring_node[n][e];
+    return L;
+}
+
+// One Horner step at a cell from its 6 neighbours (missing ones excluded), in
+// the order of stencil(); ring terms when ring != nullptr.
+__device__ __forceinline__ float horner_cell(const LayerCoef &L, float e1, float s, float xc, float xn, float xs,
+                                             float xe, float xw, float xu, float xd, bool hn, bool hs, bool he,
+                                             bool hw, bool hu, bool hd, const float *ring, int i, int j, int nr,
+                                             int nc) {
+    float flux = 0.f;
+    if (hn) flux = L.gx * (xn - xc);
+    if (hs) flux += L.gx * (xs - xc);
+    if (he) flux += L.gy * (xe - xc);
+    if (hw) flux += L.gy * (xw - xc);
+    if (hu) flux += L.gu * (xu - xc);
+    if (hd) flux += L.gd * (xd - xc);
+    if (ring) {
+        if (i == 0 && L.rn[0] >= 0) flux += L.rgn * (ring[0] - xc);
+        if (i == nr - 1 && L.rn[1] >= 0) flux += L.rgn * (ring[1] - xc);
+        if (j == nc - 1 && L.rn[2] >= 0) flux += L.rge * (ring[2] - xc);
+        if (j == 0 && L.rn[3] >= 0) flux += L.rge * (ring[3] - xc);
+    }
+    return e1 + s * (L.ci * (flux - L.gam * xc));
+}
+
+__global__ void __launch_bounds__(kBlock) k_tay_pair(const ProbDesc *__restrict__ D,
+                                                     const WorkItem *__restrict__ W, int nper, int e1b, int d0b,
+                                                     int d1b, int j, int kg, const int *__restrict__ kmax,
+                                                     const double *__restrict__ trec) {
+    if (j - 1 > *kmax) return;
+    const TayItem it = tay_item(W, nper, trec);
+    PairPlan pp;
+    if (!pair_plan(it.K, kg, j, it.tau, e1b, d0b, d1b, pp)) return;
+    const ProbDesc &d = D[it.prob];
+    const int nr = d.nr, nc = d.nc, nl = d.nl;
+    const int PX = min(kPX, nc), PY = min(kPY, nr);
+    const int npj = nc / PX;
+    const int pi = it.tile / npj, pj = it.tile - pi * npj;
+    const int i0 = pi * PY, j0 = pj * PX;
+    const float *h0 = d.b32[pp.in], *e1 = d.b32[e1b];
+    float *out = d.b32[pp.out];
+    const unsigned plane = (unsigned)nr * nc, ncell = plane * nl;
+    __shared__ float H0[3][kPY + 4][kPX + 4];
+    __shared__ float H1[3][kPY + 2][kPX + 2];
+    const int W0 = PX + 4, R0 = PY + 4, W1 = PX + 2, R1 = PY + 2;
+    for (int t = 0; t <= nl + 1; t++) {
+        if (t < nl) {   // h0 of plane t on the patch + 2 (0 outside the grid)
+            for (int idx = threadIdx.x; idx < R0 * W0; idx += kBlock) {
+                const int r = idx / W0, c = idx - r * W0;
+                const int gi = i0 - 2 + r, gj = j0 - 2 + c;
+                float v = 0.f;
+                if (gi >= 0 && gi < nr && gj >= 0 && gj < nc) v = ld_at(h0, t * plane + (unsigned)gi * nc + gj);
+                H0[t % 3][r][c] = v;
+            }
+        }
+        __syncthreads();
+        const int m = t - 1;   // h1 of plane m on the patch + 1
+        if (m >= 0 && m < nl) {
+            const LayerCoef L = layer_coef(d, m);
+            float ring[4];
+            for (int e = 0; e < 4; e++) ring[e] = ld_at(h0, ncell + max(L.rn[e], 0));
+            const int sm = m % 3, su = (m + 2) % 3, sd = (m + 1) % 3;
+            for (int idx = threadIdx.x; idx < R1 * W1; idx += kBlock) {
+                const int r = idx / W1, c = idx - r * W1;
+                const int gi = i0 - 1 + r, gj = j0 - 1 + c;
+                float v = 0.f;
+                if (gi >= 0 && gi < nr && gj >= 0 && gj < nc) {
+                    const float e1v = ld_at(e1, m * plane + (unsigned)gi * nc + gj);
+                    const float s = pp.mode == 2 ? pp.s1 : pp.s2;
+                    v = horner_cell(L, e1v, s, H0[sm][r + 1][c + 1], H0[sm][r][c + 1], H0[sm][r + 2][c + 1],
+                                    H0[sm][r + 1][c + 2], H0[sm][r + 1][c], m > 0 ? H0[su][r + 1][c + 1] : 0.f,
+                                    m < nl - 1 ? H0[sd][r + 1][c + 1] : 0.f, gi > 0, gi < nr - 1, gj < nc - 1,
+                                    gj > 0, m > 0, m < nl - 1, ring, gi, gj, nr, nc);
+                }
+                H1[sm][r][c] = v;
+            }
+        }
+        __syncthreads();
+        const int m2 = t - 2;   // h2 of plane m2 on the patch (or h1 alone in mode 1)
+        if (m2 >= 0 && m2 < nl) {
+            const LayerCoef L = layer_coef(d, m2);
+            const int sm = m2 % 3, su = (m2 + 2) % 3, sd = (m2 + 1) % 3;
+            for (int idx = threadIdx.x; idx < PY * PX; idx += kBlock) {
+                const int r = idx / PX, c = idx - r * PX;
+                const int gi = i0 + r, gj = j0 + c;
+                const unsigned a = m2 * plane + (unsigned)gi * nc + gj;
+                float v;
+                if (pp.mode == 1) {
+                    v = H1[sm][r + 1][c + 1];
+                } else {
+                    v = horner_cell(L, ld_at(e1, a), pp.s2, H1[sm][r + 1][c + 1], H1[sm][r][c + 1],
+                                    H1[sm][r + 2][c + 1], H1[sm][r + 1][c + 2], H1[sm][r + 1][c],
+                                    m2 > 0 ? H1[su][r + 1][c + 1] : 0.f, m2 < nl - 1 ? H1[sd][r + 1][c + 1] : 0.f,
+                                    gi > 0, gi < nr - 1, gj < nc - 1, gj > 0, m2 > 0, m2 < nl - 1, nullptr, gi, gj,
+                                    nr, nc);
+                }
+                st_at(out, a, v);
+            }
+        }
+    }
+}
+
+// h1 at one cell from h0 (every term, rings included)
+__device__ __forceinline__ float pair_h1_at(const ProbDesc &d, const float *h0, const float *e1, float s1, int n,
+                                            int i, int j) {
+    const int nr = d.nr, nc = d.nc, nl = d.nl;
+    const unsigned plane = (unsigned)nr * nc, ncell = plane * nl, c = n * plane + (unsigned)i * nc + j;
+    const LayerCoef L = layer_coef(d, n);
+    float ring[4];
+    for (int e = 0; e < 4; e++) ring[e] = ld_at(h0, ncell + max(L.rn[e], 0));
+    const float xc = ld_at(h0, c);
+    return horner_cell(L, ld_at(e1, c), s1, xc, i > 0 ? ld_at(h0, c - nc) : 0.f,
+                       i < nr - 1 ? ld_at(h0, c + nc) : 0.f, j < nc - 1 ? ld_at(h0, c + 1) : 0.f,
+                       j > 0 ? ld_at(h0, c - 1) : 0.f, n > 0 ? ld_at(h0, c - plane) : 0.f,
+                       n < nl - 1 ? ld_at(h0, c + plane) : 0.f, i > 0, i < nr - 1, j < nc - 1, j > 0, n > 0,
+                       n < nl - 1, ring, i, j, nr, nc);
+}
+
+// After a pair launch, one workgroup per problem: the package nodes' h1 (from
+// h0) and h2 (from h1 on the boundary rows, recomputed), and the ring terms
+// of h2 at the boundary cells of ring layers.  Mode 1: the package nodes' one
+// step.
+__global__ void __launch_bounds__(kBlock) k_tay_pair_fix(const ProbDesc *__restrict__ D, int e1b, int d0b,
+                                                         int d1b, int j, int kg, const int *__restrict__ kmax,
+                                                         const double *__restrict__ trec) {
+    if (j - 1 > *kmax) return;
+    const int p = blockIdx.x;
+    PairPlan pp;
+    if (!pair_plan((int)trec[2 * p + 1], kg, j, trec[2 * p], e1b, d0b, d1b, pp)) return;
+    const ProbDesc &d = D[p];
+    const int nr = d.nr, nc = d.nc, nl = d.nl;
+    const unsigned plane = (unsigned)nr * nc, ncell = plane * nl;
+    const float *h0 = d.b32[pp.in], *e1 = d.b32[e1b];
+    float *out = d.b32[pp.out];
+    __shared__ float xf[kMaxExtra], xd[kMaxExtra], h1x[kMaxExtra], xf1[kMaxExtra];
+    LdPlain<float> ld{h0};
+    package_terms<float>(d, ld, true, xf, xd);
+    const int k0 = threadIdx.x;
+    const float s_first = pp.mode == 2 ? pp.s1 : pp.s2;
+    if (k0 < d.nx)
+        h1x[k0] = e1[ncell + k0] + s_first * ((float)d.x_cinv[k0] * (xf[k0] - (float)d.x_gamb[k0] * h0[ncell + k0]));
+    __syncthreads();
+    if (pp.mode == 1) {
+        if (k0 < d.nx) out[ncell + k0] = h1x[k0];
+        return;
+    }
+    // boundary rows of the ring layers: h1 recomputed, its ring sum, and the
+    // ring term of h2 (corner cells below)
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    for (int k = w; k < d.nx; k += kBlock / kWave) {
+        const int L_ = d.x_ring_layer[k];
+        if (L_ < 0) {
+            if (lane == 0) xf1[k] = 0.f;
+            continue;
+        }
+        const int e = d.x_ring_edge[k];
+        const int m = e < 2 ? nc : nr;
+        const float ci = (float)d.cinv[L_], rg = (float)d.x_ring_g[k];
+        double s = 0.0;
+        for (int t = lane; t < m; t += kWave) {
+            const int i = e == 0 ? 0 : e == 1 ? nr - 1 : t, jj = e == 2 ? nc - 1 : e == 3 ? 0 : t;
+            const float h1c = pair_h1_at(d, h0, e1, pp.s1, L_, i, jj);
+            s += (double)(h1c - h1x[k]);
+            if (t != 0 && t != m - 1) {
+                const unsigned c = L_ * plane + (unsigned)i * nc + jj;
+                out[c] += pp.s2 * (ci * (rg * (h1x[k] - h1c)));
+            }
+        }
+        s = wave_sum(s);
+        if (lane == 0) xf1[k] = rg * (float)s;
+    }
+    // corner cells: every ring term of the cell
+    for (int q = threadIdx.x; q < 4 * nl; q += kBlock) {
+        const int n = q >> 2, corner = q & 3;
+        const int i = (corner & 1) ? nr - 1 : 0, jj = (corner & 2) ? nc - 1 : 0;
+        const int *rn = d.ring_node[n];
+        const bool tn = i == 0 && rn[0] >= 0, ts = i == nr - 1 && rn[1] >= 0;
+        const bool te = jj == nc - 1 && rn[2] >= 0, tw = jj == 0 && rn[3] >= 0;
+        if (!(tn || ts || te || tw)) continue;
+        const float h1c = pair_h1_at(d, h0, e1, pp.s1, n, i, jj);
+        const float rgn = (float)d.ring_gns[n], rge = (float)d.ring_gew[n];
+        float f = 0.f;
+        if (tn) f += rgn * (h1x[rn[0]] - h1c);
+        if (ts) f += rgn * (h1x[rn[1]] - h1c);
+        if (te) f += rge * (h1x[rn[2]] - h1c);
+        if (tw) f += rge * (h1x[rn[3]] - h1c);
+        const unsigned c = n * plane + (unsigned)i * nc + jj;
+        out[c] += pp.s2 * ((float)d.cinv[n] * f);
+    }
+    __syncthreads();
+    if (k0 < d.nx) {
+        float flux = xf1[k0];
+        for (int q = 0; q < d.x_ncpl[k0]; q++) {
+            if (d.x_cpl_steady_only[k0][q]) continue;
+            flux += (float)d.x_cpl_g[k0][q] * (h1x[d.x_cpl_node[k0][q]] - h1x[k0]);
+        }
+        out[ncell + k0] = e1[ncell + k0] + pp.s2 * ((float)d.x_cinv[k0] * (flux - (float)d.x_gamb[k0] * h1x[k0]));
+    }
+}
+
 // y += h (the sum of the sub-step's terms)
 template <typename TD>
 __global__ void __launch_bounds__(kBlock) k_tay_final(const ProbDesc *__restrict__ D,
                                                       const WorkItem *__restrict__ W, int nper, int e1b,
-                                                      int d0b, int d1b, const double *__restrict__ trec) {
+                                                      int d0b, int d1b, const double *__restrict__ trec,
+                                                      int pair_kg) {
     const TayItem it = tay_item(W, nper, trec);
     if (it.K == 0) return;
     const ProbDesc &d = D[it.prob];
     const WorkItem wi{it.prob, it.tile};
     const int ypar = (int)d.scal[S_YPAR];
     double *y = d.b64[B_Y0 + ypar];
-    const int hb = horner_result(it.K, e1b, d0b, d1b);
+    const int hb = pair_kg > 0 ? pair_result(it.K, pair_kg, e1b, d0b, d1b) : horner_result(it.K, e1b, d0b, d1b);
     const TD *acc = sizeof(TD) == 8 ? (const TD *)d.b64[hb] : (const TD *)d.b32[hb];
     if (wi.tile < d.ntiles) {
         struct V {

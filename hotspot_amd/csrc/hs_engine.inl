@@ -25,7 +25,37 @@ struct Group {
     // work-list of the float4 Taylor sweep over the term tiling (t_*)
     WorkItem *d_twork = nullptr;
     int ntwork = 0, tnper = 0;
+    // paired Taylor sweeps (FP32 homogeneous groups): patches through all layers
+    bool pairs = false;
+    WorkItem *d_pwork = nullptr;
+    int npwork = 0, pnper = 0;
 };
+
+// Paired sweeps when HS_PAIRS=1 (off by default: single sweeps are faster
+// until the paired kernel hides its per-layer latency).
+static bool pairs_enabled() {
+    static const bool on = [] {
+        const char *e = getenv("HS_PAIRS");
+        return e && atoi(e) == 1;
+    }();
+    return on;
+}
+
+// Work-list of the paired sweeps: (problem, patch) over kPX x kPY patches
+// (at most the grid) through all layers; no package item (k_tay_pair_fix
+// takes the package nodes).  Returns the items per problem when equal for
+// every problem, else 0.
+static int pair_items(const std::vector<ProbDesc> &desc, std::vector<WorkItem> &w) {
+    w.clear();
+    int per = -1;
+    for (int i = 0; i < (int)desc.size(); i++) {
+        const ProbDesc &h = desc[i];
+        const int n = (h.nr / std::min(kPY, h.nr)) * (h.nc / std::min(kPX, h.nc));
+        for (int t = 0; t < n; t++) w.push_back(WorkItem{i, t});
+        per = (per == -1 || per == n) ? n : 0;
+    }
+    return per < 0 ? 0 : per;
+}
 
 // A homogeneous problem whose tiles are at least 16 columns wide with at
 // least 4 rows per thread runs its FP32 term sweeps with float4 accesses
@@ -662,6 +692,11 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
         std::vector<WorkItem> tw;
         const int per = term_items({h}, tw);
         upload_term_items(D->grp, tw, per, D->cap_twork);
+        std::vector<WorkItem> pw;
+        D->grp.pnper = pair_items({h}, pw);
+        D->grp.d_pwork = D->upload(pw.data(), pw.size());
+        D->grp.npwork = (int)pw.size();
+        D->grp.pairs = !D->hetero && pairs_enabled();
     }
     if (D->hetero) assemble_hetero(M, *D);
     // transient step bound
@@ -1272,23 +1307,39 @@ static void taylor_interval(const Group &G, int m, int K, int seq) {
     const int db0 = f32 ? F_D0 : B_KA, db1 = f32 ? F_D1 : B_KB, accb = f32 ? F_ACC : B_ACC;
     for (int s = 0; s < m; s++) {
         launch_named("k_tay_first", k_tay_first<TD, HET>, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax);
+        const bool pairs = f32 && !HET && G.pairs && K >= 3;
         FinArgs a{};
         a.sub = s;
         a.seq = seq;
         a.kmax = G.d_kmax;
         a.trec = G.d_trec;
+        a.kg = K;
+        a.pairs = pairs ? 1 : 0;
         finish(G, FIN_TAY, a);
         const int *km = G.d_kmax;
         const double *tr = G.d_trec;
-        for (int j = K; j >= 2; j--) {
+        auto single = [&](int j) {
             if (f32 && !HET && G.vec4)
                 launch_named("k_tay_term", k_tay_term4, G.ntwork, G.stream, dd, (const WorkItem *)G.d_twork,
                              G.tnper, accb, db0, db1, j, km, tr);
             else
                 launch_named("k_tay_term", k_tay_term<TD, HET>, G.nwork, G.stream, dd, w, G.nper, accb, db0, db1,
                              j, km, tr);
+        };
+        if (pairs) {
+            // step K alone when K is even, then the pairs (j, j-1) down to (3, 2)
+            if (K % 2 == 0) single(K);
+            for (int j = K % 2 ? K : K - 1; j >= 3; j -= 2) {
+                launch_named("k_tay_pair", k_tay_pair, G.npwork, G.stream, dd, (const WorkItem *)G.d_pwork, G.pnper,
+                             accb, db0, db1, j, K, km, tr);
+                hipLaunchKernelGGL(k_tay_pair_fix, dim3(G.nprob), dim3(kBlock), 0, G.stream, dd, accb, db0, db1, j,
+                                   K, km, tr);
+            }
+        } else {
+            for (int j = K; j >= 2; j--) single(j);
         }
-        launch_named("k_tay_final", k_tay_final<TD>, G.nwork, G.stream, dd, w, G.nper, accb, db0, db1, tr);
+        launch_named("k_tay_final", k_tay_final<TD>, G.nwork, G.stream, dd, w, G.nper, accb, db0, db1, tr,
+                     pairs ? K : 0);
     }
 }
 
@@ -1488,7 +1539,7 @@ static void seq_body(const Group &G, const SeqForm &f) {
 // scalar slots that start a sequence of n intervals of length dt
 static std::vector<std::pair<int, double>> seq_slots(int n, double dt) {
     return {{S_STEP, 0.0}, {S_NSTEPS, (double)n}, {S_TCALLS, 0.0}, {S_CALLS, 0.0},
-            {S_STATUS, (double)RK_DONE}, {S_DT, dt}, {S_TTERMS, 0.0}};
+            {S_STATUS, (double)RK_DONE}, {S_DT, dt}, {S_TTERMS, 0.0}, {S_TBYTES, 0.0}};
 }
 
 void dev_transient_run(Model &M, const double *power_seq, int steps, double dt, const double *init_temp,

@@ -62,9 +62,10 @@ enum Scal {
     S_KPLAN,                     // Taylor: most terms a sub-step may take
     S_TTERMS,                    // Taylor: term sweeps (E_k -> E_k+1) summed over a sequence
     S_MGINIT,                    // multigrid-preconditioned inner solve: first V-cycle pending
+    S_TBYTES,                    // Taylor: algorithmic bytes per node of the dominant sweep kernel, summed
     NSCAL = 48
 };
-static_assert(S_MGINIT < NSCAL, "scalar slots");
+static_assert(S_TBYTES < NSCAL, "scalar slots");
 enum RkStatus { RK_NEED_K1 = 0, RK_ATTEMPT = 1, RK_DONE = 2 };
 
 struct ProbDesc {

@@ -429,6 +429,8 @@ typedef struct hs_job_info_st {
     double wait_s;            /* submission -> admission */
     double latency_s;         /* submission -> result (so far, while running) */
     double device_bytes;      /* device memory held while resident */
+    double taylor_node_bytes; /* algorithmic bytes per node moved by the dominant Taylor sweep
+                               * kernel over the whole sequence (throughput accounting) */
 } hs_job_info_t;
 int hs_engine_job_info(hs_engine_t *e, int job, hs_job_info_t *info);
 /* frees a finished job's results */
