@@ -118,12 +118,16 @@ struct Cell {
 
 // Element access through a 32-bit byte offset from a wave-uniform base: the
 // address is an SGPR pair plus one VGPR (global_load ... saddr), half the
-// registers of a 64-bit address per load.
+// registers of a 64-bit address per load.  Every buffer these take is device
+// memory, so the pointers are cast to the global address space: loads through
+// generic pointers read from descriptors would otherwise be flat loads, which
+// also count against the LDS/scalar counter and so wait behind each other.
+#define HS_GLOBAL __attribute__((address_space(1)))
 template <typename T> __device__ __forceinline__ T ld_at(const T *base, unsigned i) {
-    return *(const T *)((const char *)base + i * (unsigned)sizeof(T));
+    return *(const HS_GLOBAL T *)((const HS_GLOBAL char *)base + i * (unsigned)sizeof(T));
 }
 template <typename T> __device__ __forceinline__ void st_at(T *base, unsigned i, T v) {
-    *(T *)((char *)base + i * (unsigned)sizeof(T)) = v;
+    *(HS_GLOBAL T *)((HS_GLOBAL char *)base + i * (unsigned)sizeof(T)) = v;
 }
 
 // A tile is TX columns x (TY * rpt) rows of one layer; thread (tx, ty) owns
@@ -1890,11 +1894,14 @@ __device__ __forceinline__ TayItem tay_item(const WorkItem *__restrict__ W, int 
 // same tile, each thread owns 4 consecutive columns x rpt/4 rows (rpt >= 4,
 // tile width >= 16), so a cell costs 2 load instructions instead of 8.  The
 // arithmetic per cell is that of stencil() in the same order.
+typedef float hs_f4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float4 ld4(const float *base, unsigned i) {
-    return *(const float4 *)((const char *)base + i * 4u);
+    const hs_f4 v = *(const HS_GLOBAL hs_f4 *)((const HS_GLOBAL char *)base + i * 4u);
+    return make_float4(v.x, v.y, v.z, v.w);
 }
 __device__ __forceinline__ void st4(float *base, unsigned i, float4 v) {
-    *(float4 *)((char *)base + i * 4u) = v;
+    const hs_f4 w = {v.x, v.y, v.z, v.w};
+    *(HS_GLOBAL hs_f4 *)((HS_GLOBAL char *)base + i * 4u) = w;
 }
 __device__ __forceinline__ float comp(const float4 &v, int k) {
     return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
@@ -2122,6 +2129,19 @@ __device__ __forceinline__ LayerCoef layer_coef(const ProbDesc &d, int n) {
     return L;
 }
 
+// position of a perimeter cell (i, j) in a layer's perimeter row of the
+// side buffer: top row, bottom row, then the left and right columns
+__device__ __forceinline__ int perim_len(const ProbDesc &d) { return 2 * d.nc + 2 * (d.nr - 2); }
+__device__ __forceinline__ int perim_idx(const ProbDesc &d, int i, int j) {
+    if (i == 0) return j;
+    if (i == d.nr - 1) return d.nc + j;
+    if (j == 0) return 2 * d.nc + (i - 1);
+    return 2 * d.nc + (d.nr - 2) + (i - 1);
+}
+__device__ __forceinline__ bool has_ring(const LayerCoef &L) {
+    return L.rn[0] >= 0 || L.rn[1] >= 0 || L.rn[2] >= 0 || L.rn[3] >= 0;
+}
+
 // One Horner step at a cell from its 6 neighbours (missing ones excluded), in
 // the order of stencil(); ring terms when ring != nullptr.
 __device__ __forceinline__ float horner_cell(const LayerCoef &L, float e1, float s, float xc, float xn, float xs,
@@ -2144,10 +2164,14 @@ __device__ __forceinline__ float horner_cell(const LayerCoef &L, float e1, float
     return e1 + s * (L.ci * (flux - L.gam * xc));
 }
 
-__global__ void __launch_bounds__(kBlock) k_tay_pair(const ProbDesc *__restrict__ D,
-                                                     const WorkItem *__restrict__ W, int nper, int e1b, int d0b,
-                                                     int d1b, int j, int kg, const int *__restrict__ kmax,
-                                                     const double *__restrict__ trec) {
+// 8 workgroups per CU (64 VGPRs, ~20 KB of LDS each): a 16-job batch of
+// 256 x 256 grids (2048 patches) is resident at once.  The loop body is
+// branch-free (clamped addresses, a spare LDS row for cells outside a
+// region) so that the loads issued one stage ahead stay in flight.
+__global__ void __launch_bounds__(kBlock, 8) k_tay_pair(const ProbDesc *__restrict__ D,
+                                                        const WorkItem *__restrict__ W, int nper, int e1b, int d0b,
+                                                        int d1b, int j, int kg, const int *__restrict__ kmax,
+                                                        const double *__restrict__ trec) {
     if (j - 1 > *kmax) return;
     const TayItem it = tay_item(W, nper, trec);
     PairPlan pp;
@@ -2161,144 +2185,202 @@ __global__ void __launch_bounds__(kBlock) k_tay_pair(const ProbDesc *__restrict_
     const float *h0 = d.b32[pp.in], *e1 = d.b32[e1b];
     float *out = d.b32[pp.out];
     const unsigned plane = (unsigned)nr * nc, ncell = plane * nl;
-    __shared__ float H0[3][kPY + 4][kPX + 4];
-    __shared__ float H1[3][kPY + 2][kPX + 2];
+    __shared__ float H0[3][kPY + 5][kPX + 4];   // last row: writes of cells outside the region
+    __shared__ float H1[3][kPY + 3][kPX + 2];
+    __shared__ LayerCoef LC[kMaxLayers];
+    __shared__ float XR[kMaxLayers][4];
+    for (int n = threadIdx.x; n < nl; n += kBlock) {
+        LC[n] = layer_coef(d, n);
+        for (int e = 0; e < 4; e++) XR[n][e] = ld_at(h0, ncell + max(d.ring_node[n][e], 0));
+    }
+    // Fixed cells of each thread in the three regions (patch + 2, patch + 1,
+    // patch): rows ty, ty + 4, ty + 8 of column tx, and, for 64-wide patches,
+    // the columns past 64 taken by the first threads.
+    const int tx = threadIdx.x & (kWave - 1), ty = threadIdx.x >> 6;
     const int W0 = PX + 4, R0 = PY + 4, W1 = PX + 2, R1 = PY + 2;
+    int ar[4], ac[4], cr[4], cc[4];
+    unsigned ag[4], cg[4], eg[2];
+    bool ain[4], cin_[4], cok[4];
+    for (int k = 0; k < 4; k++) {
+        int r, c;
+        if (k < 3) {
+            r = ty + 4 * k;
+            c = tx;
+        } else {
+            r = threadIdx.x >> 2;
+            c = 64 + (threadIdx.x & 3);
+        }
+        const bool reg = r < R0 && c < W0 && (k < 3 || W0 > 64);
+        const int gi = i0 - 2 + r, gj = j0 - 2 + c;
+        ain[k] = reg && gi >= 0 && gi < nr && gj >= 0 && gj < nc;
+        ar[k] = reg ? r : kPY + 4;
+        ac[k] = reg ? c : 0;
+        ag[k] = ain[k] ? (unsigned)(gi * nc + gj) : 0u;
+    }
+    for (int k = 0; k < 4; k++) {
+        int r, c;
+        if (k < 3) {
+            r = ty + 4 * k;
+            c = tx;
+        } else {
+            r = threadIdx.x >> 1;
+            c = 64 + (threadIdx.x & 1);
+        }
+        const bool reg = r < R1 && c < W1 && (k < 3 || W1 > 64);
+        const int gi = i0 - 1 + r, gj = j0 - 1 + c;
+        cin_[k] = reg && gi >= 0 && gi < nr && gj >= 0 && gj < nc;
+        cok[k] = reg;
+        cr[k] = reg ? r : 0;
+        cc[k] = reg ? c : 0;
+        cg[k] = cin_[k] ? (unsigned)(gi * nc + gj) : 0u;
+    }
+    bool eok[2];
+    for (int k = 0; k < 2; k++) {
+        const int r = ty + 4 * k;
+        eok[k] = r < PY && tx < PX;
+        eg[k] = eok[k] ? (unsigned)((i0 + r) * nc + j0 + tx) : 0u;
+    }
+    float pa[4], pc[4], pe[2];
+    for (int k = 0; k < 4; k++) pa[k] = ld_at(h0, ag[k]);
     for (int t = 0; t <= nl + 1; t++) {
         if (t < nl) {   // h0 of plane t on the patch + 2 (0 outside the grid)
-            for (int idx = threadIdx.x; idx < R0 * W0; idx += kBlock) {
-                const int r = idx / W0, c = idx - r * W0;
-                const int gi = i0 - 2 + r, gj = j0 - 2 + c;
-                float v = 0.f;
-                if (gi >= 0 && gi < nr && gj >= 0 && gj < nc) v = ld_at(h0, t * plane + (unsigned)gi * nc + gj);
-                H0[t % 3][r][c] = v;
-            }
+            for (int k = 0; k < 4; k++) H0[t % 3][ar[k]][ac[k]] = ain[k] ? pa[k] : 0.f;
+            const unsigned tp = (unsigned)min(t + 1, nl - 1) * plane;
+            for (int k = 0; k < 4; k++) pa[k] = ld_at(h0, tp + ag[k]);
         }
         __syncthreads();
         const int m = t - 1;   // h1 of plane m on the patch + 1
         if (m >= 0 && m < nl) {
-            const LayerCoef L = layer_coef(d, m);
-            float ring[4];
-            for (int e = 0; e < 4; e++) ring[e] = ld_at(h0, ncell + max(L.rn[e], 0));
+            const LayerCoef &L = LC[m];
+            const float *ring = XR[m];
             const int sm = m % 3, su = (m + 2) % 3, sd = (m + 1) % 3;
-            for (int idx = threadIdx.x; idx < R1 * W1; idx += kBlock) {
-                const int r = idx / W1, c = idx - r * W1;
+            const float s = pp.mode == 2 ? pp.s1 : pp.s2;
+            const bool hu = m > 0, hd = m < nl - 1;
+            for (int k = 0; k < 4; k++) {
+                const int r = cr[k], c = cc[k];
                 const int gi = i0 - 1 + r, gj = j0 - 1 + c;
-                float v = 0.f;
-                if (gi >= 0 && gi < nr && gj >= 0 && gj < nc) {
-                    const float e1v = ld_at(e1, m * plane + (unsigned)gi * nc + gj);
-                    const float s = pp.mode == 2 ? pp.s1 : pp.s2;
-                    v = horner_cell(L, e1v, s, H0[sm][r + 1][c + 1], H0[sm][r][c + 1], H0[sm][r + 2][c + 1],
-                                    H0[sm][r + 1][c + 2], H0[sm][r + 1][c], m > 0 ? H0[su][r + 1][c + 1] : 0.f,
-                                    m < nl - 1 ? H0[sd][r + 1][c + 1] : 0.f, gi > 0, gi < nr - 1, gj < nc - 1,
-                                    gj > 0, m > 0, m < nl - 1, ring, gi, gj, nr, nc);
-                }
-                H1[sm][r][c] = v;
+                const float xc = H0[sm][r + 1][c + 1];
+                const float v = horner_cell(L, pc[k], s, xc, H0[sm][r][c + 1], H0[sm][r + 2][c + 1],
+                                            H0[sm][r + 1][c + 2], H0[sm][r + 1][c], H0[su][r + 1][c + 1],
+                                            H0[sd][r + 1][c + 1], gi > 0, gi < nr - 1, gj < nc - 1, gj > 0, hu, hd,
+                                            ring, gi, gj, nr, nc);
+                H1[sm][cok[k] ? r : kPY + 2][c] = cin_[k] ? v : 0.f;
+                // h1 on the perimeter of a ring layer, for the fix-up (each
+                // cell written by the patch that owns it)
+                if (pp.mode == 2 && cin_[k] && (gi == 0 || gi == nr - 1 || gj == 0 || gj == nc - 1) && r >= 1 &&
+                    r <= PY && c >= 1 && c <= PX && has_ring(L))
+                    st_at(d.perim_h1, (unsigned)(m * perim_len(d) + perim_idx(d, gi, gj)), v);
             }
         }
+        if (t < nl)   // E_1 of the next h1 plane
+            for (int k = 0; k < 4; k++) pc[k] = ld_at(e1, (unsigned)t * plane + cg[k]);
         __syncthreads();
         const int m2 = t - 2;   // h2 of plane m2 on the patch (or h1 alone in mode 1)
         if (m2 >= 0 && m2 < nl) {
-            const LayerCoef L = layer_coef(d, m2);
+            const LayerCoef &L = LC[m2];
             const int sm = m2 % 3, su = (m2 + 2) % 3, sd = (m2 + 1) % 3;
-            for (int idx = threadIdx.x; idx < PY * PX; idx += kBlock) {
-                const int r = idx / PX, c = idx - r * PX;
-                const int gi = i0 + r, gj = j0 + c;
-                const unsigned a = m2 * plane + (unsigned)gi * nc + gj;
+            const bool hu = m2 > 0, hd = m2 < nl - 1;
+            for (int k = 0; k < 2; k++) {
+                const int r = ty + 4 * k, c = tx;
+                const int rr = eok[k] ? r : 0, cq = eok[k] ? c : 0;
+                const int gi = i0 + rr, gj = j0 + cq;
                 float v;
                 if (pp.mode == 1) {
-                    v = H1[sm][r + 1][c + 1];
+                    v = H1[sm][rr + 1][cq + 1];
                 } else {
-                    v = horner_cell(L, ld_at(e1, a), pp.s2, H1[sm][r + 1][c + 1], H1[sm][r][c + 1],
-                                    H1[sm][r + 2][c + 1], H1[sm][r + 1][c + 2], H1[sm][r + 1][c],
-                                    m2 > 0 ? H1[su][r + 1][c + 1] : 0.f, m2 < nl - 1 ? H1[sd][r + 1][c + 1] : 0.f,
-                                    gi > 0, gi < nr - 1, gj < nc - 1, gj > 0, m2 > 0, m2 < nl - 1, nullptr, gi, gj,
-                                    nr, nc);
+                    v = horner_cell(L, pe[k], pp.s2, H1[sm][rr + 1][cq + 1], H1[sm][rr][cq + 1], H1[sm][rr + 2][cq + 1],
+                                    H1[sm][rr + 1][cq + 2], H1[sm][rr + 1][cq], H1[su][rr + 1][cq + 1],
+                                    H1[sd][rr + 1][cq + 1], gi > 0, gi < nr - 1, gj < nc - 1, gj > 0, hu, hd,
+                                    nullptr, gi, gj, nr, nc);
                 }
-                st_at(out, a, v);
+                if (eok[k]) st_at(out, (unsigned)m2 * plane + eg[k], v);
             }
         }
+        if (t - 1 >= 0 && t - 1 < nl)   // E_1 of the next h2 plane
+            for (int k = 0; k < 2; k++) pe[k] = ld_at(e1, (unsigned)(t - 1) * plane + eg[k]);
     }
 }
 
-// h1 at one cell from h0 (every term, rings included)
-__device__ __forceinline__ float pair_h1_at(const ProbDesc &d, const float *h0, const float *e1, float s1, int n,
-                                            int i, int j) {
-    const int nr = d.nr, nc = d.nc, nl = d.nl;
-    const unsigned plane = (unsigned)nr * nc, ncell = plane * nl, c = n * plane + (unsigned)i * nc + j;
-    const LayerCoef L = layer_coef(d, n);
-    float ring[4];
-    for (int e = 0; e < 4; e++) ring[e] = ld_at(h0, ncell + max(L.rn[e], 0));
-    const float xc = ld_at(h0, c);
-    return horner_cell(L, ld_at(e1, c), s1, xc, i > 0 ? ld_at(h0, c - nc) : 0.f,
-                       i < nr - 1 ? ld_at(h0, c + nc) : 0.f, j < nc - 1 ? ld_at(h0, c + 1) : 0.f,
-                       j > 0 ? ld_at(h0, c - 1) : 0.f, n > 0 ? ld_at(h0, c - plane) : 0.f,
-                       n < nl - 1 ? ld_at(h0, c + plane) : 0.f, i > 0, i < nr - 1, j < nc - 1, j > 0, n > 0,
-                       n < nl - 1, ring, i, j, nr, nc);
-}
+// After a pair launch, kPairFixBlocks workgroups per problem: the package
+// nodes' h1 (from h0) and h2 (from h1 on the boundary rows, saved by
+// k_tay_pair), and the ring terms of h2 on the perimeter of ring layers.
+// Mode 1: the package nodes' one step.
+constexpr int kPairFixBlocks = 4;
 
-// After a pair launch, one workgroup per problem: the package nodes' h1 (from
-// h0) and h2 (from h1 on the boundary rows, recomputed), and the ring terms
-// of h2 at the boundary cells of ring layers.  Mode 1: the package nodes' one
-// step.
 __global__ void __launch_bounds__(kBlock) k_tay_pair_fix(const ProbDesc *__restrict__ D, int e1b, int d0b,
                                                          int d1b, int j, int kg, const int *__restrict__ kmax,
                                                          const double *__restrict__ trec) {
     if (j - 1 > *kmax) return;
-    const int p = blockIdx.x;
+    const int p = blockIdx.x / kPairFixBlocks, blk = blockIdx.x - p * kPairFixBlocks;
     PairPlan pp;
     if (!pair_plan((int)trec[2 * p + 1], kg, j, trec[2 * p], e1b, d0b, d1b, pp)) return;
     const ProbDesc &d = D[p];
-    const int nr = d.nr, nc = d.nc, nl = d.nl;
+    const int nr = d.nr, nc = d.nc, nl = d.nl, nx = d.nx;
     const unsigned plane = (unsigned)nr * nc, ncell = plane * nl;
     const float *h0 = d.b32[pp.in], *e1 = d.b32[e1b];
     float *out = d.b32[pp.out];
-    __shared__ float xf[kMaxExtra], xd[kMaxExtra], h1x[kMaxExtra], xf1[kMaxExtra];
-    LdPlain<float> ld{h0};
-    package_terms<float>(d, ld, true, xf, xd);
-    const int k0 = threadIdx.x;
-    const float s_first = pp.mode == 2 ? pp.s1 : pp.s2;
-    if (k0 < d.nx)
-        h1x[k0] = e1[ncell + k0] + s_first * ((float)d.x_cinv[k0] * (xf[k0] - (float)d.x_gamb[k0] * h0[ncell + k0]));
-    __syncthreads();
+    __shared__ float h1x[kMaxExtra];
+    __shared__ double s0[kMaxExtra], s1[kMaxExtra];
     if (pp.mode == 1) {
-        if (k0 < d.nx) out[ncell + k0] = h1x[k0];
+        if (blk) return;
+        __shared__ float xf[kMaxExtra], xd[kMaxExtra];
+        LdPlain<float> ld{h0};
+        package_terms<float>(d, ld, true, xf, xd);
+        const int k = threadIdx.x;
+        if (k < nx)
+            out[ncell + k] = e1[ncell + k] + pp.s2 * ((float)d.x_cinv[k] * (xf[k] - (float)d.x_gamb[k] * h0[ncell + k]));
         return;
     }
-    // boundary rows of the ring layers: h1 recomputed, its ring sum, and the
-    // ring term of h2 (corner cells below)
+    // ring sums of h0 and of the saved h1 over each package node's boundary row
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-    for (int k = w; k < d.nx; k += kBlock / kWave) {
+    const int plen = perim_len(d);
+    for (int k = w; k < nx; k += kBlock / kWave) {
         const int L_ = d.x_ring_layer[k];
-        if (L_ < 0) {
-            if (lane == 0) xf1[k] = 0.f;
-            continue;
-        }
-        const int e = d.x_ring_edge[k];
-        const int m = e < 2 ? nc : nr;
-        const float ci = (float)d.cinv[L_], rg = (float)d.x_ring_g[k];
-        double s = 0.0;
-        for (int t = lane; t < m; t += kWave) {
-            const int i = e == 0 ? 0 : e == 1 ? nr - 1 : t, jj = e == 2 ? nc - 1 : e == 3 ? 0 : t;
-            const float h1c = pair_h1_at(d, h0, e1, pp.s1, L_, i, jj);
-            s += (double)(h1c - h1x[k]);
-            if (t != 0 && t != m - 1) {
-                const unsigned c = L_ * plane + (unsigned)i * nc + jj;
-                out[c] += pp.s2 * (ci * (rg * (h1x[k] - h1c)));
+        double a0 = 0.0, a1 = 0.0;
+        if (L_ >= 0) {
+            const int e = d.x_ring_edge[k];
+            const int m = e < 2 ? nc : nr;
+            const float xk = h0[ncell + k];
+            for (int t = lane; t < m; t += kWave) {
+                const int i = e == 0 ? 0 : e == 1 ? nr - 1 : t, jj = e == 2 ? nc - 1 : e == 3 ? 0 : t;
+                a0 += (double)(ld_at(h0, L_ * plane + (unsigned)i * nc + jj) - xk);
+                a1 += (double)d.perim_h1[L_ * plen + perim_idx(d, i, jj)];
             }
+            a0 = wave_sum(a0);
+            a1 = wave_sum(a1);
         }
-        s = wave_sum(s);
-        if (lane == 0) xf1[k] = rg * (float)s;
+        if (lane == 0) {
+            s0[k] = a0;
+            s1[k] = a1;
+        }
     }
-    // corner cells: every ring term of the cell
-    for (int q = threadIdx.x; q < 4 * nl; q += kBlock) {
-        const int n = q >> 2, corner = q & 3;
-        const int i = (corner & 1) ? nr - 1 : 0, jj = (corner & 2) ? nc - 1 : 0;
+    __syncthreads();
+    if ((int)threadIdx.x < nx) {   // h1 of the package nodes (the terms of package_terms)
+        const int k = threadIdx.x, L_ = d.x_ring_layer[k];
+        const float xk = h0[ncell + k];
+        float flux = L_ >= 0 ? (float)d.x_ring_g[k] * (float)s0[k] : 0.f;
+        for (int q = 0; q < d.x_ncpl[k]; q++) {
+            if (d.x_cpl_steady_only[k][q]) continue;
+            flux += (float)d.x_cpl_g[k][q] * (h0[ncell + d.x_cpl_node[k][q]] - xk);
+        }
+        h1x[k] = e1[ncell + k] + pp.s1 * ((float)d.x_cinv[k] * (flux - (float)d.x_gamb[k] * xk));
+    }
+    __syncthreads();
+    // ring terms of h2 on the perimeter of every ring layer, cells spread
+    // over the problem's workgroups (one writer per cell)
+    for (int q = blk * kBlock + threadIdx.x; q < nl * plen; q += kPairFixBlocks * kBlock) {
+        const int n = q / plen, pq = q - n * plen;
         const int *rn = d.ring_node[n];
+        if (rn[0] < 0 && rn[1] < 0 && rn[2] < 0 && rn[3] < 0) continue;
+        int i, jj;
+        if (pq < nc) { i = 0; jj = pq; }
+        else if (pq < 2 * nc) { i = nr - 1; jj = pq - nc; }
+        else if (pq < 2 * nc + nr - 2) { i = pq - 2 * nc + 1; jj = 0; }
+        else { i = pq - 2 * nc - (nr - 2) + 1; jj = nc - 1; }
         const bool tn = i == 0 && rn[0] >= 0, ts = i == nr - 1 && rn[1] >= 0;
         const bool te = jj == nc - 1 && rn[2] >= 0, tw = jj == 0 && rn[3] >= 0;
         if (!(tn || ts || te || tw)) continue;
-        const float h1c = pair_h1_at(d, h0, e1, pp.s1, n, i, jj);
+        const float h1c = d.perim_h1[q];
         const float rgn = (float)d.ring_gns[n], rge = (float)d.ring_gew[n];
         float f = 0.f;
         if (tn) f += rgn * (h1x[rn[0]] - h1c);
@@ -2308,14 +2390,18 @@ __global__ void __launch_bounds__(kBlock) k_tay_pair_fix(const ProbDesc *__restr
         const unsigned c = n * plane + (unsigned)i * nc + jj;
         out[c] += pp.s2 * ((float)d.cinv[n] * f);
     }
-    __syncthreads();
-    if (k0 < d.nx) {
-        float flux = xf1[k0];
-        for (int q = 0; q < d.x_ncpl[k0]; q++) {
-            if (d.x_cpl_steady_only[k0][q]) continue;
-            flux += (float)d.x_cpl_g[k0][q] * (h1x[d.x_cpl_node[k0][q]] - h1x[k0]);
+    if (blk == 0 && (int)threadIdx.x < nx) {   // h2 of the package nodes
+        const int k = threadIdx.x, L_ = d.x_ring_layer[k];
+        float flux = 0.f;
+        if (L_ >= 0) {
+            const int m = d.x_ring_edge[k] < 2 ? nc : nr;
+            flux = (float)d.x_ring_g[k] * (float)(s1[k] - (double)m * h1x[k]);
         }
-        out[ncell + k0] = e1[ncell + k0] + pp.s2 * ((float)d.x_cinv[k0] * (flux - (float)d.x_gamb[k0] * h1x[k0]));
+        for (int q = 0; q < d.x_ncpl[k]; q++) {
+            if (d.x_cpl_steady_only[k][q]) continue;
+            flux += (float)d.x_cpl_g[k][q] * (h1x[d.x_cpl_node[k][q]] - h1x[k]);
+        }
+        out[ncell + k] = e1[ncell + k] + pp.s2 * ((float)d.x_cinv[k] * (flux - (float)d.x_gamb[k] * h1x[k]));
     }
 }
 

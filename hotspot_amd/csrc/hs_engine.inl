@@ -31,14 +31,13 @@ struct Group {
     int npwork = 0, pnper = 0;
 };
 
-// Paired sweeps when HS_PAIRS=1 (off by default: single sweeps are faster
-// until the paired kernel hides its per-layer latency).
+// Paired sweeps when HS_PAIRS=1 at model or group set-up.  Off by default:
+// on the 16-job batch of 256 x 256 x 9 stacks a pair sweep takes 70 us
+// against 2 x 24 us for two single float4 sweeps (its LDS and VALU work per
+// layer, not HBM, bounds it), though it halves the HBM bytes per step.
 static bool pairs_enabled() {
-    static const bool on = [] {
-        const char *e = getenv("HS_PAIRS");
-        return e && atoi(e) == 1;
-    }();
-    return on;
+    const char *e = getenv("HS_PAIRS");
+    return e && atoi(e) == 1;
 }
 
 // Work-list of the paired sweeps: (problem, patch) over kPX x kPY patches
@@ -622,6 +621,8 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
         h.b32[b] = D->alloc<float>(D->vlen);
         HIP_OK(hipMemset(h.b32[b], 0, D->vlen * sizeof(float)));
     }
+    // h1 on every layer's perimeter (paired Taylor sweeps)
+    h.perim_h1 = D->alloc<float>((size_t)h.nl * (2 * h.nc + 2 * std::max(h.nr - 2, 0)));
     // block <-> grid maps
     std::vector<int> base;
     layer_bases(M, base);
@@ -1332,7 +1333,8 @@ static void taylor_interval(const Group &G, int m, int K, int seq) {
             for (int j = K % 2 ? K : K - 1; j >= 3; j -= 2) {
                 launch_named("k_tay_pair", k_tay_pair, G.npwork, G.stream, dd, (const WorkItem *)G.d_pwork, G.pnper,
                              accb, db0, db1, j, K, km, tr);
-                hipLaunchKernelGGL(k_tay_pair_fix, dim3(G.nprob), dim3(kBlock), 0, G.stream, dd, accb, db0, db1, j,
+                hipLaunchKernelGGL(k_tay_pair_fix, dim3(G.nprob * kPairFixBlocks), dim3(kBlock), 0, G.stream, dd, accb,
+                                   db0, db1, j,
                                    K, km, tr);
             }
         } else {

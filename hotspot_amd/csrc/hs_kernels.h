@@ -108,6 +108,9 @@ struct ProbDesc {
     // backward-Euler descriptors of heterogeneous models: sigma C per cell,
     // added to the ambient conductance of the stencil (nullptr otherwise)
     const double *bshift;
+    // paired Taylor sweeps: h1 on the perimeter of every layer,
+    // [nl][2 nc + 2 (nr - 2)] (top row, bottom row, left and right columns)
+    float *perim_h1;
     // block <-> grid maps (per layer)
     const int *map_ptr[kMaxLayers];     // cell -> entry range
     const int *map_unit[kMaxLayers];
