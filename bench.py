@@ -422,6 +422,7 @@ def main():
         admit_t.append(ta)
         lat += [i.latency_s for i in infos]
         step_sweeps = float(sum(i.taylor_sweeps for i in infos))
+        step_node_bytes = float(sum(i.taylor_node_bytes for i in infos))
         log(f"step {k}: batch of {args.batch} in {t:.3f} s (admission {ta*1e3:.0f} ms), "
             f"steady iters {[i.steady_iters for i in infos][:4]}...")
     t_step = float(np.mean(times))
@@ -532,6 +533,7 @@ def main():
                      "bytes_per_launch": bytes_per_launch,
                      "algorithmic_bytes_avg_per_launch": (node_bytes * nvec / nt) if nt else None,
                      "steps_per_batch": step_sweeps,
+                     "node_bytes_per_batch": step_node_bytes,
                      "traffic_note": "HBM bytes per launch from FETCH_SIZE and WRITE_SIZE (rocprofv3 "
                                      "--pmc, scaled against a plain copy)"},
         "single_job": single,

@@ -440,22 +440,72 @@ __global__ void __launch_bounds__(kBlock) k_b2g(const ProbDesc *__restrict__ D,
         const double *vb = b + d.layer_base[n];
         const double *area = d.unit_area[n];
         const double src = d.gamb[n] * d.ambient;
-        tile_loop<1>(d, wi.tile, [&](const Cell &) { return 0; }, [&](const Cell &q, int) {
-            const long cc = (long)q.i * d.nc + q.j;
-            double val = 0.0;
-            if (type == V_POWER) {
-                // power density of each block times its occupancy, times the cell area
-                for (int e = ptr[cc]; e < ptr[cc + 1]; e++) val += (occ[e] * vb[unit[e]]) / area[unit[e]];
-                val *= d.cell_area;
-            } else {
-                for (int e = ptr[cc]; e < ptr[cc + 1]; e++) val += occ[e] * vb[unit[e]];
-            }
-            if (add_sources) {
-                val += src;
-                if (d.bconst) val += ld_at(d.bconst, q.c);
-            }
-            st_at(out, q.c, val);
-        });
+        // A layer whose blocks all carry zero power (bond, spreader and sink
+        // layers of a chip stack) maps to zero without reading its map.
+        __shared__ int nonzero;
+        bool mapped = true;
+        if (type == V_POWER) {
+            const int nu = (n + 1 < d.nl ? d.layer_base[n + 1] : d.nblocks) - d.layer_base[n];
+            if (threadIdx.x == 0) nonzero = 0;
+            __syncthreads();
+            for (int u = threadIdx.x; u < nu; u += kBlock)
+                if (vb[u] != 0.0) nonzero = 1;
+            __syncthreads();
+            mapped = nonzero != 0;
+        }
+        if (!mapped) {
+            tile_loop<8>(d, wi.tile, [&](const Cell &) { return 0; }, [&](const Cell &q, int) {
+                double val = 0.0;
+                if (add_sources) {
+                    val += src;
+                    if (d.bconst) val += ld_at(d.bconst, q.c);
+                }
+                st_at(out, q.c, val);
+            });
+            return;
+        }
+        // a cell's first two map entries are gathered for 8 cells at a time
+        // (three dependent loads: entry range, entries, block values); cells
+        // covered by more blocks finish the rest in a loop
+        struct Ent {
+            int e0, e1;
+            int u0, u1;
+            double o0, o1;
+        };
+        tile_loop<8>(
+            d, wi.tile,
+            [&](const Cell &q) {
+                Ent t;
+                const unsigned cc = (unsigned)q.i * d.nc + q.j;
+                t.e0 = ld_at(ptr, cc);
+                t.e1 = ld_at(ptr, cc + 1);
+                const unsigned f = (unsigned)t.e0, g = (unsigned)max(min(t.e0 + 1, t.e1 - 1), t.e0);
+                t.u0 = ld_at(unit, f);
+                t.o0 = ld_at(occ, f);
+                t.u1 = ld_at(unit, g);
+                t.o1 = ld_at(occ, g);
+                return t;
+            },
+            [&](const Cell &q, const Ent &t) {
+                double val = 0.0;
+                const int cnt = t.e1 - t.e0;
+                if (type == V_POWER) {
+                    // power density of each block times its occupancy, times the cell area
+                    if (cnt > 0) val += (t.o0 * vb[t.u0]) / area[t.u0];
+                    if (cnt > 1) val += (t.o1 * vb[t.u1]) / area[t.u1];
+                    for (int e = t.e0 + 2; e < t.e1; e++) val += (occ[e] * vb[unit[e]]) / area[unit[e]];
+                    val *= d.cell_area;
+                } else {
+                    if (cnt > 0) val += t.o0 * vb[t.u0];
+                    if (cnt > 1) val += t.o1 * vb[t.u1];
+                    for (int e = t.e0 + 2; e < t.e1; e++) val += occ[e] * vb[unit[e]];
+                }
+                if (add_sources) {
+                    val += src;
+                    if (d.bconst) val += ld_at(d.bconst, q.c);
+                }
+                st_at(out, q.c, val);
+            });
     } else {
         const int k = threadIdx.x;
         if (k < d.nx) {

@@ -631,8 +631,13 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
     for (size_t mi = 0; mi < M.maps.size(); mi++) {
         const BlockMap &bm = M.maps[mi];
         dptr[mi] = D->upload(bm.cell_ptr.data(), bm.cell_ptr.size());
-        dunit[mi] = D->upload(bm.ent_unit.data(), bm.ent_unit.size());
-        docc[mi] = D->upload(bm.ent_occ.data(), bm.ent_occ.size());
+        // one spare entry: k_b2g reads a cell's first entry even when it has none
+        std::vector<int> u(bm.ent_unit);
+        std::vector<double> o(bm.ent_occ);
+        u.push_back(0);
+        o.push_back(0.0);
+        dunit[mi] = D->upload(u.data(), u.size());
+        docc[mi] = D->upload(o.data(), o.size());
     }
     std::vector<int4> rng;
     std::vector<int> blayer;
