@@ -1900,6 +1900,112 @@ __global__ void __launch_bounds__(kBlock) k_tay_first(const ProbDesc *__restrict
     write_partials<1>(d, wi.tile, v);
 }
 
+// E_1 of a homogeneous problem for the FP32 series with double2 accesses:
+// within the standard tile, each thread owns 2 consecutive columns x rpt/2
+// rows, two rows at a time.  The arithmetic per cell is that of stencil()
+// and k_tay_first in the same order.
+typedef double hs_d2 __attribute__((ext_vector_type(2)));
+typedef float hs_f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ hs_d2 ld2(const double *base, unsigned i) {
+    return *(const HS_GLOBAL hs_d2 *)((const HS_GLOBAL char *)base + i * 8u);
+}
+
+__global__ void __launch_bounds__(kBlock) k_tay_first2(const ProbDesc *__restrict__ D,
+                                                       const WorkItem *__restrict__ W, int e1b, int sub, int seq,
+                                                       int *__restrict__ kmax) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *kmax = 0;   // raised by the finish of each problem
+    const WorkItem wi = W[work_index()];
+    const ProbDesc &d = D[wi.prob];
+    if (sub >= (int)d.scal[S_TK] || seq_finished(d, seq)) return;
+    const int ypar = (int)d.scal[S_YPAR];
+    const double *y = d.b64[B_Y0 + ypar], *rhs = d.b64[B_RHS];
+    float *e1 = d.b32[e1b];
+    const double tau = d.scal[S_TAU];
+    double v[1] = {0};
+    if (wi.tile < d.ntiles) {
+        const int tile = wi.tile, tpl = d.nti * d.ntj;
+        const int n = tile / tpl, rem = tile - n * tpl;
+        const int ti = rem / d.ntj, tj = rem - ti * d.ntj;
+        const int sh = d.tx_shift;
+        const int tx2 = threadIdx.x & ((1 << (sh - 1)) - 1), ty2 = threadIdx.x >> (sh - 1);
+        const int G = d.rpt >> 1;
+        const int rows = (kBlock >> sh) * d.rpt;
+        const int nr = d.nr, nc = d.nc, nl = d.nl;
+        const unsigned plane = (unsigned)nr * nc, ncell = plane * nl;
+        const int j0 = (tj << sh) + 2 * tx2;
+        const double gx = d.gx[n], gy = d.gy[n];
+        const double gu = n > 0 ? d.gz[max(n - 1, 0)] : 0.0, gd = n < nl - 1 ? d.gz[n] : 0.0;
+        const double ci = d.cinv[n], gam = d.gamb[n];
+        const int *rn = d.ring_node[n];
+        const double rgn = d.ring_gns[n], rge = d.ring_gew[n];
+        double ring[4];
+        for (int e = 0; e < 4; e++) ring[e] = ld_at(y, ncell + max(rn[e], 0));
+        for (int g0 = 0; g0 < G; g0 += 2) {
+            hs_d2 C[2], N[2], S[2], U[2], Dn[2], B[2];
+            double Wv[2], Ev[2];
+            int I[2];
+            unsigned cc[2];
+#pragma unroll
+            for (int g = 0; g < 2; g++) {
+                I[g] = ti * rows + ty2 * G + g0 + g;
+                const int i = min(I[g], nr - 1);
+                const unsigned c = ((unsigned)n * nr + i) * nc + j0;
+                cc[g] = c;
+                C[g] = ld2(y, c);
+                N[g] = ld2(y, i > 0 ? c - nc : c);
+                S[g] = ld2(y, i < nr - 1 ? c + nc : c);
+                U[g] = ld2(y, n > 0 ? c - plane : c);
+                Dn[g] = ld2(y, n < nl - 1 ? c + plane : c);
+                Wv[g] = ld_at(y, j0 > 0 ? c - 1 : c);
+                Ev[g] = ld_at(y, j0 + 2 < nc ? c + 2 : c);
+                B[g] = ld2(rhs, c);
+            }
+#pragma unroll
+            for (int g = 0; g < 2; g++) {
+                if (g0 + g >= G || I[g] >= nr) continue;
+                const int i = I[g];
+                const double gn = i > 0 ? gx : 0.0, gs = i < nr - 1 ? gx : 0.0;
+                float e[2];
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    const int j = j0 + k;
+                    const double xc = C[g][k];
+                    const double xw = k == 0 ? Wv[g] : C[g][0];
+                    const double xe = k == 1 ? Ev[g] : C[g][1];
+                    const double ge = j < nc - 1 ? gy : 0.0, gw = j > 0 ? gy : 0.0;
+                    double flux = gn * (N[g][k] - xc);
+                    flux += gs * (S[g][k] - xc);
+                    flux += ge * (xe - xc);
+                    flux += gw * (xw - xc);
+                    flux += gu * (U[g][k] - xc);
+                    flux += gd * (Dn[g][k] - xc);
+                    if (i == 0 && rn[0] >= 0) flux += rgn * (ring[0] - xc);
+                    if (i == nr - 1 && rn[1] >= 0) flux += rgn * (ring[1] - xc);
+                    if (j == nc - 1 && rn[2] >= 0) flux += rge * (ring[2] - xc);
+                    if (j == 0 && rn[3] >= 0) flux += rge * (ring[3] - xc);
+                    e[k] = (float)(tau * (ci * (B[g][k] + flux - gam * xc)));
+                    v[0] = fmax(v[0], fabs((double)e[k]));
+                }
+                const hs_f2 ev = {e[0], e[1]};
+                *(HS_GLOBAL hs_f2 *)((HS_GLOBAL char *)e1 + cc[g] * 4u) = ev;
+            }
+        }
+    } else {
+        __shared__ double xf[kMaxExtra], xd[kMaxExtra];
+        LdPlain<double> ld{y};
+        package_terms<double>(d, ld, true, xf, xd);
+        const int k = threadIdx.x;
+        if (k < d.nx) {
+            const long c = (long)d.nl * d.nr * d.nc + k;
+            const float e = (float)(tau * (d.x_cinv[k] * (rhs[c] + xf[k] - d.x_gamb[k] * y[c])));
+            e1[c] = e;
+            v[0] = fmax(v[0], fabs((double)e));
+        }
+    }
+    block_reduce<1, true>(v);
+    write_partials<1>(d, wi.tile, v);
+}
+
 // Buffers of Horner step j of a problem with K = S_KNEED terms: the first
 // step (j = K) reads h = E_1; the steps then alternate d0 -> d1 -> d0.
 struct HornerBufs {

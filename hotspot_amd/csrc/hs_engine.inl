@@ -1312,7 +1312,10 @@ static void taylor_interval(const Group &G, int m, int K, int seq) {
     const bool f32 = sizeof(TD) == 4;
     const int db0 = f32 ? F_D0 : B_KA, db1 = f32 ? F_D1 : B_KB, accb = f32 ? F_ACC : B_ACC;
     for (int s = 0; s < m; s++) {
-        launch_named("k_tay_first", k_tay_first<TD, HET>, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax);
+        if (f32 && !HET && G.vec4)
+            launch_named("k_tay_first", k_tay_first2, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax);
+        else
+            launch_named("k_tay_first", k_tay_first<TD, HET>, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax);
         const bool pairs = f32 && !HET && G.pairs && K >= 3;
         FinArgs a{};
         a.sub = s;
