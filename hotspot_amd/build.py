@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhotspot_amd.so")
 
-SOURCES = ["hs_host.cpp", "hs_config.cpp", "hs_device.hip"]
+SOURCES = ["hs_host.cpp", "hs_config.cpp", "hs_package.cpp", "hs_device.hip"]
 DEPS = ["hs_internal.h", "hs_kernels.h", "hs_engine.inl", "hs_batch.inl",
         os.path.join("..", "..", "include", "hotspot_amd.h")]
 

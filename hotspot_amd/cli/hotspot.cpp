@@ -11,9 +11,14 @@
 // (hs_transient_run, equal to one compute_temp call per row); with it, row by
 // row so the grid state can be written after each interval.
 //
-// Not provided: the block model (model_type block) and the natural-convection
-// package model (package_model_used 1) -- both stop with an error.
+// With -package_model_used 1 the heat sink's convection resistance comes
+// from the package model (its parameters in -package_config_file); with
+// natural convection it is updated from the average sink temperature before
+// every interval and iterated with the steady solve until it settles
+// (hotspot.c:447-455, :558-566, :624-641).  Not provided: the block model
+// (model_type block), which stops with an error.
 #include <cctype>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -25,6 +30,9 @@
 namespace {
 
 constexpr int kMaxEntries = 512;   // util.h:30
+// package.h:31-33: starting sink-to-ambient difference, r_convec tolerance of
+// the natural-convection iteration, sink temperature of a thermal runaway
+constexpr double kSmallForConvec = 0.01, kNaturalConvecTol = 0.01, kMaxSinkTemp = 1000;
 const char *kNull = "(null)";
 
 [[noreturn]] void die(const std::string &msg) {
@@ -184,8 +192,13 @@ int main(int argc, char **argv) {
         microchannel_config_add_from_strs(&channel_cfg, &materials, table, size);
         channel = &channel_cfg;
     }
-    if (get_str_index(table, size, const_cast<char *>("package_model_used")) >= 0 && config.package_model_used)
-        die("package_model_used: the natural-convection package model is not part of this simulator\n");
+    // package model: r_convec from the sink and fan (hotspot.c:447-455)
+    int natural = 0;
+    if (get_str_index(table, size, const_cast<char *>("package_model_used")) >= 0 && config.package_model_used) {
+        natural = package_model(&config, table, size, config.ambient + kSmallForConvec);
+        if (config.r_convec < 0.01 || config.r_convec > 50.0)
+            printf("Warning: Heatsink convection resistance is not realistic, double-check your package settings...\n");
+    }
     if (strcmp(config.model_type, "grid"))
         die("this simulator provides the grid model only: use -model_type grid\n");
 
@@ -253,7 +266,9 @@ int main(int argc, char **argv) {
 
     const int nvec = grid->total_n_blocks + (config.model_secondary ? EXTRA + EXTRA_SEC : EXTRA);
     const double dt = model->config->sampling_intvl;
-    const bool per_row = strcmp(model->config->grid_transient_file, kNull) != 0;
+    // row by row when the grid state is written after each interval or the
+    // convection resistance follows the sink temperature
+    const bool per_row = strcmp(model->config->grid_transient_file, kNull) != 0 || natural;
     std::vector<double> seq;   // power rows, block-vector order
     int rows = 0;
     for (std::vector<double> v; !(v = read_vals(pin)).empty(); rows++) {
@@ -262,9 +277,14 @@ int main(int argc, char **argv) {
         for (int k = 0; k < n_power; k++) overall[slot[k]] += power[slot[k]];
         if (!transient) continue;
         if (per_row) {
+            if (natural) {   // hotspot.c:590-595
+                natural = package_model(model->config, table, size, calc_sink_temp(model, temp));
+                populate_R_model(model, flp);
+            }
             printf("Computing temperatures for t = %e...\n", rows * dt);
             compute_temp(model, power, rows == 0 ? temp : nullptr, dt);
-            dump_transient_temp_grid(grid, rows, dt, model->config->grid_transient_file);
+            if (strcmp(model->config->grid_transient_file, kNull))
+                dump_transient_temp_grid(grid, rows, dt, model->config->grid_transient_file);
             std::vector<double> out(n_power);
             for (int k = 0; k < n_power; k++) out[k] = temp[slot[k]];
             write_row(tout, out);
@@ -285,8 +305,21 @@ int main(int argc, char **argv) {
     }
     for (int k = 0; k < n_power; k++) overall[slot[k]] /= rows;
 
-    fprintf(stderr, "Computing steady-state temperatures...\n");
-    steady_state_temp(model, overall, steady);
+    if (natural) {   // r_convec iterated with the steady state (hotspot.c:624-638)
+        for (bool converged = false; !converged;) {
+            const double r_old = model->config->r_convec;
+            steady_state_temp(model, overall, steady);
+            const double sink = calc_sink_temp(model, steady) + kSmallForConvec;
+            natural = package_model(model->config, table, size, sink);
+            populate_R_model(model, flp);
+            if (sink > kMaxSinkTemp)
+                die("too high power for a natural convection package -- possible thermal runaway\n");
+            converged = std::fabs(model->config->r_convec - r_old) < kNaturalConvecTol;
+        }
+    } else {
+        fprintf(stderr, "Computing steady-state temperatures...\n");
+        steady_state_temp(model, overall, steady);
+    }
     if (strcmp(model->config->steady_file, kNull)) dump_temp(model, steady, model->config->steady_file);
     if (strcmp(model->config->grid_steady_file, kNull)) dump_steady_temp_grid(grid, model->config->grid_steady_file);
 

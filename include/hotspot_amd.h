@@ -291,6 +291,39 @@ int thermal_config_to_strs(thermal_config_t *config, str_pair *table, int max_en
 /* microchannel.c:49-108 */
 void microchannel_config_add_from_strs(microchannel_config_t *config, materials_list_t *materials_list,
                                        str_pair *table, int size);
+/* ---- package model: package.h:46-115, package.c ----------------------- */
+typedef struct package_config_t_st {
+    int natural_convec;    /* 0: forced convection, 1: natural convection */
+    int flow_type;         /* 0: lateral air flow from the sink side, 1: impinging from the top */
+    int sink_type;         /* 0: fin-channel sink, 1: pin-fin sink */
+    double fin_height, fin_width, channel_width;
+    double pin_height, pin_diam, pin_dist;
+    double fan_radius, motor_radius;
+    int rpm;
+} package_config_t;
+
+typedef struct convection_t_st {
+    double n_fin, sur_area_fin;
+    double n_pin, sur_area_pin;
+    double reynolds, nusselt, h_coeff, v;
+    double r_th;           /* lumped convection resistance of the sink (K/W) */
+} convection_t;
+
+/* package.c:18-51 */
+package_config_t default_package_config(void);
+/* package.c:54-108 */
+void package_config_add_from_strs(package_config_t *config, str_pair *table, int size);
+/* package.c:110-140: returns 12 */
+int package_config_to_strs(package_config_t *config, str_pair *table, int max_entries);
+/* package.c:159-278: forced convection */
+void calculate_flow(convection_t *p, package_config_t *config, thermal_config_t *thermal_config);
+/* package.c:281-335: natural convection at an average sink temperature */
+void calc_natural_convec(convection_t *p, package_config_t *config, thermal_config_t *thermal_config,
+                         double sink_temp);
+/* package.c:353-397: sets thermal_config->r_convec from the package file named in the
+ * table (package_config_file); returns natural_convec */
+int package_model(thermal_config_t *thermal_config, str_pair *table, int size, double sink_temp);
+
 /* microchannel.c:114-156: writes 17 name/value pairs and returns 16, as the
  * reference does (the last pair, n_fluid_cells, is not counted) */
 int microchannel_config_to_strs(microchannel_config_t *config, str_pair *table, int max_entries);

@@ -32,6 +32,15 @@ CASES = {
         ("-c example.config -init_file @init -f ev6.flp -p gcc.ptrace -materials_file example.materials "
          "-model_type grid -o gcc.ttrace", ["gcc.ttrace"]),
     ]),
+    "ev6_package": ("example2", ["ev6.flp", "gcc.ptrace", "example.config", "example.materials",
+                                 "@example1/package.config", "@natural.config"], [
+        ("-c example.config -f ev6.flp -p gcc.ptrace -materials_file example.materials -model_type grid "
+         "-package_model_used 1 -package_config_file package.config -steady_file forced.steady",
+         ["forced.steady"]),
+        ("-c example.config -f ev6.flp -p gcc.ptrace -materials_file example.materials -model_type grid "
+         "-package_model_used 1 -package_config_file natural.config -steady_file natural.steady "
+         "-grid_steady_file natural.grid.steady", ["natural.steady", "natural.grid.steady"]),
+    ]),
     "lcf_detailed": ("example3", ["example.config", "example.lcf", "example.materials", "example.ptrace",
                                   "floorplan1.flp", "floorplan2.flp"], [
         ("-c example.config -p example.ptrace -grid_layer_file example.lcf -materials_file example.materials "
@@ -53,15 +62,32 @@ CASES = {
 }
 
 
+# a package file of this build's own: natural convection over a fin-channel sink
+NATURAL = "# natural convection, fin-channel sink\n-natural_convec\t1\n-sink_type\t0\n-fin_height\t0.03\n" \
+          "-fin_width\t0.001\n-channel_width\t0.002\n"
+
+
+def input_file(exdir, f, dst):
+    """copies an input: 'name' from the example's directory, '@dir/name' from
+    another example, '@natural.config' is written here"""
+    if f == "@natural.config":
+        with open(os.path.join(dst, "natural.config"), "w") as fo:
+            fo.write(NATURAL)
+    elif f.startswith("@"):
+        shutil.copy(os.path.join(REF, f[1:]), os.path.join(dst, os.path.basename(f)))
+    else:
+        shutil.copy(os.path.join(REF, exdir, f), os.path.join(dst, f))
+
+
 def run_case(name, spec):
     exdir, inputs, runs = spec
     dst = os.path.join(OUT, name)
     os.makedirs(os.path.join(dst, "expected"), exist_ok=True)
     for f in inputs:
-        shutil.copy(os.path.join(REF, exdir, f), os.path.join(dst, f))
+        input_file(exdir, f, dst)
     with tempfile.TemporaryDirectory() as tmp:
         for f in inputs:
-            shutil.copy(os.path.join(REF, exdir, f), os.path.join(tmp, f))
+            input_file(exdir, f, tmp)
         init = None
         for argv, keep in runs:
             args = argv.split()
@@ -85,8 +111,10 @@ def run_case(name, spec):
 def main():
     if not os.path.exists(EXE):
         sys.exit("build the reference first: make -C oracle ref cli")
+    only = sys.argv[1:]
     for name, spec in CASES.items():
-        run_case(name, spec)
+        if not only or name in only:
+            run_case(name, spec)
 
 
 if __name__ == "__main__":
