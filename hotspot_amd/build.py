@@ -12,7 +12,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhotspot_amd.so")
 
-SOURCES = ["hs_host.cpp", "hs_config.cpp", "hs_package.cpp", "hs_device.hip"]
+SOURCES = ["hs_device.hip", "hs_host.cpp", "hs_config.cpp", "hs_package.cpp", "hs_block.cpp",
+           "hs_block.hip"]
 DEPS = ["hs_internal.h", "hs_kernels.h", "hs_engine.inl", "hs_batch.inl",
         os.path.join("..", "..", "include", "hotspot_amd.h")]
 
@@ -35,7 +36,7 @@ def build(verbose: bool = True, jobs: int = 2) -> str:
            "-packed-fp32-ops", "-munsafe-fp-atomics"]
     objs, procs = [], []
     for src in SOURCES:
-        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        obj = os.path.join(objdir, src + ".o")
         objs.append(obj)
         cmd = [hipcc()] + common + (gpu if src.endswith(".hip") else ["-x", "c++"]) + \
               ["-c", os.path.join(CSRC, src), "-o", obj]

@@ -1,5 +1,6 @@
 // hotspot -- the reference's command-line thermal simulator (hotspot.c:368-735)
-// for the grid model, over libhotspot_amd (every solve runs on the GPU).
+// for the grid and the block model, over libhotspot_amd (every solve runs on
+// the GPU).
 //
 //   hotspot -f <flp> -p <ptrace> [-o <ttrace>] [-c <config>] [-d <dump>] [-<name> <value>]...
 //
@@ -15,14 +16,15 @@
 // from the package model (its parameters in -package_config_file); with
 // natural convection it is updated from the average sink temperature before
 // every interval and iterated with the steady solve until it settles
-// (hotspot.c:447-455, :558-566, :624-641).  Not provided: the block model
-// (model_type block), which stops with an error.
+// (hotspot.c:447-455, :558-566, :624-641).  The block model (model_type
+// block) runs row by row with compute_temp, as in the reference.
 #include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <strings.h>
 #include <vector>
 
 #include "hotspot_amd.h"
@@ -42,7 +44,7 @@ const char *kNull = "(null)";
 
 void usage(const char *prog) {
     printf("Usage: %s -f <file> -p <file> [-o <file>] [-c <file>] [-d <file>] [options]\n", prog);
-    printf("Grid-model thermal simulation of a power trace on the GPU.\n");
+    printf("Thermal simulation of a power trace on the GPU (grid or block model).\n");
     printf("   -f <file>\tfloorplan (e.g. ev6.flp); a -grid_layer_file overrides it\n");
     printf("   -p <file>\tpower trace input\n");
     printf("  [-o <file>]\ttransient temperature trace output; without it only the steady\n");
@@ -53,7 +55,7 @@ void usage(const char *prog) {
     printf("  [-detailed_3D <on/off>]\theterogeneous per-cell R and C (needs a layer file)\n");
     printf("  [-use_microchannels <0/1>]\tmicrochannel cooling layers (needs detailed_3D on)\n");
     printf("  [options]\t\"-<name> <value>\" pairs override the configuration file,\n");
-    printf("           \te.g. \"-model_type grid\", \"-grid_rows 128\"\n");
+    printf("           \te.g. \"-model_type block\", \"-grid_rows 128\"\n");
 }
 
 // command-line-only parameters (hotspot.c:94-147)
@@ -199,8 +201,7 @@ int main(int argc, char **argv) {
         if (config.r_convec < 0.01 || config.r_convec > 50.0)
             printf("Warning: Heatsink convection resistance is not realistic, double-check your package settings...\n");
     }
-    if (strcmp(config.model_type, "grid"))
-        die("this simulator provides the grid model only: use -model_type grid\n");
+    const bool block = !strcasecmp(config.model_type, "block");
 
     if (g.dump != kNull) {
         static str_pair out[kMaxEntries];
@@ -211,7 +212,10 @@ int main(int argc, char **argv) {
     }
 
     flp_t *flp = nullptr;
-    if (strcmp(config.grid_layer_file, kNull)) {
+    if (block) {
+        if (g.flp == kNull) die("the block model needs a floorplan file (-f)\n");
+        flp = read_flp(&g.flp[0], 0, 0);
+    } else if (strcmp(config.grid_layer_file, kNull)) {
         if (g.flp != kNull)
             fprintf(stderr, "Warning: Layer Configuration File %s specified. Overriding floorplan file %s\n",
                     config.grid_layer_file, g.flp.c_str());
@@ -221,8 +225,9 @@ int main(int argc, char **argv) {
         die("Either LCF or FLP file must be specified\n");
     }
     RC_model_t *model = alloc_RC_model(&config, flp, channel, &materials, detailed, g.microchannels);
-    grid_model_t *grid = model->grid;
-    if (!grid->has_lcf && detailed)
+    if (!model) die("could not create the thermal model\n");
+    grid_model_t *grid = block ? nullptr : model->grid;
+    if (grid && !grid->has_lcf && detailed)
         die("-do_detailed_3D can only be used in 3D mode (if a grid_layer_file is specified)\n");
     if (g.microchannels && !detailed)
         die("-use_microchannels requires -model_type grid and do_detailed_3D on options\n");
@@ -243,9 +248,10 @@ int main(int argc, char **argv) {
     }
 
     // trace column -> block-vector entry, over the power-dissipating layers
+    // (the block model: the floorplan's units)
     std::vector<int> slot;
-    int n_power = 0;
-    for (int l = 0; l < grid->n_layers; l++)
+    int n_power = block ? flp->n_units : 0;
+    for (int l = 0; grid && l < grid->n_layers; l++)
         if (grid->layers[l].has_power) n_power += grid->layers[l].flp->n_units;
     FILE *pin = fopen(g.ptrace.c_str(), "r");
     if (!pin) die("unable to open power trace input file\n");
@@ -253,7 +259,9 @@ int main(int argc, char **argv) {
     if (transient && !(tout = fopen(g.ttrace.c_str(), "w"))) die("unable to open temperature trace file for output\n");
     std::vector<std::string> names = read_names(pin);
     if ((int)names.size() != n_power) die("no. of units in floorplan and trace file differ\n");
-    for (int l = 0, base = 0, col = 0; l < grid->n_layers; l++) {
+    if (block)
+        for (int k = 0; k < n_power; k++) slot.push_back(get_blk_index(flp, &names[k][0]));
+    for (int l = 0, base = 0, col = 0; grid && l < grid->n_layers; l++) {
         flp_t *f = grid->layers[l].flp;
         if (grid->layers[l].has_power) {
             for (int j = 0; j < f->n_units; j++)
@@ -264,11 +272,14 @@ int main(int argc, char **argv) {
     }
     if (transient) write_row(tout, names);
 
-    const int nvec = grid->total_n_blocks + (config.model_secondary ? EXTRA + EXTRA_SEC : EXTRA);
+    const int nvec = block ? model->block->n_nodes
+                           : grid->total_n_blocks + (config.model_secondary ? EXTRA + EXTRA_SEC : EXTRA);
     const double dt = model->config->sampling_intvl;
-    // row by row when the grid state is written after each interval or the
-    // convection resistance follows the sink temperature
-    const bool per_row = strcmp(model->config->grid_transient_file, kNull) != 0 || natural;
+    // row by row for the block model, when the grid state is written after
+    // each interval, or when the convection resistance follows the sink
+    // temperature
+    const bool grid_trace = !block && strcmp(model->config->grid_transient_file, kNull) != 0;
+    const bool per_row = block || grid_trace || natural;
     std::vector<double> seq;   // power rows, block-vector order
     int rows = 0;
     for (std::vector<double> v; !(v = read_vals(pin)).empty(); rows++) {
@@ -282,9 +293,8 @@ int main(int argc, char **argv) {
                 populate_R_model(model, flp);
             }
             printf("Computing temperatures for t = %e...\n", rows * dt);
-            compute_temp(model, power, rows == 0 ? temp : nullptr, dt);
-            if (strcmp(model->config->grid_transient_file, kNull))
-                dump_transient_temp_grid(grid, rows, dt, model->config->grid_transient_file);
+            compute_temp(model, power, block || rows == 0 ? temp : nullptr, dt);
+            if (grid_trace) dump_transient_temp_grid(grid, rows, dt, model->config->grid_transient_file);
             std::vector<double> out(n_power);
             for (int k = 0; k < n_power; k++) out[k] = temp[slot[k]];
             write_row(tout, out);
@@ -321,12 +331,13 @@ int main(int argc, char **argv) {
         steady_state_temp(model, overall, steady);
     }
     if (strcmp(model->config->steady_file, kNull)) dump_temp(model, steady, model->config->steady_file);
-    if (strcmp(model->config->grid_steady_file, kNull)) dump_steady_temp_grid(grid, model->config->grid_steady_file);
+    if (grid && strcmp(model->config->grid_steady_file, kNull))
+        dump_steady_temp_grid(grid, model->config->grid_steady_file);
 
     fclose(pin);
     if (tout) fclose(tout);
-    if (!grid->has_lcf) free_flp(flp, 0, 0);
     delete_RC_model(model);
+    if (flp) free_flp(flp, 0, 0);
     if (temp) free_dvector(temp);
     free_materials(&materials);
     free_dvector(power);

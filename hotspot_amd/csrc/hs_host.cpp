@@ -51,6 +51,7 @@ void report(const std::string &msg) {
 
 Model *model_of(RC_model_t *m) {
     if (!m) fatal("null model\n");
+    if (m->type == BLOCK_MODEL) fatal("this call needs a grid model (model_type grid)\n");
     if (m->type != GRID_MODEL) fatal("unknown model type\n");
     return reinterpret_cast<Handle *>(m)->impl;
 }
@@ -64,7 +65,6 @@ Model *model_of(grid_model_t *g) {
 // ---------------------------------------------------------------------------
 // small numerics (util.c:17-20, :57-77; RCutil.c:20-31)
 // ---------------------------------------------------------------------------
-static inline bool eq(double x, double y) { return std::fabs(x - y) < kDelta; }
 static int tolerant_ceil(double v) {
     double nearest = std::floor(v + 0.5);
     return eq(v, nearest) ? (int)nearest : (int)std::ceil(v);
@@ -73,14 +73,12 @@ static int tolerant_floor(double v) {
     double nearest = std::floor(v + 0.5);
     return eq(v, nearest) ? (int)nearest : (int)std::floor(v);
 }
-static inline double getr(double k, double t, double area) { return t / (k * area); }
-static inline double getcap(double sp, double t, double area) { return kCFactor * sp * t * area; }
 
 // ---------------------------------------------------------------------------
 // text input helpers
 // ---------------------------------------------------------------------------
 // First whitespace-delimited token of a line, "" for blank/comment lines.
-static bool first_token(const char *line, std::string &tok) {
+bool first_token(const char *line, std::string &tok) {
     const char *p = line;
     while (*p && strchr(" \r\t\n", *p)) p++;
     if (!*p || *p == '#') return false;
@@ -90,7 +88,7 @@ static bool first_token(const char *line, std::string &tok) {
     return true;
 }
 
-static std::vector<std::string> read_lines(const char *file) {
+std::vector<std::string> read_lines(const char *file) {
     FILE *fp = strcasecmp(file, "stdin") ? fopen(file, "r") : stdin;
     if (!fp) fatal(std::string("error opening file ") + file + "\n");
     std::vector<std::string> out;
@@ -193,7 +191,7 @@ static flp_t *flp_from_lines(const std::vector<std::string> &lines, int read_con
     return flp;
 }
 
-static double total_width(const flp_t *f) {
+double total_width(const flp_t *f) {
     double mn = f->units[0].leftx, mx = f->units[0].leftx + f->units[0].width;
     for (int i = 1; i < f->n_units; i++) {
         mn = std::min(mn, f->units[i].leftx);
@@ -201,7 +199,7 @@ static double total_width(const flp_t *f) {
     }
     return mx - mn;
 }
-static double total_height(const flp_t *f) {
+double total_height(const flp_t *f) {
     double mn = f->units[0].bottomy, mx = f->units[0].bottomy + f->units[0].height;
     for (int i = 1; i < f->n_units; i++) {
         mn = std::min(mn, f->units[i].bottomy);
@@ -241,7 +239,7 @@ constexpr double K_SOLDER = 1.0 / 0.06, K_PCB = 1.0 / 0.333;
 constexpr double SPEC_HEAT_METAL = 3.55e6, SPEC_HEAT_C4 = 1.65e6, SPEC_HEAT_SUB = 1.6e6;
 constexpr double SPEC_HEAT_SOLDER = 2.1e6, SPEC_HEAT_PCB = 1.32e6;
 
-static void package_R(package_RC_t &p, const thermal_config_t &c, double w, double h) {
+void package_R(package_RC_t &p, const thermal_config_t &c, double w, double h) {
     const double ssp = c.s_spreader, tsp = c.t_spreader, shs = c.s_sink, ths = c.t_sink;
     const double khs = c.k_sink, ksp = c.k_spreader, rcv = c.r_convec;
     const double ssub = c.s_sub, tsub = c.t_sub, ssol = c.s_solder, tsol = c.t_solder;
@@ -287,7 +285,7 @@ static void package_R(package_RC_t &p, const thermal_config_t &c, double w, doub
     p.r_amb_sec_per = rsec * (spcb * spcb) / ((spcb * spcb - ssol * ssol) / 4.0);
 }
 
-static void package_C(package_RC_t &p, const thermal_config_t &c, double w, double h) {
+void package_C(package_RC_t &p, const thermal_config_t &c, double w, double h) {
     const double ssp = c.s_spreader, tsp = c.t_spreader, shs = c.s_sink, ths = c.t_sink;
     const double ccv = c.c_convec, phs = c.p_sink, psp = c.p_spreader;
     const double ssub = c.s_sub, tsub = c.t_sub, ssol = c.s_solder, tsol = c.t_solder;
@@ -1061,8 +1059,8 @@ RC_model_t *alloc_RC_model(thermal_config_t *config, flp_t *placeholder,
     Handle *h = nullptr;
     Model *M = nullptr;
     HS_BEGIN
-    if (strcasecmp(config->model_type, "grid"))
-        fatal("this library implements the grid model only (model_type grid)\n");
+    if (!strcasecmp(config->model_type, "block")) return block_rc_alloc(config, placeholder);
+    if (strcasecmp(config->model_type, "grid")) fatal("unknown model type\n");
     if (config->grid_rows & (config->grid_rows - 1) || config->grid_cols & (config->grid_cols - 1) ||
         config->grid_rows <= 0 || config->grid_cols <= 0)
         fatal("grid rows and columns should both be powers of two\n");
@@ -1117,6 +1115,10 @@ RC_model_t *alloc_RC_model(thermal_config_t *config, flp_t *placeholder,
 
 void delete_RC_model(RC_model_t *model) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) {
+        delete_block_model(model->block);
+        return;
+    }
     if (!model) return;
     Model *M = model_of(model);
     delete_model(M);
@@ -1126,6 +1128,10 @@ void delete_RC_model(RC_model_t *model) {
 
 void populate_R_model(RC_model_t *model, flp_t *flp) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) {
+        populate_R_model_block(model->block, flp);
+        return;
+    }
     Model &M = *model_of(model);
     grid_model_t &g = *M.g;
     const int nl = g.n_layers;
@@ -1165,6 +1171,10 @@ void populate_R_model(RC_model_t *model, flp_t *flp) {
 
 void populate_C_model(RC_model_t *model, flp_t *flp) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) {
+        populate_C_model_block(model->block, flp);
+        return;
+    }
     Model &M = *model_of(model);
     grid_model_t &g = *M.g;
     if (!g.r_ready) fatal("R model not ready\n");
@@ -1185,6 +1195,7 @@ void populate_C_model(RC_model_t *model, flp_t *flp) {
 // ---- vectors (temperature_grid.c:1182-1259, :1449-1455) --------------------
 double *hotspot_vector(RC_model_t *model) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) return hotspot_vector_block(model->block);
     Model &M = *model_of(model);
     if (M.g->total_n_blocks <= 0) fatal("total_n_blocks is not greater than zero\n");
     double *v = (double *)calloc(total_nodes(M), sizeof(double));
@@ -1197,6 +1208,10 @@ void free_dvector(double *v) { free(v); }
 
 void set_temp(RC_model_t *model, double *temp, double val) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) {
+        set_temp_block(model->block, temp, val);
+        return;
+    }
     Model &M = *model_of(model);
     for (int i = 0; i < total_nodes(M); i++) temp[i] = val;
     HS_END()
@@ -1204,6 +1219,10 @@ void set_temp(RC_model_t *model, double *temp, double val) {
 
 void copy_temp(RC_model_t *model, double *dst, double *src) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) {
+        copy_temp_block(model->block, dst, src);
+        return;
+    }
     Model &M = *model_of(model);
     memcpy(dst, src, sizeof(double) * total_nodes(M));
     HS_END()
@@ -1211,6 +1230,10 @@ void copy_temp(RC_model_t *model, double *dst, double *src) {
 
 void trim_hotspot_vector(RC_model_t *model, double *dst, double *src, int at, int size) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) {
+        trim_hotspot_vector_block(model->block, dst, src, at, size);
+        return;
+    }
     Model &M = *model_of(model);
     if (M.g->has_lcf) fatal("trim_hotspot_vector_grid called with lcf file\n");
     const int tn = total_nodes(M);
@@ -1221,6 +1244,10 @@ void trim_hotspot_vector(RC_model_t *model, double *dst, double *src, int at, in
 
 void resize_thermal_model(RC_model_t *model, int n_units) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) {
+        resize_thermal_model_block(model->block, n_units);
+        return;
+    }
     Model &M = *model_of(model);
     if (M.g->has_lcf) fatal("resize_thermal_model_grid called with lcf file\n");
     if (n_units > M.g->base_n_units) fatal("resizing grid model to more than the allocated space\n");
@@ -1233,6 +1260,10 @@ void resize_thermal_model(RC_model_t *model, int n_units) {
 // ---- text formats (temperature_grid.c:1262-1764) ---------------------------
 void dump_temp(RC_model_t *model, double *temp, char *file) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) {
+        dump_temp_block(model->block, temp, file);
+        return;
+    }
     Model &M = *model_of(model);
     FILE *fp = !strcasecmp(file, "stdout") ? stdout : !strcasecmp(file, "stderr") ? stderr : fopen(file, "w");
     if (!fp) fatal(std::string("error: ") + file + " could not be opened for writing\n");
@@ -1251,6 +1282,10 @@ void dump_temp(RC_model_t *model, double *temp, char *file) {
 
 void read_temp(RC_model_t *model, double *temp, char *file, int clip) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) {
+        read_temp_block(model->block, temp, file, clip);
+        return;
+    }
     Model &M = *model_of(model);
     grid_model_t &g = *M.g;
     auto lines = read_lines(file);
@@ -1307,6 +1342,10 @@ void read_temp(RC_model_t *model, double *temp, char *file, int clip) {
 
 void dump_power(RC_model_t *model, double *power, char *file) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) {
+        dump_power_block(model->block, power, file);
+        return;
+    }
     Model &M = *model_of(model);
     FILE *fp = !strcasecmp(file, "stdout") ? stdout : !strcasecmp(file, "stderr") ? stderr : fopen(file, "w");
     if (!fp) fatal(std::string("error: ") + file + " could not be opened for writing\n");
@@ -1326,6 +1365,10 @@ void dump_power(RC_model_t *model, double *power, char *file) {
 
 void read_power(RC_model_t *model, double *power, char *file) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) {
+        read_power_block(model->block, power, file);
+        return;
+    }
     Model &M = *model_of(model);
     auto lines = read_lines(file);
     char name[STR_SIZE];
@@ -1369,6 +1412,7 @@ void read_power(RC_model_t *model, double *power, char *file) {
 
 double find_max_temp(RC_model_t *model, double *temp) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) return find_max_temp_block(model->block, temp);
     Model &M = *model_of(model);
     int si = !M.g->config.model_secondary ? 0 : (M.g->has_lcf ? 3 : 5);
     double mx = 0.0;
@@ -1383,6 +1427,7 @@ double find_max_temp(RC_model_t *model, double *temp) {
 
 double find_avg_temp(RC_model_t *model, double *temp) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) return find_avg_temp_block(model->block, temp);
     Model &M = *model_of(model);
     int base = 0, count = 0;
     double sum = 0.0;
@@ -1404,6 +1449,7 @@ double find_avg_temp(RC_model_t *model, double *temp) {
 
 double calc_sink_temp(RC_model_t *model, double *temp) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) return calc_sink_temp_block(model->block, temp, model->config);
     Model &M = *model_of(model);
     grid_model_t &g = *M.g;
     const thermal_config_t &c = g.config;
@@ -1485,8 +1531,39 @@ static void steady_grid(Model &M, double *power, double *temp) {
     M.mirror_steady = 0;
 }
 
+// temperature-leakage fixed point of the block model (temperature.c:670-705)
+static void steady_block(block_model_t *b, double *power, double *temp) {
+    const thermal_config_t &c = b->config;
+    if (!c.leakage_used) {
+        block_steady(b, power, temp);
+        return;
+    }
+    const int n = b->flp->n_units;
+    std::vector<double> temp_old(b->n_nodes, 0.0), power_new(b->n_nodes, 0.0);
+    bool conv = false;
+    for (int it = 0; !conv && it <= 100; it++) {
+        for (int i = 0; i < n; i++) {
+            const unit_t &u = b->flp->units[i];
+            power_new[i] = power[i] + calc_leakage(c.leakage_mode, u.height, u.width, temp[i]);
+            temp_old[i] = temp[i];
+        }
+        block_steady(b, power_new.data(), temp);
+        double dmax = 0.0;
+        for (int i = 0; i < n; i++) dmax = std::max(dmax, temp[i] - temp_old[i]);
+        if (dmax < 0.01) conv = true;
+        if (dmax > 500.0 && it > 1)
+            fatal("temperature is too high, possible thermal runaway. Double-check power inputs and package settings.\n");
+    }
+    if (!conv)
+        fatal("too many iterations before temperature-leakage convergence -- possible thermal runaway\n");
+}
+
 void steady_state_temp(RC_model_t *model, double *power, double *temp) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) {
+        steady_block(model->block, power, temp);
+        return;
+    }
     Model &M = *model_of(model);
     grid_model_t &g = *M.g;
     if (!g.config.leakage_used) {
@@ -1541,6 +1618,10 @@ void steady_state_temp(RC_model_t *model, double *power, double *temp) {
 
 void compute_temp(RC_model_t *model, double *power, double *temp, double time_elapsed) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) {
+        compute_temp_block(model->block, power, temp, time_elapsed);
+        return;
+    }
     Model &M = *model_of(model);
     grid_model_t &g = *M.g;
     if (!g.r_ready || !g.c_ready) fatal("grid model not ready\n");
@@ -1576,6 +1657,10 @@ void hs_set_options(RC_model_t *model, const hs_options_t *opt) {
 
 void hs_get_stats(RC_model_t *model, hs_stats_t *stats) {
     HS_BEGIN
+    if (model && model->type == BLOCK_MODEL) {
+        *stats = block_of(model->block)->stats;
+        return;
+    }
     *stats = model_of(model)->stats;
     HS_END()
 }

@@ -55,6 +55,19 @@ void report(const std::string &msg);
         return ret;                                   \
     }
 
+// ---- small numerics and host helpers (util.c:17-20, RCutil.c:20-31; hs_host.cpp)
+inline bool eq(double x, double y) { return (x - y < 0 ? y - x : x - y) < kDelta; }
+inline double getr(double k, double t, double area) { return t / (k * area); }
+inline double getcap(double sp, double t, double area) { return kCFactor * sp * t * area; }
+// first whitespace-delimited token of a line; false for blank/comment lines
+bool first_token(const char *line, std::string &tok);
+std::vector<std::string> read_lines(const char *file);
+double total_width(const flp_t *f);    // get_total_width, flp.c:1501-1515
+double total_height(const flp_t *f);   // get_total_height, flp.c:1517-1531
+// populate_package_R / populate_package_C (temperature.c:458-569)
+void package_R(package_RC_t &p, const thermal_config_t &c, double w, double h);
+void package_C(package_RC_t &p, const thermal_config_t &c, double w, double h);
+
 // ---- block<->grid map of one distinct floorplan ---------------------------
 struct BlockMap {
     // per cell (row-major i*nc+j): the blocks overlapping it, in block order
@@ -131,6 +144,38 @@ void dev_set_grid(Model &M, int which, const double *in);
 void dev_slope(Model &M, const double *block_power, const double *grid_temp, double *out);
 void dev_b2g(Model &M, const double *block_vec, double *grid_vec, int type);
 void dev_g2b(Model &M, const double *grid_vec, double *block_vec);
+
+// ---- block model (hs_block.cpp host side, hs_block.hip device side) -------
+struct BlockDev;  // device state, defined in hs_block.hip
+
+struct BlockModel {
+    block_model_t *m = nullptr;       // the public struct
+    BlockDev *dev = nullptr;
+    bool lu_stale = true;             // b changed since the device factorisation
+    bool c_stale = true;              // c / inva changed since their upload
+    hs_stats_t stats{};
+};
+
+// handle layout: alloc_RC_model returns .rc, alloc_block_model returns .pub
+struct BlockHandle {
+    RC_model_t rc;
+    block_model_t pub;
+    BlockModel *impl;
+};
+BlockModel *block_of(block_model_t *b);
+RC_model_t *block_rc_alloc(thermal_config_t *config, flp_t *placeholder);
+// steady_state_temp_block without the C-boundary error conversion
+void block_steady(block_model_t *model, double *power, double *temp);
+
+// device entry points (hs_block.hip)
+bool blk_dev_present();                    // a GPU is visible
+void blk_dev_release(BlockModel &B);
+// LU factors of b on the device (lupdcmp); host copies into lu and p
+void blk_dev_factor(BlockModel &B);
+// B T = P by the LU factors (refactorised on the device when b changed)
+void blk_dev_steady(BlockModel &B, const double *power, double *temp);
+// compute_temp_block's adaptive RK4 over dt, temp in and out
+void blk_dev_transient(BlockModel &B, const double *power, double *temp, double dt);
 
 // batched engine (hs_batch.inl)
 struct Engine;

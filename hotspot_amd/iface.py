@@ -181,10 +181,41 @@ class GridModel(C.Structure):
     ]
 
 
+class BlockModel(C.Structure):
+    """block_model_t (temperature_block.h:16-67)."""
+    _fields_ = [
+        ("flp", C.POINTER(Flp)), ("config", ThermalConfig),
+        ("b", C.POINTER(c_double_p)), ("lu", C.POINTER(c_double_p)), ("p", C.POINTER(C.c_int)),
+        ("a", c_double_p), ("inva", c_double_p), ("c", C.POINTER(c_double_p)),
+        ("pack", PackageRC),
+        ("gx", c_double_p), ("gy", c_double_p), ("gx_int", c_double_p), ("gy_int", c_double_p),
+        ("gx_sp", c_double_p), ("gy_sp", c_double_p), ("gx_hs", c_double_p), ("gy_hs", c_double_p),
+        ("g_amb", c_double_p), ("t_vector", c_double_p),
+        ("len", C.POINTER(c_double_p)), ("g", C.POINTER(c_double_p)),
+        ("border", C.POINTER(C.POINTER(C.c_int))),
+        ("n_nodes", C.c_int), ("n_units", C.c_int), ("base_n_units", C.c_int),
+        ("r_ready", C.c_int), ("c_ready", C.c_int),
+    ]
+
+
 class RCModel(C.Structure):
-    """RC_model_t (temperature.h:347-357); the union holds a grid_model_t* here."""
+    """RC_model_t (temperature.h:347-357); the union holds a grid_model_t* or,
+    for type BLOCK_MODEL (0), a block_model_t* (see ``block_of``)."""
     _fields_ = [("grid", C.POINTER(GridModel)), ("type", C.c_int),
                 ("config", C.POINTER(ThermalConfig))]
+
+
+BLOCK_MODEL, GRID_MODEL = 0, 1
+
+
+def block_of(model) -> BlockModel:
+    """The block_model_t of an RC_model_t of type BLOCK_MODEL."""
+    return C.cast(model.contents.grid, C.POINTER(BlockModel)).contents
+
+
+def block_matrix(ptr, n: int) -> np.ndarray:
+    """Copy of an n x n dmatrix (row pointers into one contiguous block)."""
+    return np.ctypeslib.as_array(ptr[0], shape=(n * n,)).reshape(n, n).copy()
 
 
 RCModelP = C.POINTER(RCModel)
@@ -299,6 +330,8 @@ class HotSpotLib:
         return cfg
 
     def n_vector(self, model: RCModelP) -> int:
+        if model.contents.type == BLOCK_MODEL:
+            return block_of(model).n_nodes
         g = model.contents.grid.contents
         extra = EXTRA + (EXTRA_SEC if g.config.model_secondary else 0)
         return g.total_n_blocks + extra

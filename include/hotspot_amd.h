@@ -1,7 +1,7 @@
 /*
  * hotspot_amd.h -- C ABI of the MI355X grid thermal model.
  *
- * Drop-in for HotSpot's grid path: the entry points keep the names,
+ * Drop-in for HotSpot's thermal models (grid and block): the entry points keep the names,
  * signatures and argument meaning of the reference library, and the structs
  * keep the reference *implementation* layouts (temperature.h, flp.h,
  * temperature_grid.h, microchannel.h, materials.h), so code compiled against
@@ -192,6 +192,47 @@ typedef struct grid_model_t_st {
     microchannel_config_t *default_microchannel_config;
 } grid_model_t;
 
+/* ---- block model: temperature_block.h:5-67 ------------------------------
+ * Four node layers per floorplan unit (silicon, interface, spreader, sink),
+ * then the 12 package nodes: node ids layer*n_units + unit, NL*n_units + k. */
+#ifndef NL
+#define NL 4
+#endif
+#ifndef IFACE
+#define IFACE 1
+#endif
+#ifndef HSP
+#define HSP 2
+#endif
+#ifndef HSINK
+#define HSINK 3
+#endif
+
+typedef struct block_model_t_st {
+    flp_t *flp;
+    thermal_config_t config;
+    double **b;          /* conductance matrix, n_nodes x n_nodes */
+    double **lu;         /* its LU factors (device-computed copy) */
+    int *p;              /* row permutation of the factorisation */
+    double *a;           /* node capacitances */
+    double *inva;        /* 1 / a */
+    double **c;          /* inva * b */
+    package_RC_t pack;
+    double *gx, *gy;
+    double *gx_int, *gy_int;
+    double *gx_sp, *gy_sp;
+    double *gx_hs, *gy_hs;
+    double *g_amb;
+    double *t_vector;
+    double **len, **g;
+    int **border;
+    int n_nodes;
+    int n_units;
+    int base_n_units;
+    int r_ready;
+    int c_ready;
+} block_model_t;
+
 typedef struct RC_model_t_st {
     union {
         struct block_model_t_st *block;
@@ -218,7 +259,7 @@ void default_materials(materials_list_t *materials_list);
 void materials_add_from_file(materials_list_t *materials_list, char *materials_filename);
 /* materials.c:152-161 */
 void free_materials(materials_list_t *materials_list);
-/* temperature.c:616-633 (grid model only: model_type must be "grid") */
+/* temperature.c:616-633 (model_type "grid" or "block") */
 RC_model_t *alloc_RC_model(thermal_config_t *config, flp_t *placeholder,
                            microchannel_config_t *microchannel_config,
                            materials_list_t *materials_list, int do_detailed_3D,
@@ -263,6 +304,46 @@ void resize_thermal_model(RC_model_t *model, int n_units);
 double calc_leakage(int mode, double h, double w, double temp);
 /* temperature_grid.c:1262-1295 */
 void dump_steady_temp_grid(grid_model_t *model, char *file);
+
+/* ---- block model (temperature_block.c), also reached through the RC_model_t
+ * entry points above when model_type is "block".  The factorisation, the
+ * steady solve and the transient integration run on the device. */
+/* temperature_block.c:19-56 */
+block_model_t *alloc_block_model(thermal_config_t *config, flp_t *placeholder);
+/* temperature_block.c:833-856 */
+void delete_block_model(block_model_t *model);
+/* temperature_block.c:64-296 */
+void populate_R_model_block(block_model_t *model, flp_t *flp);
+/* temperature_block.c:303-376 */
+void populate_C_model_block(block_model_t *model, flp_t *flp);
+/* temperature_block.c:398-411 */
+void steady_state_temp_block(block_model_t *model, double *power, double *temp);
+/* temperature_block.c:449-486 */
+void compute_temp_block(block_model_t *model, double *power, double *temp, double time_elapsed);
+/* temperature_block.c:489-492 */
+double *hotspot_vector_block(block_model_t *model);
+/* temperature_block.c:498-507 */
+void trim_hotspot_vector_block(block_model_t *model, double *dst, double *src, int at, int size);
+/* temperature_block.c:510-522 */
+void resize_thermal_model_block(block_model_t *model, int n_units);
+/* temperature_block.c:525-530 */
+void set_temp_block(block_model_t *model, double *temp, double val);
+/* temperature_block.c:533-575 */
+void dump_temp_block(block_model_t *model, double *temp, char *file);
+/* temperature_block.c:577-580 */
+void copy_temp_block(block_model_t *model, double *dst, double *src);
+/* temperature_block.c:587-695 */
+void read_temp_block(block_model_t *model, double *temp, char *file, int clip);
+/* temperature_block.c:698-720 */
+void dump_power_block(block_model_t *model, double *power, char *file);
+/* temperature_block.c:726-764 */
+void read_power_block(block_model_t *model, double *power, char *file);
+/* temperature_block.c:766-778 */
+double find_max_temp_block(block_model_t *model, double *temp);
+/* temperature_block.c:780-792 */
+double find_avg_temp_block(block_model_t *model, double *temp);
+/* temperature_block.c:795-831 */
+double calc_sink_temp_block(block_model_t *model, double *temp, thermal_config_t *config);
 /* temperature_grid.c:1297-1319 */
 void dump_transient_temp_grid(grid_model_t *model, int trace_num, double sampling_intvl,
                               char *filename);

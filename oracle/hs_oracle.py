@@ -941,6 +941,199 @@ class OracleModel:
         return y, calls
 
 
+# --------------------------------------------------------------------------
+# the block model (temperature_block.c)
+# --------------------------------------------------------------------------
+def _touch(a1, a2, b1, b2):
+    """The closed intervals [a1, a2] and [b1, b2] meet (flp.c:1431-1433)."""
+    return (a1 <= b1 <= a2) or (a1 <= b2 <= a2) or (b1 <= a1 <= b2) or (b1 <= a2 <= b2)
+
+
+def unit_neighbours(u, v):
+    """How two floorplan units border each other (flp.c:1404-1499): returns
+    ("x", shared length) for units side by side (a common vertical edge),
+    ("y", shared length) for one above the other, or (None, 0).  Units that
+    touch at a corner only are not neighbours.  All comparisons of edges use
+    the floorplan tolerance eq() (1e-6 m)."""
+    x1, x2, y1, y2 = u.leftx, u.leftx + u.width, u.bottomy, u.bottomy + u.height
+    x3, x4, y3, y4 = v.leftx, v.leftx + v.width, v.bottomy, v.bottomy + v.height
+    corner = ((eq(x2, x3) and eq(y2, y3)) or (eq(x1, x4) and eq(y1, y4)) or
+              (eq(x2, x3) and eq(y1, y4)) or (eq(x1, x4) and eq(y2, y3)))
+    if corner:
+        return None, 0.0
+    side = (eq(x1, x4) or eq(x2, x3)) and _touch(y1, y2, y3, y4)
+    above = (eq(y1, y4) or eq(y2, y3)) and _touch(x1, x2, x3, x4)
+    if above:       # get_shared_len takes the vertical case last
+        return ("x" if side else "y"), min(x2, x4) - max(x1, x3)
+    if side:
+        return "x", min(y2, y4) - max(y1, y3)
+    return None, 0.0
+
+
+class OracleBlockModel:
+    """HotSpot's block model (temperature_block.c) as a dense linear system.
+
+    Nodes: four per floorplan unit, silicon/interface/spreader/sink (node
+    layer*n + unit), then the 12 package nodes at 4n + k (SP_*, SINK_C_*,
+    SINK_*, temperature.h:32-77).  Every link is a conductance between two
+    nodes; the reference stores each link as two half-link conductances and
+    combines them in series (populate_R_model_block, temperature_block.c:
+    64-296).  Here each link's conductance is written directly:
+
+      lateral, layer l, units i|j sharing an edge of length L:
+          g_i = L * k_l t_l h_i / (w_i / 2) / h_i   (side by side, per unit
+          width and height), series of g_i and g_j
+      vertical, unit i: 1/r with r = t / (k A) of the upper layer
+      spreader (sink) unit on the chip's north edge to SP_N (SINK_C_N):
+          1 / (1/g_y + r_sp1_y g_N / g_y), g_N = sum of the edge units' g_y
+          (likewise S with g_y, E and W with g_x)
+      SP_d - SINK_C_d: 1/r_sp_per; SINK_C_d - SINK_d: 1/(r_hs + r_hs2)
+      ambient: sink unit 1/(r_sink + r_convec s_sink^2 / A), SINK_C_d
+          1/(r_hs_c_per + r_amb_c_per), SINK_d 1/(r_hs_per + r_amb_per)
+
+    B = Laplacian of the links + diag(ambient conductances); the ambient
+    enters as power amb * g_amb at the sink and package nodes
+    (set_internal_power_block :382-389).  Steady: B T = P (a dense solve, the
+    reference factorises B once, RCutil.c:42-183).  Transient: dT/dt =
+    A^-1 P - (A^-1 B) T with the reference's adaptive RK4 (the same
+    controller as the grid model, RCutil.c:440-640, compute_temp_block
+    :449-486); A holds the node capacitances (populate_C_model_block
+    :303-376).
+
+    Pinned in tests/test_block_model.py against the reference library built
+    from its sources: B, A and the package R/C to 1e-13 relative, steady and
+    transient temperatures to 1e-9 K, plus the invariants (B symmetric, row
+    sums equal to the ambient conductances, energy balance).
+    """
+
+    def __init__(self, config: dict | None = None, flp: str | None = None, units=None):
+        cfg = dict(DEFAULT_CONFIG)
+        cfg.update(dict(r_convec_sec=1.0, c_convec_sec=140.4, block_omit_lateral=0))
+        cfg.update(config or {})
+        self.cfg = cfg
+        self.units = units if units is not None else read_flp(flp)
+        self.n_units = n = len(self.units)
+        self.N = 4 * n + EXTRA
+        self.width, self.height = total_width(self.units), total_height(self.units)
+        self._assemble_B()
+        self._assemble_A()
+
+    def _assemble_B(self):
+        cfg, U, n, N = self.cfg, self.units, self.n_units, self.N
+        w, h = self.width, self.height
+        if w > cfg["s_sink"] or h > cfg["s_sink"] or w > cfg["s_spreader"] or h > cfg["s_spreader"]:
+            raise ValueError("inordinate floorplan size!")
+        layers = [(cfg["k_chip"], cfg["t_chip"]), (cfg["k_interface"], cfg["t_interface"]),
+                  (cfg["k_spreader"], cfg["t_spreader"]), (cfg["k_sink"], cfg["t_sink"])]
+        # centre-to-edge lateral conductance of every unit per layer:
+        # across half the width (gx) and half the height (gy)
+        gx = np.zeros((4, n))
+        gy = np.zeros((4, n))
+        for l, (k, t) in enumerate(layers):
+            if l == 0 and cfg["block_omit_lateral"]:
+                continue
+            for i, u in enumerate(U):
+                gx[l, i] = 1.0 / getr(k, u.width / 2.0, u.height * t)
+                gy[l, i] = 1.0 / getr(k, u.height / 2.0, u.width * t)
+        self.gx, self.gy = gx, gy
+        pk = package_R(cfg, w, h)
+        self.pack_R = pk
+        B = np.zeros((N, N))
+
+        def link(a, b, g):
+            if g == 0.0:
+                return
+            B[a, b] -= g
+            B[b, a] -= g
+            B[a, a] += g
+            B[b, b] += g
+
+        for i in range(n):
+            for j in range(i + 1, n):
+                kind, L = unit_neighbours(U[i], U[j])
+                if kind is None or L == 0.0:
+                    continue
+                for l in range(4):
+                    if kind == "x":
+                        gi, gj = gx[l, i] / U[i].height * L, gx[l, j] / U[j].height * L
+                    else:
+                        gi, gj = gy[l, i] / U[i].width * L, gy[l, j] / U[j].width * L
+                    if gi != 0.0 and gj != 0.0:
+                        link(l * n + i, l * n + j, 1.0 / (1.0 / gi + 1.0 / gj))
+        X = 4 * n
+        gamb = np.zeros(N)
+        # edge membership (west, east, north, south) and the edge sums
+        edge = np.zeros((n, 4), dtype=bool)
+        for i, u in enumerate(U):
+            edge[i] = [eq(u.leftx, 0), eq(u.leftx + u.width, w), eq(u.bottomy + u.height, h), eq(u.bottomy, 0)]
+        self.border = edge.astype(int)
+        sides = [(SP_W, SINK_C_W, "x"), (SP_E, SINK_C_E, "x"), (SP_N, SINK_C_N, "y"), (SP_S, SINK_C_S, "y")]
+        for i, u in enumerate(U):
+            area = u.width * u.height
+            link(i, n + i, 1.0 / getr(cfg["k_chip"], cfg["t_chip"], area))
+            link(n + i, 2 * n + i, 1.0 / getr(cfg["k_interface"], cfg["t_interface"], area))
+            link(2 * n + i, 3 * n + i, 1.0 / getr(cfg["k_spreader"], cfg["t_spreader"], area))
+            gamb[3 * n + i] = 1.0 / (getr(cfg["k_sink"], cfg["t_sink"], area)
+                                     + cfg["r_convec"] * cfg["s_sink"] ** 2 / area)
+            for e, (sp_node, hs_node, d) in enumerate(sides):
+                if not edge[i, e]:
+                    continue
+                gl = gx if d == "x" else gy
+                r_sp1 = pk["r_sp1_" + d]
+                r_hs1 = pk["r_hs1_" + d]
+                g_sp_edge = gl[2, edge[:, e]].sum()
+                g_hs_edge = gl[3, edge[:, e]].sum()
+                link(2 * n + i, X + sp_node, 1.0 / (1.0 / gl[2, i] + r_sp1 * g_sp_edge / gl[2, i]))
+                link(3 * n + i, X + hs_node, 1.0 / (1.0 / gl[3, i] + r_hs1 * g_hs_edge / gl[3, i]))
+        for sp_node, hsc, hso, d in ((SP_N, SINK_C_N, SINK_N, "y"), (SP_S, SINK_C_S, SINK_S, "y"),
+                                     (SP_E, SINK_C_E, SINK_E, "x"), (SP_W, SINK_C_W, SINK_W, "x")):
+            link(X + sp_node, X + hsc, 1.0 / pk["r_sp_per_" + d])
+            link(X + hsc, X + hso, 1.0 / (pk["r_hs"] + pk["r_hs2_" + d]))
+            gamb[X + hsc] = 1.0 / (pk["r_hs_c_per_" + d] + pk["r_amb_c_per_" + d])
+            gamb[X + hso] = 1.0 / (pk["r_hs_per"] + pk["r_amb_per"])
+        B += np.diag(gamb)
+        self.B, self.gamb = B, gamb
+
+    def _assemble_A(self):
+        cfg, U, n, N = self.cfg, self.units, self.n_units, self.N
+        pk = package_C(cfg, self.width, self.height)
+        self.pack_C = pk
+        a = np.zeros(N)
+        for i, u in enumerate(U):
+            area = u.width * u.height
+            a[i] = getcap(cfg["p_chip"], cfg["t_chip"], area)
+            a[n + i] = getcap(cfg["p_interface"], cfg["t_interface"], area)
+            a[2 * n + i] = getcap(cfg["p_spreader"], cfg["t_spreader"], area)
+            a[3 * n + i] = (getcap(cfg["p_sink"], cfg["t_sink"], area)
+                            + C_FACTOR * cfg["c_convec"] / cfg["s_sink"] ** 2 * area)
+        a[4 * n:] = package_node_caps(pk, False)
+        self.A = a
+        self.Cm = (1.0 / a)[:, None] * self.B
+
+    def node_power(self, unit_power):
+        """Node power vector: the units' power on the silicon nodes, the
+        ambient's injection amb * g_amb on sink and package nodes."""
+        P = self.cfg["ambient"] * self.gamb
+        P[:self.n_units] = unit_power[:self.n_units]
+        return P
+
+    def steady(self, unit_power):
+        return np.linalg.solve(self.B, self.node_power(unit_power))
+
+    def slope(self, T, P):
+        """dT/dt = A^-1 P - C T  (slope_fn_block, temperature_block.c:416-446)."""
+        return P / self.A - self.Cm @ T
+
+    # the adaptive RK4 and the interval loop are the grid model's, over this slope
+    rk4_core = OracleModel.rk4_core
+    rk4 = OracleModel.rk4
+    advance = OracleModel.advance
+
+    def transient(self, unit_power, T, dt):
+        """compute_temp_block: returns (T after dt, RK4 calls)."""
+        return self.advance(np.asarray(T, dtype=float), self.node_power(unit_power), dt)
+
+
 def package_R(cfg, width, height):
     """populate_package_R (temperature.c:458-523)."""
     s_sp, t_sp, s_hs, t_hs = cfg["s_spreader"], cfg["t_spreader"], cfg["s_sink"], cfg["t_sink"]

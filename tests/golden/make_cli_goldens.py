@@ -1,5 +1,5 @@
 """Writes the command-line golden files under tests/golden/cli/: the inputs of
-the reference's examples 2-5 (floorplans, layer files, power traces, configs,
+the reference's examples 1-4 (floorplans, layer files, power traces, configs,
 materials: data files) and the outputs of the reference's own command-line
 simulator on them, run with the command lines of the examples' run.sh.
 
@@ -24,8 +24,29 @@ EXE = os.path.join(ROOT, "oracle", "_ref", "hotspot_ref")
 OUT = os.path.join(HERE, "cli")
 
 # case -> (example dir, input files, runs); a run is (argv, outputs to keep).
-# "@init" in argv names the steady file of the case's first run as input.
+# "@init" in argv names the steady file of the case's first run as input; an
+# output named "!name" marks a run that must fail, and keeps its error line.
 CASES = {
+    "ev6_block": ("example1", ["ev6.flp", "gcc.ptrace", "example.config", "example.materials",
+                               "package.config", "@natural.config"], [
+        ("-c example.config -f ev6.flp -p gcc.ptrace -materials_file example.materials -model_type block "
+         "-steady_file gcc.steady -o first.ttrace", ["gcc.steady", "first.ttrace"]),
+        ("-c example.config -init_file @init -f ev6.flp -p gcc.ptrace -materials_file example.materials "
+         "-model_type block -o gcc.ttrace", ["gcc.ttrace"]),
+        ("-c example.config -f ev6.flp -p gcc.ptrace -materials_file example.materials -model_type block "
+         "-package_model_used 1 -package_config_file natural.config -steady_file natural.steady",
+         ["natural.steady"]),
+        # the transient starts at ambient (init_temp = ambient in example 1): a
+        # natural-convection sink at ambient has no convection, B loses its
+        # ground and the LU stops at a pivot below 1e-6 in the first interval
+        ("-c example.config -f ev6.flp -p gcc.ptrace -materials_file example.materials -model_type block "
+         "-package_model_used 1 -package_config_file natural.config -o natural.ttrace", ["!natural.err"]),
+        ("-c example.config -f ev6.flp -p gcc.ptrace -materials_file example.materials -model_type block "
+         "-package_model_used 1 -package_config_file package.config -steady_file forced.steady",
+         ["forced.steady"]),
+        ("-c example.config -f ev6.flp -p gcc.ptrace -materials_file example.materials -model_type block "
+         "-leakage_used 1 -steady_file leak.steady", ["leak.steady"]),
+    ]),
     "ev6": ("example2", ["ev6.flp", "gcc.ptrace", "example.config", "example.materials"], [
         ("-c example.config -f ev6.flp -p gcc.ptrace -materials_file example.materials -model_type grid "
          "-steady_file gcc.steady -grid_steady_file gcc.grid.steady", ["gcc.steady", "gcc.grid.steady"]),
@@ -94,6 +115,13 @@ def run_case(name, spec):
             if "@init" in args:
                 args[args.index("@init")] = init
             print(name, " ".join(args), flush=True)
+            if keep and keep[0].startswith("!"):   # a run that must fail: keep its message
+                r = subprocess.run([EXE] + args, cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                                   text=True)
+                assert r.returncode != 0, "expected the reference to stop with an error"
+                with open(os.path.join(dst, "expected", keep[0][1:]), "w") as fo:
+                    fo.write(r.stderr.strip().splitlines()[-1] + "\n")
+                continue
             subprocess.run([EXE] + args, cwd=tmp, check=True, stdout=subprocess.DEVNULL)
             for k in keep:
                 src = os.path.join(tmp, k)

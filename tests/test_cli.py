@@ -145,12 +145,14 @@ def test_cli_usage_without_enough_arguments(exe):
     assert r.returncode == 1 and "Usage:" in r.stdout
 
 
-def test_cli_refuses_block_model(exe, tmp_path):
-    d = os.path.join(CLI_DIR, "ev6")
-    base = [exe, "-c", os.path.join(d, "example.config"), "-f", os.path.join(d, "ev6.flp"), "-p",
-            os.path.join(d, "gcc.ptrace"), "-materials_file", os.path.join(d, "example.materials")]
+def test_cli_block_model_needs_a_floorplan(exe, tmp_path):
+    """The block model is built from a floorplan file; without -f the run
+    stops before any GPU use."""
+    d = os.path.join(CLI_DIR, "ev6_block")
+    base = [exe, "-c", os.path.join(d, "example.config"), "-p", os.path.join(d, "gcc.ptrace"),
+            "-materials_file", os.path.join(d, "example.materials")]
     r = subprocess.run(base + ["-model_type", "block"], capture_output=True, text=True, cwd=tmp_path)
-    assert r.returncode == 1 and "grid model only" in r.stderr
+    assert r.returncode == 1 and "needs a floorplan file" in r.stderr
 
 
 def test_cli_dumps_its_configuration(exe, tmp_path, ref):

@@ -1,5 +1,5 @@
 """The command-line simulator (hotspot_amd/hotspot) on the reference's
-examples 2-4, against the reference's own command-line outputs on the same
+examples 1-4, against the reference's own command-line outputs on the same
 inputs (tests/golden/cli, written by tests/golden/make_cli_goldens.py).
 
 Both print temperatures with "%.2f".  Steady: the reference stops its
@@ -132,3 +132,46 @@ def test_cli_package_model_matches_reference(gpu, tmp_path):
             frac = 0.4 if f.startswith("natural") else 0.5
             m, ex = compare(open(tmp_path / f).read(), read_text(os.path.join(src, "expected", f)), frac)
             print(f, "max |diff|", m, "exact", ex)
+
+
+# The block model on the reference's example 1 (its run.sh command lines),
+# with the package model (forced convection; natural convection in the
+# steady state) and the leakage fixed point.  The steady state comes from
+# the reference's LU factors (within 1e-12 K) and the transient is
+# bit-identical to the reference's, so every printed value agrees.
+# A natural-convection transient that starts at ambient stops in the first
+# interval (no convection at a sink at ambient: the LU finds no pivot), in
+# both, with the same message.
+BLOCK_RUNS = [
+    ("-c example.config -f ev6.flp -p gcc.ptrace -materials_file example.materials -model_type block "
+     "-steady_file gcc.steady -o first.ttrace", ["gcc.steady", "first.ttrace"]),
+    ("-c example.config -init_file @init -f ev6.flp -p gcc.ptrace -materials_file example.materials "
+     "-model_type block -o gcc.ttrace", ["gcc.ttrace"]),
+    ("-c example.config -f ev6.flp -p gcc.ptrace -materials_file example.materials -model_type block "
+     "-package_model_used 1 -package_config_file natural.config -steady_file natural.steady",
+     ["natural.steady"]),
+    ("-c example.config -f ev6.flp -p gcc.ptrace -materials_file example.materials -model_type block "
+     "-package_model_used 1 -package_config_file package.config -steady_file forced.steady",
+     ["forced.steady"]),
+    ("-c example.config -f ev6.flp -p gcc.ptrace -materials_file example.materials -model_type block "
+     "-leakage_used 1 -steady_file leak.steady", ["leak.steady"]),
+]
+
+
+def test_cli_block_model_matches_reference(gpu, tmp_path):
+    src = os.path.join(CLI_DIR, "ev6_block")
+    for f in os.listdir(src):
+        if os.path.isfile(os.path.join(src, f)):
+            shutil.copy(os.path.join(src, f), tmp_path / f)
+    shutil.copy(os.path.join(src, "expected", "gcc.steady"), tmp_path / "ref.init")
+    for argv, outs in BLOCK_RUNS:
+        run_cli(argv.replace("@init", "ref.init"), tmp_path)
+        for f in outs:
+            m, ex = compare(open(tmp_path / f).read(), read_text(os.path.join(src, "expected", f)), 1.0)
+            print(f, "max |diff|", m, "exact", ex)
+    argv = ("-c example.config -f ev6.flp -p gcc.ptrace -materials_file example.materials -model_type block "
+            "-package_model_used 1 -package_config_file natural.config -o natural.ttrace")
+    print("cli:", argv, flush=True)
+    r = subprocess.run([EXE] + argv.split(), cwd=tmp_path, capture_output=True, text=True, timeout=600)
+    want = open(os.path.join(src, "expected", "natural.err")).read().strip()
+    assert r.returncode != 0 and want in r.stderr, r.stderr[-2000:]
