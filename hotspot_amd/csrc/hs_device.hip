@@ -2654,6 +2654,151 @@ struct AsmIn {
 };
 
 // ---------------------------------------------------------------------------
+// Pressure network of a microchannel layer (microchannel.c:378-565).  FLUID
+// cells satisfy sum over fluid-type neighbours (p - p_nbr) = 0 (the uniform
+// hydraulic conductance of the cells cancels), with p the pumping pressure at
+// INLET cells and 0 at OUTLET cells.  The reference assembles the dense
+// system over the fluid cells and solves it by Gauss-Jordan (gaussj,
+// util.c:621-699), O(n_fluid^3).  Here the same system, symmetric positive
+// definite once the fixed pressures move to the right-hand side, is solved
+// by Jacobi-preconditioned CG over the plane (every other cell held fixed)
+// to a 1e-15 relative residual.  Scalars stay on the device: partial sums
+// per workgroup, one workgroup folding them in a fixed order (so the result
+// does not depend on scheduling), and a done flag that freezes every kernel
+// once the residual is reached.
+// ---------------------------------------------------------------------------
+struct PrState {
+    double rz, bb, rr, alpha, beta;
+    int done, iters;
+};
+
+__device__ __forceinline__ bool pr_fluid_type(int t) { return t == CELL_FLUID || t == CELL_INLET || t == CELL_OUTLET; }
+
+// fluid-type neighbours of cell c = i*nc + j, in the order N, S, W, E
+template <typename F>
+__device__ __forceinline__ void pr_nbrs(int nr, int nc, int i, int j, long c, F &&fn) {
+    if (i > 0) fn(c - nc);
+    if (i < nr - 1) fn(c + nc);
+    if (j > 0) fn(c - 1);
+    if (j < nc - 1) fn(c + 1);
+}
+
+// x: fixed values (pump at inlets) and 0; d: fluid-type neighbour count of
+// FLUID cells; r = b (fixed neighbours' pressure), z = r/d, s = z;
+// partials {r.z, b.b}
+__global__ void __launch_bounds__(kBlock) k_pr_setup(const int *__restrict__ types, int nr, int nc, double pump,
+                                                     double *__restrict__ x, double *__restrict__ d,
+                                                     double *__restrict__ r, double *__restrict__ z,
+                                                     double *__restrict__ s, double *__restrict__ part) {
+    const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (c < (long)nr * nc) {
+        const int t = types[c], i = (int)(c / nc), j = (int)(c % nc);
+        double dc = 0.0, bc = 0.0;
+        if (t == CELL_FLUID)
+            pr_nbrs(nr, nc, i, j, c, [&](long o) {
+                const int to = types[o];
+                if (!pr_fluid_type(to)) return;
+                dc += 1.0;
+                if (to == CELL_INLET) bc += pump;
+            });
+        x[c] = t == CELL_INLET ? pump : 0.0;
+        d[c] = dc;
+        r[c] = bc;
+        const double zc = dc > 0.0 ? bc / dc : 0.0;
+        z[c] = zc;
+        s[c] = zc;
+        v[0] = bc * zc;
+        v[1] = bc * bc;
+    }
+    block_reduce<2>(v);
+    if (threadIdx.x == 0) part[2 * blockIdx.x] = v[0], part[2 * blockIdx.x + 1] = v[1];
+}
+
+// q = A s over the FLUID cells; partial s.q
+__global__ void __launch_bounds__(kBlock) k_pr_apply(const int *__restrict__ types, int nr, int nc,
+                                                     const double *__restrict__ d, const double *__restrict__ s,
+                                                     double *__restrict__ q, double *__restrict__ part,
+                                                     const PrState *__restrict__ st) {
+    if (st->done) return;
+    const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+    double v[1] = {0.0};
+    if (c < (long)nr * nc && types[c] == CELL_FLUID) {
+        const int i = (int)(c / nc), j = (int)(c % nc);
+        double acc = d[c] * s[c];
+        pr_nbrs(nr, nc, i, j, c, [&](long o) {
+            if (types[o] == CELL_FLUID) acc -= s[o];
+        });
+        q[c] = acc;
+        v[0] = s[c] * acc;
+    }
+    block_reduce<1>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// x += alpha s, r -= alpha q, z = r/d; partials {r.r, r.z}
+__global__ void __launch_bounds__(kBlock) k_pr_update(const int *__restrict__ types, long n,
+                                                      const double *__restrict__ d, const double *__restrict__ s,
+                                                      const double *__restrict__ q, double *__restrict__ x,
+                                                      double *__restrict__ r, double *__restrict__ z,
+                                                      double *__restrict__ part, const PrState *__restrict__ st) {
+    if (st->done) return;
+    const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (c < n && types[c] == CELL_FLUID) {
+        const double a = st->alpha;
+        x[c] += a * s[c];
+        const double rc = r[c] - a * q[c];
+        r[c] = rc;
+        const double zc = rc / d[c];
+        z[c] = zc;
+        v[0] = rc * rc;
+        v[1] = rc * zc;
+    }
+    block_reduce<2>(v);
+    if (threadIdx.x == 0) part[2 * blockIdx.x] = v[0], part[2 * blockIdx.x + 1] = v[1];
+}
+
+// s = z + beta s
+__global__ void __launch_bounds__(kBlock) k_pr_dir(const int *__restrict__ types, long n,
+                                                   const double *__restrict__ z, double *__restrict__ s,
+                                                   const PrState *__restrict__ st) {
+    if (st->done) return;
+    const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (c < n && types[c] == CELL_FLUID) s[c] = z[c] + st->beta * s[c];
+}
+
+// the scalars of one phase from the partial sums (one workgroup, fixed order):
+// phase 0 after setup, 1 after apply (alpha), 2 after update (beta, stop)
+__global__ void __launch_bounds__(kBlock) k_pr_scalars(const double *__restrict__ part, int nparts, int phase,
+                                                       PrState *__restrict__ st) {
+    if (phase > 0 && st->done) return;
+    const int w = phase == 1 ? 1 : 2;
+    double v[2] = {0.0, 0.0};
+    for (int k = threadIdx.x; k < nparts; k += kBlock)
+        for (int q = 0; q < w; q++) v[q] += part[(long)w * k + q];
+    block_reduce<2>(v);
+    if (threadIdx.x != 0) return;
+    if (phase == 0) {
+        st->rz = v[0];
+        st->bb = v[1];
+        st->iters = 0;
+        st->done = v[1] == 0.0;
+    } else if (phase == 1) {
+        st->alpha = st->rz / v[0];
+    } else {
+        st->rr = v[0];
+        st->iters += 1;
+        if (v[0] <= 1e-30 * st->bb) {
+            st->done = 1;
+        } else {
+            st->beta = v[1] / st->rz;
+            st->rz = v[1];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // backward Euler (the SuperLU path's integrator, RCutil.c:268-428 as called
 // by compute_temp_grid :3632-3662): one implicit step over the interval,
 //   (C/h + G) y' = C y / h + P + sources,

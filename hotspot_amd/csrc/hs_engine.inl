@@ -451,6 +451,56 @@ static void cell_resistances(const Model &M, int n, double *rx, double *ry, doub
     }
 }
 
+// The pressure network of one microchannel layer on the device (k_pr_*):
+// CG iterations in groups of 32 launches between checks of the done flag.
+static void pressure_on_device(hipStream_t stream, int nr, int nc, const std::vector<int> &types, double pump,
+                               std::vector<double> &p) {
+    const long n = (long)nr * nc;
+    const int nb = (int)((n + kBlock - 1) / kBlock);
+    std::vector<void *> bufs;
+    auto alloc = [&](size_t bytes) {
+        void *q = nullptr;
+        HIP_OK(hipMalloc(&q, bytes));
+        bufs.push_back(q);
+        return q;
+    };
+    struct Free {
+        std::vector<void *> &b;
+        ~Free() {
+            for (void *q : b) (void)hipFree(q);
+        }
+    } release{bufs};
+    int *t = (int *)alloc(n * sizeof(int));
+    double *x = (double *)alloc(n * sizeof(double)), *d = (double *)alloc(n * sizeof(double));
+    double *r = (double *)alloc(n * sizeof(double)), *z = (double *)alloc(n * sizeof(double));
+    double *s = (double *)alloc(n * sizeof(double)), *q = (double *)alloc(n * sizeof(double));
+    double *part = (double *)alloc(2 * nb * sizeof(double));
+    PrState *st = (PrState *)alloc(sizeof(PrState));
+    HIP_OK(hipMemcpyAsync(t, types.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIP_OK(hipMemsetAsync(q, 0, n * sizeof(double), stream));
+    launch(k_pr_setup, nb, stream, (const int *)t, nr, nc, pump, x, d, r, z, s, part);
+    launch(k_pr_scalars, 1, stream, (const double *)part, nb, 0, st);
+    PrState hs{};
+    for (int it = 0; it < 100000; it += 32) {
+        for (int k = 0; k < 32; k++) {
+            launch(k_pr_apply, nb, stream, (const int *)t, nr, nc, (const double *)d, (const double *)s, q, part,
+                   (const PrState *)st);
+            launch(k_pr_scalars, 1, stream, (const double *)part, nb, 1, st);
+            launch(k_pr_update, nb, stream, (const int *)t, n, (const double *)d, (const double *)s,
+                   (const double *)q, x, r, z, part, (const PrState *)st);
+            launch(k_pr_scalars, 1, stream, (const double *)part, nb, 2, st);
+            launch(k_pr_dir, nb, stream, (const int *)t, n, (const double *)z, s, (const PrState *)st);
+        }
+        HIP_OK(hipMemcpyAsync(&hs, st, sizeof hs, hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        if (hs.done) break;
+    }
+    if (!hs.done) fatal("the microchannel pressure network did not converge\n");
+    p.assign(n, 0.0);
+    HIP_OK(hipMemcpyAsync(p.data(), x, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+}
+
 static void assemble_hetero(Model &M, DevModel &D) {
     grid_model_t &g = *M.g;
     const int nl = g.n_layers, nr = g.rows, nc = g.cols;
@@ -463,7 +513,10 @@ static void assemble_hetero(Model &M, DevModel &D) {
         cell_resistances(M, n, &rx[n * plane], &ry[n * plane], &rz[n * plane], &cap[n * plane]);
         thick[n] = g.layers[n].thickness;
         if (M.mc[n]) {
-            const Microchannel &mc = *M.mc[n];
+            Microchannel &mc = *M.mc[n];
+            // solved once per model, on its first device build
+            if (mc.pressure.empty())
+                pressure_on_device(D.stream, nr, nc, mc.types, mc.cfg.pumping_pressure, mc.pressure);
             htc[n] = mc.cfg.htc;
             hydro[n] = mc.hydro_c;
             cv[n] = mc.cfg.coolant_capac;

@@ -321,86 +321,9 @@ static double hydro_c(double h, double w, double L, double visc) {
 
 static inline bool is_fluid(int t) { return t == CELL_FLUID || t == CELL_INLET || t == CELL_OUTLET; }
 
-// Pressure of every fluid cell for an ideal pump (microchannel.c:378-565):
-// FLUID cells satisfy sum_nbr hC (p - p_nbr) = 0 over fluid neighbours,
-// p = pumping pressure at inlets and 0 at outlets.  The reference solves the
-// dense system by Gauss-Jordan (util.c:621-699); here the equivalent SPD
-// system on the FLUID cells (Dirichlet values moved to the right-hand side)
-// is solved by Jacobi-preconditioned CG to a 1e-15 relative residual.
-static void solve_pressure(int nr, int nc, const std::vector<int> &types, double pump,
-                           std::vector<double> &p) {
-    const size_t N = (size_t)nr * nc;
-    p.assign(N, 0.0);
-    std::vector<int> id(N, -1);
-    std::vector<size_t> cells;
-    for (size_t c = 0; c < N; c++) {
-        if (types[c] == CELL_INLET) p[c] = pump;
-        if (types[c] == CELL_FLUID) {
-            id[c] = (int)cells.size();
-            cells.push_back(c);
-        }
-    }
-    const size_t n = cells.size();
-    if (!n) return;
-    std::vector<double> b(n, 0.0), d(n, 0.0), x(n, 0.0), r, z(n), q(n), s(n);
-    auto nbrs = [&](size_t c, auto &&fn) {
-        const int i = (int)(c / nc), j = (int)(c % nc);
-        if (i > 0) fn(c - nc);
-        if (i < nr - 1) fn(c + nc);
-        if (j > 0) fn(c - 1);
-        if (j < nc - 1) fn(c + 1);
-    };
-    for (size_t k = 0; k < n; k++) {
-        nbrs(cells[k], [&](size_t o) {
-            if (!is_fluid(types[o])) return;
-            d[k] += 1.0;
-            if (id[o] < 0) b[k] += p[o];   // inlet/outlet: known pressure
-        });
-        if (d[k] == 0.0) fatal("isolated fluid cell in the microchannel network\n");
-    }
-    auto apply = [&](const std::vector<double> &v, std::vector<double> &out) {
-        for (size_t k = 0; k < n; k++) {
-            double acc = d[k] * v[k];
-            nbrs(cells[k], [&](size_t o) {
-                if (id[o] >= 0) acc -= v[id[o]];
-            });
-            out[k] = acc;
-        }
-    };
-    r = b;
-    double bb = 0, rz = 0;
-    for (size_t k = 0; k < n; k++) {
-        bb += b[k] * b[k];
-        z[k] = r[k] / d[k];
-        s[k] = z[k];
-        rz += r[k] * z[k];
-    }
-    if (bb == 0) return;
-    for (int it = 0; it < 100000; it++) {
-        apply(s, q);
-        double sq = 0;
-        for (size_t k = 0; k < n; k++) sq += s[k] * q[k];
-        const double a = rz / sq;
-        double rr = 0, rz2 = 0;
-        for (size_t k = 0; k < n; k++) {
-            x[k] += a * s[k];
-            r[k] -= a * q[k];
-            rr += r[k] * r[k];
-            z[k] = r[k] / d[k];
-            rz2 += r[k] * z[k];
-        }
-        if (rr <= 1e-30 * bb) break;
-        const double beta = rz2 / rz;
-        rz = rz2;
-        for (size_t k = 0; k < n; k++) s[k] = z[k] + beta * s[k];
-    }
-    for (size_t k = 0; k < n; k++) p[cells[k]] = x[k];
-}
-
 // Parses the CSV network, writes the per-cell floorplan next to it exactly as
-// the reference does, solves the pressure network and returns the layer's
-// floorplan (read back from the written file, so every value carries the same
-// '%e' rounding as in the reference).
+// the reference does and returns the layer's floorplan, whose values carry
+// the same '%e' rounding as the reference's (which reads the file back).
 static flp_t *build_network(Microchannel &mc, int nr, int nc) {
     microchannel_config_t &c = mc.cfg;
     FILE *fp = fopen(c.network_file, "r");
@@ -441,36 +364,105 @@ static flp_t *build_network(Microchannel &mc, int nr, int nc) {
         c.cell_types[r] = (int *)calloc(nc, sizeof(int));
         for (int q = 0; q < nc; q++) c.cell_types[r][q] = mc.types[(size_t)r * nc + q];
     }
-    // generated floorplan (microchannel.c:326-351)
+    // generated floorplan (microchannel.c:326-351): one unit per cell, written
+    // with "%e" and used as read back.  Only the row and column positions and
+    // six other values are distinct, so each is formatted (and read back) once.
     snprintf(c.floorplan_file, sizeof(c.floorplan_file), "%s", c.network_file);
     char *ext = strstr(c.floorplan_file, ".csv");
     if (ext) strcpy(ext, ".flp");
-    std::vector<std::string> lines;
-    lines.emplace_back("# Name\tw\th\tx\ty\tc_v\tp\n");
-    char buf[1024];
+    auto fmt = [](double v) {
+        char b[64];
+        snprintf(b, sizeof b, "%e", v);
+        return std::string(b);
+    };
+    std::vector<std::string> sx(nc), sy(nr);
+    for (int q = 0; q < nc; q++) sx[q] = fmt(q * c.cell_width);
+    for (int r = 0; r < nr; r++) sy[r] = fmt((nr - r - 1) * c.cell_height);
+    const std::string sw = fmt(c.cell_width), sh = fmt(c.cell_height);
+    const std::string scv[2] = {fmt(c.wall_capac), fmt(c.coolant_capac)};
+    const std::string srs[2] = {fmt(c.wall_res), fmt(c.coolant_res)};
+    auto val = [](const std::string &t) { return strtod(t.c_str(), nullptr); };
+    flp_t *flp = (flp_t *)calloc(1, sizeof(flp_t));
+    if (!flp) fatal("memory allocation error\n");
+    flp->n_units = nr * nc;
+    flp->units = (unit_t *)calloc((size_t)nr * nc, sizeof(unit_t));
+    if (!flp->units) {
+        free(flp);
+        fatal("memory allocation error\n");
+    }
+    FILE *out = fopen(c.floorplan_file, "w");
+    if (out) fputs("# Name\tw\th\tx\ty\tc_v\tp\n", out);
+    std::string text;
     for (int r = 0; r < nr; r++)
         for (int q = 0; q < nc; q++) {
-            bool f = is_fluid(mc.types[(size_t)r * nc + q]);
-            snprintf(buf, sizeof buf, "Cell_%d_%d\t%e\t%e\t%e\t%e\t%e\t%e\n", r, q, c.cell_width,
-                     c.cell_height, q * c.cell_width, (nr - r - 1) * c.cell_height,
-                     f ? c.coolant_capac : c.wall_capac, f ? c.coolant_res : c.wall_res);
-            lines.emplace_back(buf);
+            const int f = is_fluid(mc.types[(size_t)r * nc + q]) ? 1 : 0;
+            unit_t &u = flp->units[(size_t)r * nc + q];
+            snprintf(u.name, STR_SIZE, "Cell_%d_%d", r, q);
+            u.width = val(sw);
+            u.height = val(sh);
+            u.leftx = val(sx[q]);
+            u.bottomy = val(sy[r]);
+            u.specificheat = val(scv[f]);
+            u.resistivity = val(srs[f]);
+            u.hasRes = u.hasSh = 1;
+            if (out) {
+                text.assign(u.name);
+                for (const std::string *t : {&sw, &sh, (const std::string *)&sx[q], (const std::string *)&sy[r],
+                                             &scv[f], &srs[f]})
+                    text += '\t', text += *t;
+                text += '\n';
+                fputs(text.c_str(), out);
+            }
         }
-    FILE *out = fopen(c.floorplan_file, "w");
-    if (out) {
-        for (auto &l : lines) fputs(l.c_str(), out);
-        fclose(out);
-    }
+    if (out) fclose(out);
     // pressure network (microchannel.c:378-565)
     int nfl = 0;
     for (int t : mc.types) nfl += is_fluid(t);
     c.n_fluid_cells = nfl;
     if (c.pump_internal_res != 0.0)
         fatal("a pump with internal resistance is not supported (ideal pump only)\n");
-    mc.pressure.assign((size_t)nr * nc, 0.0);
-    solve_pressure(nr, nc, mc.types, c.pumping_pressure, mc.pressure);
+    // the pressure network is solved on the device at the model's first
+    // device build (pressure_on_device); a FLUID cell without a fluid-type
+    // neighbour would leave it singular
+    for (int r = 0; r < nr; r++)
+        for (int q = 0; q < nc; q++) {
+            if (mc.types[(size_t)r * nc + q] != CELL_FLUID) continue;
+            const bool linked = (r > 0 && is_fluid(mc.types[(size_t)(r - 1) * nc + q])) ||
+                                (r < nr - 1 && is_fluid(mc.types[(size_t)(r + 1) * nc + q])) ||
+                                (q > 0 && is_fluid(mc.types[(size_t)r * nc + q - 1])) ||
+                                (q < nc - 1 && is_fluid(mc.types[(size_t)r * nc + q + 1]));
+            if (!linked) fatal("isolated fluid cell in the microchannel network\n");
+        }
+    mc.pressure.clear();
     mc.hydro_c = hydro_c(c.cell_thickness, c.cell_width, c.cell_height, c.coolant_visc);
-    return flp_from_lines(lines, 0, 0);
+    return flp;
+}
+
+// A second layer on the same network: its cell types, counts and per-cell
+// floorplan are those of the first (whose floorplan file is already written).
+static flp_t *copy_network(Microchannel &mc, const Microchannel &from, const flp_t *from_flp, int nr, int nc) {
+    microchannel_config_t &c = mc.cfg;
+    mc.types = from.types;
+    c.num_rows = nr;
+    c.num_columns = nc;
+    c.n_fluid_cells = from.cfg.n_fluid_cells;
+    snprintf(c.floorplan_file, sizeof(c.floorplan_file), "%s", from.cfg.floorplan_file);
+    if (c.pump_internal_res != 0.0)
+        fatal("a pump with internal resistance is not supported (ideal pump only)\n");
+    c.cell_types = (int **)calloc(nr, sizeof(int *));
+    for (int r = 0; r < nr; r++) {
+        c.cell_types[r] = (int *)calloc(nc, sizeof(int));
+        for (int q = 0; q < nc; q++) c.cell_types[r][q] = mc.types[(size_t)r * nc + q];
+    }
+    mc.pressure.clear();
+    mc.hydro_c = hydro_c(c.cell_thickness, c.cell_width, c.cell_height, c.coolant_visc);
+    flp_t *f = (flp_t *)calloc(1, sizeof(flp_t));
+    if (!f) fatal("memory allocation error\n");
+    f->n_units = from_flp->n_units;
+    f->units = (unit_t *)malloc(sizeof(unit_t) * (size_t)f->n_units);
+    if (!f->units) fatal("memory allocation error\n");
+    memcpy(f->units, from_flp->units, sizeof(unit_t) * (size_t)f->n_units);
+    return f;
 }
 
 // ---------------------------------------------------------------------------
@@ -592,7 +584,17 @@ static void build_layers(Model &M, flp_t *flp_default, materials_list_t *ml) {
                 mc->cfg.cell_height = g.height / g.rows;
                 mc->cfg.cell_thickness = r.thickness;
                 snprintf(mc->cfg.network_file, STR_SIZE, "%s", r.flp.c_str());
-                L.flp = build_network(*mc, g.rows, g.cols);
+                // a network already built for an earlier layer with the same
+                // file and cell geometry is the same network: copy it
+                const Microchannel *same = nullptr;
+                for (int k = 0; k < i && !same; k++) {
+                    const Microchannel *o = M.mc[k];
+                    if (o && !strcmp(o->cfg.network_file, mc->cfg.network_file) &&
+                        o->cfg.cell_width == mc->cfg.cell_width && o->cfg.cell_height == mc->cfg.cell_height &&
+                        o->cfg.cell_thickness == mc->cfg.cell_thickness)
+                        same = o, L.flp = copy_network(*mc, *o, M.layers[k].flp, g.rows, g.cols);
+                }
+                if (!same) L.flp = build_network(*mc, g.rows, g.cols);
                 L.is_microchannel = 1;
                 L.microchannel_config = &mc->cfg;
                 M.mc[i] = mc;

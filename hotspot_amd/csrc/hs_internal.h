@@ -84,7 +84,7 @@ struct BlockMap {
 struct Microchannel {
     microchannel_config_t cfg;        // owned copy (cell_types etc. allocated)
     std::vector<int> types;           // nr*nc cell types
-    std::vector<double> pressure;     // nr*nc (0 for solid cells)
+    std::vector<double> pressure;     // nr*nc (0 for solid cells); empty until the first device build
     double hydro_c = 0.0;             // hydraulic conductance of one cell
 };
 
