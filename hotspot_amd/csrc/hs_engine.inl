@@ -1424,9 +1424,13 @@ static void transient_taylor(Model &M, DevModel &D, double dt) {
 // :6750-6768).  FP64 Jacobi-preconditioned CG from y, to the steady
 // tolerance; the shift C/h goes into the ambient conductances of a second
 // descriptor (per layer, or per cell for heterogeneous models).
-static void build_be(DevModel &D, double h) {
+// With xb/rhsb given, the shifted descriptor's B_STEADY and B_RHS slots point
+// at those buffers (the GMRES kernels solve into B_STEADY from B_RHS).
+static void build_be(DevModel &D, double h, int xb = -1, int rhsb = -1) {
     const double sigma = 1.0 / h;
     ProbDesc b = D.h;
+    if (xb >= 0) b.b64[B_STEADY] = D.h.b64[xb];
+    if (rhsb >= 0) b.b64[B_RHS] = D.h.b64[rhsb];
     if (!D.hetero)
         for (int n = 0; n < b.nl; n++) b.gamb[n] += sigma / b.cinv[n];
     for (int k = 0; k < std::min(b.nx, (int)EXTRA); k++) b.x_gamb[k] += sigma / b.x_cinv[k];
@@ -1450,15 +1454,29 @@ static void build_be(DevModel &D, double h) {
 
 template <bool HET>
 static void transient_be(Model &M, DevModel &D, double dt) {
-    if (D.h.has_adv) fatal("backward Euler: models with microchannel layers are not supported\n");
-    build_be(D, dt);
     read_scal(D);
     const int yb = B_Y0 + (int)D.h_scal[S_YPAR];
+    const bool adv = D.h.has_adv;
+    if (adv) ensure_gmres(D, false);   // the shifted descriptor copies its buffers
+    build_be(D, dt, adv ? yb : -1, adv ? (int)B_T1 : -1);
     launch_named("k_be_rhs", k_be_rhs<HET>, D.grp.nwork, D.stream, (const ProbDesc *)D.d_desc,
                  (const WorkItem *)D.d_work, yb, (int)B_RHS, (int)B_T1, 1.0 / dt);
     // C y / h dominates the right-hand side's norm, so the relative residual
     // goes 100x below the steady tolerance to keep every node as close
     const double rtol = 1e-2 * M.opt.steady_rtol, rtol2 = rtol * rtol;
+    if (adv) {
+        // coolant advection makes the system non-symmetric: FP64 GMRES
+        // cycles from y, inside FP64 refinement
+        write_scal(D, kMixedStart);
+        for (int guard = 0;; guard++) {
+            gm_round<double, true>(D.be_grp, rtol2, 1e-12, kGmM);
+            read_scal(D);
+            if (D.h_scal[S_DONE] != 0.0) break;
+            if (guard > 100000) fatal("backward Euler: the non-symmetric solve did not finish\n");
+        }
+        M.stats.be_iters = (int)D.h_scal[S_TITERS];
+        return;
+    }
     write_scal(D, {{S_DONE, 0.0}});   // the CG start skips a problem still marked done
     jpcg64_start<HET>(D.be_grp, rtol2, M.opt.max_iters, yb, B_T1);
     int chunk = 8;

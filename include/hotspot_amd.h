@@ -434,8 +434,8 @@ void hs_set_fatal_mode(int exit_on_error);
 #define HS_TRANS_TAYLOR 1 /* exponential (Taylor) integrator of the linear ODE, error bounded a priori */
 #define HS_TRANS_BE     2 /* the reference's SuperLU-build integrator: one backward-Euler step per
                            * interval, (C/h + G) T' = C T / h + P (RCutil.c:268-428), solved by FP64
-                           * Jacobi CG; secondary package nodes without capacitance; not for
-                           * microchannel models */
+                           * Jacobi CG (FP64 GMRES for microchannel models, whose G is
+                           * non-symmetric); secondary package nodes without capacitance */
 
 typedef struct hs_options_st {
     int precision;        /* HS_PREC_* */

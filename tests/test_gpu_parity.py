@@ -255,6 +255,28 @@ def test_microchannel_steady_multigrid_at_scale(lib):
     run.free()
 
 
+def test_microchannel_backward_euler_at_scale(lib):
+    """Backward Euler on the 64 x 64 microchannel stack, 10 ms intervals
+    (one implicit step each, FP64 GMRES), against the oracle's direct solve
+    of the same step."""
+    d = tempfile.mkdtemp()
+    case, pw = cases.microchannel_case(64, 3, d, die=16e-3, width_cells=2, pitch_cells=4)
+    run = cases.Run(lib, case, d)
+    set_options(lib, run, transient=2)
+    om = oracle_for(run)
+    P = run.power_vector(pw)
+    T0, _ = om.steady(P)
+    seq = cases.jittered_powers(P, 3, seed=4)
+    tr = cases.run_transient(run, T0, seq, 1e-2)
+    check_error(lib)
+    Tg = om.b2g(T0, power=False)
+    for k, p in enumerate(seq):
+        Tg = om.advance_be(Tg, om.power_grid(p), 1e-2)
+        assert np.abs(tr[k] - om.g2b(Tg)).max() <= 1e-5
+    assert stats(lib, run).be_iters > 0
+    run.free()
+
+
 def test_two_channel_stack_multigrid(lib):
     """Device Si / Si / channels / Si / channels / Si cap over the package
     (the large steady configuration's stack) at 64 x 64: FP32 multigrid-
@@ -327,10 +349,12 @@ def test_rectangular_grid_taylor_matches_exact(lib, name, case, pw):
 # system mixes rows scaled by C/h with the capacitance-free rows of the
 # secondary package nodes, and the CG and the direct solve then differ by up
 # to ~2e-6 K there; the bar is 1e-5 K, ten times under the FP64 bar of
-# 1e-4 K.  Short (3.333 us) and long (10 ms) intervals.
+# 1e-4 K.  Short (3.333 us) and long (10 ms) intervals.  A microchannel
+# model's system is non-symmetric (coolant advection) and is solved by FP64
+# GMRES from the previous state, to the same tolerance.
 @pytest.mark.parametrize("dt", [3.333e-6, 1e-2], ids=["3.3us", "10ms"])
-@pytest.mark.parametrize("name,case,pw", [CASES[0], CASES[1], CASES[3], CASES[4]],
-                         ids=["default-64", "secondary-16", "3d-stack-32", "detailed-3d-16"])
+@pytest.mark.parametrize("name,case,pw", [CASES[0], CASES[1], CASES[3], CASES[4], ("microchannel-16",) + _mc_case()],
+                         ids=["default-64", "secondary-16", "3d-stack-32", "detailed-3d-16", "microchannel-16"])
 def test_transient_backward_euler_matches_oracle(lib, name, case, pw, dt):
     run = make(lib, case)
     set_options(lib, run, transient=2)
