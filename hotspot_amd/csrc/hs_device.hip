@@ -970,6 +970,7 @@ struct FinArgs {
     double *trec;  // FIN_TAY: per problem {tau, terms (0: no sub-step)}
     int kg;        // FIN_TAY: the group's plan (launches of the sub-step)
     int pairs;     // FIN_TAY: steps run two per sweep (k_tay_pair)
+    int fused;     // FIN_TAY: the last step adds its result into the state (k_tay_term4)
 };
 
 // ---------------------------------------------------------------------------
@@ -1768,6 +1769,11 @@ __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ 
                 const int jf = (K & 1) ? K : (single ? K - 1 : K + 1);
                 const int np = (K >= 2 && jf >= 3) ? (jf - 3) / 2 + 1 : 0;
                 s[S_TBYTES] += np > 0 ? (single ? 12.0 * np : 8.0 + 12.0 * (np - 1)) : 0.0;
+            } else if (a.fused) {
+                // ... and the last step (j = 2) reads the FP64 state and
+                // writes it back instead of writing h: +16 B; a problem with
+                // one term adds E_1 to the state in that launch (20 B)
+                s[S_TBYTES] += K >= 3 ? 8.0 + 12.0 * (K - 3) + 24.0 : K >= 1 ? 20.0 : 0.0;
             } else {
                 s[S_TBYTES] += K >= 2 ? 8.0 + 12.0 * (K - 2) : 0.0;
             }
@@ -2068,8 +2074,24 @@ __device__ __forceinline__ float comp(const float4 &v, int k) {
 __device__ __forceinline__ int term_ntj(const ProbDesc &d) { return d.nc >> d.t_shift; }
 __device__ __forceinline__ int term_layer(const ProbDesc &d, int tile) { return tile / (d.t_nti * term_ntj(d)); }
 
+__device__ __forceinline__ void st2(double *base, unsigned i, hs_d2 v) {
+    *(HS_GLOBAL hs_d2 *)((HS_GLOBAL char *)base + i * 8u) = v;
+}
+// y[c..c+3] += v (the FP64 state takes the sub-step's increment)
+__device__ __forceinline__ void add_state4(double *y, unsigned c, const float (&v)[4]) {
+    hs_d2 a = ld2(y, c), b = ld2(y, c + 2);
+    a.x += (double)v[0];
+    a.y += (double)v[1];
+    b.x += (double)v[2];
+    b.y += (double)v[3];
+    st2(y, c, a);
+    st2(y, c + 2, b);
+}
+
+// With y given (the last step, j = 2), the result is the sub-step's
+// increment and goes into the FP64 state instead of hout.
 __device__ __forceinline__ void term_vec4(const ProbDesc &d, int tile, const float *hin, float *hout,
-                                          const float *e1, float s, const float (&ring)[4]) {
+                                          const float *e1, float s, const float (&ring)[4], double *y) {
     const int ntj = term_ntj(d), tpl = d.t_nti * ntj;
     const int n = tile / tpl, rem = tile - n * tpl;
     const int ti = rem / ntj, tj = rem - ti * ntj;
@@ -2132,7 +2154,28 @@ __device__ __forceinline__ void term_vec4(const ProbDesc &d, int tile, const flo
             if (j == 0 && rn[3] >= 0) flux += rge * (ring[3] - xc);
             e[k] = comp(A[g], k) + s * (ci * (flux - gam * xc));
         }
-        st4(hout, cc[g], make_float4(e[0], e[1], e[2], e[3]));
+        if (y) add_state4(y, cc[g], e);
+        else st4(hout, cc[g], make_float4(e[0], e[1], e[2], e[3]));
+    }
+}
+
+// A problem whose series has one term: y += E_1 over the term tile.
+__device__ __forceinline__ void add_e1_vec4(const ProbDesc &d, int tile, const float *e1, double *y) {
+    const int ntj = term_ntj(d), tpl = d.t_nti * ntj;
+    const int n = tile / tpl, rem = tile - n * tpl;
+    const int ti = rem / ntj, tj = rem - ti * ntj;
+    const int sh = d.t_shift;
+    const int tx4 = threadIdx.x & ((1 << (sh - 2)) - 1), ty4 = threadIdx.x >> (sh - 2);
+    const int G = d.t_rpt >> 2;
+    const int rows = (kBlock >> sh) * d.t_rpt;
+    const int j0 = (tj << sh) + 4 * tx4;
+    for (int g = 0; g < G; g++) {
+        const int i = ti * rows + ty4 * G + g;
+        if (i >= d.nr) continue;
+        const unsigned c = ((unsigned)n * d.nr + i) * d.nc + j0;
+        const float4 a = ld4(e1, c);
+        const float v[4] = {a.x, a.y, a.z, a.w};
+        add_state4(y, c, v);
     }
 }
 
@@ -2189,30 +2232,47 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict_
 // k_tay_term<float, false> for groups whose problems all take float4 tiles
 // (vec4_ok), over the term tiling (t_*) and its work-list: the package item
 // is the scalar kernel's.
+//
+// With fin set (the launch of step j = 2, the last of every series), the
+// step's result -- the sub-step's increment -- is added to the FP64 state
+// in place of k_tay_final, and a problem whose series has one term adds
+// E_1 in the same launch.
 __global__ void __launch_bounds__(kBlock) k_tay_term4(const ProbDesc *__restrict__ D,
                                                       const WorkItem *__restrict__ W, int nper, int e1b,
                                                       int d0b, int d1b, int j, const int *__restrict__ kmax,
-                                                      const double *__restrict__ trec) {
-    if (j > *kmax) return;
+                                                      const double *__restrict__ trec, int fin) {
+    if (!fin && j > *kmax) return;
     const TayItem it = tay_item(W, nper, trec);
-    if (j > it.K) return;
+    if (it.K == 0 || (!fin && j > it.K)) return;
     const ProbDesc &d = D[it.prob];
+    const float *e1 = d.b32[e1b];
+    double *y = fin ? d.b64[B_Y0 + (int)d.scal[S_YPAR]] : nullptr;
+    if (j > it.K) {   // fin, one term
+        if (it.tile < d.t_ntiles) add_e1_vec4(d, it.tile, e1, y);
+        else if ((int)threadIdx.x < d.nx) {
+            const unsigned c = (unsigned)d.nl * d.nr * d.nc + threadIdx.x;
+            y[c] += (double)e1[c];
+        }
+        return;
+    }
     const HornerBufs hb = horner_bufs(it.K, j, e1b, d0b, d1b);
-    const float *hin = d.b32[hb.in], *e1 = d.b32[e1b];
+    const float *hin = d.b32[hb.in];
     float *hout = d.b32[hb.out];
     const float s = (float)(it.tau / j);
     LdPlain<float> ld{hin};
     if (it.tile < d.t_ntiles) {
         float ring[4];
         load_ring<float>(d, ld, term_layer(d, it.tile), ring);
-        term_vec4(d, it.tile, hin, hout, e1, s, ring);
+        term_vec4(d, it.tile, hin, hout, e1, s, ring, y);
     } else {
         __shared__ float xf[kMaxExtra], xd[kMaxExtra];
         package_terms<float>(d, ld, true, xf, xd);
         const int kk = threadIdx.x;
         if (kk < d.nx) {
             const unsigned c = (unsigned)d.nl * d.nr * d.nc + kk;
-            hout[c] = e1[c] + s * ((float)d.x_cinv[kk] * (xf[kk] - (float)d.x_gamb[kk] * hin[c]));
+            const float v = e1[c] + s * ((float)d.x_cinv[kk] * (xf[kk] - (float)d.x_gamb[kk] * hin[c]));
+            if (fin) y[c] += (double)v;
+            else hout[c] = v;
         }
     }
 }

@@ -287,7 +287,7 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap)
                 if (r == kWarm) HIP_OK(hipEventRecord(e0, G.stream));
                 hipLaunchKernelGGL(k_tay_term4, dim3(G.ntwork), dim3(kBlock), 0, G.stream,
                                    (const ProbDesc *)G.d_desc, (const WorkItem *)G.d_twork, G.tnper, (int)F_ACC,
-                                   (int)F_D0, (int)F_D1, 2, (const int *)G.d_kmax, (const double *)G.d_trec);
+                                   (int)F_D0, (int)F_D1, 2, (const int *)G.d_kmax, (const double *)G.d_trec, 0);
             }
             HIP_OK(hipEventRecord(e1, G.stream));
             HIP_OK(hipEventSynchronize(e1));
@@ -1370,6 +1370,8 @@ static void taylor_interval(const Group &G, int m, int K, int seq) {
         else
             launch_named("k_tay_first", k_tay_first<TD, HET>, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax);
         const bool pairs = f32 && !HET && G.pairs && K >= 3;
+        // float4 sweeps finish the sub-step in their last launch (j = 2)
+        const bool fused = f32 && !HET && G.vec4 && !pairs && K >= 2;
         FinArgs a{};
         a.sub = s;
         a.seq = seq;
@@ -1377,13 +1379,14 @@ static void taylor_interval(const Group &G, int m, int K, int seq) {
         a.trec = G.d_trec;
         a.kg = K;
         a.pairs = pairs ? 1 : 0;
+        a.fused = fused ? 1 : 0;
         finish(G, FIN_TAY, a);
         const int *km = G.d_kmax;
         const double *tr = G.d_trec;
         auto single = [&](int j) {
             if (f32 && !HET && G.vec4)
                 launch_named("k_tay_term", k_tay_term4, G.ntwork, G.stream, dd, (const WorkItem *)G.d_twork,
-                             G.tnper, accb, db0, db1, j, km, tr);
+                             G.tnper, accb, db0, db1, j, km, tr, (fused && j == 2) ? 1 : 0);
             else
                 launch_named("k_tay_term", k_tay_term<TD, HET>, G.nwork, G.stream, dd, w, G.nper, accb, db0, db1,
                              j, km, tr);
@@ -1401,8 +1404,9 @@ static void taylor_interval(const Group &G, int m, int K, int seq) {
         } else {
             for (int j = K; j >= 2; j--) single(j);
         }
-        launch_named("k_tay_final", k_tay_final<TD>, G.nwork, G.stream, dd, w, G.nper, accb, db0, db1, tr,
-                     pairs ? K : 0);
+        if (!fused)
+            launch_named("k_tay_final", k_tay_final<TD>, G.nwork, G.stream, dd, w, G.nper, accb, db0, db1, tr,
+                         pairs ? K : 0);
     }
 }
 
