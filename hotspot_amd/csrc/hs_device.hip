@@ -422,13 +422,23 @@ __device__ __forceinline__ bool seq_finished(const ProbDesc &d, int seq) {
 // from_seq: 1 reads the block vector of the current sequence step, 2 the
 // block output vector (a steady result handed to a transient), 0 the staging
 // vector.
+//
+// With power32 (V_POWER, no sources), the cell power goes to the FP32 buffer
+// out_buf; the tiles of a layer whose blocks all carry zero power write
+// nothing and mark the layer in lzero instead (every workgroup of the layer
+// writes the same mark, which the next kernel reads).
 __global__ void __launch_bounds__(kBlock) k_b2g(const ProbDesc *__restrict__ D,
                                                 const WorkItem *__restrict__ W, int type,
-                                                int out_buf, int add_sources, int from_seq) {
+                                                int out_buf, int add_sources, int from_seq, int power32) {
     const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     if (seq_finished(d, from_seq == 1)) return;
-    double *out = d.b64[out_buf];
+    double *out = power32 ? nullptr : d.b64[out_buf];
+    float *out32 = power32 ? d.b32[out_buf] : nullptr;
+    auto put = [&](unsigned c, double v) {
+        if (power32) st_at(out32, c, (float)v);
+        else st_at(out, c, v);
+    };
     const double *b = from_seq == 1 ? d.seq_in + (long)d.scal[S_STEP] * (d.nblocks + d.nx)
                       : from_seq == 2 ? d.blk_out : d.blk_in;
     const long ncell = (long)d.nl * d.nr * d.nc;
@@ -453,7 +463,9 @@ __global__ void __launch_bounds__(kBlock) k_b2g(const ProbDesc *__restrict__ D,
             __syncthreads();
             mapped = nonzero != 0;
         }
+        if (power32 && threadIdx.x == 0) d.lzero[n] = mapped ? 0 : 1;
         if (!mapped) {
+            if (power32) return;
             tile_loop<8>(d, wi.tile, [&](const Cell &) { return 0; }, [&](const Cell &q, int) {
                 double val = 0.0;
                 if (add_sources) {
@@ -504,11 +516,11 @@ __global__ void __launch_bounds__(kBlock) k_b2g(const ProbDesc *__restrict__ D,
                     val += src;
                     if (d.bconst) val += ld_at(d.bconst, q.c);
                 }
-                st_at(out, q.c, val);
+                put(q.c, val);
             });
     } else {
         const int k = threadIdx.x;
-        if (k < d.nx) {
+        if (k < d.nx && !power32) {
             double v = (type == V_POWER) ? 0.0 : b[d.nblocks + k];
             if (add_sources) v += d.x_gamb[k] * d.ambient;
             out[ncell + k] = v;
@@ -1916,15 +1928,21 @@ __device__ __forceinline__ hs_d2 ld2(const double *base, unsigned i) {
     return *(const HS_GLOBAL hs_d2 *)((const HS_GLOBAL char *)base + i * 8u);
 }
 
+//
+// With power32 the right-hand side is the FP32 cell power of F_PW (none for
+// a layer marked in lzero) plus the ambient sources, formed here: P + gamb
+// T_ambient on cells, gamb T_ambient on the package nodes, as k_b2g adds
+// them otherwise.
 __global__ void __launch_bounds__(kBlock) k_tay_first2(const ProbDesc *__restrict__ D,
                                                        const WorkItem *__restrict__ W, int e1b, int sub, int seq,
-                                                       int *__restrict__ kmax) {
+                                                       int *__restrict__ kmax, int power32) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *kmax = 0;   // raised by the finish of each problem
     const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     if (sub >= (int)d.scal[S_TK] || seq_finished(d, seq)) return;
     const int ypar = (int)d.scal[S_YPAR];
     const double *y = d.b64[B_Y0 + ypar], *rhs = d.b64[B_RHS];
+    const float *pw = d.b32[F_PW];
     float *e1 = d.b32[e1b];
     const double tau = d.scal[S_TAU];
     double v[1] = {0};
@@ -1944,6 +1962,8 @@ __global__ void __launch_bounds__(kBlock) k_tay_first2(const ProbDesc *__restric
         const double ci = d.cinv[n], gam = d.gamb[n];
         const int *rn = d.ring_node[n];
         const double rgn = d.ring_gns[n], rge = d.ring_gew[n];
+        const double src = gam * d.ambient;
+        const bool pz = power32 && d.lzero[n];
         double ring[4];
         for (int e = 0; e < 4; e++) ring[e] = ld_at(y, ncell + max(rn[e], 0));
         for (int g0 = 0; g0 < G; g0 += 2) {
@@ -1964,7 +1984,14 @@ __global__ void __launch_bounds__(kBlock) k_tay_first2(const ProbDesc *__restric
                 Dn[g] = ld2(y, n < nl - 1 ? c + plane : c);
                 Wv[g] = ld_at(y, j0 > 0 ? c - 1 : c);
                 Ev[g] = ld_at(y, j0 + 2 < nc ? c + 2 : c);
-                B[g] = ld2(rhs, c);
+                if (!power32) {
+                    B[g] = ld2(rhs, c);
+                } else if (pz) {
+                    B[g] = hs_d2{0.0 + src, 0.0 + src};
+                } else {
+                    const hs_f2 p = *(const HS_GLOBAL hs_f2 *)((const HS_GLOBAL char *)pw + c * 4u);
+                    B[g] = hs_d2{(double)p.x + src, (double)p.y + src};
+                }
             }
 #pragma unroll
             for (int g = 0; g < 2; g++) {
@@ -2003,7 +2030,8 @@ __global__ void __launch_bounds__(kBlock) k_tay_first2(const ProbDesc *__restric
         const int k = threadIdx.x;
         if (k < d.nx) {
             const long c = (long)d.nl * d.nr * d.nc + k;
-            const float e = (float)(tau * (d.x_cinv[k] * (rhs[c] + xf[k] - d.x_gamb[k] * y[c])));
+            const double b = power32 ? 0.0 + d.x_gamb[k] * d.ambient : rhs[c];
+            const float e = (float)(tau * (d.x_cinv[k] * (b + xf[k] - d.x_gamb[k] * y[c])));
             e1[c] = e;
             v[0] = fmax(v[0], fabs((double)e));
         }

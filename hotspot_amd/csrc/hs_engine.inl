@@ -670,6 +670,8 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
         h.b64[b] = D->alloc<double>(D->vlen);
         HIP_OK(hipMemset(h.b64[b], 0, D->vlen * sizeof(double)));
     }
+    h.lzero = D->alloc<int>(kMaxLayers);
+    HIP_OK(hipMemset(h.lzero, 0, kMaxLayers * sizeof(int)));
     for (int b = 0; b < NB32; b++) {
         h.b32[b] = D->alloc<float>(D->vlen);
         HIP_OK(hipMemset(h.b32[b], 0, D->vlen * sizeof(float)));
@@ -827,7 +829,16 @@ static inline int small_grid(int n) { return (n + 63) / 64; }
 
 static void run_b2g(const Group &G, int type, int out, int sources, int from_seq = 0) {
     launch(k_b2g, G.nwork, G.stream, (const ProbDesc *)G.d_desc, (const WorkItem *)G.d_work, type, out,
-           sources, from_seq);
+           sources, from_seq, 0);
+}
+
+// The power of an interval for the FP32 Taylor sweeps of homogeneous float4
+// groups: cell power alone, in FP32 (F_PW), layers without power left
+// unwritten; k_tay_first2 adds the ambient sources itself.
+static bool power32(const Group &G, bool f32, bool het) { return f32 && !het && G.vec4; }
+static void run_b2g_power32(const Group &G, int from_seq = 0) {
+    launch(k_b2g, G.nwork, G.stream, (const ProbDesc *)G.d_desc, (const WorkItem *)G.d_work, (int)V_POWER,
+           (int)F_PW, 0, from_seq, 1);
 }
 
 static void run_g2b(const Group &G, int in, int to_seq = 0) {
@@ -1366,7 +1377,7 @@ static void taylor_interval(const Group &G, int m, int K, int seq) {
     const int db0 = f32 ? F_D0 : B_KA, db1 = f32 ? F_D1 : B_KB, accb = f32 ? F_ACC : B_ACC;
     for (int s = 0; s < m; s++) {
         if (f32 && !HET && G.vec4)
-            launch_named("k_tay_first", k_tay_first2, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax);
+            launch_named("k_tay_first", k_tay_first2, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax, 1);
         else
             launch_named("k_tay_first", k_tay_first<TD, HET>, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax);
         const bool pairs = f32 && !HET && G.pairs && K >= 3;
@@ -1511,7 +1522,10 @@ void dev_transient(Model &M, const double *block_power, const double *block_temp
     M.stats.be_iters = 0;
     HIP_OK(hipEventRecord(D.ev0, D.stream));
     upload_blocks(D, block_power);
-    run_b2g(D.grp, V_POWER, B_RHS, 1);
+    if (M.opt.transient == HS_TRANS_TAYLOR && power32(D.grp, M.opt.precision == HS_PREC_MIXED, D.hetero))
+        run_b2g_power32(D.grp);
+    else
+        run_b2g(D.grp, V_POWER, B_RHS, 1);
     if (block_temp_in) {
         HIP_OK(hipStreamSynchronize(D.stream));
         upload_blocks(D, block_temp_in);
@@ -1596,7 +1610,8 @@ static void rk4_seq_body(const Group &G, int slots) {
 
 template <typename TD, bool HET>
 static void taylor_seq_body(const Group &G, int m, int K) {
-    run_b2g(G, V_POWER, B_RHS, 1, 1);
+    if (power32(G, sizeof(TD) == 4, HET)) run_b2g_power32(G, 1);
+    else run_b2g(G, V_POWER, B_RHS, 1, 1);
     taylor_interval<TD, HET>(G, m, K, 1);
     run_g2b(G, -1, 1);
     hipLaunchKernelGGL(k_step_advance, dim3(small_grid(G.nprob)), dim3(64), 0, G.stream,

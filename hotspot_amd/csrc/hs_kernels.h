@@ -40,6 +40,7 @@ enum Buf64 {
 enum Buf32 {
     F_R = 0, F_P0, F_P1, F_Q, F_X, F_Z,   // inner CG of the mixed steady solve
     F_D0, F_D1, F_ACC,               // Taylor terms (mixed transient)
+    F_PW,                            // cell power without sources (FP32 Taylor groups)
     NB32
 };
 
@@ -125,6 +126,7 @@ struct ProbDesc {
     double *seq_in;                     // [steps][nblocks + nx] power sequence
     double *seq_out;                    // [steps][nblocks + nx] temperatures
     double *bconst;                     // per-cell constant source (inlet), hetero only
+    int *lzero;                         // per layer: F_PW holds no power (all blocks 0), not written
     // GMRES (non-symmetric steady solves; allocated on first use)
     double *gm;                         // per-problem state (GmState)
     double *gpart;                      // [ntiles+1][kGmM+1] partials
