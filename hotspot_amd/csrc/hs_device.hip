@@ -464,8 +464,28 @@ __global__ void __launch_bounds__(kBlock) k_b2g(const ProbDesc *__restrict__ D,
             mapped = nonzero != 0;
         }
         if (power32 && threadIdx.x == 0) d.lzero[n] = mapped ? 0 : 1;
+        if (power32) {
+            if (!mapped) return;
+            // a cell of one or two blocks: their weights times their power;
+            // a cell of three or more: its map entries
+            const int4 *uw = d.cell_uw[n];
+            tile_loop<8>(
+                d, wi.tile,
+                [&](const Cell &q) { return uw[(unsigned)q.i * d.nc + q.j]; },
+                [&](const Cell &q, const int4 &t) {
+                    double val = 0.0;
+                    if (t.x >= 0) {
+                        val = (double)__int_as_float(t.z) * vb[t.x] + (double)__int_as_float(t.w) * vb[t.y];
+                    } else {
+                        const unsigned cc = (unsigned)q.i * d.nc + q.j;
+                        for (int e = ptr[cc]; e < ptr[cc + 1]; e++) val += (occ[e] * vb[unit[e]]) / area[unit[e]];
+                        val *= d.cell_area;
+                    }
+                    st_at(out32, q.c, (float)val);
+                });
+            return;
+        }
         if (!mapped) {
-            if (power32) return;
             tile_loop<8>(d, wi.tile, [&](const Cell &) { return 0; }, [&](const Cell &q, int) {
                 double val = 0.0;
                 if (add_sources) {

@@ -696,6 +696,7 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
     }
     std::vector<int4> rng;
     std::vector<int> blayer;
+    std::vector<const int4 *> dcu(M.maps.size(), nullptr);
     for (int n = 0; n < g.n_layers; n++) {
         const int mi = M.layer_map[n];
         h.map_ptr[n] = dptr[mi];
@@ -711,6 +712,26 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
             blayer.push_back(n);
         }
         h.unit_area[n] = D->upload(area.data(), area.size());
+        if (!dcu[mi]) {   // layers on one map share its floorplan, hence the areas
+            const BlockMap &bm = M.maps[mi];
+            const size_t plane = bm.cell_ptr.size() - 1;
+            std::vector<int4> uw(plane, make_int4(-1, -1, 0, 0));
+            auto bits = [](float f) {
+                int b;
+                memcpy(&b, &f, sizeof b);
+                return b;
+            };
+            for (size_t c = 0; c < plane; c++) {
+                const int e = bm.cell_ptr[c], cnt = bm.cell_ptr[c + 1] - e;
+                if (cnt < 1 || cnt > 2) continue;
+                const int u0 = bm.ent_unit[e], u1 = bm.ent_unit[e + cnt - 1];
+                const float w0 = (float)(bm.ent_occ[e] * h.cell_area / area[u0]);
+                const float w1 = cnt == 2 ? (float)(bm.ent_occ[e + 1] * h.cell_area / area[u1]) : 0.f;
+                uw[c] = make_int4(u0, u1, bits(w0), bits(w1));
+            }
+            dcu[mi] = D->upload(uw.data(), uw.size());
+        }
+        h.cell_uw[n] = dcu[mi];
     }
     h.nblocks = g.total_n_blocks;
     h.blk_range = D->upload(rng.data(), rng.size());

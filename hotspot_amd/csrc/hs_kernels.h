@@ -117,6 +117,10 @@ struct ProbDesc {
     const int *map_unit[kMaxLayers];
     const double *map_occ[kMaxLayers];
     const double *unit_area[kMaxLayers];
+    // per cell of a layer: its first two blocks and their weights occ * cell
+    // area / block area as {u0, u1, w0, w1} (weights bit-cast; u1 = u0 and
+    // w1 = 0 for a cell inside one block, u0 = -1 for a cell of 3+ blocks)
+    const int4 *cell_uw[kMaxLayers];
     int layer_base[kMaxLayers];         // block-vector offset per layer
     const int4 *blk_range;              // per block of the vector: i1,i2,j1,j2
     const int *blk_layer;
