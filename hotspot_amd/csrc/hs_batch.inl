@@ -216,7 +216,9 @@ static void group_sync(Engine &E, BatchGroup &B, hipStream_t s, bool het, bool t
         int nlmax = 1;
         for (int i = 0; i < n; i++) {
             const DevModel &D = *B.members[i]->D;
-            ld[i] = l == 0 ? D.h : D.mg[l].h;
+            // level 0 is the model itself, but for the upwind advection of
+            // a non-symmetric model's preconditioner
+            ld[i] = l == 0 && !D.h.has_adv ? D.h : D.mg[l].h;
             nlmax = std::max(nlmax, ld[i].nl);
             auto ci = column_items(ld[i], i);
             lw.insert(lw.end(), ci.begin(), ci.end());

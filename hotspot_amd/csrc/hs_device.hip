@@ -1103,6 +1103,37 @@ __global__ void k_mg_coarsen(const ProbDesc *__restrict__ F, const ProbDesc *__r
     ((float *)c.fdinv)[t] = 1.f / diag;
 }
 
+// Upwind form of the coolant advection, for the multigrid preconditioner of
+// the non-symmetric solves.  The operator advects with central averages,
+// Cv Q (T_nbr + T) / 2 on every fluid face (temperature_grid.c:3256-3590),
+// which leaves a fluid row without diagonal dominance at high cell Peclet
+// numbers, where relaxation stalls.  The V-cycle works on the upwind operator
+// instead -- heat enters at the upstream cell's temperature and leaves at the
+// cell's own, an M-matrix -- and GMRES corrects the difference.  In the
+// stencil's form sum a (T_nbr + T) + self T: a' = 2a on inflow faces, 0 on
+// outflow faces, self' = self - 2 sum |a|.
+__global__ void k_upwind_coef(const ProbDesc *__restrict__ D, long ncell, float *uN, float *uS, float *uE,
+                              float *uW, float *uSelf, float *udinv) {
+    const ProbDesc &d = *D;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const double a[4] = {d.aN[c], d.aS[c], d.aE[c], d.aW[c]};
+    double u[4], sa = 0.0, su = 0.0, sabs = 0.0;
+    for (int k = 0; k < 4; k++) {
+        u[k] = a[k] > 0.0 ? 2.0 * a[k] : 0.0;
+        sa += a[k];
+        su += u[k];
+        sabs += fabs(a[k]);
+    }
+    const double self = d.aSelf[c], uself = self - 2.0 * sabs;
+    uN[c] = (float)u[0];
+    uS[c] = (float)u[1];
+    uE[c] = (float)u[2];
+    uW[c] = (float)u[3];
+    uSelf[c] = (float)uself;
+    udinv[c] = (float)(1.0 / (1.0 / d.dinvc[c] + (sa + self) - (su + uself)));
+}
+
 // when a multigrid kernel runs for a problem (flags of the fine problem,
 // whose scalars every level shares)
 enum MgWhen { MG_RUNNING = 0, MG_INIT = 1, MG_CORRECT = 2 };

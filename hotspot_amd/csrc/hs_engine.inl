@@ -967,6 +967,21 @@ static void build_mg(DevModel &D) {
     MgLevel l0;
     l0.h = D.h;
     l0.d_desc = D.d_desc;
+    if (D.h.has_adv) {
+        // the preconditioner's finest level: the upwind advection (k_upwind_coef)
+        const long ncell = (long)D.h.nl * D.h.nr * D.h.nc;
+        float *u[6];
+        for (float *&p : u) p = D.alloc<float>(ncell);
+        hipLaunchKernelGGL(k_upwind_coef, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, D.stream,
+                           (const ProbDesc *)D.d_desc, ncell, u[0], u[1], u[2], u[3], u[4], u[5]);
+        l0.h.faN = u[0];
+        l0.h.faS = u[1];
+        l0.h.faE = u[2];
+        l0.h.faW = u[3];
+        l0.h.faSelf = u[4];
+        l0.h.fdinv = u[5];
+        l0.d_desc = D.upload(&l0.h, 1);
+    }
     auto w0 = column_items(D.h, 0);
     l0.d_cwork = D.upload(w0.data(), w0.size());
     l0.ncwork = (int)w0.size();
