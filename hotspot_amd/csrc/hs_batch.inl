@@ -300,9 +300,10 @@ static void admit(Engine &E, EngJob &j) {
         const int nbk = (int)((D.vlen + kBlock - 1) / kBlock);
         launch(k_fill, nbk, D.stream, D.h.b64[B_STEADY], D.vlen, M.g->config.ambient);
         if (j.adv) {
-            const bool mg = E.opt.steady_solver == HS_STEADY_MGPCG;
-            ensure_gmres(D, E.opt.precision == HS_PREC_MIXED || mg);
-            if (mg) build_mg(D);
+            // non-symmetric: GMRES with the upwind multigrid preconditioner
+            // in every form (Jacobi does not converge at channel Peclet numbers)
+            ensure_gmres(D, true);
+            build_mg(D);
             write_scal(D, kMixedStart);
         } else if (E.opt.steady_solver == HS_STEADY_MGPCG) {
             build_mg(D);
@@ -457,7 +458,7 @@ int eng_submit(Engine &E, Model &M, const double *power, int rows, const double 
     j->dt = dt;
     j->hetero = M.detailed;
     j->adv = adv;
-    j->footprint = dev_footprint(M, nsteps, !init && E.opt.steady_solver == HS_STEADY_MGPCG);
+    j->footprint = dev_footprint(M, nsteps, !init && (E.opt.steady_solver == HS_STEADY_MGPCG || adv));
     if (j->footprint > E.budget)
         fatal("hs_engine_submit: the job needs " + std::to_string(j->footprint) +
               " bytes of device memory, above the engine budget of " + std::to_string(E.budget) + "\n");

@@ -1315,9 +1315,10 @@ void dev_steady(Model &M, const double *block_power, double *block_temp) {
     }
     write_scal(D, {{S_DONE, 0.0}});
     if (D.h.has_adv) {
-        if (M.opt.steady_solver == HS_STEADY_MGPCG) steady_gmres_mg(M, D);
-        else if (M.opt.precision == HS_PREC_MIXED) steady_gmres<float, true>(M, D);
-        else steady_gmres<double, true>(M, D);
+        // non-symmetric: GMRES with the upwind multigrid preconditioner in
+        // every form; Jacobi-preconditioned GMRES does not converge at the
+        // cell Peclet numbers of real channels
+        steady_gmres_mg(M, D);
     } else if (M.opt.precision == HS_PREC_MIXED || M.opt.steady_solver == HS_STEADY_MGPCG) {
         if (D.hetero) steady_mixed<true>(M, D);
         else steady_mixed<false>(M, D);
@@ -1331,6 +1332,14 @@ void dev_steady(Model &M, const double *block_power, double *block_temp) {
     float ms = 0;
     HIP_OK(hipEventElapsedTime(&ms, D.ev0, D.ev1));
     M.stats.last_solve_ms = ms;
+    // a solve that stopped at its iteration limits far from the tolerance
+    const double rel = M.stats.steady_rel_resid;
+    if (!(rel <= std::max(100.0 * M.opt.steady_rtol, 1e-6))) {
+        char msg[160];
+        snprintf(msg, sizeof msg, "steady solve did not converge (relative residual %.2e after %d iterations)\n",
+                 rel, M.stats.steady_iters);
+        fatal(msg);
+    }
 }
 
 // ---- transient ---------------------------------------------------------------

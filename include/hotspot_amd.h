@@ -422,13 +422,16 @@ void hs_set_fatal_mode(int exit_on_error);
 #define HS_PREC_FP64  0   /* every sweep in FP64 */
 #define HS_PREC_MIXED 1   /* FP32 sweeps inside FP64 refinement / FP64 residuals */
 /* steady solvers */
-#define HS_STEADY_JPCG  0 /* Jacobi-preconditioned CG where G is symmetric; where coolant
-                           * advection (microchannel layers) makes G non-symmetric, restarted
-                           * GMRES(32) cycles, Jacobi right-preconditioned, inside FP64
-                           * iterative refinement */
+#define HS_STEADY_JPCG  0 /* Jacobi-preconditioned CG (G symmetric) */
 #define HS_STEADY_MGPCG 1 /* FP32 CG preconditioned by a geometric-multigrid V-cycle (z-line
                            * Jacobi smoothing, coarsening rows and columns), inside FP64
-                           * iterative refinement, for either precision option */
+                           * iterative refinement, for either precision option.
+                           * Where coolant advection (microchannel layers) makes G
+                           * non-symmetric, either option solves by restarted GMRES(32)
+                           * cycles inside FP64 refinement, right-preconditioned by the
+                           * V-cycle of the upwind form of the advection.  A steady solve
+                           * that stops at its iteration limits above max(100 steady_rtol,
+                           * 1e-6) relative residual is an error. */
 /* transient integrators */
 #define HS_TRANS_RK4    0 /* the reference's adaptive step-doubling RK4 */
 #define HS_TRANS_TAYLOR 1 /* exponential (Taylor) integrator of the linear ODE, error bounded a priori */

@@ -6,6 +6,8 @@ converged reference operator; mixed-precision path within 1e-2 K.
 """
 import tempfile
 
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -137,9 +139,10 @@ def test_steady_multigrid_matches_oracle(lib, name, case, pw, prec):
 @pytest.mark.parametrize("solver", [0, 1], ids=["jacobi", "multigrid"])
 @pytest.mark.parametrize("prec,tol", [(0, FP64_TOL), (1, MIXED_TOL)], ids=["fp64", "mixed"])
 def test_microchannel_steady_gmres(lib, prec, tol, solver):
-    """Advection makes G non-symmetric: the steady solve runs restarted
-    GMRES cycles (FP64 or FP32) inside FP64 refinement and is checked against
-    the oracle's direct sparse solve of the same operator."""
+    """Advection makes G non-symmetric: with either solver option the steady
+    solve runs restarted GMRES cycles inside FP64 refinement, preconditioned
+    by the multigrid V-cycle of the upwind advection, and is checked against
+    the oracle's direct sparse solve of the same (central) operator."""
     case, pw = _mc_case()
     run = make(lib, case)
     set_options(lib, run, precision=prec, steady_solver=solver)
@@ -237,9 +240,8 @@ def test_reference_transient_trajectory(lib, ref):
 
 def test_microchannel_steady_multigrid_at_scale(lib):
     """A 64 x 64 microchannel stack (2-cell channels on a 4-cell pitch): the
-    multigrid-preconditioned GMRES converges in a few hundred iterations
-    where Jacobi-preconditioned GMRES needs thousands, to the oracle's direct
-    solve."""
+    GMRES preconditioned by the upwind multigrid V-cycle converges in a few
+    hundred iterations, to the oracle's direct solve."""
     d = tempfile.mkdtemp()
     case, pw = cases.microchannel_case(64, 3, d, die=16e-3, width_cells=2, pitch_cells=4)
     run = cases.Run(lib, case, d)
@@ -252,6 +254,40 @@ def test_microchannel_steady_multigrid_at_scale(lib):
     To, _ = om.steady(P)
     assert np.abs(T - To).max() <= FP64_TOL
     assert st.steady_rel_resid <= 1e-10 and st.steady_iters < 1000
+    run.free()
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "mixed"])
+def test_two_channel_stack_128_default_solver(lib, prec):
+    """The two-microchannel stack at 128 x 128 (cell Peclet number ~10^3 in
+    the channels) with the default solver option: Jacobi-preconditioned GMRES
+    stalls here (relative residual 1e-3 after 64 000 iterations); the upwind
+    multigrid preconditioner converges in a few hundred, to the oracle's
+    direct solve."""
+    d = tempfile.mkdtemp()
+    case, pw = cases.two_channel_stack_case(128, 7, d)
+    run = cases.Run(lib, case, d)
+    set_options(lib, run, precision=prec, steady_solver=0)
+    P = run.power_vector(pw)
+    T = run.steady(P.copy())
+    check_error(lib)
+    st = stats(lib, run)
+    assert st.steady_rel_resid <= 1e-11 and st.steady_iters < 2000
+    To, _ = oracle_for(run).steady(P)
+    assert np.abs(T - To).max() <= (FP64_TOL if prec == 0 else MIXED_TOL)
+    run.free()
+
+
+def test_unconverged_steady_solve_is_an_error(lib):
+    """A solve stopped by its iteration limit far above the tolerance reports
+    an error instead of returning the unconverged state."""
+    name, case, pw = CASES[0]
+    run = make(lib, case)
+    set_options(lib, run, max_iters=2)
+    run.steady(run.power_vector(pw))
+    buf = C.create_string_buffer(512)
+    assert lib.lib.hs_last_error(buf, 512) == 1
+    assert "did not converge" in buf.value.decode()
     run.free()
 
 
