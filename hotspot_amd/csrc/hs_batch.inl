@@ -146,6 +146,7 @@ static void group_sync(Engine &E, BatchGroup &B, hipStream_t s, bool het, bool t
     const int n = (int)B.members.size();
     std::vector<ProbDesc> desc(n);
     std::vector<WorkItem> w, gw;
+    std::vector<double> rec;   // Taylor records (step factors), transient Taylor groups
     int m = 1, K = 1;
     bool v4 = true;
     for (int i = 0; i < n; i++) {
@@ -156,9 +157,11 @@ static void group_sync(Engine &E, BatchGroup &B, hipStream_t s, bool het, bool t
         for (int u = 0; u < D.h.nblocks; u += kBlock / kWave) gw.push_back(WorkItem{i, u});
         gw.push_back(WorkItem{i, -1});
         if (trans && E.opt.transient == HS_TRANS_TAYLOR) {
-            const TaylorPlan p = taylor_plan(D.rho, B.members[i]->dt, E.opt.taylor_tol);
+            const TaylorPlan p = taylor_plan(D, B.members[i]->dt, E.opt.taylor_tol);
             m = std::max(m, p.m);
             K = std::max(K, p.K);
+            rec.resize((size_t)kTrec * n);
+            taylor_record(p, &rec[(size_t)kTrec * i]);
         }
     }
     B.rp.reset();
@@ -166,7 +169,8 @@ static void group_sync(Engine &E, BatchGroup &B, hipStream_t s, bool het, bool t
     ensure_dev(B.g.d_work, B.cap_work, w.size());
     ensure_dev(B.g.d_gwork, B.cap_gwork, gw.size());
     if (!B.g.d_kmax) HIP_OK(hipMalloc((void **)&B.g.d_kmax, sizeof(int)));
-    ensure_dev(B.g.d_trec, B.cap_trec, (size_t)2 * n);
+    ensure_dev(B.g.d_trec, B.cap_trec, (size_t)kTrec * n);
+    if (!rec.empty()) HIP_OK(hipMemcpy(B.g.d_trec, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
     if ((size_t)n * 8 > B.cap_flags) {
         if (B.d_flags) HIP_OK(hipFree(B.d_flags));
         if (B.h_flags) HIP_OK(hipHostFree(B.h_flags));
@@ -260,7 +264,7 @@ static void start_transient(Engine &E, EngJob &j) {
     auto kv = seq_slots(j.nsteps, j.dt);
     kv.push_back({S_YPAR, 0.0});
     if (E.opt.transient == HS_TRANS_TAYLOR) {
-        const auto t = taylor_slots(taylor_plan(D.rho, j.dt, E.opt.taylor_tol), D.rho);
+        const auto t = taylor_slots(taylor_plan(D, j.dt, E.opt.taylor_tol), D.rho);
         kv.insert(kv.end(), t.begin(), t.end());
     }
     write_scal(D, kv);

@@ -1801,18 +1801,19 @@ __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ 
         }
         case FIN_TAY: {
             // number of terms: the smallest K whose geometric tail bound from
-            // max|E_1| (||E_{k+1}|| <= ||E_k|| rho tau/(k+1)) is under tol
+            // max|E_1| (||E_{k+1}|| <= ||E_k|| z/(k+1)) is under tol, with
+            // z = rho tau, or sigma = rho tau / 2 for a shifted series
             if (a.sub >= (int)s[S_TK] || seq_finished(d, a.seq)) {
                 if (threadIdx.x == 0) {
-                    a.trec[2 * blockIdx.x] = 0.0;
-                    a.trec[2 * blockIdx.x + 1] = 0.0;
+                    a.trec[kTrec * blockIdx.x] = 0.0;
+                    a.trec[kTrec * blockIdx.x + 1] = 0.0;
                 }
                 return;
             }
             double w[1];
             reduce_slots<1, true>(d, w);
             if (threadIdx.x) return;
-            const double z = s[S_RHO] * s[S_TAU];
+            const double z = s[S_TSIG] > 0.0 ? s[S_TSIG] : s[S_RHO] * s[S_TAU];
             const int kplan = (int)s[S_KPLAN];
             double term = w[0];
             int K = 1;
@@ -1840,8 +1841,8 @@ __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ 
             } else {
                 s[S_TBYTES] += K >= 2 ? 8.0 + 12.0 * (K - 2) : 0.0;
             }
-            a.trec[2 * blockIdx.x] = s[S_TAU];
-            a.trec[2 * blockIdx.x + 1] = K;
+            a.trec[kTrec * blockIdx.x] = s[S_TAU];
+            a.trec[kTrec * blockIdx.x + 1] = K;
             atomicMax(a.kmax, K);
             return;
         }
@@ -1906,7 +1907,11 @@ __global__ void k_scatter(const ProbDesc *__restrict__ D, int nprob, Slots sl, c
 //   h = E_1 + (tau M/2)(E_1 + (tau M/3)(E_1 + ... + (tau M/K) E_1)),
 // one sweep per step h <- E_1 + (tau/j) M h for j = K, ..., 2: a sweep reads
 // h (with its halo) and E_1 and writes h, 12 bytes per FP32 node instead of
-// the 16 of a forward sweep that also carries the running sum.  E_1 comes
+// the 16 of a forward sweep that also carries the running sum.  Conduction
+// problems expand the series about the centre of M's spectrum instead (the
+// host's plan gives the factors): E_1 = c_0 tau f(y) and the step is
+// h <- E_1 + s_j M h + t_j h, with s_j, t_j from the problem's Taylor record
+// (tau/j and 0 unshifted).  E_1 comes
 // from the FP64 residual; h and E_1 are kept in TD (FP32 in the mixed form:
 // they are temperature changes, not temperatures).  K comes from max|E_1|
 // and the geometric tail bound (||C^-1 G||_inf <= rho), so the number of
@@ -1926,7 +1931,7 @@ __global__ void __launch_bounds__(kBlock) k_tay_first(const ProbDesc *__restrict
     const int ypar = (int)d.scal[S_YPAR];
     const double *y = d.b64[B_Y0 + ypar], *rhs = d.b64[B_RHS];
     TD *e1 = sizeof(TD) == 8 ? (TD *)d.b64[e1b] : (TD *)d.b32[e1b];
-    const double tau = d.scal[S_TAU];
+    const double tau = d.scal[S_TAU1];
     LdPlain<double> ld{y};
     double v[1] = {0};
     auto fin = [&](unsigned c, double f) {
@@ -1995,7 +2000,7 @@ __global__ void __launch_bounds__(kBlock) k_tay_first2(const ProbDesc *__restric
     const double *y = d.b64[B_Y0 + ypar], *rhs = d.b64[B_RHS];
     const float *pw = d.b32[F_PW];
     float *e1 = d.b32[e1b];
-    const double tau = d.scal[S_TAU];
+    const double tau = d.scal[S_TAU1];
     double v[1] = {0};
     if (wi.tile < d.ntiles) {
         const int tile = wi.tile, tpl = d.nti * d.ntj;
@@ -2113,6 +2118,8 @@ __device__ __forceinline__ int horner_result(int K, int e1b, int d0b, int d1b) {
 struct TayItem {
     int prob, tile, K;
     double tau;
+    const double *rec;   // the problem's record (step factors at [2 j], [2 j + 1])
+    const float *recf;   // the same in FP32
 };
 __device__ __forceinline__ TayItem tay_item(const WorkItem *__restrict__ W, int nper,
                                             const double *__restrict__ trec) {
@@ -2126,8 +2133,10 @@ __device__ __forceinline__ TayItem tay_item(const WorkItem *__restrict__ W, int 
         t.prob = wi.prob;
         t.tile = wi.tile;
     }
-    t.tau = trec[2 * t.prob];
-    t.K = (int)trec[2 * t.prob + 1];
+    t.rec = trec + kTrec * t.prob;
+    t.recf = (const float *)(t.rec + kTrecF);
+    t.tau = t.rec[0];
+    t.K = (int)t.rec[1];
     return t;
 }
 
@@ -2170,7 +2179,7 @@ __device__ __forceinline__ void add_state4(double *y, unsigned c, const float (&
 // With y given (the last step, j = 2), the result is the sub-step's
 // increment and goes into the FP64 state instead of hout.
 __device__ __forceinline__ void term_vec4(const ProbDesc &d, int tile, const float *hin, float *hout,
-                                          const float *e1, float s, const float (&ring)[4], double *y) {
+                                          const float *e1, float s, float sft, const float (&ring)[4], double *y) {
     const int ntj = term_ntj(d), tpl = d.t_nti * ntj;
     const int n = tile / tpl, rem = tile - n * tpl;
     const int ti = rem / ntj, tj = rem - ti * ntj;
@@ -2231,7 +2240,7 @@ __device__ __forceinline__ void term_vec4(const ProbDesc &d, int tile, const flo
             if (i == nr - 1 && rn[1] >= 0) flux += rgn * (ring[1] - xc);
             if (j == nc - 1 && rn[2] >= 0) flux += rge * (ring[2] - xc);
             if (j == 0 && rn[3] >= 0) flux += rge * (ring[3] - xc);
-            e[k] = comp(A[g], k) + s * (ci * (flux - gam * xc));
+            e[k] = comp(A[g], k) + s * (ci * (flux - gam * xc)) + sft * xc;
         }
         if (y) add_state4(y, cc[g], e);
         else st4(hout, cc[g], make_float4(e[0], e[1], e[2], e[3]));
@@ -2272,7 +2281,8 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict_
     auto buf = [&](int b) -> TD * { return sizeof(TD) == 8 ? (TD *)d.b64[b] : (TD *)d.b32[b]; };
     const TD *hin = buf(hb.in), *e1 = buf(e1b);
     TD *hout = buf(hb.out);
-    const TD s = (TD)(it.tau / j);
+    const TD s = sizeof(TD) == 4 ? (TD)it.recf[2 * j] : (TD)it.rec[2 * j];
+    const TD sft = sizeof(TD) == 4 ? (TD)it.recf[2 * j + 1] : (TD)it.rec[2 * j + 1];
     LdPlain<TD> ld{hin};
     const WorkItem wi{it.prob, it.tile};
     if (wi.tile < d.ntiles) {
@@ -2295,7 +2305,7 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict_
                 TD flux, diag, gam;
                 stencil<TD, HET>(d, q, t.p, ring, flux, diag, gam);
                 const TD ci = HET ? t.ci : (TD)d.cinv[q.n];
-                st_at(hout, q.c, t.e1 + s * (ci * (flux - gam * t.p.c)));
+                st_at(hout, q.c, t.e1 + s * (ci * (flux - gam * t.p.c)) + sft * t.p.c);
             });
     } else {
         __shared__ TD xf[kMaxExtra], xd[kMaxExtra];
@@ -2303,7 +2313,7 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict_
         const int kk = threadIdx.x;
         if (kk < d.nx) {
             const long c = (long)d.nl * d.nr * d.nc + kk;
-            hout[c] = e1[c] + s * ((TD)d.x_cinv[kk] * (xf[kk] - (TD)d.x_gamb[kk] * hin[c]));
+            hout[c] = e1[c] + s * ((TD)d.x_cinv[kk] * (xf[kk] - (TD)d.x_gamb[kk] * hin[c])) + sft * hin[c];
         }
     }
 }
@@ -2337,19 +2347,19 @@ __global__ void __launch_bounds__(kBlock) k_tay_term4(const ProbDesc *__restrict
     const HornerBufs hb = horner_bufs(it.K, j, e1b, d0b, d1b);
     const float *hin = d.b32[hb.in];
     float *hout = d.b32[hb.out];
-    const float s = (float)(it.tau / j);
+    const float s = it.recf[2 * j], sft = it.recf[2 * j + 1];
     LdPlain<float> ld{hin};
     if (it.tile < d.t_ntiles) {
         float ring[4];
         load_ring<float>(d, ld, term_layer(d, it.tile), ring);
-        term_vec4(d, it.tile, hin, hout, e1, s, ring, y);
+        term_vec4(d, it.tile, hin, hout, e1, s, sft, ring, y);
     } else {
         __shared__ float xf[kMaxExtra], xd[kMaxExtra];
         package_terms<float>(d, ld, true, xf, xd);
         const int kk = threadIdx.x;
         if (kk < d.nx) {
             const unsigned c = (unsigned)d.nl * d.nr * d.nc + kk;
-            const float v = e1[c] + s * ((float)d.x_cinv[kk] * (xf[kk] - (float)d.x_gamb[kk] * hin[c]));
+            const float v = e1[c] + s * ((float)d.x_cinv[kk] * (xf[kk] - (float)d.x_gamb[kk] * hin[c])) + sft * hin[c];
             if (fin) y[c] += (double)v;
             else hout[c] = v;
         }
@@ -2375,13 +2385,13 @@ constexpr int kPX = 64, kPY = 8;
 
 struct PairPlan {
     int in, out, mode;   // mode 2: steps j and j-1; 1: step j-1 only
-    float s1, s2;
+    float s1, s2, t1, t2;   // factors of steps j and j-1 (Taylor record)
 };
 
 // The pair of top step j for a problem of K terms in a group of plan kg:
 // false when the problem takes no step of it.  Buffers alternate between the
 // problem's active launches, the first reading E_1.
-__device__ __forceinline__ bool pair_plan(int K, int kg, int j, double tau, int e1b, int d0b, int d1b,
+__device__ __forceinline__ bool pair_plan(int K, int kg, int j, const double *rec, int e1b, int d0b, int d1b,
                                           PairPlan &p) {
     if (K < 2 || j - 1 > K) return false;
     const bool single = (kg % 2 == 0) && K == kg;   // step K ran alone first
@@ -2391,8 +2401,11 @@ __device__ __forceinline__ bool pair_plan(int K, int kg, int j, double tau, int 
     p.mode = j <= K ? 2 : 1;
     p.in = idx == 0 ? e1b : (((idx - 1) & 1) ? d1b : d0b);
     p.out = (idx & 1) ? d1b : d0b;
-    p.s1 = (float)(tau / j);
-    p.s2 = (float)(tau / (j - 1));
+    const float *rf = (const float *)(rec + kTrecF);
+    p.s1 = rf[2 * j];
+    p.t1 = rf[2 * j + 1];
+    p.s2 = rf[2 * (j - 1)];
+    p.t2 = rf[2 * (j - 1) + 1];
     return true;
 }
 
@@ -2439,7 +2452,8 @@ __device__ __forceinline__ bool has_ring(const LayerCoef &L) {
 
 // One Horner step at a cell from its 6 neighbours (missing ones excluded), in
 // the order of stencil(); ring terms when ring != nullptr.
-__device__ __forceinline__ float horner_cell(const LayerCoef &L, float e1, float s, float xc, float xn, float xs,
+__device__ __forceinline__ float horner_cell(const LayerCoef &L, float e1, float s, float sft, float xc, float xn,
+                                             float xs,
                                              float xe, float xw, float xu, float xd, bool hn, bool hs, bool he,
                                              bool hw, bool hu, bool hd, const float *ring, int i, int j, int nr,
                                              int nc) {
@@ -2456,7 +2470,7 @@ __device__ __forceinline__ float horner_cell(const LayerCoef &L, float e1, float
         if (j == nc - 1 && L.rn[2] >= 0) flux += L.rge * (ring[2] - xc);
         if (j == 0 && L.rn[3] >= 0) flux += L.rge * (ring[3] - xc);
     }
-    return e1 + s * (L.ci * (flux - L.gam * xc));
+    return e1 + s * (L.ci * (flux - L.gam * xc)) + sft * xc;
 }
 
 // 8 workgroups per CU (64 VGPRs, ~20 KB of LDS each): a 16-job batch of
@@ -2470,7 +2484,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_tay_pair(const ProbDesc *__restri
     if (j - 1 > *kmax) return;
     const TayItem it = tay_item(W, nper, trec);
     PairPlan pp;
-    if (!pair_plan(it.K, kg, j, it.tau, e1b, d0b, d1b, pp)) return;
+    if (!pair_plan(it.K, kg, j, it.rec, e1b, d0b, d1b, pp)) return;
     const ProbDesc &d = D[it.prob];
     const int nr = d.nr, nc = d.nc, nl = d.nl;
     const int PX = min(kPX, nc), PY = min(kPY, nr);
@@ -2549,13 +2563,13 @@ __global__ void __launch_bounds__(kBlock, 8) k_tay_pair(const ProbDesc *__restri
             const LayerCoef &L = LC[m];
             const float *ring = XR[m];
             const int sm = m % 3, su = (m + 2) % 3, sd = (m + 1) % 3;
-            const float s = pp.mode == 2 ? pp.s1 : pp.s2;
+            const float s = pp.mode == 2 ? pp.s1 : pp.s2, sft = pp.mode == 2 ? pp.t1 : pp.t2;
             const bool hu = m > 0, hd = m < nl - 1;
             for (int k = 0; k < 4; k++) {
                 const int r = cr[k], c = cc[k];
                 const int gi = i0 - 1 + r, gj = j0 - 1 + c;
                 const float xc = H0[sm][r + 1][c + 1];
-                const float v = horner_cell(L, pc[k], s, xc, H0[sm][r][c + 1], H0[sm][r + 2][c + 1],
+                const float v = horner_cell(L, pc[k], s, sft, xc, H0[sm][r][c + 1], H0[sm][r + 2][c + 1],
                                             H0[sm][r + 1][c + 2], H0[sm][r + 1][c], H0[su][r + 1][c + 1],
                                             H0[sd][r + 1][c + 1], gi > 0, gi < nr - 1, gj < nc - 1, gj > 0, hu, hd,
                                             ring, gi, gj, nr, nc);
@@ -2583,7 +2597,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_tay_pair(const ProbDesc *__restri
                 if (pp.mode == 1) {
                     v = H1[sm][rr + 1][cq + 1];
                 } else {
-                    v = horner_cell(L, pe[k], pp.s2, H1[sm][rr + 1][cq + 1], H1[sm][rr][cq + 1], H1[sm][rr + 2][cq + 1],
+                    v = horner_cell(L, pe[k], pp.s2, pp.t2, H1[sm][rr + 1][cq + 1], H1[sm][rr][cq + 1], H1[sm][rr + 2][cq + 1],
                                     H1[sm][rr + 1][cq + 2], H1[sm][rr + 1][cq], H1[su][rr + 1][cq + 1],
                                     H1[sd][rr + 1][cq + 1], gi > 0, gi < nr - 1, gj < nc - 1, gj > 0, hu, hd,
                                     nullptr, gi, gj, nr, nc);
@@ -2608,7 +2622,7 @@ __global__ void __launch_bounds__(kBlock) k_tay_pair_fix(const ProbDesc *__restr
     if (j - 1 > *kmax) return;
     const int p = blockIdx.x / kPairFixBlocks, blk = blockIdx.x - p * kPairFixBlocks;
     PairPlan pp;
-    if (!pair_plan((int)trec[2 * p + 1], kg, j, trec[2 * p], e1b, d0b, d1b, pp)) return;
+    if (!pair_plan((int)trec[kTrec * p + 1], kg, j, trec + kTrec * p, e1b, d0b, d1b, pp)) return;
     const ProbDesc &d = D[p];
     const int nr = d.nr, nc = d.nc, nl = d.nl, nx = d.nx;
     const unsigned plane = (unsigned)nr * nc, ncell = plane * nl;
@@ -2623,7 +2637,8 @@ __global__ void __launch_bounds__(kBlock) k_tay_pair_fix(const ProbDesc *__restr
         package_terms<float>(d, ld, true, xf, xd);
         const int k = threadIdx.x;
         if (k < nx)
-            out[ncell + k] = e1[ncell + k] + pp.s2 * ((float)d.x_cinv[k] * (xf[k] - (float)d.x_gamb[k] * h0[ncell + k]));
+            out[ncell + k] = e1[ncell + k] + pp.s2 * ((float)d.x_cinv[k] * (xf[k] - (float)d.x_gamb[k] * h0[ncell + k])) +
+                             pp.t2 * h0[ncell + k];
         return;
     }
     // ring sums of h0 and of the saved h1 over each package node's boundary row
@@ -2658,7 +2673,7 @@ __global__ void __launch_bounds__(kBlock) k_tay_pair_fix(const ProbDesc *__restr
             if (d.x_cpl_steady_only[k][q]) continue;
             flux += (float)d.x_cpl_g[k][q] * (h0[ncell + d.x_cpl_node[k][q]] - xk);
         }
-        h1x[k] = e1[ncell + k] + pp.s1 * ((float)d.x_cinv[k] * (flux - (float)d.x_gamb[k] * xk));
+        h1x[k] = e1[ncell + k] + pp.s1 * ((float)d.x_cinv[k] * (flux - (float)d.x_gamb[k] * xk)) + pp.t1 * xk;
     }
     __syncthreads();
     // ring terms of h2 on the perimeter of every ring layer, cells spread
@@ -2696,7 +2711,8 @@ __global__ void __launch_bounds__(kBlock) k_tay_pair_fix(const ProbDesc *__restr
             if (d.x_cpl_steady_only[k][q]) continue;
             flux += (float)d.x_cpl_g[k][q] * (h1x[d.x_cpl_node[k][q]] - h1x[k]);
         }
-        out[ncell + k] = e1[ncell + k] + pp.s2 * ((float)d.x_cinv[k] * (flux - (float)d.x_gamb[k] * h1x[k]));
+        out[ncell + k] = e1[ncell + k] + pp.s2 * ((float)d.x_cinv[k] * (flux - (float)d.x_gamb[k] * h1x[k])) +
+                         pp.t2 * h1x[k];
     }
 }
 

@@ -93,6 +93,7 @@ struct DevModel {
     long vlen = 0;
     double rho = 0.0;                   // bound on ||C^-1 G||_inf (transient)
     bool hetero = false, has_c = false;
+    std::vector<double> trec_plan;      // Taylor record of grp.d_trec as last uploaded (step factors)
     size_t cap_twork = 0;               // grp.d_twork capacity (its own allocation)
     bool term_tuned = false;            // term tiling chosen by timing
     // backward Euler: descriptor of the shifted operator G + C/h (its group
@@ -265,10 +266,14 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap)
     for (const ProbDesc &h : desc) min_nc = std::min(min_nc, h.nc);
     // every problem active for one step j = 2 (its first: E_1 buffer -> term buffer)
     const int n = (int)desc.size();
-    std::vector<double> rec(2 * n);
+    // the group's Taylor records (step factors) are restored after timing
+    std::vector<double> saved((size_t)kTrec * n), rec((size_t)kTrec * n, 0.0);
+    HIP_OK(hipMemcpy(saved.data(), G.d_trec, saved.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int i = 0; i < n; i++) {
-        rec[2 * i] = 1e-9;
-        rec[2 * i + 1] = 2.0;
+        rec[(size_t)kTrec * i] = 1e-9;
+        rec[(size_t)kTrec * i + 1] = 2.0;
+        rec[(size_t)kTrec * i + 4] = 1e-9 / 2;   // s_2 (unshifted)
+        ((float *)&rec[(size_t)kTrec * i + kTrecF])[4] = (float)(1e-9 / 2);
     }
     const int two = 2;
     HIP_OK(hipMemcpy(G.d_trec, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -304,6 +309,7 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap)
         }
     HIP_OK(hipEventDestroy(e0));
     HIP_OK(hipEventDestroy(e1));
+    HIP_OK(hipMemcpy(G.d_trec, saved.data(), saved.size() * sizeof(double), hipMemcpyHostToDevice));
     cache[key] = best;
     apply(best);
 }
@@ -768,7 +774,7 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
     D->grp.hetero = M.detailed;
     D->grp.vec4 = vec4_ok(h);
     D->grp.d_kmax = D->alloc<int>(1);
-    D->grp.d_trec = D->alloc<double>(2);
+    D->grp.d_trec = D->alloc<double>(kTrec);
     D->grp.nper = D->nwork;
     {
         std::vector<WorkItem> tw;
@@ -1380,19 +1386,54 @@ static void transient_rk4(Model &M, DevModel &D, double dt) {
     M.stats.rk4_calls = (int)D.h_scal[S_CALLS];
 }
 
-// Taylor set-up: sub-steps keep rho*tau <= 4; the term count is the a-priori
-// bound for a 1000 K change per sub-step, and the device stops each series
-// early once its actual tail bound is below the tolerance.
+// Taylor set-up: sub-steps keep z = rho*tau <= 4; the term count is the
+// a-priori bound for a 1000 K change per sub-step, and the device stops each
+// series early once its actual tail bound is below the tolerance.
+//
+// The sub-step's increment is tau phi_1(-tau A) f(y), A = C^-1 G, phi_1(x) =
+// (e^x - 1)/x.  Unshifted, the series is Taylor's about x = 0 and its terms
+// shrink like z^k/(k+1)!.  When every row of A is weakly diagonally dominant
+// (conduction, no coolant advection), the series is expanded about the
+// centre of the spectrum instead, x0 = -sigma with sigma = z/2:
+//   phi_1(X) = sum_k c_k (X + sigma I)^k,  c_k = (1/k!) int_0^1 s^k e^{-sigma s} ds,
+// and ||X + sigma I||_inf <= z/2 (row i: |sigma - a_ii tau| + sum_j |a_ij| tau
+// <= z/2 when a_ii + sum_j |a_ij| <= rho and sum_j |a_ij| <= a_ii).  Since
+// c_k <= c_0/(k+1)! (Chebyshev's integral inequality: s^k rises where
+// e^{-sigma s} falls), the tail after K terms is bounded by the unshifted
+// rule with z replaced by z/2, so a series needs about half the terms' worth
+// of z: 8-9 instead of 12-13 Horner steps at z = 2.3.  In Horner form with
+// E_1 = c_0 tau f(y) (S_TAU1) and h_k scaled by c_k/c_0, step j (k = j - 2)
+// is h <- E_1 + r_k (tau M h + sigma h), r_k = c_{k+1}/c_k: the Taylor record
+// holds s_j = r_k tau and t_j = r_k sigma.  Unshifted, r_k = 1/(k+2) (s_j =
+// tau/j, t_j = 0) and c_0 = 1.
 struct TaylorPlan {
     int m = 1, K = 1;
     double tau = 0.0, tol = 0.0;
+    double sigma = 0.0, tau1 = 0.0;   // shift of the series; factor of the slope in E_1
 };
-static TaylorPlan taylor_plan(double rho, double dt, double taylor_tol) {
+static bool taylor_shift_enabled() {
+    static const bool off = getenv("HS_TAY_SHIFT") && atoi(getenv("HS_TAY_SHIFT")) == 0;
+    return !off;
+}
+// k! c_k = sum_i (-sigma)^i / (i! (k + i + 1)); sigma <= 2, so 40 terms reach
+// the double rounding
+static double shifted_phi_coef(int k, double sigma) {
+    double term = 1.0, sum = 0.0;
+    for (int i = 0; i < 40; i++) {
+        sum += term / (k + i + 1);
+        term *= -sigma / (i + 1);
+    }
+    return sum;
+}
+static TaylorPlan taylor_plan(const DevModel &D, double dt, double taylor_tol) {
     TaylorPlan p;
     const double zmax = 4.0;
-    p.m = std::max(1, (int)std::ceil(rho * dt / zmax));
+    p.m = std::max(1, (int)std::ceil(D.rho * dt / zmax));
     p.tau = dt / p.m;
-    const double z = rho * p.tau;
+    const bool shift = taylor_shift_enabled() && !D.h.has_adv;
+    p.sigma = shift ? 0.5 * D.rho * p.tau : 0.0;
+    p.tau1 = shift ? p.tau * shifted_phi_coef(0, p.sigma) : p.tau;
+    const double z = shift ? p.sigma : D.rho * p.tau;
     p.tol = taylor_tol / p.m;
     p.K = 1;
     double term = 1000.0;
@@ -1407,7 +1448,41 @@ static TaylorPlan taylor_plan(double rho, double dt, double taylor_tol) {
 
 // scalar slots of a problem's Taylor plan
 static std::vector<std::pair<int, double>> taylor_slots(const TaylorPlan &p, double rho) {
-    return {{S_TAU, p.tau}, {S_TK, (double)p.m}, {S_RHO, rho}, {S_TTOL, p.tol}, {S_KPLAN, (double)p.K}};
+    return {{S_TAU, p.tau}, {S_TK, (double)p.m}, {S_RHO, rho}, {S_TTOL, p.tol}, {S_KPLAN, (double)p.K},
+            {S_TAU1, p.tau1}, {S_TSIG, p.sigma}};
+}
+
+// step factors of a problem's Taylor record (kTrec doubles; [0] and [1] are
+// the device's)
+static void taylor_record(const TaylorPlan &p, double *rec) {
+    std::fill(rec, rec + kTrec, 0.0);
+    float *rf = (float *)(rec + kTrecF);
+    double ck = shifted_phi_coef(0, p.sigma);
+    for (int j = 2; j <= std::min(p.K, kTayMaxK); j++) {
+        if (p.sigma == 0.0) {
+            rec[2 * j] = p.tau / j;
+        } else {
+            const int k = j - 2;
+            const double ck1 = shifted_phi_coef(k + 1, p.sigma);
+            const double r = ck1 / (ck * (k + 1));
+            rec[2 * j] = r * p.tau;
+            rec[2 * j + 1] = r * p.sigma;
+            ck = ck1;
+        }
+        rf[2 * j] = (float)rec[2 * j];
+        rf[2 * j + 1] = (float)rec[2 * j + 1];
+    }
+}
+
+// a single model's record, uploaded when its plan changes
+static void upload_taylor_record(DevModel &D, const TaylorPlan &p) {
+    std::vector<double> rec(kTrec);
+    taylor_record(p, rec.data());
+    if (rec == D.trec_plan) return;
+    D.trec_plan = rec;
+    HIP_OK(hipMemcpyAsync(D.grp.d_trec, D.trec_plan.data(), kTrec * sizeof(double), hipMemcpyHostToDevice,
+                          D.stream));
+    HIP_OK(hipStreamSynchronize(D.stream));
 }
 
 // One interval of every problem: m sub-steps of at most K terms (the largest
@@ -1468,8 +1543,9 @@ static void taylor_interval(const Group &G, int m, int K, int seq) {
 
 template <typename TD, bool HET>
 static void transient_taylor(Model &M, DevModel &D, double dt) {
-    const TaylorPlan p = taylor_plan(D.rho, dt, M.opt.taylor_tol);
+    const TaylorPlan p = taylor_plan(D, dt, M.opt.taylor_tol);
     write_scal(D, taylor_slots(p, D.rho));
+    upload_taylor_record(D, p);
     taylor_interval<TD, HET>(D.grp, p.m, p.K, 0);
     M.stats.taylor_terms = p.K;
     M.stats.taylor_substeps = p.m;
@@ -1726,8 +1802,9 @@ void dev_transient_run(Model &M, const double *power_seq, int steps, double dt, 
     write_scal(D, seq_slots(steps, dt));
     SeqForm f{M.opt.precision, M.opt.transient, 1, 1};
     if (M.opt.transient == HS_TRANS_TAYLOR) {
-        const TaylorPlan p = taylor_plan(D.rho, dt, M.opt.taylor_tol);
+        const TaylorPlan p = taylor_plan(D, dt, M.opt.taylor_tol);
         write_scal(D, taylor_slots(p, D.rho));
+        upload_taylor_record(D, p);
         f.m = p.m;
         f.K = p.K;
         M.stats.taylor_terms = p.K;

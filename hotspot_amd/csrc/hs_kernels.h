@@ -26,6 +26,17 @@ constexpr int kWave = 64;
 constexpr int kRed = 4;   // values reduced per slot
 constexpr int kGmM = 32;  // GMRES restart length (non-symmetric steady solves)
 
+// Per-problem Taylor record of a group (doubles): [0] the sub-step tau and
+// [1] its term count (0: nothing to do), written by the finish of the first
+// term; then, per Horner step j = 2..kTayMaxK, the factors of the step
+//   h <- E_1 + s_j M h + t_j h,   M = -C^-1 G,
+// at [2 j] (s_j) and [2 j + 1] (t_j), written by the host with the plan,
+// and the same factors rounded to FP32 at float index 2 j and 2 j + 1 from
+// [kTrecF] on (uniform FP32 operands for the FP32 sweeps).
+constexpr int kTayMaxK = 60;
+constexpr int kTrecF = 2 * (kTayMaxK + 1);
+constexpr int kTrec = kTrecF + (kTayMaxK + 1);
+
 // FP64 buffers of a problem
 enum Buf64 {
     B_STEADY = 0,  // steady solution (last_steady)
@@ -64,9 +75,11 @@ enum Scal {
     S_TTERMS,                    // Taylor: term sweeps (E_k -> E_k+1) summed over a sequence
     S_MGINIT,                    // multigrid-preconditioned inner solve: first V-cycle pending
     S_TBYTES,                    // Taylor: algorithmic bytes per node of the dominant sweep kernel, summed
+    S_TAU1,                      // Taylor: factor of the slope in E_1 (tau phi_1(-sigma); tau unshifted)
+    S_TSIG,                      // Taylor: shift sigma = rho tau / 2 of the series (0: unshifted)
     NSCAL = 48
 };
-static_assert(S_TBYTES < NSCAL, "scalar slots");
+static_assert(S_TSIG < NSCAL, "scalar slots");
 enum RkStatus { RK_NEED_K1 = 0, RK_ATTEMPT = 1, RK_DONE = 2 };
 
 struct ProbDesc {
