@@ -168,7 +168,9 @@ static void group_sync(Engine &E, BatchGroup &B, hipStream_t s, bool het, bool t
     ensure_dev(B.g.d_desc, B.cap_desc, (size_t)n);
     ensure_dev(B.g.d_work, B.cap_work, w.size());
     ensure_dev(B.g.d_gwork, B.cap_gwork, gw.size());
-    if (!B.g.d_kmax) HIP_OK(hipMalloc((void **)&B.g.d_kmax, sizeof(int)));
+    if (!B.g.d_kmax) HIP_OK(hipMalloc((void **)&B.g.d_kmax, 2 * sizeof(int)));
+    HIP_OK(hipMemset(B.g.d_kmax, 0, 2 * sizeof(int)));
+    B.g.kgraph = 0;
     ensure_dev(B.g.d_trec, B.cap_trec, (size_t)kTrec * n);
     if (!rec.empty()) HIP_OK(hipMemcpy(B.g.d_trec, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
     if ((size_t)n * 8 > B.cap_flags) {
@@ -511,6 +513,25 @@ int eng_tick(Engine &E) {
     }
     HIP_OK(hipStreamSynchronize(E.s_steady));
     HIP_OK(hipStreamSynchronize(E.s_trans));
+    // Taylor groups of one sub-step per interval launch the Horner steps
+    // that the tick's largest term count needs, plus one: fewer launches that
+    // skip every problem.  A problem that needs more is held (its interval
+    // not advanced) until the next tick raises the count.  A change
+    // recaptures the group's graph; the count falls only by two or more.
+    for (int h = 0; h < 2; h++) {
+        BatchGroup &B = E.trans[h];
+        if (!ran_trans[h] || B.form.transient != HS_TRANS_TAYLOR || B.form.m != 1 || !kgraph_enabled()) continue;
+        int kt[2] = {0, 0};
+        HIP_OK(hipMemcpy(kt, B.g.d_kmax, sizeof kt, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemsetAsync(B.g.d_kmax + 1, 0, sizeof(int), E.s_trans));
+        if (kt[1] <= 0) continue;
+        const int cur = B.g.kgraph > 0 ? B.g.kgraph : B.form.K;
+        const int want = std::min(B.form.K, std::max(2, kt[1] + 1));
+        if (kt[1] > cur || want < cur - 1) {
+            B.g.kgraph = want >= B.form.K ? 0 : want;
+            B.rp.reset();
+        }
+    }
     for (auto &kv : E.steady) {
         BatchGroup &B = kv.second;
         std::vector<std::pair<EngJob *, std::vector<double>>> fin;

@@ -998,7 +998,7 @@ enum FinOp { FIN_CG_INIT = 0, FIN_ALPHA, FIN_BETA, FIN_IR, FIN_TAY, FIN_GM_IR, F
 struct FinArgs {
     double rtol2, inner_rtol2;
     int max_iters, max_refine, done_slot, tol_slot, sub, seq, mg;
-    int *kmax;     // FIN_TAY: largest term count of the group
+    int *kmax;     // FIN_TAY: [0] largest term count of the group's sub-step, [1] the same since the host cleared it
     double *trec;  // FIN_TAY: per problem {tau, terms (0: no sub-step)}
     int kg;        // FIN_TAY: the group's plan (launches of the sub-step)
     int pairs;     // FIN_TAY: steps run two per sweep (k_tay_pair)
@@ -1823,8 +1823,20 @@ __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ 
                 if (q < 1.0 && term / (1.0 - q) <= s[S_TTOL]) break;
                 K++;
             }
+            atomicMax(a.kmax + 1, K);   // largest since the host last cleared it, held problems included
             s[S_TMAX] = w[0];
             s[S_KNEED] = K;
+            // A group launching fewer steps than this problem needs (kg < K,
+            // single sub-step groups only) holds the problem: no step takes
+            // it, its state stays, and its interval is not advanced, so it
+            // runs the same interval again once the host has raised kg.
+            const bool hold = K > a.kg;
+            s[S_THOLD] = hold ? 1.0 : 0.0;
+            if (hold) {
+                a.trec[kTrec * blockIdx.x] = s[S_TAU];
+                a.trec[kTrec * blockIdx.x + 1] = 0.0;
+                return;
+            }
             s[S_TTERMS] += K - 1;
             // algorithmic bytes per node of the dominant sweeps: 12 (read h
             // and E_1, write h), 8 when h is E_1 itself (a problem's first)
@@ -1876,7 +1888,7 @@ __global__ void k_rk_end(const ProbDesc *__restrict__ D, int nprob) {
 __global__ void k_step_advance(const ProbDesc *__restrict__ D, int nprob) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nprob) return;
-    if (seq_finished(D[p], 1) || D[p].scal[S_STATUS] != (double)RK_DONE) return;
+    if (seq_finished(D[p], 1) || D[p].scal[S_STATUS] != (double)RK_DONE || D[p].scal[S_THOLD] != 0.0) return;
     D[p].scal[S_STEP] += 1.0;
 }
 
@@ -2121,10 +2133,9 @@ struct TayItem {
     const double *rec;   // the problem's record (step factors at [2 j], [2 j + 1])
     const float *recf;   // the same in FP32
 };
-__device__ __forceinline__ TayItem tay_item(const WorkItem *__restrict__ W, int nper,
-                                            const double *__restrict__ trec) {
+__device__ __forceinline__ TayItem tay_item_at(int b, const WorkItem *__restrict__ W, int nper,
+                                               const double *__restrict__ trec) {
     TayItem t;
-    const int b = work_index();
     if (nper > 0) {
         t.prob = b / nper;
         t.tile = b - t.prob * nper;
@@ -2138,6 +2149,10 @@ __device__ __forceinline__ TayItem tay_item(const WorkItem *__restrict__ W, int 
     t.tau = t.rec[0];
     t.K = (int)t.rec[1];
     return t;
+}
+__device__ __forceinline__ TayItem tay_item(const WorkItem *__restrict__ W, int nper,
+                                            const double *__restrict__ trec) {
+    return tay_item_at(work_index(), W, nper, trec);
 }
 
 // FP32 Horner step of a homogeneous problem with float4 accesses: within the
@@ -2326,12 +2341,8 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict_
 // step's result -- the sub-step's increment -- is added to the FP64 state
 // in place of k_tay_final, and a problem whose series has one term adds
 // E_1 in the same launch.
-__global__ void __launch_bounds__(kBlock) k_tay_term4(const ProbDesc *__restrict__ D,
-                                                      const WorkItem *__restrict__ W, int nper, int e1b,
-                                                      int d0b, int d1b, int j, const int *__restrict__ kmax,
-                                                      const double *__restrict__ trec, int fin) {
-    if (!fin && j > *kmax) return;
-    const TayItem it = tay_item(W, nper, trec);
+__device__ __forceinline__ void term4_item(const ProbDesc *__restrict__ D, const TayItem &it, int e1b, int d0b,
+                                           int d1b, int j, int fin) {
     if (it.K == 0 || (!fin && j > it.K)) return;
     const ProbDesc &d = D[it.prob];
     const float *e1 = d.b32[e1b];
@@ -2364,6 +2375,14 @@ __global__ void __launch_bounds__(kBlock) k_tay_term4(const ProbDesc *__restrict
             else hout[c] = v;
         }
     }
+}
+
+__global__ void __launch_bounds__(kBlock) k_tay_term4(const ProbDesc *__restrict__ D,
+                                                      const WorkItem *__restrict__ W, int nper, int e1b,
+                                                      int d0b, int d1b, int j, const int *__restrict__ kmax,
+                                                      const double *__restrict__ trec, int fin) {
+    if (!fin && j > *kmax) return;
+    term4_item(D, tay_item(W, nper, trec), e1b, d0b, d1b, j, fin);
 }
 
 // ---------------------------------------------------------------------------

@@ -19,7 +19,8 @@ struct Group {
     hipStream_t stream = nullptr;
     bool hetero = false;
     bool vec4 = false;   // every problem takes float4 term sweeps (vec4_ok)
-    int *d_kmax = nullptr;   // Taylor: largest term count of the current sub-step
+    int *d_kmax = nullptr;   // Taylor: largest term count of the current sub-step, and since last cleared
+    int kgraph = 0;          // Taylor: Horner steps launched per sub-step, below the plan (0: the plan's)
     double *d_trec = nullptr;   // Taylor: per problem {tau, terms} of the current sub-step
     int nper = 0;   // work items per problem when every problem has the same count, else 0
     // work-list of the float4 Taylor sweep over the term tiling (t_*)
@@ -38,6 +39,13 @@ struct Group {
 static bool pairs_enabled() {
     const char *e = getenv("HS_PAIRS");
     return e && atoi(e) == 1;
+}
+
+// Engine groups launch the Horner steps their recent term counts need
+// (kgraph); HS_FULL_PLAN=1 launches every step of the plan.
+static bool kgraph_enabled() {
+    static const bool off = getenv("HS_FULL_PLAN") && atoi(getenv("HS_FULL_PLAN")) == 1;
+    return !off;
 }
 
 // Work-list of the paired sweeps: (problem, patch) over kPX x kPY patches
@@ -773,7 +781,8 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
     D->grp.stream = D->stream;
     D->grp.hetero = M.detailed;
     D->grp.vec4 = vec4_ok(h);
-    D->grp.d_kmax = D->alloc<int>(1);
+    D->grp.d_kmax = D->alloc<int>(2);
+    HIP_OK(hipMemset(D->grp.d_kmax, 0, 2 * sizeof(int)));
     D->grp.d_trec = D->alloc<double>(kTrec);
     D->grp.nper = D->nwork;
     {
@@ -1449,7 +1458,7 @@ static TaylorPlan taylor_plan(const DevModel &D, double dt, double taylor_tol) {
 // scalar slots of a problem's Taylor plan
 static std::vector<std::pair<int, double>> taylor_slots(const TaylorPlan &p, double rho) {
     return {{S_TAU, p.tau}, {S_TK, (double)p.m}, {S_RHO, rho}, {S_TTOL, p.tol}, {S_KPLAN, (double)p.K},
-            {S_TAU1, p.tau1}, {S_TSIG, p.sigma}};
+            {S_TAU1, p.tau1}, {S_TSIG, p.sigma}, {S_THOLD, 0.0}};
 }
 
 // step factors of a problem's Taylor record (kTrec doubles; [0] and [1] are
@@ -1490,7 +1499,10 @@ static void upload_taylor_record(DevModel &D, const TaylorPlan &p) {
 // j = K..2 alternate between the two term buffers, and each problem takes the
 // last of them that its own term count needs.
 template <typename TD, bool HET>
-static void taylor_interval(const Group &G, int m, int K, int seq) {
+static void taylor_interval(const Group &G, int m, int Kplan, int seq) {
+    // the steps launched: the plan's, or fewer where the engine has set
+    // kgraph (a problem needing more is held, see FIN_TAY)
+    const int K = G.kgraph > 0 ? std::min(Kplan, G.kgraph) : Kplan;
     const ProbDesc *dd = G.d_desc;
     const WorkItem *w = G.d_work;
     const bool f32 = sizeof(TD) == 4;

@@ -77,9 +77,10 @@ enum Scal {
     S_TBYTES,                    // Taylor: algorithmic bytes per node of the dominant sweep kernel, summed
     S_TAU1,                      // Taylor: factor of the slope in E_1 (tau phi_1(-sigma); tau unshifted)
     S_TSIG,                      // Taylor: shift sigma = rho tau / 2 of the series (0: unshifted)
+    S_THOLD,                     // Taylor: interval held (the group launched fewer steps than it needs)
     NSCAL = 48
 };
-static_assert(S_TSIG < NSCAL, "scalar slots");
+static_assert(S_THOLD < NSCAL, "scalar slots");
 enum RkStatus { RK_NEED_K1 = 0, RK_ATTEMPT = 1, RK_DONE = 2 };
 
 struct ProbDesc {

@@ -280,6 +280,35 @@ def test_backward_euler_sequence_equals_calls(lib):
     b.free()
 
 
+def test_reduced_step_count_equals_full_plan(lib):
+    """Once a tick has shown the term counts its jobs take, an engine group
+    launches only the Horner steps they need (plus one); a power step (x3
+    from interval 40 on) makes the jobs need more, so they are held for the
+    rest of the tick and run the interval again after the group has raised
+    its count.  The results equal the single-model sequences (every step of
+    the plan launched) bit for bit."""
+    runs, seqs, inits = [], [], []
+    for k in range(2):
+        c, pw = cases.stacked_case(256, seed=20 + k, blocks_per_die=16)
+        r = make(lib, c)
+        set_options(lib, r, precision=1, transient=1)
+        P = r.power_vector(pw)
+        seq = np.array(cases.jittered_powers(P, 60, seed=30 + k))
+        seq[40:] *= 3.0
+        runs.append(r)
+        seqs.append(seq)
+        inits.append(np.full(r.n, 330.0) + cases.rng(40 + k).uniform(0, 5, r.n))
+    eng = engine.Engine(lib, engine.MIXED, engine.TAYLOR)
+    eng.set_tick(10, 128)
+    ids = [eng.submit(r.model, s, init=i, n_steps=len(s), dt=DT) for r, s, i in zip(runs, seqs, inits)]
+    eng.run()
+    served = [eng.result_seq(i) for i in ids]
+    eng.close()
+    for r, s, i, got in zip(runs, seqs, inits, served):
+        assert np.array_equal(transient_run(lib, r, s, i), got)
+        r.free()
+
+
 def test_paired_sweeps_match_single_sweeps(lib, monkeypatch):
     """The paired Taylor sweeps (two Horner steps per pass, HS_PAIRS=1) give
     the single sweeps' result up to FP32 rounding (the ring terms of the
