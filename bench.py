@@ -441,17 +441,21 @@ def main():
     t_step = float(np.mean(times))
     value = args.batch * updates_job / t_step
 
-    # roofline pass: the same batch with every launch of the dominant kernel
-    # bracketed by device events on the engine's stream (plain launches
-    # instead of graph replay while the timer is on)
-    # The dominant kernel is the paired Horner sweep k_tay_pair (two steps
-    # h <- E_1 + (tau/j) M h per pass).  Its algorithmic bytes for one job:
-    # FP32 read of h, read of E_1, write of the new h = 12 B per node; 8 B when
-    # h is E_1 itself (the job's first sweep of a sub-step).  A launch skips
-    # the jobs whose series needs fewer steps, so the achieved rate is the
-    # bytes of the sweeps the jobs actually took (counted on the device per
-    # job: taylor_node_bytes x nodes) over the summed duration of every
-    # launch, skipping ones included.
+    # roofline pass: the same batch with the launches of the dominant kernel
+    # timed by device events on the engine's stream (plain launches instead
+    # of graph replay while the timer is on; consecutive launches of the
+    # kernel share one pair of events around their run, so the mean duration
+    # per launch is the run's span over its launch count, gaps included).
+    # The dominant kernel is the Horner sweep k_tay_term4 (one step
+    # h <- E_1 + s_j M h + t_j h per pass; k_tay_pair, two steps per pass,
+    # when HS_PAIRS=1).  Its algorithmic bytes for one job: FP32 read of h,
+    # read of E_1, write of the new h = 12 B per node; 8 B when h is E_1
+    # itself (the job's first sweep of a sub-step); 24 B for the last step,
+    # which adds the increment into the FP64 state.  A launch skips the jobs
+    # whose series needs fewer steps, so the achieved rate is the bytes of the
+    # sweeps the jobs actually took (counted on the device per job:
+    # taylor_node_bytes x nodes) over the summed duration of every launch,
+    # skipping ones included.
     achieved = avg_ms = nt = sweeps = None
     nvec = cells + iface.EXTRA
     bytes_per_sweep = nvec * 12
@@ -470,11 +474,12 @@ def main():
         sweeps = float(sum(i.taylor_sweeps for i in rinfo))
         node_bytes = float(sum(i.taylor_node_bytes for i in rinfo))
         tot_ms = tot.value
-    traffic = None
+    traffic = traffic_ratio = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{kernel}_3dstack{args.grid}_batch{args.batch}.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tj = json.load(f)
+        traffic, traffic_ratio = tj.get("bytes_per_launch"), tj.get("traffic_over_algorithmic")
 
     # ---- the same first job alone through the single-model API ------------------
     j0 = jobs[0]
@@ -547,8 +552,11 @@ def main():
                      "algorithmic_bytes_avg_per_launch": (node_bytes * nvec / nt) if nt else None,
                      "steps_per_batch": step_sweeps,
                      "node_bytes_per_batch": step_node_bytes,
+                     "traffic_over_algorithmic": traffic_ratio,
                      "traffic_note": "HBM bytes per launch from FETCH_SIZE and WRITE_SIZE (rocprofv3 "
-                                     "--pmc, scaled against a plain copy)"},
+                                     "--pmc, scaled against a plain copy), 300 intervals in ticks of 20: "
+                                     "the same launch mix as this run's, whose algorithmic bytes per "
+                                     "launch it exceeds by traffic_over_algorithmic"},
         "single_job": single,
         "fp64_gpu": fp64,
         "speedup_vs_fp64_gpu": (value / fp64["value"]) if fp64 else None,

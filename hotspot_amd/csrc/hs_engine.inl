@@ -140,15 +140,26 @@ void dev_release(Model &M) {
     M.dev = nullptr;
 }
 
-// Optional per-launch timing of one kernel family (events around each
-// launch, capped): the live measurement behind the roofline figures.
+// Optional timing of one kernel family (capped): the live measurement
+// behind the roofline figures.  Consecutive launches of the family share one
+// pair of events around the whole run (their summed duration, the gaps
+// between them included), so the events add no time per launch; any other
+// launch through launch_named, or one on another stream, ends the run.
 struct KernelTimer {
+    struct Run {
+        hipEvent_t a, b;
+        int n;
+    };
     std::string name;
     int cap = 0;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;   // pending (plain launches)
+    std::vector<Run> ev;                                 // pending (plain launches)
+    std::vector<hipEvent_t> spent;                       // end events replaced by a longer run
+    bool open = false;                                   // the last launch extended ev.back()
+    hipStream_t stream = nullptr;                        // stream of the open run
+    int launched = 0;
     double graph_ms = 0.0;                               // sampled from graph replays
     int graph_n = 0;
-    int taken() const { return (int)ev.size() + graph_n; }
+    int taken() const { return launched + graph_n; }
 };
 static KernelTimer g_timer;
 
@@ -156,16 +167,26 @@ template <typename K, typename... A>
 static inline void launch_named(const char *name, K kern, int nblocks, hipStream_t s, A... args) {
     if (nblocks <= 0) return;
     const bool timed = g_timer.cap > 0 && g_timer.taken() < g_timer.cap && g_timer.name == name;
+    if (!timed || s != g_timer.stream) g_timer.open = false;
     hipEvent_t a = nullptr, b = nullptr;
-    if (timed) {
+    if (timed && !g_timer.open) {
         HIP_OK(hipEventCreate(&a));
-        HIP_OK(hipEventCreate(&b));
         HIP_OK(hipEventRecord(a, s));
     }
     hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kBlock), 0, s, args...);
     if (timed) {
+        HIP_OK(hipEventCreate(&b));
         HIP_OK(hipEventRecord(b, s));
-        g_timer.ev.emplace_back(a, b);
+        if (g_timer.open) {
+            g_timer.spent.push_back(g_timer.ev.back().b);
+            g_timer.ev.back().b = b;
+            g_timer.ev.back().n++;
+        } else {
+            g_timer.ev.push_back({a, b, 1});
+        }
+        g_timer.open = true;
+        g_timer.stream = s;
+        g_timer.launched++;
     }
 }
 #define launch(kern, ...) launch_named(#kern, kern, __VA_ARGS__)
@@ -1540,9 +1561,8 @@ static void taylor_interval(const Group &G, int m, int Kplan, int seq) {
             for (int j = K % 2 ? K : K - 1; j >= 3; j -= 2) {
                 launch_named("k_tay_pair", k_tay_pair, G.npwork, G.stream, dd, (const WorkItem *)G.d_pwork, G.pnper,
                              accb, db0, db1, j, K, km, tr);
-                hipLaunchKernelGGL(k_tay_pair_fix, dim3(G.nprob * kPairFixBlocks), dim3(kBlock), 0, G.stream, dd, accb,
-                                   db0, db1, j,
-                                   K, km, tr);
+                launch_named("k_tay_pair_fix", k_tay_pair_fix, G.nprob * kPairFixBlocks, G.stream, dd, accb, db0,
+                             db1, j, K, km, tr);
             }
         } else {
             for (int j = K; j >= 2; j--) single(j);
@@ -1916,10 +1936,15 @@ extern "C" {
 void hs_kernel_timer(const char *name, int max_samples) {
     using namespace hs;
     for (auto &e : g_timer.ev) {
-        (void)hipEventDestroy(e.first);
-        (void)hipEventDestroy(e.second);
+        (void)hipEventSynchronize(e.b);
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
     }
+    for (auto &e : g_timer.spent) (void)hipEventDestroy(e);
     g_timer.ev.clear();
+    g_timer.spent.clear();
+    g_timer.open = false;
+    g_timer.launched = 0;
     g_timer.graph_ms = 0.0;
     g_timer.graph_n = 0;
     g_timer.name = name ? name : "";
@@ -1931,11 +1956,11 @@ int hs_kernel_timer_read(double *avg_ms, double *total_ms) {
     double tot = g_timer.graph_ms;
     int n = g_timer.graph_n;
     for (auto &e : g_timer.ev) {
-        if (hipEventSynchronize(e.second) != hipSuccess) continue;
+        if (hipEventSynchronize(e.b) != hipSuccess) continue;
         float ms = 0;
-        if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
+        if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
             tot += ms;
-            n++;
+            n += e.n;
         }
     }
     if (avg_ms) *avg_ms = n ? tot / n : 0.0;

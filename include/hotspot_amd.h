@@ -490,8 +490,10 @@ void hs_grid_to_block(RC_model_t *model, const double *grid_vec, double *block_v
 void hs_transient_run(RC_model_t *model, const double *power_seq, int n_steps, double dt,
                       const double *init_temp, double *temp_seq);
 
-/* Times every launch of one kernel family (e.g. "k_tay_term", "k_rk_stage",
- * "k_cg_dir") with device events, up to max_samples launches; NULL stops. */
+/* Times the launches of one kernel family (e.g. "k_tay_term", "k_rk_stage",
+ * "k_cg_dir") with device events, up to max_samples launches; consecutive
+ * launches of the family share one pair of events around their run.  NULL
+ * stops. */
 void hs_kernel_timer(const char *name, int max_samples);
 /* Number of timed launches; their mean and total duration in ms. */
 int hs_kernel_timer_read(double *avg_ms, double *total_ms);
