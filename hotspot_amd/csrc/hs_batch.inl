@@ -89,20 +89,22 @@ struct Engine {
     int next_id = 0;
     int tick_intervals = 32;            // transient intervals per tick
     int tick_iters = 128;               // steady iterations per tick
-    hipStream_t s_steady = nullptr, s_trans = nullptr;
+    hipStream_t s_steady = nullptr, s_trans[2] = {nullptr, nullptr};
     std::deque<EngJob *> queue;
     std::map<int, std::unique_ptr<EngJob>> jobs;
     // steady groups by key: kind (0 homogeneous CG, 1 heterogeneous CG,
     // 2 non-symmetric GMRES) + 4 x multigrid levels (0: Jacobi);
-    // transient: homogeneous, heterogeneous
+    // transient: homogeneous, heterogeneous, each in two halves on the two
+    // transient streams, [hetero + 2 * half]
     std::map<int, BatchGroup> steady;
-    BatchGroup trans[2];
+    BatchGroup trans[4];
     ~Engine() {
         for (auto &kv : jobs) kv.second->D.reset();
         for (auto &kv : steady) kv.second.release();
         for (auto &g : trans) g.release();
         if (s_steady) (void)hipStreamDestroy(s_steady);
-        if (s_trans) (void)hipStreamDestroy(s_trans);
+        for (auto s : s_trans)
+            if (s) (void)hipStreamDestroy(s);
     }
 };
 
@@ -271,7 +273,15 @@ static void start_transient(Engine &E, EngJob &j) {
     }
     write_scal(D, kv);
     j.phase = J_TRANS;
-    BatchGroup &B = E.trans[j.hetero];
+    // the half with fewer members: while one half runs the latency-bound
+    // kernels of its interval (power map, first term, finishes), the other
+    // one's Horner sweeps use the memory bandwidth.  One half only with
+    // HS_ONE_TRANS_STREAM=1, or while a kernel timer runs (launches of the
+    // two halves would overlap in its event spans).
+    static const bool one_env = getenv("HS_ONE_TRANS_STREAM") && atoi(getenv("HS_ONE_TRANS_STREAM")) == 1;
+    const bool one = one_env || g_timer.cap > 0;
+    const int h0 = j.hetero, h1 = j.hetero + 2;
+    BatchGroup &B = E.trans[(one || E.trans[h0].members.size() <= E.trans[h1].members.size()) ? h0 : h1];
     B.members.push_back(&j);
     B.dirty = true;
 }
@@ -409,7 +419,7 @@ Engine *eng_create(int precision, int transient) {
     HIP_OK(hipMemGetInfo(&fr, &tot));
     E->budget = (size_t)(0.8 * (double)fr);
     HIP_OK(hipStreamCreateWithFlags(&E->s_steady, hipStreamNonBlocking));
-    HIP_OK(hipStreamCreateWithFlags(&E->s_trans, hipStreamNonBlocking));
+    for (auto &s : E->s_trans) HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     return E.release();
 }
 
@@ -491,7 +501,7 @@ int eng_tick(Engine &E) {
     // members launched this tick per group: only their gathered flags are
     // fresh (a group may gain members below, or be empty this tick)
     std::map<int, int> ran_steady;
-    int ran_trans[2] = {0, 0};
+    int ran_trans[4] = {0, 0, 0, 0};
     for (auto &kv : E.steady) {
         BatchGroup &B = kv.second;
         const int kind = kv.first % 4;
@@ -503,27 +513,27 @@ int eng_tick(Engine &E) {
         const bool refined = E.opt.precision == HS_PREC_MIXED || kind == 2 || !B.lv.empty();
         gather(B, {S_DONE, refined ? S_TITERS : S_ITERS, S_NREFINE});
     }
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < 4; h++) {
         BatchGroup &B = E.trans[h];
         if (B.members.empty()) continue;
-        group_sync(E, B, E.s_trans, h, true);
+        group_sync(E, B, E.s_trans[h / 2], h % 2, true);
         ran_trans[h] = B.g.nprob;
-        B.rp.run(E.s_trans, E.tick_intervals, [&] { seq_body(B.g, B.form); });
+        B.rp.run(E.s_trans[h / 2], E.tick_intervals, [&] { seq_body(B.g, B.form); });
         gather(B, {S_STEP, S_TCALLS, S_TTERMS, S_TBYTES});
     }
     HIP_OK(hipStreamSynchronize(E.s_steady));
-    HIP_OK(hipStreamSynchronize(E.s_trans));
+    for (auto st : E.s_trans) HIP_OK(hipStreamSynchronize(st));
     // Taylor groups of one sub-step per interval launch the Horner steps
     // that the tick's largest term count needs, plus one: fewer launches that
     // skip every problem.  A problem that needs more is held (its interval
     // not advanced) until the next tick raises the count.  A change
     // recaptures the group's graph; the count falls only by two or more.
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < 4; h++) {
         BatchGroup &B = E.trans[h];
         if (!ran_trans[h] || B.form.transient != HS_TRANS_TAYLOR || B.form.m != 1 || !kgraph_enabled()) continue;
         int kt[2] = {0, 0};
         HIP_OK(hipMemcpy(kt, B.g.d_kmax, sizeof kt, hipMemcpyDeviceToHost));
-        HIP_OK(hipMemsetAsync(B.g.d_kmax + 1, 0, sizeof(int), E.s_trans));
+        HIP_OK(hipMemsetAsync(B.g.d_kmax + 1, 0, sizeof(int), E.s_trans[h / 2]));
         if (kt[1] <= 0) continue;
         const int cur = B.g.kgraph > 0 ? B.g.kgraph : B.form.K;
         const int want = std::min(B.form.K, std::max(2, kt[1] + 1));
@@ -543,7 +553,7 @@ int eng_tick(Engine &E) {
             steady_done(E, *f.first, f.second.data());
         }
     }
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < 4; h++) {
         BatchGroup &B = E.trans[h];
         std::vector<std::pair<EngJob *, std::vector<double>>> fin;
         for (int i = 0; i < ran_trans[h]; i++)
