@@ -83,21 +83,32 @@ struct BatchGroup {
     }
 };
 
+// Transient streams of an engine; its transient groups are split into
+// HS_TRANS_STREAMS parts (default 2, at most kTransStreams), one per stream.
+constexpr int kTransStreams = 3;
+static int trans_streams() {
+    static const int n = [] {
+        const char *e = getenv("HS_TRANS_STREAMS");
+        return e ? std::max(1, std::min(kTransStreams, atoi(e))) : 2;
+    }();
+    return n;
+}
+
 struct Engine {
     hs_options_t opt{};
     size_t budget = 0, used = 0;
     int next_id = 0;
     int tick_intervals = 32;            // transient intervals per tick
     int tick_iters = 128;               // steady iterations per tick
-    hipStream_t s_steady = nullptr, s_trans[2] = {nullptr, nullptr};
+    hipStream_t s_steady = nullptr, s_trans[kTransStreams] = {};
     std::deque<EngJob *> queue;
     std::map<int, std::unique_ptr<EngJob>> jobs;
     // steady groups by key: kind (0 homogeneous CG, 1 heterogeneous CG,
     // 2 non-symmetric GMRES) + 4 x multigrid levels (0: Jacobi);
-    // transient: homogeneous, heterogeneous, each in two halves on the two
-    // transient streams, [hetero + 2 * half]
+    // transient: homogeneous, heterogeneous, each in parts on the transient
+    // streams, [hetero + 2 * part]
     std::map<int, BatchGroup> steady;
-    BatchGroup trans[4];
+    BatchGroup trans[2 * kTransStreams];
     ~Engine() {
         for (auto &kv : jobs) kv.second->D.reset();
         for (auto &kv : steady) kv.second.release();
@@ -273,15 +284,16 @@ static void start_transient(Engine &E, EngJob &j) {
     }
     write_scal(D, kv);
     j.phase = J_TRANS;
-    // the half with fewer members: while one half runs the latency-bound
-    // kernels of its interval (power map, first term, finishes), the other
-    // one's Horner sweeps use the memory bandwidth.  One half only with
-    // HS_ONE_TRANS_STREAM=1, or while a kernel timer runs (launches of the
-    // two halves would overlap in its event spans).
-    static const bool one_env = getenv("HS_ONE_TRANS_STREAM") && atoi(getenv("HS_ONE_TRANS_STREAM")) == 1;
-    const bool one = one_env || g_timer.cap > 0;
-    const int h0 = j.hetero, h1 = j.hetero + 2;
-    BatchGroup &B = E.trans[(one || E.trans[h0].members.size() <= E.trans[h1].members.size()) ? h0 : h1];
+    // the part with the fewest members: while one part runs the latency-
+    // bound kernels of its interval (power map, first term, finishes), the
+    // others' Horner sweeps use the memory bandwidth.  One part only while a
+    // kernel timer runs (launches of the parts would overlap in its event
+    // spans).
+    const int parts = g_timer.cap > 0 ? 1 : trans_streams();
+    int best = j.hetero;
+    for (int q = 1; q < parts; q++)
+        if (E.trans[j.hetero + 2 * q].members.size() < E.trans[best].members.size()) best = j.hetero + 2 * q;
+    BatchGroup &B = E.trans[best];
     B.members.push_back(&j);
     B.dirty = true;
 }
@@ -501,7 +513,7 @@ int eng_tick(Engine &E) {
     // members launched this tick per group: only their gathered flags are
     // fresh (a group may gain members below, or be empty this tick)
     std::map<int, int> ran_steady;
-    int ran_trans[4] = {0, 0, 0, 0};
+    int ran_trans[2 * kTransStreams] = {};
     for (auto &kv : E.steady) {
         BatchGroup &B = kv.second;
         const int kind = kv.first % 4;
@@ -513,7 +525,7 @@ int eng_tick(Engine &E) {
         const bool refined = E.opt.precision == HS_PREC_MIXED || kind == 2 || !B.lv.empty();
         gather(B, {S_DONE, refined ? S_TITERS : S_ITERS, S_NREFINE});
     }
-    for (int h = 0; h < 4; h++) {
+    for (int h = 0; h < 2 * kTransStreams; h++) {
         BatchGroup &B = E.trans[h];
         if (B.members.empty()) continue;
         group_sync(E, B, E.s_trans[h / 2], h % 2, true);
@@ -528,7 +540,7 @@ int eng_tick(Engine &E) {
     // skip every problem.  A problem that needs more is held (its interval
     // not advanced) until the next tick raises the count.  A change
     // recaptures the group's graph; the count falls only by two or more.
-    for (int h = 0; h < 4; h++) {
+    for (int h = 0; h < 2 * kTransStreams; h++) {
         BatchGroup &B = E.trans[h];
         if (!ran_trans[h] || B.form.transient != HS_TRANS_TAYLOR || B.form.m != 1 || !kgraph_enabled()) continue;
         int kt[2] = {0, 0};
@@ -553,7 +565,7 @@ int eng_tick(Engine &E) {
             steady_done(E, *f.first, f.second.data());
         }
     }
-    for (int h = 0; h < 4; h++) {
+    for (int h = 0; h < 2 * kTransStreams; h++) {
         BatchGroup &B = E.trans[h];
         std::vector<std::pair<EngJob *, std::vector<double>>> fin;
         for (int i = 0; i < ran_trans[h]; i++)
