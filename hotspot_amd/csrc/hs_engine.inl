@@ -141,10 +141,13 @@ void dev_release(Model &M) {
 }
 
 // Optional timing of one kernel family (capped): the live measurement
-// behind the roofline figures.  Consecutive launches of the family share one
-// pair of events around the whole run (their summed duration, the gaps
-// between them included), so the events add no time per launch; any other
-// launch through launch_named, or one on another stream, ends the run.
+// behind the roofline figures.  Consecutive launches of the family on one
+// stream form a run, timed by one pair of events: the start event before its
+// first launch and the end event when the run closes (at the next launch of
+// another kernel through launch_named, a launch on another stream, or the
+// read), so no event sits between the run's kernels.  The mean per launch is
+// the run's span over its launch count, the gaps between the launches
+// included.
 struct KernelTimer {
     struct Run {
         hipEvent_t a, b;
@@ -153,13 +156,18 @@ struct KernelTimer {
     std::string name;
     int cap = 0;
     std::vector<Run> ev;                                 // pending (plain launches)
-    std::vector<hipEvent_t> spent;                       // end events replaced by a longer run
-    bool open = false;                                   // the last launch extended ev.back()
+    bool open = false;                                   // ev.back() still takes launches (no end event yet)
     hipStream_t stream = nullptr;                        // stream of the open run
     int launched = 0;
     double graph_ms = 0.0;                               // sampled from graph replays
     int graph_n = 0;
     int taken() const { return launched + graph_n; }
+    void close() {
+        if (!open) return;
+        HIP_OK(hipEventCreate(&ev.back().b));
+        HIP_OK(hipEventRecord(ev.back().b, stream));
+        open = false;
+    }
 };
 static KernelTimer g_timer;
 
@@ -167,25 +175,18 @@ template <typename K, typename... A>
 static inline void launch_named(const char *name, K kern, int nblocks, hipStream_t s, A... args) {
     if (nblocks <= 0) return;
     const bool timed = g_timer.cap > 0 && g_timer.taken() < g_timer.cap && g_timer.name == name;
-    if (!timed || s != g_timer.stream) g_timer.open = false;
-    hipEvent_t a = nullptr, b = nullptr;
+    if (!timed || s != g_timer.stream) g_timer.close();
     if (timed && !g_timer.open) {
+        hipEvent_t a = nullptr;
         HIP_OK(hipEventCreate(&a));
         HIP_OK(hipEventRecord(a, s));
+        g_timer.ev.push_back({a, nullptr, 0});
+        g_timer.open = true;
+        g_timer.stream = s;
     }
     hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kBlock), 0, s, args...);
     if (timed) {
-        HIP_OK(hipEventCreate(&b));
-        HIP_OK(hipEventRecord(b, s));
-        if (g_timer.open) {
-            g_timer.spent.push_back(g_timer.ev.back().b);
-            g_timer.ev.back().b = b;
-            g_timer.ev.back().n++;
-        } else {
-            g_timer.ev.push_back({a, b, 1});
-        }
-        g_timer.open = true;
-        g_timer.stream = s;
+        g_timer.ev.back().n++;
         g_timer.launched++;
     }
 }
@@ -1935,15 +1936,13 @@ void dev_g2b(Model &M, const double *grid_vec, double *block_vec) {
 extern "C" {
 void hs_kernel_timer(const char *name, int max_samples) {
     using namespace hs;
+    g_timer.close();
     for (auto &e : g_timer.ev) {
         (void)hipEventSynchronize(e.b);
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
     }
-    for (auto &e : g_timer.spent) (void)hipEventDestroy(e);
     g_timer.ev.clear();
-    g_timer.spent.clear();
-    g_timer.open = false;
     g_timer.launched = 0;
     g_timer.graph_ms = 0.0;
     g_timer.graph_n = 0;
@@ -1955,6 +1954,7 @@ int hs_kernel_timer_read(double *avg_ms, double *total_ms) {
     using namespace hs;
     double tot = g_timer.graph_ms;
     int n = g_timer.graph_n;
+    g_timer.close();
     for (auto &e : g_timer.ev) {
         if (hipEventSynchronize(e.b) != hipSuccess) continue;
         float ms = 0;
