@@ -205,6 +205,42 @@ def test_transient_taylor_matches_exact(lib, name, case, pw, prec, tol):
     run.free()
 
 
+# Long intervals take many sub-steps of rho tau <= 4, each with the series
+# expanded about the centre of the spectrum (shift rho tau / 2) for
+# conduction models; a microchannel model (coolant advection) keeps the
+# unshifted series.  Both against the exact solution of each interval.
+def _microchannel16():
+    import tempfile
+    return cases.microchannel_case(16, 5, tempfile.mkdtemp())
+
+
+@pytest.mark.parametrize("dt", [1e-4, 1e-2], ids=["100us", "10ms"])
+@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (1, 1e-4)], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("which", ["default-64", "secondary-16", "3d-stack-32", "detailed-3d-16", "microchannel-16"])
+def test_transient_taylor_long_intervals_match_exact(lib, which, prec, tol, dt):
+    if which == "microchannel-16":
+        case, pw = _microchannel16()
+    else:
+        _, case, pw = next(c for c in CASES if c[0] == which)
+    run = make(lib, case)
+    set_options(lib, run, transient=1, precision=prec)
+    om = oracle_for(run)
+    P = run.power_vector(pw)
+    T0, _ = om.steady(P)
+    seq = cases.jittered_powers(P, 3, seed=11)
+    tr = cases.run_transient(run, T0, seq, dt)
+    check_error(lib)
+    st = stats(lib, run)
+    assert st.taylor_substeps >= 1
+    Tg = om.b2g(T0, power=False)
+    worst = 0.0
+    for k, p in enumerate(seq):
+        Tg = om.advance_exact(Tg, om.power_grid(p), dt)
+        worst = max(worst, np.abs(tr[k] - om.g2b(Tg)).max())
+    assert worst <= tol, worst
+    run.free()
+
+
 def test_reference_steady_within_its_own_stopping_error(lib, ref):
     """The stock reference output stops its Gauss-Seidel sweeps at a 1e-6 K
     per-sweep change, which leaves it ~1e-3 K from the converged operator;
