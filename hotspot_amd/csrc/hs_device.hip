@@ -2001,6 +2001,7 @@ __device__ __forceinline__ hs_d2 ld2(const double *base, unsigned i) {
 // a layer marked in lzero) plus the ambient sources, formed here: P + gamb
 // T_ambient on cells, gamb T_ambient on the package nodes, as k_b2g adds
 // them otherwise.
+constexpr int kF2Rows = 1;   // rows of a thread whose loads are in flight together
 __global__ void __launch_bounds__(kBlock) k_tay_first2(const ProbDesc *__restrict__ D,
                                                        const WorkItem *__restrict__ W, int e1b, int sub, int seq,
                                                        int *__restrict__ kmax, int power32) {
@@ -2034,13 +2035,13 @@ __global__ void __launch_bounds__(kBlock) k_tay_first2(const ProbDesc *__restric
         const bool pz = power32 && d.lzero[n];
         double ring[4];
         for (int e = 0; e < 4; e++) ring[e] = ld_at(y, ncell + max(rn[e], 0));
-        for (int g0 = 0; g0 < G; g0 += 2) {
-            hs_d2 C[2], N[2], S[2], U[2], Dn[2], B[2];
-            double Wv[2], Ev[2];
-            int I[2];
-            unsigned cc[2];
+        for (int g0 = 0; g0 < G; g0 += kF2Rows) {
+            hs_d2 C[kF2Rows], N[kF2Rows], S[kF2Rows], U[kF2Rows], Dn[kF2Rows], B[kF2Rows];
+            double Wv[kF2Rows], Ev[kF2Rows];
+            int I[kF2Rows];
+            unsigned cc[kF2Rows];
 #pragma unroll
-            for (int g = 0; g < 2; g++) {
+            for (int g = 0; g < kF2Rows; g++) {
                 I[g] = ti * rows + ty2 * G + g0 + g;
                 const int i = min(I[g], nr - 1);
                 const unsigned c = ((unsigned)n * nr + i) * nc + j0;
@@ -2062,7 +2063,7 @@ __global__ void __launch_bounds__(kBlock) k_tay_first2(const ProbDesc *__restric
                 }
             }
 #pragma unroll
-            for (int g = 0; g < 2; g++) {
+            for (int g = 0; g < kF2Rows; g++) {
                 if (g0 + g >= G || I[g] >= nr) continue;
                 const int i = I[g];
                 const double gn = i > 0 ? gx : 0.0, gs = i < nr - 1 ? gx : 0.0;
