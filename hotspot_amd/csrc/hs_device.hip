@@ -1988,7 +1988,7 @@ __global__ void __launch_bounds__(kBlock) k_tay_first(const ProbDesc *__restrict
 
 // E_1 of a homogeneous problem for the FP32 series with double2 accesses:
 // within the standard tile, each thread owns 2 consecutive columns x rpt/2
-// rows, two rows at a time.  The arithmetic per cell is that of stencil()
+// rows, one row at a time (kF2Rows).  The arithmetic per cell is that of stencil()
 // and k_tay_first in the same order.
 typedef double hs_d2 __attribute__((ext_vector_type(2)));
 typedef float hs_f2 __attribute__((ext_vector_type(2)));
