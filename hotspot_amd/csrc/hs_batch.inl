@@ -548,7 +548,8 @@ int eng_tick(Engine &E) {
         HIP_OK(hipMemsetAsync(B.g.d_kmax + 1, 0, sizeof(int), E.s_trans[h / 2]));
         if (kt[1] <= 0) continue;
         const int cur = B.g.kgraph > 0 ? B.g.kgraph : B.form.K;
-        const int want = std::min(B.form.K, std::max(2, kt[1] + 1));
+        static const int margin = getenv("HS_KGRAPH_MARGIN") ? std::max(0, atoi(getenv("HS_KGRAPH_MARGIN"))) : 1;
+        const int want = std::min(B.form.K, std::max(2, kt[1] + margin));
         if (kt[1] > cur || want < cur - 1) {
             B.g.kgraph = want >= B.form.K ? 0 : want;
             B.rp.reset();
