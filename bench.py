@@ -447,8 +447,7 @@ def main():
     # kernel share one pair of events around their run, so the mean duration
     # per launch is the run's span over its launch count, gaps included).
     # The dominant kernel is the Horner sweep k_tay_term4 (one step
-    # h <- E_1 + s_j M h + t_j h per pass; k_tay_pair, two steps per pass,
-    # when HS_PAIRS=1).  Its algorithmic bytes for one job: FP32 read of h,
+    # h <- E_1 + s_j M h + t_j h per pass).  Its algorithmic bytes for one job: FP32 read of h,
     # read of E_1, write of the new h = 12 B per node; 8 B when h is E_1
     # itself (the job's first sweep of a sub-step); 24 B for the last step,
     # which adds the increment into the FP64 state.  A launch skips the jobs
@@ -460,16 +459,13 @@ def main():
     nvec = cells + iface.EXTRA
     bytes_per_sweep = nvec * 12
     bytes_per_launch = args.batch * bytes_per_sweep
-    kernel = "k_tay_pair"
+    kernel = "k_tay_term"
     if not args.no_roofline:
-        for kernel in ("k_tay_pair", "k_tay_term"):
-            t_roof, _, rinfo, _ = served(lib, jobs, args.intervals, args.tick, timer=kernel.encode())
-            device_sync()
-            a, tot = C.c_double(), C.c_double()
-            nt = L.hs_kernel_timer_read(C.byref(a), C.byref(tot))
-            L.hs_kernel_timer(None, 0)
-            if nt:
-                break
+        t_roof, _, rinfo, _ = served(lib, jobs, args.intervals, args.tick, timer=kernel.encode())
+        device_sync()
+        a, tot = C.c_double(), C.c_double()
+        nt = L.hs_kernel_timer_read(C.byref(a), C.byref(tot))
+        L.hs_kernel_timer(None, 0)
         avg_ms = a.value
         sweeps = float(sum(i.taylor_sweeps for i in rinfo))
         node_bytes = float(sum(i.taylor_node_bytes for i in rinfo))
@@ -540,10 +536,7 @@ def main():
                        "note": "submission -> result on the host, admission included"},
         "admission_ms": float(np.mean(admit_t)) * 1e3,
         "value_with_admission": args.batch * updates_job / (t_step + float(np.mean(admit_t))),
-        "roofline": {"bound": "hbm", "kernel": {"k_tay_pair": "k_tay_pair (FP32, two Horner steps of the Taylor "
-                                                              "series per sweep)",
-                                                "k_tay_term": "k_tay_term4 (FP32 Horner step of the Taylor series, "
-                                                              "float4 tiles)"}[kernel],
+        "roofline": {"bound": "hbm", "kernel": "k_tay_term4 (FP32 Horner step of the Taylor series, float4 tiles)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic, "launches_timed": nt,

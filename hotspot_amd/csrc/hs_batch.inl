@@ -1,17 +1,21 @@
 // Batched engine: many models advanced by shared launches.  Jobs (a steady
 // solve, a transient sequence, or a steady solve followed by a transient
 // sequence from its block temperatures) are admitted first come, first
-// served while their device footprint fits the engine's budget; every tick
-// runs a chunk of iterations for all jobs in their steady phase and a chunk
-// of intervals for all jobs in their transient phase, on two streams, then
-// moves jobs between phases by their device-side flags.  Each problem's
-// arithmetic is the same as when it is solved alone (per-problem scalars,
-// per-problem stopping, reductions in a fixed slot order), so a job's result
-// does not depend on what it is batched with.  Included at the end of
-// hs_engine.inl.
+// served while their device footprint fits the engine's budget.  Resident
+// jobs form groups by phase and kernel form.  Each group advances in chunks
+// (a few steady iterations, or a slice of transient intervals) on its own
+// stream, and the host takes a group's results once the event recorded
+// after its chunk has completed, independently of the other groups: a
+// steady solve that converges is reported while the transient groups keep
+// running, and a job that arrives joins its group at the group's next chunk.
+// Each problem's arithmetic is the same as when it is solved alone
+// (per-problem scalars, per-problem stopping, reductions in a fixed slot
+// order), so a job's result does not depend on what it is batched with or
+// when it arrived.  Included at the end of hs_engine.inl.
 #include <chrono>
 #include <deque>
 #include <map>
+#include <thread>
 
 namespace hs {
 
@@ -41,7 +45,22 @@ struct EngJob {
     int steady_iters = 0, refinements = 0, rk4_calls = 0;
     long taylor_sweeps = 0;
     double taylor_node_bytes = 0;
+    int step_seen = 0;                  // intervals done at the last look at its group's flags
+    int holds = 0;                      // Taylor: intervals held and run again
+    int kplan = 0;                      // Taylor: Horner steps per sub-step of its plan
+    int kmin = 0;                       // Taylor: fewest steps its group launched per sub-step (0: the plan's)
 };
+
+// Transient parts of an engine: each transient form (homogeneous,
+// heterogeneous) is split into this many groups, one per stream.  While one
+// part runs the latency-bound kernels of its interval (power map, first
+// term, finishes), the other's Horner sweeps use the memory bandwidth.
+constexpr int kTransParts = 2;
+// device time a transient chunk aims at: long enough to amortise the host's
+// look at the flags, short enough that an arriving job waits little to join
+constexpr double kSliceMs = 20.0;
+// longest a tick waits for some group's chunk to end
+constexpr double kWaitMs = 2.0;
 
 // A group of resident jobs in one phase with one kernel form (homogeneous or
 // heterogeneous coefficients): descriptor array and work-lists on the device.
@@ -49,32 +68,48 @@ struct BatchGroup {
     Group g;
     std::vector<EngJob *> members;
     bool dirty = true;
-    size_t cap_desc = 0, cap_work = 0, cap_gwork = 0, cap_flags = 0, cap_trec = 0, cap_twork = 0, cap_pwork = 0;
+    DevPool *pool = nullptr, *hpool = nullptr;
+    hipStream_t stream = nullptr;       // the stream its chunks run on
+    size_t cap_desc = 0, cap_work = 0, cap_gwork = 0, cap_flags = 0, cap_trec = 0, cap_twork = 0;
     double *d_flags = nullptr, *h_flags = nullptr;
-    Replayer rp;
+    int *h_kmax = nullptr;              // Taylor: the term counts of the last chunk (pinned)
+    // interval bodies captured per count of Horner steps launched (kgraph)
+    Replayer rp[kTayMaxK + 1];
     SeqForm form{};
     // multigrid steady groups: per level, the descriptors and column work-lists
     std::vector<MgGroup> lv;
     std::vector<void *> lv_bufs;
+    // the chunk in flight
+    bool busy = false;
+    int ran = 0;                        // members launched in it (the first `ran`; later ones joined since)
+    int chunk = 0;                      // transient intervals in it
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    double ms_per_interval = 0.0;       // device time per interval of the last chunk
 
+    void reset_graphs() {
+        for (Replayer &r : rp) r.reset();
+    }
     void release() {
-        rp.reset();
-        for (void *p : lv_bufs) (void)hipFree(p);
+        reset_graphs();
+        for (void *p : lv_bufs) dfree(pool, p);
         lv_bufs.clear();
         lv.clear();
-        if (g.d_desc) (void)hipFree(g.d_desc);
-        if (g.d_work) (void)hipFree(g.d_work);
-        if (g.d_gwork) (void)hipFree(g.d_gwork);
-        if (d_flags) (void)hipFree(d_flags);
-        if (h_flags) (void)hipHostFree(h_flags);
-        if (g.d_kmax) (void)hipFree(g.d_kmax);
-        if (g.d_trec) (void)hipFree(g.d_trec);
-        if (g.d_twork) (void)hipFree(g.d_twork);
-        if (g.d_pwork) (void)hipFree(g.d_pwork);
+        dfree(pool, g.d_desc);
+        dfree(pool, g.d_work);
+        dfree(pool, g.d_gwork);
+        dfree(pool, d_flags);
+        hfree(hpool, h_flags);
+        hfree(hpool, h_kmax);
+        dfree(pool, g.d_kmax);
+        dfree(pool, g.d_trec);
+        dfree(pool, g.d_twork);
+        if (ev_a) (void)hipEventDestroy(ev_a);
+        if (ev_b) (void)hipEventDestroy(ev_b);
+        ev_a = ev_b = nullptr;
         g = Group{};
-        cap_trec = cap_twork = cap_pwork = 0;
         d_flags = h_flags = nullptr;
-        cap_desc = cap_work = cap_gwork = cap_flags = 0;
+        h_kmax = nullptr;
+        cap_desc = cap_work = cap_gwork = cap_flags = cap_trec = cap_twork = 0;
     }
     ~BatchGroup() { release(); }
     void remove(EngJob *j) {
@@ -83,24 +118,15 @@ struct BatchGroup {
     }
 };
 
-// Transient streams of an engine; its transient groups are split into
-// HS_TRANS_STREAMS parts (default 2, at most kTransStreams), one per stream.
-constexpr int kTransStreams = 3;
-static int trans_streams() {
-    static const int n = [] {
-        const char *e = getenv("HS_TRANS_STREAMS");
-        return e ? std::max(1, std::min(kTransStreams, atoi(e))) : 2;
-    }();
-    return n;
-}
-
 struct Engine {
+    DevPool dpool, hpool;               // declared first: destroyed after the groups that use them
     hs_options_t opt{};
     size_t budget = 0, used = 0;
     int next_id = 0;
-    int tick_intervals = 32;            // transient intervals per tick
-    int tick_iters = 128;               // steady iterations per tick
-    hipStream_t s_steady = nullptr, s_trans[kTransStreams] = {};
+    int tick_intervals = 32;            // most transient intervals per chunk
+    int tick_iters = 128;               // steady iterations per chunk
+    // s_setup: admission, phase changes and results (never waits on a chunk)
+    hipStream_t s_setup = nullptr, s_steady = nullptr, s_trans[kTransParts] = {};
     std::deque<EngJob *> queue;
     std::map<int, std::unique_ptr<EngJob>> jobs;
     // steady groups by key: kind (0 homogeneous CG, 1 heterogeneous CG,
@@ -108,13 +134,14 @@ struct Engine {
     // transient: homogeneous, heterogeneous, each in parts on the transient
     // streams, [hetero + 2 * part]
     std::map<int, BatchGroup> steady;
-    BatchGroup trans[2 * kTransStreams];
+    BatchGroup trans[2 * kTransParts];
     ~Engine() {
+        for (hipStream_t s : {s_setup, s_steady, s_trans[0], s_trans[1]})
+            if (s) (void)hipStreamSynchronize(s);
         for (auto &kv : jobs) kv.second->D.reset();
         for (auto &kv : steady) kv.second.release();
         for (auto &g : trans) g.release();
-        if (s_steady) (void)hipStreamDestroy(s_steady);
-        for (auto s : s_trans)
+        for (hipStream_t s : {s_setup, s_steady, s_trans[0], s_trans[1]})
             if (s) (void)hipStreamDestroy(s);
     }
 };
@@ -143,19 +170,23 @@ static size_t dev_footprint(const Model &M, int nsteps, bool mg) {
         b += (ncell / 3 + 64 * 8) * 7 * 4 + (size_t)g.n_layers * 4096;
     if (adv)   // GMRES state, partials and one basis (FP32 or FP64)
         b += GM_SIZE * 8 + (size_t)(t.ntiles + 1) * (kGmM + 1) * 8 + (size_t)(kGmM + 1) * vlen * 8;
-    return b + b / 50 + (64 << 10);
+    // pool size classes round each allocation up by at most an eighth
+    return b + b / 8 + (64 << 10);
 }
 
-template <typename T> static void ensure_dev(T *&p, size_t &cap, size_t n) {
+template <typename T> static void ensure_dev(DevPool *pool, T *&p, size_t &cap, size_t n) {
     if (n <= cap) return;
-    if (p) HIP_OK(hipFree(p));
+    dfree(pool, p);
     cap = std::max(n, cap * 2);
-    HIP_OK(hipMalloc((void **)&p, cap * sizeof(T)));
+    p = (T *)dmalloc(pool, cap * sizeof(T));
 }
 
-// Uploads the group's descriptors and work-lists after a membership change.
-static void group_sync(Engine &E, BatchGroup &B, hipStream_t s, bool het, bool trans) {
+// Uploads the group's descriptors and work-lists after a membership change
+// (on the set-up stream: the group's own stream may carry another group's
+// chunk, and this group is between chunks).
+static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans) {
     if (!B.dirty) return;
+    hipStream_t s = E.s_setup;
     const int n = (int)B.members.size();
     std::vector<ProbDesc> desc(n);
     std::vector<WorkItem> w, gw;
@@ -177,55 +208,49 @@ static void group_sync(Engine &E, BatchGroup &B, hipStream_t s, bool het, bool t
             taylor_record(p, &rec[(size_t)kTrec * i]);
         }
     }
-    B.rp.reset();
-    ensure_dev(B.g.d_desc, B.cap_desc, (size_t)n);
-    ensure_dev(B.g.d_work, B.cap_work, w.size());
-    ensure_dev(B.g.d_gwork, B.cap_gwork, gw.size());
-    if (!B.g.d_kmax) HIP_OK(hipMalloc((void **)&B.g.d_kmax, 2 * sizeof(int)));
-    HIP_OK(hipMemset(B.g.d_kmax, 0, 2 * sizeof(int)));
+    B.reset_graphs();
+    ensure_dev(B.pool, B.g.d_desc, B.cap_desc, (size_t)n);
+    ensure_dev(B.pool, B.g.d_work, B.cap_work, w.size());
+    ensure_dev(B.pool, B.g.d_gwork, B.cap_gwork, gw.size());
+    if (!B.g.d_kmax) B.g.d_kmax = (int *)dmalloc(B.pool, 2 * sizeof(int));
+    if (!B.h_kmax) B.h_kmax = (int *)hmalloc(B.hpool, 2 * sizeof(int));
+    const int zero[2] = {0, 0};
+    h2d(B.g.d_kmax, zero, sizeof zero, s);
+    B.h_kmax[0] = B.h_kmax[1] = 0;
     B.g.kgraph = 0;
-    ensure_dev(B.g.d_trec, B.cap_trec, (size_t)kTrec * n);
-    if (!rec.empty()) HIP_OK(hipMemcpy(B.g.d_trec, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
+    ensure_dev(B.pool, B.g.d_trec, B.cap_trec, (size_t)kTrec * std::max(n, 1));
+    if (!rec.empty()) h2d(B.g.d_trec, rec.data(), rec.size() * sizeof(double), s);
     if ((size_t)n * 8 > B.cap_flags) {
-        if (B.d_flags) HIP_OK(hipFree(B.d_flags));
-        if (B.h_flags) HIP_OK(hipHostFree(B.h_flags));
+        dfree(B.pool, B.d_flags);
+        hfree(B.hpool, B.h_flags);
         B.cap_flags = std::max((size_t)n * 8, B.cap_flags * 2);
-        HIP_OK(hipMalloc((void **)&B.d_flags, B.cap_flags * sizeof(double)));
-        HIP_OK(hipHostMalloc((void **)&B.h_flags, B.cap_flags * sizeof(double), 0));
+        B.d_flags = (double *)dmalloc(B.pool, B.cap_flags * sizeof(double));
+        B.h_flags = (double *)hmalloc(B.hpool, B.cap_flags * sizeof(double));
     }
-    HIP_OK(hipMemcpy(B.g.d_desc, desc.data(), n * sizeof(ProbDesc), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(B.g.d_work, w.data(), w.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(B.g.d_gwork, gw.data(), gw.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    h2d(B.g.d_desc, desc.data(), n * sizeof(ProbDesc), s);
+    h2d(B.g.d_work, w.data(), w.size() * sizeof(WorkItem), s);
+    h2d(B.g.d_gwork, gw.data(), gw.size() * sizeof(WorkItem), s);
     B.g.nprob = n;
     B.g.nwork = (int)w.size();
     B.g.ngwork = (int)gw.size();
-    B.g.stream = s;
+    B.g.stream = B.stream;
     B.g.hetero = het;
     B.g.vec4 = v4;
     B.g.nper = (n > 0 && (int)w.size() == n * (desc[0].ntiles + 1)) ? desc[0].ntiles + 1 : 0;
     for (int i = 0; i < n && B.g.nper; i++)
         if (desc[i].ntiles != desc[0].ntiles) B.g.nper = 0;
     B.form = SeqForm{E.opt.precision, E.opt.transient, m, K};
-    // the paired sweeps' work-list (patches through all layers)
-    {
-        std::vector<WorkItem> pw;
-        B.g.pairs = trans && !het && E.opt.transient == HS_TRANS_TAYLOR && E.opt.precision == HS_PREC_MIXED &&
-                    pairs_enabled();
-        B.g.pnper = pair_items(desc, pw);
-        ensure_dev(B.g.d_pwork, B.cap_pwork, pw.size());
-        HIP_OK(hipMemcpy(B.g.d_pwork, pw.data(), pw.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
-        B.g.npwork = (int)pw.size();
-    }
-    // the float4 term sweep's work-list; its tiling timed for FP32 Taylor groups
+    // the float4 term sweep's work-list; its tiling timed per size class for
+    // FP32 Taylor groups
     if (trans && v4 && E.opt.transient == HS_TRANS_TAYLOR && E.opt.precision == HS_PREC_MIXED) {
-        tune_term_tiling(B.g, desc, B.cap_twork);
+        tune_term_tiling(B.g, desc, B.cap_twork, B.pool, s);
     } else {
         std::vector<WorkItem> tw;
         const int per = term_items(desc, tw);
-        upload_term_items(B.g, tw, per, B.cap_twork);
+        upload_term_items(B.g, tw, per, B.cap_twork, B.pool, s);
     }
     // multigrid levels of a steady group (every member has the same count)
-    for (void *p : B.lv_bufs) HIP_OK(hipFree(p));
+    for (void *p : B.lv_bufs) dfree(B.pool, p);
     B.lv_bufs.clear();
     B.lv.clear();
     const int L = (!trans && n) ? (int)B.members[0]->D->mg.size() : 0;
@@ -242,19 +267,19 @@ static void group_sync(Engine &E, BatchGroup &B, hipStream_t s, bool het, bool t
             auto ci = column_items(ld[i], i);
             lw.insert(lw.end(), ci.begin(), ci.end());
         }
-        ProbDesc *dd = nullptr;
-        WorkItem *dw = nullptr;
-        HIP_OK(hipMalloc((void **)&dd, n * sizeof(ProbDesc)));
-        HIP_OK(hipMalloc((void **)&dw, lw.size() * sizeof(WorkItem)));
+        ProbDesc *dd = (ProbDesc *)dmalloc(B.pool, n * sizeof(ProbDesc));
+        WorkItem *dw = (WorkItem *)dmalloc(B.pool, lw.size() * sizeof(WorkItem));
         B.lv_bufs.push_back(dd);
         B.lv_bufs.push_back(dw);
-        HIP_OK(hipMemcpy(dd, ld.data(), n * sizeof(ProbDesc), hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(dw, lw.data(), lw.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+        h2d(dd, ld.data(), n * sizeof(ProbDesc), s);
+        h2d(dw, lw.data(), lw.size() * sizeof(WorkItem), s);
         B.lv.push_back(MgGroup{dd, dw, (int)lw.size(), nlmax, l > 0 || het});
     }
     B.dirty = false;
 }
 
+// copies the given scalar slots of every member into the pinned flags
+// (after the chunk, on the group's stream)
 static void gather(BatchGroup &B, std::initializer_list<int> slots) {
     Slots sl{};
     for (int v : slots) sl.s[sl.n++] = v;
@@ -279,17 +304,17 @@ static void start_transient(Engine &E, EngJob &j) {
     auto kv = seq_slots(j.nsteps, j.dt);
     kv.push_back({S_YPAR, 0.0});
     if (E.opt.transient == HS_TRANS_TAYLOR) {
-        const auto t = taylor_slots(taylor_plan(D, j.dt, E.opt.taylor_tol), D.rho);
+        const TaylorPlan p = taylor_plan(D, j.dt, E.opt.taylor_tol);
+        const auto t = taylor_slots(p, D.rho);
         kv.insert(kv.end(), t.begin(), t.end());
+        j.kplan = p.K;
     }
     write_scal(D, kv);
     j.phase = J_TRANS;
-    // the part with the fewest members: while one part runs the latency-
-    // bound kernels of its interval (power map, first term, finishes), the
-    // others' Horner sweeps use the memory bandwidth.  One part only while a
-    // kernel timer runs (launches of the parts would overlap in its event
-    // spans).
-    const int parts = g_timer.cap > 0 ? 1 : trans_streams();
+    j.step_seen = 0;
+    // the part with the fewest members; one part only while a kernel timer
+    // runs (launches of the parts would overlap in its event spans)
+    const int parts = g_timer.cap > 0 ? 1 : kTransParts;
     int best = j.hetero;
     for (int q = 1; q < parts; q++)
         if (E.trans[j.hetero + 2 * q].members.size() < E.trans[best].members.size()) best = j.hetero + 2 * q;
@@ -300,7 +325,7 @@ static void start_transient(Engine &E, EngJob &j) {
 
 static void admit(Engine &E, EngJob &j) {
     Model &M = *j.M;
-    j.D.reset(dev_build(M, E.s_steady));
+    j.D.reset(dev_build(M, E.s_setup, &E.dpool, &E.hpool));
     DevModel &D = *j.D;
     j.t_admit = now_s();
     const size_t nb = (size_t)D.nblk;
@@ -312,13 +337,11 @@ static void admit(Engine &E, EngJob &j) {
         if (j.rows == 1) {
             std::vector<double> rep((size_t)j.nsteps * nb);
             for (int k = 0; k < j.nsteps; k++) std::copy(j.power.begin(), j.power.end(), rep.begin() + k * nb);
-            HIP_OK(hipMemcpy(D.h.seq_in, rep.data(), rep.size() * sizeof(double), hipMemcpyHostToDevice));
+            h2d(D.h.seq_in, rep.data(), rep.size() * sizeof(double), D.stream);
         } else {
-            HIP_OK(hipMemcpyAsync(D.h.seq_in, j.power.data() + first * nb, (size_t)j.nsteps * nb * sizeof(double),
-                                  hipMemcpyHostToDevice, D.stream));
+            h2d(D.h.seq_in, j.power.data() + first * nb, (size_t)j.nsteps * nb * sizeof(double), D.stream);
         }
-        HIP_OK(hipMemcpyAsync(D.d_desc, &D.h, sizeof(ProbDesc), hipMemcpyHostToDevice, D.stream));
-        HIP_OK(hipStreamSynchronize(D.stream));
+        h2d(D.d_desc, &D.h, sizeof(ProbDesc), D.stream);
     }
     j.held = D.bytes;
     E.used += j.held;
@@ -344,6 +367,7 @@ static void admit(Engine &E, EngJob &j) {
             if (j.hetero) jpcg64_start<true>(D.grp, rtol2, E.opt.max_iters);
             else jpcg64_start<false>(D.grp, rtol2, E.opt.max_iters);
         }
+        HIP_OK(hipStreamSynchronize(D.stream));
         j.phase = J_STEADY;
         BatchGroup &B = E.steady[steady_key(j)];
         B.members.push_back(&j);
@@ -385,10 +409,10 @@ static void trans_done(Engine &E, EngJob &j, const double *flags) {
     j.rk4_calls = E.opt.transient == HS_TRANS_RK4 ? (int)flags[1] : 0;
     j.taylor_sweeps = E.opt.transient == HS_TRANS_TAYLOR ? (long)flags[2] : 0;
     j.taylor_node_bytes = E.opt.transient == HS_TRANS_TAYLOR ? flags[3] : 0.0;
+    j.holds = (int)flags[4];
     const size_t n = (size_t)j.nsteps * D.nblk;
     j.seq.resize(n);
-    HIP_OK(hipMemcpyAsync(j.seq.data(), D.h.seq_out, n * sizeof(double), hipMemcpyDeviceToHost, D.stream));
-    HIP_OK(hipStreamSynchronize(D.stream));
+    d2h(j.seq.data(), D.h.seq_out, n * sizeof(double), D.stream);
     j.result.assign(j.seq.end() - D.nblk, j.seq.end());
     j.phase = J_DONE;
     j.t_done = now_s();
@@ -416,9 +440,140 @@ static void steady_chunk(Engine &E, const Group &G, bool adv, const std::vector<
     }
 }
 
+static void group_events(BatchGroup &B) {
+    if (!B.ev_a) HIP_OK(hipEventCreate(&B.ev_a));
+    if (!B.ev_b) HIP_OK(hipEventCreate(&B.ev_b));
+}
+
+// one chunk of every group that is between chunks
+static void launch_chunks(Engine &E) {
+    for (auto &kv : E.steady) {
+        BatchGroup &B = kv.second;
+        if (B.busy || B.members.empty()) continue;
+        const int kind = kv.first % 4;
+        B.pool = &E.dpool;
+        B.hpool = &E.hpool;
+        B.stream = E.s_steady;
+        group_sync(E, B, kind > 0, false);
+        group_events(B);
+        HIP_OK(hipEventRecord(B.ev_a, B.stream));
+        if (kind) steady_chunk<true>(E, B.g, kind == 2, B.lv);
+        else steady_chunk<false>(E, B.g, false, B.lv);
+        const bool refined = E.opt.precision == HS_PREC_MIXED || kind == 2 || !B.lv.empty();
+        gather(B, {S_DONE, refined ? S_TITERS : S_ITERS, S_NREFINE});
+        HIP_OK(hipEventRecord(B.ev_b, B.stream));
+        B.ran = B.g.nprob;
+        B.busy = true;
+    }
+    for (int h = 0; h < 2 * kTransParts; h++) {
+        BatchGroup &B = E.trans[h];
+        if (B.busy || B.members.empty()) continue;
+        B.pool = &E.dpool;
+        B.hpool = &E.hpool;
+        B.stream = E.s_trans[h / 2];
+        group_sync(E, B, h % 2, true);
+        group_events(B);
+        // the intervals left to the member furthest from its end, within the
+        // tick's intervals and the device-time slice
+        int rem = 1;
+        for (const EngJob *j : B.members) rem = std::max(rem, j->nsteps - j->step_seen);
+        int n = std::min(E.tick_intervals, rem);
+        if (B.ms_per_interval > 0) n = std::min(n, std::max(1, (int)(kSliceMs / B.ms_per_interval)));
+        B.chunk = n;
+        const int kl = B.g.kgraph > 0 ? std::min(B.g.kgraph, B.form.K) : B.form.K;
+        for (EngJob *j : B.members)
+            if (j->kplan && kl < j->kplan && (j->kmin == 0 || kl < j->kmin)) j->kmin = kl;
+        HIP_OK(hipEventRecord(B.ev_a, B.stream));
+        B.rp[B.g.kgraph].run(B.stream, n, [&] { seq_body(B.g, B.form); });
+        gather(B, {S_STEP, S_TCALLS, S_TTERMS, S_TBYTES, S_NHOLD});
+        if (B.form.transient == HS_TRANS_TAYLOR) {
+            // the term counts of this chunk; the largest since cleared starts again
+            HIP_OK(hipMemcpyAsync(B.h_kmax, B.g.d_kmax, 2 * sizeof(int), hipMemcpyDeviceToHost, B.stream));
+            HIP_OK(hipMemsetAsync(B.g.d_kmax + 1, 0, sizeof(int), B.stream));
+        }
+        HIP_OK(hipEventRecord(B.ev_b, B.stream));
+        B.ran = B.g.nprob;
+        B.busy = true;
+    }
+}
+
+static bool chunk_done(BatchGroup &B) {
+    if (!B.busy) return false;
+    const hipError_t r = hipEventQuery(B.ev_b);
+    if (r == hipErrorNotReady) return false;
+    HIP_OK(r);
+    return true;
+}
+
+// waits until some group's chunk has ended, or kWaitMs
+static void wait_chunk(Engine &E) {
+    const double t0 = now_s();
+    for (;;) {
+        bool busy = false, done = false;
+        for (auto &kv : E.steady) {
+            busy |= kv.second.busy;
+            done |= chunk_done(kv.second);
+        }
+        for (auto &B : E.trans) {
+            busy |= B.busy;
+            done |= chunk_done(B);
+        }
+        if (!busy || done || now_s() - t0 > kWaitMs * 1e-3) return;
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+
+// results of every group whose chunk has ended: finished jobs leave their
+// groups (a steady solve with intervals to follow joins a transient group)
+static void harvest(Engine &E) {
+    for (auto &kv : E.steady) {
+        BatchGroup &B = kv.second;
+        if (!chunk_done(B)) continue;
+        B.busy = false;
+        std::vector<std::pair<EngJob *, std::vector<double>>> fin;
+        for (int i = 0; i < B.ran; i++)
+            if (B.h_flags[i * 8] != 0.0)
+                fin.push_back({B.members[i], std::vector<double>(B.h_flags + i * 8, B.h_flags + i * 8 + 8)});
+        for (auto &f : fin) {
+            B.remove(f.first);
+            steady_done(E, *f.first, f.second.data());
+        }
+    }
+    for (int h = 0; h < 2 * kTransParts; h++) {
+        BatchGroup &B = E.trans[h];
+        if (!chunk_done(B)) continue;
+        B.busy = false;
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, B.ev_a, B.ev_b));
+        B.ms_per_interval = (double)ms / std::max(1, B.chunk);
+        // Taylor groups of one sub-step per interval launch the Horner steps
+        // that the last chunk's largest term count needs instead of every
+        // step of the plan, so no launch skips every problem.  A problem that
+        // needs more is held (its interval not advanced) until the next chunk
+        // raises the count; each count's interval body is captured once.
+        if (B.form.transient == HS_TRANS_TAYLOR && B.form.m == 1 && B.h_kmax[1] > 0) {
+            const int want = std::min(B.form.K, std::max(2, B.h_kmax[1]));
+            B.g.kgraph = want >= B.form.K ? 0 : want;
+        }
+        std::vector<std::pair<EngJob *, std::vector<double>>> fin;
+        for (int i = 0; i < B.ran; i++) {
+            EngJob *j = B.members[i];
+            const double *f = B.h_flags + i * 8;
+            j->step_seen = (int)f[0];
+            j->holds = (int)f[4];
+            if (f[0] >= (double)j->nsteps) fin.push_back({j, std::vector<double>(f, f + 8)});
+        }
+        for (auto &f : fin) {
+            B.remove(f.first);
+            trans_done(E, *f.first, f.second.data());
+        }
+    }
+}
+
 // ---- entry points (wrapped by the C ABI) -----------------------------------------
 Engine *eng_create(int precision, int transient) {
     std::unique_ptr<Engine> E(new Engine());
+    E->hpool.host = true;
     E->opt.precision = precision;
     // the mixed (served) form solves steady states with multigrid-preconditioned CG
     E->opt.steady_solver = precision == HS_PREC_MIXED ? HS_STEADY_MGPCG : HS_STEADY_JPCG;
@@ -430,6 +585,7 @@ Engine *eng_create(int precision, int transient) {
     size_t fr = 0, tot = 0;
     HIP_OK(hipMemGetInfo(&fr, &tot));
     E->budget = (size_t)(0.8 * (double)fr);
+    HIP_OK(hipStreamCreateWithFlags(&E->s_setup, hipStreamNonBlocking));
     HIP_OK(hipStreamCreateWithFlags(&E->s_steady, hipStreamNonBlocking));
     for (auto &s : E->s_trans) HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     return E.release();
@@ -502,6 +658,8 @@ int eng_admit(Engine &E) {
     while (!E.queue.empty()) {
         EngJob &j = *E.queue.front();
         if (E.used + j.footprint > E.budget && E.used > 0) break;
+        // the pool's idle blocks count against the budget until returned
+        if (E.used + E.dpool.idle_bytes + j.footprint > E.budget) E.dpool.trim();
         E.queue.pop_front();
         admit(E, j);
     }
@@ -510,73 +668,9 @@ int eng_admit(Engine &E) {
 
 int eng_tick(Engine &E) {
     eng_admit(E);
-    // members launched this tick per group: only their gathered flags are
-    // fresh (a group may gain members below, or be empty this tick)
-    std::map<int, int> ran_steady;
-    int ran_trans[2 * kTransStreams] = {};
-    for (auto &kv : E.steady) {
-        BatchGroup &B = kv.second;
-        const int kind = kv.first % 4;
-        if (B.members.empty()) continue;
-        group_sync(E, B, E.s_steady, kind > 0, false);
-        ran_steady[kv.first] = B.g.nprob;
-        if (kind) steady_chunk<true>(E, B.g, kind == 2, B.lv);
-        else steady_chunk<false>(E, B.g, false, B.lv);
-        const bool refined = E.opt.precision == HS_PREC_MIXED || kind == 2 || !B.lv.empty();
-        gather(B, {S_DONE, refined ? S_TITERS : S_ITERS, S_NREFINE});
-    }
-    for (int h = 0; h < 2 * kTransStreams; h++) {
-        BatchGroup &B = E.trans[h];
-        if (B.members.empty()) continue;
-        group_sync(E, B, E.s_trans[h / 2], h % 2, true);
-        ran_trans[h] = B.g.nprob;
-        B.rp.run(E.s_trans[h / 2], E.tick_intervals, [&] { seq_body(B.g, B.form); });
-        gather(B, {S_STEP, S_TCALLS, S_TTERMS, S_TBYTES});
-    }
-    HIP_OK(hipStreamSynchronize(E.s_steady));
-    for (auto st : E.s_trans) HIP_OK(hipStreamSynchronize(st));
-    // Taylor groups of one sub-step per interval launch the Horner steps
-    // that the tick's largest term count needs, plus one: fewer launches that
-    // skip every problem.  A problem that needs more is held (its interval
-    // not advanced) until the next tick raises the count.  A change
-    // recaptures the group's graph; the count falls only by two or more.
-    for (int h = 0; h < 2 * kTransStreams; h++) {
-        BatchGroup &B = E.trans[h];
-        if (!ran_trans[h] || B.form.transient != HS_TRANS_TAYLOR || B.form.m != 1 || !kgraph_enabled()) continue;
-        int kt[2] = {0, 0};
-        HIP_OK(hipMemcpy(kt, B.g.d_kmax, sizeof kt, hipMemcpyDeviceToHost));
-        HIP_OK(hipMemsetAsync(B.g.d_kmax + 1, 0, sizeof(int), E.s_trans[h / 2]));
-        if (kt[1] <= 0) continue;
-        const int cur = B.g.kgraph > 0 ? B.g.kgraph : B.form.K;
-        static const int margin = getenv("HS_KGRAPH_MARGIN") ? std::max(0, atoi(getenv("HS_KGRAPH_MARGIN"))) : 1;
-        const int want = std::min(B.form.K, std::max(2, kt[1] + margin));
-        if (kt[1] > cur || want < cur - 1) {
-            B.g.kgraph = want >= B.form.K ? 0 : want;
-            B.rp.reset();
-        }
-    }
-    for (auto &kv : E.steady) {
-        BatchGroup &B = kv.second;
-        std::vector<std::pair<EngJob *, std::vector<double>>> fin;
-        for (int i = 0; i < ran_steady[kv.first]; i++)
-            if (B.h_flags[i * 8] != 0.0)
-                fin.push_back({B.members[i], std::vector<double>(B.h_flags + i * 8, B.h_flags + i * 8 + 8)});
-        for (auto &f : fin) {
-            B.remove(f.first);
-            steady_done(E, *f.first, f.second.data());
-        }
-    }
-    for (int h = 0; h < 2 * kTransStreams; h++) {
-        BatchGroup &B = E.trans[h];
-        std::vector<std::pair<EngJob *, std::vector<double>>> fin;
-        for (int i = 0; i < ran_trans[h]; i++)
-            if (B.h_flags[i * 8] >= (double)B.members[i]->nsteps)
-                fin.push_back({B.members[i], std::vector<double>(B.h_flags + i * 8, B.h_flags + i * 8 + 8)});
-        for (auto &f : fin) {
-            B.remove(f.first);
-            trans_done(E, *f.first, f.second.data());
-        }
-    }
+    launch_chunks(E);
+    wait_chunk(E);
+    harvest(E);
     return eng_active(E);
 }
 
@@ -612,6 +706,9 @@ void eng_job_info(Engine &E, int id, hs_job_info_t *info) {
     info->wait_s = j.phase == J_QUEUED ? now_s() - j.t_submit : j.t_admit - j.t_submit;
     info->latency_s = j.phase == J_DONE ? j.t_done - j.t_submit : now_s() - j.t_submit;
     info->device_bytes = (double)(j.held ? j.held : j.footprint);
+    info->held_intervals = j.holds;
+    info->horner_steps_min = j.kmin;
+    info->horner_steps_plan = j.kplan;
 }
 
 void eng_release(Engine &E, int id) {

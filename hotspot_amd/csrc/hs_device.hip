@@ -1001,7 +1001,6 @@ struct FinArgs {
     int *kmax;     // FIN_TAY: [0] largest term count of the group's sub-step, [1] the same since the host cleared it
     double *trec;  // FIN_TAY: per problem {tau, terms (0: no sub-step)}
     int kg;        // FIN_TAY: the group's plan (launches of the sub-step)
-    int pairs;     // FIN_TAY: steps run two per sweep (k_tay_pair)
     int fused;     // FIN_TAY: the last step adds its result into the state (k_tay_term4)
 };
 
@@ -1833,6 +1832,7 @@ __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ 
             const bool hold = K > a.kg;
             s[S_THOLD] = hold ? 1.0 : 0.0;
             if (hold) {
+                s[S_NHOLD] += 1.0;
                 a.trec[kTrec * blockIdx.x] = s[S_TAU];
                 a.trec[kTrec * blockIdx.x + 1] = 0.0;
                 return;
@@ -1840,12 +1840,7 @@ __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ 
             s[S_TTERMS] += K - 1;
             // algorithmic bytes per node of the dominant sweeps: 12 (read h
             // and E_1, write h), 8 when h is E_1 itself (a problem's first)
-            if (a.pairs) {
-                const bool single = (a.kg % 2 == 0) && K == a.kg;
-                const int jf = (K & 1) ? K : (single ? K - 1 : K + 1);
-                const int np = (K >= 2 && jf >= 3) ? (jf - 3) / 2 + 1 : 0;
-                s[S_TBYTES] += np > 0 ? (single ? 12.0 * np : 8.0 + 12.0 * (np - 1)) : 0.0;
-            } else if (a.fused) {
+            if (a.fused) {
                 // ... and the last step (j = 2) reads the FP64 state and
                 // writes it back instead of writing h: +16 B; a problem with
                 // one term adds E_1 to the state in that launch (20 B)
@@ -2386,369 +2381,18 @@ __global__ void __launch_bounds__(kBlock) k_tay_term4(const ProbDesc *__restrict
     term4_item(D, tay_item(W, nper, trec), e1b, d0b, d1b, j, fin);
 }
 
-// ---------------------------------------------------------------------------
-// Two Horner steps per sweep (FP32, homogeneous layers): h1 = E_1 + s1 M h0,
-// h2 = E_1 + s2 M h1 in one pass over HBM (read h0 and E_1, write h2: 12
-// bytes per node for two steps instead of 24).  A workgroup owns a patch of
-// kPX x kPY cells through every layer and streams down the layers: plane t
-// of h0 (patch + 2) goes to LDS, then h1 on plane t-1 (patch + 1) from three
-// h0 planes, then h2 on plane t-2 (the patch) from three h1 planes.  Ring
-// terms need the package nodes' h1, which depends on whole boundary rows:
-// h2 at the boundary cells of ring layers is written without them, and
-// k_tay_pair_fix (one workgroup per problem) adds them and computes the
-// package nodes.  Launches pair the steps (3, 2), (5, 4), ...; a group whose
-// largest plan Kg is even runs its step Kg alone first.  A problem whose
-// term count K is even and below Kg takes only the second step of its first
-// pair.
-// ---------------------------------------------------------------------------
-constexpr int kPX = 64, kPY = 8;
-
-struct PairPlan {
-    int in, out, mode;   // mode 2: steps j and j-1; 1: step j-1 only
-    float s1, s2, t1, t2;   // factors of steps j and j-1 (Taylor record)
-};
-
-// The pair of top step j for a problem of K terms in a group of plan kg:
-// false when the problem takes no step of it.  Buffers alternate between the
-// problem's active launches, the first reading E_1.
-__device__ __forceinline__ bool pair_plan(int K, int kg, int j, const double *rec, int e1b, int d0b, int d1b,
-                                          PairPlan &p) {
-    if (K < 2 || j - 1 > K) return false;
-    const bool single = (kg % 2 == 0) && K == kg;   // step K ran alone first
-    const int jf = (K & 1) ? K : (single ? K - 1 : K + 1);
-    if (j > jf) return false;
-    const int idx = (single ? 1 : 0) + (jf - j) / 2;
-    p.mode = j <= K ? 2 : 1;
-    p.in = idx == 0 ? e1b : (((idx - 1) & 1) ? d1b : d0b);
-    p.out = (idx & 1) ? d1b : d0b;
-    const float *rf = (const float *)(rec + kTrecF);
-    p.s1 = rf[2 * j];
-    p.t1 = rf[2 * j + 1];
-    p.s2 = rf[2 * (j - 1)];
-    p.t2 = rf[2 * (j - 1) + 1];
-    return true;
-}
-
-// the buffer holding the sum after the last launch of a paired sequence
-__device__ __forceinline__ int pair_result(int K, int kg, int e1b, int d0b, int d1b) {
-    if (K < 2) return e1b;
-    const bool single = (kg % 2 == 0) && K == kg;
-    const int jf = (K & 1) ? K : (single ? K - 1 : K + 1);
-    const int n_act = (single ? 1 : 0) + (jf >= 3 ? (jf - 3) / 2 + 1 : 0);
-    return ((n_act - 1) & 1) ? d1b : d0b;
-}
-
-// per-layer coefficients of the FP32 homogeneous stencil
-struct LayerCoef {
-    float gx, gy, gu, gd, ci, gam, rgn, rge;
-    int rn[4];
-};
-__device__ __forceinline__ LayerCoef layer_coef(const ProbDesc &d, int n) {
-    LayerCoef L;
-    L.gx = (float)d.gx[n];
-    L.gy = (float)d.gy[n];
-    L.gu = n > 0 ? (float)d.gz[n - 1] : 0.f;
-    L.gd = n < d.nl - 1 ? (float)d.gz[n] : 0.f;
-    L.ci = (float)d.cinv[n];
-    L.gam = (float)d.gamb[n];
-    L.rgn = (float)d.ring_gns[n];
-    L.rge = (float)d.ring_gew[n];
-    for (int e = 0; e < 4; e++) L.rn[e] = d.ring_node[n][e];
-    return L;
-}
-
-// position of a perimeter cell (i, j) in a layer's perimeter row of the
-// side buffer: top row, bottom row, then the left and right columns
-__device__ __forceinline__ int perim_len(const ProbDesc &d) { return 2 * d.nc + 2 * (d.nr - 2); }
-__device__ __forceinline__ int perim_idx(const ProbDesc &d, int i, int j) {
-    if (i == 0) return j;
-    if (i == d.nr - 1) return d.nc + j;
-    if (j == 0) return 2 * d.nc + (i - 1);
-    return 2 * d.nc + (d.nr - 2) + (i - 1);
-}
-__device__ __forceinline__ bool has_ring(const LayerCoef &L) {
-    return L.rn[0] >= 0 || L.rn[1] >= 0 || L.rn[2] >= 0 || L.rn[3] >= 0;
-}
-
-// One Horner step at a cell from its 6 neighbours (missing ones excluded), in
-// the order of stencil(); ring terms when ring != nullptr.
-__device__ __forceinline__ float horner_cell(const LayerCoef &L, float e1, float s, float sft, float xc, float xn,
-                                             float xs,
-                                             float xe, float xw, float xu, float xd, bool hn, bool hs, bool he,
-                                             bool hw, bool hu, bool hd, const float *ring, int i, int j, int nr,
-                                             int nc) {
-    float flux = 0.f;
-    if (hn) flux = L.gx * (xn - xc);
-    if (hs) flux += L.gx * (xs - xc);
-    if (he) flux += L.gy * (xe - xc);
-    if (hw) flux += L.gy * (xw - xc);
-    if (hu) flux += L.gu * (xu - xc);
-    if (hd) flux += L.gd * (xd - xc);
-    if (ring) {
-        if (i == 0 && L.rn[0] >= 0) flux += L.rgn * (ring[0] - xc);
-        if (i == nr - 1 && L.rn[1] >= 0) flux += L.rgn * (ring[1] - xc);
-        if (j == nc - 1 && L.rn[2] >= 0) flux += L.rge * (ring[2] - xc);
-        if (j == 0 && L.rn[3] >= 0) flux += L.rge * (ring[3] - xc);
-    }
-    return e1 + s * (L.ci * (flux - L.gam * xc)) + sft * xc;
-}
-
-// 8 workgroups per CU (64 VGPRs, ~20 KB of LDS each): a 16-job batch of
-// 256 x 256 grids (2048 patches) is resident at once.  The loop body is
-// branch-free (clamped addresses, a spare LDS row for cells outside a
-// region) so that the loads issued one stage ahead stay in flight.
-__global__ void __launch_bounds__(kBlock, 8) k_tay_pair(const ProbDesc *__restrict__ D,
-                                                        const WorkItem *__restrict__ W, int nper, int e1b, int d0b,
-                                                        int d1b, int j, int kg, const int *__restrict__ kmax,
-                                                        const double *__restrict__ trec) {
-    if (j - 1 > *kmax) return;
-    const TayItem it = tay_item(W, nper, trec);
-    PairPlan pp;
-    if (!pair_plan(it.K, kg, j, it.rec, e1b, d0b, d1b, pp)) return;
-    const ProbDesc &d = D[it.prob];
-    const int nr = d.nr, nc = d.nc, nl = d.nl;
-    const int PX = min(kPX, nc), PY = min(kPY, nr);
-    const int npj = nc / PX;
-    const int pi = it.tile / npj, pj = it.tile - pi * npj;
-    const int i0 = pi * PY, j0 = pj * PX;
-    const float *h0 = d.b32[pp.in], *e1 = d.b32[e1b];
-    float *out = d.b32[pp.out];
-    const unsigned plane = (unsigned)nr * nc, ncell = plane * nl;
-    __shared__ float H0[3][kPY + 5][kPX + 4];   // last row: writes of cells outside the region
-    __shared__ float H1[3][kPY + 3][kPX + 2];
-    __shared__ LayerCoef LC[kMaxLayers];
-    __shared__ float XR[kMaxLayers][4];
-    for (int n = threadIdx.x; n < nl; n += kBlock) {
-        LC[n] = layer_coef(d, n);
-        for (int e = 0; e < 4; e++) XR[n][e] = ld_at(h0, ncell + max(d.ring_node[n][e], 0));
-    }
-    // Fixed cells of each thread in the three regions (patch + 2, patch + 1,
-    // patch): rows ty, ty + 4, ty + 8 of column tx, and, for 64-wide patches,
-    // the columns past 64 taken by the first threads.
-    const int tx = threadIdx.x & (kWave - 1), ty = threadIdx.x >> 6;
-    const int W0 = PX + 4, R0 = PY + 4, W1 = PX + 2, R1 = PY + 2;
-    int ar[4], ac[4], cr[4], cc[4];
-    unsigned ag[4], cg[4], eg[2];
-    bool ain[4], cin_[4], cok[4];
-    for (int k = 0; k < 4; k++) {
-        int r, c;
-        if (k < 3) {
-            r = ty + 4 * k;
-            c = tx;
-        } else {
-            r = threadIdx.x >> 2;
-            c = 64 + (threadIdx.x & 3);
-        }
-        const bool reg = r < R0 && c < W0 && (k < 3 || W0 > 64);
-        const int gi = i0 - 2 + r, gj = j0 - 2 + c;
-        ain[k] = reg && gi >= 0 && gi < nr && gj >= 0 && gj < nc;
-        ar[k] = reg ? r : kPY + 4;
-        ac[k] = reg ? c : 0;
-        ag[k] = ain[k] ? (unsigned)(gi * nc + gj) : 0u;
-    }
-    for (int k = 0; k < 4; k++) {
-        int r, c;
-        if (k < 3) {
-            r = ty + 4 * k;
-            c = tx;
-        } else {
-            r = threadIdx.x >> 1;
-            c = 64 + (threadIdx.x & 1);
-        }
-        const bool reg = r < R1 && c < W1 && (k < 3 || W1 > 64);
-        const int gi = i0 - 1 + r, gj = j0 - 1 + c;
-        cin_[k] = reg && gi >= 0 && gi < nr && gj >= 0 && gj < nc;
-        cok[k] = reg;
-        cr[k] = reg ? r : 0;
-        cc[k] = reg ? c : 0;
-        cg[k] = cin_[k] ? (unsigned)(gi * nc + gj) : 0u;
-    }
-    bool eok[2];
-    for (int k = 0; k < 2; k++) {
-        const int r = ty + 4 * k;
-        eok[k] = r < PY && tx < PX;
-        eg[k] = eok[k] ? (unsigned)((i0 + r) * nc + j0 + tx) : 0u;
-    }
-    float pa[4], pc[4], pe[2];
-    for (int k = 0; k < 4; k++) pa[k] = ld_at(h0, ag[k]);
-    for (int t = 0; t <= nl + 1; t++) {
-        if (t < nl) {   // h0 of plane t on the patch + 2 (0 outside the grid)
-            for (int k = 0; k < 4; k++) H0[t % 3][ar[k]][ac[k]] = ain[k] ? pa[k] : 0.f;
-            const unsigned tp = (unsigned)min(t + 1, nl - 1) * plane;
-            for (int k = 0; k < 4; k++) pa[k] = ld_at(h0, tp + ag[k]);
-        }
-        __syncthreads();
-        const int m = t - 1;   // h1 of plane m on the patch + 1
-        if (m >= 0 && m < nl) {
-            const LayerCoef &L = LC[m];
-            const float *ring = XR[m];
-            const int sm = m % 3, su = (m + 2) % 3, sd = (m + 1) % 3;
-            const float s = pp.mode == 2 ? pp.s1 : pp.s2, sft = pp.mode == 2 ? pp.t1 : pp.t2;
-            const bool hu = m > 0, hd = m < nl - 1;
-            for (int k = 0; k < 4; k++) {
-                const int r = cr[k], c = cc[k];
-                const int gi = i0 - 1 + r, gj = j0 - 1 + c;
-                const float xc = H0[sm][r + 1][c + 1];
-                const float v = horner_cell(L, pc[k], s, sft, xc, H0[sm][r][c + 1], H0[sm][r + 2][c + 1],
-                                            H0[sm][r + 1][c + 2], H0[sm][r + 1][c], H0[su][r + 1][c + 1],
-                                            H0[sd][r + 1][c + 1], gi > 0, gi < nr - 1, gj < nc - 1, gj > 0, hu, hd,
-                                            ring, gi, gj, nr, nc);
-                H1[sm][cok[k] ? r : kPY + 2][c] = cin_[k] ? v : 0.f;
-                // h1 on the perimeter of a ring layer, for the fix-up (each
-                // cell written by the patch that owns it)
-                if (pp.mode == 2 && cin_[k] && (gi == 0 || gi == nr - 1 || gj == 0 || gj == nc - 1) && r >= 1 &&
-                    r <= PY && c >= 1 && c <= PX && has_ring(L))
-                    st_at(d.perim_h1, (unsigned)(m * perim_len(d) + perim_idx(d, gi, gj)), v);
-            }
-        }
-        if (t < nl)   // E_1 of the next h1 plane
-            for (int k = 0; k < 4; k++) pc[k] = ld_at(e1, (unsigned)t * plane + cg[k]);
-        __syncthreads();
-        const int m2 = t - 2;   // h2 of plane m2 on the patch (or h1 alone in mode 1)
-        if (m2 >= 0 && m2 < nl) {
-            const LayerCoef &L = LC[m2];
-            const int sm = m2 % 3, su = (m2 + 2) % 3, sd = (m2 + 1) % 3;
-            const bool hu = m2 > 0, hd = m2 < nl - 1;
-            for (int k = 0; k < 2; k++) {
-                const int r = ty + 4 * k, c = tx;
-                const int rr = eok[k] ? r : 0, cq = eok[k] ? c : 0;
-                const int gi = i0 + rr, gj = j0 + cq;
-                float v;
-                if (pp.mode == 1) {
-                    v = H1[sm][rr + 1][cq + 1];
-                } else {
-                    v = horner_cell(L, pe[k], pp.s2, pp.t2, H1[sm][rr + 1][cq + 1], H1[sm][rr][cq + 1], H1[sm][rr + 2][cq + 1],
-                                    H1[sm][rr + 1][cq + 2], H1[sm][rr + 1][cq], H1[su][rr + 1][cq + 1],
-                                    H1[sd][rr + 1][cq + 1], gi > 0, gi < nr - 1, gj < nc - 1, gj > 0, hu, hd,
-                                    nullptr, gi, gj, nr, nc);
-                }
-                if (eok[k]) st_at(out, (unsigned)m2 * plane + eg[k], v);
-            }
-        }
-        if (t - 1 >= 0 && t - 1 < nl)   // E_1 of the next h2 plane
-            for (int k = 0; k < 2; k++) pe[k] = ld_at(e1, (unsigned)(t - 1) * plane + eg[k]);
-    }
-}
-
-// After a pair launch, kPairFixBlocks workgroups per problem: the package
-// nodes' h1 (from h0) and h2 (from h1 on the boundary rows, saved by
-// k_tay_pair), and the ring terms of h2 on the perimeter of ring layers.
-// Mode 1: the package nodes' one step.
-constexpr int kPairFixBlocks = 4;
-
-__global__ void __launch_bounds__(kBlock) k_tay_pair_fix(const ProbDesc *__restrict__ D, int e1b, int d0b,
-                                                         int d1b, int j, int kg, const int *__restrict__ kmax,
-                                                         const double *__restrict__ trec) {
-    if (j - 1 > *kmax) return;
-    const int p = blockIdx.x / kPairFixBlocks, blk = blockIdx.x - p * kPairFixBlocks;
-    PairPlan pp;
-    if (!pair_plan((int)trec[kTrec * p + 1], kg, j, trec + kTrec * p, e1b, d0b, d1b, pp)) return;
-    const ProbDesc &d = D[p];
-    const int nr = d.nr, nc = d.nc, nl = d.nl, nx = d.nx;
-    const unsigned plane = (unsigned)nr * nc, ncell = plane * nl;
-    const float *h0 = d.b32[pp.in], *e1 = d.b32[e1b];
-    float *out = d.b32[pp.out];
-    __shared__ float h1x[kMaxExtra];
-    __shared__ double s0[kMaxExtra], s1[kMaxExtra];
-    if (pp.mode == 1) {
-        if (blk) return;
-        __shared__ float xf[kMaxExtra], xd[kMaxExtra];
-        LdPlain<float> ld{h0};
-        package_terms<float>(d, ld, true, xf, xd);
-        const int k = threadIdx.x;
-        if (k < nx)
-            out[ncell + k] = e1[ncell + k] + pp.s2 * ((float)d.x_cinv[k] * (xf[k] - (float)d.x_gamb[k] * h0[ncell + k])) +
-                             pp.t2 * h0[ncell + k];
-        return;
-    }
-    // ring sums of h0 and of the saved h1 over each package node's boundary row
-    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-    const int plen = perim_len(d);
-    for (int k = w; k < nx; k += kBlock / kWave) {
-        const int L_ = d.x_ring_layer[k];
-        double a0 = 0.0, a1 = 0.0;
-        if (L_ >= 0) {
-            const int e = d.x_ring_edge[k];
-            const int m = e < 2 ? nc : nr;
-            const float xk = h0[ncell + k];
-            for (int t = lane; t < m; t += kWave) {
-                const int i = e == 0 ? 0 : e == 1 ? nr - 1 : t, jj = e == 2 ? nc - 1 : e == 3 ? 0 : t;
-                a0 += (double)(ld_at(h0, L_ * plane + (unsigned)i * nc + jj) - xk);
-                a1 += (double)d.perim_h1[L_ * plen + perim_idx(d, i, jj)];
-            }
-            a0 = wave_sum(a0);
-            a1 = wave_sum(a1);
-        }
-        if (lane == 0) {
-            s0[k] = a0;
-            s1[k] = a1;
-        }
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < nx) {   // h1 of the package nodes (the terms of package_terms)
-        const int k = threadIdx.x, L_ = d.x_ring_layer[k];
-        const float xk = h0[ncell + k];
-        float flux = L_ >= 0 ? (float)d.x_ring_g[k] * (float)s0[k] : 0.f;
-        for (int q = 0; q < d.x_ncpl[k]; q++) {
-            if (d.x_cpl_steady_only[k][q]) continue;
-            flux += (float)d.x_cpl_g[k][q] * (h0[ncell + d.x_cpl_node[k][q]] - xk);
-        }
-        h1x[k] = e1[ncell + k] + pp.s1 * ((float)d.x_cinv[k] * (flux - (float)d.x_gamb[k] * xk)) + pp.t1 * xk;
-    }
-    __syncthreads();
-    // ring terms of h2 on the perimeter of every ring layer, cells spread
-    // over the problem's workgroups (one writer per cell)
-    for (int q = blk * kBlock + threadIdx.x; q < nl * plen; q += kPairFixBlocks * kBlock) {
-        const int n = q / plen, pq = q - n * plen;
-        const int *rn = d.ring_node[n];
-        if (rn[0] < 0 && rn[1] < 0 && rn[2] < 0 && rn[3] < 0) continue;
-        int i, jj;
-        if (pq < nc) { i = 0; jj = pq; }
-        else if (pq < 2 * nc) { i = nr - 1; jj = pq - nc; }
-        else if (pq < 2 * nc + nr - 2) { i = pq - 2 * nc + 1; jj = 0; }
-        else { i = pq - 2 * nc - (nr - 2) + 1; jj = nc - 1; }
-        const bool tn = i == 0 && rn[0] >= 0, ts = i == nr - 1 && rn[1] >= 0;
-        const bool te = jj == nc - 1 && rn[2] >= 0, tw = jj == 0 && rn[3] >= 0;
-        if (!(tn || ts || te || tw)) continue;
-        const float h1c = d.perim_h1[q];
-        const float rgn = (float)d.ring_gns[n], rge = (float)d.ring_gew[n];
-        float f = 0.f;
-        if (tn) f += rgn * (h1x[rn[0]] - h1c);
-        if (ts) f += rgn * (h1x[rn[1]] - h1c);
-        if (te) f += rge * (h1x[rn[2]] - h1c);
-        if (tw) f += rge * (h1x[rn[3]] - h1c);
-        const unsigned c = n * plane + (unsigned)i * nc + jj;
-        out[c] += pp.s2 * ((float)d.cinv[n] * f);
-    }
-    if (blk == 0 && (int)threadIdx.x < nx) {   // h2 of the package nodes
-        const int k = threadIdx.x, L_ = d.x_ring_layer[k];
-        float flux = 0.f;
-        if (L_ >= 0) {
-            const int m = d.x_ring_edge[k] < 2 ? nc : nr;
-            flux = (float)d.x_ring_g[k] * (float)(s1[k] - (double)m * h1x[k]);
-        }
-        for (int q = 0; q < d.x_ncpl[k]; q++) {
-            if (d.x_cpl_steady_only[k][q]) continue;
-            flux += (float)d.x_cpl_g[k][q] * (h1x[d.x_cpl_node[k][q]] - h1x[k]);
-        }
-        out[ncell + k] = e1[ncell + k] + pp.s2 * ((float)d.x_cinv[k] * (flux - (float)d.x_gamb[k] * h1x[k])) +
-                         pp.t2 * h1x[k];
-    }
-}
-
 // y += h (the sum of the sub-step's terms)
 template <typename TD>
 __global__ void __launch_bounds__(kBlock) k_tay_final(const ProbDesc *__restrict__ D,
                                                       const WorkItem *__restrict__ W, int nper, int e1b,
-                                                      int d0b, int d1b, const double *__restrict__ trec,
-                                                      int pair_kg) {
+                                                      int d0b, int d1b, const double *__restrict__ trec) {
     const TayItem it = tay_item(W, nper, trec);
     if (it.K == 0) return;
     const ProbDesc &d = D[it.prob];
     const WorkItem wi{it.prob, it.tile};
     const int ypar = (int)d.scal[S_YPAR];
     double *y = d.b64[B_Y0 + ypar];
-    const int hb = pair_kg > 0 ? pair_result(it.K, pair_kg, e1b, d0b, d1b) : horner_result(it.K, e1b, d0b, d1b);
+    const int hb = horner_result(it.K, e1b, d0b, d1b);
     const TD *acc = sizeof(TD) == 8 ? (const TD *)d.b64[hb] : (const TD *)d.b32[hb];
     if (wi.tile < d.ntiles) {
         struct V {

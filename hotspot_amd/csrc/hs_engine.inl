@@ -1,10 +1,106 @@
 // Host side of the device engine: per-model device state, descriptor
 // building, heterogeneous assembly, and the solve drivers.  Included at the
 // end of hs_device.hip (same translation unit as the kernels).
+#include <array>
+#include <map>
 #include <memory>
 #include <numeric>
+#include <unordered_map>
 
 namespace hs {
+
+// Device (or pinned host) memory kept for reuse instead of returned to the
+// runtime.  hipFree waits for every stream of the device, so in the engine a
+// job that ends would stall the launches of every other resident job; its
+// blocks go back to the pool instead and serve the next job of a similar
+// size.  Blocks are reused by size class (steps of 1/8 between powers of
+// two, 256 B below 4 KiB).
+struct DevPool {
+    bool host = false;
+    std::multimap<size_t, void *> idle;
+    std::unordered_map<void *, size_t> cls;
+    size_t idle_bytes = 0, total_bytes = 0;
+    static size_t size_class(size_t n) {
+        if (n <= 4096) return (std::max<size_t>(n, 1) + 255) & ~(size_t)255;
+        size_t p = 4096;
+        while (p * 2 <= n) p *= 2;
+        const size_t step = p / 8;
+        return (n + step - 1) / step * step;
+    }
+    void *get(size_t n, size_t *got) {
+        const size_t s = size_class(n);
+        void *p = nullptr;
+        auto it = idle.find(s);
+        if (it != idle.end()) {
+            p = it->second;
+            idle.erase(it);
+            idle_bytes -= s;
+        } else {
+            if (host) HIP_OK(hipHostMalloc(&p, s, 0));
+            else HIP_OK(hipMalloc(&p, s));
+            cls[p] = s;
+            total_bytes += s;
+        }
+        if (got) *got = s;
+        return p;
+    }
+    void put(void *p) {
+        const size_t s = cls.at(p);
+        idle.insert({s, p});
+        idle_bytes += s;
+    }
+    // returns the idle blocks to the runtime (hipFree waits for the device)
+    void trim() {
+        for (auto &kv : idle) {
+            if (host) (void)hipHostFree(kv.second);
+            else (void)hipFree(kv.second);
+            cls.erase(kv.second);
+            total_bytes -= kv.first;
+        }
+        idle.clear();
+        idle_bytes = 0;
+    }
+    ~DevPool() { trim(); }
+};
+
+static void *dmalloc(DevPool *pool, size_t bytes, size_t *got = nullptr) {
+    if (pool) return pool->get(bytes, got);
+    void *p = nullptr;
+    HIP_OK(hipMalloc(&p, std::max<size_t>(bytes, 1)));
+    if (got) *got = std::max<size_t>(bytes, 1);
+    return p;
+}
+static void dfree(DevPool *pool, void *p) {
+    if (!p) return;
+    if (pool) pool->put(p);
+    else (void)hipFree(p);
+}
+static void *hmalloc(DevPool *pool, size_t bytes) {
+    if (pool) return pool->get(bytes, nullptr);
+    void *p = nullptr;
+    HIP_OK(hipHostMalloc(&p, std::max<size_t>(bytes, 1), 0));
+    return p;
+}
+static void hfree(DevPool *pool, void *p) {
+    if (!p) return;
+    if (pool) pool->put(p);
+    else (void)hipHostFree(p);
+}
+
+// Copies ordered on a stream (never the null stream, whose work can queue
+// behind other streams' launches on a shared hardware queue); they return
+// once the copy is done, so the host source may go out of scope.
+static void h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (!bytes) return;
+    HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    HIP_OK(hipStreamSynchronize(s));
+}
+static void d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (!bytes) return;
+    HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+}
+static void dzero(void *p, size_t bytes, hipStream_t s) { HIP_OK(hipMemsetAsync(p, 0, bytes, s)); }
 
 // A set of problems advanced by shared launches: the descriptor array, the
 // cell-tile work-list (tiles of every problem, then its package item) and the
@@ -26,43 +122,7 @@ struct Group {
     // work-list of the float4 Taylor sweep over the term tiling (t_*)
     WorkItem *d_twork = nullptr;
     int ntwork = 0, tnper = 0;
-    // paired Taylor sweeps (FP32 homogeneous groups): patches through all layers
-    bool pairs = false;
-    WorkItem *d_pwork = nullptr;
-    int npwork = 0, pnper = 0;
 };
-
-// Paired sweeps when HS_PAIRS=1 at model or group set-up.  Off by default:
-// on the 16-job batch of 256 x 256 x 9 stacks a pair sweep takes 70 us
-// against 2 x 24 us for two single float4 sweeps (its LDS and VALU work per
-// layer, not HBM, bounds it), though it halves the HBM bytes per step.
-static bool pairs_enabled() {
-    const char *e = getenv("HS_PAIRS");
-    return e && atoi(e) == 1;
-}
-
-// Engine groups launch the Horner steps their recent term counts need
-// (kgraph); HS_FULL_PLAN=1 launches every step of the plan.
-static bool kgraph_enabled() {
-    static const bool off = getenv("HS_FULL_PLAN") && atoi(getenv("HS_FULL_PLAN")) == 1;
-    return !off;
-}
-
-// Work-list of the paired sweeps: (problem, patch) over kPX x kPY patches
-// (at most the grid) through all layers; no package item (k_tay_pair_fix
-// takes the package nodes).  Returns the items per problem when equal for
-// every problem, else 0.
-static int pair_items(const std::vector<ProbDesc> &desc, std::vector<WorkItem> &w) {
-    w.clear();
-    int per = -1;
-    for (int i = 0; i < (int)desc.size(); i++) {
-        const ProbDesc &h = desc[i];
-        const int n = (h.nr / std::min(kPY, h.nr)) * (h.nc / std::min(kPX, h.nc));
-        for (int t = 0; t < n; t++) w.push_back(WorkItem{i, t});
-        per = (per == -1 || per == n) ? n : 0;
-    }
-    return per < 0 ? 0 : per;
-}
 
 // A homogeneous problem whose tiles are at least 16 columns wide with at
 // least 4 rows per thread runs its FP32 term sweeps with float4 accesses
@@ -110,25 +170,40 @@ struct DevModel {
     Group be_grp;
     double *be_shift = nullptr, *be_dinv = nullptr;
 
+    DevPool *pool = nullptr, *hpool = nullptr;   // an engine's pools (nullptr: the runtime's)
+
+    std::vector<size_t> alloc_sz;       // bytes of each allocation in allocs
+
     template <typename T> T *alloc(size_t n) {
-        void *p = nullptr;
-        const size_t sz = std::max<size_t>(n, 1) * sizeof(T);
-        HIP_OK(hipMalloc(&p, sz));
+        size_t got = 0;
+        void *p = dmalloc(pool, std::max<size_t>(n, 1) * sizeof(T), &got);
         allocs.push_back(p);
-        bytes += sz;
+        alloc_sz.push_back(got);
+        bytes += got;
         return (T *)p;
     }
     template <typename T> T *upload(const T *src, size_t n) {
         T *p = alloc<T>(n);
-        if (n) HIP_OK(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+        h2d(p, src, n * sizeof(T), stream);
         return p;
+    }
+    // returns one allocation of alloc() (e.g. a job's sequence buffers)
+    void release(void *p) {
+        for (size_t k = 0; k < allocs.size(); k++)
+            if (allocs[k] == p) {
+                bytes -= alloc_sz[k];
+                allocs.erase(allocs.begin() + k);
+                alloc_sz.erase(alloc_sz.begin() + k);
+                dfree(pool, p);
+                return;
+            }
     }
     ~DevModel() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (void *p : allocs) (void)hipFree(p);
-        if (grp.d_twork) (void)hipFree(grp.d_twork);
-        if (h_scal) (void)hipHostFree(h_scal);
-        if (h_blk) (void)hipHostFree(h_blk);
+        for (void *p : allocs) dfree(pool, p);
+        dfree(pool, grp.d_twork);
+        hfree(hpool, h_scal);
+        hfree(hpool, h_blk);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream && own_stream) (void)hipStreamDestroy(stream);
@@ -204,11 +279,6 @@ static int ilog2(int v) {
 // along its rows: fewer, longer-lived workgroups amortise each workgroup's
 // fixed cost (work item, descriptor and flag reads) over more cells.
 static int rows_per_thread(int nr, int ty) {
-    static const int forced = [] {
-        const char *e = getenv("HS_ROWS");
-        return e ? atoi(e) : 0;
-    }();
-    if (forced == 1 || forced == 2 || forced == 4 || forced == 8) return forced;
     int r = 1;
     while (r < 8 && nr >= ty * r * 2 * 8) r *= 2;
     return r;
@@ -248,100 +318,109 @@ static int term_items(const std::vector<ProbDesc> &desc, std::vector<WorkItem> &
     return per;
 }
 
-// Tile shape of the float4 term sweep, picked by timing.  Candidates: tile
-// widths of 16 to 128 columns (at most nc) with 4 or 8 rows per thread.
-// Each is timed over a few sweeps of the group's own problems on its stream
-// (one Horner step reading the E_1 buffer and writing a term buffer: both
-// are free between intervals), and the fastest kept.  The arithmetic of a
-// cell is the same under every tiling, so the choice changes only the speed.
-// Choices are remembered per group composition (problem shapes).
+// Tile shape of the float4 term sweep, picked by timing per size class
+// (layers, rows, columns).  Candidates: tile widths of 16 to 128 columns (at
+// most nc) with 4 or 8 rows per thread.  Each is timed over a few sweeps of
+// the group's problems of that class (one Horner step reading the E_1 buffer
+// and writing a term buffer: both are free between intervals), and the
+// fastest kept.  The arithmetic of a cell is the same under every tiling, so
+// the choice changes only the speed.  Choices are remembered per class, so a
+// group whose membership changes times only the classes it has not met.
 struct TermShape {
     int shift = 0, rpt = 0;
 };
-static std::map<std::vector<int>, TermShape> &term_shape_cache() {
-    static std::map<std::vector<int>, TermShape> m;
+using ShapeKey = std::array<int, 3>;
+static std::map<ShapeKey, TermShape> &term_shape_cache() {
+    static std::map<ShapeKey, TermShape> m;
     return m;
 }
+static ShapeKey shape_key(const ProbDesc &h) { return ShapeKey{h.nl, h.nr, h.nc}; }
 
-static void upload_term_items(Group &G, const std::vector<WorkItem> &w, int per, size_t &cap) {
+static void upload_term_items(Group &G, const std::vector<WorkItem> &w, int per, size_t &cap, DevPool *pool,
+                              hipStream_t s) {
     if (w.size() > cap) {
-        if (G.d_twork) HIP_OK(hipFree(G.d_twork));
+        dfree(pool, G.d_twork);
         cap = std::max(w.size(), 2 * cap);
-        HIP_OK(hipMalloc((void **)&G.d_twork, cap * sizeof(WorkItem)));
+        G.d_twork = (WorkItem *)dmalloc(pool, cap * sizeof(WorkItem));
     }
-    HIP_OK(hipMemcpy(G.d_twork, w.data(), w.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    h2d(G.d_twork, w.data(), w.size() * sizeof(WorkItem), s);
     G.ntwork = (int)w.size();
     G.tnper = per;
 }
 
-static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap) {
-    std::vector<int> key;
-    for (const ProbDesc &h : desc) key.insert(key.end(), {h.nl, h.nr, h.nc});
-    auto apply = [&](TermShape t) {
-        for (ProbDesc &h : desc) set_term_tiling(h, std::min(t.shift, ilog2(h.nc)), t.rpt);
-        HIP_OK(hipMemcpy(G.d_desc, desc.data(), desc.size() * sizeof(ProbDesc), hipMemcpyHostToDevice));
-        std::vector<WorkItem> w;
-        const int per = term_items(desc, w);
-        upload_term_items(G, w, per, cap);
-    };
-    static const bool off = getenv("HS_NO_TUNE") != nullptr;
+// Times the candidates on stream s (the group's own stream may be busy
+// with another group's launches) and applies each problem's class shape.
+static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap, DevPool *pool, hipStream_t s) {
     auto &cache = term_shape_cache();
-    auto hit = cache.find(key);
-    if (hit != cache.end() || off) {
-        if (hit != cache.end()) apply(hit->second);
-        else apply(TermShape{desc[0].t_shift, desc[0].t_rpt});
-        return;
-    }
-    int min_nc = 1 << 30;
-    for (const ProbDesc &h : desc) min_nc = std::min(min_nc, h.nc);
-    // every problem active for one step j = 2 (its first: E_1 buffer -> term buffer)
     const int n = (int)desc.size();
-    // the group's Taylor records (step factors) are restored after timing
-    std::vector<double> saved((size_t)kTrec * n), rec((size_t)kTrec * n, 0.0);
-    HIP_OK(hipMemcpy(saved.data(), G.d_trec, saved.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (int i = 0; i < n; i++) {
-        rec[(size_t)kTrec * i] = 1e-9;
-        rec[(size_t)kTrec * i + 1] = 2.0;
-        rec[(size_t)kTrec * i + 4] = 1e-9 / 2;   // s_2 (unshifted)
-        ((float *)&rec[(size_t)kTrec * i + kTrecF])[4] = (float)(1e-9 / 2);
-    }
-    const int two = 2;
-    HIP_OK(hipMemcpy(G.d_trec, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(G.d_kmax, &two, sizeof(int), hipMemcpyHostToDevice));
-    hipEvent_t e0, e1;
-    HIP_OK(hipEventCreate(&e0));
-    HIP_OK(hipEventCreate(&e1));
-    TermShape best;
-    float best_ms = 1e30f;
-    for (int shift = 4; shift <= std::min(7, ilog2(min_nc)); shift++)
-        for (int rpt : {4, 8}) {
-            const TermShape t{shift, rpt};
-            apply(t);
-            constexpr int kWarm = 2, kTimed = 6;
-            for (int r = 0; r < kWarm + kTimed; r++) {
-                if (r == kWarm) HIP_OK(hipEventRecord(e0, G.stream));
-                hipLaunchKernelGGL(k_tay_term4, dim3(G.ntwork), dim3(kBlock), 0, G.stream,
-                                   (const ProbDesc *)G.d_desc, (const WorkItem *)G.d_twork, G.tnper, (int)F_ACC,
-                                   (int)F_D0, (int)F_D1, 2, (const int *)G.d_kmax, (const double *)G.d_trec, 0);
-            }
-            HIP_OK(hipEventRecord(e1, G.stream));
-            HIP_OK(hipEventSynchronize(e1));
-            float ms = 0;
-            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-            static const bool log = getenv("HS_TUNE_LOG") != nullptr;
-            if (log)
-                fprintf(stderr, "term tiling %d problems: %d columns x %d rows per thread: %.2f us per sweep\n",
-                        n, 1 << shift, rpt, 1e3f * ms / kTimed);
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = t;
-            }
+    std::vector<ShapeKey> todo;
+    for (const ProbDesc &h : desc)
+        if (!cache.count(shape_key(h)) && std::find(todo.begin(), todo.end(), shape_key(h)) == todo.end())
+            todo.push_back(shape_key(h));
+    if (!todo.empty()) {
+        // every problem active for one step j = 2 (its first: E_1 buffer ->
+        // term buffer); the group's Taylor records are restored after timing
+        std::vector<double> saved((size_t)kTrec * n), rec((size_t)kTrec * n, 0.0);
+        d2h(saved.data(), G.d_trec, saved.size() * sizeof(double), s);
+        for (int i = 0; i < n; i++) {
+            rec[(size_t)kTrec * i] = 1e-9;
+            rec[(size_t)kTrec * i + 1] = 2.0;
+            rec[(size_t)kTrec * i + 4] = 1e-9 / 2;   // s_2 (unshifted)
+            ((float *)&rec[(size_t)kTrec * i + kTrecF])[4] = (float)(1e-9 / 2);
         }
-    HIP_OK(hipEventDestroy(e0));
-    HIP_OK(hipEventDestroy(e1));
-    HIP_OK(hipMemcpy(G.d_trec, saved.data(), saved.size() * sizeof(double), hipMemcpyHostToDevice));
-    cache[key] = best;
-    apply(best);
+        const int kt[2] = {2, 0};
+        h2d(G.d_trec, rec.data(), rec.size() * sizeof(double), s);
+        h2d(G.d_kmax, kt, sizeof kt, s);
+        hipEvent_t e0, e1;
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        for (const ShapeKey &k : todo) {
+            std::vector<int> mem;
+            for (int i = 0; i < n; i++)
+                if (shape_key(desc[i]) == k) mem.push_back(i);
+            TermShape best;
+            float best_ms = 1e30f;
+            for (int shift = 4; shift <= std::min(7, ilog2(k[2])); shift++)
+                for (int rpt : {4, 8}) {
+                    for (int i : mem) set_term_tiling(desc[i], shift, rpt);
+                    h2d(G.d_desc, desc.data(), desc.size() * sizeof(ProbDesc), s);
+                    std::vector<WorkItem> w;
+                    for (int i : mem)
+                        for (int t = 0; t <= desc[i].t_ntiles; t++) w.push_back(WorkItem{i, t});
+                    upload_term_items(G, w, 0, cap, pool, s);
+                    constexpr int kWarm = 2, kTimed = 6;
+                    for (int r = 0; r < kWarm + kTimed; r++) {
+                        if (r == kWarm) HIP_OK(hipEventRecord(e0, s));
+                        hipLaunchKernelGGL(k_tay_term4, dim3(G.ntwork), dim3(kBlock), 0, s,
+                                           (const ProbDesc *)G.d_desc, (const WorkItem *)G.d_twork, 0, (int)F_ACC,
+                                           (int)F_D0, (int)F_D1, 2, (const int *)G.d_kmax,
+                                           (const double *)G.d_trec, 0);
+                    }
+                    HIP_OK(hipEventRecord(e1, s));
+                    HIP_OK(hipEventSynchronize(e1));
+                    float ms = 0;
+                    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = TermShape{shift, rpt};
+                    }
+                }
+            cache[k] = best;
+        }
+        HIP_OK(hipEventDestroy(e0));
+        HIP_OK(hipEventDestroy(e1));
+        const int zero[2] = {0, 0};
+        h2d(G.d_trec, saved.data(), saved.size() * sizeof(double), s);
+        h2d(G.d_kmax, zero, sizeof zero, s);
+    }
+    for (ProbDesc &h : desc) {
+        const TermShape t = cache.at(shape_key(h));
+        set_term_tiling(h, std::min(t.shift, ilog2(h.nc)), t.rpt);
+    }
+    h2d(G.d_desc, desc.data(), desc.size() * sizeof(ProbDesc), s);
+    std::vector<WorkItem> w;
+    const int per = term_items(desc, w);
+    upload_term_items(G, w, per, cap, pool, s);
 }
 
 
@@ -489,23 +568,25 @@ static void cell_resistances(const Model &M, int n, double *rx, double *ry, doub
 
 // The pressure network of one microchannel layer on the device (k_pr_*):
 // CG iterations in groups of 32 launches between checks of the done flag.
-static void pressure_on_device(hipStream_t stream, int nr, int nc, const std::vector<int> &types, double pump,
-                               std::vector<double> &p) {
+static void pressure_on_device(hipStream_t stream, DevPool *pool, int nr, int nc, const std::vector<int> &types,
+                               double pump, std::vector<double> &p) {
     const long n = (long)nr * nc;
     const int nb = (int)((n + kBlock - 1) / kBlock);
     std::vector<void *> bufs;
     auto alloc = [&](size_t bytes) {
-        void *q = nullptr;
-        HIP_OK(hipMalloc(&q, bytes));
+        void *q = dmalloc(pool, bytes);
         bufs.push_back(q);
         return q;
     };
     struct Free {
         std::vector<void *> &b;
+        DevPool *pool;
+        hipStream_t s;
         ~Free() {
-            for (void *q : b) (void)hipFree(q);
+            (void)hipStreamSynchronize(s);
+            for (void *q : b) dfree(pool, q);
         }
-    } release{bufs};
+    } release{bufs, pool, stream};
     int *t = (int *)alloc(n * sizeof(int));
     double *x = (double *)alloc(n * sizeof(double)), *d = (double *)alloc(n * sizeof(double));
     double *r = (double *)alloc(n * sizeof(double)), *z = (double *)alloc(n * sizeof(double));
@@ -552,7 +633,7 @@ static void assemble_hetero(Model &M, DevModel &D) {
             Microchannel &mc = *M.mc[n];
             // solved once per model, on its first device build
             if (mc.pressure.empty())
-                pressure_on_device(D.stream, nr, nc, mc.types, mc.cfg.pumping_pressure, mc.pressure);
+                pressure_on_device(D.stream, D.pool, nr, nc, mc.types, mc.cfg.pumping_pressure, mc.pressure);
             htc[n] = mc.cfg.htc;
             hydro[n] = mc.hydro_c;
             cv[n] = mc.cfg.coolant_capac;
@@ -677,12 +758,15 @@ static double package_rho(const Model &M, const ProbDesc &h) {
 }
 
 // Device state of a model: coefficients, maps, work-lists and the solver
-// buffers.  With `shared` the launches go to that stream (the engine's).
-static DevModel *dev_build(Model &M, hipStream_t shared) {
+// buffers.  With `shared` the launches go to that stream (the engine's);
+// with pools, device and pinned host memory come from them.
+static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr, DevPool *hpool = nullptr) {
     grid_model_t &g = *M.g;
     if (g.n_layers > kMaxLayers) fatal("too many layers for the device model\n");
     std::unique_ptr<DevModel> own(new DevModel());
     DevModel *D = own.get();
+    D->pool = pool;
+    D->hpool = hpool;
     if (shared) {
         D->stream = shared;
         D->own_stream = false;
@@ -704,16 +788,14 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
     h.vlen = (unsigned)D->vlen;
     for (int b = 0; b < NB64; b++) {
         h.b64[b] = D->alloc<double>(D->vlen);
-        HIP_OK(hipMemset(h.b64[b], 0, D->vlen * sizeof(double)));
+        dzero(h.b64[b], D->vlen * sizeof(double), D->stream);
     }
     h.lzero = D->alloc<int>(kMaxLayers);
-    HIP_OK(hipMemset(h.lzero, 0, kMaxLayers * sizeof(int)));
+    dzero(h.lzero, kMaxLayers * sizeof(int), D->stream);
     for (int b = 0; b < NB32; b++) {
         h.b32[b] = D->alloc<float>(D->vlen);
-        HIP_OK(hipMemset(h.b32[b], 0, D->vlen * sizeof(float)));
+        dzero(h.b32[b], D->vlen * sizeof(float), D->stream);
     }
-    // h1 on every layer's perimeter (paired Taylor sweeps)
-    h.perim_h1 = D->alloc<float>((size_t)h.nl * (2 * h.nc + 2 * std::max(h.nr - 2, 0)));
     // block <-> grid maps
     std::vector<int> base;
     layer_bases(M, base);
@@ -775,15 +857,15 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
     D->nblk = g.total_n_blocks + M.nx;
     h.blk_in = D->alloc<double>(D->nblk);
     h.blk_out = D->alloc<double>(D->nblk);
-    HIP_OK(hipHostMalloc((void **)&D->h_blk, D->nblk * sizeof(double), 0));
-    HIP_OK(hipHostMalloc((void **)&D->h_scal, NSCAL * sizeof(double), 0));
+    D->h_blk = (double *)hmalloc(hpool, D->nblk * sizeof(double));
+    D->h_scal = (double *)hmalloc(hpool, NSCAL * sizeof(double));
     h.partial = D->alloc<double>((size_t)slot_count(h) * kRed);
     h.counter = D->alloc<unsigned>(1);
-    HIP_OK(hipMemset(h.counter, 0, sizeof(unsigned)));
+    dzero(h.counter, sizeof(unsigned), D->stream);
     h.scal = D->alloc<double>(NSCAL);
-    HIP_OK(hipMemset(h.scal, 0, NSCAL * sizeof(double)));
+    dzero(h.scal, NSCAL * sizeof(double), D->stream);
     D->d_desc = D->alloc<ProbDesc>(1);
-    HIP_OK(hipMemcpy(D->d_desc, &h, sizeof(ProbDesc), hipMemcpyHostToDevice));
+    h2d(D->d_desc, &h, sizeof(ProbDesc), D->stream);
     // work lists
     std::vector<WorkItem> w(h.ntiles + 1);
     for (int t = 0; t <= h.ntiles; t++) w[t] = WorkItem{0, t};
@@ -804,29 +886,24 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
     D->grp.hetero = M.detailed;
     D->grp.vec4 = vec4_ok(h);
     D->grp.d_kmax = D->alloc<int>(2);
-    HIP_OK(hipMemset(D->grp.d_kmax, 0, 2 * sizeof(int)));
+    dzero(D->grp.d_kmax, 2 * sizeof(int), D->stream);
     D->grp.d_trec = D->alloc<double>(kTrec);
     D->grp.nper = D->nwork;
     {
         std::vector<WorkItem> tw;
         const int per = term_items({h}, tw);
-        upload_term_items(D->grp, tw, per, D->cap_twork);
-        std::vector<WorkItem> pw;
-        D->grp.pnper = pair_items({h}, pw);
-        D->grp.d_pwork = D->upload(pw.data(), pw.size());
-        D->grp.npwork = (int)pw.size();
-        D->grp.pairs = !D->hetero && pairs_enabled();
+        upload_term_items(D->grp, tw, per, D->cap_twork, pool, D->stream);
     }
     if (D->hetero) assemble_hetero(M, *D);
     // transient step bound
     if (g.c_ready) {
         if (D->hetero) {
             unsigned long long *r = D->alloc<unsigned long long>(1);
-            HIP_OK(hipMemset(r, 0, sizeof *r));
+            dzero(r, sizeof *r, D->stream);
             launch(k_rho, (int)((ncell + kBlock - 1) / kBlock), D->stream, (const ProbDesc *)D->d_desc,
                    ncell, r);
             unsigned long long bits = 0;
-            HIP_OK(hipMemcpy(&bits, r, sizeof bits, hipMemcpyDeviceToHost));
+            d2h(&bits, r, sizeof bits, D->stream);
             double v;
             memcpy(&v, &bits, sizeof v);
             D->rho = v;
@@ -835,7 +912,7 @@ static DevModel *dev_build(Model &M, hipStream_t shared) {
         }
         D->rho = std::max(D->rho, package_rho(M, h));
     }
-    HIP_OK(hipMemcpy(D->d_desc, &h, sizeof(ProbDesc), hipMemcpyHostToDevice));
+    h2d(D->d_desc, &h, sizeof(ProbDesc), D->stream);
     return own.release();
 }
 
@@ -954,7 +1031,7 @@ static void ensure_gmres(DevModel &D, bool f32) {
         D.h.gmv64 = D.alloc<double>((size_t)(kGmM + 1) * D.vlen);
         changed = true;
     }
-    if (changed) HIP_OK(hipMemcpy(D.d_desc, &D.h, sizeof(ProbDesc), hipMemcpyHostToDevice));
+    if (changed) h2d(D.d_desc, &D.h, sizeof(ProbDesc), D.stream);
 }
 
 // One refinement round of the non-symmetric solve: problems whose GMRES
@@ -1048,7 +1125,7 @@ static void build_mg(DevModel &D) {
         for (int b = 0; b < NB32; b++) h.b32[b] = nullptr;
         for (int b : {(int)F_R, (int)F_D0, (int)F_D1}) {
             h.b32[b] = D.alloc<float>(vlen);
-            HIP_OK(hipMemset(h.b32[b], 0, vlen * sizeof(float)));
+            dzero(h.b32[b], vlen * sizeof(float), D.stream);
         }
         h.gE = h.gS = h.gD = h.cinvc = h.dinvc = nullptr;
         h.aN = h.aS = h.aE = h.aW = h.aSelf = nullptr;
@@ -1442,10 +1519,6 @@ struct TaylorPlan {
     double tau = 0.0, tol = 0.0;
     double sigma = 0.0, tau1 = 0.0;   // shift of the series; factor of the slope in E_1
 };
-static bool taylor_shift_enabled() {
-    static const bool off = getenv("HS_TAY_SHIFT") && atoi(getenv("HS_TAY_SHIFT")) == 0;
-    return !off;
-}
 // k! c_k = sum_i (-sigma)^i / (i! (k + i + 1)); sigma <= 2, so 40 terms reach
 // the double rounding
 static double shifted_phi_coef(int k, double sigma) {
@@ -1461,7 +1534,7 @@ static TaylorPlan taylor_plan(const DevModel &D, double dt, double taylor_tol) {
     const double zmax = 4.0;
     p.m = std::max(1, (int)std::ceil(D.rho * dt / zmax));
     p.tau = dt / p.m;
-    const bool shift = taylor_shift_enabled() && !D.h.has_adv;
+    const bool shift = !D.h.has_adv;
     p.sigma = shift ? 0.5 * D.rho * p.tau : 0.0;
     p.tau1 = shift ? p.tau * shifted_phi_coef(0, p.sigma) : p.tau;
     const double z = shift ? p.sigma : D.rho * p.tau;
@@ -1534,43 +1607,28 @@ static void taylor_interval(const Group &G, int m, int Kplan, int seq) {
             launch_named("k_tay_first", k_tay_first2, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax, 1);
         else
             launch_named("k_tay_first", k_tay_first<TD, HET>, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax);
-        const bool pairs = f32 && !HET && G.pairs && K >= 3;
         // float4 sweeps finish the sub-step in their last launch (j = 2)
-        const bool fused = f32 && !HET && G.vec4 && !pairs && K >= 2;
+        const bool fused = f32 && !HET && G.vec4 && K >= 2;
         FinArgs a{};
         a.sub = s;
         a.seq = seq;
         a.kmax = G.d_kmax;
         a.trec = G.d_trec;
         a.kg = K;
-        a.pairs = pairs ? 1 : 0;
         a.fused = fused ? 1 : 0;
         finish(G, FIN_TAY, a);
         const int *km = G.d_kmax;
         const double *tr = G.d_trec;
-        auto single = [&](int j) {
+        for (int j = K; j >= 2; j--) {
             if (f32 && !HET && G.vec4)
                 launch_named("k_tay_term", k_tay_term4, G.ntwork, G.stream, dd, (const WorkItem *)G.d_twork,
                              G.tnper, accb, db0, db1, j, km, tr, (fused && j == 2) ? 1 : 0);
             else
                 launch_named("k_tay_term", k_tay_term<TD, HET>, G.nwork, G.stream, dd, w, G.nper, accb, db0, db1,
                              j, km, tr);
-        };
-        if (pairs) {
-            // step K alone when K is even, then the pairs (j, j-1) down to (3, 2)
-            if (K % 2 == 0) single(K);
-            for (int j = K % 2 ? K : K - 1; j >= 3; j -= 2) {
-                launch_named("k_tay_pair", k_tay_pair, G.npwork, G.stream, dd, (const WorkItem *)G.d_pwork, G.pnper,
-                             accb, db0, db1, j, K, km, tr);
-                launch_named("k_tay_pair_fix", k_tay_pair_fix, G.nprob * kPairFixBlocks, G.stream, dd, accb, db0,
-                             db1, j, K, km, tr);
-            }
-        } else {
-            for (int j = K; j >= 2; j--) single(j);
         }
         if (!fused)
-            launch_named("k_tay_final", k_tay_final<TD>, G.nwork, G.stream, dd, w, G.nper, accb, db0, db1, tr,
-                         pairs ? K : 0);
+            launch_named("k_tay_final", k_tay_final<TD>, G.nwork, G.stream, dd, w, G.nper, accb, db0, db1, tr);
     }
 }
 
@@ -1663,7 +1721,7 @@ static void ensure_term_tuned(const Model &M, DevModel &D) {
     if (D.term_tuned || !D.grp.vec4 || M.opt.transient != HS_TRANS_TAYLOR || M.opt.precision != HS_PREC_MIXED)
         return;
     std::vector<ProbDesc> d{D.h};
-    tune_term_tiling(D.grp, d, D.cap_twork);
+    tune_term_tiling(D.grp, d, D.cap_twork, D.pool, D.stream);
     D.h = d[0];
     D.term_tuned = true;
 }
@@ -1715,10 +1773,7 @@ void dev_transient(Model &M, const double *block_power, const double *block_temp
 // Captures one interval's launches once as a hipGraph and replays it for
 // every interval (the kernels take the interval index from device memory, so
 // the same graph serves all of them).
-static bool graphs_enabled() {
-    static const bool off = getenv("HS_NO_GRAPH") != nullptr;
-    return !off && g_timer.cap == 0;
-}
+static bool graphs_enabled() { return g_timer.cap == 0; }
 
 struct Replayer {
     hipGraph_t graph = nullptr;
@@ -1793,7 +1848,7 @@ static void seq_body(const Group &G, const SeqForm &f) {
 // scalar slots that start a sequence of n intervals of length dt
 static std::vector<std::pair<int, double>> seq_slots(int n, double dt) {
     return {{S_STEP, 0.0}, {S_NSTEPS, (double)n}, {S_TCALLS, 0.0}, {S_CALLS, 0.0},
-            {S_STATUS, (double)RK_DONE}, {S_DT, dt}, {S_TTERMS, 0.0}, {S_TBYTES, 0.0}};
+            {S_STATUS, (double)RK_DONE}, {S_DT, dt}, {S_TTERMS, 0.0}, {S_TBYTES, 0.0}, {S_NHOLD, 0.0}};
 }
 
 void dev_transient_run(Model &M, const double *power_seq, int steps, double dt, const double *init_temp,
@@ -1823,7 +1878,7 @@ void dev_transient_run(Model &M, const double *power_seq, int steps, double dt, 
     tmp.p.push_back(seq_out);
     D.h.seq_in = seq_in;
     D.h.seq_out = seq_out;
-    HIP_OK(hipMemcpy(D.d_desc, &D.h, sizeof(ProbDesc), hipMemcpyHostToDevice));
+    h2d(D.d_desc, &D.h, sizeof(ProbDesc), D.stream);
     HIP_OK(hipEventRecord(D.ev0, D.stream));
     HIP_OK(hipMemcpyAsync(seq_in, power_seq, n * sizeof(double), hipMemcpyHostToDevice, D.stream));
     if (init_temp) {
@@ -1863,7 +1918,7 @@ void dev_transient_run(Model &M, const double *power_seq, int steps, double dt, 
     HIP_OK(hipEventElapsedTime(&ms, D.ev0, D.ev1));
     M.stats.last_solve_ms = ms;
     D.h.seq_in = D.h.seq_out = nullptr;
-    HIP_OK(hipMemcpy(D.d_desc, &D.h, sizeof(ProbDesc), hipMemcpyHostToDevice));
+    h2d(D.d_desc, &D.h, sizeof(ProbDesc), D.stream);
 }
 
 // ---- grid-level access ---------------------------------------------------------

@@ -11,8 +11,9 @@
 //
 // Per-problem scalars (CG coefficients, RK4 step sizes, flags) live in device
 // memory; reductions are two-level and deterministic: each workgroup writes
-// its partial into a fixed slot, and the last workgroup of the problem (found
-// with an atomic ticket) combines the slots in slot order.
+// its partial into a fixed slot with plain stores, and a finish kernel (one
+// workgroup per problem, launched after the sweep) combines the slots in
+// slot order.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -78,9 +79,10 @@ enum Scal {
     S_TAU1,                      // Taylor: factor of the slope in E_1 (tau phi_1(-sigma); tau unshifted)
     S_TSIG,                      // Taylor: shift sigma = rho tau / 2 of the series (0: unshifted)
     S_THOLD,                     // Taylor: interval held (the group launched fewer steps than it needs)
+    S_NHOLD,                     // Taylor: held intervals summed over a sequence
     NSCAL = 48
 };
-static_assert(S_THOLD < NSCAL, "scalar slots");
+static_assert(S_NHOLD < NSCAL, "scalar slots");
 enum RkStatus { RK_NEED_K1 = 0, RK_ATTEMPT = 1, RK_DONE = 2 };
 
 struct ProbDesc {
@@ -123,9 +125,6 @@ struct ProbDesc {
     // backward-Euler descriptors of heterogeneous models: sigma C per cell,
     // added to the ambient conductance of the stencil (nullptr otherwise)
     const double *bshift;
-    // paired Taylor sweeps: h1 on the perimeter of every layer,
-    // [nl][2 nc + 2 (nr - 2)] (top row, bottom row, left and right columns)
-    float *perim_h1;
     // block <-> grid maps (per layer)
     const int *map_ptr[kMaxLayers];     // cell -> entry range
     const int *map_unit[kMaxLayers];

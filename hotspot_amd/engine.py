@@ -25,7 +25,8 @@ class JobInfo(C.Structure):
     """hs_job_info_t"""
     _fields_ = [("state", C.c_int), ("steady_iters", C.c_int), ("steady_refinements", C.c_int),
                 ("rk4_calls", C.c_int), ("taylor_sweeps", C.c_double), ("wait_s", C.c_double), ("latency_s", C.c_double),
-                ("device_bytes", C.c_double), ("taylor_node_bytes", C.c_double)]
+                ("device_bytes", C.c_double), ("taylor_node_bytes", C.c_double), ("held_intervals", C.c_int),
+                ("horner_steps_min", C.c_int), ("horner_steps_plan", C.c_int)]
 
 
 def _bind(L):

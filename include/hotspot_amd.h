@@ -501,18 +501,22 @@ int hs_kernel_timer_read(double *avg_ms, double *total_ms);
 /* ---- batched engine: many models advanced by shared launches -------------
  * Jobs are admitted first come, first served while their device footprint
  * fits the budget (default: 80 % of the free device memory at creation).
- * Each tick runs a chunk of steady iterations for every job in its steady
- * phase and a chunk of transient intervals for every job in its transient
- * phase (one launch sequence per group, on two streams), then moves jobs
- * between phases.  A job's result is the same as that of the same solve run
- * alone.  The engine's options (hs_engine_set_options) apply to every job;
- * a model must stay allocated and unchanged until its jobs are admitted. */
+ * Resident jobs form groups by phase (steady, transient) and kernel form;
+ * each group advances in chunks of steady iterations or transient intervals
+ * (one launch sequence per group) on its own stream, and the engine takes a
+ * group's results when its chunk ends, independently of the other groups:
+ * a finished steady solve is reported without waiting for the transient
+ * groups, and an arriving job joins its group at the group's next chunk.
+ * A job's result is the same as that of the same solve run alone.  The
+ * engine's options (hs_engine_set_options) apply to every job; a model must
+ * stay allocated and unchanged until its jobs are admitted. */
 typedef struct hs_engine_st hs_engine_t;
 hs_engine_t *hs_engine_create(int precision, int transient);
 void hs_engine_destroy(hs_engine_t *e);
 /* only while no job is queued or running */
 void hs_engine_set_options(hs_engine_t *e, const hs_options_t *opt);
-/* transient intervals and steady iterations per tick (defaults 32, 128) */
+/* most transient intervals and the steady iterations of one chunk (defaults
+ * 32, 128); a transient chunk also stops at about 20 ms of device time */
 void hs_engine_set_tick(hs_engine_t *e, int intervals, int steady_iters);
 /* bytes of device memory the engine may use for resident problems */
 void hs_engine_set_budget(hs_engine_t *e, double bytes);
@@ -530,8 +534,10 @@ int hs_engine_submit(hs_engine_t *e, RC_model_t *model, const double *power, int
 /* Admits queued jobs while they fit (their inputs are then resident on the
  * device); returns the number still queued.  hs_engine_tick admits too. */
 int hs_engine_admit(hs_engine_t *e);
-/* Admits what fits and advances every resident job by one tick; returns the
- * number of jobs queued or running. */
+/* Admits what fits, starts a chunk for every group that is between chunks,
+ * waits up to 2 ms for some chunk to end and takes the results of every
+ * group whose chunk has ended; returns the number of jobs queued or running.
+ * Call it in a loop (submitting arrivals in between) until it returns 0. */
 int hs_engine_tick(hs_engine_t *e);
 /* 1 when finished (the final block vector is copied into out, may be NULL),
  * 0 while queued or running, -1 for an unknown job. */
@@ -550,6 +556,11 @@ typedef struct hs_job_info_st {
     double device_bytes;      /* device memory held while resident */
     double taylor_node_bytes; /* algorithmic bytes per node moved by the dominant Taylor sweep
                                * kernel over the whole sequence (throughput accounting) */
+    int held_intervals;       /* Taylor: intervals held and run again because its group launched
+                               * fewer Horner steps than the interval needed */
+    int horner_steps_min;     /* Taylor: fewest Horner steps per sub-step its group launched while
+                               * it ran, when below its plan (0: the plan's every time) */
+    int horner_steps_plan;    /* Taylor: Horner steps per sub-step of its a-priori plan */
 } hs_job_info_t;
 int hs_engine_job_info(hs_engine_t *e, int job, hs_job_info_t *info);
 /* frees a finished job's results */

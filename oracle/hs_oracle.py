@@ -840,8 +840,11 @@ class OracleModel:
         return Pgrid + self.b - self.G @ T
 
     def steady_grid(self, Pgrid):
-        """Converged steady state of the reference operator."""
-        if self.N <= 400_000:
+        """Converged steady state of the reference operator: a sparse direct
+        solve, or above 150 000 nodes of a symmetric operator (400 000 of a
+        non-symmetric one) Jacobi-preconditioned CG (BiCGStab) to a 1e-14
+        relative residual, where the direct solve's fill-in takes minutes."""
+        if self.N <= (150_000 if self.symmetric else 400_000):
             return spla.spsolve(self.G.tocsc(), Pgrid + self.b)
         return self._cg(Pgrid + self.b)
 

@@ -7,8 +7,10 @@ converged reference operator; mixed-precision path within 1e-2 K.
 import tempfile
 
 import ctypes as C
+import math
 
 import numpy as np
+import scipy.sparse as sp
 import pytest
 
 import hs_oracle as O
@@ -231,7 +233,13 @@ def test_transient_taylor_long_intervals_match_exact(lib, which, prec, tol, dt):
     tr = cases.run_transient(run, T0, seq, dt)
     check_error(lib)
     st = stats(lib, run)
-    assert st.taylor_substeps >= 1
+    # sub-steps keep rho tau <= 4, rho the Gershgorin bound on ||C^-1 G||
+    A = abs(sp.diags(1.0 / om.Cvec) @ om.Gt)
+    rho = float(np.asarray(A.sum(axis=1)).max())
+    want = math.ceil(rho * dt / 4.0)
+    assert abs(st.taylor_substeps - want) <= 1, (st.taylor_substeps, want)
+    if want > 2:
+        assert st.taylor_substeps > 1
     Tg = om.b2g(T0, power=False)
     worst = 0.0
     for k, p in enumerate(seq):
