@@ -35,6 +35,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The engine runs its set-up, steady and transient work on seven streams;
+# with a hardware queue each, no stream's work waits behind another's
+# launches in a shared queue (HIP's default is four queues per process).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 from hotspot_amd import cases, engine, iface  # noqa: E402
 
@@ -286,7 +290,7 @@ def mode_serve(lib, args):
         if t > args.serve_seconds:
             break
         m = int(g.integers(len(pool)))
-        steps = 0 if g.random() < 0.6 else max(1, int(round(int(g.integers(100, 10001)) * args.serve_steps_scale)))
+        steps = 0 if g.random() < args.serve_steady_frac else max(1, int(round(int(g.integers(100, 10001)) * args.serve_steps_scale)))
         reqs.append((t, m, steps, pool[m][1] * g.uniform(0.8, 1.2, size=pool[m][1].shape)))
     updates = sum(pool[m][2] * (steps if steps else 1) for _, m, steps, _ in reqs)
     log(f"{len(reqs)} requests, {sum(1 for r in reqs if r[2])} transient, {updates:.3e} cell-updates")
@@ -309,12 +313,16 @@ def mode_serve(lib, args):
             if active == 0 and k < len(reqs):
                 time.sleep(max(0.0, reqs[k][0] - (time.perf_counter() - t0)))
         wall = time.perf_counter() - t0
-        lat = np.array([eng.info(i).latency_s for i in ids])
-        steady_lat = np.array([eng.info(i).latency_s for i, r in zip(ids, reqs) if not r[2]])
+        infos = [eng.info(i) for i in ids]
+        lat = np.array([x.latency_s for x in infos])
+        steady_lat = np.array([x.latency_s for x, r in zip(infos, reqs) if not r[2]])
         out[name] = dict(value=updates / wall, wall_s=wall, p50_ms=float(np.percentile(lat, 50)) * 1e3,
                          p99_ms=float(np.percentile(lat, 99)) * 1e3,
                          steady_p50_ms=float(np.percentile(steady_lat, 50)) * 1e3 if len(steady_lat) else None,
-                         steady_p99_ms=float(np.percentile(steady_lat, 99)) * 1e3 if len(steady_lat) else None)
+                         steady_p99_ms=float(np.percentile(steady_lat, 99)) * 1e3 if len(steady_lat) else None,
+                         wait_p50_ms=float(np.percentile([x.wait_s for x in infos], 50)) * 1e3,
+                         setup_mean_ms=float(np.mean([x.setup_s for x in infos])) * 1e3,
+                         resident_frac=float(np.mean([x.resident_model for x in infos])))
         for i in ids:
             eng.release(i)
         eng.close()
@@ -325,7 +333,8 @@ def mode_serve(lib, args):
             "value": out["mixed"]["value"], "unit": "cell-updates/s", "n_gpus": 1,
             "config": {"workload": f"Poisson {args.serve_rate}/s for {args.serve_seconds} s, pool of "
                                    f"{args.serve_pool} models 64^2-512^2 x 4-8 layers (10 % microchannel), "
-                                   f"60 % steady / 40 % transient of U{{100..10000}} x {args.serve_steps_scale} intervals",
+                                   f"{100 * args.serve_steady_frac:.0f} % steady / {100 - 100 * args.serve_steady_frac:.0f} % "
+                                   f"transient of U{{100..10000}} x {args.serve_steps_scale} intervals",
                        "requests": len(reqs), "cell_updates": updates},
             "forms": out,
             "speedup_vs_fp64": (out["mixed"]["value"] / out["fp64"]["value"]) if "fp64" in out else None}
@@ -388,6 +397,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-pool", type=int, default=20)
     ap.add_argument("--serve-steps-scale", type=float, default=0.1)
+    ap.add_argument("--serve-steady-frac", type=float, default=0.6, help="share of steady requests")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)

@@ -49,18 +49,39 @@ struct EngJob {
     int holds = 0;                      // Taylor: intervals held and run again
     int kplan = 0;                      // Taylor: Horner steps per sub-step of its plan
     int kmin = 0;                       // Taylor: fewest steps its group launched per sub-step (0: the plan's)
+    unsigned long stamp = 0;            // the model's coefficients when its device state was built
+    bool use_mg = false;                // steady phase: multigrid-preconditioned solve
+    double setup_s = 0.0;               // host time of the admission (device state, inputs)
+    bool resident = false;              // admitted on its model's resident device state
 };
 
-// Transient parts of an engine: each transient form (homogeneous,
-// heterogeneous) is split into this many groups, one per stream.  While one
+// Kernel forms of the transient groups: homogeneous problems whose FP32
+// Taylor sweeps take float4 tiles, other homogeneous problems, heterogeneous
+// problems.  A group's launches take one form for all its members, and the
+// forms round differently, so a problem always goes to the group of its own
+// form: its result does not depend on what else is resident.
+constexpr int kTransForms = 3;
+static int trans_form(const EngJob &j);
+// Transient parts of an engine: each transient form is split into this many
+// groups, one per stream.  While one
 // part runs the latency-bound kernels of its interval (power map, first
 // term, finishes), the other's Horner sweeps use the memory bandwidth.
 constexpr int kTransParts = 2;
 // device time a transient chunk aims at: long enough to amortise the host's
 // look at the flags, short enough that an arriving job waits little to join
 constexpr double kSliceMs = 20.0;
+// ... and while steady solves are resident (they go first)
+constexpr double kSliceYieldMs = 2.0;
 // longest a tick waits for some group's chunk to end
 constexpr double kWaitMs = 2.0;
+// streams of the steady groups: the kernels of a steady round are small and
+// latency-bound, so groups of different shapes overlap on the device
+constexpr int kSteadyStreams = 4;
+// inner iterations of a multigrid-preconditioned CG round: an FP32 inner
+// solve gains its four digits in about 10 of them, and a refinement step
+// takes place at the start of a round, so a solve takes one round per
+// refinement (rounds of 3 at the default 128 steady iterations per chunk)
+constexpr int kMgRoundIters = 12;
 
 // A group of resident jobs in one phase with one kernel form (homogeneous or
 // heterogeneous coefficients): descriptor array and work-lists on the device.
@@ -118,6 +139,16 @@ struct BatchGroup {
     }
 };
 
+// The device state of a model whose job has ended, kept for the model's next
+// job (the RC model stays resident: coefficients, maps, multigrid levels and
+// GMRES buffers are not built again), while its coefficients are unchanged.
+struct Resident {
+    const Model *M = nullptr;
+    unsigned long stamp = 0;
+    std::unique_ptr<DevModel> D;
+    double last_use = 0.0;
+};
+
 struct Engine {
     DevPool dpool, hpool;               // declared first: destroyed after the groups that use them
     hs_options_t opt{};
@@ -126,23 +157,33 @@ struct Engine {
     int tick_intervals = 32;            // most transient intervals per chunk
     int tick_iters = 128;               // steady iterations per chunk
     // s_setup: admission, phase changes and results (never waits on a chunk)
-    hipStream_t s_setup = nullptr, s_steady = nullptr, s_trans[kTransParts] = {};
+    hipStream_t s_setup = nullptr, s_steady[kSteadyStreams] = {}, s_trans[kTransParts] = {};
     std::deque<EngJob *> queue;
     std::map<int, std::unique_ptr<EngJob>> jobs;
+    std::vector<Resident> resident;     // idle device models of ended jobs
+    size_t resident_bytes = 0;
+    hs_engine_stats_t stats{};
     // steady groups by key: kind (0 homogeneous CG, 1 heterogeneous CG,
     // 2 non-symmetric GMRES) + 4 x multigrid levels (0: Jacobi);
-    // transient: homogeneous, heterogeneous, each in parts on the transient
-    // streams, [hetero + 2 * part]
+    // transient: by kernel form (trans_form), each in parts on the transient
+    // streams, [form + kTransForms * part]
     std::map<int, BatchGroup> steady;
-    BatchGroup trans[2 * kTransParts];
+    BatchGroup trans[kTransForms * kTransParts];
     ~Engine() {
-        for (hipStream_t s : {s_setup, s_steady, s_trans[0], s_trans[1]})
+        for (hipStream_t s : streams())
             if (s) (void)hipStreamSynchronize(s);
         for (auto &kv : jobs) kv.second->D.reset();
+        resident.clear();
         for (auto &kv : steady) kv.second.release();
         for (auto &g : trans) g.release();
-        for (hipStream_t s : {s_setup, s_steady, s_trans[0], s_trans[1]})
+        for (hipStream_t s : streams())
             if (s) (void)hipStreamDestroy(s);
+    }
+    std::vector<hipStream_t> streams() const {
+        std::vector<hipStream_t> v{s_setup};
+        v.insert(v.end(), s_steady, s_steady + kSteadyStreams);
+        v.insert(v.end(), s_trans, s_trans + kTransParts);
+        return v;
     }
 };
 
@@ -186,6 +227,11 @@ template <typename T> static void ensure_dev(DevPool *pool, T *&p, size_t &cap, 
 // chunk, and this group is between chunks).
 static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans) {
     if (!B.dirty) return;
+    struct Timed {
+        double &acc;
+        double t0 = now_s();
+        ~Timed() { acc += now_s() - t0; }
+    } timed{E.stats.sync_s};
     hipStream_t s = E.s_setup;
     const int n = (int)B.members.size();
     std::vector<ProbDesc> desc(n);
@@ -253,7 +299,7 @@ static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans) {
     for (void *p : B.lv_bufs) dfree(B.pool, p);
     B.lv_bufs.clear();
     B.lv.clear();
-    const int L = (!trans && n) ? (int)B.members[0]->D->mg.size() : 0;
+    const int L = (!trans && n && B.members[0]->use_mg) ? (int)B.members[0]->D->mg.size() : 0;
     for (int l = 0; l < L; l++) {
         std::vector<ProbDesc> ld(n);
         std::vector<WorkItem> lw;
@@ -289,15 +335,88 @@ static void gather(BatchGroup &B, std::initializer_list<int> slots) {
                           B.g.stream));
 }
 
+// A job's device model goes back to the resident set (its sequence buffers
+// to the pool) when its model is unchanged, else to the pool.
 static void release_device(Engine &E, EngJob &j) {
     E.used -= j.held;
     j.held = 0;
-    j.D.reset();
+    DevModel &D = *j.D;
+    D.release(D.h.seq_in);
+    D.release(D.h.seq_out);
+    D.h.seq_in = D.h.seq_out = nullptr;
+    D.h.seq_one = 0;
+    if (j.M->stamp != j.stamp) {
+        j.D.reset();
+        return;
+    }
+    E.resident_bytes += D.bytes;
+    E.resident.push_back(Resident{j.M, j.stamp, std::move(j.D), now_s()});
+}
+
+// Zeroes the state of a resident device model for its next job, as a fresh
+// build leaves it; the coefficients, maps and multigrid levels stay.
+static void dev_reset_state(DevModel &D) {
+    hipStream_t s = D.stream;
+    ProbDesc &h = D.h;
+    for (int b = 0; b < NB64; b++) dzero(h.b64[b], D.vlen * sizeof(double), s);
+    for (int b = 0; b < NB32; b++) dzero(h.b32[b], D.vlen * sizeof(float), s);
+    dzero(h.lzero, kMaxLayers * sizeof(int), s);
+    dzero(h.partial, (size_t)slot_count(h) * kRed * sizeof(double), s);
+    dzero(h.counter, sizeof(unsigned), s);
+    dzero(h.scal, NSCAL * sizeof(double), s);
+    dzero(D.grp.d_kmax, 2 * sizeof(int), s);
+    dzero(D.grp.d_trec, kTrec * sizeof(double), s);
+    dzero(h.blk_in, D.nblk * sizeof(double), s);
+    dzero(h.blk_out, D.nblk * sizeof(double), s);
+    for (size_t l = 1; l < D.mg.size(); l++) {
+        const ProbDesc &c = D.mg[l].h;
+        for (int b : {(int)F_R, (int)F_D0, (int)F_D1}) dzero(c.b32[b], c.vlen * sizeof(float), s);
+        dzero(c.partial, (size_t)slot_count(c) * kRed * sizeof(double), s);
+    }
+    if (h.gm) {
+        dzero(h.gm, GM_SIZE * sizeof(double), s);
+        dzero(h.gpart, (size_t)(h.ntiles + 1) * (kGmM + 1) * sizeof(double), s);
+    }
+    if (h.gmv32) dzero(h.gmv32, (size_t)(kGmM + 1) * D.vlen * sizeof(float), s);
+    if (h.gmv64) dzero(h.gmv64, (size_t)(kGmM + 1) * D.vlen * sizeof(double), s);
+    h2d(D.d_desc, &h, sizeof(ProbDesc), s);
+}
+
+// The resident device model of M's current coefficients, if any; resident
+// models of a changed (or deleted and replaced) model are dropped.
+static DevModel *take_resident(Engine &E, const Model &M) {
+    DevModel *found = nullptr;
+    for (size_t k = 0; k < E.resident.size();) {
+        Resident &r = E.resident[k];
+        if (r.M != &M || (r.stamp == M.stamp && found)) {
+            k++;
+            continue;
+        }
+        // M's: the first of its current coefficients is taken, stale ones dropped
+        E.resident_bytes -= r.D->bytes;
+        if (r.stamp == M.stamp) found = r.D.release();
+        E.resident.erase(E.resident.begin() + k);
+    }
+    return found;
+}
+
+// Drops the least recently used resident models until `need` more bytes fit
+// the budget beside the resident jobs (their blocks go to the pool).
+static void evict_resident(Engine &E, size_t need) {
+    while (!E.resident.empty() && E.used + E.resident_bytes + need > E.budget) {
+        size_t k = 0;
+        for (size_t i = 1; i < E.resident.size(); i++)
+            if (E.resident[i].last_use < E.resident[k].last_use) k = i;
+        E.resident_bytes -= E.resident[k].D->bytes;
+        E.resident.erase(E.resident.begin() + k);
+    }
 }
 
 static int steady_key(const EngJob &j) {
-    return (j.adv ? 2 : j.hetero ? 1 : 0) + 4 * (int)j.D->mg.size();
+    return (j.adv ? 2 : j.hetero ? 1 : 0) + 4 * (j.use_mg ? (int)j.D->mg.size() : 0);
 }
+
+static int trans_form(const EngJob &j) { return j.hetero ? 2 : vec4_ok(j.D->h) ? 0 : 1; }
 
 static void start_transient(Engine &E, EngJob &j) {
     DevModel &D = *j.D;
@@ -315,9 +434,11 @@ static void start_transient(Engine &E, EngJob &j) {
     // the part with the fewest members; one part only while a kernel timer
     // runs (launches of the parts would overlap in its event spans)
     const int parts = g_timer.cap > 0 ? 1 : kTransParts;
-    int best = j.hetero;
+    const int form = trans_form(j);
+    int best = form;
     for (int q = 1; q < parts; q++)
-        if (E.trans[j.hetero + 2 * q].members.size() < E.trans[best].members.size()) best = j.hetero + 2 * q;
+        if (E.trans[form + kTransForms * q].members.size() < E.trans[best].members.size())
+            best = form + kTransForms * q;
     BatchGroup &B = E.trans[best];
     B.members.push_back(&j);
     B.dirty = true;
@@ -325,22 +446,25 @@ static void start_transient(Engine &E, EngJob &j) {
 
 static void admit(Engine &E, EngJob &j) {
     Model &M = *j.M;
-    j.D.reset(dev_build(M, E.s_setup, &E.dpool, &E.hpool));
-    DevModel &D = *j.D;
     j.t_admit = now_s();
+    j.stamp = M.stamp;
+    if (DevModel *R = take_resident(E, M)) {
+        j.D.reset(R);
+        j.resident = true;
+        dev_reset_state(*R);
+    } else {
+        j.D.reset(dev_build(M, E.s_setup, &E.dpool, &E.hpool));
+    }
+    DevModel &D = *j.D;
     const size_t nb = (size_t)D.nblk;
     if (j.nsteps > 0) {
-        D.h.seq_in = D.alloc<double>((size_t)j.nsteps * nb);
-        D.h.seq_out = D.alloc<double>((size_t)j.nsteps * nb);
         // interval powers: one row for all, one per interval, or the steady row first
         const int first = j.rows == j.nsteps + 1 ? 1 : 0;
-        if (j.rows == 1) {
-            std::vector<double> rep((size_t)j.nsteps * nb);
-            for (int k = 0; k < j.nsteps; k++) std::copy(j.power.begin(), j.power.end(), rep.begin() + k * nb);
-            h2d(D.h.seq_in, rep.data(), rep.size() * sizeof(double), D.stream);
-        } else {
-            h2d(D.h.seq_in, j.power.data() + first * nb, (size_t)j.nsteps * nb * sizeof(double), D.stream);
-        }
+        const size_t in_rows = j.rows == 1 ? 1 : (size_t)j.nsteps;
+        D.h.seq_in = D.alloc<double>(in_rows * nb);
+        D.h.seq_out = D.alloc<double>((size_t)j.nsteps * nb);
+        D.h.seq_one = j.rows == 1 ? 1 : 0;
+        h2d(D.h.seq_in, j.power.data() + first * nb, in_rows * nb * sizeof(double), D.stream);
         h2d(D.d_desc, &D.h, sizeof(ProbDesc), D.stream);
     }
     j.held = D.bytes;
@@ -350,13 +474,14 @@ static void admit(Engine &E, EngJob &j) {
         run_b2g(D.grp, V_POWER, B_RHS, 1);
         const int nbk = (int)((D.vlen + kBlock - 1) / kBlock);
         launch(k_fill, nbk, D.stream, D.h.b64[B_STEADY], D.vlen, M.g->config.ambient);
+        j.use_mg = j.adv || E.opt.steady_solver == HS_STEADY_MGPCG;
         if (j.adv) {
             // non-symmetric: GMRES with the upwind multigrid preconditioner
             // in every form (Jacobi does not converge at channel Peclet numbers)
             ensure_gmres(D, true);
             build_mg(D);
             write_scal(D, kMixedStart);
-        } else if (E.opt.steady_solver == HS_STEADY_MGPCG) {
+        } else if (j.use_mg) {
             build_mg(D);
             write_scal(D, kMixedStart);
         } else if (E.opt.precision == HS_PREC_MIXED) {
@@ -381,6 +506,7 @@ static void admit(Engine &E, EngJob &j) {
     // device memory held, multigrid levels and GMRES state included
     E.used += D.bytes - j.held;
     j.held = D.bytes;
+    j.setup_s = now_s() - j.t_admit;
 }
 
 // A finished steady solve: block temperatures, then either the result or
@@ -419,25 +545,32 @@ static void trans_done(Engine &E, EngJob &j, const double *flags) {
     release_device(E, j);
 }
 
+// One round of a steady group: the unit that a chunk replays (captured once
+// per membership as a graph).
 template <bool HET>
-static void steady_chunk(Engine &E, const Group &G, bool adv, const std::vector<MgGroup> &lv) {
+static void steady_round(Engine &E, const Group &G, bool adv, const std::vector<MgGroup> &lv) {
     const double rtol2 = E.opt.steady_rtol * E.opt.steady_rtol;
     if (!lv.empty() && adv) {   // multigrid-preconditioned GMRES cycles
-        for (int done = 0; done < E.tick_iters; done += 64) gm_round_mg<true>(G, lv, rtol2, 1e-8, kGmM);
+        gm_round_mg<true>(G, lv, rtol2, 1e-8, kGmM);
     } else if (!lv.empty()) {   // multigrid-preconditioned CG rounds
-        for (int done = 0; done < E.tick_iters; done += 64) mixed_round_mg<HET>(G, lv, rtol2, E.opt.max_iters, 16);
+        mixed_round_mg<HET>(G, lv, rtol2, E.opt.max_iters, kMgRoundIters);
     } else if (adv) {   // GMRES cycles inside FP64 refinement
-        const int cycles = std::max(1, E.tick_iters / kGmM);
-        for (int c = 0; c < cycles; c++) {
-            if (E.opt.precision == HS_PREC_MIXED) gm_round<float, true>(G, rtol2, 1e-8, kGmM);
-            else gm_round<double, true>(G, rtol2, 1e-12, kGmM);
-        }
+        if (E.opt.precision == HS_PREC_MIXED) gm_round<float, true>(G, rtol2, 1e-8, kGmM);
+        else gm_round<double, true>(G, rtol2, 1e-12, kGmM);
     } else if (E.opt.precision == HS_PREC_MIXED) {
-        for (int done = 0; done < E.tick_iters; done += 64) mixed_round<HET>(G, rtol2, E.opt.max_iters, 64);
+        mixed_round<HET>(G, rtol2, E.opt.max_iters, 64);
     } else {
-        cg_iterations<double, HET>(G, E.tick_iters, B_STEADY, B_R, B_Z, B_P0, B_Q, S_DONE, S_TOL2,
-                                   E.opt.max_iters);
+        cg_iterations<double, HET>(G, 64, B_STEADY, B_R, B_Z, B_P0, B_Q, S_DONE, S_TOL2, E.opt.max_iters);
     }
+}
+
+// rounds of a chunk: one per 64 of the steady iterations per chunk, one per
+// kGmM for GMRES cycles without multigrid, and for multigrid-preconditioned
+// CG one per kMgRoundIters + 28 (a V-cycle costs several plain iterations)
+static int steady_rounds(const Engine &E, bool adv, bool mg) {
+    if (adv && !mg) return std::max(1, E.tick_iters / kGmM);
+    if (mg && !adv) return std::max(1, E.tick_iters / (kMgRoundIters + 28));
+    return std::max(1, E.tick_iters / 64);
 }
 
 static void group_events(BatchGroup &B) {
@@ -453,32 +586,48 @@ static void launch_chunks(Engine &E) {
         const int kind = kv.first % 4;
         B.pool = &E.dpool;
         B.hpool = &E.hpool;
-        B.stream = E.s_steady;
+        // groups of different multigrid depths (grid sizes) on different streams
+        B.stream = E.s_steady[(kv.first / 4 + kind) % kSteadyStreams];
         group_sync(E, B, kind > 0, false);
         group_events(B);
         HIP_OK(hipEventRecord(B.ev_a, B.stream));
-        if (kind) steady_chunk<true>(E, B.g, kind == 2, B.lv);
-        else steady_chunk<false>(E, B.g, false, B.lv);
+        const int rounds = steady_rounds(E, kind == 2, !B.lv.empty());
+        B.rp[0].run(B.stream, rounds, [&] {
+            if (kind) steady_round<true>(E, B.g, kind == 2, B.lv);
+            else steady_round<false>(E, B.g, false, B.lv);
+        });
         const bool refined = E.opt.precision == HS_PREC_MIXED || kind == 2 || !B.lv.empty();
         gather(B, {S_DONE, refined ? S_TITERS : S_ITERS, S_NREFINE});
         HIP_OK(hipEventRecord(B.ev_b, B.stream));
+        E.stats.steady_chunks++;
         B.ran = B.g.nprob;
         B.busy = true;
     }
-    for (int h = 0; h < 2 * kTransParts; h++) {
+    // Steady solves go first: a steady round is a long chain of small,
+    // latency-bound kernels, each of which would wait for compute units
+    // behind the transient sweeps.  While a steady group has members, a
+    // transient chunk starts only when no steady chunk runs, and lasts at
+    // most kSliceYieldMs, so that steady work arriving meanwhile waits little.
+    bool steady_resident = false, steady_busy = false;
+    for (auto &kv : E.steady) {
+        steady_resident |= !kv.second.members.empty();
+        steady_busy |= kv.second.busy;
+    }
+    for (int h = 0; h < kTransForms * kTransParts; h++) {
         BatchGroup &B = E.trans[h];
-        if (B.busy || B.members.empty()) continue;
+        if (B.busy || B.members.empty() || steady_busy) continue;
         B.pool = &E.dpool;
         B.hpool = &E.hpool;
-        B.stream = E.s_trans[h / 2];
-        group_sync(E, B, h % 2, true);
+        B.stream = E.s_trans[h / kTransForms];
+        group_sync(E, B, h % kTransForms == 2, true);
         group_events(B);
         // the intervals left to the member furthest from its end, within the
         // tick's intervals and the device-time slice
         int rem = 1;
         for (const EngJob *j : B.members) rem = std::max(rem, j->nsteps - j->step_seen);
         int n = std::min(E.tick_intervals, rem);
-        if (B.ms_per_interval > 0) n = std::min(n, std::max(1, (int)(kSliceMs / B.ms_per_interval)));
+        const double slice = steady_resident ? kSliceYieldMs : kSliceMs;
+        if (B.ms_per_interval > 0) n = std::min(n, std::max(1, (int)(slice / B.ms_per_interval)));
         B.chunk = n;
         const int kl = B.g.kgraph > 0 ? std::min(B.g.kgraph, B.form.K) : B.form.K;
         for (EngJob *j : B.members)
@@ -492,6 +641,7 @@ static void launch_chunks(Engine &E) {
             HIP_OK(hipMemsetAsync(B.g.d_kmax + 1, 0, sizeof(int), B.stream));
         }
         HIP_OK(hipEventRecord(B.ev_b, B.stream));
+        E.stats.transient_chunks++;
         B.ran = B.g.nprob;
         B.busy = true;
     }
@@ -530,6 +680,9 @@ static void harvest(Engine &E) {
         BatchGroup &B = kv.second;
         if (!chunk_done(B)) continue;
         B.busy = false;
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, B.ev_a, B.ev_b));
+        E.stats.steady_chunk_s += 1e-3 * ms;
         std::vector<std::pair<EngJob *, std::vector<double>>> fin;
         for (int i = 0; i < B.ran; i++)
             if (B.h_flags[i * 8] != 0.0)
@@ -539,13 +692,14 @@ static void harvest(Engine &E) {
             steady_done(E, *f.first, f.second.data());
         }
     }
-    for (int h = 0; h < 2 * kTransParts; h++) {
+    for (int h = 0; h < kTransForms * kTransParts; h++) {
         BatchGroup &B = E.trans[h];
         if (!chunk_done(B)) continue;
         B.busy = false;
         float ms = 0.f;
         HIP_OK(hipEventElapsedTime(&ms, B.ev_a, B.ev_b));
         B.ms_per_interval = (double)ms / std::max(1, B.chunk);
+        E.stats.transient_chunk_s += 1e-3 * ms;
         // Taylor groups of one sub-step per interval launch the Horner steps
         // that the last chunk's largest term count needs instead of every
         // step of the plan, so no launch skips every problem.  A problem that
@@ -586,7 +740,7 @@ Engine *eng_create(int precision, int transient) {
     HIP_OK(hipMemGetInfo(&fr, &tot));
     E->budget = (size_t)(0.8 * (double)fr);
     HIP_OK(hipStreamCreateWithFlags(&E->s_setup, hipStreamNonBlocking));
-    HIP_OK(hipStreamCreateWithFlags(&E->s_steady, hipStreamNonBlocking));
+    for (auto &s : E->s_steady) HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     for (auto &s : E->s_trans) HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     return E.release();
 }
@@ -658,21 +812,35 @@ int eng_admit(Engine &E) {
     while (!E.queue.empty()) {
         EngJob &j = *E.queue.front();
         if (E.used + j.footprint > E.budget && E.used > 0) break;
-        // the pool's idle blocks count against the budget until returned
-        if (E.used + E.dpool.idle_bytes + j.footprint > E.budget) E.dpool.trim();
+        // resident models and the pool's idle blocks count against the
+        // budget until dropped or returned to the runtime
+        evict_resident(E, j.footprint);
+        if (E.used + E.resident_bytes + E.dpool.idle_bytes + j.footprint > E.budget) E.dpool.trim();
         E.queue.pop_front();
+        const double t0 = now_s();
         admit(E, j);
+        E.stats.admit_s += now_s() - t0;
+        E.stats.resident_hits += j.resident ? 1 : 0;
     }
     return (int)E.queue.size();
 }
 
 int eng_tick(Engine &E) {
     eng_admit(E);
+    double t0 = now_s();
+    const double s0 = E.stats.sync_s;
     launch_chunks(E);
+    double t1 = now_s();
+    E.stats.launch_s += (t1 - t0) - (E.stats.sync_s - s0);
     wait_chunk(E);
+    t0 = now_s();
+    E.stats.wait_s += t0 - t1;
     harvest(E);
+    E.stats.harvest_s += now_s() - t0;
     return eng_active(E);
 }
+
+void eng_get_stats(Engine &E, hs_engine_stats_t *st) { *st = E.stats; }
 
 static EngJob &job_of(Engine &E, int id) {
     auto it = E.jobs.find(id);
@@ -709,6 +877,8 @@ void eng_job_info(Engine &E, int id, hs_job_info_t *info) {
     info->held_intervals = j.holds;
     info->horner_steps_min = j.kmin;
     info->horner_steps_plan = j.kplan;
+    info->setup_s = j.setup_s;
+    info->resident_model = j.resident ? 1 : 0;
 }
 
 void eng_release(Engine &E, int id) {

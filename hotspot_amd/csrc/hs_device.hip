@@ -439,7 +439,7 @@ __global__ void __launch_bounds__(kBlock) k_b2g(const ProbDesc *__restrict__ D,
         if (power32) st_at(out32, c, (float)v);
         else st_at(out, c, v);
     };
-    const double *b = from_seq == 1 ? d.seq_in + (long)d.scal[S_STEP] * (d.nblocks + d.nx)
+    const double *b = from_seq == 1 ? d.seq_in + (d.seq_one ? 0L : (long)d.scal[S_STEP] * (d.nblocks + d.nx))
                       : from_seq == 2 ? d.blk_out : d.blk_in;
     const long ncell = (long)d.nl * d.nr * d.nc;
     if (wi.tile < d.ntiles) {

@@ -174,12 +174,14 @@ struct DevModel {
 
     std::vector<size_t> alloc_sz;       // bytes of each allocation in allocs
 
+    // zeroed, whether fresh or a pool block of an earlier job
     template <typename T> T *alloc(size_t n) {
         size_t got = 0;
         void *p = dmalloc(pool, std::max<size_t>(n, 1) * sizeof(T), &got);
         allocs.push_back(p);
         alloc_sz.push_back(got);
         bytes += got;
+        dzero(p, got, stream);
         return (T *)p;
     }
     template <typename T> T *upload(const T *src, size_t n) {

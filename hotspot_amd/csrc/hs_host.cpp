@@ -817,6 +817,14 @@ static void refresh_mirror(Model &M, int which) {
     }
 }
 
+// stamps of model coefficients: unique over the process, so a device state
+// kept for a model is recognised as stale after any change (or when another
+// model takes the same address)
+static unsigned long next_stamp() {
+    static unsigned long s = 0;
+    return ++s;
+}
+
 static void delete_model(Model *M) {
     if (!M) return;
     dev_release(*M);
@@ -1068,6 +1076,7 @@ RC_model_t *alloc_RC_model(thermal_config_t *config, flp_t *placeholder,
         fatal("grid rows and columns should both be powers of two\n");
     h = (Handle *)calloc(1, sizeof(Handle));
     M = new Model();
+    M->stamp = next_stamp();
     h->impl = M;
     M->g = &h->pub;
     grid_model_t &g = h->pub;
@@ -1168,6 +1177,7 @@ void populate_R_model(RC_model_t *model, flp_t *flp) {
     }
     g.r_ready = 1;
     M.dev_dirty = true;
+    M.stamp = next_stamp();
     HS_END()
 }
 
@@ -1191,6 +1201,7 @@ void populate_C_model(RC_model_t *model, flp_t *flp) {
         g.layers[M.pcbidx].c += kCFactor * g.config.c_convec_sec * (cw * ch) / (g.config.s_pcb * g.config.s_pcb);
     g.c_ready = 1;
     M.dev_dirty = true;
+    M.stamp = next_stamp();
     HS_END()
 }
 
@@ -1256,6 +1267,7 @@ void resize_thermal_model(RC_model_t *model, int n_units) {
     M.g->total_n_blocks = 0;
     for (int i = 0; i < M.nl(); i++) M.g->total_n_blocks += M.layers[i].flp->n_units;
     M.dev_dirty = true;
+    M.stamp = next_stamp();
     HS_END()
 }
 
@@ -1834,6 +1846,13 @@ int hs_engine_active(hs_engine_t *e) {
     HS_BEGIN
     return eng_active(engine_of(e));
     HS_END(-1)
+}
+
+void hs_engine_get_stats(hs_engine_t *e, hs_engine_stats_t *stats) {
+    HS_BEGIN
+    if (!stats) fatal("null engine stats\n");
+    eng_get_stats(engine_of(e), stats);
+    HS_END()
 }
 
 }  // extern "C"

@@ -107,6 +107,8 @@ struct Model {
     hs_stats_t stats{};
     DevModel *dev = nullptr;
     bool dev_dirty = true;            // host coefficients changed since upload
+    unsigned long stamp = 0;          // process-unique, renewed whenever the coefficients change
+                                      // (an engine keeps a model's device state while it matches)
     int steady_valid = 0;             // last_steady holds a solution
     int trans_valid = 0;              // last_trans holds a state
     int mirror_steady = 0, mirror_trans = 0;  // host mirrors up to date
@@ -192,5 +194,6 @@ int eng_result_seq(Engine &E, int id, double *out);
 void eng_job_info(Engine &E, int id, hs_job_info_t *info);
 void eng_release(Engine &E, int id);
 int eng_active(Engine &E);
+void eng_get_stats(Engine &E, hs_engine_stats_t *st);
 
 }  // namespace hs

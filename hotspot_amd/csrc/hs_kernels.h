@@ -149,7 +149,8 @@ struct ProbDesc {
     double *gpart;                      // [ntiles+1][kGmM+1] partials
     float *gmv32;                       // [kGmM+1][vlen] basis, FP32 inner solves
     double *gmv64;                      // [kGmM+1][vlen] basis, FP64 inner solves
-    unsigned vlen, pad2;
+    unsigned vlen;
+    int seq_one;                        // 1: every interval of the sequence takes seq_in's one row
     // reduction state
     double *partial;                    // [ntiles+1][kRed]
     unsigned *counter;

@@ -26,7 +26,16 @@ class JobInfo(C.Structure):
     _fields_ = [("state", C.c_int), ("steady_iters", C.c_int), ("steady_refinements", C.c_int),
                 ("rk4_calls", C.c_int), ("taylor_sweeps", C.c_double), ("wait_s", C.c_double), ("latency_s", C.c_double),
                 ("device_bytes", C.c_double), ("taylor_node_bytes", C.c_double), ("held_intervals", C.c_int),
-                ("horner_steps_min", C.c_int), ("horner_steps_plan", C.c_int)]
+                ("horner_steps_min", C.c_int), ("horner_steps_plan", C.c_int), ("setup_s", C.c_double),
+                ("resident_model", C.c_int)]
+
+
+class EngineStats(C.Structure):
+    """hs_engine_stats_t"""
+    _fields_ = [("admit_s", C.c_double), ("sync_s", C.c_double), ("launch_s", C.c_double),
+                ("wait_s", C.c_double), ("harvest_s", C.c_double), ("steady_chunks", C.c_long),
+                ("transient_chunks", C.c_long), ("resident_hits", C.c_long),
+                ("steady_chunk_s", C.c_double), ("transient_chunk_s", C.c_double)]
 
 
 def _bind(L):
@@ -57,6 +66,8 @@ def _bind(L):
     L.hs_engine_release.restype = None
     L.hs_engine_active.argtypes = [C.c_void_p]
     L.hs_engine_active.restype = C.c_int
+    L.hs_engine_get_stats.argtypes = [C.c_void_p, C.POINTER(EngineStats)]
+    L.hs_engine_get_stats.restype = None
     L.hs_last_error.argtypes = [C.c_char_p, C.c_int]
     L.hs_last_error.restype = C.c_int
 
@@ -146,6 +157,12 @@ class Engine:
         self.L.hs_engine_job_info(self.h, job, C.byref(i))
         self._check()
         return i
+
+    def stats(self):
+        st = EngineStats()
+        self.L.hs_engine_get_stats(self.h, C.byref(st))
+        self._check()
+        return st
 
     def release(self, job):
         self.L.hs_engine_release(self.h, job)

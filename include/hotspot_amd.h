@@ -561,11 +561,28 @@ typedef struct hs_job_info_st {
     int horner_steps_min;     /* Taylor: fewest Horner steps per sub-step its group launched while
                                * it ran, when below its plan (0: the plan's every time) */
     int horner_steps_plan;    /* Taylor: Horner steps per sub-step of its a-priori plan */
+    double setup_s;           /* host time of its admission: device state and inputs */
+    int resident_model;       /* 1: admitted on the device state its model kept from an earlier job
+                               * of this engine (coefficients, maps, multigrid levels not rebuilt) */
 } hs_job_info_t;
 int hs_engine_job_info(hs_engine_t *e, int job, hs_job_info_t *info);
 /* frees a finished job's results */
 void hs_engine_release(hs_engine_t *e, int job);
 int hs_engine_active(hs_engine_t *e);
+/* where the engine's host thread has spent its time since creation, and
+ * the chunks it has run (serving observability) */
+typedef struct hs_engine_stats_st {
+    double admit_s;           /* admissions: device state, inputs */
+    double sync_s;            /* uploads of group descriptors after membership changes, tuning */
+    double launch_s;          /* launching chunks */
+    double wait_s;            /* waiting for a chunk to end */
+    double harvest_s;         /* results: phase changes, downloads */
+    long steady_chunks, transient_chunks;
+    long resident_hits;       /* admissions on a model's resident device state */
+    double steady_chunk_s;    /* device time of the steady chunks (first launch to last, summed) */
+    double transient_chunk_s; /* the same for the transient chunks */
+} hs_engine_stats_t;
+void hs_engine_get_stats(hs_engine_t *e, hs_engine_stats_t *stats);
 
 #ifdef __cplusplus
 }

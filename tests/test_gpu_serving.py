@@ -49,11 +49,26 @@ def serving_specs():
     ]
 
 
+# later jobs on the models of earlier ones (the engine keeps a model's device
+# state for its next job): (name, index of the job whose model it uses,
+# power scale, intervals, arrival tick)
+SHARED = [("default-128 steady again", 1, 1.3, 0, 30),
+          ("microchannel-32 again steady+3", 3, 0.8, 3, 40),
+          ("chip-128 again steady+4", 5, 1.1, 4, 60)]
+
+
 class Job:
-    def __init__(self, lib, spec):
-        self.name, case, pw, self.steps, self.from_init, self.arrival = spec
-        self.run = make(lib, case)
-        self.P = self.run.power_vector(pw)
+    def __init__(self, lib, spec, share=None):
+        if share is None:
+            self.name, case, pw, self.steps, self.from_init, self.arrival = spec
+            self.run = make(lib, case)
+            self.P = self.run.power_vector(pw)
+        else:
+            self.name, _, scale, self.steps, self.arrival = spec
+            self.from_init = False
+            self.run = share.run
+            self.P = share.P * scale
+        self.owner = share is None
         self.seq = np.array(cases.jittered_powers(self.P, self.steps, seed=11)) if self.steps else None
         self.init = (np.full(self.run.n, 325.0) + cases.rng(12).uniform(0, 10, self.run.n)) if self.from_init else None
 
@@ -69,9 +84,11 @@ class Job:
 @pytest.fixture(scope="module")
 def jobs(prod, gpu):
     js = [Job(prod, s) for s in serving_specs()]
+    js += [Job(prod, s, share=js[s[1]]) for s in SHARED]
     yield js
     for j in js:
-        j.run.free()
+        if j.owner:
+            j.run.free()
 
 
 def footprints(lib, jobs, form):
@@ -159,6 +176,29 @@ def test_streamed_jobs_equal_alone(prod, jobs, form):
         assert info.latency_s >= info.wait_s >= 0
         assert info.device_bytes <= budget
     _results[form] = res
+
+
+@pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
+def test_resident_model_equals_fresh(prod, jobs, form):
+    """A job admitted on the device state its model kept from an earlier job
+    of the engine (no rebuild of coefficients, maps or multigrid levels)
+    gives the result of a fresh engine, bit for bit."""
+    first, again = jobs[1], next(j for j in jobs if j.name.startswith("chip-128 again"))
+    share = next(j for j in jobs if j.name == "chip-128 steady")
+    eng = engine.Engine(prod, *form)
+    eng.set_tick(8, 128)
+    a = first.submit(eng)
+    b = share.submit(eng)
+    eng.run()
+    c = again.submit(eng)   # same model as b, which has finished
+    eng.run()
+    info = eng.info(c)
+    r, seq = eng.poll(c), eng.result_seq(c)
+    assert not eng.info(a).resident_model and not eng.info(b).resident_model
+    eng.close()
+    assert info.resident_model == 1
+    ra, sa = alone(prod, again, form)
+    assert np.array_equal(r, ra) and np.array_equal(seq, sa)
 
 
 def test_streamed_fp64_matches_oracle(jobs):
