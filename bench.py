@@ -150,39 +150,76 @@ class Job:
         return t2 - t0, t1 - t0, t2 - t1, Ts, out
 
 
-def cpu_baseline(n, intervals_sample):
-    """The reference library (compiled from its own sources) on one host core:
-    compute_temp on the same 256 x 256 x 9 stack, a bounded sample of
-    intervals from a uniform start, extrapolated per interval."""
-    if not os.path.exists(REF_LIB):
-        return None
+def _cpu_worker(task):
+    """One host process: the reference library (compiled from its own
+    sources) runs compute_temp on the stack for k intervals from the given
+    block temperatures; returns the seconds taken."""
+    n, k, T0 = task
     ref = iface.HotSpotLib(REF_LIB)
     case, pw = cases.stacked_case(n, seed=2)
     run = cases.Run(ref, case, tempfile.mkdtemp(prefix="hsref"))
     P = run.power_vector(pw)
-    T0 = np.full(run.n, 333.15)
-    seq = cases.jittered_powers(P, intervals_sample, seed=7)
+    seq = cases.jittered_powers(P, k, seed=7)
     t0 = time.perf_counter()
     cases.run_transient(run, T0, seq, DT)
     dt = time.perf_counter() - t0
-    g = run.grid
-    cells = g.n_layers * g.rows * g.cols
     run.free()
-    return dict(value=cells * intervals_sample / dt, unit="cell-updates/s", cores=1,
-                kind="reference",
-                sample=f"reference compute_temp, {intervals_sample} intervals of 3.333 us on the "
-                       f"{n}x{n}x{g.n_layers} stack ({dt:.1f} s), single-threaded; the steady "
-                       f"solve (1 of {1 + 10000} cell-updates per cell) is not sampled")
+    return dt
 
 
-def served(lib, jobs, intervals, tick, keep_first=False, timer=None):
-    """One batch of jobs through the engine (mixed precision, Taylor): the
-    jobs are submitted and admitted (inputs copied into device memory) before
-    the timed region, which runs ticks until every result is back on the
-    host.  Returns (timed seconds, admission seconds, job infos, first job's
-    block-temperature sequence or None)."""
-    eng = engine.Engine(lib, engine.MIXED, engine.TAYLOR)
-    eng.set_options(precision=engine.MIXED, steady_solver=1, transient=engine.TAYLOR, taylor_tol=TAYLOR_TOL)
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(n, k, T0):
+    """The reference library on the host: compute_temp on the same stack for
+    a bounded sample of k intervals from the steady state's block
+    temperatures (per-interval cost is flat, so the rate extrapolates), once
+    in one process and then in one single-threaded process per available
+    core at the same time (the reference has no threads; independent jobs
+    are how a host would use its cores).  Processes are spawned fresh, so
+    none touches the GPU."""
+    if not os.path.exists(REF_LIB):
+        return None
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    avail = len(os.sched_getaffinity(0))
+    procs = max(1, min(avail, 16))
+    with ctx.Pool(1) as pool:
+        t1 = pool.map(_cpu_worker, [(n, k, T0)])[0]
+    with ctx.Pool(procs) as pool:
+        tp = pool.map(_cpu_worker, [(n, k, T0)] * procs)
+    case, _ = cases.stacked_case(n, seed=2)
+    cells = n * n * (len(case.lcf) + 2)
+    one = cells * k / t1
+    allc = cells * k * procs / max(tp)
+    return dict(value=allc, unit="cell-updates/s", cores=procs, kind="reference",
+                sample=f"reference compute_temp, {k} intervals of 3.333 us on the {n}x{n}x{len(case.lcf) + 2} "
+                       f"stack from the steady state's block temperatures, in each of {procs} single-threaded "
+                       f"processes at once ({max(tp):.1f} s); the steady solve (1 of 10001 cell-updates per cell) "
+                       f"is not sampled",
+                single_core=dict(value=one, seconds=t1),
+                nproc=os.cpu_count(), cores_available=avail, cpu_model=cpu_model())
+
+
+def served(lib, jobs, intervals, tick, keep="none", timer=None, form=None):
+    """One batch of jobs through the engine (by default the mixed form: MG-PCG
+    steady, FP32 Taylor transient): the jobs are submitted and admitted
+    (inputs copied into device memory) before the timed region, which runs
+    ticks until every result is back on the host.  Returns (timed seconds,
+    admission seconds, job infos, block-temperature sequences: none, the
+    first job's, or all)."""
+    prec, trans = form or (engine.MIXED, engine.TAYLOR)
+    eng = engine.Engine(lib, prec, trans)
+    eng.set_options(precision=prec, steady_solver=1 if prec == engine.MIXED else 0, transient=trans,
+                    taylor_tol=TAYLOR_TOL)
     eng.set_tick(tick, 128)
     ids = [eng.submit(j.run.model, j.rows, n_steps=intervals, dt=DT) for j in jobs]
     ta = time.perf_counter()
@@ -195,11 +232,15 @@ def served(lib, jobs, intervals, tick, keep_first=False, timer=None):
     eng.run()
     t1 = time.perf_counter()
     infos = [eng.info(i) for i in ids]
-    first = eng.result_seq(ids[0]) if keep_first else None
+    seqs = None
+    if keep == "first":
+        seqs = [eng.result_seq(ids[0])]
+    elif keep == "all":
+        seqs = [eng.result_seq(i) for i in ids]
     for i in ids:
         eng.release(i)
     eng.close()
-    return t1 - t0, t0 - ta, infos, first
+    return t1 - t0, t0 - ta, infos, seqs
 
 
 def mode_many(lib, args):
@@ -409,6 +450,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served Poisson stream")
+    ap.add_argument("--roofline-batch", type=int, default=48,
+                    help="jobs of the roofline pass (one launch's working set above the Infinity Cache)")
+    ap.add_argument("--roofline-intervals", type=int, default=300)
     ap.add_argument("--taylor-tol", type=float, default=TAYLOR_TOL,
                     help="bound on the Taylor truncation error per interval (K)")
     args = ap.parse_args()
@@ -433,8 +478,9 @@ def main():
         j.rows = np.ascontiguousarray(np.vstack([j.P[None, :], j.seq]))
     cells = jobs[0].cells
     updates_job = cells * (1 + args.intervals)
+    updates_batch = args.batch * updates_job
 
-    # ---- served form: the batch through the engine ----------------------------
+    # ---- the headline: the batch through the engine in the mixed form ---------
     for w in range(args.warmup):
         tw, _, _, _ = served(lib, jobs, args.intervals, args.tick)
         log(f"warmup {w}: batch of {args.batch} in {tw:.3f} s")
@@ -444,83 +490,133 @@ def main():
         times.append(t)
         admit_t.append(ta)
         lat += [i.latency_s for i in infos]
-        step_sweeps = float(sum(i.taylor_sweeps for i in infos))
-        step_node_bytes = float(sum(i.taylor_node_bytes for i in infos))
         log(f"step {k}: batch of {args.batch} in {t:.3f} s (admission {ta*1e3:.0f} ms), "
             f"steady iters {[i.steady_iters for i in infos][:4]}...")
     t_step = float(np.mean(times))
-    value = args.batch * updates_job / t_step
+    value = updates_batch / t_step
 
-    # roofline pass: the same batch with the launches of the dominant kernel
-    # timed by device events on the engine's stream (plain launches instead
-    # of graph replay while the timer is on; consecutive launches of the
-    # kernel share one pair of events around their run, so the mean duration
-    # per launch is the run's span over its launch count, gaps included).
-    # The dominant kernel is the Horner sweep k_tay_term4 (one step
-    # h <- E_1 + s_j M h + t_j h per pass).  Its algorithmic bytes for one job: FP32 read of h,
-    # read of E_1, write of the new h = 12 B per node; 8 B when h is E_1
-    # itself (the job's first sweep of a sub-step); 24 B for the last step,
-    # which adds the increment into the FP64 state.  A launch skips the jobs
-    # whose series needs fewer steps, so the achieved rate is the bytes of the
-    # sweeps the jobs actually took (counted on the device per job:
-    # taylor_node_bytes x nodes) over the summed duration of every launch,
-    # skipping ones included.
-    achieved = avg_ms = nt = sweeps = None
-    nvec = cells + iface.EXTRA
-    bytes_per_sweep = nvec * 12
-    bytes_per_launch = args.batch * bytes_per_sweep
+    # ---- where the speedup comes from: the same batch in other forms ----------
+    # every job of the batch is kept in each form, so accuracy is stated over
+    # the whole batch: mixed against FP64 (the metric's 0.01 K), and each
+    # served job against the same job alone through the single-model API
+    t_mix, _, _, seq_mix = served(lib, jobs, args.intervals, args.tick, keep="all")
+    forms = {}
+    fp64_batch_seqs = None
+    if not args.no_fp64:
+        for name, form in (("fp64_rk4", (engine.FP64, engine.RK4)), ("fp64_taylor", (engine.FP64, engine.TAYLOR))):
+            tf, _, finfo, fseq = served(lib, jobs, args.intervals, args.tick, form=form,
+                                       keep="all" if name == "fp64_rk4" else "none")
+            forms[name] = dict(value=updates_batch / tf, seconds=tf,
+                               rk4_calls_per_interval=float(np.mean([i.rk4_calls for i in finfo])) / args.intervals)
+            if name == "fp64_rk4":
+                fp64_batch_seqs = fseq
+            log(f"batch in form {name}: {tf:.3f} s, {updates_batch / tf:.3e} cell-updates/s")
+    forms["mixed_taylor"] = dict(value=value, seconds=t_step)
+
+    # ---- the same jobs alone through the single-model API ---------------------
+    single = None
+    alone_equal = True
+    for b, j in enumerate(jobs):
+        j.options(1, 1)
+        if b == 0:
+            j.once()    # device set-up and term tiling outside the timing
+        t1s, ts1, tt1, Ts_b, seq_b = j.once(keep_seq=True)
+        alone_equal &= bool(np.array_equal(seq_b, seq_mix[b]))
+        if b == 0:
+            st = j.stats()
+            single = dict(value=updates_job / t1s, job_s=t1s, steady_ms=ts1 * 1e3, transient_s=tt1,
+                          steady_inner_iters=st.steady_iters, steady_refinements=st.steady_refinements,
+                          taylor_terms_max=st.taylor_terms, taylor_substeps=st.taylor_substeps)
+            log(f"single mixed job {t1s:.3f} s (steady {ts1*1e3:.1f} ms, transient {tt1:.3f} s)")
+    single["served_equals_single_all_jobs"] = alone_equal
+    log(f"every served job equals its single-model run bit for bit: {alone_equal}")
+
+    # ---- the plain FP64 GPU form: one job at a time ---------------------------
+    fp64 = None
+    Ts64 = None
+    if not args.no_fp64:
+        j0 = jobs[0]
+        j0.options(0, 0)    # FP64 Jacobi-CG + the reference's adaptive RK4
+        t64, ts64, tt64, Ts64, seq64 = j0.once(keep_seq=True)
+        st64 = j0.stats()
+        err_batch = float(max(np.abs(a - b).max() for a, b in zip(seq_mix, fp64_batch_seqs)))
+        fp64 = dict(value=updates_job / t64, job_s=t64, steady_ms=ts64 * 1e3, transient_s=tt64,
+                    rk4_calls_per_interval=st64.rk4_calls / args.intervals,
+                    max_abs_diff_K_vs_mixed=float(np.abs(seq64 - seq_mix[0]).max()),
+                    max_abs_diff_K_mixed_vs_fp64_whole_batch=err_batch,
+                    fp64_batch_equals_single=bool(np.array_equal(seq64, fp64_batch_seqs[0])))
+        log(f"fp64 job {t64:.3f} s (steady {ts64*1e3:.1f} ms); max |dT| mixed vs fp64 over the batch "
+            f"{err_batch:.2e} K")
+    del seq_mix, fp64_batch_seqs
+
+    # ---- roofline pass of the dominant kernel ----------------------------------
+    # The Horner sweep k_tay_term4 (one step h <- E_1 + s_j M h + t_j h per
+    # launch) timed by device events on the engine's stream: plain launches
+    # instead of graph replay while the timer is on, one transient stream;
+    # consecutive launches of the kernel share one pair of events around
+    # their run, so the mean duration per launch is the run's span over its
+    # launch count, gaps included.  Algorithmic bytes for one job: FP32 read
+    # of h, read of E_1, write of the new h = 12 B per node; 8 B when h is E_1
+    # itself (a job's first step of a sub-step); 24 B for the last step, which
+    # adds the increment into the FP64 state.  A launch skips the jobs whose
+    # series needs fewer steps, so the achieved rate is the bytes of the steps
+    # the jobs took (counted on the device per job: taylor_node_bytes x
+    # nodes) over the summed duration of every launch.  The pass serves
+    # --roofline-batch jobs so that one launch's working set (3 FP32 vectors
+    # per job) is above the 256 MB Infinity Cache: the bytes are HBM bytes.
     kernel = "k_tay_term"
+    nvec = cells + iface.EXTRA
+    rb = args.roofline_batch
+    achieved = avg_ms = nt = None
+    node_bytes = tot_ms = 0.0
     if not args.no_roofline:
-        t_roof, _, rinfo, _ = served(lib, jobs, args.intervals, args.tick, timer=kernel.encode())
+        rjobs = jobs + [Job(lib, args.grid, args.roofline_intervals, seed=2 + b) for b in range(args.batch, rb)]
+        full_rows = [j.rows for j in jobs]
+        for j in rjobs:
+            j.rows = np.ascontiguousarray(np.vstack([j.P[None, :], j.seq[:args.roofline_intervals]]))
+        served(lib, rjobs, args.roofline_intervals, args.tick)
+        _, _, rinfo, _ = served(lib, rjobs, args.roofline_intervals, args.tick, timer=kernel.encode())
         device_sync()
         a, tot = C.c_double(), C.c_double()
         nt = L.hs_kernel_timer_read(C.byref(a), C.byref(tot))
         L.hs_kernel_timer(None, 0)
-        avg_ms = a.value
-        sweeps = float(sum(i.taylor_sweeps for i in rinfo))
+        avg_ms, tot_ms = a.value, tot.value
         node_bytes = float(sum(i.taylor_node_bytes for i in rinfo))
-        tot_ms = tot.value
+        if nt:
+            achieved = node_bytes * nvec / (tot_ms * 1e-3) / 1e9
+            log(f"roofline pass ({rb} jobs): {nt} launches of {kernel}, mean {avg_ms*1e3:.2f} us, "
+                f"{node_bytes * nvec / nt / 1e6:.1f} MB per launch on average, {achieved:.0f} GB/s")
+        for j in rjobs[args.batch:]:
+            j.run.free()
+        for j, r in zip(jobs, full_rows):
+            j.rows = r
     traffic = traffic_ratio = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_{kernel}_3dstack{args.grid}_batch{args.batch}.json")
+    tf = os.path.join(ROOT, "profiles", f"traffic_{kernel}_3dstack{args.grid}_batch{rb}.json")
     if os.path.exists(tf):
         with open(tf) as f:
             tj = json.load(f)
         traffic, traffic_ratio = tj.get("bytes_per_launch"), tj.get("traffic_over_algorithmic")
 
-    # ---- the same first job alone through the single-model API ------------------
-    j0 = jobs[0]
-    j0.options(1, 1)
-    j0.once()
-    t1s, ts1, tt1, Ts_mixed, seq_mixed = j0.once(keep_seq=True)
-    st = j0.stats()
-    log(f"single mixed job {t1s:.3f} s (steady {ts1*1e3:.1f} ms, transient {tt1:.3f} s)")
-    single = dict(value=updates_job / t1s, job_s=t1s, steady_ms=ts1 * 1e3, transient_s=tt1,
-                  steady_inner_iters=st.steady_iters, steady_refinements=st.steady_refinements,
-                  taylor_terms_max=st.taylor_terms, taylor_substeps=st.taylor_substeps)
-    # the served result of the same job equals it bit for bit
-    _, _, _, served_seq = served(lib, jobs[:1], args.intervals, args.tick, keep_first=True)
-    single["served_equals_single"] = bool(np.array_equal(served_seq, seq_mixed))
-    if nt:
-        achieved = node_bytes * nvec / (tot_ms * 1e-3) / 1e9
-        log(f"roofline pass: {nt} launches of {kernel}, mean {avg_ms*1e3:.2f} us, "
-            f"{node_bytes * nvec / nt / 1e6:.1f} MB per launch on average, {achieved:.0f} GB/s")
+    # ---- the served Poisson stream (requests arriving while others run) -------
+    stream = None
+    if not args.no_serve:
+        stream = mode_serve(lib, args)
 
-    fp64 = None
-    if not args.no_fp64:
-        j0.options(0, 0)    # FP64 Jacobi-CG + the reference's adaptive RK4
-        t64, ts64, tt64, Ts64, seq64 = j0.once(keep_seq=True)
-        st64 = j0.stats()
-        err = float(max(np.abs(seq64 - seq_mixed).max(), np.abs(Ts64 - Ts_mixed).max()))
-        fp64 = dict(value=updates_job / t64, job_s=t64, steady_ms=ts64 * 1e3, transient_s=tt64,
-                    rk4_calls_per_interval=st64.rk4_calls / args.intervals,
-                    max_abs_diff_K_vs_mixed=err)
-        log(f"fp64 job {t64:.3f} s (steady {ts64*1e3:.1f} ms), max |dT| mixed vs fp64 {err:.2e} K")
-
+    # ---- the reference library on the host --------------------------------------
     cpu = None
     if not args.no_cpu:
         log("timing the reference library on the host")
-        cpu = cpu_baseline(args.grid, args.cpu_intervals)
+        T0 = Ts64 if Ts64 is not None else jobs[0].run.steady(jobs[0].P.copy())
+        cpu = cpu_baseline(args.grid, args.cpu_intervals, T0)
 
+    lat_note = "submission -> result on the host, admission included"
+    latency = {"batch_job_p50": float(np.percentile(lat, 50)) * 1e3,
+               "batch_job_p99": float(np.percentile(lat, 99)) * 1e3,
+               "batch_note": "the headline batch: all jobs submitted together; " + lat_note}
+    if stream:
+        for form_name, f in stream["forms"].items():
+            latency[f"stream_{form_name}"] = {k: f[k] for k in ("p50_ms", "p99_ms", "steady_p50_ms", "steady_p99_ms")}
+        latency["stream_note"] = ("served Poisson stream (" + stream["config"]["workload"] + "); " + lat_note)
     res = {
         "metric": "cell-updates/s + p50/p99 solve latency vs plain FP64 GPU at <=0.01 K max err",
         "value": value,
@@ -541,28 +637,35 @@ def main():
                    "tick_intervals": args.tick, "precision": "mixed",
                    "steady": "multigrid-preconditioned CG, FP32 inside FP64 refinement",
                    "transient": "Taylor (exponential) FP32/FP64", "taylor_tol_K": TAYLOR_TOL},
-        "latency_ms": {"job_p50": float(np.percentile(lat, 50)) * 1e3,
-                       "job_p99": float(np.percentile(lat, 99)) * 1e3,
-                       "note": "submission -> result on the host, admission included"},
+        "latency_ms": latency,
         "admission_ms": float(np.mean(admit_t)) * 1e3,
-        "value_with_admission": args.batch * updates_job / (t_step + float(np.mean(admit_t))),
+        "value_with_admission": updates_batch / (t_step + float(np.mean(admit_t))),
         "roofline": {"bound": "hbm", "kernel": "k_tay_term4 (FP32 Horner step of the Taylor series, float4 tiles)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic, "launches_timed": nt,
                      "avg_launch_us": avg_ms * 1e3 if avg_ms else None,
-                     "bytes_per_launch": bytes_per_launch,
                      "algorithmic_bytes_avg_per_launch": (node_bytes * nvec / nt) if nt else None,
-                     "steps_per_batch": step_sweeps,
-                     "node_bytes_per_batch": step_node_bytes,
+                     "jobs_per_launch": rb, "intervals": args.roofline_intervals,
+                     "working_set_per_launch_MB": rb * nvec * 12 / 1e6,
                      "traffic_over_algorithmic": traffic_ratio,
-                     "traffic_note": "HBM bytes per launch from FETCH_SIZE and WRITE_SIZE (rocprofv3 "
-                                     "--pmc, scaled against a plain copy), 300 intervals in ticks of 20: "
-                                     "the same launch mix as this run's, whose algorithmic bytes per "
-                                     "launch it exceeds by traffic_over_algorithmic"},
+                     "note": f"{rb} jobs per launch: 3 FP32 vectors per job, {rb * nvec * 12 / 1e6:.0f} MB per "
+                             f"launch, above the 256 MB Infinity Cache; traffic from rocprofv3 FETCH_SIZE and "
+                             f"WRITE_SIZE of the same pass, scaled against a plain copy"},
+        "forms_batch": forms,
+        "speedup_split": ({"batching_fp64_rk4_over_single_fp64": forms["fp64_rk4"]["value"] / fp64["value"],
+                           "integrator_taylor_over_rk4_fp64_batched": forms["fp64_taylor"]["value"] /
+                           forms["fp64_rk4"]["value"],
+                           "precision_mixed_over_fp64_taylor_batched": value / forms["fp64_taylor"]["value"]}
+                          if fp64 and "fp64_rk4" in forms else None),
         "single_job": single,
         "fp64_gpu": fp64,
         "speedup_vs_fp64_gpu": (value / fp64["value"]) if fp64 else None,
+        "served_stream": ({"value_mixed": stream["forms"]["mixed"]["value"],
+                           "value_fp64": stream["forms"].get("fp64", {}).get("value"),
+                           "speedup_vs_fp64": stream["speedup_vs_fp64"],
+                           "requests": stream["config"]["requests"],
+                           "forms": stream["forms"]} if stream else None),
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
