@@ -335,8 +335,8 @@ static void gather(BatchGroup &B, std::initializer_list<int> slots) {
                           B.g.stream));
 }
 
-// A job's device model goes back to the resident set (its sequence buffers
-// to the pool) when its model is unchanged, else to the pool.
+// A job's device model goes back to the resident set, its sequence buffers
+// to the pool.
 static void release_device(Engine &E, EngJob &j) {
     E.used -= j.held;
     j.held = 0;
@@ -345,10 +345,9 @@ static void release_device(Engine &E, EngJob &j) {
     D.release(D.h.seq_out);
     D.h.seq_in = D.h.seq_out = nullptr;
     D.h.seq_one = 0;
-    if (j.M->stamp != j.stamp) {
-        j.D.reset();
-        return;
-    }
+    // kept under the stamp of the coefficients it was built from (the model
+    // itself may be gone by now: a later model, or this one repopulated,
+    // carries another stamp and never matches)
     E.resident_bytes += D.bytes;
     E.resident.push_back(Resident{j.M, j.stamp, std::move(j.D), now_s()});
 }
