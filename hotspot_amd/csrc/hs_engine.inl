@@ -4,6 +4,7 @@
 #include <array>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <unordered_map>
 
@@ -2025,4 +2026,76 @@ int hs_kernel_timer_read(double *avg_ms, double *total_ms) {
     return n;
 }
 
+}
+
+// ---- the operator with no device allocation ----------------------------------------
+namespace hs {
+
+// Device scratch of hs_grid_slope_noalloc: the operator's descriptor (pointing
+// at the caller's buffers) in the first 256-byte-aligned bytes, then the
+// cell power map (one FP64 grid vector).  Every GPU the library runs on holds
+// its own copy until the process ends.  In host code `scratch` only names the
+// symbol: its device address, per device, comes from hipGetSymbolAddress.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static unsigned char *scratch_on_current_device() {
+    static std::mutex mu;
+    static std::vector<unsigned char *> ptr;
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)ptr.size() <= dev) ptr.resize(dev + 1, nullptr);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            fatal(std::string("hs_grid_slope_noalloc: no address for the device scratch: ") + hipGetErrorString(e) +
+                  "\n");
+        ptr[dev] = (unsigned char *)p;
+    }
+    return ptr[dev];
+}
+
+static void slope_noalloc(Model &M, const double *d_power, const double *d_temp, double *d_slope) {
+    const size_t desc_bytes = (sizeof(ProbDesc) + 255) / 256 * 256;
+    const size_t vlen = ((size_t)M.ncell() + M.nx + 63) / 64 * 64;
+    const size_t need = desc_bytes + vlen * sizeof(double);
+    if (need > kScratchBytes)
+        fatal("hs_grid_slope_noalloc: the model needs " + std::to_string(need) + " bytes of scratch, above its " +
+              std::to_string(kScratchBytes) + "\n");
+    if (!M.dev || M.dev_dirty || M.dev->has_c != (bool)M.g->c_ready)
+        fatal("hs_grid_slope_noalloc: the model has no device state (build it first, e.g. with hs_grid_slope)\n");
+    if (!d_power || !d_temp || !d_slope) fatal("hs_grid_slope_noalloc: null device buffer\n");
+    DevModel &D = *M.dev;
+    unsigned char *base = scratch_on_current_device();
+    ProbDesc h = D.h;
+    h.blk_in = const_cast<double *>(d_power);
+    h.b64[B_RHS] = (double *)(base + desc_bytes);
+    h.b64[B_TMP] = const_cast<double *>(d_temp);
+    h.b64[B_KA] = d_slope;
+    ProbDesc *dd = (ProbDesc *)base;
+    HIP_OK(hipMemcpyAsync(dd, &h, sizeof h, hipMemcpyHostToDevice, D.stream));
+    launch(k_b2g, D.nwork, D.stream, (const ProbDesc *)dd, (const WorkItem *)D.d_work, (int)V_POWER, (int)B_RHS, 1, 0,
+           0);
+    if (D.hetero)
+        launch(k_slope<true>, D.nwork, D.stream, (const ProbDesc *)dd, (const WorkItem *)D.d_work, (int)B_TMP,
+               (int)B_KA);
+    else
+        launch(k_slope<false>, D.nwork, D.stream, (const ProbDesc *)dd, (const WorkItem *)D.d_work, (int)B_TMP,
+               (int)B_KA);
+    HIP_OK(hipStreamSynchronize(D.stream));
+}
+
+}  // namespace hs
+
+extern "C" int hs_grid_slope_noalloc(RC_model_t *model, const double *d_block_power, const double *d_grid_temp,
+                                     double *d_grid_slope) {
+    HS_BEGIN
+    if (!model || model->type != GRID_MODEL) hs::fatal("hs_grid_slope_noalloc: not a grid model\n");
+    hs::Model &M = *hs::model_of(model);
+    if (!M.g->r_ready || !M.g->c_ready) hs::fatal("grid model not ready\n");
+    hs::slope_noalloc(M, d_block_power, d_grid_temp, d_grid_slope);
+    return 0;
+    HS_END(-1)
 }

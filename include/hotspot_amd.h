@@ -475,6 +475,20 @@ void hs_set_grid_temp(RC_model_t *model, int which, const double *in);
  * the flat grid layout; for operator-level checks. */
 void hs_grid_slope(RC_model_t *model, const double *block_power, const double *grid_temp,
                    double *grid_slope);
+/* hs_grid_slope on device buffers, allocating no device memory.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, allocation-free serving loops).
+ * d_block_power (total blocks + package nodes; package entries ignored),
+ * d_grid_temp and d_grid_slope (hs_grid_size doubles each) are device
+ * pointers the caller owns.  The descriptor and the cell power map go to a
+ * fixed 32 MiB library scratch array per GPU, so grids of more than about
+ * 4.19 million nodes are refused, and the model's device state must exist
+ * already (built by an earlier call such as hs_grid_slope or a solve), else
+ * the call fails.  All calls on one GPU share its scratch: calls that
+ * overlap in time on one GPU race.  Returns 0, or -1 on error
+ * (hs_last_error). */
+int hs_grid_slope_noalloc(RC_model_t *model, const double *d_block_power, const double *d_grid_temp,
+                          double *d_grid_slope);
 /* block -> grid translation on the device (type V_POWER or V_TEMP) */
 void hs_block_to_grid(RC_model_t *model, const double *block_vec, double *grid_vec, int type);
 /* grid -> block translation on the device (the model's grid_map_mode) */
