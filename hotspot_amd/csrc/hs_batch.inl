@@ -294,6 +294,8 @@ static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans) {
         std::vector<WorkItem> tw;
         const int per = term_items(desc, tw);
         upload_term_items(B.g, tw, per, B.cap_twork, B.pool, s);
+        B.g.tz = term_form(desc);
+        B.g.tg = term_groups(desc);
     }
     // multigrid levels of a steady group (every member has the same count)
     for (void *p : B.lv_bufs) dfree(B.pool, p);

@@ -2187,8 +2187,83 @@ __device__ __forceinline__ void add_state4(double *y, unsigned c, const float (&
     st2(y, c + 2, b);
 }
 
+// Coefficients of one layer for the FP32 Horner steps.
+struct LayerF {
+    float gx, gy, gu, gd, ci, gam, rgn, rge;
+    const int *rn;
+};
+__device__ __forceinline__ LayerF layer_f(const ProbDesc &d, int n) {
+    LayerF L;
+    L.gx = (float)d.gx[n];
+    L.gy = (float)d.gy[n];
+    L.gu = n > 0 ? (float)d.gz[max(n - 1, 0)] : 0.f;
+    L.gd = n < d.nl - 1 ? (float)d.gz[n] : 0.f;
+    L.ci = (float)d.cinv[n];
+    L.gam = (float)d.gamb[n];
+    L.rn = d.ring_node[n];
+    L.rgn = (float)d.ring_gns[n];
+    L.rge = (float)d.ring_gew[n];
+    return L;
+}
+
+// One Horner step on 4 cells of row i, columns j0..j0+3 of a layer, from the
+// cells (C), the rows above and below (N, S), the layers above and below (U,
+// D), the columns left of j0 and right of j0+3 (xw0, xe3) and E_1 (A):
+//   e = E_1 + s C^-1 (flux - g_amb x) + t x.
+// A missing neighbour has zero conductance (its value is the cell's own).
+// Both float4 sweeps (per layer and z-marching) compute a cell with this one
+// function, so its arithmetic does not depend on the sweep or its tiling.
+__device__ __forceinline__ void horner_row4(const ProbDesc &d, const LayerF &L, int i, int j0, const float4 &C,
+                                            const float4 &N, const float4 &S, const float4 &U, const float4 &D,
+                                            float xw0, float xe3, const float4 &A, float s, float sft,
+                                            const float (&ring)[4], float (&e)[4]) {
+    // multiply-adds fuse only within a statement, as written, wherever the
+    // function is inlined
+#pragma clang fp contract(on)
+    const int nr = d.nr, nc = d.nc;
+    if (i > 0 && i < nr - 1 && j0 > 0 && j0 + 4 < nc) {
+        // away from the edges: every neighbour exists and no ring term
+        // applies (the same operations as below with those known)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float xc = comp(C, k);
+            const float xw = k == 0 ? xw0 : comp(C, k - 1);
+            const float xe = k == 3 ? xe3 : comp(C, k + 1);
+            float flux = L.gx * (comp(N, k) - xc);
+            flux += L.gx * (comp(S, k) - xc);
+            flux += L.gy * (xe - xc);
+            flux += L.gy * (xw - xc);
+            flux += L.gu * (comp(U, k) - xc);
+            flux += L.gd * (comp(D, k) - xc);
+            e[k] = comp(A, k) + s * (L.ci * (flux - L.gam * xc)) + sft * xc;
+        }
+        return;
+    }
+    const float gn = i > 0 ? L.gx : 0.f, gs = i < nr - 1 ? L.gx : 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int j = j0 + k;
+        const float xc = comp(C, k);
+        const float xw = k == 0 ? xw0 : comp(C, k - 1);
+        const float xe = k == 3 ? xe3 : comp(C, k + 1);
+        const float ge = j < nc - 1 ? L.gy : 0.f, gw = j > 0 ? L.gy : 0.f;
+        float flux = gn * (comp(N, k) - xc);
+        flux += gs * (comp(S, k) - xc);
+        flux += ge * (xe - xc);
+        flux += gw * (xw - xc);
+        flux += L.gu * (comp(U, k) - xc);
+        flux += L.gd * (comp(D, k) - xc);
+        if (i == 0 && L.rn[0] >= 0) flux += L.rgn * (ring[0] - xc);
+        if (i == nr - 1 && L.rn[1] >= 0) flux += L.rgn * (ring[1] - xc);
+        if (j == nc - 1 && L.rn[2] >= 0) flux += L.rge * (ring[2] - xc);
+        if (j == 0 && L.rn[3] >= 0) flux += L.rge * (ring[3] - xc);
+        e[k] = comp(A, k) + s * (L.ci * (flux - L.gam * xc)) + sft * xc;
+    }
+}
+
 // With y given (the last step, j = 2), the result is the sub-step's
 // increment and goes into the FP64 state instead of hout.
+template <int GM>
 __device__ __forceinline__ void term_vec4(const ProbDesc &d, int tile, const float *hin, float *hout,
                                           const float *e1, float s, float sft, const float (&ring)[4], double *y) {
     const int ntj = term_ntj(d), tpl = d.t_nti * ntj;
@@ -2201,12 +2276,7 @@ __device__ __forceinline__ void term_vec4(const ProbDesc &d, int tile, const flo
     const int nr = d.nr, nc = d.nc, nl = d.nl;
     const unsigned plane = (unsigned)nr * nc;
     const int j0 = (tj << sh) + 4 * tx4;
-    const float gx = (float)d.gx[n], gy = (float)d.gy[n];
-    const float gu = n > 0 ? (float)d.gz[max(n - 1, 0)] : 0.f, gd = n < nl - 1 ? (float)d.gz[n] : 0.f;
-    const float ci = (float)d.cinv[n], gam = (float)d.gamb[n];
-    const int *rn = d.ring_node[n];
-    const float rgn = (float)d.ring_gns[n], rge = (float)d.ring_gew[n];
-    constexpr int GM = 2;
+    const LayerF L = layer_f(d, n);
     float4 C[GM], N[GM], S[GM], U[GM], Dn[GM], A[GM];
     float Wv[GM], Ev[GM];
     int I[GM];
@@ -2231,30 +2301,83 @@ __device__ __forceinline__ void term_vec4(const ProbDesc &d, int tile, const flo
 #pragma unroll
     for (int g = 0; g < GM; g++) {
         if (g >= G || I[g] >= nr) continue;
-        const int i = I[g];
-        const float gn = i > 0 ? gx : 0.f, gs = i < nr - 1 ? gx : 0.f;
         float e[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int j = j0 + k;
-            const float xc = comp(C[g], k);
-            const float xw = k == 0 ? Wv[g] : comp(C[g], k - 1);
-            const float xe = k == 3 ? Ev[g] : comp(C[g], k + 1);
-            const float ge = j < nc - 1 ? gy : 0.f, gw = j > 0 ? gy : 0.f;
-            float flux = gn * (comp(N[g], k) - xc);
-            flux += gs * (comp(S[g], k) - xc);
-            flux += ge * (xe - xc);
-            flux += gw * (xw - xc);
-            flux += gu * (comp(U[g], k) - xc);
-            flux += gd * (comp(Dn[g], k) - xc);
-            if (i == 0 && rn[0] >= 0) flux += rgn * (ring[0] - xc);
-            if (i == nr - 1 && rn[1] >= 0) flux += rgn * (ring[1] - xc);
-            if (j == nc - 1 && rn[2] >= 0) flux += rge * (ring[2] - xc);
-            if (j == 0 && rn[3] >= 0) flux += rge * (ring[3] - xc);
-            e[k] = comp(A[g], k) + s * (ci * (flux - gam * xc)) + sft * xc;
-        }
+        horner_row4(d, L, I[g], j0, C[g], N[g], S[g], U[g], Dn[g], Wv[g], Ev[g], A[g], s, sft, ring, e);
         if (y) add_state4(y, cc[g], e);
         else st4(hout, cc[g], make_float4(e[0], e[1], e[2], e[3]));
+    }
+}
+
+// The same step z-marching: the workgroup takes a column tile (the term
+// tiling's columns and rows) through every layer from the top, so each
+// thread keeps the layers above and below its cells in registers and reads
+// each plane of h once (a cell costs a third of the load instructions of the
+// per-layer sweep, and the L2 serves a fraction of the bytes).  The rows
+// above and below a thread's own come from its registers but at the thread's
+// first and last row.
+template <int GM>
+__device__ __forceinline__ void term_vec4z(const ProbDesc &d, int ztile, const float *hin, float *hout,
+                                           const float *e1, float s, float sft, double *yb) {
+    const int ntj = term_ntj(d);
+    const int ti = ztile / ntj, tj = ztile - ti * ntj;
+    const int sh = d.t_shift;
+    const int tx4 = threadIdx.x & ((1 << (sh - 2)) - 1), ty4 = threadIdx.x >> (sh - 2);
+    const int G = d.t_rpt >> 2;
+    const int rows = (kBlock >> sh) * d.t_rpt;
+    const int nr = d.nr, nc = d.nc, nl = d.nl;
+    const unsigned plane = (unsigned)nr * nc;
+    const unsigned ncell = plane * (unsigned)nl;
+    const int j0 = (tj << sh) + 4 * tx4;
+    int I[GM], ic[GM];
+    float4 U[GM], C[GM], Dn[GM];
+#pragma unroll
+    for (int g = 0; g < GM; g++) {
+        I[g] = ti * rows + ty4 * G + g;
+        ic[g] = min(I[g], nr - 1);
+        if (g < G) {
+            C[g] = ld4(hin, (unsigned)ic[g] * nc + j0);
+            U[g] = C[g];
+        }
+    }
+#pragma unroll 1
+    for (int n = 0; n < nl; n++) {
+        const unsigned off = (unsigned)n * plane;
+        float4 N0, SG, A[GM];
+        float Wv[GM], Ev[GM];
+#pragma unroll
+        for (int g = 0; g < GM; g++) {
+            if (g >= G) continue;
+            const int i = ic[g];
+            const unsigned c = off + (unsigned)i * nc + j0;
+            Dn[g] = n < nl - 1 ? ld4(hin, c + plane) : C[g];
+            if (g == 0) N0 = ld4(hin, i > 0 ? c - nc : c);
+            if (g == G - 1) SG = ld4(hin, i < nr - 1 ? c + nc : c);
+            Wv[g] = ld_at(hin, j0 > 0 ? c - 1 : c);
+            Ev[g] = ld_at(hin, j0 + 4 < nc ? c + 4 : c);
+            A[g] = ld4(e1, c);
+        }
+        float ring[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) ring[q] = ld_at(hin, ncell + max(d.ring_node[n][q], 0));
+        const LayerF L = layer_f(d, n);
+#pragma unroll
+        for (int g = 0; g < GM; g++) {
+            if (g >= G || I[g] >= nr) continue;
+            // the rows next to row g inside the thread's own rows are its
+            // registers (the clamps of the loads give the same values)
+            const float4 N = g == 0 ? N0 : C[g - 1];
+            const float4 S = g == G - 1 ? SG : C[g + 1];
+            const unsigned c = off + (unsigned)ic[g] * nc + j0;
+            float e[4];
+            horner_row4(d, L, I[g], j0, C[g], N, S, U[g], Dn[g], Wv[g], Ev[g], A[g], s, sft, ring, e);
+            if (yb) add_state4(yb, c, e);
+            else st4(hout, c, make_float4(e[0], e[1], e[2], e[3]));
+        }
+#pragma unroll
+        for (int g = 0; g < GM; g++) {
+            U[g] = C[g];
+            C[g] = Dn[g];
+        }
     }
 }
 
@@ -2330,22 +2453,36 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(const ProbDesc *__restrict_
 }
 
 // k_tay_term<float, false> for groups whose problems all take float4 tiles
-// (vec4_ok), over the term tiling (t_*) and its work-list: the package item
-// is the scalar kernel's.
+// (vec4_ok), over the term tiling (t_*) and its work-list: per layer tiles,
+// or (t_z) column tiles through every layer; the package item is the scalar
+// kernel's.
 //
 // With fin set (the launch of step j = 2, the last of every series), the
 // step's result -- the sub-step's increment -- is added to the FP64 state
 // in place of k_tay_final, and a problem whose series has one term adds
 // E_1 in the same launch.
+// ZF: 0 every problem of the launch takes per-layer tiles, 1 every problem
+// z-marches, 2 either by its t_z; GM: most row groups per thread (t_rpt / 4)
+// of the launch's problems.  Each form compiled alone keeps its own register
+// budget.
+template <int ZF, int GM, int FIN>
 __device__ __forceinline__ void term4_item(const ProbDesc *__restrict__ D, const TayItem &it, int e1b, int d0b,
-                                           int d1b, int j, int fin) {
+                                           int d1b, int j) {
+    constexpr bool fin = FIN != 0;
     if (it.K == 0 || (!fin && j > it.K)) return;
     const ProbDesc &d = D[it.prob];
+    const bool z = ZF == 2 ? d.t_z != 0 : ZF == 1;
+    const int tpl = d.t_ntiles / d.nl;                 // term tiles per layer
+    const int ntile = z ? tpl : d.t_ntiles;            // cell items of the problem
     const float *e1 = d.b32[e1b];
     double *y = fin ? d.b64[B_Y0 + (int)d.scal[S_YPAR]] : nullptr;
     if (j > it.K) {   // fin, one term
-        if (it.tile < d.t_ntiles) add_e1_vec4(d, it.tile, e1, y);
-        else if ((int)threadIdx.x < d.nx) {
+        if (it.tile < ntile) {
+            if (z)
+                for (int n = 0; n < d.nl; n++) add_e1_vec4(d, n * tpl + it.tile, e1, y);
+            else
+                add_e1_vec4(d, it.tile, e1, y);
+        } else if ((int)threadIdx.x < d.nx) {
             const unsigned c = (unsigned)d.nl * d.nr * d.nc + threadIdx.x;
             y[c] += (double)e1[c];
         }
@@ -2356,10 +2493,14 @@ __device__ __forceinline__ void term4_item(const ProbDesc *__restrict__ D, const
     float *hout = d.b32[hb.out];
     const float s = it.recf[2 * j], sft = it.recf[2 * j + 1];
     LdPlain<float> ld{hin};
-    if (it.tile < d.t_ntiles) {
-        float ring[4];
-        load_ring<float>(d, ld, term_layer(d, it.tile), ring);
-        term_vec4(d, it.tile, hin, hout, e1, s, sft, ring, y);
+    if (it.tile < ntile) {
+        if (z) {
+            term_vec4z<GM>(d, it.tile, hin, hout, e1, s, sft, y);
+        } else {
+            float ring[4];
+            load_ring<float>(d, ld, term_layer(d, it.tile), ring);
+            term_vec4<GM>(d, it.tile, hin, hout, e1, s, sft, ring, y);
+        }
     } else {
         __shared__ float xf[kMaxExtra], xd[kMaxExtra];
         package_terms<float>(d, ld, true, xf, xd);
@@ -2373,12 +2514,14 @@ __device__ __forceinline__ void term4_item(const ProbDesc *__restrict__ D, const
     }
 }
 
+// FIN: the launch of step j = 2, which adds into the FP64 state
+template <int ZF, int GM, int FIN>
 __global__ void __launch_bounds__(kBlock) k_tay_term4(const ProbDesc *__restrict__ D,
                                                       const WorkItem *__restrict__ W, int nper, int e1b,
                                                       int d0b, int d1b, int j, const int *__restrict__ kmax,
-                                                      const double *__restrict__ trec, int fin) {
-    if (!fin && j > *kmax) return;
-    term4_item(D, tay_item(W, nper, trec), e1b, d0b, d1b, j, fin);
+                                                      const double *__restrict__ trec) {
+    if (!FIN && j > *kmax) return;
+    term4_item<ZF, GM, FIN>(D, tay_item(W, nper, trec), e1b, d0b, d1b, j);
 }
 
 // y += h (the sum of the sub-step's terms)

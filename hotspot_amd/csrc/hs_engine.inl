@@ -123,6 +123,8 @@ struct Group {
     // work-list of the float4 Taylor sweep over the term tiling (t_*)
     WorkItem *d_twork = nullptr;
     int ntwork = 0, tnper = 0;
+    int tz = 0;   // its tiles: 0 per layer for every problem, 1 z-marching for every problem, 2 mixed
+    int tg = 2;   // most row groups of 4 rows per thread among its problems (t_rpt / 4)
 };
 
 // A homogeneous problem whose tiles are at least 16 columns wide with at
@@ -288,14 +290,18 @@ static int rows_per_thread(int nr, int ty) {
 }
 
 // tiling of the float4 Taylor sweep: tile width 1 << shift (at most nc),
-// rpt rows per thread
-static void set_term_tiling(ProbDesc &h, int shift, int rpt) {
+// rpt rows per thread; z: column tiles through every layer instead of one
+// tile per layer
+static void set_term_tiling(ProbDesc &h, int shift, int rpt, int z = 0) {
     const int ty = kBlock >> shift;
     h.t_shift = shift;
     h.t_rpt = rpt;
     h.t_nti = (h.nr + ty * rpt - 1) / (ty * rpt);
     h.t_ntiles = h.nl * h.t_nti * (h.nc >> shift);
+    h.t_z = z;
 }
+// cell items of the float4 sweep's work-list for a problem
+static int term_cell_items(const ProbDesc &h) { return h.t_z ? h.t_ntiles / h.nl : h.t_ntiles; }
 
 static void set_tiling(ProbDesc &h) {
     h.tx_shift = ilog2(std::min(h.nc, 64));
@@ -311,12 +317,47 @@ static void set_tiling(ProbDesc &h) {
 // Work-list of the float4 term sweep: (problem, tile) over each problem's
 // term tiling, its package item last.  Returns the items per problem when
 // every problem has the same count, else 0.
+// the tile form of a launch over these problems (Group::tz, Group::tg)
+static int term_form(const std::vector<ProbDesc> &desc) {
+    int z = 0, l = 0;
+    for (const ProbDesc &h : desc) (h.t_z ? z : l)++;
+    return l == 0 && z > 0 ? 1 : z == 0 ? 0 : 2;
+}
+static int term_groups(const std::vector<ProbDesc> &desc) {
+    int g = 1;
+    for (const ProbDesc &h : desc) g = std::max(g, h.t_rpt / 4);
+    return g;
+}
+
+// the float4 Horner step over the group's term work-list, in the kernel of
+// its tile form (fin: the step adds its result into the FP64 state)
+static void launch_term4(const Group &G, hipStream_t s, int e1b, int d0b, int d1b, int j, int fin, bool named) {
+    auto go = [&](auto kern) {
+        if (named)
+            launch_named("k_tay_term", kern, G.ntwork, s, (const ProbDesc *)G.d_desc, (const WorkItem *)G.d_twork,
+                         G.tnper, e1b, d0b, d1b, j, (const int *)G.d_kmax, (const double *)G.d_trec);
+        else
+            hipLaunchKernelGGL(kern, dim3(G.ntwork), dim3(kBlock), 0, s, (const ProbDesc *)G.d_desc,
+                               (const WorkItem *)G.d_twork, G.tnper, e1b, d0b, d1b, j, (const int *)G.d_kmax,
+                               (const double *)G.d_trec);
+    };
+    auto by_fin = [&](auto k0, auto k1) { fin ? go(k1) : go(k0); };
+    const bool one = G.tg == 1;
+    if (G.tz == 1)
+        one ? by_fin(k_tay_term4<1, 1, 0>, k_tay_term4<1, 1, 1>) : by_fin(k_tay_term4<1, 2, 0>, k_tay_term4<1, 2, 1>);
+    else if (G.tz == 0)
+        one ? by_fin(k_tay_term4<0, 1, 0>, k_tay_term4<0, 1, 1>) : by_fin(k_tay_term4<0, 2, 0>, k_tay_term4<0, 2, 1>);
+    else
+        one ? by_fin(k_tay_term4<2, 1, 0>, k_tay_term4<2, 1, 1>) : by_fin(k_tay_term4<2, 2, 0>, k_tay_term4<2, 2, 1>);
+}
+
 static int term_items(const std::vector<ProbDesc> &desc, std::vector<WorkItem> &w) {
     w.clear();
-    int per = desc.empty() ? 0 : desc[0].t_ntiles + 1;
+    int per = desc.empty() ? 0 : term_cell_items(desc[0]) + 1;
     for (int i = 0; i < (int)desc.size(); i++) {
-        for (int t = 0; t <= desc[i].t_ntiles; t++) w.push_back(WorkItem{i, t});
-        if (desc[i].t_ntiles + 1 != per) per = 0;
+        const int n = term_cell_items(desc[i]);
+        for (int t = 0; t <= n; t++) w.push_back(WorkItem{i, t});
+        if (n + 1 != per) per = 0;
     }
     return per;
 }
@@ -330,7 +371,7 @@ static int term_items(const std::vector<ProbDesc> &desc, std::vector<WorkItem> &
 // the choice changes only the speed.  Choices are remembered per class, so a
 // group whose membership changes times only the classes it has not met.
 struct TermShape {
-    int shift = 0, rpt = 0;
+    int shift = 0, rpt = 0, z = 0;
 };
 using ShapeKey = std::array<int, 3>;
 static std::map<ShapeKey, TermShape> &term_shape_cache() {
@@ -361,17 +402,20 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap,
         if (!cache.count(shape_key(h)) && std::find(todo.begin(), todo.end(), shape_key(h)) == todo.end())
             todo.push_back(shape_key(h));
     if (!todo.empty()) {
-        // every problem active for one step j = 2 (its first: E_1 buffer ->
-        // term buffer); the group's Taylor records are restored after timing
+        // every problem active for step j = 2 of a three-term series: h from
+        // one term buffer, E_1 from the accumulator, the result to the other
+        // term buffer (12 B per node, as the steps of an interval; the term
+        // buffers are free between intervals); the group's Taylor records
+        // are restored after timing
         std::vector<double> saved((size_t)kTrec * n), rec((size_t)kTrec * n, 0.0);
         d2h(saved.data(), G.d_trec, saved.size() * sizeof(double), s);
         for (int i = 0; i < n; i++) {
             rec[(size_t)kTrec * i] = 1e-9;
-            rec[(size_t)kTrec * i + 1] = 2.0;
+            rec[(size_t)kTrec * i + 1] = 3.0;
             rec[(size_t)kTrec * i + 4] = 1e-9 / 2;   // s_2 (unshifted)
             ((float *)&rec[(size_t)kTrec * i + kTrecF])[4] = (float)(1e-9 / 2);
         }
-        const int kt[2] = {2, 0};
+        const int kt[2] = {3, 0};
         h2d(G.d_trec, rec.data(), rec.size() * sizeof(double), s);
         h2d(G.d_kmax, kt, sizeof kt, s);
         hipEvent_t e0, e1;
@@ -384,20 +428,21 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap,
             TermShape best;
             float best_ms = 1e30f;
             for (int shift = 4; shift <= std::min(7, ilog2(k[2])); shift++)
-                for (int rpt : {4, 8}) {
-                    for (int i : mem) set_term_tiling(desc[i], shift, rpt);
+                for (int rpt : {4, 8})
+                for (int z : {0, 1}) {
+                    for (int i : mem) set_term_tiling(desc[i], shift, rpt, z);
                     h2d(G.d_desc, desc.data(), desc.size() * sizeof(ProbDesc), s);
                     std::vector<WorkItem> w;
                     for (int i : mem)
-                        for (int t = 0; t <= desc[i].t_ntiles; t++) w.push_back(WorkItem{i, t});
+                        for (int t = 0; t <= term_cell_items(desc[i]); t++) w.push_back(WorkItem{i, t});
                     upload_term_items(G, w, 0, cap, pool, s);
                     constexpr int kWarm = 2, kTimed = 6;
+                    // the launch covers this class's problems only
+                    G.tz = z;
+                    G.tg = rpt / 4;
                     for (int r = 0; r < kWarm + kTimed; r++) {
                         if (r == kWarm) HIP_OK(hipEventRecord(e0, s));
-                        hipLaunchKernelGGL(k_tay_term4, dim3(G.ntwork), dim3(kBlock), 0, s,
-                                           (const ProbDesc *)G.d_desc, (const WorkItem *)G.d_twork, 0, (int)F_ACC,
-                                           (int)F_D0, (int)F_D1, 2, (const int *)G.d_kmax,
-                                           (const double *)G.d_trec, 0);
+                        launch_term4(G, s, (int)F_ACC, (int)F_D0, (int)F_D1, 2, 0, false);
                     }
                     HIP_OK(hipEventRecord(e1, s));
                     HIP_OK(hipEventSynchronize(e1));
@@ -405,7 +450,7 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap,
                     HIP_OK(hipEventElapsedTime(&ms, e0, e1));
                     if (ms < best_ms) {
                         best_ms = ms;
-                        best = TermShape{shift, rpt};
+                        best = TermShape{shift, rpt, z};
                     }
                 }
             cache[k] = best;
@@ -418,12 +463,14 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap,
     }
     for (ProbDesc &h : desc) {
         const TermShape t = cache.at(shape_key(h));
-        set_term_tiling(h, std::min(t.shift, ilog2(h.nc)), t.rpt);
+        set_term_tiling(h, std::min(t.shift, ilog2(h.nc)), t.rpt, t.z);
     }
     h2d(G.d_desc, desc.data(), desc.size() * sizeof(ProbDesc), s);
     std::vector<WorkItem> w;
     const int per = term_items(desc, w);
     upload_term_items(G, w, per, cap, pool, s);
+    G.tz = term_form(desc);
+    G.tg = term_groups(desc);
 }
 
 
@@ -1623,12 +1670,12 @@ static void taylor_interval(const Group &G, int m, int Kplan, int seq) {
         const int *km = G.d_kmax;
         const double *tr = G.d_trec;
         for (int j = K; j >= 2; j--) {
-            if (f32 && !HET && G.vec4)
-                launch_named("k_tay_term", k_tay_term4, G.ntwork, G.stream, dd, (const WorkItem *)G.d_twork,
-                             G.tnper, accb, db0, db1, j, km, tr, (fused && j == 2) ? 1 : 0);
-            else
+            if (f32 && !HET && G.vec4) {
+                launch_term4(G, G.stream, accb, db0, db1, j, (fused && j == 2) ? 1 : 0, true);
+            } else {
                 launch_named("k_tay_term", k_tay_term<TD, HET>, G.nwork, G.stream, dd, w, G.nper, accb, db0, db1,
                              j, km, tr);
+            }
         }
         if (!fused)
             launch_named("k_tay_final", k_tay_final<TD>, G.nwork, G.stream, dd, w, G.nper, accb, db0, db1, tr);

@@ -96,6 +96,7 @@ struct ProbDesc {
     // a cell does not depend on it): tile width 1 << t_shift, t_rpt rows per
     // thread, t_nti tiles along the rows, t_ntiles tiles, then the package item
     int t_shift, t_rpt, t_nti, t_ntiles;
+    int t_z, pad0;                    // 1: the sweep z-marches (column tiles through every layer)
     double ambient;
     double cell_area;
     // homogeneous layer coefficients
