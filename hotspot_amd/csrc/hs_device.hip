@@ -2168,8 +2168,9 @@ __device__ __forceinline__ float comp(const float4 &v, int k) {
     return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
 }
 
-// Grid sizes are powers of two, so a row holds whole tiles (tile width
-// min(nc, 64)) and every float4 group lies inside its row.
+// The float4 sweeps take problems whose rows hold whole tiles (tile width
+// 1 << t_shift, 16 to 256, dividing nc), so every float4 group lies inside
+// its row.
 __device__ __forceinline__ int term_ntj(const ProbDesc &d) { return d.nc >> d.t_shift; }
 __device__ __forceinline__ int term_layer(const ProbDesc &d, int tile) { return tile / (d.t_nti * term_ntj(d)); }
 
@@ -2311,23 +2312,31 @@ __device__ __forceinline__ void term_vec4(const ProbDesc &d, int tile, const flo
 // The same step z-marching: the workgroup takes a column tile (the term
 // tiling's columns and rows) through every layer from the top, so each
 // thread keeps the layers above and below its cells in registers and reads
-// each plane of h once (a cell costs a third of the load instructions of the
-// per-layer sweep, and the L2 serves a fraction of the bytes).  The rows
-// above and below a thread's own come from its registers but at the thread's
-// first and last row.
+// each plane of h and E_1 once.  The neighbours in the plane come from the
+// other threads: the columns left and right by a shuffle within the tile
+// row, the rows above and below from the thread's own registers or, at its
+// first and last row, from the threads above and below through LDS (one
+// barrier per layer, two LDS buffers).  Only the tile's outer halo is read
+// from memory, by the threads at the tile's edge; a missing neighbour takes
+// the cell's own value, as the clamped loads of the per-layer sweep do.
 template <int GM>
 __device__ __forceinline__ void term_vec4z(const ProbDesc &d, int ztile, const float *hin, float *hout,
                                            const float *e1, float s, float sft, double *yb) {
+    __shared__ float4 edge[2][2][kBlock];   // [layer parity][first row, last row][thread]
     const int ntj = term_ntj(d);
     const int ti = ztile / ntj, tj = ztile - ti * ntj;
     const int sh = d.t_shift;
-    const int tx4 = threadIdx.x & ((1 << (sh - 2)) - 1), ty4 = threadIdx.x >> (sh - 2);
+    const int tw = 1 << (sh - 2);           // threads along a tile row
+    const int tx4 = threadIdx.x & (tw - 1), ty4 = threadIdx.x >> (sh - 2);
+    const int tyn = kBlock >> (sh - 2);     // thread rows
     const int G = d.t_rpt >> 2;
-    const int rows = (kBlock >> sh) * d.t_rpt;
+    const int rows = tyn * G;
     const int nr = d.nr, nc = d.nc, nl = d.nl;
     const unsigned plane = (unsigned)nr * nc;
     const unsigned ncell = plane * (unsigned)nl;
     const int j0 = (tj << sh) + 4 * tx4;
+    const bool top = ty4 == 0, bottom = ty4 == tyn - 1;
+    const bool left = tx4 == 0, right = tx4 == tw - 1;
     int I[GM], ic[GM];
     float4 U[GM], C[GM], Dn[GM];
 #pragma unroll
@@ -2342,29 +2351,45 @@ __device__ __forceinline__ void term_vec4z(const ProbDesc &d, int ztile, const f
 #pragma unroll 1
     for (int n = 0; n < nl; n++) {
         const unsigned off = (unsigned)n * plane;
-        float4 N0, SG, A[GM];
+        const int p = n & 1;
+        const float4 Cf = C[0];
+        float4 Cl = C[0];
+#pragma unroll
+        for (int g = 1; g < GM; g++)
+            if (g == G - 1) Cl = C[g];
+        edge[p][0][threadIdx.x] = Cf;
+        edge[p][1][threadIdx.x] = Cl;
+        float4 N0 = Cf, SG = Cl, A[GM];
         float Wv[GM], Ev[GM];
+        {
+            const unsigned cf = off + (unsigned)ic[0] * nc + j0;
+            const unsigned cl = off + (unsigned)ic[G - 1] * nc + j0;
+            if (top && ic[0] > 0) N0 = ld4(hin, cf - nc);
+            if (bottom && ic[G - 1] < nr - 1) SG = ld4(hin, cl + nc);
+        }
 #pragma unroll
         for (int g = 0; g < GM; g++) {
             if (g >= G) continue;
-            const int i = ic[g];
-            const unsigned c = off + (unsigned)i * nc + j0;
+            const unsigned c = off + (unsigned)ic[g] * nc + j0;
             Dn[g] = n < nl - 1 ? ld4(hin, c + plane) : C[g];
-            if (g == 0) N0 = ld4(hin, i > 0 ? c - nc : c);
-            if (g == G - 1) SG = ld4(hin, i < nr - 1 ? c + nc : c);
-            Wv[g] = ld_at(hin, j0 > 0 ? c - 1 : c);
-            Ev[g] = ld_at(hin, j0 + 4 < nc ? c + 4 : c);
             A[g] = ld4(e1, c);
+            Wv[g] = __shfl_up(C[g].w, 1, tw);
+            Ev[g] = __shfl_down(C[g].x, 1, tw);
+            if (left) Wv[g] = j0 > 0 ? ld_at(hin, c - 1) : C[g].x;
+            if (right) Ev[g] = j0 + 4 < nc ? ld_at(hin, c + 4) : C[g].x;
         }
         float ring[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) ring[q] = ld_at(hin, ncell + max(d.ring_node[n][q], 0));
         const LayerF L = layer_f(d, n);
+        __syncthreads();
+        if (!top) N0 = edge[p][1][threadIdx.x - tw];
+        if (!bottom) SG = edge[p][0][threadIdx.x + tw];
 #pragma unroll
         for (int g = 0; g < GM; g++) {
             if (g >= G || I[g] >= nr) continue;
             // the rows next to row g inside the thread's own rows are its
-            // registers (the clamps of the loads give the same values)
+            // registers
             const float4 N = g == 0 ? N0 : C[g - 1];
             const float4 S = g == G - 1 ? SG : C[g + 1];
             const unsigned c = off + (unsigned)ic[g] * nc + j0;

@@ -427,7 +427,9 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap,
                 if (shape_key(desc[i]) == k) mem.push_back(i);
             TermShape best;
             float best_ms = 1e30f;
-            for (int shift = 4; shift <= std::min(7, ilog2(k[2])); shift++)
+            // tile width at most 256: a tile row's 64 threads stay within one
+            // wavefront (the z-marching sweep shuffles along the row)
+            for (int shift = 4; shift <= std::min(8, ilog2(k[2])); shift++)
                 for (int rpt : {4, 8})
                 for (int z : {0, 1}) {
                     for (int i : mem) set_term_tiling(desc[i], shift, rpt, z);
@@ -471,6 +473,23 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap,
     upload_term_items(G, w, per, cap, pool, s);
     G.tz = term_form(desc);
     G.tg = term_groups(desc);
+}
+
+// Pins the float4 sweep tiling of the model's grid shape (layers, rows,
+// columns) in place of the tuner's choice, for the groups set up after the
+// call; shift 0 drops the pin, and the next setup times the candidates.
+static void pin_term_tiling(const Model &M, int shift, int rpt, int z) {
+    const grid_model_t &g = *M.g;
+    const ShapeKey k{g.n_layers, g.rows, g.cols};
+    if (shift == 0) {
+        term_shape_cache().erase(k);
+        return;
+    }
+    if (shift < 4 || shift > 8 || g.cols % (1 << shift) != 0)
+        fatal("sweep tiling: the tile width 1 << shift must be 16 to 256 and divide the grid's columns\n");
+    if (rpt != 4 && rpt != 8) fatal("sweep tiling: rows per thread must be 4 or 8\n");
+    if (z != 0 && z != 1) fatal("sweep tiling: z must be 0 (per-layer tiles) or 1 (z-marching column tiles)\n");
+    term_shape_cache()[k] = TermShape{shift, rpt, z};
 }
 
 
@@ -2143,6 +2162,14 @@ extern "C" int hs_grid_slope_noalloc(RC_model_t *model, const double *d_block_po
     hs::Model &M = *hs::model_of(model);
     if (!M.g->r_ready || !M.g->c_ready) hs::fatal("grid model not ready\n");
     hs::slope_noalloc(M, d_block_power, d_grid_temp, d_grid_slope);
+    return 0;
+    HS_END(-1)
+}
+
+extern "C" int hs_pin_sweep_tiling(RC_model_t *model, int shift, int rows_per_thread, int z) {
+    HS_BEGIN
+    if (!model || model->type != GRID_MODEL) hs::fatal("hs_pin_sweep_tiling: not a grid model\n");
+    hs::pin_term_tiling(*hs::model_of(model), shift, rows_per_thread, z);
     return 0;
     HS_END(-1)
 }

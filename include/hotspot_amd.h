@@ -465,6 +465,19 @@ void hs_get_options(RC_model_t *model, hs_options_t *opt);
 void hs_set_options(RC_model_t *model, const hs_options_t *opt);
 void hs_get_stats(RC_model_t *model, hs_stats_t *stats);
 
+/* Pins the tiling of the FP32 Taylor sweep for every model of this model's
+ * grid shape (layers x rows x columns) set up after the call, in place of
+ * the tiling the library times on first use of a shape.  No counterpart in
+ * the reference: the tiling changes speed only, never results, and the pin
+ * is for reproducible timings and for tests of every tiling.  shift: tile
+ * width 1 << shift, 4..8, dividing the grid's columns (0 drops the pin);
+ * rows_per_thread 4 or 8; z 0 per-layer tiles, 1 column tiles marched
+ * through the layers.  Problems that do not take float4 sweeps (layers of
+ * several materials, grids narrower than 16 columns) ignore it.  Not to be
+ * called while a solve or an engine tick runs.  Returns 0, or -1 on error
+ * (hs_last_error). */
+int hs_pin_sweep_tiling(RC_model_t *model, int shift, int rows_per_thread, int z);
+
 /* Grid-level state in the reference's flat layout [n_layers*rows*cols + extras]
  * (A3D, temperature_grid.c:2854).  which: 0 = last steady, 1 = last transient. */
 int hs_grid_size(RC_model_t *model);
