@@ -417,6 +417,28 @@ __device__ __forceinline__ bool seq_finished(const ProbDesc &d, int seq) {
 // block <-> grid translation (xlate_vector_b2g :1888-1929, xlate_temp_g2b
 // :1932-2015)
 // ---------------------------------------------------------------------------
+// Power of plane cell cc of layer n for the FP32 Taylor groups, from its
+// cell_uw record t and the layer's block values vb: the power density of
+// each block times its occupancy, times the cell area (blist_avg x area,
+// temperature_grid.c:1908-1910), with the weights of the first two blocks in
+// FP32.  k_b2g and k_tay_first2 both map power with it.
+__device__ __forceinline__ double cell_power_uw(const ProbDesc &d, int n, unsigned cc, const int4 &t,
+                                                const double *vb) {
+    if (t.x >= 0) return (double)__int_as_float(t.z) * vb[t.x] + (double)__int_as_float(t.w) * vb[t.y];
+    const int *ptr = d.map_ptr[n], *unit = d.map_unit[n];
+    const double *occ = d.map_occ[n], *area = d.unit_area[n];
+    double val = 0.0;
+    for (int e = ptr[cc]; e < ptr[cc + 1]; e++) val += (occ[e] * vb[unit[e]]) / area[unit[e]];
+    return val * d.cell_area;
+}
+// The same from the cell's one-block code: a cell inside one block takes the
+// block's weight times its value (what cell_uw gives it, less a zero term).
+__device__ __forceinline__ float cell_power_code(const ProbDesc &d, int n, unsigned cc, unsigned code,
+                                                 const double *vb) {
+    if (code != kCellMixed) return (float)((double)d.unit_w[n][code] * vb[code]);
+    return (float)cell_power_uw(d, n, cc, d.cell_uw[n][cc], vb);
+}
+
 // out = sum over the cell's blocks of occ * v_u (/ area_u * cell area for
 // power); with add_sources the ambient / inlet constants are added (rhs).
 // from_seq: 1 reads the block vector of the current sequence step, 2 the
@@ -473,15 +495,7 @@ __global__ void __launch_bounds__(kBlock) k_b2g(const ProbDesc *__restrict__ D,
                 d, wi.tile,
                 [&](const Cell &q) { return uw[(unsigned)q.i * d.nc + q.j]; },
                 [&](const Cell &q, const int4 &t) {
-                    double val = 0.0;
-                    if (t.x >= 0) {
-                        val = (double)__int_as_float(t.z) * vb[t.x] + (double)__int_as_float(t.w) * vb[t.y];
-                    } else {
-                        const unsigned cc = (unsigned)q.i * d.nc + q.j;
-                        for (int e = ptr[cc]; e < ptr[cc + 1]; e++) val += (occ[e] * vb[unit[e]]) / area[unit[e]];
-                        val *= d.cell_area;
-                    }
-                    st_at(out32, q.c, (float)val);
+                    st_at(out32, q.c, (float)cell_power_uw(d, n, (unsigned)q.i * d.nc + q.j, t, vb));
                 });
             return;
         }
@@ -1992,10 +2006,13 @@ __device__ __forceinline__ hs_d2 ld2(const double *base, unsigned i) {
 }
 
 //
-// With power32 the right-hand side is the FP32 cell power of F_PW (none for
-// a layer marked in lzero) plus the ambient sources, formed here: P + gamb
-// T_ambient on cells, gamb T_ambient on the package nodes, as k_b2g adds
-// them otherwise.
+// With power32 the right-hand side is the FP32 cell power plus the ambient
+// sources, formed here: P + gamb T_ambient on cells, gamb T_ambient on the
+// package nodes, as k_b2g adds them otherwise.  power32 1: the cell power of
+// F_PW (none for a layer marked in lzero); 2 (sequence mode): mapped here
+// from the sequence's block powers of the current interval through the
+// cells' one-block codes, a layer whose blocks all carry zero power taking
+// none.
 constexpr int kF2Rows = 1;   // rows of a thread whose loads are in flight together
 __global__ void __launch_bounds__(kBlock) k_tay_first2(const ProbDesc *__restrict__ D,
                                                        const WorkItem *__restrict__ W, int e1b, int sub, int seq,
@@ -2027,7 +2044,15 @@ __global__ void __launch_bounds__(kBlock) k_tay_first2(const ProbDesc *__restric
         const int *rn = d.ring_node[n];
         const double rgn = d.ring_gns[n], rge = d.ring_gew[n];
         const double src = gam * d.ambient;
-        const bool pz = power32 && d.lzero[n];
+        bool pz = power32 == 1 && d.lzero[n];
+        const double *vb = nullptr;
+        if (power32 == 2) {
+            vb = d.seq_in + (d.seq_one ? 0L : (long)d.scal[S_STEP] * (d.nblocks + d.nx)) + d.layer_base[n];
+            const int nu = (n + 1 < nl ? d.layer_base[n + 1] : d.nblocks) - d.layer_base[n];
+            int nz = 0;
+            for (int u = threadIdx.x; u < nu; u += kBlock) nz |= vb[u] != 0.0;
+            pz = !__syncthreads_or(nz);
+        }
         double ring[4];
         for (int e = 0; e < 4; e++) ring[e] = ld_at(y, ncell + max(rn[e], 0));
         for (int g0 = 0; g0 < G; g0 += kF2Rows) {
@@ -2052,6 +2077,12 @@ __global__ void __launch_bounds__(kBlock) k_tay_first2(const ProbDesc *__restric
                     B[g] = ld2(rhs, c);
                 } else if (pz) {
                     B[g] = hs_d2{0.0 + src, 0.0 + src};
+                } else if (power32 == 2) {
+                    const unsigned pc = (unsigned)i * nc + j0;
+                    const unsigned cd = *(const HS_GLOBAL unsigned *)((const HS_GLOBAL char *)d.cell_u16[n] + pc * 2u);
+                    const float p0 = cell_power_code(d, n, pc, cd & 0xffffu, vb);
+                    const float p1 = cell_power_code(d, n, pc + 1, cd >> 16, vb);
+                    B[g] = hs_d2{(double)p0 + src, (double)p1 + src};
                 } else {
                     const hs_f2 p = *(const HS_GLOBAL hs_f2 *)((const HS_GLOBAL char *)pw + c * 4u);
                     B[g] = hs_d2{(double)p.x + src, (double)p.y + src};

@@ -125,6 +125,7 @@ struct Group {
     int ntwork = 0, tnper = 0;
     int tz = 0;   // its tiles: 0 per layer for every problem, 1 z-marching for every problem, 2 mixed
     int tg = 2;   // most row groups of 4 rows per thread among its problems (t_rpt / 4)
+    bool pcodes = false;   // every problem has cell_u16 on every layer (k_tay_first2 maps the power)
 };
 
 // A homogeneous problem whose tiles are at least 16 columns wide with at
@@ -132,6 +133,11 @@ struct Group {
 // (rows hold whole tiles: grid sizes are powers of two).
 static bool vec4_ok(const ProbDesc &h) {
     return !h.hetero && h.t_rpt >= 4 && h.t_shift >= 4 && h.nc % (1 << h.t_shift) == 0;
+}
+static bool pcodes_ok(const ProbDesc &h) {
+    for (int n = 0; n < h.nl; n++)
+        if (!h.cell_u16[n]) return false;
+    return true;
 }
 
 // One multigrid level of one problem: its descriptor (the finest level is
@@ -884,6 +890,8 @@ static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr
     std::vector<int4> rng;
     std::vector<int> blayer;
     std::vector<const int4 *> dcu(M.maps.size(), nullptr);
+    std::vector<const unsigned short *> dcode(M.maps.size(), nullptr);
+    std::vector<const float *> dw(M.maps.size(), nullptr);
     for (int n = 0; n < g.n_layers; n++) {
         const int mi = M.layer_map[n];
         h.map_ptr[n] = dptr[mi];
@@ -917,8 +925,23 @@ static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr
                 uw[c] = make_int4(u0, u1, bits(w0), bits(w1));
             }
             dcu[mi] = D->upload(uw.data(), uw.size());
+            if (f->n_units < (int)kCellMixed) {
+                std::vector<unsigned short> code(plane, kCellMixed);
+                for (size_t c = 0; c < plane; c++) {
+                    const int e = bm.cell_ptr[c];
+                    if (bm.cell_ptr[c + 1] - e == 1 && bm.ent_occ[e] == 1.0)
+                        code[c] = (unsigned short)bm.ent_unit[e];
+                }
+                // the weight of a cell inside one block, as cell_uw has it
+                std::vector<float> w(f->n_units);
+                for (int u = 0; u < f->n_units; u++) w[u] = (float)(1.0 * h.cell_area / area[u]);
+                dcode[mi] = D->upload(code.data(), code.size());
+                dw[mi] = D->upload(w.data(), w.size());
+            }
         }
         h.cell_uw[n] = dcu[mi];
+        h.cell_u16[n] = dcode[mi];
+        h.unit_w[n] = dw[mi];
     }
     h.nblocks = g.total_n_blocks;
     h.blk_range = D->upload(rng.data(), rng.size());
@@ -954,6 +977,7 @@ static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr
     D->grp.stream = D->stream;
     D->grp.hetero = M.detailed;
     D->grp.vec4 = vec4_ok(h);
+    D->grp.pcodes = pcodes_ok(h);
     D->grp.d_kmax = D->alloc<int>(2);
     dzero(D->grp.d_kmax, 2 * sizeof(int), D->stream);
     D->grp.d_trec = D->alloc<double>(kTrec);
@@ -1673,7 +1697,8 @@ static void taylor_interval(const Group &G, int m, int Kplan, int seq) {
     const int db0 = f32 ? F_D0 : B_KA, db1 = f32 ? F_D1 : B_KB, accb = f32 ? F_ACC : B_ACC;
     for (int s = 0; s < m; s++) {
         if (f32 && !HET && G.vec4)
-            launch_named("k_tay_first", k_tay_first2, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax, 1);
+            launch_named("k_tay_first", k_tay_first2, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax,
+                         seq && G.pcodes ? 2 : 1);
         else
             launch_named("k_tay_first", k_tay_first<TD, HET>, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax);
         // float4 sweeps finish the sub-step in their last launch (j = 2)
@@ -1888,8 +1913,13 @@ static void rk4_seq_body(const Group &G, int slots) {
 
 template <typename TD, bool HET>
 static void taylor_seq_body(const Group &G, int m, int K) {
-    if (power32(G, sizeof(TD) == 4, HET)) run_b2g_power32(G, 1);
-    else run_b2g(G, V_POWER, B_RHS, 1, 1);
+    // FP32 groups whose cells have one-block codes map the power in
+    // k_tay_first2 itself
+    if (power32(G, sizeof(TD) == 4, HET)) {
+        if (!G.pcodes) run_b2g_power32(G, 1);
+    } else {
+        run_b2g(G, V_POWER, B_RHS, 1, 1);
+    }
     taylor_interval<TD, HET>(G, m, K, 1);
     run_g2b(G, -1, 1);
     hipLaunchKernelGGL(k_step_advance, dim3(small_grid(G.nprob)), dim3(64), 0, G.stream,

@@ -26,6 +26,7 @@ constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kRed = 4;   // values reduced per slot
 constexpr int kGmM = 32;  // GMRES restart length (non-symmetric steady solves)
+constexpr unsigned short kCellMixed = 0xffff;   // cell_u16: not inside one block
 
 // Per-problem Taylor record of a group (doubles): [0] the sub-step tau and
 // [1] its term count (0: nothing to do), written by the finish of the first
@@ -135,6 +136,12 @@ struct ProbDesc {
     // area / block area as {u0, u1, w0, w1} (weights bit-cast; u1 = u0 and
     // w1 = 0 for a cell inside one block, u0 = -1 for a cell of 3+ blocks)
     const int4 *cell_uw[kMaxLayers];
+    // per cell of a layer: the one block that covers it whole, or kCellMixed
+    // (the cell takes cell_uw), and per block of the layer its weight
+    // (float)(cell area / block area); null where a layer's floorplan has
+    // kCellMixed blocks or more
+    const unsigned short *cell_u16[kMaxLayers];
+    const float *unit_w[kMaxLayers];
     int layer_base[kMaxLayers];         // block-vector offset per layer
     const int4 *blk_range;              // per block of the vector: i1,i2,j1,j2
     const int *blk_layer;
