@@ -282,7 +282,6 @@ static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans) {
     B.g.stream = B.stream;
     B.g.hetero = het;
     B.g.vec4 = v4;
-    B.g.pcodes = std::all_of(desc.begin(), desc.end(), pcodes_ok);
     B.g.nper = (n > 0 && (int)w.size() == n * (desc[0].ntiles + 1)) ? desc[0].ntiles + 1 : 0;
     for (int i = 0; i < n && B.g.nper; i++)
         if (desc[i].ntiles != desc[0].ntiles) B.g.nper = 0;
@@ -297,6 +296,7 @@ static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans) {
         upload_term_items(B.g, tw, per, B.cap_twork, B.pool, s);
         B.g.tz = term_form(desc);
         B.g.tg = term_groups(desc);
+        B.g.zfirst = false;
     }
     // multigrid levels of a steady group (every member has the same count)
     for (void *p : B.lv_bufs) dfree(B.pool, p);

@@ -418,26 +418,25 @@ __device__ __forceinline__ bool seq_finished(const ProbDesc &d, int seq) {
 // :1932-2015)
 // ---------------------------------------------------------------------------
 // Power of plane cell cc of layer n for the FP32 Taylor groups, from its
-// cell_uw record t and the layer's block values vb: the power density of
-// each block times its occupancy, times the cell area (blist_avg x area,
-// temperature_grid.c:1908-1910), with the weights of the first two blocks in
-// FP32.  k_b2g and k_tay_first2 both map power with it.
-__device__ __forceinline__ double cell_power_uw(const ProbDesc &d, int n, unsigned cc, const int4 &t,
-                                                const double *vb) {
-    if (t.x >= 0) return (double)__int_as_float(t.z) * vb[t.x] + (double)__int_as_float(t.w) * vb[t.y];
+// cell_uw record (units u, weights w) and the layer's block values vb: the
+// power density of each block times its occupancy, times the cell area
+// (blist_avg x area, temperature_grid.c:1908-1910), with the weights of up to
+// four blocks in FP32.  k_b2g and k_tay_first4z both map power with it.
+__device__ __forceinline__ double four_block_power(const int4 &w, double v0, double v1, double v2, double v3) {
+#pragma clang fp contract(on)
+    return (double)__int_as_float(w.x) * v0 + (double)__int_as_float(w.y) * v1 +
+           (double)__int_as_float(w.z) * v2 + (double)__int_as_float(w.w) * v3;
+}
+__device__ __forceinline__ double cell_power_uw(const ProbDesc &d, int n, unsigned cc, const int4 &u,
+                                                const int4 &w, const double *vb) {
+    if (u.x >= 0) return four_block_power(w, vb[u.x], vb[u.y], vb[u.z], vb[u.w]);
     const int *ptr = d.map_ptr[n], *unit = d.map_unit[n];
     const double *occ = d.map_occ[n], *area = d.unit_area[n];
     double val = 0.0;
     for (int e = ptr[cc]; e < ptr[cc + 1]; e++) val += (occ[e] * vb[unit[e]]) / area[unit[e]];
     return val * d.cell_area;
 }
-// The same from the cell's one-block code: a cell inside one block takes the
-// block's weight times its value (what cell_uw gives it, less a zero term).
-__device__ __forceinline__ float cell_power_code(const ProbDesc &d, int n, unsigned cc, unsigned code,
-                                                 const double *vb) {
-    if (code != kCellMixed) return (float)((double)d.unit_w[n][code] * vb[code]);
-    return (float)cell_power_uw(d, n, cc, d.cell_uw[n][cc], vb);
-}
+constexpr int kPowTab = 1024;   // blocks of a layer whose powers k_b2g tabulates in LDS
 
 // out = sum over the cell's blocks of occ * v_u (/ area_u * cell area for
 // power); with add_sources the ambient / inlet constants are added (rhs).
@@ -449,20 +448,30 @@ __device__ __forceinline__ float cell_power_code(const ProbDesc &d, int n, unsig
 // out_buf; the tiles of a layer whose blocks all carry zero power write
 // nothing and mark the layer in lzero instead (every workgroup of the layer
 // writes the same mark, which the next kernel reads).
+__device__ __forceinline__ const double *b2g_source(const ProbDesc &d, int from_seq) {
+    return from_seq == 1 ? d.seq_in + (d.seq_one ? 0L : (long)d.scal[S_STEP] * (d.nblocks + d.nx))
+           : from_seq == 2 ? d.blk_out : d.blk_in;
+}
+// whole workgroup: whether some block of layer n carries power in vb
+__device__ __forceinline__ bool layer_has_power(const ProbDesc &d, int n, const double *vb) {
+    __shared__ int nonzero;
+    const int nu = (n + 1 < d.nl ? d.layer_base[n + 1] : d.nblocks) - d.layer_base[n];
+    if (threadIdx.x == 0) nonzero = 0;
+    __syncthreads();
+    for (int u = threadIdx.x; u < nu; u += kBlock)
+        if (vb[u] != 0.0) nonzero = 1;
+    __syncthreads();
+    return nonzero != 0;
+}
 __global__ void __launch_bounds__(kBlock) k_b2g(const ProbDesc *__restrict__ D,
                                                 const WorkItem *__restrict__ W, int type,
-                                                int out_buf, int add_sources, int from_seq, int power32) {
+                                                int out_buf, int add_sources, int from_seq) {
     const WorkItem wi = W[work_index()];
     const ProbDesc &d = D[wi.prob];
     if (seq_finished(d, from_seq == 1)) return;
-    double *out = power32 ? nullptr : d.b64[out_buf];
-    float *out32 = power32 ? d.b32[out_buf] : nullptr;
-    auto put = [&](unsigned c, double v) {
-        if (power32) st_at(out32, c, (float)v);
-        else st_at(out, c, v);
-    };
-    const double *b = from_seq == 1 ? d.seq_in + (d.seq_one ? 0L : (long)d.scal[S_STEP] * (d.nblocks + d.nx))
-                      : from_seq == 2 ? d.blk_out : d.blk_in;
+    double *out = d.b64[out_buf];
+    auto put = [&](unsigned c, double v) { st_at(out, c, v); };
+    const double *b = b2g_source(d, from_seq);
     const long ncell = (long)d.nl * d.nr * d.nc;
     if (wi.tile < d.ntiles) {
         const int n = tile_layer(d, wi.tile);
@@ -474,31 +483,7 @@ __global__ void __launch_bounds__(kBlock) k_b2g(const ProbDesc *__restrict__ D,
         const double src = d.gamb[n] * d.ambient;
         // A layer whose blocks all carry zero power (bond, spreader and sink
         // layers of a chip stack) maps to zero without reading its map.
-        __shared__ int nonzero;
-        bool mapped = true;
-        if (type == V_POWER) {
-            const int nu = (n + 1 < d.nl ? d.layer_base[n + 1] : d.nblocks) - d.layer_base[n];
-            if (threadIdx.x == 0) nonzero = 0;
-            __syncthreads();
-            for (int u = threadIdx.x; u < nu; u += kBlock)
-                if (vb[u] != 0.0) nonzero = 1;
-            __syncthreads();
-            mapped = nonzero != 0;
-        }
-        if (power32 && threadIdx.x == 0) d.lzero[n] = mapped ? 0 : 1;
-        if (power32) {
-            if (!mapped) return;
-            // a cell of one or two blocks: their weights times their power;
-            // a cell of three or more: its map entries
-            const int4 *uw = d.cell_uw[n];
-            tile_loop<8>(
-                d, wi.tile,
-                [&](const Cell &q) { return uw[(unsigned)q.i * d.nc + q.j]; },
-                [&](const Cell &q, const int4 &t) {
-                    st_at(out32, q.c, (float)cell_power_uw(d, n, (unsigned)q.i * d.nc + q.j, t, vb));
-                });
-            return;
-        }
+        const bool mapped = type != V_POWER || layer_has_power(d, n, vb);
         if (!mapped) {
             tile_loop<8>(d, wi.tile, [&](const Cell &) { return 0; }, [&](const Cell &q, int) {
                 double val = 0.0;
@@ -554,12 +539,76 @@ __global__ void __launch_bounds__(kBlock) k_b2g(const ProbDesc *__restrict__ D,
             });
     } else {
         const int k = threadIdx.x;
-        if (k < d.nx && !power32) {
+        if (k < d.nx) {
             double v = (type == V_POWER) ? 0.0 : b[d.nblocks + k];
             if (add_sources) v += d.x_gamb[k] * d.ambient;
             out[ncell + k] = v;
         }
     }
+}
+
+// The cell power alone in FP32 for the FP32 Taylor groups (F_PW): the tiles
+// of a layer whose blocks all carry zero power write nothing and mark the
+// layer in lzero instead (every workgroup of the layer writes the same mark,
+// which the next kernel reads); the package nodes take none.  A layer with
+// one-block codes reads a cell's code and, at a block boundary, its record,
+// with the layer's block powers and the power of a cell inside each block
+// tabulated in LDS; otherwise every cell reads its record.
+__global__ void __launch_bounds__(kBlock) k_b2g_power32(const ProbDesc *__restrict__ D,
+                                                        const WorkItem *__restrict__ W, int from_seq) {
+    __shared__ float cpt[kPowTab];
+    __shared__ double bpt[kPowTab];
+    const WorkItem wi = W[work_index()];
+    const ProbDesc &d = D[wi.prob];
+    if (seq_finished(d, from_seq == 1) || wi.tile >= d.ntiles) return;
+    const int n = tile_layer(d, wi.tile);
+    const double *vb = b2g_source(d, from_seq) + d.layer_base[n];
+    const bool mapped = layer_has_power(d, n, vb);
+    if (threadIdx.x == 0) d.lzero[n] = mapped ? 0 : 1;
+    if (!mapped) return;
+    float *out32 = d.b32[F_PW];
+    // a cell of up to four blocks: their weights times their power; a cell
+    // of five or more: its map entries
+    const int4 *uw = d.cell_uw[n];
+    const int nu = (n + 1 < d.nl ? d.layer_base[n + 1] : d.nblocks) - d.layer_base[n];
+    if (d.cell_u16[n] && nu <= kPowTab) {
+        for (int u = threadIdx.x; u < nu; u += kBlock) {
+            cpt[u] = (float)((double)d.unit_w[n][u] * vb[u]);
+            bpt[u] = vb[u];
+        }
+        __syncthreads();
+        // the cells inside one block, then the tile's boundary cells from
+        // its list
+        const unsigned short *code = d.cell_u16[n];
+        tile_loop<8>(
+            d, wi.tile, [&](const Cell &q) { return (unsigned)code[(unsigned)q.i * d.nc + q.j]; },
+            [&](const Cell &q, unsigned k) {
+                if (k != kCellMixed) st_at(out32, q.c, cpt[k]);
+            });
+        const int rem = wi.tile - n * d.nti * d.ntj;
+        const int e0 = d.mix_off[n][rem], e1 = d.mix_off[n][rem + 1];
+        const unsigned lo = (unsigned)n * d.nr * d.nc;
+        for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlock) {
+            const unsigned cc = (unsigned)d.mix_cell[n][e];
+            const int4 u = d.mix_uw[n][2 * e], w = d.mix_uw[n][2 * e + 1];
+            const float p = u.x >= 0 ? (float)four_block_power(w, bpt[u.x], bpt[u.y], bpt[u.z], bpt[u.w])
+                                     : (float)cell_power_uw(d, n, cc, u, w, vb);
+            st_at(out32, lo + cc, p);
+        }
+        return;
+    }
+    struct Rec {
+        int4 u, w;
+    };
+    tile_loop<4>(
+        d, wi.tile,
+        [&](const Cell &q) {
+            const unsigned cc = (unsigned)q.i * d.nc + q.j;
+            return Rec{uw[2 * cc], uw[2 * cc + 1]};
+        },
+        [&](const Cell &q, const Rec &r) {
+            st_at(out32, q.c, (float)cell_power_uw(d, n, (unsigned)q.i * d.nc + q.j, r.u, r.w, vb));
+        });
 }
 
 // One wavefront per block of the vector; work items: (problem, first block).
@@ -1016,6 +1065,7 @@ struct FinArgs {
     double *trec;  // FIN_TAY: per problem {tau, terms (0: no sub-step)}
     int kg;        // FIN_TAY: the group's plan (launches of the sub-step)
     int fused;     // FIN_TAY: the last step adds its result into the state (k_tay_term4)
+    int zslots;    // FIN_TAY: the first term z-marched (k_tay_first4z): t_ntiles / nl + 1 slots
 };
 
 // ---------------------------------------------------------------------------
@@ -1824,7 +1874,7 @@ __global__ void __launch_bounds__(kBlock) k_finish(const ProbDesc *__restrict__ 
                 return;
             }
             double w[1];
-            reduce_slots<1, true>(d, w);
+            reduce_slots<1, true>(d, w, a.zslots ? d.t_ntiles / d.nl + 1 : -1);
             if (threadIdx.x) return;
             const double z = s[S_TSIG] > 0.0 ? s[S_TSIG] : s[S_RHO] * s[S_TAU];
             const int kplan = (int)s[S_KPLAN];
@@ -1995,25 +2045,83 @@ __global__ void __launch_bounds__(kBlock) k_tay_first(const ProbDesc *__restrict
     write_partials<1>(d, wi.tile, v);
 }
 
-// E_1 of a homogeneous problem for the FP32 series with double2 accesses:
-// within the standard tile, each thread owns 2 consecutive columns x rpt/2
-// rows, one row at a time (kF2Rows).  The arithmetic per cell is that of stencil()
-// and k_tay_first in the same order.
 typedef double hs_d2 __attribute__((ext_vector_type(2)));
 typedef float hs_f2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ hs_d2 ld2(const double *base, unsigned i) {
     return *(const HS_GLOBAL hs_d2 *)((const HS_GLOBAL char *)base + i * 8u);
 }
 
+// Coefficients of one layer for the first term of the FP32 series.
+struct FirstLayer {
+    double gx, gy, gu, gd, ci, gam, rgn, rge, src;
+    const int *rn;
+};
+__device__ __forceinline__ FirstLayer first_layer(const ProbDesc &d, int n) {
+    FirstLayer L;
+    L.gx = d.gx[n];
+    L.gy = d.gy[n];
+    L.gu = n > 0 ? d.gz[max(n - 1, 0)] : 0.0;
+    L.gd = n < d.nl - 1 ? d.gz[n] : 0.0;
+    L.ci = d.cinv[n];
+    L.gam = d.gamb[n];
+    L.rgn = d.ring_gns[n];
+    L.rge = d.ring_gew[n];
+    L.src = L.gam * d.ambient;
+    L.rn = d.ring_node[n];
+    return L;
+}
+
+// E_1 = tau C^-1 (b + flux - g_amb x) of cell (i, j) in FP64, rounded to
+// FP32, from the cell (xc), its neighbours in the plane (n, s, w, e) and the
+// layers (u, d), and b = P + gamb T_ambient: the arithmetic of stencil() and
+// k_tay_first in the same order.  A missing neighbour has zero conductance.
+// Both first-term kernels compute a cell with this one function, so E_1
+// does not depend on the kernel or its tiling.
+__device__ __forceinline__ float first_cell(const ProbDesc &d, const FirstLayer &L, int i, int j, double xc,
+                                            double xn, double xs, double xw, double xe, double xu, double xd,
+                                            double b, double tau, const double (&ring)[4]) {
+#pragma clang fp contract(on)
+    const int nr = d.nr, nc = d.nc;
+    const double gn = i > 0 ? L.gx : 0.0, gs = i < nr - 1 ? L.gx : 0.0;
+    const double ge = j < nc - 1 ? L.gy : 0.0, gw = j > 0 ? L.gy : 0.0;
+    double flux = gn * (xn - xc);
+    flux += gs * (xs - xc);
+    flux += ge * (xe - xc);
+    flux += gw * (xw - xc);
+    flux += L.gu * (xu - xc);
+    flux += L.gd * (xd - xc);
+    if (i == 0 && L.rn[0] >= 0) flux += L.rgn * (ring[0] - xc);
+    if (i == nr - 1 && L.rn[1] >= 0) flux += L.rgn * (ring[1] - xc);
+    if (j == nc - 1 && L.rn[2] >= 0) flux += L.rge * (ring[2] - xc);
+    if (j == 0 && L.rn[3] >= 0) flux += L.rge * (ring[3] - xc);
+    return (float)(tau * (L.ci * (b + flux - L.gam * xc)));
+}
+
+// The package nodes' E_1 (the package item of both first-term kernels).
+__device__ __forceinline__ void first_package(const ProbDesc &d, const double *y, const double *rhs, float *e1,
+                                              double tau, int power32, double (&v)[1]) {
+    __shared__ double xf[kMaxExtra], xd[kMaxExtra];
+    LdPlain<double> ld{y};
+    package_terms<double>(d, ld, true, xf, xd);
+    const int k = threadIdx.x;
+    if (k < d.nx) {
+        const long c = (long)d.nl * d.nr * d.nc + k;
+        const double b = power32 ? 0.0 + d.x_gamb[k] * d.ambient : rhs[c];
+        const float e = (float)(tau * (d.x_cinv[k] * (b + xf[k] - d.x_gamb[k] * y[c])));
+        e1[c] = e;
+        v[0] = fmax(v[0], fabs((double)e));
+    }
+}
+
+
+// E_1 of a homogeneous problem for the FP32 series with double2 accesses:
+// within the standard tile, each thread owns 2 consecutive columns x rpt/2
+// rows, one row at a time.
 //
-// With power32 the right-hand side is the FP32 cell power plus the ambient
-// sources, formed here: P + gamb T_ambient on cells, gamb T_ambient on the
-// package nodes, as k_b2g adds them otherwise.  power32 1: the cell power of
-// F_PW (none for a layer marked in lzero); 2 (sequence mode): mapped here
-// from the sequence's block powers of the current interval through the
-// cells' one-block codes, a layer whose blocks all carry zero power taking
-// none.
-constexpr int kF2Rows = 1;   // rows of a thread whose loads are in flight together
+// With power32 the right-hand side is the FP32 cell power of F_PW (none for
+// a layer marked in lzero) plus the ambient sources, formed here: P + gamb
+// T_ambient on cells, gamb T_ambient on the package nodes, as k_b2g adds
+// them otherwise.
 __global__ void __launch_bounds__(kBlock) k_tay_first2(const ProbDesc *__restrict__ D,
                                                        const WorkItem *__restrict__ W, int e1b, int sub, int seq,
                                                        int *__restrict__ kmax, int power32) {
@@ -2038,98 +2146,44 @@ __global__ void __launch_bounds__(kBlock) k_tay_first2(const ProbDesc *__restric
         const int nr = d.nr, nc = d.nc, nl = d.nl;
         const unsigned plane = (unsigned)nr * nc, ncell = plane * nl;
         const int j0 = (tj << sh) + 2 * tx2;
-        const double gx = d.gx[n], gy = d.gy[n];
-        const double gu = n > 0 ? d.gz[max(n - 1, 0)] : 0.0, gd = n < nl - 1 ? d.gz[n] : 0.0;
-        const double ci = d.cinv[n], gam = d.gamb[n];
-        const int *rn = d.ring_node[n];
-        const double rgn = d.ring_gns[n], rge = d.ring_gew[n];
-        const double src = gam * d.ambient;
-        bool pz = power32 == 1 && d.lzero[n];
-        const double *vb = nullptr;
-        if (power32 == 2) {
-            vb = d.seq_in + (d.seq_one ? 0L : (long)d.scal[S_STEP] * (d.nblocks + d.nx)) + d.layer_base[n];
-            const int nu = (n + 1 < nl ? d.layer_base[n + 1] : d.nblocks) - d.layer_base[n];
-            int nz = 0;
-            for (int u = threadIdx.x; u < nu; u += kBlock) nz |= vb[u] != 0.0;
-            pz = !__syncthreads_or(nz);
-        }
+        const FirstLayer L = first_layer(d, n);
+        const bool pz = power32 && d.lzero[n];
         double ring[4];
-        for (int e = 0; e < 4; e++) ring[e] = ld_at(y, ncell + max(rn[e], 0));
-        for (int g0 = 0; g0 < G; g0 += kF2Rows) {
-            hs_d2 C[kF2Rows], N[kF2Rows], S[kF2Rows], U[kF2Rows], Dn[kF2Rows], B[kF2Rows];
-            double Wv[kF2Rows], Ev[kF2Rows];
-            int I[kF2Rows];
-            unsigned cc[kF2Rows];
-#pragma unroll
-            for (int g = 0; g < kF2Rows; g++) {
-                I[g] = ti * rows + ty2 * G + g0 + g;
-                const int i = min(I[g], nr - 1);
-                const unsigned c = ((unsigned)n * nr + i) * nc + j0;
-                cc[g] = c;
-                C[g] = ld2(y, c);
-                N[g] = ld2(y, i > 0 ? c - nc : c);
-                S[g] = ld2(y, i < nr - 1 ? c + nc : c);
-                U[g] = ld2(y, n > 0 ? c - plane : c);
-                Dn[g] = ld2(y, n < nl - 1 ? c + plane : c);
-                Wv[g] = ld_at(y, j0 > 0 ? c - 1 : c);
-                Ev[g] = ld_at(y, j0 + 2 < nc ? c + 2 : c);
-                if (!power32) {
-                    B[g] = ld2(rhs, c);
-                } else if (pz) {
-                    B[g] = hs_d2{0.0 + src, 0.0 + src};
-                } else if (power32 == 2) {
-                    const unsigned pc = (unsigned)i * nc + j0;
-                    const unsigned cd = *(const HS_GLOBAL unsigned *)((const HS_GLOBAL char *)d.cell_u16[n] + pc * 2u);
-                    const float p0 = cell_power_code(d, n, pc, cd & 0xffffu, vb);
-                    const float p1 = cell_power_code(d, n, pc + 1, cd >> 16, vb);
-                    B[g] = hs_d2{(double)p0 + src, (double)p1 + src};
-                } else {
-                    const hs_f2 p = *(const HS_GLOBAL hs_f2 *)((const HS_GLOBAL char *)pw + c * 4u);
-                    B[g] = hs_d2{(double)p.x + src, (double)p.y + src};
-                }
+        for (int e = 0; e < 4; e++) ring[e] = ld_at(y, ncell + max(L.rn[e], 0));
+        for (int g = 0; g < G; g++) {
+            const int I = ti * rows + ty2 * G + g;
+            const int i = min(I, nr - 1);
+            const unsigned c = ((unsigned)n * nr + i) * nc + j0;
+            const hs_d2 C = ld2(y, c);
+            const hs_d2 N = ld2(y, i > 0 ? c - nc : c);
+            const hs_d2 S = ld2(y, i < nr - 1 ? c + nc : c);
+            const hs_d2 U = ld2(y, n > 0 ? c - plane : c);
+            const hs_d2 Dn = ld2(y, n < nl - 1 ? c + plane : c);
+            const double Wv = ld_at(y, j0 > 0 ? c - 1 : c);
+            const double Ev = ld_at(y, j0 + 2 < nc ? c + 2 : c);
+            hs_d2 B;
+            if (!power32) {
+                B = ld2(rhs, c);
+            } else if (pz) {
+                B = hs_d2{0.0 + L.src, 0.0 + L.src};
+            } else {
+                const hs_f2 p = *(const HS_GLOBAL hs_f2 *)((const HS_GLOBAL char *)pw + c * 4u);
+                B = hs_d2{(double)p.x + L.src, (double)p.y + L.src};
             }
+            if (I >= nr) continue;
+            float e[2];
 #pragma unroll
-            for (int g = 0; g < kF2Rows; g++) {
-                if (g0 + g >= G || I[g] >= nr) continue;
-                const int i = I[g];
-                const double gn = i > 0 ? gx : 0.0, gs = i < nr - 1 ? gx : 0.0;
-                float e[2];
-#pragma unroll
-                for (int k = 0; k < 2; k++) {
-                    const int j = j0 + k;
-                    const double xc = C[g][k];
-                    const double xw = k == 0 ? Wv[g] : C[g][0];
-                    const double xe = k == 1 ? Ev[g] : C[g][1];
-                    const double ge = j < nc - 1 ? gy : 0.0, gw = j > 0 ? gy : 0.0;
-                    double flux = gn * (N[g][k] - xc);
-                    flux += gs * (S[g][k] - xc);
-                    flux += ge * (xe - xc);
-                    flux += gw * (xw - xc);
-                    flux += gu * (U[g][k] - xc);
-                    flux += gd * (Dn[g][k] - xc);
-                    if (i == 0 && rn[0] >= 0) flux += rgn * (ring[0] - xc);
-                    if (i == nr - 1 && rn[1] >= 0) flux += rgn * (ring[1] - xc);
-                    if (j == nc - 1 && rn[2] >= 0) flux += rge * (ring[2] - xc);
-                    if (j == 0 && rn[3] >= 0) flux += rge * (ring[3] - xc);
-                    e[k] = (float)(tau * (ci * (B[g][k] + flux - gam * xc)));
-                    v[0] = fmax(v[0], fabs((double)e[k]));
-                }
-                const hs_f2 ev = {e[0], e[1]};
-                *(HS_GLOBAL hs_f2 *)((HS_GLOBAL char *)e1 + cc[g] * 4u) = ev;
+            for (int k = 0; k < 2; k++) {
+                const double xw = k == 0 ? Wv : C[0];
+                const double xe = k == 1 ? Ev : C[1];
+                e[k] = first_cell(d, L, i, j0 + k, C[k], N[k], S[k], xw, xe, U[k], Dn[k], B[k], tau, ring);
+                v[0] = fmax(v[0], fabs((double)e[k]));
             }
+            const hs_f2 ev = {e[0], e[1]};
+            *(HS_GLOBAL hs_f2 *)((HS_GLOBAL char *)e1 + c * 4u) = ev;
         }
     } else {
-        __shared__ double xf[kMaxExtra], xd[kMaxExtra];
-        LdPlain<double> ld{y};
-        package_terms<double>(d, ld, true, xf, xd);
-        const int k = threadIdx.x;
-        if (k < d.nx) {
-            const long c = (long)d.nl * d.nr * d.nc + k;
-            const double b = power32 ? 0.0 + d.x_gamb[k] * d.ambient : rhs[c];
-            const float e = (float)(tau * (d.x_cinv[k] * (b + xf[k] - d.x_gamb[k] * y[c])));
-            e1[c] = e;
-            v[0] = fmax(v[0], fabs((double)e));
-        }
+        first_package(d, y, rhs, e1, tau, power32, v);
     }
     block_reduce<1, true>(v);
     write_partials<1>(d, wi.tile, v);
@@ -2455,6 +2509,171 @@ __device__ __forceinline__ void add_e1_vec4(const ProbDesc &d, int tile, const f
         const float v[4] = {a.x, a.y, a.z, a.w};
         add_state4(y, c, v);
     }
+}
+
+// E_1 of a homogeneous problem z-marching over the float4 term tiling's
+// column tiles (groups whose Horner steps z-march): each thread owns 4
+// consecutive columns x t_rpt/4 rows through every layer and keeps the
+// layers above and below in registers; its neighbours in the plane come as
+// in term_vec4z (a shuffle along the tile row, LDS across thread rows, the
+// tile's outer halo from memory).  A cell's E_1 is first_cell's, as in
+// k_tay_first2, with the FP32 cell power of F_PW (none for a layer marked in
+// lzero).
+__device__ __forceinline__ double comp2(const hs_d2 &a, const hs_d2 &b, int k) { return k < 2 ? a[k] : b[k - 2]; }
+template <int GM>
+__device__ __forceinline__ void first_vec4z(const ProbDesc &d, int ztile, const double *y, float *e1, double tau,
+                                            double (&v)[1]) {
+    // [layer parity][first row, last row (one row a thread: the same)][columns 0-1, 2-3][thread]
+    __shared__ hs_d2 edge[2][GM][2][kBlock];
+    const int ntj = term_ntj(d);
+    const int ti = ztile / ntj, tj = ztile - ti * ntj;
+    const int sh = d.t_shift;
+    const int tw = 1 << (sh - 2);             // threads along a tile row
+    const int tx4 = threadIdx.x & (tw - 1), ty4 = threadIdx.x >> (sh - 2);
+    const int tyn = kBlock >> (sh - 2);       // thread rows
+    const int G = d.t_rpt >> 2;
+    const int rows = tyn * G;
+    const int nr = d.nr, nc = d.nc, nl = d.nl;
+    const unsigned plane = (unsigned)nr * nc;
+    const unsigned ncell = plane * (unsigned)nl;
+    const int j0 = (tj << sh) + 4 * tx4;
+    const bool top = ty4 == 0, bottom = ty4 == tyn - 1;
+    const bool left = tx4 == 0, right = tx4 == tw - 1;
+    const float *pw = d.b32[F_PW];
+    int I[GM], ic[GM];
+    hs_d2 Ua[GM], Ub[GM], Ca[GM], Cb[GM], Da[GM], Db[GM];
+#pragma unroll
+    for (int g = 0; g < GM; g++) {
+        I[g] = ti * rows + ty4 * G + g;
+        ic[g] = min(I[g], nr - 1);
+        if (g < G) {
+            const unsigned c = (unsigned)ic[g] * nc + j0;
+            Ca[g] = ld2(y, c);
+            Cb[g] = ld2(y, c + 2);
+            Ua[g] = Ca[g];
+            Ub[g] = Cb[g];
+        }
+    }
+#pragma unroll 1
+    for (int n = 0; n < nl; n++) {
+        const unsigned off = (unsigned)n * plane;
+        const int p = n & 1;
+        const FirstLayer L = first_layer(d, n);
+        const bool pz = d.lzero[n] != 0;
+        hs_d2 Fa = Ca[0], Fb = Cb[0], La = Ca[0], Lb = Cb[0];
+#pragma unroll
+        for (int g = 1; g < GM; g++)
+            if (g == G - 1) {
+                La = Ca[g];
+                Lb = Cb[g];
+            }
+        edge[p][0][0][threadIdx.x] = Fa;
+        edge[p][0][1][threadIdx.x] = Fb;
+        edge[p][GM - 1][0][threadIdx.x] = La;
+        edge[p][GM - 1][1][threadIdx.x] = Lb;
+        hs_d2 Na = Fa, Nb = Fb, Sa = La, Sb = Lb;
+        {
+            const unsigned cf = off + (unsigned)ic[0] * nc + j0;
+            const unsigned cl = off + (unsigned)ic[G - 1] * nc + j0;
+            if (top && ic[0] > 0) {
+                Na = ld2(y, cf - nc);
+                Nb = ld2(y, cf - nc + 2);
+            }
+            if (bottom && ic[G - 1] < nr - 1) {
+                Sa = ld2(y, cl + nc);
+                Sb = ld2(y, cl + nc + 2);
+            }
+        }
+        double Wv[GM], Ev[GM];
+        float4 P[GM];   // cell power
+#pragma unroll
+        for (int g = 0; g < GM; g++) {
+            if (g >= G) continue;
+            const unsigned c = off + (unsigned)ic[g] * nc + j0;
+            if (n < nl - 1) {
+                Da[g] = ld2(y, c + plane);
+                Db[g] = ld2(y, c + plane + 2);
+            } else {
+                Da[g] = Ca[g];
+                Db[g] = Cb[g];
+            }
+            Wv[g] = __shfl_up(Cb[g][1], 1, tw);
+            Ev[g] = __shfl_down(Ca[g][0], 1, tw);
+            if (left) Wv[g] = j0 > 0 ? ld_at(y, c - 1) : Ca[g][0];
+            if (right) Ev[g] = j0 + 4 < nc ? ld_at(y, c + 4) : Ca[g][0];
+            P[g] = pz ? make_float4(0.f, 0.f, 0.f, 0.f) : ld4(pw, c);
+        }
+        double ring[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) ring[q] = ld_at(y, ncell + max(L.rn[q], 0));
+        __syncthreads();
+        if (!top) {
+            Na = edge[p][GM - 1][0][threadIdx.x - tw];
+            Nb = edge[p][GM - 1][1][threadIdx.x - tw];
+        }
+        if (!bottom) {
+            Sa = edge[p][0][0][threadIdx.x + tw];
+            Sb = edge[p][0][1][threadIdx.x + tw];
+        }
+#pragma unroll
+        for (int g = 0; g < GM; g++) {
+            if (g >= G || I[g] >= nr) continue;
+            // the rows next to row g inside the thread's own rows are its
+            // registers
+            const hs_d2 na = g == 0 ? Na : Ca[g - 1], nb = g == 0 ? Nb : Cb[g - 1];
+            const hs_d2 sa = g == G - 1 ? Sa : Ca[g + 1], sb = g == G - 1 ? Sb : Cb[g + 1];
+            float e[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const double xc = comp2(Ca[g], Cb[g], k);
+                const double xw = k == 0 ? Wv[g] : comp2(Ca[g], Cb[g], k - 1);
+                const double xe = k == 3 ? Ev[g] : comp2(Ca[g], Cb[g], k + 1);
+                const double b = (double)comp(P[g], k) + L.src;
+                e[k] = first_cell(d, L, I[g], j0 + k, xc, comp2(na, nb, k), comp2(sa, sb, k), xw, xe,
+                                  comp2(Ua[g], Ub[g], k), comp2(Da[g], Db[g], k), b, tau, ring);
+                v[0] = fmax(v[0], fabs((double)e[k]));
+            }
+            st4(e1, off + (unsigned)ic[g] * nc + j0, make_float4(e[0], e[1], e[2], e[3]));
+        }
+#pragma unroll
+        for (int g = 0; g < GM; g++) {
+            Ua[g] = Ca[g];
+            Ub[g] = Cb[g];
+            Ca[g] = Da[g];
+            Cb[g] = Db[g];
+        }
+    }
+}
+
+// The first term over the z-marching term work-list of a FP32 Taylor group
+// (power32 1): the partial max of a column tile goes to its slot, the
+// package item's to the slot after the last column tile (the finish reduces
+// t_ntiles / nl + 1 slots).
+template <int GM>
+__global__ void __launch_bounds__(kBlock) k_tay_first4z(const ProbDesc *__restrict__ D,
+                                                        const WorkItem *__restrict__ W, int nper, int e1b, int sub,
+                                                        int seq, int *__restrict__ kmax) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *kmax = 0;   // raised by the finish of each problem
+    const int b = work_index();
+    int prob, ztile;
+    if (nper > 0) {
+        prob = b / nper;
+        ztile = b - prob * nper;
+    } else {
+        const WorkItem wi = W[b];
+        prob = wi.prob;
+        ztile = wi.tile;
+    }
+    const ProbDesc &d = D[prob];
+    if (sub >= (int)d.scal[S_TK] || seq_finished(d, seq)) return;
+    const double *y = d.b64[B_Y0 + (int)d.scal[S_YPAR]], *rhs = d.b64[B_RHS];
+    float *e1 = d.b32[e1b];
+    const double tau = d.scal[S_TAU1];
+    double v[1] = {0};
+    if (ztile < d.t_ntiles / d.nl) first_vec4z<GM>(d, ztile, y, e1, tau, v);
+    else first_package(d, y, rhs, e1, tau, 1, v);
+    block_reduce<1, true>(v);
+    write_partials<1>(d, ztile, v);
 }
 
 // Horner step j: h <- E_1 + (tau/j) M h

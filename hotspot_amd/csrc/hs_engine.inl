@@ -125,7 +125,7 @@ struct Group {
     int ntwork = 0, tnper = 0;
     int tz = 0;   // its tiles: 0 per layer for every problem, 1 z-marching for every problem, 2 mixed
     int tg = 2;   // most row groups of 4 rows per thread among its problems (t_rpt / 4)
-    bool pcodes = false;   // every problem has cell_u16 on every layer (k_tay_first2 maps the power)
+    bool zfirst = false;   // the first term z-marches over the term work-list (k_tay_first4z)
 };
 
 // A homogeneous problem whose tiles are at least 16 columns wide with at
@@ -133,11 +133,6 @@ struct Group {
 // (rows hold whole tiles: grid sizes are powers of two).
 static bool vec4_ok(const ProbDesc &h) {
     return !h.hetero && h.t_rpt >= 4 && h.t_shift >= 4 && h.nc % (1 << h.t_shift) == 0;
-}
-static bool pcodes_ok(const ProbDesc &h) {
-    for (int n = 0; n < h.nl; n++)
-        if (!h.cell_u16[n]) return false;
-    return true;
 }
 
 // One multigrid level of one problem: its descriptor (the finest level is
@@ -334,6 +329,14 @@ static int term_groups(const std::vector<ProbDesc> &desc) {
     for (const ProbDesc &h : desc) g = std::max(g, h.t_rpt / 4);
     return g;
 }
+static int slot_count(const ProbDesc &h);
+// every problem z-marches and its column tiles and package item fit its
+// reduction slots: the first term can z-march too
+static bool first_z_ok(const std::vector<ProbDesc> &desc) {
+    for (const ProbDesc &h : desc)
+        if (!h.t_z || h.t_ntiles / h.nl + 1 > slot_count(h)) return false;
+    return !desc.empty();
+}
 
 // the float4 Horner step over the group's term work-list, in the kernel of
 // its tile form (fin: the step adds its result into the FP64 state)
@@ -479,6 +482,7 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap,
     upload_term_items(G, w, per, cap, pool, s);
     G.tz = term_form(desc);
     G.tg = term_groups(desc);
+    G.zfirst = first_z_ok(desc);
 }
 
 // Pins the float4 sweep tiling of the model's grid shape (layers, rows,
@@ -892,6 +896,8 @@ static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr
     std::vector<const int4 *> dcu(M.maps.size(), nullptr);
     std::vector<const unsigned short *> dcode(M.maps.size(), nullptr);
     std::vector<const float *> dw(M.maps.size(), nullptr);
+    std::vector<const int *> dmoff(M.maps.size(), nullptr), dmcell(M.maps.size(), nullptr);
+    std::vector<const int4 *> dmuw(M.maps.size(), nullptr);
     for (int n = 0; n < g.n_layers; n++) {
         const int mi = M.layer_map[n];
         h.map_ptr[n] = dptr[mi];
@@ -910,7 +916,10 @@ static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr
         if (!dcu[mi]) {   // layers on one map share its floorplan, hence the areas
             const BlockMap &bm = M.maps[mi];
             const size_t plane = bm.cell_ptr.size() - 1;
-            std::vector<int4> uw(plane, make_int4(-1, -1, 0, 0));
+            // per cell {units}, {weights}: up to four blocks, unused slots
+            // repeat the first block with weight 0; a cell of five or more
+            // blocks has unit -1 and takes its map entries
+            std::vector<int4> uw(2 * plane, make_int4(-1, -1, -1, -1));
             auto bits = [](float f) {
                 int b;
                 memcpy(&b, &f, sizeof b);
@@ -918,11 +927,15 @@ static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr
             };
             for (size_t c = 0; c < plane; c++) {
                 const int e = bm.cell_ptr[c], cnt = bm.cell_ptr[c + 1] - e;
-                if (cnt < 1 || cnt > 2) continue;
-                const int u0 = bm.ent_unit[e], u1 = bm.ent_unit[e + cnt - 1];
-                const float w0 = (float)(bm.ent_occ[e] * h.cell_area / area[u0]);
-                const float w1 = cnt == 2 ? (float)(bm.ent_occ[e + 1] * h.cell_area / area[u1]) : 0.f;
-                uw[c] = make_int4(u0, u1, bits(w0), bits(w1));
+                if (cnt < 1 || cnt > 4) continue;
+                int u[4];
+                float w[4];
+                for (int q = 0; q < 4; q++) {
+                    u[q] = bm.ent_unit[e + (q < cnt ? q : 0)];
+                    w[q] = q < cnt ? (float)(bm.ent_occ[e + q] * h.cell_area / area[u[q]]) : 0.f;
+                }
+                uw[2 * c] = make_int4(u[0], u[1], u[2], u[3]);
+                uw[2 * c + 1] = make_int4(bits(w[0]), bits(w[1]), bits(w[2]), bits(w[3]));
             }
             dcu[mi] = D->upload(uw.data(), uw.size());
             if (f->n_units < (int)kCellMixed) {
@@ -937,11 +950,36 @@ static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr
                 for (int u = 0; u < f->n_units; u++) w[u] = (float)(1.0 * h.cell_area / area[u]);
                 dcode[mi] = D->upload(code.data(), code.size());
                 dw[mi] = D->upload(w.data(), w.size());
+                // the boundary cells by tile of the standard tiling (rows of
+                // TY * rpt, columns of TX; tile ti * ntj + tj)
+                const int tx = 1 << h.tx_shift, rows = (kBlock >> h.tx_shift) * h.rpt;
+                std::vector<int> off(1, 0), cell;
+                std::vector<int4> rec;
+                for (int ti = 0; ti < h.nti; ti++)
+                    for (int tj = 0; tj < h.ntj; tj++) {
+                        for (int i = ti * rows; i < std::min(g.rows, (ti + 1) * rows); i++)
+                            for (int j = tj * tx; j < std::min(g.cols, (tj + 1) * tx); j++) {
+                                const size_t c = (size_t)i * g.cols + j;
+                                if (code[c] != kCellMixed) continue;
+                                cell.push_back((int)c);
+                                rec.push_back(uw[2 * c]);
+                                rec.push_back(uw[2 * c + 1]);
+                            }
+                        off.push_back((int)cell.size());
+                    }
+                cell.push_back(0);   // no empty uploads
+                rec.push_back(make_int4(0, 0, 0, 0));
+                dmoff[mi] = D->upload(off.data(), off.size());
+                dmcell[mi] = D->upload(cell.data(), cell.size());
+                dmuw[mi] = D->upload(rec.data(), rec.size());
             }
         }
         h.cell_uw[n] = dcu[mi];
         h.cell_u16[n] = dcode[mi];
         h.unit_w[n] = dw[mi];
+        h.mix_off[n] = dmoff[mi];
+        h.mix_cell[n] = dmcell[mi];
+        h.mix_uw[n] = dmuw[mi];
     }
     h.nblocks = g.total_n_blocks;
     h.blk_range = D->upload(rng.data(), rng.size());
@@ -977,7 +1015,6 @@ static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr
     D->grp.stream = D->stream;
     D->grp.hetero = M.detailed;
     D->grp.vec4 = vec4_ok(h);
-    D->grp.pcodes = pcodes_ok(h);
     D->grp.d_kmax = D->alloc<int>(2);
     dzero(D->grp.d_kmax, 2 * sizeof(int), D->stream);
     D->grp.d_trec = D->alloc<double>(kTrec);
@@ -1057,7 +1094,7 @@ static inline int small_grid(int n) { return (n + 63) / 64; }
 
 static void run_b2g(const Group &G, int type, int out, int sources, int from_seq = 0) {
     launch(k_b2g, G.nwork, G.stream, (const ProbDesc *)G.d_desc, (const WorkItem *)G.d_work, type, out,
-           sources, from_seq, 0);
+           sources, from_seq);
 }
 
 // The power of an interval for the FP32 Taylor sweeps of homogeneous float4
@@ -1065,8 +1102,7 @@ static void run_b2g(const Group &G, int type, int out, int sources, int from_seq
 // unwritten; k_tay_first2 adds the ambient sources itself.
 static bool power32(const Group &G, bool f32, bool het) { return f32 && !het && G.vec4; }
 static void run_b2g_power32(const Group &G, int from_seq = 0) {
-    launch(k_b2g, G.nwork, G.stream, (const ProbDesc *)G.d_desc, (const WorkItem *)G.d_work, (int)V_POWER,
-           (int)F_PW, 0, from_seq, 1);
+    launch(k_b2g_power32, G.nwork, G.stream, (const ProbDesc *)G.d_desc, (const WorkItem *)G.d_work, from_seq);
 }
 
 static void run_g2b(const Group &G, int in, int to_seq = 0) {
@@ -1696,9 +1732,12 @@ static void taylor_interval(const Group &G, int m, int Kplan, int seq) {
     const bool f32 = sizeof(TD) == 4;
     const int db0 = f32 ? F_D0 : B_KA, db1 = f32 ? F_D1 : B_KB, accb = f32 ? F_ACC : B_ACC;
     for (int s = 0; s < m; s++) {
-        if (f32 && !HET && G.vec4)
-            launch_named("k_tay_first", k_tay_first2, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax,
-                         seq && G.pcodes ? 2 : 1);
+        const bool zf = f32 && !HET && G.vec4 && G.zfirst;
+        if (zf)
+            launch_named("k_tay_first", G.tg == 1 ? k_tay_first4z<1> : k_tay_first4z<2>, G.ntwork, G.stream, dd,
+                         (const WorkItem *)G.d_twork, G.tnper, accb, s, seq, G.d_kmax);
+        else if (f32 && !HET && G.vec4)
+            launch_named("k_tay_first", k_tay_first2, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax, 1);
         else
             launch_named("k_tay_first", k_tay_first<TD, HET>, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax);
         // float4 sweeps finish the sub-step in their last launch (j = 2)
@@ -1710,6 +1749,7 @@ static void taylor_interval(const Group &G, int m, int Kplan, int seq) {
         a.trec = G.d_trec;
         a.kg = K;
         a.fused = fused ? 1 : 0;
+        a.zslots = zf ? 1 : 0;
         finish(G, FIN_TAY, a);
         const int *km = G.d_kmax;
         const double *tr = G.d_trec;
@@ -1913,13 +1953,8 @@ static void rk4_seq_body(const Group &G, int slots) {
 
 template <typename TD, bool HET>
 static void taylor_seq_body(const Group &G, int m, int K) {
-    // FP32 groups whose cells have one-block codes map the power in
-    // k_tay_first2 itself
-    if (power32(G, sizeof(TD) == 4, HET)) {
-        if (!G.pcodes) run_b2g_power32(G, 1);
-    } else {
-        run_b2g(G, V_POWER, B_RHS, 1, 1);
-    }
+    if (power32(G, sizeof(TD) == 4, HET)) run_b2g_power32(G, 1);
+    else run_b2g(G, V_POWER, B_RHS, 1, 1);
     taylor_interval<TD, HET>(G, m, K, 1);
     run_g2b(G, -1, 1);
     hipLaunchKernelGGL(k_step_advance, dim3(small_grid(G.nprob)), dim3(64), 0, G.stream,
@@ -2172,8 +2207,7 @@ static void slope_noalloc(Model &M, const double *d_power, const double *d_temp,
     h.b64[B_KA] = d_slope;
     ProbDesc *dd = (ProbDesc *)base;
     HIP_OK(hipMemcpyAsync(dd, &h, sizeof h, hipMemcpyHostToDevice, D.stream));
-    launch(k_b2g, D.nwork, D.stream, (const ProbDesc *)dd, (const WorkItem *)D.d_work, (int)V_POWER, (int)B_RHS, 1, 0,
-           0);
+    launch(k_b2g, D.nwork, D.stream, (const ProbDesc *)dd, (const WorkItem *)D.d_work, (int)V_POWER, (int)B_RHS, 1, 0);
     if (D.hetero)
         launch(k_slope<true>, D.nwork, D.stream, (const ProbDesc *)dd, (const WorkItem *)D.d_work, (int)B_TMP,
                (int)B_KA);

@@ -132,9 +132,10 @@ struct ProbDesc {
     const int *map_unit[kMaxLayers];
     const double *map_occ[kMaxLayers];
     const double *unit_area[kMaxLayers];
-    // per cell of a layer: its first two blocks and their weights occ * cell
-    // area / block area as {u0, u1, w0, w1} (weights bit-cast; u1 = u0 and
-    // w1 = 0 for a cell inside one block, u0 = -1 for a cell of 3+ blocks)
+    // per cell of a layer: its blocks (up to four) and their weights occ *
+    // cell area / block area, as [2 c] {u0, u1, u2, u3} and [2 c + 1] the
+    // weights bit-cast (unused slots: u0 with weight 0; u0 = -1 for a cell
+    // of five or more blocks, which takes its map entries)
     const int4 *cell_uw[kMaxLayers];
     // per cell of a layer: the one block that covers it whole, or kCellMixed
     // (the cell takes cell_uw), and per block of the layer its weight
@@ -142,6 +143,12 @@ struct ProbDesc {
     // kCellMixed blocks or more
     const unsigned short *cell_u16[kMaxLayers];
     const float *unit_w[kMaxLayers];
+    // with cell_u16: the cells marked kCellMixed listed by tile of the
+    // standard tiling, in tile order (mix_off: per tile of a layer, the first
+    // entry, then the count), each with its plane index and cell_uw record
+    const int *mix_off[kMaxLayers];
+    const int *mix_cell[kMaxLayers];
+    const int4 *mix_uw[kMaxLayers];
     int layer_base[kMaxLayers];         // block-vector offset per layer
     const int4 *blk_range;              // per block of the vector: i1,i2,j1,j2
     const int *blk_layer;
