@@ -425,6 +425,50 @@ def mode_large(lib, args):
     return res
 
 
+def roofline_pass(lib, args, jobs, nvec):
+    """The Horner sweep k_tay_term4 (one step h <- E_1 + s_j M h + t_j h per
+    launch) timed by device events on the engine's stream: plain launches
+    instead of graph replay while the timer is on, one transient stream;
+    consecutive launches of the kernel share one pair of events around their
+    run, so the mean duration per launch is the run's span over its launch
+    count, gaps included.  Algorithmic bytes for one job: FP32 read of h,
+    read of E_1, write of the new h = 12 B per node; 8 B when h is E_1 itself
+    (a job's first step of a sub-step); 24 B for the last step, which adds
+    the increment into the FP64 state.  A launch skips the jobs whose series
+    needs fewer steps, so the achieved rate is the bytes of the steps the
+    jobs took (counted on the device per job: taylor_node_bytes x nodes) over
+    the summed duration of every launch.  The pass serves --roofline-batch
+    jobs (the given ones first) so that one launch's working set (3 FP32
+    vectors per job) is above the 256 MB Infinity Cache: the bytes are HBM
+    bytes.  Returns (GB/s, mean ms per launch, total ms, launches, node
+    bytes summed over the jobs)."""
+    L = lib.lib
+    kernel = "k_tay_term"
+    rb = args.roofline_batch
+    rjobs = jobs + [Job(lib, args.grid, args.roofline_intervals, seed=2 + b) for b in range(len(jobs), rb)]
+    full_rows = [j.rows for j in jobs]
+    for j in rjobs:
+        j.rows = np.ascontiguousarray(np.vstack([j.P[None, :], j.seq[:args.roofline_intervals]]))
+    served(lib, rjobs, args.roofline_intervals, args.tick)
+    _, _, rinfo, _ = served(lib, rjobs, args.roofline_intervals, args.tick, timer=kernel.encode())
+    device_sync()
+    a, tot = C.c_double(), C.c_double()
+    nt = L.hs_kernel_timer_read(C.byref(a), C.byref(tot))
+    L.hs_kernel_timer(None, 0)
+    avg_ms, tot_ms = a.value, tot.value
+    node_bytes = float(sum(i.taylor_node_bytes for i in rinfo))
+    achieved = None
+    if nt:
+        achieved = node_bytes * nvec / (tot_ms * 1e-3) / 1e9
+        log(f"roofline pass ({rb} jobs): {nt} launches of {kernel}, mean {avg_ms*1e3:.2f} us, "
+            f"{node_bytes * nvec / nt / 1e6:.1f} MB per launch on average, {achieved:.0f} GB/s")
+    for j in rjobs[len(jobs):]:
+        j.run.free()
+    for j, r in zip(jobs, full_rows):
+        j.rows = r
+    return achieved, avg_ms, tot_ms, nt, node_bytes
+
+
 def main():
     global TAYLOR_TOL
     ap = argparse.ArgumentParser()
@@ -454,6 +498,8 @@ def main():
     ap.add_argument("--roofline-batch", type=int, default=48,
                     help="jobs of the roofline pass (one launch's working set above the Infinity Cache)")
     ap.add_argument("--roofline-intervals", type=int, default=300)
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="run only the roofline pass (for kernel traces and counters) and print its figures")
     ap.add_argument("--taylor-tol", type=float, default=TAYLOR_TOL,
                     help="bound on the Taylor truncation error per interval (K)")
     args = ap.parse_args()
@@ -470,6 +516,18 @@ def main():
     if args.mode != "batch":
         res = {"many": mode_many, "serve": mode_serve, "large": mode_large}[args.mode](lib, args)
         print(json.dumps(res), flush=True)
+        return
+
+    if args.roofline_only:
+        jobs = [Job(lib, args.grid, args.roofline_intervals, seed=2 + b) for b in range(args.roofline_batch)]
+        nvec = jobs[0].cells + iface.EXTRA
+        achieved, avg_ms, tot_ms, nt, node_bytes = roofline_pass(lib, args, jobs, nvec)
+        print(json.dumps({"kernel": "k_tay_term4", "jobs": args.roofline_batch,
+                          "intervals": args.roofline_intervals, "launches": nt,
+                          "avg_launch_us": avg_ms * 1e3, "achieved_GBs": achieved,
+                          "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                          "algorithmic_bytes_avg_per_launch": node_bytes * nvec / nt if nt else None}),
+              flush=True)
         return
 
     log(f"building {args.batch} jobs: {args.grid}x{args.grid} 3D stack, {args.intervals} intervals")
@@ -550,46 +608,13 @@ def main():
     del seq_mix, fp64_batch_seqs
 
     # ---- roofline pass of the dominant kernel ----------------------------------
-    # The Horner sweep k_tay_term4 (one step h <- E_1 + s_j M h + t_j h per
-    # launch) timed by device events on the engine's stream: plain launches
-    # instead of graph replay while the timer is on, one transient stream;
-    # consecutive launches of the kernel share one pair of events around
-    # their run, so the mean duration per launch is the run's span over its
-    # launch count, gaps included.  Algorithmic bytes for one job: FP32 read
-    # of h, read of E_1, write of the new h = 12 B per node; 8 B when h is E_1
-    # itself (a job's first step of a sub-step); 24 B for the last step, which
-    # adds the increment into the FP64 state.  A launch skips the jobs whose
-    # series needs fewer steps, so the achieved rate is the bytes of the steps
-    # the jobs took (counted on the device per job: taylor_node_bytes x
-    # nodes) over the summed duration of every launch.  The pass serves
-    # --roofline-batch jobs so that one launch's working set (3 FP32 vectors
-    # per job) is above the 256 MB Infinity Cache: the bytes are HBM bytes.
     kernel = "k_tay_term"
     nvec = cells + iface.EXTRA
     rb = args.roofline_batch
     achieved = avg_ms = nt = None
     node_bytes = tot_ms = 0.0
     if not args.no_roofline:
-        rjobs = jobs + [Job(lib, args.grid, args.roofline_intervals, seed=2 + b) for b in range(args.batch, rb)]
-        full_rows = [j.rows for j in jobs]
-        for j in rjobs:
-            j.rows = np.ascontiguousarray(np.vstack([j.P[None, :], j.seq[:args.roofline_intervals]]))
-        served(lib, rjobs, args.roofline_intervals, args.tick)
-        _, _, rinfo, _ = served(lib, rjobs, args.roofline_intervals, args.tick, timer=kernel.encode())
-        device_sync()
-        a, tot = C.c_double(), C.c_double()
-        nt = L.hs_kernel_timer_read(C.byref(a), C.byref(tot))
-        L.hs_kernel_timer(None, 0)
-        avg_ms, tot_ms = a.value, tot.value
-        node_bytes = float(sum(i.taylor_node_bytes for i in rinfo))
-        if nt:
-            achieved = node_bytes * nvec / (tot_ms * 1e-3) / 1e9
-            log(f"roofline pass ({rb} jobs): {nt} launches of {kernel}, mean {avg_ms*1e3:.2f} us, "
-                f"{node_bytes * nvec / nt / 1e6:.1f} MB per launch on average, {achieved:.0f} GB/s")
-        for j in rjobs[args.batch:]:
-            j.run.free()
-        for j, r in zip(jobs, full_rows):
-            j.rows = r
+        achieved, avg_ms, tot_ms, nt, node_bytes = roofline_pass(lib, args, jobs, nvec)
     traffic = traffic_ratio = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{kernel}_3dstack{args.grid}_batch{rb}.json")
     if os.path.exists(tf):

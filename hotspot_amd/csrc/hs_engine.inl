@@ -372,22 +372,33 @@ static int term_items(const std::vector<ProbDesc> &desc, std::vector<WorkItem> &
 }
 
 // Tile shape of the float4 term sweep, picked by timing per size class
-// (layers, rows, columns).  Candidates: tile widths of 16 to 128 columns (at
-// most nc) with 4 or 8 rows per thread.  Each is timed over a few sweeps of
-// the group's problems of that class (one Horner step reading the E_1 buffer
-// and writing a term buffer: both are free between intervals), and the
-// fastest kept.  The arithmetic of a cell is the same under every tiling, so
-// the choice changes only the speed.  Choices are remembered per class, so a
-// group whose membership changes times only the classes it has not met.
+// (layers, rows, columns, and the power of two below the number of the
+// group's problems of that shape: how many share a launch changes what the
+// caches hold and how the workgroups fill the device).  Candidates: tile
+// widths of 16 to 256 columns (at most nc) with 4 or 8 rows per thread, per
+// layer or z-marching.  Each is timed over the Horner steps of an interval
+// of the class's problems (five steps reading h and E_1 and writing a term
+// buffer, then the last step, which adds into the state: the term buffers
+// are free between intervals and the state is redirected to free FP64
+// buffers while timing), and the fastest kept.  The arithmetic of a cell is
+// the same under every tiling, so the choice changes only the speed.
+// Choices are remembered per class, so a group whose membership changes
+// times only the classes it has not met; a shape pinned by
+// hs_pin_sweep_tiling takes its pin whatever the count.
 struct TermShape {
     int shift = 0, rpt = 0, z = 0;
 };
-using ShapeKey = std::array<int, 3>;
+using ShapeKey = std::array<int, 4>;
+using GridKey = std::array<int, 3>;
 static std::map<ShapeKey, TermShape> &term_shape_cache() {
     static std::map<ShapeKey, TermShape> m;
     return m;
 }
-static ShapeKey shape_key(const ProbDesc &h) { return ShapeKey{h.nl, h.nr, h.nc}; }
+static std::map<GridKey, TermShape> &term_shape_pins() {
+    static std::map<GridKey, TermShape> m;
+    return m;
+}
+static GridKey grid_key(const ProbDesc &h) { return GridKey{h.nl, h.nr, h.nc}; }
 
 static void upload_term_items(Group &G, const std::vector<WorkItem> &w, int per, size_t &cap, DevPool *pool,
                               hipStream_t s) {
@@ -405,17 +416,25 @@ static void upload_term_items(Group &G, const std::vector<WorkItem> &w, int per,
 // with another group's launches) and applies each problem's class shape.
 static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap, DevPool *pool, hipStream_t s) {
     auto &cache = term_shape_cache();
+    const auto &pins = term_shape_pins();
     const int n = (int)desc.size();
+    std::map<GridKey, int> count;
+    for (const ProbDesc &h : desc) count[grid_key(h)]++;
+    auto key_of = [&](const ProbDesc &h) {
+        const GridKey g = grid_key(h);
+        return ShapeKey{g[0], g[1], g[2], ilog2(count[g])};
+    };
     std::vector<ShapeKey> todo;
-    for (const ProbDesc &h : desc)
-        if (!cache.count(shape_key(h)) && std::find(todo.begin(), todo.end(), shape_key(h)) == todo.end())
-            todo.push_back(shape_key(h));
+    for (const ProbDesc &h : desc) {
+        const ShapeKey k = key_of(h);
+        if (!pins.count(grid_key(h)) && !cache.count(k) && std::find(todo.begin(), todo.end(), k) == todo.end())
+            todo.push_back(k);
+    }
     if (!todo.empty()) {
-        // every problem active for step j = 2 of a three-term series: h from
-        // one term buffer, E_1 from the accumulator, the result to the other
-        // term buffer (12 B per node, as the steps of an interval; the term
-        // buffers are free between intervals); the group's Taylor records
-        // are restored after timing
+        // every problem active for steps j = 2 of a three-term series: h
+        // from one term buffer, E_1 from the accumulator, the result to the
+        // other term buffer, or (the last step) into the state, here the RK4
+        // work buffers; the group's Taylor records are restored after timing
         std::vector<double> saved((size_t)kTrec * n), rec((size_t)kTrec * n, 0.0);
         d2h(saved.data(), G.d_trec, saved.size() * sizeof(double), s);
         for (int i = 0; i < n; i++) {
@@ -427,13 +446,18 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap,
         const int kt[2] = {3, 0};
         h2d(G.d_trec, rec.data(), rec.size() * sizeof(double), s);
         h2d(G.d_kmax, kt, sizeof kt, s);
+        std::vector<ProbDesc> tdesc(desc);
+        for (ProbDesc &h : tdesc) {
+            h.b64[B_Y0] = h.b64[B_KA];
+            h.b64[B_Y1] = h.b64[B_KB];
+        }
         hipEvent_t e0, e1;
         HIP_OK(hipEventCreate(&e0));
         HIP_OK(hipEventCreate(&e1));
         for (const ShapeKey &k : todo) {
             std::vector<int> mem;
             for (int i = 0; i < n; i++)
-                if (shape_key(desc[i]) == k) mem.push_back(i);
+                if (key_of(desc[i]) == k) mem.push_back(i);
             TermShape best;
             float best_ms = 1e30f;
             // tile width at most 256: a tile row's 64 threads stay within one
@@ -441,19 +465,21 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap,
             for (int shift = 4; shift <= std::min(8, ilog2(k[2])); shift++)
                 for (int rpt : {4, 8})
                 for (int z : {0, 1}) {
-                    for (int i : mem) set_term_tiling(desc[i], shift, rpt, z);
-                    h2d(G.d_desc, desc.data(), desc.size() * sizeof(ProbDesc), s);
+                    for (int i : mem) set_term_tiling(tdesc[i], shift, rpt, z);
+                    h2d(G.d_desc, tdesc.data(), tdesc.size() * sizeof(ProbDesc), s);
                     std::vector<WorkItem> w;
                     for (int i : mem)
-                        for (int t = 0; t <= term_cell_items(desc[i]); t++) w.push_back(WorkItem{i, t});
+                        for (int t = 0; t <= term_cell_items(tdesc[i]); t++) w.push_back(WorkItem{i, t});
                     upload_term_items(G, w, 0, cap, pool, s);
-                    constexpr int kWarm = 2, kTimed = 6;
+                    constexpr int kWarm = 1, kTimed = 2, kSteps = 5;
                     // the launch covers this class's problems only
                     G.tz = z;
                     G.tg = rpt / 4;
                     for (int r = 0; r < kWarm + kTimed; r++) {
                         if (r == kWarm) HIP_OK(hipEventRecord(e0, s));
-                        launch_term4(G, s, (int)F_ACC, (int)F_D0, (int)F_D1, 2, 0, false);
+                        for (int q = 0; q < kSteps; q++)
+                            launch_term4(G, s, (int)F_ACC, (int)F_D0, (int)F_D1, 2, 0, false);
+                        launch_term4(G, s, (int)F_ACC, (int)F_D0, (int)F_D1, 2, 1, false);
                     }
                     HIP_OK(hipEventRecord(e1, s));
                     HIP_OK(hipEventSynchronize(e1));
@@ -473,7 +499,8 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap,
         h2d(G.d_kmax, zero, sizeof zero, s);
     }
     for (ProbDesc &h : desc) {
-        const TermShape t = cache.at(shape_key(h));
+        const auto pin = pins.find(grid_key(h));
+        const TermShape t = pin != pins.end() ? pin->second : cache.at(key_of(h));
         set_term_tiling(h, std::min(t.shift, ilog2(h.nc)), t.rpt, t.z);
     }
     h2d(G.d_desc, desc.data(), desc.size() * sizeof(ProbDesc), s);
@@ -490,16 +517,16 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap,
 // call; shift 0 drops the pin, and the next setup times the candidates.
 static void pin_term_tiling(const Model &M, int shift, int rpt, int z) {
     const grid_model_t &g = *M.g;
-    const ShapeKey k{g.n_layers, g.rows, g.cols};
+    const GridKey k{g.n_layers, g.rows, g.cols};
     if (shift == 0) {
-        term_shape_cache().erase(k);
+        term_shape_pins().erase(k);
         return;
     }
     if (shift < 4 || shift > 8 || g.cols % (1 << shift) != 0)
         fatal("sweep tiling: the tile width 1 << shift must be 16 to 256 and divide the grid's columns\n");
     if (rpt != 4 && rpt != 8) fatal("sweep tiling: rows per thread must be 4 or 8\n");
     if (z != 0 && z != 1) fatal("sweep tiling: z must be 0 (per-layer tiles) or 1 (z-marching column tiles)\n");
-    term_shape_cache()[k] = TermShape{shift, rpt, z};
+    term_shape_pins()[k] = TermShape{shift, rpt, z};
 }
 
 
