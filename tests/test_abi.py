@@ -1,8 +1,9 @@
 """The C ABI: every function declared in include/hotspot_amd.h is exported by
-the built library, and the struct layouts of the header equal the
-reference's implementation layouts (checked by compiling a probe with the
-header and comparing offsets with the ctypes mirror of the reference
-structs); a C program compiled against the header runs the host-side API."""
+the built library; the struct layouts of the header equal the ctypes mirror
+the tests use and, where the reference sources are present, the layouts the
+reference's own headers give (a probe compiled once against each header set,
+every field offset and size compared); a C program compiled against the
+header runs the host-side API."""
 import ctypes as C
 import os
 import re
@@ -72,6 +73,61 @@ def test_struct_layouts_match_reference_mirror():
             assert getattr(CTYPES[t], f).offset == int(val), key
         else:
             assert C.sizeof(CTYPES[key]) == int(val), key
+
+
+REF_DIR = "/root/reference"
+# typedef name -> ctypes mirror (which lists every field) or the field names
+REF_STRUCTS = {
+    "thermal_config_t": iface.ThermalConfig, "unit_t": iface.Unit, "flp_t": iface.Flp,
+    "package_RC_t": iface.PackageRC, "layer_t": iface.Layer, "grid_model_t": iface.GridModel,
+    "grid_model_vector_t": iface.GridVector, "RC_model_t": iface.RCModel,
+    "block_model_t": iface.BlockModel, "microchannel_config_t": iface.MicrochannelConfig,
+    "material_t": iface.Material, "materials_list_t": iface.MaterialsList, "blist_t": iface.Blist,
+    "glist_t": iface.Glist, "str_pair": iface.StrPair,
+    "package_config_t": ["natural_convec", "flow_type", "sink_type", "fin_height", "fin_width",
+                         "channel_width", "pin_height", "pin_diam", "pin_dist", "fan_radius",
+                         "motor_radius", "rpm"],
+    "convection_t": ["n_fin", "sur_area_fin", "n_pin", "sur_area_pin", "reynolds", "nusselt",
+                     "h_coeff", "v", "r_th"],
+}
+
+
+def layout_probe(includes):
+    lines = ["#include <stddef.h>", "#include <stdio.h>"] + [f'#include "{h}"' for h in includes]
+    lines.append("int main(void) {")
+    for t, spec in REF_STRUCTS.items():
+        fields = [f[0] for f in spec._fields_] if hasattr(spec, "_fields_") else spec
+        lines.append(f'  printf("{t} %zu\\n", sizeof({t}));')
+        for f in fields:
+            lines.append(f'  printf("{t}.{f} %zu\\n", offsetof({t}, {f}));')
+    lines += ["  return 0;", "}"]
+    return "\n".join(lines) + "\n"
+
+
+def run_probe(src_text, flags, tmp):
+    src = os.path.join(tmp, "probe.c")
+    exe = os.path.join(tmp, "probe")
+    with open(src, "w") as f:
+        f.write(src_text)
+    subprocess.check_call(["gcc", "-std=gnu11", "-w"] + flags + [src, "-o", exe])
+    return subprocess.check_output([exe]).decode().split("\n")
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_DIR), reason="reference sources absent")
+def test_struct_layouts_match_reference_headers(tmp_path):
+    """Every field offset and struct size of include/hotspot_amd.h equals the
+    one the reference's own headers give (temperature.h:141-357,
+    temperature_grid.h:63-191, temperature_block.h:18-62, flp.h:122-144,
+    microchannel.h:29-90, materials.h:9-27, util.h:92, package.h:48-104),
+    compiled as the reference compiles them (no SuperLU, no math library)."""
+    (tmp_path / "ours").mkdir()
+    ours = run_probe(layout_probe(["hotspot_amd.h"]), ["-I" + os.path.join(ROOT, "include")],
+                     str(tmp_path / "ours"))
+    theirs = run_probe(layout_probe(["temperature.h", "temperature_grid.h", "temperature_block.h",
+                                     "package.h"]),
+                       ["-I" + REF_DIR, "-DSUPERLU=0", "-DMATHACCEL=0"], str(tmp_path))
+    assert len(ours) > 150
+    assert ours == theirs, [(a, b) for a, b in zip(ours, theirs) if a != b][:10]
 
 
 DRIVER = r"""
