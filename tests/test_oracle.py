@@ -267,3 +267,55 @@ def test_backward_euler_limits():
 def spsolve_t(om, Pg):
     import scipy.sparse.linalg as spla
     return spla.spsolve(om.G.tocsc(), Pg + om.b)
+
+
+def _be_with_caps(om, Tg, Pg, h, cbe, G):
+    import scipy.sparse.linalg as spla
+    A = (sp.diags(cbe / h) + G).tocsc()
+    return spla.spsolve(A, cbe / h * Tg + Pg + om.b)
+
+
+def test_backward_euler_secondary_model():
+    """advance_be on the secondary-path model (the SuperLU build's matrix,
+    temperature_grid.c:6354, :6440 and build_diagonal_matrix :6750-6768):
+    the secondary package nodes carry no capacitance there, so after every
+    step they satisfy their steady equations exactly, whatever the state
+    before it; the other rows satisfy C (T' - T)/h = P + b - G T'; and the
+    step's fixed point for h -> infinity is the steady state of the steady
+    operator G (with the solder-substrate links), not of the transient Gt.
+    The same checks fail on two planted mistakes: secondary nodes with their
+    slope_fn_pack capacitances, and the transient operator Gt in place of G."""
+    om, P, _ = _load("secondary16")
+    assert om.secondary
+    Pg = om.power_grid(P)
+    rhs = Pg + om.b
+    sec = np.arange(om.ncell + O.EXTRA, om.N)
+    prim = np.arange(om.ncell + O.EXTRA)
+    y = om.b2g(np.full(om.n_vector(), om.cfg["ambient"] + 20.0), power=False)
+    y[sec] += np.linspace(-5.0, 5.0, len(sec))      # secondary nodes off their balance
+    h = 3.333e-6
+
+    def violations(ynew, yold, G, cbe):
+        r = rhs - G @ ynew
+        alg = np.abs(r[sec]).max() / np.abs(rhs[sec] + (G @ ynew)[sec]).max()
+        dyn = (np.abs(cbe[prim] * (ynew[prim] - yold[prim]) / h - r[prim]).max()
+               / np.abs(cbe[prim] * ynew[prim] / h).max())
+        return alg, dyn
+
+    for k in range(3):
+        y1 = om.advance_be(y, Pg, h)
+        alg, dyn = violations(y1, y, om.G, np.where(np.arange(om.N) < om.ncell + O.EXTRA, om.Cvec, 0.0))
+        assert alg <= 1e-9 and dyn <= 1e-9, (k, alg, dyn)
+        y = y1
+    tinf = spsolve_t(om, Pg)
+    assert np.abs(om.advance_be(y, Pg, 1e9) - tinf).max() <= 1e-6
+    # planted mistake 1: the secondary nodes keep their transient capacitances
+    y0 = om.b2g(np.full(om.n_vector(), om.cfg["ambient"] + 20.0), power=False)
+    y0[sec] += np.linspace(-5.0, 5.0, len(sec))
+    bad = _be_with_caps(om, y0, Pg, h, om.Cvec, om.G)
+    alg_bad, _ = violations(bad, y0, om.G, om.Cvec)
+    assert alg_bad > 1e-3
+    # planted mistake 2: the transient operator in the implicit step
+    cbe = np.where(np.arange(om.N) < om.ncell + O.EXTRA, om.Cvec, 0.0)
+    bad2 = _be_with_caps(om, y0, Pg, 1e9, cbe, om.Gt)
+    assert np.abs(bad2 - tinf).max() > 1e-3
