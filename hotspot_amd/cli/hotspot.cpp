@@ -56,7 +56,11 @@ void usage(const char *prog) {
     printf("  [-use_microchannels <0/1>]\tmicrochannel cooling layers (needs detailed_3D on)\n");
     printf("  [options]\t\"-<name> <value>\" pairs override the configuration file,\n");
     printf("           \te.g. \"-model_type block\", \"-grid_rows 128\"\n");
+    printf("  [-transient_solver <rk4|be|taylor>]\tgrid transient integrator: the reference's adaptive\n");
+    printf("           \tRK4 (default), the backward-Euler step of its SuperLU build, or the Taylor series\n");
+    printf("  [-precision <fp64|mixed>]\tgrid solves in FP64 (default) or FP32 inside FP64 refinement\n");
 }
+
 
 // command-line-only parameters (hotspot.c:94-147)
 struct Global {
@@ -166,6 +170,23 @@ void write_row(FILE *fp, const std::vector<double> &v) {
     for (size_t i = 0; i < v.size(); i++) fprintf(fp, "%.2f%c", v[i], i + 1 < v.size() ? '\t' : '\n');
 }
 
+// grid solver options of this library (hs_options_t); the reference's
+// parsers ignore these names
+void apply_solver_options(RC_model_t *model, str_pair *t, int n) {
+    hs_options_t o;
+    hs_get_options(model, &o);
+    const std::string ts = value_of(t, n, "transient_solver", "rk4", "transient_solver");
+    if (ts == "rk4") o.transient = HS_TRANS_RK4;
+    else if (ts == "be") o.transient = HS_TRANS_BE;
+    else if (ts == "taylor") o.transient = HS_TRANS_TAYLOR;
+    else die("invalid value for -transient_solver (rk4, be or taylor)\n");
+    const std::string pr = value_of(t, n, "precision", "fp64", "precision");
+    if (pr == "fp64") o.precision = HS_PREC_FP64;
+    else if (pr == "mixed") o.precision = HS_PREC_MIXED;
+    else die("invalid value for -precision (fp64 or mixed)\n");
+    hs_set_options(model, &o);
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -231,6 +252,8 @@ int main(int argc, char **argv) {
         die("-do_detailed_3D can only be used in 3D mode (if a grid_layer_file is specified)\n");
     if (g.microchannels && !detailed)
         die("-use_microchannels requires -model_type grid and do_detailed_3D on options\n");
+
+    if (grid) apply_solver_options(model, table, size);
 
     printf("Creating thermal circuit...\n");
     populate_R_model(model, flp);

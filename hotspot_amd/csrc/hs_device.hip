@@ -1197,6 +1197,32 @@ __global__ void k_upwind_coef(const ProbDesc *__restrict__ D, long ncell, float 
     udinv[c] = (float)(1.0 / (1.0 / d.dinvc[c] + (sa + self) - (su + uself)));
 }
 
+// Backward-Euler levels of the multigrid preconditioner of a non-symmetric
+// implicit step: the step's diagonal shift sigma C per cell (k_be_coef) goes
+// into each level's ambient term, a coarse cell's shift being the sum of its
+// four children's, and into the level's Jacobi diagonal (base_dinv: the
+// level's own, unshifted).  One thread per cell of level C; F is the level
+// above (nullptr: C is the finest level and takes fine_shift itself).
+__global__ void k_mg_shift(const ProbDesc *__restrict__ F, const ProbDesc *__restrict__ Cp,
+                           const double *__restrict__ fine_shift, const float *__restrict__ base_dinv) {
+    const ProbDesc &c = *Cp;
+    const unsigned ncell = (unsigned)c.nl * c.nr * c.nc;
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncell) return;
+    double s;
+    if (!F) {
+        s = fine_shift[t];
+    } else {
+        const ProbDesc &f = *F;
+        const unsigned plane = (unsigned)c.nr * c.nc;
+        const unsigned n = t / plane, rem = t - n * plane, I = rem / c.nc, J = rem - I * c.nc;
+        const unsigned f0 = (n * f.nr + 2 * I) * f.nc + 2 * J;
+        s = (f.bshift[f0] + f.bshift[f0 + 1]) + (f.bshift[f0 + f.nc] + f.bshift[f0 + f.nc + 1]);
+    }
+    ((double *)c.bshift)[t] = s;
+    ((float *)c.fdinv)[t] = 1.f / (1.f / base_dinv[t] + (float)s);
+}
+
 // when a multigrid kernel runs for a problem (flags of the fine problem,
 // whose scalars every level shares)
 enum MgWhen { MG_RUNNING = 0, MG_INIT = 1, MG_CORRECT = 2 };

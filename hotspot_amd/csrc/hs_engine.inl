@@ -129,10 +129,13 @@ struct Group {
 };
 
 // A homogeneous problem whose tiles are at least 16 columns wide with at
-// least 4 rows per thread runs its FP32 term sweeps with float4 accesses
-// (rows hold whole tiles: grid sizes are powers of two).
+// least 4 rows per thread runs its FP32 term sweeps with float4 accesses,
+// when its rows hold whole tiles of both its tilings (the float4 term tiling
+// and the per-layer tiling of the double2 first term); partial row tiles are
+// fine (rows past the grid are skipped), partial column tiles are not.
 static bool vec4_ok(const ProbDesc &h) {
-    return !h.hetero && h.t_rpt >= 4 && h.t_shift >= 4 && h.nc % (1 << h.t_shift) == 0;
+    return !h.hetero && h.t_rpt >= 4 && h.t_shift >= 4 && h.nc % (1 << h.t_shift) == 0 &&
+           h.nc % (1 << h.tx_shift) == 0;
 }
 
 // One multigrid level of one problem: its descriptor (the finest level is
@@ -173,6 +176,8 @@ struct DevModel {
     ProbDesc *d_be_desc = nullptr;
     Group be_grp;
     double *be_shift = nullptr, *be_dinv = nullptr;
+    std::vector<MgLevel> be_mg;         // multigrid levels of the shifted step (non-symmetric steps)
+    double be_sigma = 0.0;              // the shift 1/h they hold
 
     DevPool *pool = nullptr, *hpool = nullptr;   // an engine's pools (nullptr: the runtime's)
 
@@ -464,6 +469,7 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap,
             // wavefront (the z-marching sweep shuffles along the row)
             for (int shift = 4; shift <= std::min(8, ilog2(k[2])); shift++)
                 for (int rpt : {4, 8})
+                if (k[2] % (1 << shift) == 0)
                 for (int z : {0, 1}) {
                     for (int i : mem) set_term_tiling(tdesc[i], shift, rpt, z);
                     h2d(G.d_desc, tdesc.data(), tdesc.size() * sizeof(ProbDesc), s);
@@ -869,6 +875,9 @@ static double package_rho(const Model &M, const ProbDesc &h) {
 static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr, DevPool *hpool = nullptr) {
     grid_model_t &g = *M.g;
     if (g.n_layers > kMaxLayers) fatal("too many layers for the device model\n");
+    // the kernels address a vector through 32-bit byte offsets (8 B a node)
+    if ((double)M.ncell() + M.nx + 64 >= (double)(1u << 29))
+        fatal("grid too large for the device model: layers x rows x columns must stay below 2^29 nodes\n");
     std::unique_ptr<DevModel> own(new DevModel());
     DevModel *D = own.get();
     D->pool = pool;
@@ -1840,12 +1849,50 @@ static void build_be(DevModel &D, double h, int xb = -1, int rhsb = -1) {
     D.be_grp.d_desc = D.d_be_desc;
 }
 
+// The multigrid levels of the backward-Euler step's operator C/h + G for
+// non-symmetric (advection) models: the steady hierarchy's coefficients
+// (upwind advection, aggregated faces) with the step's shift added per cell
+// on every level (k_mg_shift) and to the primary package nodes, as build_be
+// adds it to the fine operator.  Rebuilt when h changes; D.be_shift must
+// hold the fine shift (build_be).
+static std::vector<MgGroup> be_levels(DevModel &D, double sigma) {
+    build_mg(D);
+    if (D.be_mg.empty()) {
+        for (const MgLevel &m : D.mg) {
+            MgLevel L = m;
+            const long ncell = (long)L.h.nl * L.h.nr * L.h.nc;
+            L.h.bshift = D.alloc<double>(ncell);
+            L.h.fdinv = D.alloc<float>(ncell);
+            L.d_desc = D.alloc<ProbDesc>(1);
+            D.be_mg.push_back(L);
+        }
+        D.be_sigma = 0.0;
+    }
+    if (sigma != D.be_sigma) {
+        for (size_t l = 0; l < D.be_mg.size(); l++) {
+            ProbDesc &h = D.be_mg[l].h;
+            for (int k = 0; k < std::min(h.nx, (int)EXTRA); k++) h.x_gamb[k] = D.mg[l].h.x_gamb[k] + sigma / h.x_cinv[k];
+            h2d(D.be_mg[l].d_desc, &h, sizeof(ProbDesc), D.stream);
+            const long ncell = (long)h.nl * h.nr * h.nc;
+            hipLaunchKernelGGL(k_mg_shift, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, D.stream,
+                               l == 0 ? (const ProbDesc *)nullptr : (const ProbDesc *)D.be_mg[l - 1].d_desc,
+                               (const ProbDesc *)D.be_mg[l].d_desc, (const double *)D.be_shift,
+                               (const float *)D.mg[l].h.fdinv);
+        }
+        HIP_OK(hipStreamSynchronize(D.stream));
+        D.be_sigma = sigma;
+    }
+    std::vector<MgGroup> lv;
+    for (const MgLevel &L : D.be_mg) lv.push_back(MgGroup{L.d_desc, L.d_cwork, L.ncwork, D.h.nl, true});
+    return lv;
+}
+
 template <bool HET>
 static void transient_be(Model &M, DevModel &D, double dt) {
     read_scal(D);
     const int yb = B_Y0 + (int)D.h_scal[S_YPAR];
     const bool adv = D.h.has_adv;
-    if (adv) ensure_gmres(D, false);   // the shifted descriptor copies its buffers
+    if (adv) ensure_gmres(D, true);   // the shifted descriptor copies its buffers
     build_be(D, dt, adv ? yb : -1, adv ? (int)B_T1 : -1);
     launch_named("k_be_rhs", k_be_rhs<HET>, D.grp.nwork, D.stream, (const ProbDesc *)D.d_desc,
                  (const WorkItem *)D.d_work, yb, (int)B_RHS, (int)B_T1, 1.0 / dt);
@@ -1853,28 +1900,39 @@ static void transient_be(Model &M, DevModel &D, double dt) {
     // goes 100x below the steady tolerance to keep every node as close
     const double rtol = 1e-2 * M.opt.steady_rtol, rtol2 = rtol * rtol;
     if (adv) {
-        // coolant advection makes the system non-symmetric: FP64 GMRES
-        // cycles from y, inside FP64 refinement
+        // coolant advection makes the system non-symmetric: FP32 GMRES
+        // cycles from y, right-preconditioned by the multigrid V-cycle of
+        // the shifted upwind operator, inside FP64 refinement (Jacobi-
+        // preconditioned GMRES stalls on these systems as on the steady ones)
+        const std::vector<MgGroup> lv = be_levels(D, 1.0 / dt);
         write_scal(D, kMixedStart);
         for (int guard = 0;; guard++) {
-            gm_round<double, true>(D.be_grp, rtol2, 1e-12, kGmM);
+            gm_round_mg<true>(D.be_grp, lv, rtol2, 1e-8, kGmM);
             read_scal(D);
             if (D.h_scal[S_DONE] != 0.0) break;
             if (guard > 100000) fatal("backward Euler: the non-symmetric solve did not finish\n");
         }
         M.stats.be_iters = (int)D.h_scal[S_TITERS];
-        return;
+    } else {
+        write_scal(D, {{S_DONE, 0.0}});   // the CG start skips a problem still marked done
+        jpcg64_start<HET>(D.be_grp, rtol2, M.opt.max_iters, yb, B_T1);
+        int chunk = 8;
+        for (;;) {
+            read_scal(D);
+            if (D.h_scal[S_DONE] != 0.0) break;
+            cg_iterations<double, HET>(D.be_grp, chunk, yb, B_R, B_Z, B_P0, B_Q, S_DONE, S_TOL2, M.opt.max_iters);
+            chunk = std::min(chunk * 2, 64);
+        }
+        M.stats.be_iters = (int)D.h_scal[S_ITERS];
     }
-    write_scal(D, {{S_DONE, 0.0}});   // the CG start skips a problem still marked done
-    jpcg64_start<HET>(D.be_grp, rtol2, M.opt.max_iters, yb, B_T1);
-    int chunk = 8;
-    for (;;) {
-        read_scal(D);
-        if (D.h_scal[S_DONE] != 0.0) break;
-        cg_iterations<double, HET>(D.be_grp, chunk, yb, B_R, B_Z, B_P0, B_Q, S_DONE, S_TOL2, M.opt.max_iters);
-        chunk = std::min(chunk * 2, 64);
+    // a step stopped at its iteration limits far above its tolerance
+    const double rel = D.h_scal[S_BB] > 0 ? std::sqrt(D.h_scal[S_RR] / D.h_scal[S_BB]) : 0.0;
+    if (!(rel <= std::max(100.0 * rtol, 1e-9))) {
+        char msg[160];
+        snprintf(msg, sizeof msg, "backward Euler step did not converge (relative residual %.2e after %d iterations)\n",
+                 rel, M.stats.be_iters);
+        fatal(msg);
     }
-    M.stats.be_iters = (int)D.h_scal[S_ITERS];
 }
 
 // the float4 term tiling of a single model, timed on its first FP32 Taylor use

@@ -1071,9 +1071,12 @@ RC_model_t *alloc_RC_model(thermal_config_t *config, flp_t *placeholder,
     HS_BEGIN
     if (!strcasecmp(config->model_type, "block")) return block_rc_alloc(config, placeholder);
     if (strcasecmp(config->model_type, "grid")) fatal("unknown model type\n");
-    if (config->grid_rows & (config->grid_rows - 1) || config->grid_cols & (config->grid_cols - 1) ||
-        config->grid_rows <= 0 || config->grid_cols <= 0)
-        fatal("grid rows and columns should both be powers of two\n");
+    // Any grid size, as the reference's SuperLU build takes (its iterative
+    // build asks for powers of two, temperature_grid.c:901-905, for its own
+    // multigrid): the tile kernels cover partial tiles, and multigrid
+    // coarsens while both sizes stay even.
+    if (config->grid_rows <= 0 || config->grid_cols <= 0)
+        fatal("grid rows and columns must be positive\n");
     h = (Handle *)calloc(1, sizeof(Handle));
     M = new Model();
     M->stamp = next_stamp();

@@ -346,9 +346,11 @@ class OracleModel:
         if lcf is not None:
             cfg["grid_layer_file"] = lcf
         self.cfg = cfg
+        # any grid size, as the reference's SuperLU build takes (its
+        # iterative build asks for powers of two, temperature_grid.c:901-905)
         self.nr, self.nc = int(cfg["grid_rows"]), int(cfg["grid_cols"])
-        if (self.nr & (self.nr - 1)) or (self.nc & (self.nc - 1)):
-            raise ValueError("grid rows and columns should both be powers of two")
+        if self.nr <= 0 or self.nc <= 0:
+            raise ValueError("grid rows and columns must be positive")
         self.secondary = bool(cfg["model_secondary"])
         self.detailed = bool(detailed_3D or cfg.get("detailed_3D_used", 0))
         self.use_mc = bool(use_microchannels)

@@ -1,7 +1,8 @@
 """Writes the command-line golden files under tests/golden/cli/: the inputs of
-the reference's examples 1-4 (floorplans, layer files, power traces, configs,
-materials: data files) and the outputs of the reference's own command-line
-simulator on them, run with the command lines of the examples' run.sh.
+the reference's examples 1-5 (floorplans, layer files, power traces, configs,
+materials, channel networks: data files) and the outputs of the reference's
+own command-line simulator on examples 1-4, run with the command lines of
+the examples' run.sh.
 
 Run in a container that holds the reference sources, after
 ``make -C oracle ref cli`` (which compiles the reference from its sources into
@@ -83,6 +84,24 @@ CASES = {
 }
 
 
+# inputs only: examples the reference's default build cannot run (example 5's
+# 39 x 39 microchannel stack needs its SuperLU build, absent here); the GPU
+# tests compare the command line's outputs on them with the oracle
+INPUT_ONLY = {
+    "example5_mc": ("example5", ["example.config", "example.lcf", "example.materials", "example.ptrace",
+                                 "floorplans/interposer.flp", "floorplans/silicon.flp", "floorplans/heater.flp",
+                                 "floorplans/DRAM.flp", "microchannel_geometries/horizontal.csv"]),
+}
+
+
+def copy_inputs(name, spec):
+    exdir, inputs = spec
+    for f in inputs:
+        dst = os.path.join(OUT, name, f)
+        os.makedirs(os.path.dirname(dst), exist_ok=True)
+        shutil.copy(os.path.join(REF, exdir, f), dst)
+
+
 # a package file of this build's own: natural convection over a fin-channel sink
 NATURAL = "# natural convection, fin-channel sink\n-natural_convec\t1\n-sink_type\t0\n-fin_height\t0.03\n" \
           "-fin_width\t0.001\n-channel_width\t0.002\n"
@@ -143,6 +162,9 @@ def main():
     for name, spec in CASES.items():
         if not only or name in only:
             run_case(name, spec)
+    for name, spec in INPUT_ONLY.items():
+        if not only or name in only:
+            copy_inputs(name, spec)
 
 
 if __name__ == "__main__":

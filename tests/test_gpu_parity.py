@@ -82,8 +82,14 @@ def _mc_case():
     return cases.microchannel_case(16, 5, tempfile.mkdtemp())
 
 
-@pytest.mark.parametrize("name,case,pw", CASES + [("microchannel-16",) + _mc_case()],
-                         ids=[c[0] for c in CASES] + ["microchannel-16"])
+# grid -> block in the min and max modes (temperature_grid.c:1932-2015) on a
+# floorplan whose block edges fall inside cells
+MAP_MODE_CASES = [(f"{mode}-nonaligned-16", cases.default_stack_case(16, nbx=3, nby=5, grid_map_mode=mode),
+                   [cases.rng(6).uniform(0.5, 5, 15)]) for mode in ("min", "max")]
+SLOPE_CASES = CASES + MAP_MODE_CASES + [("microchannel-16",) + _mc_case()]
+
+
+@pytest.mark.parametrize("name,case,pw", SLOPE_CASES, ids=[c[0] for c in SLOPE_CASES])
 def test_slope_and_maps_match_oracle(lib, name, case, pw):
     run = make(lib, case)
     om = oracle_for(run)

@@ -85,13 +85,20 @@ def test_errors_are_reported_not_fatal(lib):
     L = lib.lib
     L.hs_last_error.argtypes = [C.c_char_p, C.c_int]
     L.hs_last_error.restype = C.c_int
-    cfg = lib.default_config(model_type="grid", grid_rows=48, grid_cols=64)
+    cfg = lib.default_config(model_type="grid", grid_rows=0, grid_cols=64)
     tmp = tempfile.mkdtemp()
     fp = os.path.join(tmp, "a.flp")
     cases.write_flp(fp, cases.tiled_floorplan(2, 2, 0.01, 0.01))
     flp = L.read_flp(fp.encode(), 0, 0)
     m, msg = _expect_error(lib, lambda: L.alloc_RC_model(C.byref(cfg), flp, None, None, 0, 0))
-    assert not m and "powers of two" in msg
+    assert not m and "must be positive" in msg
+    # any positive size is taken (the reference's SuperLU build takes them)
+    cfg = lib.default_config(model_type="grid", grid_rows=39, grid_cols=48)
+    m = L.alloc_RC_model(C.byref(cfg), flp, None, None, 0, 0)
+    assert m
+    L.populate_R_model(m, flp)
+    assert m.contents.grid.contents.rows == 39 and m.contents.grid.contents.cols == 48
+    L.delete_RC_model(m)
     _, msg = _expect_error(lib, lambda: L.read_flp(os.path.join(tmp, "missing.flp").encode(), 0, 0))
     assert "missing.flp" in msg
     big = os.path.join(tmp, "big.flp")
