@@ -23,7 +23,7 @@ static double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-enum JobPhase { J_QUEUED = 0, J_STEADY = 1, J_TRANS = 2, J_DONE = 3 };
+enum JobPhase { J_QUEUED = 0, J_STEADY = 1, J_TRANS = 2, J_DONE = 3, J_FAILED = 4 };
 
 struct EngJob {
     int id = -1;
@@ -53,6 +53,14 @@ struct EngJob {
     bool use_mg = false;                // steady phase: multigrid-preconditioned solve
     double setup_s = 0.0;               // host time of the admission (device state, inputs)
     bool resident = false;              // admitted on its model's resident device state
+    // the temperature-leakage fixed point of the steady phase (the model's
+    // leakage_used, temperature.c:708-746): each solve's power adds the
+    // leakage at the last solve's block temperatures, the first at zero
+    bool leak = false;
+    LeakBlocks lb;                      // the power-layer blocks, copied at submission
+    std::vector<double> leak_old, leak_power;
+    int leak_iter = 0;                  // steady solves done
+    std::string error;                  // J_FAILED: the reason
 };
 
 // Kernel forms of the transient groups: homogeneous problems whose FP32
@@ -446,6 +454,45 @@ static void start_transient(Engine &E, EngJob &j) {
     B.dirty = true;
 }
 
+// Starts a steady solve from the state in B_STEADY (the ambient fill of the
+// admission, or the last solve of a leakage fixed point), with the solver
+// of the single-model API for the engine's form, and puts the job into its
+// steady group.
+static void start_steady(Engine &E, EngJob &j) {
+    DevModel &D = *j.D;
+    if (j.adv) {
+        // non-symmetric: GMRES with the upwind multigrid preconditioner
+        // in every form (Jacobi does not converge at channel Peclet numbers)
+        ensure_gmres(D, true);
+        build_mg(D);
+        write_scal(D, kMixedStart);
+    } else if (j.use_mg) {
+        build_mg(D);
+        write_scal(D, kMixedStart);
+    } else if (E.opt.precision == HS_PREC_MIXED) {
+        write_scal(D, kMixedStart);
+    } else {
+        write_scal(D, {{S_DONE, 0.0}});
+        const double rtol2 = E.opt.steady_rtol * E.opt.steady_rtol;
+        if (j.hetero) jpcg64_start<true>(D.grp, rtol2, E.opt.max_iters);
+        else jpcg64_start<false>(D.grp, rtol2, E.opt.max_iters);
+    }
+    HIP_OK(hipStreamSynchronize(D.stream));
+    j.phase = J_STEADY;
+    BatchGroup &B = E.steady[steady_key(j)];
+    B.members.push_back(&j);
+    B.dirty = true;
+}
+
+// A job that cannot finish: its device state is released and the reason
+// kept for hs_engine_poll.
+static void fail_job(Engine &E, EngJob &j, const std::string &why) {
+    j.error = why;
+    j.phase = J_FAILED;
+    j.t_done = now_s();
+    release_device(E, j);
+}
+
 static void admit(Engine &E, EngJob &j) {
     Model &M = *j.M;
     j.t_admit = now_s();
@@ -472,33 +519,21 @@ static void admit(Engine &E, EngJob &j) {
     j.held = D.bytes;
     E.used += j.held;
     if (j.init.empty()) {
-        upload_blocks(D, j.power.data());
+        if (j.leak) {
+            // the first solve's power: the leakage at zero block temperatures
+            std::vector<double> zero(nb, 0.0);
+            j.leak_old.assign(nb, 0.0);
+            j.leak_power.assign(nb, 0.0);
+            leakage_step(j.lb, j.power.data(), zero.data(), j.leak_power.data(), j.leak_old.data());
+            upload_blocks(D, j.leak_power.data());
+        } else {
+            upload_blocks(D, j.power.data());
+        }
         run_b2g(D.grp, V_POWER, B_RHS, 1);
         const int nbk = (int)((D.vlen + kBlock - 1) / kBlock);
         launch(k_fill, nbk, D.stream, D.h.b64[B_STEADY], D.vlen, M.g->config.ambient);
         j.use_mg = j.adv || E.opt.steady_solver == HS_STEADY_MGPCG;
-        if (j.adv) {
-            // non-symmetric: GMRES with the upwind multigrid preconditioner
-            // in every form (Jacobi does not converge at channel Peclet numbers)
-            ensure_gmres(D, true);
-            build_mg(D);
-            write_scal(D, kMixedStart);
-        } else if (j.use_mg) {
-            build_mg(D);
-            write_scal(D, kMixedStart);
-        } else if (E.opt.precision == HS_PREC_MIXED) {
-            write_scal(D, kMixedStart);
-        } else {
-            write_scal(D, {{S_DONE, 0.0}});
-            const double rtol2 = E.opt.steady_rtol * E.opt.steady_rtol;
-            if (j.hetero) jpcg64_start<true>(D.grp, rtol2, E.opt.max_iters);
-            else jpcg64_start<false>(D.grp, rtol2, E.opt.max_iters);
-        }
-        HIP_OK(hipStreamSynchronize(D.stream));
-        j.phase = J_STEADY;
-        BatchGroup &B = E.steady[steady_key(j)];
-        B.members.push_back(&j);
-        B.dirty = true;
+        start_steady(E, j);
     } else {
         upload_blocks(D, j.init.data());
         run_b2g(D.grp, V_TEMP, B_Y0, 0);
@@ -519,6 +554,31 @@ static void steady_done(Engine &E, EngJob &j, const double *flags) {
     j.steady_iters = (int)flags[1];
     j.refinements = refined ? (int)flags[2] : 0;
     run_g2b(D.grp, B_STEADY);
+    if (j.leak) {
+        // one step of the temperature-leakage fixed point (temperature.c:
+        // 708-746): converged, failed, or solved again from this state
+        std::vector<double> t(D.nblk);
+        download_blocks(D, t.data());
+        const double dmax = leakage_rise(j.lb, t.data(), j.leak_old.data());
+        const int it = j.leak_iter++;
+        if (!(dmax < kLeakTol)) {
+            if (dmax > kLeakTempHigh && it > 0) {
+                fail_job(E, j, "temperature is too high, possible thermal runaway. Double-check power inputs and "
+                               "package settings.\n");
+                return;
+            }
+            if (it + 1 > kLeakMaxIter) {
+                fail_job(E, j, "too many iterations before temperature-leakage convergence -- possible thermal "
+                               "runaway\n");
+                return;
+            }
+            leakage_step(j.lb, j.power.data(), t.data(), j.leak_power.data(), j.leak_old.data());
+            upload_blocks(D, j.leak_power.data());
+            run_b2g(D.grp, V_POWER, B_RHS, 1);
+            start_steady(E, j);
+            return;
+        }
+    }
     if (j.nsteps == 0) {
         j.result.resize(D.nblk);
         download_blocks(D, j.result.data());
@@ -798,6 +858,10 @@ int eng_submit(Engine &E, Model &M, const double *power, int rows, const double 
     j->dt = dt;
     j->hetero = M.detailed;
     j->adv = adv;
+    if (g.config.leakage_used && !init) {
+        j->leak = true;
+        j->lb = leak_blocks(M);
+    }
     j->footprint = dev_footprint(M, nsteps, !init && (E.opt.steady_solver == HS_STEADY_MGPCG || adv));
     if (j->footprint > E.budget)
         fatal("hs_engine_submit: the job needs " + std::to_string(j->footprint) +
@@ -852,6 +916,7 @@ static EngJob &job_of(Engine &E, int id) {
 
 int eng_poll(Engine &E, int id, double *out) {
     EngJob &j = job_of(E, id);
+    if (j.phase == J_FAILED) fatal(j.error);
     if (j.phase != J_DONE) return 0;
     if (out) memcpy(out, j.result.data(), j.result.size() * sizeof(double));
     return 1;
@@ -859,6 +924,7 @@ int eng_poll(Engine &E, int id, double *out) {
 
 int eng_result_seq(Engine &E, int id, double *out) {
     EngJob &j = job_of(E, id);
+    if (j.phase == J_FAILED) fatal(j.error);
     if (j.phase != J_DONE) return 0;
     if (j.nsteps == 0) fatal("hs_engine_result_seq: a steady job has no sequence\n");
     if (out) memcpy(out, j.seq.data(), j.seq.size() * sizeof(double));
@@ -874,18 +940,19 @@ void eng_job_info(Engine &E, int id, hs_job_info_t *info) {
     info->taylor_sweeps = (double)j.taylor_sweeps;
     info->taylor_node_bytes = j.taylor_node_bytes;
     info->wait_s = j.phase == J_QUEUED ? now_s() - j.t_submit : j.t_admit - j.t_submit;
-    info->latency_s = j.phase == J_DONE ? j.t_done - j.t_submit : now_s() - j.t_submit;
+    info->latency_s = j.phase >= J_DONE ? j.t_done - j.t_submit : now_s() - j.t_submit;
     info->device_bytes = (double)(j.held ? j.held : j.footprint);
     info->held_intervals = j.holds;
     info->horner_steps_min = j.kmin;
     info->horner_steps_plan = j.kplan;
     info->setup_s = j.setup_s;
     info->resident_model = j.resident ? 1 : 0;
+    info->steady_solves = j.leak ? j.leak_iter : (j.phase >= J_TRANS && j.init.empty() ? 1 : 0);
 }
 
 void eng_release(Engine &E, int id) {
     EngJob &j = job_of(E, id);
-    if (j.phase != J_DONE) fatal("hs_engine_release: the job has not finished\n");
+    if (j.phase != J_DONE && j.phase != J_FAILED) fatal("hs_engine_release: the job has not finished\n");
     E.jobs.erase(id);
 }
 

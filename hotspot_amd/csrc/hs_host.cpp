@@ -1496,10 +1496,16 @@ double calc_sink_temp(RC_model_t *model, double *temp) {
     HS_END(0.0)
 }
 
-double calc_leakage(int mode, double h, double w, double temp) {
-    HS_BEGIN
+// leakage power of a block (calc_leakage, temperature.c:910-930); throws
+// inside the library's own loops, so the error reaches their caller
+static double leakage_of(int mode, double h, double w, double temp) {
     if (mode) fatal("HotLeakage currently is not implemented in this release of HotSpot, please check back later.\n");
     return 1.5e+4 * h * w * std::exp(0.036 * (temp - 383.15));
+}
+
+double calc_leakage(int mode, double h, double w, double temp) {
+    HS_BEGIN
+    return leakage_of(mode, h, w, temp);
     HS_END(0.0)
 }
 
@@ -1561,7 +1567,7 @@ static void steady_block(block_model_t *b, double *power, double *temp) {
     for (int it = 0; !conv && it <= 100; it++) {
         for (int i = 0; i < n; i++) {
             const unit_t &u = b->flp->units[i];
-            power_new[i] = power[i] + calc_leakage(c.leakage_mode, u.height, u.width, temp[i]);
+            power_new[i] = power[i] + leakage_of(c.leakage_mode, u.height, u.width, temp[i]);
             temp_old[i] = temp[i];
         }
         block_steady(b, power_new.data(), temp);
@@ -1597,40 +1603,56 @@ void steady_state_temp(RC_model_t *model, double *power, double *temp) {
         int w;
         ~Restore() { m.opt.steady_warm_start = w; }
     } restore{M, warm};
-    std::vector<double> d_temp(tn, 0.0), temp_old(tn, 0.0), power_new(tn, 0.0);
+    std::vector<double> temp_old(tn, 0.0), power_new(tn, 0.0);
+    const LeakBlocks lb = leak_blocks(M);
     bool conv = false;
-    for (int it = 0; !conv && it <= 100; it++) {
-        int base = 0;
-        for (int k = 0; k < M.nl(); k++) {
-            const flp_t *f = M.layers[k].flp;
-            if (M.layers[k].has_power)
-                for (int j = 0; j < f->n_units; j++) {
-                    power_new[base + j] = power[base + j] +
-                                          calc_leakage(g.config.leakage_mode, f->units[j].height,
-                                                       f->units[j].width, temp[base + j]);
-                    temp_old[base + j] = temp[base + j];
-                }
-            base += f->n_units;
-        }
+    for (int it = 0; !conv && it <= kLeakMaxIter; it++) {
+        leakage_step(lb, power, temp, power_new.data(), temp_old.data());
         steady_grid(M, power_new.data(), temp);
-        double dmax = 0.0;
-        base = 0;
-        for (int k = 0; k < M.nl(); k++) {
-            const flp_t *f = M.layers[k].flp;
-            if (M.layers[k].has_power)
-                for (int j = 0; j < f->n_units; j++) {
-                    d_temp[base + j] = temp[base + j] - temp_old[base + j];
-                    dmax = std::max(dmax, d_temp[base + j]);
-                }
-            base += f->n_units;
-        }
-        if (dmax < 0.01) conv = true;
-        if (dmax > 500.0 && it > 0)
+        const double dmax = leakage_rise(lb, temp, temp_old.data());
+        if (dmax < kLeakTol) conv = true;
+        if (dmax > kLeakTempHigh && it > 0)
             fatal("temperature is too high, possible thermal runaway. Double-check power inputs and package settings.\n");
     }
     if (!conv)
         fatal("too many iterations before temperature-leakage convergence -- possible thermal runaway\n");
     HS_END()
+}
+
+extern "C++" {
+namespace hs {
+LeakBlocks leak_blocks(const Model &M) {
+    LeakBlocks b;
+    b.mode = M.g->config.leakage_mode;
+    int base = 0;
+    for (int k = 0; k < M.nl(); k++) {
+        const flp_t *f = M.layers[k].flp;
+        if (M.layers[k].has_power)
+            for (int j = 0; j < f->n_units; j++) {
+                b.idx.push_back(base + j);
+                b.h.push_back(f->units[j].height);
+                b.w.push_back(f->units[j].width);
+            }
+        base += f->n_units;
+    }
+    return b;
+}
+
+void leakage_step(const LeakBlocks &b, const double *power, const double *temp, double *power_new,
+                  double *temp_old) {
+    for (size_t q = 0; q < b.idx.size(); q++) {
+        const int i = b.idx[q];
+        power_new[i] = power[i] + leakage_of(b.mode, b.h[q], b.w[q], temp[i]);
+        temp_old[i] = temp[i];
+    }
+}
+
+double leakage_rise(const LeakBlocks &b, const double *temp, const double *temp_old) {
+    double dmax = 0.0;
+    for (int i : b.idx) dmax = std::max(dmax, temp[i] - temp_old[i]);
+    return dmax;
+}
+}  // namespace hs
 }
 
 void compute_temp(RC_model_t *model, double *power, double *temp, double time_elapsed) {

@@ -131,6 +131,23 @@ Model *model_of(grid_model_t *g);
 
 // host helpers used by the device side
 void layer_bases(const Model &M, std::vector<int> &base);   // block-vector offset per layer
+// The temperature-leakage fixed point of the grid model (temperature.c:
+// 708-746), one step at a time: leakage_step writes power + the leakage of
+// each power-layer block at temp into power_new and copies temp into
+// temp_old; leakage_rise is the largest rise of a power-layer block since
+// temp_old.  The loop stops when the rise is below kLeakTol, and fails above
+// kLeakMaxIter steps or when a rise after the first exceeds kLeakTempHigh.
+constexpr double kLeakTol = 0.01, kLeakTempHigh = 500.0;
+constexpr int kLeakMaxIter = 100;
+struct LeakBlocks {      // the power-layer blocks: block-vector index, height, width
+    int mode = 0;        // leakage_mode
+    std::vector<int> idx;
+    std::vector<double> h, w;
+};
+LeakBlocks leak_blocks(const Model &M);
+void leakage_step(const LeakBlocks &b, const double *power, const double *temp, double *power_new,
+                  double *temp_old);
+double leakage_rise(const LeakBlocks &b, const double *temp, const double *temp_old);
 
 // ---- device engine entry points (hs_device.hip) ---------------------------
 void dev_release(Model &M);

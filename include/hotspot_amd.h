@@ -554,8 +554,11 @@ void hs_engine_set_budget(hs_engine_t *e, double bytes);
  * n_steps == 0: a steady solve (steady_state_temp); n_steps > 0: n_steps
  * intervals of dt (compute_temp) from init_temp (a block vector) or, when
  * init_temp is NULL, from the block temperatures of a steady solve run
- * first.  Inputs are copied.  Returns the job id, or -1 on error (e.g. a
- * footprint above the budget). */
+ * first.  Inputs are copied.  On a model with leakage_used the steady
+ * solve is the temperature-leakage fixed point of steady_state_temp
+ * (temperature.c:708-746) from zero block temperatures (the command line's
+ * fresh temperature vector), each solve warm-started from the last.
+ * Returns the job id, or -1 on error (e.g. a footprint above the budget). */
 int hs_engine_submit(hs_engine_t *e, RC_model_t *model, const double *power, int power_rows,
                      const double *init_temp, int n_steps, double dt);
 /* Admits queued jobs while they fit (their inputs are then resident on the
@@ -573,7 +576,8 @@ int hs_engine_poll(hs_engine_t *e, int job, double *out);
  * finished transient job; 1 when copied, 0 while it runs, -1 on error */
 int hs_engine_result_seq(hs_engine_t *e, int job, double *out);
 typedef struct hs_job_info_st {
-    int state;                /* 0 queued, 1 steady phase, 2 transient phase, 3 done */
+    int state;                /* 0 queued, 1 steady phase, 2 transient phase, 3 done, 4 failed
+                               * (hs_engine_poll reports the reason) */
     int steady_iters;         /* CG iterations (inner ones for mixed precision) */
     int steady_refinements;
     int rk4_calls;            /* RK4 step attempts over the whole sequence */
@@ -591,6 +595,8 @@ typedef struct hs_job_info_st {
     double setup_s;           /* host time of its admission: device state and inputs */
     int resident_model;       /* 1: admitted on the device state its model kept from an earlier job
                                * of this engine (coefficients, maps, multigrid levels not rebuilt) */
+    int steady_solves;        /* steady solves of its steady phase: those of the temperature-leakage
+                               * fixed point for a model with leakage_used, else 1 (0: none yet) */
 } hs_job_info_t;
 int hs_engine_job_info(hs_engine_t *e, int job, hs_job_info_t *info);
 /* frees a finished job's results */

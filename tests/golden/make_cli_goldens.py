@@ -53,6 +53,10 @@ CASES = {
          "-steady_file gcc.steady -grid_steady_file gcc.grid.steady", ["gcc.steady", "gcc.grid.steady"]),
         ("-c example.config -init_file @init -f ev6.flp -p gcc.ptrace -materials_file example.materials "
          "-model_type grid -o gcc.ttrace", ["gcc.ttrace"]),
+        # the grid model's temperature-leakage fixed point (temperature.c:708-746)
+        ("-c example.config -f ev6.flp -p gcc.ptrace -materials_file example.materials -model_type grid "
+         "-leakage_used 1 -steady_file leak.steady -grid_steady_file leak.grid.steady",
+         ["leak.steady", "leak.grid.steady"]),
     ]),
     "ev6_package": ("example2", ["ev6.flp", "gcc.ptrace", "example.config", "example.materials",
                                  "@example1/package.config", "@natural.config"], [
