@@ -11,7 +11,11 @@
 // Each problem's arithmetic is the same as when it is solved alone
 // (per-problem scalars, per-problem stopping, reductions in a fixed slot
 // order), so a job's result does not depend on what it is batched with or
-// when it arrived.  Included at the end of hs_engine.inl.
+// when it arrived.  Backward-Euler transients (the integrator of the
+// reference's SuperLU build) advance one implicit solve per interval: the
+// solves of every such job run in shared launches, and the host moves a job
+// to its next interval when its solve has converged.  Included at the end of
+// hs_engine.inl.
 #include <chrono>
 #include <deque>
 #include <map>
@@ -60,6 +64,7 @@ struct EngJob {
     LeakBlocks lb;                      // the power-layer blocks, copied at submission
     std::vector<double> leak_old, leak_power;
     int leak_iter = 0;                  // steady solves done
+    long be_iters = 0;                  // backward Euler: solver iterations over its intervals
     std::string error;                  // J_FAILED: the reason
 };
 
@@ -165,7 +170,7 @@ struct Engine {
     int tick_intervals = 32;            // most transient intervals per chunk
     int tick_iters = 128;               // steady iterations per chunk
     // s_setup: admission, phase changes and results (never waits on a chunk)
-    hipStream_t s_setup = nullptr, s_steady[kSteadyStreams] = {}, s_trans[kTransParts] = {};
+    hipStream_t s_setup = nullptr, s_steady[kSteadyStreams] = {}, s_trans[kTransParts] = {}, s_be = nullptr;
     std::deque<EngJob *> queue;
     std::map<int, std::unique_ptr<EngJob>> jobs;
     std::vector<Resident> resident;     // idle device models of ended jobs
@@ -177,12 +182,15 @@ struct Engine {
     // streams, [form + kTransForms * part]
     std::map<int, BatchGroup> steady;
     BatchGroup trans[kTransForms * kTransParts];
+    // backward-Euler transients: the interval solves, keyed as steady groups
+    std::map<int, BatchGroup> be;
     ~Engine() {
         for (hipStream_t s : streams())
             if (s) (void)hipStreamSynchronize(s);
         for (auto &kv : jobs) kv.second->D.reset();
         resident.clear();
         for (auto &kv : steady) kv.second.release();
+        for (auto &kv : be) kv.second.release();
         for (auto &g : trans) g.release();
         for (hipStream_t s : streams())
             if (s) (void)hipStreamDestroy(s);
@@ -191,12 +199,14 @@ struct Engine {
         std::vector<hipStream_t> v{s_setup};
         v.insert(v.end(), s_steady, s_steady + kSteadyStreams);
         v.insert(v.end(), s_trans, s_trans + kTransParts);
+        v.push_back(s_be);
         return v;
     }
 };
 
-// Device bytes dev_build allocates for a model, plus the sequence buffers.
-static size_t dev_footprint(const Model &M, int nsteps, bool mg) {
+// Device bytes dev_build allocates for a model, plus the sequence buffers
+// and, for a backward-Euler transient (be), the shifted operator's arrays.
+static size_t dev_footprint(const Model &M, int nsteps, bool mg, bool be) {
     const grid_model_t &g = *M.g;
     ProbDesc t{};
     t.nl = g.n_layers;
@@ -219,6 +229,9 @@ static size_t dev_footprint(const Model &M, int nsteps, bool mg) {
         b += (ncell / 3 + 64 * 8) * 7 * 4 + (size_t)g.n_layers * 4096;
     if (adv)   // GMRES state, partials and one basis (FP32 or FP64)
         b += GM_SIZE * 8 + (size_t)(t.ntiles + 1) * (kGmM + 1) * 8 + (size_t)(kGmM + 1) * vlen * 8;
+    if (be && M.detailed) b += ncell * 16;   // per-cell shift and shifted Jacobi diagonal
+    if (be && adv)   // the shifted multigrid levels: per-cell shift (FP64) and 1/diag (FP32)
+        b += (ncell + ncell / 3 + 64 * 8) * 12 + (ncell / 3 + 64 * 8) * 7 * 4 + (size_t)g.n_layers * 4096;
     // pool size classes round each allocation up by at most an eighth
     return b + b / 8 + (64 << 10);
 }
@@ -232,8 +245,9 @@ template <typename T> static void ensure_dev(DevPool *pool, T *&p, size_t &cap, 
 
 // Uploads the group's descriptors and work-lists after a membership change
 // (on the set-up stream: the group's own stream may carry another group's
-// chunk, and this group is between chunks).
-static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans) {
+// chunk, and this group is between chunks).  A backward-Euler group (be)
+// takes each member's shifted descriptor and shifted multigrid levels.
+static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans, bool be = false) {
     if (!B.dirty) return;
     struct Timed {
         double &acc;
@@ -249,7 +263,7 @@ static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans) {
     bool v4 = true;
     for (int i = 0; i < n; i++) {
         const DevModel &D = *B.members[i]->D;
-        desc[i] = D.h;
+        desc[i] = be ? D.be_h : D.h;
         v4 = v4 && vec4_ok(D.h);
         for (int t = 0; t <= D.h.ntiles; t++) w.push_back(WorkItem{i, t});
         for (int u = 0; u < D.h.nblocks; u += kBlock / kWave) gw.push_back(WorkItem{i, u});
@@ -306,11 +320,14 @@ static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans) {
         B.g.tg = term_groups(desc);
         B.g.zfirst = false;
     }
-    // multigrid levels of a steady group (every member has the same count)
+    // multigrid levels of a steady group (every member has the same count),
+    // or of a backward-Euler group of non-symmetric steps
     for (void *p : B.lv_bufs) dfree(B.pool, p);
     B.lv_bufs.clear();
     B.lv.clear();
-    const int L = (!trans && n && B.members[0]->use_mg) ? (int)B.members[0]->D->mg.size() : 0;
+    const int L = !n ? 0
+                  : be ? (B.members[0]->adv ? (int)B.members[0]->D->be_mg.size() : 0)
+                       : (!trans && B.members[0]->use_mg) ? (int)B.members[0]->D->mg.size() : 0;
     for (int l = 0; l < L; l++) {
         std::vector<ProbDesc> ld(n);
         std::vector<WorkItem> lw;
@@ -318,8 +335,9 @@ static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans) {
         for (int i = 0; i < n; i++) {
             const DevModel &D = *B.members[i]->D;
             // level 0 is the model itself, but for the upwind advection of
-            // a non-symmetric model's preconditioner
-            ld[i] = l == 0 && !D.h.has_adv ? D.h : D.mg[l].h;
+            // a non-symmetric model's preconditioner; the shifted levels of
+            // a backward-Euler step start from the upwind level 0
+            ld[i] = be ? D.be_mg[l].h : l == 0 && !D.h.has_adv ? D.h : D.mg[l].h;
             nlmax = std::max(nlmax, ld[i].nl);
             auto ci = column_items(ld[i], i);
             lw.insert(lw.end(), ci.begin(), ci.end());
@@ -330,7 +348,7 @@ static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans) {
         B.lv_bufs.push_back(dw);
         h2d(dd, ld.data(), n * sizeof(ProbDesc), s);
         h2d(dw, lw.data(), lw.size() * sizeof(WorkItem), s);
-        B.lv.push_back(MgGroup{dd, dw, (int)lw.size(), nlmax, l > 0 || het});
+        B.lv.push_back(MgGroup{dd, dw, (int)lw.size(), nlmax, l > 0 || het || be});
     }
     B.dirty = false;
 }
@@ -428,10 +446,68 @@ static int steady_key(const EngJob &j) {
 
 static int trans_form(const EngJob &j) { return j.hetero ? 2 : vec4_ok(j.D->h) ? 0 : 1; }
 
+// ---- backward Euler (RCutil.c:268-428 as compute_temp_grid calls it,
+// temperature_grid.c:3632-3662): interval k solves
+//   (C/h + G) y' = C y / h + P_k + sources
+// from y, in place (y stays in B_Y0: the step never flips the state's
+// parity), with the solver of the single-model API: FP64 Jacobi CG for
+// symmetric G, GMRES right-preconditioned by the shifted upwind multigrid
+// V-cycle inside FP64 refinement for coolant advection; relative residual
+// 1e-2 x the steady tolerance.
+static double be_rtol(const Engine &E) { return 1e-2 * E.opt.steady_rtol; }
+
+// Starts the solve of the job's current interval (S_STEP): its power, the
+// right-hand side from the state, the solver's scalars.  On the set-up
+// stream, synchronised: the job's group is between chunks.
+static void be_begin(Engine &E, EngJob &j) {
+    DevModel &D = *j.D;
+    run_b2g(D.grp, V_POWER, B_RHS, 1, 1);
+    if (j.hetero)
+        launch_named("k_be_rhs", k_be_rhs<true>, D.grp.nwork, D.stream, (const ProbDesc *)D.d_desc,
+                     (const WorkItem *)D.d_work, (int)B_Y0, (int)B_RHS, (int)B_T1, 1.0 / j.dt);
+    else
+        launch_named("k_be_rhs", k_be_rhs<false>, D.grp.nwork, D.stream, (const ProbDesc *)D.d_desc,
+                     (const WorkItem *)D.d_work, (int)B_Y0, (int)B_RHS, (int)B_T1, 1.0 / j.dt);
+    if (j.adv) {
+        write_scal(D, kMixedStart);
+    } else {
+        write_scal(D, {{S_DONE, 0.0}});   // the CG start skips a problem still marked done
+        const double r = be_rtol(E);
+        if (j.hetero) jpcg64_start<true>(D.be_grp, r * r, E.opt.max_iters, B_Y0, B_T1);
+        else jpcg64_start<false>(D.be_grp, r * r, E.opt.max_iters, B_Y0, B_T1);
+    }
+    HIP_OK(hipStreamSynchronize(D.stream));
+}
+
+static int be_key(const EngJob &j) { return (j.adv ? 2 : j.hetero ? 1 : 0) + 4 * (j.adv ? (int)j.D->be_mg.size() : 0); }
+
+// The shifted operator of the job's step h (once per job: h is the job's
+// interval), then its first interval's solve; the job joins its group.
+static void start_be(Engine &E, EngJob &j) {
+    DevModel &D = *j.D;
+    if (j.adv) ensure_gmres(D, true);   // the shifted descriptor copies its buffers
+    build_be(D, j.dt, j.adv ? (int)B_Y0 : -1, j.adv ? (int)B_T1 : -1);
+    if (j.adv) be_levels(D, 1.0 / j.dt);
+    // device memory held, the shifted operator's arrays included
+    E.used += D.bytes - j.held;
+    j.held = D.bytes;
+    be_begin(E, j);
+    BatchGroup &B = E.be[be_key(j)];
+    B.members.push_back(&j);
+    B.dirty = true;
+}
+
 static void start_transient(Engine &E, EngJob &j) {
     DevModel &D = *j.D;
     auto kv = seq_slots(j.nsteps, j.dt);
     kv.push_back({S_YPAR, 0.0});
+    if (E.opt.transient == HS_TRANS_BE) {
+        write_scal(D, kv);
+        j.phase = J_TRANS;
+        j.step_seen = 0;
+        start_be(E, j);
+        return;
+    }
     if (E.opt.transient == HS_TRANS_TAYLOR) {
         const TaylorPlan p = taylor_plan(D, j.dt, E.opt.taylor_tol);
         const auto t = taylor_slots(p, D.rho);
@@ -607,6 +683,39 @@ static void trans_done(Engine &E, EngJob &j, const double *flags) {
     release_device(E, j);
 }
 
+// A converged interval solve of a backward-Euler job (flags: S_DONE,
+// iterations, refinements, r.r, b.b): its block temperatures go into the
+// sequence, and the job starts its next interval or finishes.  Returns
+// whether the job leaves its group.
+static bool be_step_done(Engine &E, EngJob &j, const double *flags) {
+    DevModel &D = *j.D;
+    j.be_iters += (long)flags[1];
+    // a step stopped at its iteration limits far above its tolerance
+    const double rel = flags[4] > 0 ? std::sqrt(flags[3] / flags[4]) : 0.0;
+    if (!(rel <= std::max(100.0 * be_rtol(E), 1e-9))) {
+        char msg[160];
+        snprintf(msg, sizeof msg, "backward Euler step did not converge (relative residual %.2e after %d iterations)\n",
+                 rel, (int)flags[1]);
+        fail_job(E, j, msg);
+        return true;
+    }
+    run_g2b(D.grp, B_Y0, 1);
+    j.step_seen++;
+    write_scal(D, {{S_STEP, (double)j.step_seen}});
+    if (j.step_seen < j.nsteps) {
+        be_begin(E, j);
+        return false;
+    }
+    const size_t n = (size_t)j.nsteps * D.nblk;
+    j.seq.resize(n);
+    d2h(j.seq.data(), D.h.seq_out, n * sizeof(double), D.stream);
+    j.result.assign(j.seq.end() - D.nblk, j.seq.end());
+    j.phase = J_DONE;
+    j.t_done = now_s();
+    release_device(E, j);
+    return true;
+}
+
 // One round of a steady group: the unit that a chunk replays (captured once
 // per membership as a graph).
 template <bool HET>
@@ -624,6 +733,15 @@ static void steady_round(Engine &E, const Group &G, bool adv, const std::vector<
     } else {
         cg_iterations<double, HET>(G, 64, B_STEADY, B_R, B_Z, B_P0, B_Q, S_DONE, S_TOL2, E.opt.max_iters);
     }
+}
+
+// One round of a backward-Euler group, as the single-model step iterates
+// (kind 2: non-symmetric GMRES cycles with the shifted multigrid levels)
+template <bool HET>
+static void be_round(Engine &E, const Group &G, int kind, const std::vector<MgGroup> &lv) {
+    const double r = be_rtol(E);
+    if (kind == 2) gm_round_mg<true>(G, lv, r * r, 1e-8, kGmM);
+    else cg_iterations<double, HET>(G, 64, B_Y0, B_R, B_Z, B_P0, B_Q, S_DONE, S_TOL2, E.opt.max_iters);
 }
 
 // rounds of a chunk: one per 64 of the steady iterations per chunk, one per
@@ -662,6 +780,27 @@ static void launch_chunks(Engine &E) {
         gather(B, {S_DONE, refined ? S_TITERS : S_ITERS, S_NREFINE});
         HIP_OK(hipEventRecord(B.ev_b, B.stream));
         E.stats.steady_chunks++;
+        B.ran = B.g.nprob;
+        B.busy = true;
+    }
+    // backward-Euler solves on their own stream
+    for (auto &kv : E.be) {
+        BatchGroup &B = kv.second;
+        if (B.busy || B.members.empty()) continue;
+        const int kind = kv.first % 4;
+        B.pool = &E.dpool;
+        B.hpool = &E.hpool;
+        B.stream = E.s_be;
+        group_sync(E, B, kind > 0, false, true);
+        group_events(B);
+        HIP_OK(hipEventRecord(B.ev_a, B.stream));
+        B.rp[0].run(B.stream, steady_rounds(E, kind == 2, !B.lv.empty()), [&] {
+            if (kind) be_round<true>(E, B.g, kind, B.lv);
+            else be_round<false>(E, B.g, kind, B.lv);
+        });
+        gather(B, {S_DONE, kind == 2 ? S_TITERS : S_ITERS, S_NREFINE, S_RR, S_BB});
+        HIP_OK(hipEventRecord(B.ev_b, B.stream));
+        E.stats.transient_chunks++;
         B.ran = B.g.nprob;
         B.busy = true;
     }
@@ -730,6 +869,10 @@ static void wait_chunk(Engine &E) {
             busy |= B.busy;
             done |= chunk_done(B);
         }
+        for (auto &kv : E.be) {
+            busy |= kv.second.busy;
+            done |= chunk_done(kv.second);
+        }
         if (!busy || done || now_s() - t0 > kWaitMs * 1e-3) return;
         std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
@@ -738,6 +881,19 @@ static void wait_chunk(Engine &E) {
 // results of every group whose chunk has ended: finished jobs leave their
 // groups (a steady solve with intervals to follow joins a transient group)
 static void harvest(Engine &E) {
+    for (auto &kv : E.be) {
+        BatchGroup &B = kv.second;
+        if (!chunk_done(B)) continue;
+        B.busy = false;
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, B.ev_a, B.ev_b));
+        E.stats.transient_chunk_s += 1e-3 * ms;
+        std::vector<EngJob *> fin;
+        for (int i = 0; i < B.ran; i++)
+            if (B.h_flags[i * 8] != 0.0 && be_step_done(E, *B.members[i], B.h_flags + i * 8))
+                fin.push_back(B.members[i]);
+        for (EngJob *j : fin) B.remove(j);
+    }
     for (auto &kv : E.steady) {
         BatchGroup &B = kv.second;
         if (!chunk_done(B)) continue;
@@ -804,6 +960,7 @@ Engine *eng_create(int precision, int transient) {
     HIP_OK(hipStreamCreateWithFlags(&E->s_setup, hipStreamNonBlocking));
     for (auto &s : E->s_steady) HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     for (auto &s : E->s_trans) HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIP_OK(hipStreamCreateWithFlags(&E->s_be, hipStreamNonBlocking));
     return E.release();
 }
 
@@ -813,6 +970,7 @@ int eng_active(Engine &E) {
     int n = (int)E.queue.size();
     for (auto &kv : E.steady) n += (int)kv.second.members.size();
     for (auto &g : E.trans) n += (int)g.members.size();
+    for (auto &kv : E.be) n += (int)kv.second.members.size();
     return n;
 }
 
@@ -862,7 +1020,8 @@ int eng_submit(Engine &E, Model &M, const double *power, int rows, const double 
         j->leak = true;
         j->lb = leak_blocks(M);
     }
-    j->footprint = dev_footprint(M, nsteps, !init && (E.opt.steady_solver == HS_STEADY_MGPCG || adv));
+    j->footprint = dev_footprint(M, nsteps, !init && (E.opt.steady_solver == HS_STEADY_MGPCG || adv),
+                                 nsteps > 0 && E.opt.transient == HS_TRANS_BE);
     if (j->footprint > E.budget)
         fatal("hs_engine_submit: the job needs " + std::to_string(j->footprint) +
               " bytes of device memory, above the engine budget of " + std::to_string(E.budget) + "\n");
@@ -948,6 +1107,7 @@ void eng_job_info(Engine &E, int id, hs_job_info_t *info) {
     info->setup_s = j.setup_s;
     info->resident_model = j.resident ? 1 : 0;
     info->steady_solves = j.leak ? j.leak_iter : (j.phase >= J_TRANS && j.init.empty() ? 1 : 0);
+    info->be_iters = (double)j.be_iters;
 }
 
 void eng_release(Engine &E, int id) {

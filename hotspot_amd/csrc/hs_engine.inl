@@ -174,6 +174,7 @@ struct DevModel {
     // backward Euler: descriptor of the shifted operator G + C/h (its group
     // shares the work-lists), per-cell shift and Jacobi diagonal (hetero)
     ProbDesc *d_be_desc = nullptr;
+    ProbDesc be_h{};                    // host copy of *d_be_desc
     Group be_grp;
     double *be_shift = nullptr, *be_dinv = nullptr;
     std::vector<MgLevel> be_mg;         // multigrid levels of the shifted step (non-symmetric steps)
@@ -1843,7 +1844,8 @@ static void build_be(DevModel &D, double h, int xb = -1, int rhsb = -1) {
         b.dinvc = D.be_dinv;
     }
     if (!D.d_be_desc) D.d_be_desc = D.alloc<ProbDesc>(1);
-    HIP_OK(hipMemcpyAsync(D.d_be_desc, &b, sizeof(ProbDesc), hipMemcpyHostToDevice, D.stream));
+    D.be_h = b;
+    HIP_OK(hipMemcpyAsync(D.d_be_desc, &D.be_h, sizeof(ProbDesc), hipMemcpyHostToDevice, D.stream));
     HIP_OK(hipStreamSynchronize(D.stream));
     D.be_grp = D.grp;
     D.be_grp.d_desc = D.d_be_desc;

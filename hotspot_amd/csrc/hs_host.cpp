@@ -1781,8 +1781,8 @@ static hs::Engine &engine_of(hs_engine_t *e) {
 hs_engine_t *hs_engine_create(int precision, int transient) {
     HS_BEGIN
     if (precision != HS_PREC_FP64 && precision != HS_PREC_MIXED) fatal("unknown precision\n");
-    if (transient != HS_TRANS_RK4 && transient != HS_TRANS_TAYLOR)
-        fatal("the engine integrates with HS_TRANS_RK4 or HS_TRANS_TAYLOR\n");
+    if (transient != HS_TRANS_RK4 && transient != HS_TRANS_TAYLOR && transient != HS_TRANS_BE)
+        fatal("unknown transient integrator\n");
     std::unique_ptr<hs_engine_st> e(new hs_engine_st{nullptr});
     e->impl = eng_create(precision, transient);
     return e.release();
@@ -1803,8 +1803,8 @@ void hs_engine_set_options(hs_engine_t *e, const hs_options_t *opt) {
     if (opt->precision != HS_PREC_FP64 && opt->precision != HS_PREC_MIXED) fatal("unknown precision\n");
     if (opt->steady_solver != HS_STEADY_JPCG && opt->steady_solver != HS_STEADY_MGPCG)
         fatal("unknown steady solver\n");
-    if (opt->transient != HS_TRANS_RK4 && opt->transient != HS_TRANS_TAYLOR)
-        fatal("the engine integrates with HS_TRANS_RK4 or HS_TRANS_TAYLOR\n");
+    if (opt->transient != HS_TRANS_RK4 && opt->transient != HS_TRANS_TAYLOR && opt->transient != HS_TRANS_BE)
+        fatal("unknown transient integrator\n");
     if (!(opt->steady_rtol > 0) || opt->max_iters <= 0 || !(opt->taylor_tol > 0)) fatal("invalid solver tolerances\n");
     eng_set_options(engine_of(e), *opt);
     HS_END()

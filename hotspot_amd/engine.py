@@ -10,7 +10,7 @@ import numpy as np
 from . import iface
 
 FP64, MIXED = 0, 1
-RK4, TAYLOR = 0, 1
+RK4, TAYLOR, BE = 0, 1, 2
 QUEUED, STEADY, TRANSIENT, DONE, FAILED = 0, 1, 2, 3, 4
 
 
@@ -27,7 +27,7 @@ class JobInfo(C.Structure):
                 ("rk4_calls", C.c_int), ("taylor_sweeps", C.c_double), ("wait_s", C.c_double), ("latency_s", C.c_double),
                 ("device_bytes", C.c_double), ("taylor_node_bytes", C.c_double), ("held_intervals", C.c_int),
                 ("horner_steps_min", C.c_int), ("horner_steps_plan", C.c_int), ("setup_s", C.c_double),
-                ("resident_model", C.c_int), ("steady_solves", C.c_int)]
+                ("resident_model", C.c_int), ("steady_solves", C.c_int), ("be_iters", C.c_double)]
 
 
 class EngineStats(C.Structure):

@@ -536,7 +536,11 @@ int hs_kernel_timer_read(double *avg_ms, double *total_ms);
  * groups, and an arriving job joins its group at the group's next chunk.
  * A job's result is the same as that of the same solve run alone.  The
  * engine's options (hs_engine_set_options) apply to every job; a model must
- * stay allocated and unchanged until its jobs are admitted. */
+ * stay allocated and unchanged until its jobs are admitted.  transient is
+ * HS_TRANS_RK4, HS_TRANS_TAYLOR or HS_TRANS_BE; with HS_TRANS_BE every
+ * interval of a job is one implicit solve, the solves of all such jobs
+ * sharing launches, and a job moves to its next interval when its solve has
+ * converged (a solve stopped far above its tolerance fails the job). */
 typedef struct hs_engine_st hs_engine_t;
 hs_engine_t *hs_engine_create(int precision, int transient);
 void hs_engine_destroy(hs_engine_t *e);
@@ -597,6 +601,7 @@ typedef struct hs_job_info_st {
                                * of this engine (coefficients, maps, multigrid levels not rebuilt) */
     int steady_solves;        /* steady solves of its steady phase: those of the temperature-leakage
                                * fixed point for a model with leakage_used, else 1 (0: none yet) */
+    double be_iters;          /* backward Euler: solver iterations over its intervals so far */
 } hs_job_info_t;
 int hs_engine_job_info(hs_engine_t *e, int job, hs_job_info_t *info);
 /* frees a finished job's results */

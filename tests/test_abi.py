@@ -167,3 +167,32 @@ def test_c_caller_compiled_against_header(prod, tmp_path):
     out = subprocess.check_output([str(exe), str(fp)]).decode().split()
     assert out[:3] == ["4", "16", "0"]
     assert abs(float(out[3]) - 0.016 / 32 / (100.0 * 0.016 / 32 * 0.15e-3)) < 1e-12
+
+
+def test_extension_struct_layouts_match_mirrors(tmp_path):
+    """The structs the header adds (options, statistics, engine job and
+    engine statistics records) have the field offsets and sizes of the
+    ctypes mirrors that the engine wrapper, the tests and bench.py use."""
+    import bench
+    from hotspot_amd import engine
+    from tests import helpers
+    mirrors = {"hs_options_t": [engine.Options, helpers.HsOptions, bench.Opts],
+               "hs_stats_t": [helpers.HsStats, bench.Stats],
+               "hs_job_info_t": [engine.JobInfo], "hs_engine_stats_t": [engine.EngineStats]}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "hotspot_amd.h"', "int main(void) {"]
+    for t, ms in mirrors.items():
+        lines.append(f'  printf("{t} %zu\\n", sizeof({t}));')
+        for f in ms[0]._fields_:
+            lines.append(f'  printf("{t}.{f[0]} %zu\\n", offsetof({t}, {f[0]}));')
+    lines += ["  return 0;", "}"]
+    out = run_probe("\n".join(lines) + "\n", ["-I" + os.path.join(ROOT, "include")], str(tmp_path))
+    n = 0
+    for line in out:
+        if not line:
+            continue
+        key, val = line.split()
+        t, _, f = key.partition(".")
+        for m in mirrors[t]:
+            assert (getattr(m, f).offset if f else C.sizeof(m)) == int(val), (m.__name__, key)
+            n += 1
+    assert n > 40

@@ -31,10 +31,10 @@ def lib(prod, gpu):
     return L
 
 
-def transient_run(lib, run, seq, init):
+def transient_run(lib, run, seq, init, dt=DT):
     seq = np.ascontiguousarray(seq)
     out = np.zeros_like(seq)
-    lib.lib.hs_transient_run(run.model, ptr(seq), seq.shape[0], DT, ptr(init) if init is not None else None,
+    lib.lib.hs_transient_run(run.model, ptr(seq), seq.shape[0], dt, ptr(init) if init is not None else None,
                              ptr(out))
     check_error(lib)
     return out
@@ -261,8 +261,7 @@ def test_engine_microchannel_steady_equals_single_model_api(lib, form):
 
 def test_backward_euler_sequence_equals_calls(lib):
     """hs_transient_run with the backward-Euler integrator gives, bit for
-    bit, what one compute_temp call per interval gives; the engine refuses
-    the integrator with an error."""
+    bit, what one compute_temp call per interval gives."""
     case = cases.default_stack_case(32)
     pw = [cases.rng(2).uniform(0.5, 5, 16)]
     a, b = make(lib, case), make(lib, case)
@@ -274,8 +273,6 @@ def test_backward_euler_sequence_equals_calls(lib):
     calls = cases.run_transient(a, init, list(seq), DT)
     got = transient_run(lib, b, seq, init)
     assert np.array_equal(got, calls)
-    with pytest.raises(RuntimeError, match="HS_TRANS_RK4 or HS_TRANS_TAYLOR"):
-        engine.Engine(lib, engine.MIXED, 2)
     a.free()
     b.free()
 
