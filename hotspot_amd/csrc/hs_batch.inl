@@ -63,8 +63,17 @@ struct EngJob {
     bool leak = false;
     LeakBlocks lb;                      // the power-layer blocks, copied at submission
     std::vector<double> leak_old, leak_power;
-    int leak_iter = 0;                  // steady solves done
+    int leak_iter = 0;                  // steady solves done (leakage or natural-convection fixed point)
     long be_iters = 0;                  // backward Euler: solver iterations over its intervals
+    // a steady solve under the package model (package.c:353-397): the
+    // sink's r_convec from the fan and sink (forced convection, one solve)
+    // or the natural-convection fixed point of hotspot.c:667-682, each
+    // re-solve warm-started from the last with r_convec from the average
+    // sink temperature; the device state's convection is then the job's own
+    bool package = false, natural = false;
+    package_config_t pkg{};
+    SinkModel sink;
+    double r_convec = 0.0;              // of the solve in progress, then of the model after the last
     std::string error;                  // J_FAILED: the reason
 };
 
@@ -374,6 +383,10 @@ static void release_device(Engine &E, EngJob &j) {
     D.release(D.h.seq_out);
     D.h.seq_in = D.h.seq_out = nullptr;
     D.h.seq_one = 0;
+    if (j.package) {   // coefficients of the job's own r_convec: not the model's
+        j.D.reset();
+        return;
+    }
     // kept under the stamp of the coefficients it was built from (the model
     // itself may be gone by now: a later model, or this one repopulated,
     // carries another stamp and never matches)
@@ -594,6 +607,12 @@ static void admit(Engine &E, EngJob &j) {
     }
     j.held = D.bytes;
     E.used += j.held;
+    if (j.package) {
+        double rz;
+        package_RC_t pk;
+        sink_resistances(j.sink, j.r_convec, rz, pk);
+        dev_set_convec(D, j.sink.hsidx, rz, pk);
+    }
     if (j.init.empty()) {
         if (j.leak) {
             // the first solve's power: the leakage at zero block temperatures
@@ -654,6 +673,45 @@ static void steady_done(Engine &E, EngJob &j, const double *flags) {
             start_steady(E, j);
             return;
         }
+    }
+    if (j.natural) {
+        // one step of the natural-convection fixed point: r_convec at the
+        // average sink temperature (+ kSmallForConvec); solved again, from
+        // this state, until r_convec moves by less than kNaturalConvecTol
+        std::vector<double> t(D.nblk);
+        download_blocks(D, t.data());
+        j.leak_iter++;
+        double r = 0.0, sink = 0.0;
+        try {
+            sink = sink_temp(j.sink, t.data()) + kSmallForConvec;
+            convection_t p{};
+            calc_natural_convec(&p, &j.pkg, &j.sink.c, sink);
+            r = p.r_th;
+        } catch (const Error &e) {
+            fail_job(E, j, e.msg);
+            return;
+        }
+        const double r_old = j.r_convec;
+        j.r_convec = r;
+        if (sink > kMaxSinkTemp) {
+            fail_job(E, j, "too high power for a natural convection package -- possible thermal runaway\n");
+            return;
+        }
+        if (!(std::fabs(r - r_old) < kNaturalConvecTol)) {
+            double rz;
+            package_RC_t pk;
+            sink_resistances(j.sink, r, rz, pk);
+            dev_set_convec(D, j.sink.hsidx, rz, pk);
+            upload_blocks(D, j.power.data());
+            run_b2g(D.grp, V_POWER, B_RHS, 1);
+            start_steady(E, j);
+            return;
+        }
+        j.result = t;
+        j.phase = J_DONE;
+        j.t_done = now_s();
+        release_device(E, j);
+        return;
     }
     if (j.nsteps == 0) {
         j.result.resize(D.nblk);
@@ -1032,6 +1090,32 @@ int eng_submit(Engine &E, Model &M, const double *power, int rows, const double 
     return id;
 }
 
+// A steady job under the package model: r_convec from the fan and sink
+// (forced convection), or the natural-convection fixed point starting at
+// the ambient temperature + kSmallForConvec (hotspot.c:447-455, :667-682).
+int eng_submit_package(Engine &E, Model &M, const double *power, const package_config_t &pkg) {
+    if (M.detailed) fatal("hs_engine_submit_package: a detailed-3D model's convection cannot change on the device\n");
+    for (auto *m : M.mc)
+        if (m) fatal("hs_engine_submit_package: a microchannel model's convection cannot change on the device\n");
+    if (M.g->config.leakage_used) fatal("hs_engine_submit_package: the model uses leakage\n");
+    const int id = eng_submit(E, M, power, 1, nullptr, 0, 0.0);
+    EngJob &j = *E.jobs[id];
+    j.package = true;
+    j.natural = pkg.natural_convec != 0;
+    j.pkg = pkg;
+    j.sink = sink_model(M);
+    convection_t p{};
+    if (j.natural) calc_natural_convec(&p, &j.pkg, &j.sink.c, j.sink.c.ambient + kSmallForConvec);
+    else calculate_flow(&p, &j.pkg, &j.sink.c);
+    j.r_convec = p.r_th;
+    if (!(j.r_convec > 0)) {
+        E.queue.pop_back();
+        E.jobs.erase(id);
+        fatal("hs_engine_submit_package: the package model gives no positive r_convec\n");
+    }
+    return id;
+}
+
 // admission: first come, first served while the footprint fits
 int eng_admit(Engine &E) {
     while (!E.queue.empty()) {
@@ -1106,7 +1190,8 @@ void eng_job_info(Engine &E, int id, hs_job_info_t *info) {
     info->horner_steps_plan = j.kplan;
     info->setup_s = j.setup_s;
     info->resident_model = j.resident ? 1 : 0;
-    info->steady_solves = j.leak ? j.leak_iter : (j.phase >= J_TRANS && j.init.empty() ? 1 : 0);
+    info->steady_solves = (j.leak || j.natural) ? j.leak_iter : (j.phase >= J_TRANS && j.init.empty() ? 1 : 0);
+    info->r_convec = j.package ? j.r_convec : 0.0;
     info->be_iters = (double)j.be_iters;
 }
 

@@ -540,6 +540,14 @@ static void pin_term_tiling(const Model &M, int shift, int rpt, int z) {
 // reduction slots a problem needs: layer tiles or column tiles, + package item
 static int slot_count(const ProbDesc &h) { return std::max(h.ntiles, h.nti2 * h.ntj) + 1; }
 
+// The sink's package nodes' conductances to ambient (the sink's r_convec
+// split by area; single_iteration_steady_pack :2120-2228).
+static void sink_ambient(ProbDesc &h, const package_RC_t &pk) {
+    for (int o : {SINK_N, SINK_S, SINK_W, SINK_E}) h.x_gamb[o] = 1.0 / (pk.r_hs_per + pk.r_amb_per);
+    h.x_gamb[SINK_C_N] = h.x_gamb[SINK_C_S] = 1.0 / (pk.r_hs_c_per_y + pk.r_amb_c_per_y);
+    h.x_gamb[SINK_C_W] = h.x_gamb[SINK_C_E] = 1.0 / (pk.r_hs_c_per_x + pk.r_amb_c_per_x);
+}
+
 // Layer and package-node coefficients of the descriptor from the host model.
 static void fill_coefficients(Model &M, ProbDesc &h) {
     grid_model_t &g = *M.g;
@@ -605,13 +613,11 @@ static void fill_coefficients(Model &M, ProbDesc &h) {
         h.x_cpl_steady_only[b][q] = 0;
     };
     // package network (single_iteration_steady_pack :2120-2228)
-    for (int o : {SINK_N, SINK_S, SINK_W, SINK_E}) h.x_gamb[o] = 1.0 / (pk.r_hs_per + pk.r_amb_per);
+    sink_ambient(h, pk);
     cpl(SINK_N, SINK_C_N, 1.0 / (pk.r_hs2_y + pk.r_hs));
     cpl(SINK_S, SINK_C_S, 1.0 / (pk.r_hs2_y + pk.r_hs));
     cpl(SINK_W, SINK_C_W, 1.0 / (pk.r_hs2_x + pk.r_hs));
     cpl(SINK_E, SINK_C_E, 1.0 / (pk.r_hs2_x + pk.r_hs));
-    h.x_gamb[SINK_C_N] = h.x_gamb[SINK_C_S] = 1.0 / (pk.r_hs_c_per_y + pk.r_amb_c_per_y);
-    h.x_gamb[SINK_C_W] = h.x_gamb[SINK_C_E] = 1.0 / (pk.r_hs_c_per_x + pk.r_amb_c_per_x);
     cpl(SINK_C_N, SP_N, 1.0 / pk.r_sp_per_y);
     cpl(SINK_C_S, SP_S, 1.0 / pk.r_sp_per_y);
     cpl(SINK_C_W, SP_W, 1.0 / pk.r_sp_per_x);
@@ -1323,6 +1329,37 @@ static void build_mg(DevModel &D) {
                            (const ProbDesc *)D.mg.back().d_desc, (const ProbDesc *)L.d_desc, (int)prev_het);
         prev_het = true;
         D.mg.push_back(L);
+    }
+    HIP_OK(hipStreamSynchronize(D.stream));
+}
+
+// The sink's convection resistance of a homogeneous model's device state
+// set to the one behind rz (the sink layer's vertical resistance to
+// ambient) and pk (the package resistances), as populate_R_model with that
+// r_convec would give them: the sink layer's ambient conductance, the sink
+// nodes' ones, and the multigrid levels coarsened again from them (coarse
+// ambient conductances sum the four children's, k_mg_coarsen recomputes the
+// coarse faces and diagonals).  The steady state stays: a re-solve starts
+// from it.
+static void dev_set_convec(DevModel &D, int hsidx, double rz, const package_RC_t &pk) {
+    if (D.hetero || D.h.has_adv) fatal("the sink's convection is set on the device for homogeneous models only\n");
+    ProbDesc &h = D.h;
+    h.gamb[hsidx] = 1.0 / rz;
+    if (hsidx < h.nl - 1) h.gz[hsidx] = 1.0 / rz;
+    sink_ambient(h, pk);
+    h2d(D.d_desc, &h, sizeof(ProbDesc), D.stream);
+    if (!D.mg.empty()) {
+        D.mg[0].h = h;
+        for (size_t l = 1; l < D.mg.size(); l++) {
+            const ProbDesc &f = D.mg[l - 1].h;
+            ProbDesc &c = D.mg[l].h;
+            for (int n = 0; n < c.nl; n++) c.gamb[n] = f.gamb[n] * 4.0;
+            for (int k = 0; k < kMaxExtra; k++) c.x_gamb[k] = f.x_gamb[k];
+            h2d(D.mg[l].d_desc, &c, sizeof(ProbDesc), D.stream);
+            const size_t ncell = (size_t)c.nl * c.nr * c.nc;
+            hipLaunchKernelGGL(k_mg_coarsen, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, D.stream,
+                               (const ProbDesc *)D.mg[l - 1].d_desc, (const ProbDesc *)D.mg[l].d_desc, l > 1 ? 1 : 0);
+        }
     }
     HIP_OK(hipStreamSynchronize(D.stream));
 }

@@ -1467,32 +1467,7 @@ double find_avg_temp(RC_model_t *model, double *temp) {
 double calc_sink_temp(RC_model_t *model, double *temp) {
     HS_BEGIN
     if (model && model->type == BLOCK_MODEL) return calc_sink_temp_block(model->block, temp, model->config);
-    Model &M = *model_of(model);
-    grid_model_t &g = *M.g;
-    const thermal_config_t &c = g.config;
-    int base = 0;
-    for (int n = 0; n < M.hsidx; n++) base += M.layers[n].flp->n_units;
-    const flp_t *f = M.layers[M.hsidx].flp;
-    const double spr = c.s_spreader * c.s_spreader, sink = c.s_sink * c.s_sink;
-    double sum = 0.0;
-    for (int i = base; i < base + f->n_units; i++) {
-        if (temp[i] < 0) fatal("negative temperature!\n");
-        sum += temp[i] * (f->units[i - base].width * f->units[i - base].height);
-    }
-    base = g.total_n_blocks;
-    for (int i = SINK_C_W; i <= SINK_C_E; i++) {
-        if (temp[i + base] < 0) fatal("negative temperature!\n");
-        sum += temp[i + base] * 0.25 * (c.s_spreader + g.height) * (c.s_spreader - g.width);
-    }
-    for (int i = SINK_C_N; i <= SINK_C_S; i++) {
-        if (temp[i + base] < 0) fatal("negative temperature!\n");
-        sum += temp[i + base] * 0.25 * (c.s_spreader + g.width) * (c.s_spreader - g.height);
-    }
-    for (int i = SINK_W; i <= SINK_S; i++) {
-        if (temp[i + base] < 0) fatal("negative temperature!\n");
-        sum += temp[i + base] * 0.25 * (sink - spr);
-    }
-    return sum / sink;
+    return sink_temp(sink_model(*model_of(model)), temp);
     HS_END(0.0)
 }
 
@@ -1621,6 +1596,61 @@ void steady_state_temp(RC_model_t *model, double *power, double *temp) {
 
 extern "C++" {
 namespace hs {
+
+SinkModel sink_model(const Model &M) {
+    const grid_model_t &g = *M.g;
+    SinkModel s;
+    s.c = g.config;
+    s.width = g.width;
+    s.height = g.height;
+    s.rows = g.rows;
+    s.cols = g.cols;
+    s.hsidx = M.hsidx;
+    s.k = g.layers[M.hsidx].k;
+    s.thickness = g.layers[M.hsidx].thickness;
+    for (int n = 0; n < M.hsidx; n++) s.base += M.layers[n].flp->n_units;
+    const flp_t *f = M.layers[M.hsidx].flp;
+    for (int i = 0; i < f->n_units; i++) s.area.push_back(f->units[i].width * f->units[i].height);
+    s.total_n_blocks = g.total_n_blocks;
+    return s;
+}
+
+// area-weighted average over the sink layer's blocks and the sink's package
+// nodes, over the sink's area
+double sink_temp(const SinkModel &s, const double *temp) {
+    const thermal_config_t &c = s.c;
+    const double spr = c.s_spreader * c.s_spreader, sink = c.s_sink * c.s_sink;
+    double sum = 0.0;
+    for (size_t q = 0; q < s.area.size(); q++) {
+        if (temp[s.base + q] < 0) fatal("negative temperature!\n");
+        sum += temp[s.base + q] * s.area[q];
+    }
+    const int base = s.total_n_blocks;
+    for (int i = SINK_C_W; i <= SINK_C_E; i++) {
+        if (temp[i + base] < 0) fatal("negative temperature!\n");
+        sum += temp[i + base] * 0.25 * (c.s_spreader + s.height) * (c.s_spreader - s.width);
+    }
+    for (int i = SINK_C_N; i <= SINK_C_S; i++) {
+        if (temp[i + base] < 0) fatal("negative temperature!\n");
+        sum += temp[i + base] * 0.25 * (c.s_spreader + s.width) * (c.s_spreader - s.height);
+    }
+    for (int i = SINK_W; i <= SINK_S; i++) {
+        if (temp[i + base] < 0) fatal("negative temperature!\n");
+        sum += temp[i + base] * 0.25 * (sink - spr);
+    }
+    return sum / sink;
+}
+
+// the same expressions as populate_R_model's for the sink layer and the package
+void sink_resistances(const SinkModel &s, double r, double &rz, package_RC_t &pk) {
+    thermal_config_t c = s.c;
+    c.r_convec = r;
+    package_R(pk, c, s.width, s.height);
+    const double cw = s.width / s.cols, ch = s.height / s.rows;
+    rz = getr(s.k, s.thickness, cw * ch);
+    rz += c.r_convec * (c.s_sink * c.s_sink) / (cw * ch);
+}
+
 LeakBlocks leak_blocks(const Model &M) {
     LeakBlocks b;
     b.mode = M.g->config.leakage_mode;
@@ -1826,6 +1856,14 @@ int hs_engine_submit(hs_engine_t *e, RC_model_t *model, const double *power, int
                      const double *init_temp, int n_steps, double dt) {
     HS_BEGIN
     return eng_submit(engine_of(e), *model_of(model), power, power_rows, init_temp, n_steps, dt);
+    HS_END(-1)
+}
+
+int hs_engine_submit_package(hs_engine_t *e, RC_model_t *model, const double *power,
+                             const package_config_t *package) {
+    HS_BEGIN
+    if (!package) fatal("null package configuration\n");
+    return eng_submit_package(engine_of(e), *model_of(model), power, *package);
     HS_END(-1)
 }
 

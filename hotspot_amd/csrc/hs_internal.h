@@ -149,6 +149,30 @@ void leakage_step(const LeakBlocks &b, const double *power, const double *temp, 
                   double *temp_old);
 double leakage_rise(const LeakBlocks &b, const double *temp, const double *temp_old);
 
+// The natural-convection fixed point over the steady solve (hotspot.c:
+// 667-682): what it reads of a model, copied so that a caller may delete the
+// model once it has handed it over.  sink_temp is calc_sink_temp's grid
+// branch (temperature.c:993-1031) on a block vector; sink_resistances gives
+// the sink layer's rz (populate_R_model_grid, temperature_grid.c:1042-1059)
+// and the package resistances (populate_package_R) at r_convec r.
+// calc_natural_convec's sink temperature is the average plus
+// kSmallForConvec; the loop stops when r_convec moves by less than
+// kNaturalConvecTol (package.h:31-33) and fails above kMaxSinkTemp.
+constexpr double kSmallForConvec = 0.01, kNaturalConvecTol = 0.01, kMaxSinkTemp = 1000.0;
+struct SinkModel {
+    thermal_config_t c{};              // the configuration (r_convec, ambient, s_sink, s_spreader, ...)
+    double width = 0, height = 0;      // of the grid
+    int rows = 0, cols = 0;
+    int hsidx = 0;                     // the sink layer
+    double k = 0, thickness = 0;       // its conductivity and thickness
+    int base = 0;                      // its blocks' offset in the block vector
+    std::vector<double> area;          // their width x height
+    int total_n_blocks = 0;
+};
+SinkModel sink_model(const Model &M);
+double sink_temp(const SinkModel &s, const double *temp);
+void sink_resistances(const SinkModel &s, double r, double &rz, package_RC_t &pk);
+
 // ---- device engine entry points (hs_device.hip) ---------------------------
 void dev_release(Model &M);
 // steady: block power in (package entries already zero), block temps out
@@ -204,6 +228,7 @@ void eng_set_options(Engine &E, const hs_options_t &o);
 void eng_set_tick(Engine &E, int intervals, int iters);
 void eng_set_budget(Engine &E, double bytes);
 int eng_submit(Engine &E, Model &M, const double *power, int rows, const double *init, int nsteps, double dt);
+int eng_submit_package(Engine &E, Model &M, const double *power, const package_config_t &pkg);
 int eng_admit(Engine &E);
 int eng_tick(Engine &E);
 int eng_poll(Engine &E, int id, double *out);

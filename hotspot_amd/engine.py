@@ -27,7 +27,8 @@ class JobInfo(C.Structure):
                 ("rk4_calls", C.c_int), ("taylor_sweeps", C.c_double), ("wait_s", C.c_double), ("latency_s", C.c_double),
                 ("device_bytes", C.c_double), ("taylor_node_bytes", C.c_double), ("held_intervals", C.c_int),
                 ("horner_steps_min", C.c_int), ("horner_steps_plan", C.c_int), ("setup_s", C.c_double),
-                ("resident_model", C.c_int), ("steady_solves", C.c_int), ("be_iters", C.c_double)]
+                ("resident_model", C.c_int), ("steady_solves", C.c_int), ("be_iters", C.c_double),
+                ("r_convec", C.c_double)]
 
 
 class EngineStats(C.Structure):
@@ -52,6 +53,8 @@ def _bind(L):
     L.hs_engine_set_budget.restype = None
     L.hs_engine_submit.argtypes = [C.c_void_p, iface.RCModelP, dp, C.c_int, dp, C.c_int, C.c_double]
     L.hs_engine_submit.restype = C.c_int
+    L.hs_engine_submit_package.argtypes = [C.c_void_p, iface.RCModelP, dp, C.POINTER(iface.PackageConfig)]
+    L.hs_engine_submit_package.restype = C.c_int
     L.hs_engine_admit.argtypes = [C.c_void_p]
     L.hs_engine_admit.restype = C.c_int
     L.hs_engine_tick.argtypes = [C.c_void_p]
@@ -122,6 +125,17 @@ class Engine:
         if job < 0:
             raise RuntimeError("hs_engine_submit refused the job")
         self.meta[job] = (n, n_steps)
+        return job
+
+    def submit_package(self, model, power, package):
+        """A steady job under the package model (package: iface.PackageConfig);
+        with package.natural_convec, the natural-convection fixed point."""
+        P = np.ascontiguousarray(np.asarray(power, dtype=np.float64))
+        job = self.L.hs_engine_submit_package(self.h, model, _dp(P), C.byref(package))
+        self._check()
+        if job < 0:
+            raise RuntimeError("hs_engine_submit_package refused the job")
+        self.meta[job] = (P.shape[-1], 0)
         return job
 
     def admit(self):

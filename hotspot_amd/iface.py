@@ -222,6 +222,20 @@ RCModelP = C.POINTER(RCModel)
 FlpP = C.POINTER(Flp)
 
 
+class PackageConfig(C.Structure):
+    """package_config_t (package.h:48-75)"""
+    _fields_ = [("natural_convec", C.c_int), ("flow_type", C.c_int), ("sink_type", C.c_int),
+                ("fin_height", C.c_double), ("fin_width", C.c_double), ("channel_width", C.c_double),
+                ("pin_height", C.c_double), ("pin_diam", C.c_double), ("pin_dist", C.c_double),
+                ("fan_radius", C.c_double), ("motor_radius", C.c_double), ("rpm", C.c_int)]
+
+
+class Convection(C.Structure):
+    """convection_t (package.h:78-104)"""
+    _fields_ = [(n, C.c_double) for n in ("n_fin", "sur_area_fin", "n_pin", "sur_area_pin", "reynolds",
+                                          "nusselt", "h_coeff", "v", "r_th")]
+
+
 class StrPair(C.Structure):
     """str_pair (util.h:92-96)"""
     _fields_ = [("name", C.c_char * STR_SIZE), ("value", C.c_char * STR_SIZE)]

@@ -565,6 +565,21 @@ void hs_engine_set_budget(hs_engine_t *e, double bytes);
  * Returns the job id, or -1 on error (e.g. a footprint above the budget). */
 int hs_engine_submit(hs_engine_t *e, RC_model_t *model, const double *power, int power_rows,
                      const double *init_temp, int n_steps, double dt);
+/* Submits a steady job under the package model (package.c:353-397):
+ * r_convec from the fan and sink for forced convection, one solve; with
+ * package->natural_convec, the natural-convection fixed point of the
+ * command line (hotspot.c:667-682): r_convec at the ambient temperature +
+ * 0.01 K, then, after each solve, at its average sink temperature
+ * (calc_sink_temp + 0.01 K), solved again from the last solution until
+ * r_convec moves by less than 0.01 K/W; the job fails above a 1000 K sink.
+ * The result is the last solve's block vector, hs_job_info_t's r_convec
+ * the last r_convec and steady_solves the number of solves.  Homogeneous
+ * models without leakage only (no detailed-3D or microchannel layers); the
+ * model itself is not changed.  Returns the job id, or -1 on error.  No
+ * counterpart in the reference, whose command line runs the loop around
+ * steady_state_temp and populate_R_model. */
+int hs_engine_submit_package(hs_engine_t *e, RC_model_t *model, const double *power,
+                             const package_config_t *package);
 /* Admits queued jobs while they fit (their inputs are then resident on the
  * device); returns the number still queued.  hs_engine_tick admits too. */
 int hs_engine_admit(hs_engine_t *e);
@@ -602,6 +617,8 @@ typedef struct hs_job_info_st {
     int steady_solves;        /* steady solves of its steady phase: those of the temperature-leakage
                                * fixed point for a model with leakage_used, else 1 (0: none yet) */
     double be_iters;          /* backward Euler: solver iterations over its intervals so far */
+    double r_convec;          /* package-model jobs: the sink's r_convec after its last solve
+                               * (package_model at that solve's average sink temperature) */
 } hs_job_info_t;
 int hs_engine_job_info(hs_engine_t *e, int job, hs_job_info_t *info);
 /* frees a finished job's results */

@@ -40,7 +40,8 @@ PROBE = r"""
 #define S(T) printf(#T " %zu\n", sizeof(T));
 int main(void) {
   S(thermal_config_t) S(unit_t) S(flp_t) S(package_RC_t) S(layer_t) S(grid_model_t)
-  S(RC_model_t) S(microchannel_config_t) S(blist_t) S(glist_t)
+  S(RC_model_t) S(microchannel_config_t) S(blist_t) S(glist_t) S(package_config_t) S(convection_t)
+  P(package_config_t, rpm) P(convection_t, r_th)
   P(thermal_config_t, model_secondary) P(thermal_config_t, ambient) P(thermal_config_t, grid_rows)
   P(thermal_config_t, grid_map_mode) P(thermal_config_t, grid_transient_file)
   P(thermal_config_t, detailed_3D_used) P(layer_t, flp) P(layer_t, has_power) P(layer_t, rx)
@@ -55,7 +56,8 @@ int main(void) {
 CTYPES = {"thermal_config_t": iface.ThermalConfig, "unit_t": iface.Unit, "flp_t": iface.Flp,
           "package_RC_t": iface.PackageRC, "layer_t": iface.Layer, "grid_model_t": iface.GridModel,
           "RC_model_t": iface.RCModel, "microchannel_config_t": iface.MicrochannelConfig,
-          "blist_t": iface.Blist, "glist_t": iface.Glist}
+          "blist_t": iface.Blist, "glist_t": iface.Glist, "package_config_t": iface.PackageConfig,
+          "convection_t": iface.Convection}
 
 
 def test_struct_layouts_match_reference_mirror():
@@ -84,11 +86,7 @@ REF_STRUCTS = {
     "block_model_t": iface.BlockModel, "microchannel_config_t": iface.MicrochannelConfig,
     "material_t": iface.Material, "materials_list_t": iface.MaterialsList, "blist_t": iface.Blist,
     "glist_t": iface.Glist, "str_pair": iface.StrPair,
-    "package_config_t": ["natural_convec", "flow_type", "sink_type", "fin_height", "fin_width",
-                         "channel_width", "pin_height", "pin_diam", "pin_dist", "fan_radius",
-                         "motor_radius", "rpm"],
-    "convection_t": ["n_fin", "sur_area_fin", "n_pin", "sur_area_pin", "reynolds", "nusselt",
-                     "h_coeff", "v", "r_th"],
+    "package_config_t": iface.PackageConfig, "convection_t": iface.Convection,
 }
 
 

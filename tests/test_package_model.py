@@ -11,16 +11,7 @@ import pytest
 from hotspot_amd import iface
 
 
-class PackageConfig(C.Structure):
-    _fields_ = [("natural_convec", C.c_int), ("flow_type", C.c_int), ("sink_type", C.c_int),
-                ("fin_height", C.c_double), ("fin_width", C.c_double), ("channel_width", C.c_double),
-                ("pin_height", C.c_double), ("pin_diam", C.c_double), ("pin_dist", C.c_double),
-                ("fan_radius", C.c_double), ("motor_radius", C.c_double), ("rpm", C.c_int)]
-
-
-class Convection(C.Structure):
-    _fields_ = [(n, C.c_double) for n in ("n_fin", "sur_area_fin", "n_pin", "sur_area_pin", "reynolds",
-                                          "nusselt", "h_coeff", "v", "r_th")]
+PackageConfig, Convection = iface.PackageConfig, iface.Convection
 
 
 def bind(lib):
