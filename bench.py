@@ -35,10 +35,11 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# The engine runs its set-up, steady and transient work on seven streams;
-# with a hardware queue each, no stream's work waits behind another's
-# launches in a shared queue (HIP's default is four queues per process).
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# The engine runs its set-up, steady, transient and backward-Euler work on
+# ten streams; with a hardware queue each, no stream's work waits behind
+# another's launches in a shared queue (HIP's default is four queues per
+# process).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "12")
 
 from hotspot_amd import cases, engine, iface  # noqa: E402
 
@@ -354,6 +355,7 @@ def mode_serve(lib, args):
             if active == 0 and k < len(reqs):
                 time.sleep(max(0.0, reqs[k][0] - (time.perf_counter() - t0)))
         wall = time.perf_counter() - t0
+        est = eng.stats()
         infos = [eng.info(i) for i in ids]
         lat = np.array([x.latency_s for x in infos])
         steady_lat = np.array([x.latency_s for x, r in zip(infos, reqs) if not r[2]])
@@ -363,7 +365,11 @@ def mode_serve(lib, args):
                          steady_p99_ms=float(np.percentile(steady_lat, 99)) * 1e3 if len(steady_lat) else None,
                          wait_p50_ms=float(np.percentile([x.wait_s for x in infos], 50)) * 1e3,
                          setup_mean_ms=float(np.mean([x.setup_s for x in infos])) * 1e3,
-                         resident_frac=float(np.mean([x.resident_model for x in infos])))
+                         resident_frac=float(np.mean([x.resident_model for x in infos])),
+                         steady_p50_ms_by_cells={str(c): float(np.percentile(
+                             [x.latency_s for x, r in zip(infos, reqs) if not r[2] and pool[r[1]][2] == c], 50)) * 1e3
+                             for c in sorted({pool[r[1]][2] for r in reqs if not r[2]})},
+                         engine={k: getattr(est, k) for k, _ in est._fields_})
         for i in ids:
             eng.release(i)
         eng.close()

@@ -96,9 +96,12 @@ constexpr double kSliceMs = 20.0;
 constexpr double kSliceYieldMs = 2.0;
 // longest a tick waits for some group's chunk to end
 constexpr double kWaitMs = 2.0;
+// intervals of a transient group's first chunk, which times them
+constexpr int kFirstChunk = 2;
 // streams of the steady groups: the kernels of a steady round are small and
-// latency-bound, so groups of different shapes overlap on the device
-constexpr int kSteadyStreams = 4;
+// latency-bound, so groups of different shapes overlap on the device; a
+// group's chunk takes a stream that no other steady chunk is running on
+constexpr int kSteadyStreams = 6;
 // inner iterations of a multigrid-preconditioned CG round: an FP32 inner
 // solve gains its four digits in about 10 of them, and a refinement step
 // takes place at the start of a round, so a solve takes one round per
@@ -114,33 +117,54 @@ struct BatchGroup {
     DevPool *pool = nullptr, *hpool = nullptr;
     hipStream_t stream = nullptr;       // the stream its chunks run on
     size_t cap_desc = 0, cap_work = 0, cap_gwork = 0, cap_flags = 0, cap_trec = 0, cap_twork = 0;
-    double *d_flags = nullptr, *h_flags = nullptr;
-    int *h_kmax = nullptr;              // Taylor: the term counts of the last chunk (pinned)
+    // a chunk's results, written by its last kernel straight into pinned
+    // host memory (mapped): no copy-engine transfer that would queue behind
+    // the chunk and hold up other streams' copies meanwhile
+    double *h_flags = nullptr, *m_flags = nullptr;   // host / device address
+    int *h_kmax = nullptr, *m_kmax = nullptr;         // Taylor: the term counts of the last chunk
     // interval bodies captured per count of Horner steps launched (kgraph)
     Replayer rp[kTayMaxK + 1];
     SeqForm form{};
     // multigrid steady groups: per level, the descriptors and column work-lists
     std::vector<MgGroup> lv;
     std::vector<void *> lv_bufs;
+    // set-up copies of the last membership change: pinned staging blocks in
+    // flight on the set-up stream, and the event after them that the
+    // group's next chunk waits on
+    std::vector<void *> staged;
+    hipEvent_t ev_sync = nullptr;
+    bool synced = false;                // a set-up since the last chunk: the chunk waits on ev_sync
     // the chunk in flight
     bool busy = false;
     int ran = 0;                        // members launched in it (the first `ran`; later ones joined since)
     int chunk = 0;                      // transient intervals in it
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
-    double ms_per_interval = 0.0;       // device time per interval of the last chunk
+    // device time per interval and node of the last chunk (0: none yet), and
+    // the nodes of the members that ran it: a chunk's intervals are sized by
+    // the nodes its members hold now
+    double ms_per_node_interval = 0.0;
+    double chunk_nodes = 0.0;
 
+    // a membership change: each interval body is captured again at its
+    // next use, and its executable graph updated in place from the capture
     void reset_graphs() {
-        for (Replayer &r : rp) r.reset();
+        for (Replayer &r : rp) r.stale = true;
+    }
+    void unstage() {
+        for (void *p : staged) hfree(hpool, p);
+        staged.clear();
     }
     void release() {
-        reset_graphs();
+        for (Replayer &r : rp) r.reset();
+        unstage();
+        if (ev_sync) (void)hipEventDestroy(ev_sync);
+        ev_sync = nullptr;
         for (void *p : lv_bufs) dfree(pool, p);
         lv_bufs.clear();
         lv.clear();
         dfree(pool, g.d_desc);
         dfree(pool, g.d_work);
         dfree(pool, g.d_gwork);
-        dfree(pool, d_flags);
         hfree(hpool, h_flags);
         hfree(hpool, h_kmax);
         dfree(pool, g.d_kmax);
@@ -150,8 +174,8 @@ struct BatchGroup {
         if (ev_b) (void)hipEventDestroy(ev_b);
         ev_a = ev_b = nullptr;
         g = Group{};
-        d_flags = h_flags = nullptr;
-        h_kmax = nullptr;
+        h_flags = m_flags = nullptr;
+        h_kmax = m_kmax = nullptr;
         cap_desc = cap_work = cap_gwork = cap_flags = cap_trec = cap_twork = 0;
     }
     ~BatchGroup() { release(); }
@@ -171,6 +195,18 @@ struct Resident {
     double last_use = 0.0;
 };
 
+// The read-only device data of a model (maps, block ranges, work-lists,
+// heterogeneous coefficients), built once while its coefficients are
+// unchanged; every job on the model takes a device model of its own state
+// that shares it (dev_clone), so that a job on a model already seen only
+// allocates its vectors.
+struct ModelConst {
+    const Model *M = nullptr;
+    unsigned long stamp = 0;
+    std::shared_ptr<DevModel> tpl;
+    double last_use = 0.0;
+};
+
 struct Engine {
     DevPool dpool, hpool;               // declared first: destroyed after the groups that use them
     hs_options_t opt{};
@@ -184,6 +220,8 @@ struct Engine {
     std::map<int, std::unique_ptr<EngJob>> jobs;
     std::vector<Resident> resident;     // idle device models of ended jobs
     size_t resident_bytes = 0;
+    std::vector<ModelConst> consts;     // read-only device data per model
+    size_t const_bytes = 0;
     hs_engine_stats_t stats{};
     // steady groups by key: kind (0 homogeneous CG, 1 heterogeneous CG,
     // 2 non-symmetric GMRES) + 4 x multigrid levels (0: Jacobi);
@@ -198,6 +236,7 @@ struct Engine {
             if (s) (void)hipStreamSynchronize(s);
         for (auto &kv : jobs) kv.second->D.reset();
         resident.clear();
+        consts.clear();
         for (auto &kv : steady) kv.second.release();
         for (auto &kv : be) kv.second.release();
         for (auto &g : trans) g.release();
@@ -252,18 +291,34 @@ template <typename T> static void ensure_dev(DevPool *pool, T *&p, size_t &cap, 
     p = (T *)dmalloc(pool, cap * sizeof(T));
 }
 
+// A copy of the group's set-up through pinned staging on stream s, without
+// a host wait (the staging block is returned at the group's next set-up,
+// after the chunk that waited for the copy).
+static void stage_h2d(BatchGroup &B, void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (!bytes) return;
+    void *p = hmalloc(B.hpool, bytes);
+    memcpy(p, src, bytes);
+    B.staged.push_back(p);
+    HIP_OK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, s));
+}
+
 // Uploads the group's descriptors and work-lists after a membership change
 // (on the set-up stream: the group's own stream may carry another group's
-// chunk, and this group is between chunks).  A backward-Euler group (be)
-// takes each member's shifted descriptor and shifted multigrid levels.
+// chunk, and this group is between chunks; the group's next chunk waits for
+// the copies on the device).  A backward-Euler group (be) takes each
+// member's shifted descriptor and shifted multigrid levels.
 static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans, bool be = false) {
     if (!B.dirty) return;
+    B.unstage();
     struct Timed {
         double &acc;
         double t0 = now_s();
         ~Timed() { acc += now_s() - t0; }
     } timed{E.stats.sync_s};
     hipStream_t s = E.s_setup;
+    const Upload staged = [&B](void *dst, const void *src, size_t bytes, hipStream_t st) {
+        stage_h2d(B, dst, src, bytes, st);
+    };
     const int n = (int)B.members.size();
     std::vector<ProbDesc> desc(n);
     std::vector<WorkItem> w, gw;
@@ -290,23 +345,25 @@ static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans, bool be =
     ensure_dev(B.pool, B.g.d_work, B.cap_work, w.size());
     ensure_dev(B.pool, B.g.d_gwork, B.cap_gwork, gw.size());
     if (!B.g.d_kmax) B.g.d_kmax = (int *)dmalloc(B.pool, 2 * sizeof(int));
-    if (!B.h_kmax) B.h_kmax = (int *)hmalloc(B.hpool, 2 * sizeof(int));
+    if (!B.h_kmax) {
+        B.h_kmax = (int *)hmalloc(B.hpool, 2 * sizeof(int));
+        HIP_OK(hipHostGetDevicePointer((void **)&B.m_kmax, B.h_kmax, 0));
+    }
     const int zero[2] = {0, 0};
-    h2d(B.g.d_kmax, zero, sizeof zero, s);
+    stage_h2d(B, B.g.d_kmax, zero, sizeof zero, s);
     B.h_kmax[0] = B.h_kmax[1] = 0;
     B.g.kgraph = 0;
     ensure_dev(B.pool, B.g.d_trec, B.cap_trec, (size_t)kTrec * std::max(n, 1));
-    if (!rec.empty()) h2d(B.g.d_trec, rec.data(), rec.size() * sizeof(double), s);
+    if (!rec.empty()) stage_h2d(B, B.g.d_trec, rec.data(), rec.size() * sizeof(double), s);
     if ((size_t)n * 8 > B.cap_flags) {
-        dfree(B.pool, B.d_flags);
         hfree(B.hpool, B.h_flags);
         B.cap_flags = std::max((size_t)n * 8, B.cap_flags * 2);
-        B.d_flags = (double *)dmalloc(B.pool, B.cap_flags * sizeof(double));
         B.h_flags = (double *)hmalloc(B.hpool, B.cap_flags * sizeof(double));
+        HIP_OK(hipHostGetDevicePointer((void **)&B.m_flags, B.h_flags, 0));
     }
-    h2d(B.g.d_desc, desc.data(), n * sizeof(ProbDesc), s);
-    h2d(B.g.d_work, w.data(), w.size() * sizeof(WorkItem), s);
-    h2d(B.g.d_gwork, gw.data(), gw.size() * sizeof(WorkItem), s);
+    stage_h2d(B, B.g.d_desc, desc.data(), n * sizeof(ProbDesc), s);
+    stage_h2d(B, B.g.d_work, w.data(), w.size() * sizeof(WorkItem), s);
+    stage_h2d(B, B.g.d_gwork, gw.data(), gw.size() * sizeof(WorkItem), s);
     B.g.nprob = n;
     B.g.nwork = (int)w.size();
     B.g.ngwork = (int)gw.size();
@@ -320,11 +377,11 @@ static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans, bool be =
     // the float4 term sweep's work-list; its tiling timed per size class for
     // FP32 Taylor groups
     if (trans && v4 && E.opt.transient == HS_TRANS_TAYLOR && E.opt.precision == HS_PREC_MIXED) {
-        tune_term_tiling(B.g, desc, B.cap_twork, B.pool, s);
+        tune_term_tiling(B.g, desc, B.cap_twork, B.pool, s, staged);
     } else {
         std::vector<WorkItem> tw;
         const int per = term_items(desc, tw);
-        upload_term_items(B.g, tw, per, B.cap_twork, B.pool, s);
+        upload_term_items(B.g, tw, per, B.cap_twork, B.pool, s, staged);
         B.g.tz = term_form(desc);
         B.g.tg = term_groups(desc);
         B.g.zfirst = false;
@@ -355,22 +412,31 @@ static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans, bool be =
         WorkItem *dw = (WorkItem *)dmalloc(B.pool, lw.size() * sizeof(WorkItem));
         B.lv_bufs.push_back(dd);
         B.lv_bufs.push_back(dw);
-        h2d(dd, ld.data(), n * sizeof(ProbDesc), s);
-        h2d(dw, lw.data(), lw.size() * sizeof(WorkItem), s);
+        stage_h2d(B, dd, ld.data(), n * sizeof(ProbDesc), s);
+        stage_h2d(B, dw, lw.data(), lw.size() * sizeof(WorkItem), s);
         B.lv.push_back(MgGroup{dd, dw, (int)lw.size(), nlmax, l > 0 || het || be});
     }
+    if (!B.ev_sync) HIP_OK(hipEventCreateWithFlags(&B.ev_sync, hipEventDisableTiming));
+    HIP_OK(hipEventRecord(B.ev_sync, s));
+    B.synced = true;
     B.dirty = false;
 }
 
-// copies the given scalar slots of every member into the pinned flags
-// (after the chunk, on the group's stream)
-static void gather(BatchGroup &B, std::initializer_list<int> slots) {
+// the group's chunk waits on the device for the copies of its last set-up
+static void wait_sync(BatchGroup &B) {
+    if (!B.synced) return;
+    HIP_OK(hipStreamWaitEvent(B.stream, B.ev_sync, 0));
+    B.synced = false;
+}
+
+// writes the given scalar slots of every member into the pinned flags, and
+// for a Taylor group its term counts (after the chunk, on the group's stream)
+static void gather(BatchGroup &B, std::initializer_list<int> slots, bool kmax = false) {
     Slots sl{};
     for (int v : slots) sl.s[sl.n++] = v;
     hipLaunchKernelGGL(k_gather, dim3(small_grid(B.g.nprob)), dim3(64), 0, B.g.stream,
-                       (const ProbDesc *)B.g.d_desc, B.g.nprob, sl, B.d_flags);
-    HIP_OK(hipMemcpyAsync(B.h_flags, B.d_flags, (size_t)B.g.nprob * 8 * sizeof(double), hipMemcpyDeviceToHost,
-                          B.g.stream));
+                       (const ProbDesc *)B.g.d_desc, B.g.nprob, sl, B.m_flags, kmax ? B.g.d_kmax : nullptr,
+                       kmax ? B.m_kmax : nullptr);
 }
 
 // A job's device model goes back to the resident set, its sequence buffers
@@ -383,7 +449,11 @@ static void release_device(Engine &E, EngJob &j) {
     D.release(D.h.seq_out);
     D.h.seq_in = D.h.seq_out = nullptr;
     D.h.seq_one = 0;
-    if (j.package) {   // coefficients of the job's own r_convec: not the model's
+    bool shared_data_kept = !D.base;
+    for (const ModelConst &c : E.consts) shared_data_kept |= c.tpl == D.base;
+    // not kept: coefficients of the job's own r_convec (not the model's), or
+    // read-only data that the engine no longer counts against its budget
+    if (j.package || !shared_data_kept) {
         j.D.reset();
         return;
     }
@@ -441,16 +511,47 @@ static DevModel *take_resident(Engine &E, const Model &M) {
     return found;
 }
 
-// Drops the least recently used resident models until `need` more bytes fit
-// the budget beside the resident jobs (their blocks go to the pool).
+// Drops the least recently used resident models, then the least recently
+// used read-only model data, until `need` more bytes fit the budget beside
+// the resident jobs (their blocks go to the pool; a model's read-only data
+// stays on the device while a job's device model uses it).
 static void evict_resident(Engine &E, size_t need) {
-    while (!E.resident.empty() && E.used + E.resident_bytes + need > E.budget) {
+    while (!E.resident.empty() && E.used + E.resident_bytes + E.const_bytes + need > E.budget) {
         size_t k = 0;
         for (size_t i = 1; i < E.resident.size(); i++)
             if (E.resident[i].last_use < E.resident[k].last_use) k = i;
         E.resident_bytes -= E.resident[k].D->bytes;
         E.resident.erase(E.resident.begin() + k);
     }
+    while (!E.consts.empty() && E.used + E.resident_bytes + E.const_bytes + need > E.budget) {
+        size_t k = 0;
+        for (size_t i = 1; i < E.consts.size(); i++)
+            if (E.consts[i].last_use < E.consts[k].last_use) k = i;
+        E.const_bytes -= E.consts[k].tpl->bytes;
+        E.consts.erase(E.consts.begin() + k);
+    }
+}
+
+// The read-only device data of M's current coefficients, built on first use
+// (entries of M's earlier coefficients are dropped).
+static std::shared_ptr<DevModel> model_const(Engine &E, Model &M) {
+    for (size_t k = 0; k < E.consts.size();) {
+        ModelConst &c = E.consts[k];
+        if (c.M == &M && c.stamp != M.stamp) {
+            E.const_bytes -= c.tpl->bytes;
+            E.consts.erase(E.consts.begin() + k);
+            continue;
+        }
+        if (c.M == &M) {
+            c.last_use = now_s();
+            return c.tpl;
+        }
+        k++;
+    }
+    std::shared_ptr<DevModel> t(dev_build(M, E.s_setup, &E.dpool, &E.hpool, false));
+    E.consts.push_back(ModelConst{&M, M.stamp, t, now_s()});
+    E.const_bytes += t->bytes;
+    return t;
 }
 
 static int steady_key(const EngJob &j) {
@@ -591,7 +692,7 @@ static void admit(Engine &E, EngJob &j) {
         j.resident = true;
         dev_reset_state(*R);
     } else {
-        j.D.reset(dev_build(M, E.s_setup, &E.dpool, &E.hpool));
+        j.D.reset(dev_clone(model_const(E, M), E.s_setup, &E.dpool, &E.hpool));
     }
     DevModel &D = *j.D;
     const size_t nb = (size_t)D.nblk;
@@ -818,16 +919,28 @@ static void group_events(BatchGroup &B) {
 
 // one chunk of every group that is between chunks
 static void launch_chunks(Engine &E) {
+    // steady streams that a chunk still runs on
+    bool taken[kSteadyStreams] = {};
+    for (auto &kv : E.steady)
+        for (int k = 0; k < kSteadyStreams; k++)
+            if (kv.second.busy && kv.second.stream == E.s_steady[k]) taken[k] = true;
     for (auto &kv : E.steady) {
         BatchGroup &B = kv.second;
         if (B.busy || B.members.empty()) continue;
         const int kind = kv.first % 4;
+        // a stream of its own, so that a group's chunk never queues behind
+        // another group's (all taken: the group waits for the next tick)
+        int k = 0;
+        while (k < kSteadyStreams && taken[k]) k++;
+        if (k == kSteadyStreams) continue;
+        taken[k] = true;
         B.pool = &E.dpool;
         B.hpool = &E.hpool;
-        // groups of different multigrid depths (grid sizes) on different streams
-        B.stream = E.s_steady[(kv.first / 4 + kind) % kSteadyStreams];
+        B.stream = E.s_steady[k];
         group_sync(E, B, kind > 0, false);
+        B.g.stream = B.stream;   // the launches of the chunk (and its capture) on the chosen stream
         group_events(B);
+        wait_sync(B);
         HIP_OK(hipEventRecord(B.ev_a, B.stream));
         const int rounds = steady_rounds(E, kind == 2, !B.lv.empty());
         B.rp[0].run(B.stream, rounds, [&] {
@@ -851,6 +964,7 @@ static void launch_chunks(Engine &E) {
         B.stream = E.s_be;
         group_sync(E, B, kind > 0, false, true);
         group_events(B);
+        wait_sync(B);
         HIP_OK(hipEventRecord(B.ev_a, B.stream));
         B.rp[0].run(B.stream, steady_rounds(E, kind == 2, !B.lv.empty()), [&] {
             if (kind) be_round<true>(E, B.g, kind, B.lv);
@@ -880,25 +994,29 @@ static void launch_chunks(Engine &E) {
         B.stream = E.s_trans[h / kTransForms];
         group_sync(E, B, h % kTransForms == 2, true);
         group_events(B);
+        wait_sync(B);
         // the intervals left to the member furthest from its end, within the
         // tick's intervals and the device-time slice
         int rem = 1;
         for (const EngJob *j : B.members) rem = std::max(rem, j->nsteps - j->step_seen);
         int n = std::min(E.tick_intervals, rem);
         const double slice = steady_resident ? kSliceYieldMs : kSliceMs;
-        if (B.ms_per_interval > 0) n = std::min(n, std::max(1, (int)(slice / B.ms_per_interval)));
+        double nodes = 0.0;
+        for (const EngJob *j : B.members) nodes += (double)j->D->vlen;
+        // a group's first chunk runs a few intervals to time them
+        if (B.ms_per_node_interval > 0)
+            n = std::min(n, std::max(1, (int)(slice / (B.ms_per_node_interval * nodes))));
+        else
+            n = std::min(n, kFirstChunk);
         B.chunk = n;
+        B.chunk_nodes = nodes;
         const int kl = B.g.kgraph > 0 ? std::min(B.g.kgraph, B.form.K) : B.form.K;
         for (EngJob *j : B.members)
             if (j->kplan && kl < j->kplan && (j->kmin == 0 || kl < j->kmin)) j->kmin = kl;
         HIP_OK(hipEventRecord(B.ev_a, B.stream));
         B.rp[B.g.kgraph].run(B.stream, n, [&] { seq_body(B.g, B.form); });
-        gather(B, {S_STEP, S_TCALLS, S_TTERMS, S_TBYTES, S_NHOLD});
-        if (B.form.transient == HS_TRANS_TAYLOR) {
-            // the term counts of this chunk; the largest since cleared starts again
-            HIP_OK(hipMemcpyAsync(B.h_kmax, B.g.d_kmax, 2 * sizeof(int), hipMemcpyDeviceToHost, B.stream));
-            HIP_OK(hipMemsetAsync(B.g.d_kmax + 1, 0, sizeof(int), B.stream));
-        }
+        // with the term counts of this chunk (the largest since cleared starts again)
+        gather(B, {S_STEP, S_TCALLS, S_TTERMS, S_TBYTES, S_NHOLD}, B.form.transient == HS_TRANS_TAYLOR);
         HIP_OK(hipEventRecord(B.ev_b, B.stream));
         E.stats.transient_chunks++;
         B.ran = B.g.nprob;
@@ -974,7 +1092,7 @@ static void harvest(Engine &E) {
         B.busy = false;
         float ms = 0.f;
         HIP_OK(hipEventElapsedTime(&ms, B.ev_a, B.ev_b));
-        B.ms_per_interval = (double)ms / std::max(1, B.chunk);
+        B.ms_per_node_interval = (double)ms / std::max(1, B.chunk) / std::max(1.0, B.chunk_nodes);
         E.stats.transient_chunk_s += 1e-3 * ms;
         // Taylor groups of one sub-step per interval launch the Horner steps
         // that the last chunk's largest term count needs instead of every
@@ -1124,7 +1242,7 @@ int eng_admit(Engine &E) {
         // resident models and the pool's idle blocks count against the
         // budget until dropped or returned to the runtime
         evict_resident(E, j.footprint);
-        if (E.used + E.resident_bytes + E.dpool.idle_bytes + j.footprint > E.budget) E.dpool.trim();
+        if (E.used + E.resident_bytes + E.const_bytes + E.dpool.idle_bytes + j.footprint > E.budget) E.dpool.trim();
         E.queue.pop_front();
         const double t0 = now_s();
         admit(E, j);

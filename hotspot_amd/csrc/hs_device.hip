@@ -1977,13 +1977,21 @@ __global__ void k_step_advance(const ProbDesc *__restrict__ D, int nprob) {
     D[p].scal[S_STEP] += 1.0;
 }
 
-// copies up to 8 scalar slots of every problem into out[p * 8 + i]
+// copies up to 8 scalar slots of every problem into out[p * 8 + i]; with
+// kmax, also the group's two term counts into kmax_out, and clears the
+// second (the largest since the last gather)
 struct Slots {
     int n;
     int s[8];
 };
-__global__ void k_gather(const ProbDesc *__restrict__ D, int nprob, Slots sl, double *out) {
+__global__ void k_gather(const ProbDesc *__restrict__ D, int nprob, Slots sl, double *out, int *kmax,
+                         int *kmax_out) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (kmax && p == 0) {
+        kmax_out[0] = kmax[0];
+        kmax_out[1] = kmax[1];
+        kmax[1] = 0;
+    }
     if (p >= nprob) return;
     for (int i = 0; i < sl.n; i++) out[p * 8 + i] = D[p].scal[sl.s[i]];
 }

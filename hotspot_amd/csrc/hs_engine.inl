@@ -2,6 +2,7 @@
 // building, heterogeneous assembly, and the solve drivers.  Included at the
 // end of hs_device.hip (same translation unit as the kernels).
 #include <array>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -181,6 +182,9 @@ struct DevModel {
     double be_sigma = 0.0;              // the shift 1/h they hold
 
     DevPool *pool = nullptr, *hpool = nullptr;   // an engine's pools (nullptr: the runtime's)
+    // the device model whose read-only data (maps, block ranges, work-lists,
+    // heterogeneous coefficients) this one uses instead of its own copy
+    std::shared_ptr<DevModel> base;
 
     std::vector<size_t> alloc_sz;       // bytes of each allocation in allocs
 
@@ -406,21 +410,27 @@ static std::map<GridKey, TermShape> &term_shape_pins() {
 }
 static GridKey grid_key(const ProbDesc &h) { return GridKey{h.nl, h.nr, h.nc}; }
 
+// Host-to-device copy of a set-up: by default ordered on s and waited for
+// (h2d); an engine group passes one that stages the copy without a wait.
+using Upload = std::function<void(void *, const void *, size_t, hipStream_t)>;
+static void h2d_wait(void *dst, const void *src, size_t bytes, hipStream_t s) { h2d(dst, src, bytes, s); }
+
 static void upload_term_items(Group &G, const std::vector<WorkItem> &w, int per, size_t &cap, DevPool *pool,
-                              hipStream_t s) {
+                              hipStream_t s, const Upload &up = h2d_wait) {
     if (w.size() > cap) {
         dfree(pool, G.d_twork);
         cap = std::max(w.size(), 2 * cap);
         G.d_twork = (WorkItem *)dmalloc(pool, cap * sizeof(WorkItem));
     }
-    h2d(G.d_twork, w.data(), w.size() * sizeof(WorkItem), s);
+    up(G.d_twork, w.data(), w.size() * sizeof(WorkItem), s);
     G.ntwork = (int)w.size();
     G.tnper = per;
 }
 
 // Times the candidates on stream s (the group's own stream may be busy
 // with another group's launches) and applies each problem's class shape.
-static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap, DevPool *pool, hipStream_t s) {
+static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap, DevPool *pool, hipStream_t s,
+                             const Upload &up = h2d_wait) {
     auto &cache = term_shape_cache();
     const auto &pins = term_shape_pins();
     const int n = (int)desc.size();
@@ -430,10 +440,26 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap,
         const GridKey g = grid_key(h);
         return ShapeKey{g[0], g[1], g[2], ilog2(count[g])};
     };
+    // a class (grid shape, problem count) not timed yet takes the choice of
+    // the nearest count of its grid shape, if any: a served stream forms
+    // groups of ever new compositions, and the choice changes little with
+    // the count; only grid shapes never seen are timed
+    auto resolve = [&](const ShapeKey &k) {
+        if (cache.count(k)) return k;
+        ShapeKey best = k;
+        int gap = 1 << 30;
+        for (auto it = cache.lower_bound(ShapeKey{k[0], k[1], k[2], -1});
+             it != cache.end() && it->first[0] == k[0] && it->first[1] == k[1] && it->first[2] == k[2]; ++it)
+            if (std::abs(it->first[3] - k[3]) < gap) {
+                gap = std::abs(it->first[3] - k[3]);
+                best = it->first;
+            }
+        return best;
+    };
     std::vector<ShapeKey> todo;
     for (const ProbDesc &h : desc) {
         const ShapeKey k = key_of(h);
-        if (!pins.count(grid_key(h)) && !cache.count(k) && std::find(todo.begin(), todo.end(), k) == todo.end())
+        if (!pins.count(grid_key(h)) && !cache.count(resolve(k)) && std::find(todo.begin(), todo.end(), k) == todo.end())
             todo.push_back(k);
     }
     if (!todo.empty()) {
@@ -507,13 +533,13 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap,
     }
     for (ProbDesc &h : desc) {
         const auto pin = pins.find(grid_key(h));
-        const TermShape t = pin != pins.end() ? pin->second : cache.at(key_of(h));
+        const TermShape t = pin != pins.end() ? pin->second : cache.at(resolve(key_of(h)));
         set_term_tiling(h, std::min(t.shift, ilog2(h.nc)), t.rpt, t.z);
     }
-    h2d(G.d_desc, desc.data(), desc.size() * sizeof(ProbDesc), s);
+    up(G.d_desc, desc.data(), desc.size() * sizeof(ProbDesc), s);
     std::vector<WorkItem> w;
     const int per = term_items(desc, w);
-    upload_term_items(G, w, per, cap, pool, s);
+    upload_term_items(G, w, per, cap, pool, s, up);
     G.tz = term_form(desc);
     G.tg = term_groups(desc);
     G.zfirst = first_z_ok(desc);
@@ -876,10 +902,54 @@ static double package_rho(const Model &M, const ProbDesc &h) {
     return rho;
 }
 
+// The per-job state of a device model: the FP64 and FP32 vectors, the
+// block staging vectors, reductions, scalars, the descriptor on the device
+// and the single-model group's Taylor buffers (every allocation zeroed).
+static void alloc_state(DevModel &D) {
+    ProbDesc &h = D.h;
+    for (int b = 0; b < NB64; b++) h.b64[b] = D.alloc<double>(D.vlen);
+    for (int b = 0; b < NB32; b++) h.b32[b] = D.alloc<float>(D.vlen);
+    h.lzero = D.alloc<int>(kMaxLayers);
+    h.blk_in = D.alloc<double>(D.nblk);
+    h.blk_out = D.alloc<double>(D.nblk);
+    D.h_blk = (double *)hmalloc(D.hpool, D.nblk * sizeof(double));
+    D.h_scal = (double *)hmalloc(D.hpool, NSCAL * sizeof(double));
+    h.partial = D.alloc<double>((size_t)slot_count(h) * kRed);
+    h.counter = D.alloc<unsigned>(1);
+    h.scal = D.alloc<double>(NSCAL);
+    h.seq_in = h.seq_out = nullptr;
+    h.seq_one = 0;
+    h.gm = h.gpart = nullptr;
+    h.gmv32 = nullptr;
+    h.gmv64 = nullptr;
+}
+
+// The single-model group of a device model (its descriptor and work-lists)
+// with its Taylor buffers and the float4 term work-list.
+static void single_group(DevModel &D) {
+    D.grp.d_desc = D.d_desc;
+    D.grp.nprob = 1;
+    D.grp.d_work = D.d_work;
+    D.grp.nwork = D.nwork;
+    D.grp.d_gwork = D.d_gwork;
+    D.grp.ngwork = D.ngwork;
+    D.grp.stream = D.stream;
+    D.grp.hetero = D.hetero;
+    D.grp.vec4 = vec4_ok(D.h);
+    D.grp.d_kmax = D.alloc<int>(2);
+    D.grp.d_trec = D.alloc<double>(kTrec);
+    D.grp.nper = D.nwork;
+    std::vector<WorkItem> tw;
+    const int per = term_items({D.h}, tw);
+    upload_term_items(D.grp, tw, per, D.cap_twork, D.pool, D.stream);
+}
+
 // Device state of a model: coefficients, maps, work-lists and the solver
 // buffers.  With `shared` the launches go to that stream (the engine's);
-// with pools, device and pinned host memory come from them.
-static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr, DevPool *hpool = nullptr) {
+// with pools, device and pinned host memory come from them.  Without
+// with_state only the read-only data is built (a template for dev_clone).
+static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr, DevPool *hpool = nullptr,
+                           bool with_state = true) {
     grid_model_t &g = *M.g;
     if (g.n_layers > kMaxLayers) fatal("too many layers for the device model\n");
     // the kernels address a vector through 32-bit byte offsets (8 B a node)
@@ -908,16 +978,8 @@ static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr
     const long ncell = M.ncell();
     D->vlen = ((ncell + M.nx + 63) / 64) * 64;
     h.vlen = (unsigned)D->vlen;
-    for (int b = 0; b < NB64; b++) {
-        h.b64[b] = D->alloc<double>(D->vlen);
-        dzero(h.b64[b], D->vlen * sizeof(double), D->stream);
-    }
-    h.lzero = D->alloc<int>(kMaxLayers);
-    dzero(h.lzero, kMaxLayers * sizeof(int), D->stream);
-    for (int b = 0; b < NB32; b++) {
-        h.b32[b] = D->alloc<float>(D->vlen);
-        dzero(h.b32[b], D->vlen * sizeof(float), D->stream);
-    }
+    D->nblk = g.total_n_blocks + M.nx;
+    if (with_state) alloc_state(*D);
     // block <-> grid maps
     std::vector<int> base;
     layer_bases(M, base);
@@ -1027,16 +1089,6 @@ static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr
     h.nblocks = g.total_n_blocks;
     h.blk_range = D->upload(rng.data(), rng.size());
     h.blk_layer = D->upload(blayer.data(), blayer.size());
-    D->nblk = g.total_n_blocks + M.nx;
-    h.blk_in = D->alloc<double>(D->nblk);
-    h.blk_out = D->alloc<double>(D->nblk);
-    D->h_blk = (double *)hmalloc(hpool, D->nblk * sizeof(double));
-    D->h_scal = (double *)hmalloc(hpool, NSCAL * sizeof(double));
-    h.partial = D->alloc<double>((size_t)slot_count(h) * kRed);
-    h.counter = D->alloc<unsigned>(1);
-    dzero(h.counter, sizeof(unsigned), D->stream);
-    h.scal = D->alloc<double>(NSCAL);
-    dzero(h.scal, NSCAL * sizeof(double), D->stream);
     D->d_desc = D->alloc<ProbDesc>(1);
     h2d(D->d_desc, &h, sizeof(ProbDesc), D->stream);
     // work lists
@@ -1049,24 +1101,7 @@ static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr
     gw.push_back(WorkItem{0, -1});
     D->ngwork = (int)gw.size();
     D->d_gwork = D->upload(gw.data(), gw.size());
-    D->grp.d_desc = D->d_desc;
-    D->grp.nprob = 1;
-    D->grp.d_work = D->d_work;
-    D->grp.nwork = D->nwork;
-    D->grp.d_gwork = D->d_gwork;
-    D->grp.ngwork = D->ngwork;
-    D->grp.stream = D->stream;
-    D->grp.hetero = M.detailed;
-    D->grp.vec4 = vec4_ok(h);
-    D->grp.d_kmax = D->alloc<int>(2);
-    dzero(D->grp.d_kmax, 2 * sizeof(int), D->stream);
-    D->grp.d_trec = D->alloc<double>(kTrec);
-    D->grp.nper = D->nwork;
-    {
-        std::vector<WorkItem> tw;
-        const int per = term_items({h}, tw);
-        upload_term_items(D->grp, tw, per, D->cap_twork, pool, D->stream);
-    }
+    if (with_state) single_group(*D);
     if (D->hetero) assemble_hetero(M, *D);
     // transient step bound
     if (g.c_ready) {
@@ -1086,6 +1121,37 @@ static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr
         D->rho = std::max(D->rho, package_rho(M, h));
     }
     h2d(D->d_desc, &h, sizeof(ProbDesc), D->stream);
+    return own.release();
+}
+
+// A device model of the template's model that shares the template's
+// read-only data and allocates only its own state: what a job on a model
+// already resident on the device needs.
+static DevModel *dev_clone(const std::shared_ptr<DevModel> &tpl, hipStream_t shared, DevPool *pool,
+                           DevPool *hpool) {
+    std::unique_ptr<DevModel> own(new DevModel());
+    DevModel *D = own.get();
+    D->base = tpl;
+    D->pool = pool;
+    D->hpool = hpool;
+    D->stream = shared;
+    D->own_stream = false;
+    HIP_OK(hipEventCreate(&D->ev0));
+    HIP_OK(hipEventCreate(&D->ev1));
+    D->h = tpl->h;
+    D->hetero = tpl->hetero;
+    D->has_c = tpl->has_c;
+    D->vlen = tpl->vlen;
+    D->nblk = tpl->nblk;
+    D->rho = tpl->rho;
+    D->d_work = tpl->d_work;
+    D->nwork = tpl->nwork;
+    D->d_gwork = tpl->d_gwork;
+    D->ngwork = tpl->ngwork;
+    alloc_state(*D);
+    D->d_desc = D->alloc<ProbDesc>(1);
+    single_group(*D);
+    h2d(D->d_desc, &D->h, sizeof(ProbDesc), D->stream);
     return own.release();
 }
 
@@ -2036,25 +2102,45 @@ static bool graphs_enabled() { return g_timer.cap == 0; }
 struct Replayer {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    bool stale = false;   // the body's launches changed (problems, buffers): capture again
     void reset() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr;
         graph = nullptr;
+        stale = false;
     }
     ~Replayer() { reset(); }
     // runs body `steps` times on the stream: plain launches while a kernel
-    // timer or profiler wants them, else a graph captured on first use
+    // timer or profiler wants them, else a graph captured on first use.  A
+    // stale graph is captured again and its executable updated in place
+    // when the new capture has the same nodes (only their launch parameters
+    // differ: grid sizes, buffers), which costs less than instantiating it.
     template <class F> void run(hipStream_t s, int steps, F body) {
         if (!graphs_enabled()) {
             for (int k = 0; k < steps; k++) body();
             return;
         }
-        if (!exec) {
+        if (!exec || stale) {
+            hipGraph_t g = nullptr;
             HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             body();
-            HIP_OK(hipStreamEndCapture(s, &graph));
-            HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            HIP_OK(hipStreamEndCapture(s, &g));
+            bool updated = false;
+            if (exec) {
+                hipGraphExecUpdateResult why;
+                hipGraphNode_t bad = nullptr;
+                updated = hipGraphExecUpdate(exec, g, &bad, &why) == hipSuccess && why == hipGraphExecUpdateSuccess;
+                (void)hipGetLastError();
+            }
+            if (!updated) {
+                if (exec) (void)hipGraphExecDestroy(exec);
+                exec = nullptr;
+                HIP_OK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+            }
+            if (graph) (void)hipGraphDestroy(graph);
+            graph = g;
+            stale = false;
         }
         for (int k = 0; k < steps; k++) HIP_OK(hipGraphLaunch(exec, s));
     }
