@@ -2921,6 +2921,7 @@ struct AsmIn {
     const int *types;                   // per cell network cell type
     const double *hydro, *cv, *tin;     // per layer: hydraulic conductance, coolant capacity, inlet T
     int nl, nr, nc, detailed;
+    int superlu_outlets;                // outlet cells as the SuperLU build's matrix has them
     double cw, ch;
 };
 
@@ -3169,6 +3170,23 @@ __global__ void k_assemble(AsmIn in, double *gE, double *gS, double *gD, double 
         const int t = in.types[c];
         if (t == CELL_INLET && edge) b = Cv * (-inflow) * in.tin[n];
         else if (t == CELL_OUTLET && edge) self = Cv * (-inflow);
+        if (t == CELL_OUTLET && in.superlu_outlets) {
+            // the SuperLU build's matrix (build_transient_grid_matrix,
+            // temperature_grid.c:5313-5345) takes an outlet's advection from
+            // its first fluid neighbour alone, in the order N, S, W, E, as
+            // -Cv Q/2 to that neighbour and +Cv Q/2 on the diagonal (at any
+            // outlet, on the edge or not)
+            const int first = (i > 0 && in.fluid[c - in.nc]) ? 0
+                              : (i < in.nr - 1 && in.fluid[c + in.nc]) ? 1
+                              : (j > 0 && in.fluid[c - 1]) ? 2
+                              : (j < in.nc - 1 && in.fluid[c + 1]) ? 3 : -1;
+            const double a = first == 0 ? an : first == 1 ? as : first == 2 ? aw : first == 3 ? ae : 0.0;
+            an = first == 0 ? a : 0.0;
+            as = first == 1 ? a : 0.0;
+            aw = first == 2 ? a : 0.0;
+            ae = first == 3 ? a : 0.0;
+            self = first >= 0 ? -2.0 * a : 0.0;
+        }
     }
     aN[c] = an;
     aS[c] = as;

@@ -809,6 +809,7 @@ static void assemble_hetero(Model &M, DevModel &D) {
     in.nr = nr;
     in.nc = nc;
     in.detailed = M.detailed;
+    in.superlu_outlets = M.superlu_outlets ? 1 : 0;
     in.cw = g.width / nc;
     in.ch = g.height / nr;
     ProbDesc &h = D.h;

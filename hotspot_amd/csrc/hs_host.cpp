@@ -1724,6 +1724,17 @@ void hs_set_options(RC_model_t *model, const hs_options_t *opt) {
     HS_END()
 }
 
+void hs_set_outlet_rule(RC_model_t *model, int superlu) {
+    HS_BEGIN
+    if (!model || model->type == BLOCK_MODEL) fatal("the outlet rule applies to grid models\n");
+    Model &M = *model_of(model);
+    if (M.superlu_outlets == (superlu != 0)) return;
+    M.superlu_outlets = superlu != 0;
+    M.dev_dirty = true;
+    M.stamp = next_stamp();
+    HS_END()
+}
+
 void hs_get_stats(RC_model_t *model, hs_stats_t *stats) {
     HS_BEGIN
     if (model && model->type == BLOCK_MODEL) {

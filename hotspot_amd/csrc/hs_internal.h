@@ -103,6 +103,9 @@ struct Model {
     int nx = EXTRA;
     int spidx = -1, hsidx = -1, subidx = -1, solderidx = -1, pcbidx = -1;
     bool detailed = false;
+    // coolant advection at outlet cells as the reference's SuperLU build
+    // assembles it (only the first fluid neighbour, N, S, W, E)
+    bool superlu_outlets = false;
     hs_options_t opt{};
     hs_stats_t stats{};
     DevModel *dev = nullptr;

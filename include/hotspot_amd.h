@@ -461,6 +461,18 @@ typedef struct hs_stats_st {
     int be_iters;             /* CG iterations of the last backward-Euler interval */
 } hs_stats_t;
 
+/* Coolant advection at outlet cells: 0 (default) as the reference's slope
+ * (slope_fn_grid, temperature_grid.c:3256-3590) has it, every fluid
+ * neighbour's flow plus the outflow at an edge outlet; 1 as the matrix of
+ * the reference's SuperLU build (build_transient_grid_matrix,
+ * temperature_grid.c:5313-5345) has it, the first fluid neighbour alone in
+ * the order N, S, W, E, at every outlet.  The two agree where each outlet
+ * has one fluid neighbour (examples/example5); they differ where outlets
+ * touch (channels more than one cell wide), whose outflow the SuperLU build
+ * takes from an outlet neighbour carrying no flow.  Applies to every solve
+ * of the model.  No counterpart in the reference, which picks one by its
+ * build (SUPERLU). */
+void hs_set_outlet_rule(RC_model_t *model, int superlu);
 void hs_get_options(RC_model_t *model, hs_options_t *opt);
 void hs_set_options(RC_model_t *model, const hs_options_t *opt);
 void hs_get_stats(RC_model_t *model, hs_stats_t *stats);

@@ -340,7 +340,12 @@ class OracleModel:
 
     def __init__(self, config: dict | None = None, flp: str | None = None,
                  lcf: str | None = None, detailed_3D=False, use_microchannels=False,
-                 microchannel: dict | None = None, materials: dict | None = None):
+                 microchannel: dict | None = None, materials: dict | None = None,
+                 superlu_outlets=False):
+        # superlu_outlets: outlet cells' advection as the SuperLU build's
+        # matrix has it (build_transient_grid_matrix, temperature_grid.c:
+        # 5313-5345) instead of the slope's (:3256-3590)
+        self.superlu_outlets = bool(superlu_outlets)
         cfg = dict(DEFAULT_CONFIG)
         cfg.update(config or {})
         if lcf is not None:
@@ -726,6 +731,16 @@ class OracleModel:
                     if not fl[i, j]:
                         continue
                     me = idx[n, i, j]
+                    if self.superlu_outlets and types[i, j] == OUTLET:
+                        # the first fluid neighbour alone (N, S, W, E): -Cv Q/2
+                        # to it, +Cv Q/2 on the diagonal (Jn/Js/Jw/Je, Jc)
+                        for (a, cc_) in ((i - 1, j), (i + 1, j), (i, j - 1), (i, j + 1)):
+                            if 0 <= a < nr and 0 <= cc_ < nc and fl[a, cc_]:
+                                coef = Cv * (p[a, cc_] - p[i, j]) * hC
+                                adv_r += [me, me]; adv_c += [idx[n, a, cc_], me]
+                                adv_v += [-coef / 2.0, coef / 2.0]
+                                break
+                        continue
                     inflow = 0.0
                     for (a, cc_) in ((i - 1, j), (i + 1, j), (i, j - 1), (i, j + 1)):
                         if 0 <= a < nr and 0 <= cc_ < nc and fl[a, cc_]:

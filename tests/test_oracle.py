@@ -319,3 +319,55 @@ def test_backward_euler_secondary_model():
     cbe = np.where(np.arange(om.N) < om.ncell + O.EXTRA, om.Cvec, 0.0)
     bad2 = _be_with_caps(om, y0, Pg, 1e9, cbe, om.Gt)
     assert np.abs(bad2 - tinf).max() > 1e-3
+
+
+def _mc_oracle(width_cells, superlu):
+    d = tempfile.mkdtemp()
+    case, _ = cases.microchannel_case(16, 5, d, width_cells=width_cells, pitch_cells=width_cells + 2)
+    m = case.materialize(d)
+    return O.OracleModel(m["config"], lcf=m["lcf"], detailed_3D=True, use_microchannels=True,
+                         microchannel=case.microchannel, superlu_outlets=superlu), d
+
+
+def test_superlu_outlet_rule():
+    """The SuperLU build's outlet rows (temperature_grid.c:5313-5345): an
+    outlet takes its advection from its first fluid neighbour alone, in the
+    order N, S, W, E.  Where every outlet has one fluid neighbour (one-cell
+    channels between solid rows, as examples/example5) the operator equals
+    the slope's exactly.  Where outlets touch (two-cell channels), an outlet
+    whose first fluid neighbour is another outlet (no flow between them)
+    loses its advection: the difference of the two operators is, in that row
+    alone, the slope's terms restated here -- -Cv Q/2 to the upstream cell,
+    Cv Q/2 on the diagonal.  A rule that took the upstream (W) neighbour
+    first would equal the slope's there and leave that row difference zero."""
+    a, _ = _mc_oracle(1, False)
+    b, _ = _mc_oracle(1, True)
+    assert abs(a.G - b.G).max() == 0.0
+    assert np.array_equal(a.b, b.b)
+    s, _ = _mc_oracle(2, False)
+    r, _ = _mc_oracle(2, True)
+    D = (s.G - r.G).tocsr()
+    lay = next(n for n in range(s.nl) if s.layers[n].is_microchannel)
+    mc = s.layers[lay].mc
+    types, p, hC, Cv = mc["cell_types"], mc["pressure"], mc["hydroC"], mc["coolant_capac"]
+    nr, nc = types.shape
+    rows = set(D.nonzero()[0])
+    outlets = 0
+    for i in range(nr):
+        for j in range(nc):
+            me = (lay * nr + i) * nc + j
+            if types[i, j] != O.OUTLET:
+                assert me not in rows
+                continue
+            outlets += 1
+            # the two-cell channels' outlets: the first fluid neighbour (N or
+            # S) is the other outlet of the channel, the upstream cell is W
+            assert types[i - 1, j] == O.OUTLET or types[i + 1, j] == O.OUTLET
+            q = (p[i, j - 1] - p[i, j]) * hC
+            assert q > 0
+            row = D.getrow(me).toarray().ravel()
+            want = np.zeros_like(row)
+            want[me - 1] = -Cv * q / 2.0
+            want[me] = -Cv * q / 2.0 + Cv * q     # the edge outlet's outflow
+            assert np.allclose(row, want, rtol=1e-12, atol=1e-15 * abs(s.G).max())
+    assert outlets >= 4
