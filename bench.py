@@ -36,10 +36,12 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 # The engine runs its set-up, steady, transient and backward-Euler work on
-# ten streams; with a hardware queue each, no stream's work waits behind
-# another's launches in a shared queue (HIP's default is four queues per
-# process).
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "12")
+# ten streams.  HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware
+# queues (four by default); streams that share a queue run in its order, so
+# a steady solve's set-up would wait behind a transient chunk of another
+# stream.  Sixteen queues give every stream its own; set before the first
+# HIP call.
+os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 from hotspot_amd import cases, engine, iface  # noqa: E402
 

@@ -74,6 +74,12 @@ struct EngJob {
     package_config_t pkg{};
     SinkModel sink;
     double r_convec = 0.0;              // of the solve in progress, then of the model after the last
+    // a finished transient's sequence on its way to the host: pinned
+    // staging and the event after the copy (the job is done when it completes)
+    double *h_seq = nullptr;
+    size_t dl_n = 0;                    // doubles in flight
+    bool dl_seq = false;                // the sequence (else the final block vector)
+    hipEvent_t ev_dl = nullptr;
     std::string error;                  // J_FAILED: the reason
 };
 
@@ -222,6 +228,7 @@ struct Engine {
     size_t resident_bytes = 0;
     std::vector<ModelConst> consts;     // read-only device data per model
     size_t const_bytes = 0;
+    std::vector<EngJob *> downloading;  // finished transients whose sequences are being copied
     hs_engine_stats_t stats{};
     // steady groups by key: kind (0 homogeneous CG, 1 heterogeneous CG,
     // 2 non-symmetric GMRES) + 4 x multigrid levels (0: Jacobi);
@@ -234,6 +241,10 @@ struct Engine {
     ~Engine() {
         for (hipStream_t s : streams())
             if (s) (void)hipStreamSynchronize(s);
+        for (EngJob *j : downloading) {
+            hfree(&hpool, j->h_seq);
+            (void)hipEventDestroy(j->ev_dl);
+        }
         for (auto &kv : jobs) kv.second->D.reset();
         resident.clear();
         consts.clear();
@@ -251,6 +262,8 @@ struct Engine {
         return v;
     }
 };
+
+static void start_download(Engine &E, EngJob &j, bool seq = true);
 
 // Device bytes dev_build allocates for a model, plus the sequence buffers
 // and, for a backward-Euler transient (be), the shifted operator's arrays.
@@ -572,7 +585,8 @@ static double be_rtol(const Engine &E) { return 1e-2 * E.opt.steady_rtol; }
 
 // Starts the solve of the job's current interval (S_STEP): its power, the
 // right-hand side from the state, the solver's scalars.  On the set-up
-// stream, synchronised: the job's group is between chunks.
+// stream without a host wait: the job's group is between chunks, and its
+// next chunk waits for the set-up stream on the device.
 static void be_begin(Engine &E, EngJob &j) {
     DevModel &D = *j.D;
     run_b2g(D.grp, V_POWER, B_RHS, 1, 1);
@@ -583,14 +597,13 @@ static void be_begin(Engine &E, EngJob &j) {
         launch_named("k_be_rhs", k_be_rhs<false>, D.grp.nwork, D.stream, (const ProbDesc *)D.d_desc,
                      (const WorkItem *)D.d_work, (int)B_Y0, (int)B_RHS, (int)B_T1, 1.0 / j.dt);
     if (j.adv) {
-        write_scal(D, kMixedStart);
+        set_scal(D, kMixedStart);
     } else {
-        write_scal(D, {{S_DONE, 0.0}});   // the CG start skips a problem still marked done
+        set_scal(D, {{S_DONE, 0.0}});   // the CG start skips a problem still marked done
         const double r = be_rtol(E);
         if (j.hetero) jpcg64_start<true>(D.be_grp, r * r, E.opt.max_iters, B_Y0, B_T1);
         else jpcg64_start<false>(D.be_grp, r * r, E.opt.max_iters, B_Y0, B_T1);
     }
-    HIP_OK(hipStreamSynchronize(D.stream));
 }
 
 static int be_key(const EngJob &j) { return (j.adv ? 2 : j.hetero ? 1 : 0) + 4 * (j.adv ? (int)j.D->be_mg.size() : 0); }
@@ -616,7 +629,7 @@ static void start_transient(Engine &E, EngJob &j) {
     auto kv = seq_slots(j.nsteps, j.dt);
     kv.push_back({S_YPAR, 0.0});
     if (E.opt.transient == HS_TRANS_BE) {
-        write_scal(D, kv);
+        set_scal(D, kv);
         j.phase = J_TRANS;
         j.step_seen = 0;
         start_be(E, j);
@@ -628,7 +641,7 @@ static void start_transient(Engine &E, EngJob &j) {
         kv.insert(kv.end(), t.begin(), t.end());
         j.kplan = p.K;
     }
-    write_scal(D, kv);
+    set_scal(D, kv);
     j.phase = J_TRANS;
     j.step_seen = 0;
     // the part with the fewest members; one part only while a kernel timer
@@ -655,19 +668,20 @@ static void start_steady(Engine &E, EngJob &j) {
         // in every form (Jacobi does not converge at channel Peclet numbers)
         ensure_gmres(D, true);
         build_mg(D);
-        write_scal(D, kMixedStart);
+        set_scal(D, kMixedStart);
     } else if (j.use_mg) {
         build_mg(D);
-        write_scal(D, kMixedStart);
+        set_scal(D, kMixedStart);
     } else if (E.opt.precision == HS_PREC_MIXED) {
-        write_scal(D, kMixedStart);
+        set_scal(D, kMixedStart);
     } else {
-        write_scal(D, {{S_DONE, 0.0}});
+        set_scal(D, {{S_DONE, 0.0}});
         const double rtol2 = E.opt.steady_rtol * E.opt.steady_rtol;
         if (j.hetero) jpcg64_start<true>(D.grp, rtol2, E.opt.max_iters);
         else jpcg64_start<false>(D.grp, rtol2, E.opt.max_iters);
     }
-    HIP_OK(hipStreamSynchronize(D.stream));
+    // no host wait: joining the group changes its membership, and its next
+    // chunk waits on the device for the set-up stream's work so far
     j.phase = J_STEADY;
     BatchGroup &B = E.steady[steady_key(j)];
     B.members.push_back(&j);
@@ -696,6 +710,10 @@ static void admit(Engine &E, EngJob &j) {
     }
     DevModel &D = *j.D;
     const size_t nb = (size_t)D.nblk;
+    // the interval powers go through pinned staging (a copy from pageable
+    // memory would wait on the host for the transfer), returned once the
+    // admission's last wait on the set-up stream has passed
+    void *stage = nullptr;
     if (j.nsteps > 0) {
         // interval powers: one row for all, one per interval, or the steady row first
         const int first = j.rows == j.nsteps + 1 ? 1 : 0;
@@ -703,7 +721,10 @@ static void admit(Engine &E, EngJob &j) {
         D.h.seq_in = D.alloc<double>(in_rows * nb);
         D.h.seq_out = D.alloc<double>((size_t)j.nsteps * nb);
         D.h.seq_one = j.rows == 1 ? 1 : 0;
-        h2d(D.h.seq_in, j.power.data() + first * nb, in_rows * nb * sizeof(double), D.stream);
+        const size_t bytes = in_rows * nb * sizeof(double);
+        stage = hmalloc(&E.hpool, bytes);
+        memcpy(stage, j.power.data() + first * nb, bytes);
+        HIP_OK(hipMemcpyAsync(D.h.seq_in, stage, bytes, hipMemcpyHostToDevice, D.stream));
         h2d(D.d_desc, &D.h, sizeof(ProbDesc), D.stream);
     }
     j.held = D.bytes;
@@ -739,6 +760,8 @@ static void admit(Engine &E, EngJob &j) {
     // device memory held, multigrid levels and GMRES state included
     E.used += D.bytes - j.held;
     j.held = D.bytes;
+    HIP_OK(hipStreamSynchronize(D.stream));
+    hfree(&E.hpool, stage);
     j.setup_s = now_s() - j.t_admit;
 }
 
@@ -815,31 +838,60 @@ static void steady_done(Engine &E, EngJob &j, const double *flags) {
         return;
     }
     if (j.nsteps == 0) {
-        j.result.resize(D.nblk);
-        download_blocks(D, j.result.data());
-        j.phase = J_DONE;
-        j.t_done = now_s();
-        release_device(E, j);
+        start_download(E, j, false);   // done when the block vector has arrived
         return;
     }
     run_b2g(D.grp, V_TEMP, B_Y0, 0, 2);
-    HIP_OK(hipStreamSynchronize(D.stream));
     start_transient(E, j);
 }
 
-static void trans_done(Engine &E, EngJob &j, const double *flags) {
+// A finished transient's sequence goes to pinned host memory without a host
+// wait; the job is done once the copy has completed (finish_downloads), and
+// its device state is released then.
+static void start_download(Engine &E, EngJob &j, bool seq) {
     DevModel &D = *j.D;
+    j.dl_seq = seq;
+    j.dl_n = seq ? (size_t)j.nsteps * D.nblk : (size_t)D.nblk;
+    const size_t bytes = j.dl_n * sizeof(double);
+    j.h_seq = (double *)hmalloc(&E.hpool, bytes);
+    HIP_OK(hipMemcpyAsync(j.h_seq, seq ? D.h.seq_out : D.h.blk_out, bytes, hipMemcpyDeviceToHost, D.stream));
+    HIP_OK(hipEventCreateWithFlags(&j.ev_dl, hipEventDisableTiming));
+    HIP_OK(hipEventRecord(j.ev_dl, D.stream));
+    E.downloading.push_back(&j);
+}
+
+static void finish_downloads(Engine &E) {
+    for (size_t k = 0; k < E.downloading.size();) {
+        EngJob &j = *E.downloading[k];
+        const hipError_t r = hipEventQuery(j.ev_dl);
+        if (r == hipErrorNotReady) {
+            k++;
+            continue;
+        }
+        HIP_OK(r);
+        if (j.dl_seq) {
+            j.seq.assign(j.h_seq, j.h_seq + j.dl_n);
+            j.result.assign(j.seq.end() - j.D->nblk, j.seq.end());
+        } else {
+            j.result.assign(j.h_seq, j.h_seq + j.dl_n);
+        }
+        hfree(&E.hpool, j.h_seq);
+        HIP_OK(hipEventDestroy(j.ev_dl));
+        j.h_seq = nullptr;
+        j.ev_dl = nullptr;
+        j.phase = J_DONE;
+        j.t_done = now_s();
+        release_device(E, j);
+        E.downloading.erase(E.downloading.begin() + k);
+    }
+}
+
+static void trans_done(Engine &E, EngJob &j, const double *flags) {
     j.rk4_calls = E.opt.transient == HS_TRANS_RK4 ? (int)flags[1] : 0;
     j.taylor_sweeps = E.opt.transient == HS_TRANS_TAYLOR ? (long)flags[2] : 0;
     j.taylor_node_bytes = E.opt.transient == HS_TRANS_TAYLOR ? flags[3] : 0.0;
     j.holds = (int)flags[4];
-    const size_t n = (size_t)j.nsteps * D.nblk;
-    j.seq.resize(n);
-    d2h(j.seq.data(), D.h.seq_out, n * sizeof(double), D.stream);
-    j.result.assign(j.seq.end() - D.nblk, j.seq.end());
-    j.phase = J_DONE;
-    j.t_done = now_s();
-    release_device(E, j);
+    start_download(E, j);
 }
 
 // A converged interval solve of a backward-Euler job (flags: S_DONE,
@@ -860,18 +912,12 @@ static bool be_step_done(Engine &E, EngJob &j, const double *flags) {
     }
     run_g2b(D.grp, B_Y0, 1);
     j.step_seen++;
-    write_scal(D, {{S_STEP, (double)j.step_seen}});
+    set_scal(D, {{S_STEP, (double)j.step_seen}});
     if (j.step_seen < j.nsteps) {
         be_begin(E, j);
         return false;
     }
-    const size_t n = (size_t)j.nsteps * D.nblk;
-    j.seq.resize(n);
-    d2h(j.seq.data(), D.h.seq_out, n * sizeof(double), D.stream);
-    j.result.assign(j.seq.end() - D.nblk, j.seq.end());
-    j.phase = J_DONE;
-    j.t_done = now_s();
-    release_device(E, j);
+    start_download(E, j);
     return true;
 }
 
@@ -1049,6 +1095,10 @@ static void wait_chunk(Engine &E) {
             busy |= kv.second.busy;
             done |= chunk_done(kv.second);
         }
+        for (EngJob *j : E.downloading) {
+            busy = true;
+            done |= hipEventQuery(j->ev_dl) != hipErrorNotReady;
+        }
         if (!busy || done || now_s() - t0 > kWaitMs * 1e-3) return;
         std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
@@ -1057,6 +1107,7 @@ static void wait_chunk(Engine &E) {
 // results of every group whose chunk has ended: finished jobs leave their
 // groups (a steady solve with intervals to follow joins a transient group)
 static void harvest(Engine &E) {
+    finish_downloads(E);
     for (auto &kv : E.be) {
         BatchGroup &B = kv.second;
         if (!chunk_done(B)) continue;
@@ -1065,10 +1116,18 @@ static void harvest(Engine &E) {
         HIP_OK(hipEventElapsedTime(&ms, B.ev_a, B.ev_b));
         E.stats.transient_chunk_s += 1e-3 * ms;
         std::vector<EngJob *> fin;
+        bool begun = false;   // a member started its next interval on the set-up stream
         for (int i = 0; i < B.ran; i++)
-            if (B.h_flags[i * 8] != 0.0 && be_step_done(E, *B.members[i], B.h_flags + i * 8))
-                fin.push_back(B.members[i]);
+            if (B.h_flags[i * 8] != 0.0) {
+                if (be_step_done(E, *B.members[i], B.h_flags + i * 8)) fin.push_back(B.members[i]);
+                else begun = true;
+            }
         for (EngJob *j : fin) B.remove(j);
+        if (begun) {   // the group's next chunk waits for those starts on the device
+            if (!B.ev_sync) HIP_OK(hipEventCreateWithFlags(&B.ev_sync, hipEventDisableTiming));
+            HIP_OK(hipEventRecord(B.ev_sync, E.s_setup));
+            B.synced = true;
+        }
     }
     for (auto &kv : E.steady) {
         BatchGroup &B = kv.second;
@@ -1147,7 +1206,7 @@ int eng_active(Engine &E) {
     for (auto &kv : E.steady) n += (int)kv.second.members.size();
     for (auto &g : E.trans) n += (int)g.members.size();
     for (auto &kv : E.be) n += (int)kv.second.members.size();
-    return n;
+    return n + (int)E.downloading.size();
 }
 
 void eng_set_options(Engine &E, const hs_options_t &o) {

@@ -1996,6 +1996,17 @@ __global__ void k_gather(const ProbDesc *__restrict__ D, int nprob, Slots sl, do
     for (int i = 0; i < sl.n; i++) out[p * 8 + i] = D[p].scal[sl.s[i]];
 }
 
+// sets scalar slots of one problem from the launch's arguments (no host
+// copy: the values travel with the launch)
+struct ScalSet {
+    int n;
+    int slot[16];
+    double val[16];
+};
+__global__ void k_set_scal(double *scal, ScalSet s) {
+    if ((int)threadIdx.x < s.n) scal[s.slot[threadIdx.x]] = s.val[threadIdx.x];
+}
+
 // writes scalar slots of every problem: vals[p * 8 + i] into slot sl.s[i]
 __global__ void k_scatter(const ProbDesc *__restrict__ D, int nprob, Slots sl, const double *vals) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;

@@ -1189,6 +1189,19 @@ static void write_scal(DevModel &D, const std::vector<std::pair<int, double>> &k
     HIP_OK(hipStreamSynchronize(D.stream));
 }
 
+// the given scalar slots set by a launch on the model's stream, without a
+// host wait (the engine's phase changes)
+static void set_scal(DevModel &D, const std::vector<std::pair<int, double>> &kv) {
+    for (size_t k = 0; k < kv.size(); k += 16) {
+        ScalSet s{};
+        for (size_t q = k; q < std::min(kv.size(), k + 16); q++) {
+            s.slot[s.n] = kv[q].first;
+            s.val[s.n++] = kv[q].second;
+        }
+        hipLaunchKernelGGL(k_set_scal, dim3(1), dim3(64), 0, D.stream, D.h.scal, s);
+    }
+}
+
 static void upload_blocks(DevModel &D, const double *v) {
     memcpy(D.h_blk, v, D.nblk * sizeof(double));
     HIP_OK(hipMemcpyAsync(D.h.blk_in, D.h_blk, D.nblk * sizeof(double), hipMemcpyHostToDevice, D.stream));
