@@ -263,7 +263,7 @@ struct Engine {
     }
 };
 
-static void start_download(Engine &E, EngJob &j, bool seq = true);
+static void start_download(Engine &E, EngJob &j, bool seq = false);
 
 // Device bytes dev_build allocates for a model, plus the sequence buffers
 // and, for a backward-Euler transient (be), the shifted operator's arrays.
@@ -710,10 +710,6 @@ static void admit(Engine &E, EngJob &j) {
     }
     DevModel &D = *j.D;
     const size_t nb = (size_t)D.nblk;
-    // the interval powers go through pinned staging (a copy from pageable
-    // memory would wait on the host for the transfer), returned once the
-    // admission's last wait on the set-up stream has passed
-    void *stage = nullptr;
     if (j.nsteps > 0) {
         // interval powers: one row for all, one per interval, or the steady row first
         const int first = j.rows == j.nsteps + 1 ? 1 : 0;
@@ -721,10 +717,7 @@ static void admit(Engine &E, EngJob &j) {
         D.h.seq_in = D.alloc<double>(in_rows * nb);
         D.h.seq_out = D.alloc<double>((size_t)j.nsteps * nb);
         D.h.seq_one = j.rows == 1 ? 1 : 0;
-        const size_t bytes = in_rows * nb * sizeof(double);
-        stage = hmalloc(&E.hpool, bytes);
-        memcpy(stage, j.power.data() + first * nb, bytes);
-        HIP_OK(hipMemcpyAsync(D.h.seq_in, stage, bytes, hipMemcpyHostToDevice, D.stream));
+        h2d(D.h.seq_in, j.power.data() + first * nb, in_rows * nb * sizeof(double), D.stream);
         h2d(D.d_desc, &D.h, sizeof(ProbDesc), D.stream);
     }
     j.held = D.bytes;
@@ -760,8 +753,6 @@ static void admit(Engine &E, EngJob &j) {
     // device memory held, multigrid levels and GMRES state included
     E.used += D.bytes - j.held;
     j.held = D.bytes;
-    HIP_OK(hipStreamSynchronize(D.stream));
-    hfree(&E.hpool, stage);
     j.setup_s = now_s() - j.t_admit;
 }
 
@@ -838,16 +829,16 @@ static void steady_done(Engine &E, EngJob &j, const double *flags) {
         return;
     }
     if (j.nsteps == 0) {
-        start_download(E, j, false);   // done when the block vector has arrived
+        start_download(E, j);   // done when the block vector has arrived
         return;
     }
     run_b2g(D.grp, V_TEMP, B_Y0, 0, 2);
     start_transient(E, j);
 }
 
-// A finished transient's sequence goes to pinned host memory without a host
-// wait; the job is done once the copy has completed (finish_downloads), and
-// its device state is released then.
+// A finished job's block vector (or sequence) goes to pinned host memory
+// without a host wait; the job is done once the copy has completed
+// (finish_downloads), and its device state is released then.
 static void start_download(Engine &E, EngJob &j, bool seq) {
     DevModel &D = *j.D;
     j.dl_seq = seq;
@@ -886,12 +877,26 @@ static void finish_downloads(Engine &E) {
     }
 }
 
+// A finished transient's sequence, copied to the host straight into the
+// job's result (a sequence is megabytes: staging it through pinned memory
+// per job, from an engine's fresh pool, costs more than the copy)
+static void finish_sequence(Engine &E, EngJob &j) {
+    DevModel &D = *j.D;
+    const size_t n = (size_t)j.nsteps * D.nblk;
+    j.seq.resize(n);
+    d2h(j.seq.data(), D.h.seq_out, n * sizeof(double), D.stream);
+    j.result.assign(j.seq.end() - D.nblk, j.seq.end());
+    j.phase = J_DONE;
+    j.t_done = now_s();
+    release_device(E, j);
+}
+
 static void trans_done(Engine &E, EngJob &j, const double *flags) {
     j.rk4_calls = E.opt.transient == HS_TRANS_RK4 ? (int)flags[1] : 0;
     j.taylor_sweeps = E.opt.transient == HS_TRANS_TAYLOR ? (long)flags[2] : 0;
     j.taylor_node_bytes = E.opt.transient == HS_TRANS_TAYLOR ? flags[3] : 0.0;
     j.holds = (int)flags[4];
-    start_download(E, j);
+    finish_sequence(E, j);
 }
 
 // A converged interval solve of a backward-Euler job (flags: S_DONE,
@@ -917,7 +922,7 @@ static bool be_step_done(Engine &E, EngJob &j, const double *flags) {
         be_begin(E, j);
         return false;
     }
-    start_download(E, j);
+    finish_sequence(E, j);
     return true;
 }
 
