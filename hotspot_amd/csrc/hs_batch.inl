@@ -398,6 +398,7 @@ static void group_sync(Engine &E, BatchGroup &B, bool het, bool trans, bool be =
         B.g.tz = term_form(desc);
         B.g.tg = term_groups(desc);
         B.g.zfirst = false;
+        B.g.zpow = false;
     }
     // multigrid levels of a steady group (every member has the same count),
     // or of a backward-Euler group of non-symmetric steps

@@ -2564,12 +2564,39 @@ __device__ __forceinline__ void add_e1_vec4(const ProbDesc &d, int tile, const f
 // tile's outer halo from memory).  A cell's E_1 is first_cell's, as in
 // k_tay_first2, with the FP32 cell power of F_PW (none for a layer marked in
 // lzero).
+//
+// With ZP the kernel maps the power of the current sequence step itself, as
+// k_b2g_power32 would (problems of at most kZPowTab blocks with blk_w): the
+// block powers, the power of a cell inside each block (by block index) and
+// which layers carry power are tabulated in LDS; a cell reads its code and,
+// at a block boundary, its record of up to four blocks.  The values are
+// those k_b2g_power32 writes to F_PW.
 __device__ __forceinline__ double comp2(const hs_d2 &a, const hs_d2 &b, int k) { return k < 2 ? a[k] : b[k - 2]; }
-template <int GM>
+constexpr int kZPowTab = 1024;   // blocks of a problem whose cell powers k_tay_first4z tabulates in LDS
+typedef unsigned short hs_u4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ hs_u4 ld_u16x4(const unsigned short *base, unsigned i) {
+    return *(const HS_GLOBAL hs_u4 *)((const HS_GLOBAL char *)base + i * 2u);
+}
+template <int GM, bool ZP>
 __device__ __forceinline__ void first_vec4z(const ProbDesc &d, int ztile, const double *y, float *e1, double tau,
                                             double (&v)[1]) {
     // [layer parity][first row, last row (one row a thread: the same)][columns 0-1, 2-3][thread]
     __shared__ hs_d2 edge[2][GM][2][kBlock];
+    __shared__ float zcp[ZP ? kZPowTab : 1];    // power of a cell inside block b (ZP)
+    __shared__ double zbp[ZP ? kZPowTab : 1];   // power of block b (ZP)
+    __shared__ int zpl[ZP ? kMaxLayers : 1];    // layer n carries power (ZP)
+    if (ZP) {
+        const double *vb = b2g_source(d, 1);
+        if ((int)threadIdx.x < d.nl) zpl[threadIdx.x] = 0;
+        __syncthreads();
+        for (int b = threadIdx.x; b < d.nblocks; b += kBlock) {
+            const double p = vb[b];
+            zcp[b] = (float)((double)d.blk_w[b] * p);
+            zbp[b] = p;
+            if (p != 0.0) zpl[d.blk_layer[b]] = 1;
+        }
+        __syncthreads();
+    }
     const int ntj = term_ntj(d);
     const int ti = ztile / ntj, tj = ztile - ti * ntj;
     const int sh = d.t_shift;
@@ -2604,7 +2631,7 @@ __device__ __forceinline__ void first_vec4z(const ProbDesc &d, int ztile, const 
         const unsigned off = (unsigned)n * plane;
         const int p = n & 1;
         const FirstLayer L = first_layer(d, n);
-        const bool pz = d.lzero[n] != 0;
+        const bool pz = ZP ? zpl[n] == 0 : d.lzero[n] != 0;
         hs_d2 Fa = Ca[0], Fb = Cb[0], La = Ca[0], Lb = Cb[0];
 #pragma unroll
         for (int g = 1; g < GM; g++)
@@ -2646,7 +2673,34 @@ __device__ __forceinline__ void first_vec4z(const ProbDesc &d, int ztile, const 
             Ev[g] = __shfl_down(Ca[g][0], 1, tw);
             if (left) Wv[g] = j0 > 0 ? ld_at(y, c - 1) : Ca[g][0];
             if (right) Ev[g] = j0 + 4 < nc ? ld_at(y, c + 4) : Ca[g][0];
-            P[g] = pz ? make_float4(0.f, 0.f, 0.f, 0.f) : ld4(pw, c);
+            if (pz) {
+                P[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+            } else if (ZP) {
+                // one-block cells from the table; the records of boundary
+                // cells are all loaded before any is used
+                const hs_u4 k4 = ld_u16x4(d.cell_u16[n], c - off);
+                const int lb = d.layer_base[n];
+                const int4 *uw = d.cell_uw[n] + 2 * (c - off);
+                float q[4];
+                int4 ru[4], rw[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const unsigned u = k4[k];
+                    q[k] = u != kCellMixed ? zcp[lb + u] : 0.f;
+                    if (u == kCellMixed) {
+                        ru[k] = uw[2 * k];
+                        rw[k] = uw[2 * k + 1];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (k4[k] == kCellMixed)
+                        q[k] = (float)four_block_power(rw[k], zbp[lb + ru[k].x], zbp[lb + ru[k].y],
+                                                       zbp[lb + ru[k].z], zbp[lb + ru[k].w]);
+                P[g] = make_float4(q[0], q[1], q[2], q[3]);
+            } else {
+                P[g] = ld4(pw, c);
+            }
         }
         double ring[4];
 #pragma unroll
@@ -2693,11 +2747,13 @@ __device__ __forceinline__ void first_vec4z(const ProbDesc &d, int ztile, const 
 // The first term over the z-marching term work-list of a FP32 Taylor group
 // (power32 1): the partial max of a column tile goes to its slot, the
 // package item's to the slot after the last column tile (the finish reduces
-// t_ntiles / nl + 1 slots).
-template <int GM>
-__global__ void __launch_bounds__(kBlock) k_tay_first4z(const ProbDesc *__restrict__ D,
-                                                        const WorkItem *__restrict__ W, int nper, int e1b, int sub,
-                                                        int seq, int *__restrict__ kmax) {
+// t_ntiles / nl + 1 slots).  ZP (sequence intervals, seq 1): the cell power
+// of the current sequence step is mapped here, with no k_b2g_power32 before.
+// Three waves per SIMD (at most 168 VGPRs; the ZP form takes 170 unbounded).
+template <int GM, bool ZP>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+k_tay_first4z(const ProbDesc *__restrict__ D, const WorkItem *__restrict__ W, int nper, int e1b, int sub, int seq,
+              int *__restrict__ kmax) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *kmax = 0;   // raised by the finish of each problem
     const int b = work_index();
     int prob, ztile;
@@ -2715,7 +2771,7 @@ __global__ void __launch_bounds__(kBlock) k_tay_first4z(const ProbDesc *__restri
     float *e1 = d.b32[e1b];
     const double tau = d.scal[S_TAU1];
     double v[1] = {0};
-    if (ztile < d.t_ntiles / d.nl) first_vec4z<GM>(d, ztile, y, e1, tau, v);
+    if (ztile < d.t_ntiles / d.nl) first_vec4z<GM, ZP>(d, ztile, y, e1, tau, v);
     else first_package(d, y, rhs, e1, tau, 1, v);
     block_reduce<1, true>(v);
     write_partials<1>(d, ztile, v);

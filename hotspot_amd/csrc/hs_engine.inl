@@ -127,6 +127,7 @@ struct Group {
     int tz = 0;   // its tiles: 0 per layer for every problem, 1 z-marching for every problem, 2 mixed
     int tg = 2;   // most row groups of 4 rows per thread among its problems (t_rpt / 4)
     bool zfirst = false;   // the first term z-marches over the term work-list (k_tay_first4z)
+    bool zpow = false;     // ... and maps the cell power of sequence intervals itself (zpow_ok)
 };
 
 // A homogeneous problem whose tiles are at least 16 columns wide with at
@@ -348,6 +349,15 @@ static bool first_z_ok(const std::vector<ProbDesc> &desc) {
     return !desc.empty();
 }
 
+// Whether k_tay_first4z can map the cell power itself: every problem has at
+// most kZPowTab blocks, one-block codes on every layer and cells of one to
+// four blocks (blk_w set).
+static bool zpow_ok(const std::vector<ProbDesc> &desc) {
+    for (const ProbDesc &h : desc)
+        if (h.nblocks > kZPowTab || !h.blk_w) return false;
+    return !desc.empty();
+}
+
 // the float4 Horner step over the group's term work-list, in the kernel of
 // its tile form (fin: the step adds its result into the FP64 state)
 static void launch_term4(const Group &G, hipStream_t s, int e1b, int d0b, int d1b, int j, int fin, bool named) {
@@ -543,6 +553,7 @@ static void tune_term_tiling(Group &G, std::vector<ProbDesc> &desc, size_t &cap,
     G.tz = term_form(desc);
     G.tg = term_groups(desc);
     G.zfirst = first_z_ok(desc);
+    G.zpow = G.zfirst && zpow_ok(desc);
 }
 
 // Pins the float4 sweep tiling of the model's grid shape (layers, rows,
@@ -1004,6 +1015,10 @@ static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr
     std::vector<const float *> dw(M.maps.size(), nullptr);
     std::vector<const int *> dmoff(M.maps.size(), nullptr), dmcell(M.maps.size(), nullptr);
     std::vector<const int4 *> dmuw(M.maps.size(), nullptr);
+    // per map: the weights of one-block cells (host copy), and whether some
+    // cell is covered by no block or by five or more
+    std::vector<std::vector<float>> hw(M.maps.size());
+    std::vector<char> odd(M.maps.size(), 0);
     for (int n = 0; n < g.n_layers; n++) {
         const int mi = M.layer_map[n];
         h.map_ptr[n] = dptr[mi];
@@ -1033,7 +1048,10 @@ static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr
             };
             for (size_t c = 0; c < plane; c++) {
                 const int e = bm.cell_ptr[c], cnt = bm.cell_ptr[c + 1] - e;
-                if (cnt < 1 || cnt > 4) continue;
+                if (cnt < 1 || cnt > 4) {
+                    odd[mi] = 1;
+                    continue;
+                }
                 int u[4];
                 float w[4];
                 for (int q = 0; q < 4; q++) {
@@ -1056,6 +1074,7 @@ static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr
                 for (int u = 0; u < f->n_units; u++) w[u] = (float)(1.0 * h.cell_area / area[u]);
                 dcode[mi] = D->upload(code.data(), code.size());
                 dw[mi] = D->upload(w.data(), w.size());
+                hw[mi] = w;
                 // the boundary cells by tile of the standard tiling (rows of
                 // TY * rpt, columns of TX; tile ti * ntj + tj)
                 const int tx = 1 << h.tx_shift, rows = (kBlock >> h.tx_shift) * h.rpt;
@@ -1090,6 +1109,16 @@ static DevModel *dev_build(Model &M, hipStream_t shared, DevPool *pool = nullptr
     h.nblocks = g.total_n_blocks;
     h.blk_range = D->upload(rng.data(), rng.size());
     h.blk_layer = D->upload(blayer.data(), blayer.size());
+    {
+        std::vector<float> bw(g.total_n_blocks);
+        bool ok = true;
+        for (int n = 0; n < g.n_layers && ok; n++) {
+            const int mi = M.layer_map[n];
+            ok = dcode[mi] && !odd[mi] && base[n] + hw[mi].size() <= bw.size();
+            if (ok) std::copy(hw[mi].begin(), hw[mi].end(), bw.begin() + base[n]);
+        }
+        h.blk_w = ok && !bw.empty() ? D->upload(bw.data(), bw.size()) : nullptr;
+    }
     D->d_desc = D->alloc<ProbDesc>(1);
     h2d(D->d_desc, &h, sizeof(ProbDesc), D->stream);
     // work lists
@@ -1887,10 +1916,12 @@ static void taylor_interval(const Group &G, int m, int Kplan, int seq) {
     const int db0 = f32 ? F_D0 : B_KA, db1 = f32 ? F_D1 : B_KB, accb = f32 ? F_ACC : B_ACC;
     for (int s = 0; s < m; s++) {
         const bool zf = f32 && !HET && G.vec4 && G.zfirst;
-        if (zf)
-            launch_named("k_tay_first", G.tg == 1 ? k_tay_first4z<1> : k_tay_first4z<2>, G.ntwork, G.stream, dd,
-                         (const WorkItem *)G.d_twork, G.tnper, accb, s, seq, G.d_kmax);
-        else if (f32 && !HET && G.vec4)
+        if (zf) {
+            auto k = G.zpow && seq == 1 ? (G.tg == 1 ? k_tay_first4z<1, true> : k_tay_first4z<2, true>)
+                                        : (G.tg == 1 ? k_tay_first4z<1, false> : k_tay_first4z<2, false>);
+            launch_named("k_tay_first", k, G.ntwork, G.stream, dd, (const WorkItem *)G.d_twork, G.tnper, accb, s,
+                         seq, G.d_kmax);
+        } else if (f32 && !HET && G.vec4)
             launch_named("k_tay_first", k_tay_first2, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax, 1);
         else
             launch_named("k_tay_first", k_tay_first<TD, HET>, G.nwork, G.stream, dd, w, accb, s, seq, G.d_kmax);
@@ -2177,8 +2208,12 @@ static void rk4_seq_body(const Group &G, int slots) {
 
 template <typename TD, bool HET>
 static void taylor_seq_body(const Group &G, int m, int K) {
-    if (power32(G, sizeof(TD) == 4, HET)) run_b2g_power32(G, 1);
-    else run_b2g(G, V_POWER, B_RHS, 1, 1);
+    if (power32(G, sizeof(TD) == 4, HET)) {
+        // a z-marching first term with zpow maps the power itself
+        if (!(G.zfirst && G.zpow)) run_b2g_power32(G, 1);
+    } else {
+        run_b2g(G, V_POWER, B_RHS, 1, 1);
+    }
     taylor_interval<TD, HET>(G, m, K, 1);
     run_g2b(G, -1, 1);
     hipLaunchKernelGGL(k_step_advance, dim3(small_grid(G.nprob)), dim3(64), 0, G.stream,

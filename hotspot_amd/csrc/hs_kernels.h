@@ -152,6 +152,10 @@ struct ProbDesc {
     int layer_base[kMaxLayers];         // block-vector offset per layer
     const int4 *blk_range;              // per block of the vector: i1,i2,j1,j2
     const int *blk_layer;
+    // per block of the vector: unit_w of its layer, by block index; set when
+    // every layer has cell_u16 and every kCellMixed cell has a cell_uw record
+    // of one to four blocks (null otherwise)
+    const float *blk_w;
     int map_mode, pad1;
     double *blk_in;                     // block vector (device staging)
     double *blk_out;

@@ -143,6 +143,34 @@ def test_sequence_equals_interval_by_interval(lib, jobs, form):
         set_options(lib, j.run, precision=0, transient=0)
 
 
+def test_sequence_power_mapping_equals_interval_by_interval(lib):
+    """On the headline stack (256x256, 4 dies of 64 blocks, z-marching first
+    term), a sequence maps each interval's cell power inside the first-term
+    kernel, while compute_temp maps it in its own pass: the results agree
+    bit for bit, also over intervals where a die, or every die, carries no
+    power (the layers the first term skips) and on boundary cells of two to
+    four blocks."""
+    c, pw = cases.stacked_case(256, seed=2)
+    r = make(lib, c)
+    set_options(lib, r, precision=1, transient=1)
+    steps = []
+    for k in range(6):
+        p = [np.array(x) * (1.0 + 0.1 * k) for x in pw]
+        if k == 2:
+            p[1][:] = 0.0
+        if k == 4:
+            p = [np.zeros_like(x) for x in p]
+        steps.append(r.power_vector(p))
+    seq = np.array(steps)
+    T0 = r.steady(seq[0].copy())
+    check_error(lib)
+    one = cases.run_transient(r, T0, list(seq), DT)
+    check_error(lib)
+    got = transient_run(lib, r, seq, T0)
+    assert np.array_equal(one, got)
+    r.free()
+
+
 @pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
 def test_engine_alone_equals_in_mix(lib, jobs, form):
     mix, resident, _, _ = run_engine(lib, jobs, form)
