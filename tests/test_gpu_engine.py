@@ -143,14 +143,20 @@ def test_sequence_equals_interval_by_interval(lib, jobs, form):
         set_options(lib, j.run, precision=0, transient=0)
 
 
-def test_sequence_power_mapping_equals_interval_by_interval(lib):
+@pytest.mark.parametrize("kind", ["stack-4x64-blocks", "chip-4x300-blocks"])
+def test_sequence_power_mapping_equals_interval_by_interval(lib, kind):
     """On the headline stack (256x256, 4 dies of 64 blocks, z-marching first
     term), a sequence maps each interval's cell power inside the first-term
     kernel, while compute_temp maps it in its own pass: the results agree
     bit for bit, also over intervals where a die, or every die, carries no
     power (the layers the first term skips) and on boundary cells of two to
-    four blocks."""
-    c, pw = cases.stacked_case(256, seed=2)
+    four blocks.  A stack of 4 dies of 300 blocks (1 204 blocks, over the
+    first term's table of 1 024) keeps the separate pass in its sequences
+    too; it agrees as well."""
+    if kind == "stack-4x64-blocks":
+        c, pw = cases.stacked_case(256, seed=2)
+    else:
+        c, pw = cases.chip_stack_case(256, seed=3, chip_layers=8, n_blocks=300)   # 1 204 blocks
     r = make(lib, c)
     set_options(lib, r, precision=1, transient=1)
     steps = []
